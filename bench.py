@@ -1,0 +1,181 @@
+"""MD steps/s on the 98,304-atom water box (32,768 H2O, L = 99.34 A) with the seeded
+se_a model (embed 25/50/100, fit 240x3, axis 16, rcut 6 A), NVE velocity Verlet at
+dt = 0.5 fs, 1 A neighbour skin, one MI355X.
+
+`value` is device-resident throughput: positions, momenta, neighbour lists and model
+stay in HBM; each timed step is the drift test + conditional neighbour rebuild, the
+integrator, and one full energy/force/virial evaluation.  The Simulation.run rate
+(host trajectory copies every chunk) is reported beside it as `simulation_run_steps_per_s`.
+
+Prints one JSON line.  Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, 'deepmd-jax_amd'))
+
+PEAK_FP32_MFMA = 157.3e12
+PEAK_BF16_MFMA = 2.5e15
+PEAK_HBM = 8.0e12
+KERNELS = {0: 'nlist', 1: 'embed_fwd', 2: 'fit', 3: 'embed_bwd', 4: 'integrator'}
+
+
+def flops_per_step(N, pairs, A=16, W=100, emb=(25, 50, 100), fit=240):
+    """Algorithmic FLOPs of one energy+force evaluation (no recompute, no weight grads):
+    2 (forward + input-gradient) x 2 FLOP/MAC x N x [pairs x (embedding + reducer MACs)
+    + descriptor MACs + fitting MACs]."""
+    e_mac = emb[0] + emb[0] * emb[1] + emb[1] * emb[2]
+    red_mac = 10 * W
+    g_mac = 10 * A * W
+    f_mac = A * W * fit + 2 * fit * fit + fit
+    per_kernel = {
+        'embed_fwd': 2.0 * N * (pairs * (e_mac + red_mac) + g_mac),
+        'embed_bwd': 2.0 * N * (pairs * (e_mac + red_mac) + g_mac),
+        'fit': 2.0 * 2.0 * N * f_mac,
+    }
+    return 2 * 2.0 * N * (pairs * (e_mac + red_mac) + g_mac + f_mac), per_kernel
+
+
+def cpu_baseline(params, var, seconds_budget=30.0):
+    """The CPU oracle (torch FP32 restatement with autograd forces) timed on this host on
+    a bounded sample: 4,096 H2O (12,288 atoms), E+F with neighbour lists; steps/s for the
+    98,304-atom box extrapolated linearly in N."""
+    import torch
+    sys.path.insert(0, ROOT)
+    from oracle import dp_oracle as O
+    from deepmd_jax import synthetic as syn
+    ncores = len(os.sched_getaffinity(0))
+    torch.set_num_threads(ncores)
+    coord, box, types = syn.water_box(4096, seed=0)
+    nb, _, _ = O.typed_nlist(coord, box, types, 7.0)
+    times = []
+    t_start = time.time()
+    while len(times) < 3 and (time.time() - t_start) < seconds_budget:
+        t0 = time.time()
+        O.energy_force_virial(params, var, coord, box, types, nbrs_nm=nb, dtype=torch.float32)
+        times.append(time.time() - t0)
+    t = float(np.median(times))
+    scale = 98304 / 12288
+    return {'value': 1.0 / (t * scale), 'unit': 'MD steps/s', 'cores': torch.get_num_threads(), 'kind': 'port',
+            'sample': '12,288-atom water box (4,096 H2O, seed 0), %d E+F evaluations (torch FP32, autograd), '
+                      'median %.2f s, x8 atoms extrapolated to 98,304' % (len(times), t)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=200)
+    ap.add_argument('--warmup', type=int, default=20)
+    ap.add_argument('--nmol', type=int, default=32768)
+    ap.add_argument('--precision', default='fp32')
+    ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-sim', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py runs on one GPU; multi-GPU runs are not part of this build (--gpus %d requested)' % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    from deepmd_jax import _lib, synthetic as syn
+    from deepmd_jax.constants import MASS_UNIT_CONVERSION
+
+    params, var = syn.init_model(0)
+    coord, box, types = syn.water_box(args.nmol, seed=0)
+    vel = syn.maxwell_boltzmann(types, 300.0, seed=0)
+    N = len(types)
+    model = _lib.DeviceModel(params, var)
+    sys_ = _lib.DeviceSystem(model, types, 1)
+    sys_.set_box(box)
+    sys_.set_masses(np.array(syn.MASSES)[types] * MASS_UNIT_CONVERSION)
+    sys_.set_positions(coord)
+    sys_.set_velocities(vel)
+    sys_.nlist_allocate(params['rcut'] + 1.0, 1.2)
+    p = _lib.MDParams()
+    p.routine = 0
+    p.precision = _lib.PREC[args.precision]
+    p.dt = 0.5
+    # warmup (includes first force evaluation)
+    done = 0
+    while done < args.warmup:
+        k = min(50, args.warmup - done)
+        _, _, err = sys_.md_run(p, k, record=False)
+        if err & 2:
+            sys_.nlist_allocate(params['rcut'] + 1.0, 1.25)
+            continue
+        done += k
+    sys_.synchronize()
+    sys_.timing(on=True)
+    t0 = time.perf_counter()
+    done, realloc = 0, 0
+    while done < args.steps:
+        k = min(100, args.steps - done)
+        _, _, err = sys_.md_run(p, k, record=False)
+        if err & 16:
+            raise RuntimeError('NaN/Inf during the benchmark')
+        if err & 2:
+            realloc += 1
+            sys_.nlist_allocate(params['rcut'] + 1.0, 1.2 + 0.05 * realloc)
+            continue
+        done += k
+    sys_.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    kt = {KERNELS[w]: sys_.timing(which=w) for w in KERNELS}
+    sys_.timing(on=False)
+    pairs = sys_.pair_count()
+    steps_per_s = args.steps / elapsed
+    ftot, fk = flops_per_step(N, pairs)
+    # dominant kernel by summed event time
+    dom = max(fk, key=lambda k: kt[k][0])
+    ms_dom, cnt_dom = kt[dom]
+    avg_s = ms_dom / 1e3 / max(cnt_dom, 1)
+    peak = PEAK_FP32_MFMA if args.precision == 'fp32' else PEAK_BF16_MFMA
+    roof = {'bound': 'mfma', 'kernel': dom, 'achieved': fk[dom] / avg_s / 1e12, 'peak': peak / 1e12,
+            'unit': 'TFLOP/s', 'frac': fk[dom] / avg_s / peak, 'traffic': None,
+            'launch_ms': avg_s * 1e3}
+    out = {
+        'metric': 'MD steps/s, 98k-atom water DP, fp32 vs BF16 at stated force err; p99 latency',
+        'value': steps_per_s, 'unit': 'MD steps/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': 1e3 * elapsed / args.steps, 'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': None, 'dtype': 'f32' if args.precision == 'fp32' else args.precision,
+        'data': 'synthetic (seeded water box + seeded random-init model)',
+        'config': {'workload': 'NVE velocity Verlet, %d-atom water (%d H2O, L=%.3f A), se_a 25/50/100 fit 240x3 '
+                               'axis 16 rcut 6, dt 0.5 fs, skin 1 A' % (N, args.nmol, box[0]),
+                   'atoms': N, 'precision': args.precision},
+        'atoms_steps_per_s': steps_per_s * N,
+        'ns_per_day': steps_per_s * 0.5 * 86400 * 1e-6,
+        'model_tflops': ftot * steps_per_s / 1e12,
+        'pairs_within_rcut_per_atom': pairs,
+        'kernel_ms_per_step': {k: v[0] / args.steps for k, v in kt.items()},
+        'nlist_reallocations': realloc,
+        'roofline': roof,
+    }
+    if not args.no_sim:
+        from deepmd_jax.md import Simulation
+        from deepmd_jax.dpmodel import DPModel
+        import io
+        import contextlib
+        sim = Simulation(None, box, types, syn.MASSES, 'NVE', 0.5, coord, initial_velocity=vel,
+                         report_interval=10 ** 9, neighbor_skin=1.0, precision=args.precision,
+                         model=(DPModel(params), var, model))
+        ns = max(20, args.steps // 4)
+        with contextlib.redirect_stdout(io.StringIO()):
+            sim.run(10)
+            t0 = time.perf_counter()
+            sim.run(ns)
+            t1 = time.perf_counter()
+        out['simulation_run_steps_per_s'] = ns / (t1 - t0)
+    if not args.no_cpu:
+        out['cpu_baseline'] = cpu_baseline(params, var)
+    else:
+        out['cpu_baseline'] = None
+    print(json.dumps(out))
+
+
+if __name__ == '__main__':
+    main()

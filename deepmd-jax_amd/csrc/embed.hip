@@ -1,0 +1,827 @@
+// Fused environment matrix + type-pair embedding MLP + reducer (forward) and its
+// hand-written backward, on gfx950 MFMA (v_mfma_f32_16x16x4f32).
+//
+// Math restated from deepmd-jax:
+//   pair geometry  deepmd_jax/utils.py:20-23,79-82   x = minimg(r_j - r_i), r = |x|
+//   switch         deepmd_jax/utils.py:29-32          s = (1-3t^2+2t^3)/(r+1e-15), t=r/rc
+//   env rows       deepmd_jax/dpmodel.py:37-47, utils.py:51-52
+//                  R = [sh, sqrt3 sh u, 3 sh (u_c^2-1/3), 3 sh sqrt2 u_c u_{c-1}], u = x/(r+1e-16)
+//   embedding      deepmd_jax/utils.py:107-125        h1=tanh(st K0+b0); h_{l+1}=tanh(h_l K_l+b_l)+repeat(h_l)
+//   reducer        deepmd_jax/dpmodel.py:50-51        T = sum_j R^T h3 / Nnbrs
+//   descriptor G   deepmd_jax/dpmodel.py:74-78
+// Neighbours beyond rcut have s = 0 and R = 0 exactly, so each centre's skin list is
+// compacted to the pairs with r < rcut before the MLP.
+//
+// Orientation: every layer is computed transposed, features x neighbours, on 16 x 16
+// tiles.  A 16-neighbour chunk of one centre occupies the 16 columns; lane (q, n)
+// holds features {16*tile + 4q + i} of neighbour n.  That accumulator layout is the
+// B-operand layout of the next layer (K = features), so the MLP chain runs without any
+// data movement; the repeat-residual parent of register r is register r/Z in the same
+// lane because of the feature layout (EmbedLayout).  Only the reducer and the
+// backward's dT tables go through LDS.
+#include "internal.h"
+#include <math.h>
+#include <string.h>
+#include <algorithm>
+#include <type_traits>
+
+namespace dpmd {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ floatx4 mfma4(float a, float b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---------------------------------------------------------------------------------
+// Host-side layout and packing
+
+bool EmbedLayout::init(const int* widths) {
+  for (int l = 0; l < 3; ++l) w[l] = widths[l];
+  if (w[0] <= 0 || w[0] > 32) return false;
+  z[0] = 1;
+  for (int l = 1; l < 3; ++l) {
+    if (w[l] == w[l - 1]) z[l] = 1;
+    else if (w[l] == 2 * w[l - 1]) z[l] = 2;
+    else return false;
+  }
+  int acc = 0;
+  for (int q = 0; q < 4; ++q) {
+    size[0][q] = w[0] / 4 + (q < w[0] % 4 ? 1 : 0);
+    base[0][q] = acc;
+    acc += size[0][q];
+  }
+  for (int l = 1; l < 3; ++l)
+    for (int q = 0; q < 4; ++q) {
+      size[l][q] = z[l] * size[l - 1][q];
+      base[l][q] = z[l] * base[l - 1][q];
+    }
+  for (int l = 0; l < 3; ++l) {
+    int mx = 0;
+    for (int q = 0; q < 4; ++q) mx = std::max(mx, size[l][q]);
+    tiles[l] = (mx + 3) / 4;
+  }
+  return true;
+}
+
+int EmbedLayout::logical(int layer, int slot) const {
+  int tile = slot / 16, rem = slot % 16, q = rem / 4, i = rem % 4;
+  int r = 4 * tile + i;
+  if (tile >= tiles[layer] || r >= size[layer][q]) return -1;
+  return base[layer][q] + r;
+}
+
+EmbedNetOffsets embed_net_offsets(const EmbedLayout& L) {
+  EmbedNetOffsets o;
+  int t1 = L.tiles[0], t2 = L.tiles[1], t3 = L.tiles[2];
+  int p = 0;
+  o.k0 = p; p += 16 * t1;
+  o.b0 = p; p += 16 * t1;
+  o.b1 = p; p += 16 * t2;
+  o.b2 = p; p += 16 * t3;
+  p = (p + 63) / 64 * 64;
+  o.w1a = p; p += t2 * 4 * t1 * 64;
+  o.w2a = p; p += t3 * 4 * t2 * 64;
+  o.w2b = p; p += t2 * 4 * t3 * 64;
+  o.w1b = p; p += t1 * 4 * t2 * 64;
+  o.total = p;
+  return o;
+}
+
+// K0 (1,w0), K1 (w0,w1), K2 (w1,w2) in flax (in,out) row-major layout.
+void embed_pack_net(const EmbedLayout& L, const float* K0, const float* B0, const float* K1,
+                    const float* B1, const float* K2, const float* B2, float* out) {
+  EmbedNetOffsets o = embed_net_offsets(L);
+  memset(out, 0, sizeof(float) * o.total);
+  int t1 = L.tiles[0], t2 = L.tiles[1], t3 = L.tiles[2];
+  for (int s = 0; s < 16 * t1; ++s) {
+    int a = L.logical(0, s);
+    if (a >= 0) { out[o.k0 + s] = K0[a]; out[o.b0 + s] = B0[a]; }
+  }
+  for (int s = 0; s < 16 * t2; ++s) {
+    int c = L.logical(1, s);
+    if (c >= 0) out[o.b1 + s] = B1[c];
+  }
+  for (int s = 0; s < 16 * t3; ++s) {
+    int w = L.logical(2, s);
+    if (w >= 0) out[o.b2 + s] = B2[w];
+  }
+  const int w1 = L.w[1], w2 = L.w[2];
+  for (int lane = 0; lane < 64; ++lane) {
+    int m = lane % 16, q = lane / 16;
+    // layer-2 forward: A[m=out slot][k=in slot]
+    for (int ot = 0; ot < t2; ++ot)
+      for (int st = 0; st < 4 * t1; ++st) {
+        int a = L.logical(0, 16 * (st / 4) + 4 * q + st % 4);
+        int c = L.logical(1, 16 * ot + m);
+        out[o.w1a + (ot * 4 * t1 + st) * 64 + lane] = (a >= 0 && c >= 0) ? K1[a * w1 + c] : 0.f;
+      }
+    for (int ot = 0; ot < t3; ++ot)
+      for (int st = 0; st < 4 * t2; ++st) {
+        int c = L.logical(1, 16 * (st / 4) + 4 * q + st % 4);
+        int w = L.logical(2, 16 * ot + m);
+        out[o.w2a + (ot * 4 * t2 + st) * 64 + lane] = (c >= 0 && w >= 0) ? K2[c * w2 + w] : 0.f;
+      }
+    // backward: A[m=in slot][k=out slot]
+    for (int ct = 0; ct < t2; ++ct)
+      for (int st = 0; st < 4 * t3; ++st) {
+        int c = L.logical(1, 16 * ct + m);
+        int w = L.logical(2, 16 * (st / 4) + 4 * q + st % 4);
+        out[o.w2b + (ct * 4 * t3 + st) * 64 + lane] = (c >= 0 && w >= 0) ? K2[c * w2 + w] : 0.f;
+      }
+    for (int at = 0; at < t1; ++at)
+      for (int st = 0; st < 4 * t2; ++st) {
+        int a = L.logical(0, 16 * at + m);
+        int c = L.logical(1, 16 * (st / 4) + 4 * q + st % 4);
+        out[o.w1b + (at * 4 * t2 + st) * 64 + lane] = (a >= 0 && c >= 0) ? K1[a * w1 + c] : 0.f;
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// Device helpers
+
+struct EmbedArgs {
+  const float4* pos;           // (R*N) wrapped fp32 positions, type-sorted
+  const int* nbr[kMaxT];       // (R*N, cap_j) type-local indices, padded with N
+  const int* nbr_cnt[kMaxT];   // (R*N)
+  int cap[kMaxT];
+  int off[kMaxT + 1];
+  int count[kMaxT];
+  int rowbase[kMaxT];
+  int ntypes, natoms, nrep, blocks_per_rep;
+  float box[3];
+  float rcut;
+  float mean[kMaxT], stdv[kMaxT];
+  const float* net;            // packed nets, index (i*ntypes+j)*net_stride
+  int net_stride;
+  int base3[4], size3[4];
+  int W, A;
+  float inv_nnbrs;
+  const float* tbias;
+  float* T;                    // (R*N, 10, W)
+  float* G;                    // (mrows, A*W)
+  const float* dG;             // (mrows, A*W)
+  float* force;                // (R*N, 3)
+  double* partial;             // (gridDim.x, 9) virial partials
+  int* paircnt;                // (R*N)
+  int list_cap;                // LDS list entries per wave
+  int net_bytes;
+  int lds_wave_floats;
+};
+
+// floor-mod shift to [-L/2, L/2): ((d - L/2) mod L) - L/2, as jnp.remainder computes it
+__device__ __forceinline__ float minimg(float d, float L) {
+  const float h = L * 0.5f;
+  float r = fmodf(d - h, L);
+  if (r != 0.f && ((r < 0.f) != (L < 0.f))) r += L;
+  return r - h;
+}
+
+__device__ __forceinline__ float safe_c(float x) { return fabsf(x) > 1e-15f ? x : 1e-15f; }
+
+__device__ __forceinline__ float pair_r(float x, float y, float z) {
+  float a = safe_c(x), b = safe_c(y), c = safe_c(z);
+  return sqrtf(a * a + b * b + c * c);
+}
+
+struct Geo {
+  float x[3], u[3], r, s, sh, st, dsdr;
+};
+
+__device__ __forceinline__ void env_forward(const float4 e, bool valid, float rcut, float mean,
+                                            float stdv, Geo& g, float R[10]) {
+  g.x[0] = valid ? e.x : 0.f;
+  g.x[1] = valid ? e.y : 0.f;
+  g.x[2] = valid ? e.z : 0.f;
+  float r = valid ? pair_r(g.x[0], g.x[1], g.x[2]) : 0.f;
+  float t = r / rcut;
+  float sw = 1.f - 3.f * t * t + 2.f * t * t * t;
+  bool on = (r > 1e-14f) && (t < 1.f);
+  float inv = 1.f / (r + 1e-15f);
+  float s = on ? inv * sw : 0.f;
+  g.r = r;
+  g.s = s;
+  g.dsdr = on ? ((-6.f * t + 6.f * t * t) / rcut) * inv - sw * inv * inv : 0.f;
+  g.sh = s / stdv;
+  g.st = valid ? (s - mean) / stdv : 0.f;
+  const float rr = r + 1e-16f;
+  for (int c = 0; c < 3; ++c) g.u[c] = g.x[c] / rr;
+  const float s3 = 1.7320508075688772f * g.sh, s33 = 3.f * g.sh;
+  R[0] = g.sh;
+  for (int c = 0; c < 3; ++c) R[1 + c] = s3 * g.u[c];
+  for (int c = 0; c < 3; ++c) R[4 + c] = s33 * (g.u[c] * g.u[c] - (1.f / 3.f));
+  for (int c = 0; c < 3; ++c) R[7 + c] = s33 * (1.4142135623730951f * g.u[c] * g.u[(c + 2) % 3]);
+}
+
+// dE/dx of one pair given dE/dR (10) and dE/dst.
+__device__ __forceinline__ void env_backward(const Geo& g, const float dR[10], float dst, float stdv,
+                                             float out[3]) {
+  const float s3 = 1.7320508075688772f, r2 = 1.4142135623730951f;
+  float dsh = dR[0];
+  float du[3];
+  for (int c = 0; c < 3; ++c) {
+    const int cm = (c + 2) % 3, cp = (c + 1) % 3;
+    dsh += s3 * dR[1 + c] * g.u[c] + 3.f * dR[4 + c] * (g.u[c] * g.u[c] - (1.f / 3.f)) +
+           3.f * r2 * dR[7 + c] * g.u[c] * g.u[cm];
+    du[c] = s3 * g.sh * dR[1 + c] + 6.f * g.sh * g.u[c] * dR[4 + c] +
+            3.f * r2 * g.sh * (dR[7 + c] * g.u[cm] + dR[7 + cp] * g.u[cp]);
+  }
+  const float ds = (dsh + dst) / stdv;
+  const float rr = g.r + 1e-16f;
+  float dr = ds * g.dsdr;
+  for (int c = 0; c < 3; ++c) dr -= du[c] * g.x[c] / (rr * rr);
+  for (int c = 0; c < 3; ++c) {
+    const float drdx = (fabsf(g.x[c]) > 1e-15f && g.r > 0.f) ? g.x[c] / g.r : 0.f;
+    out[c] = du[c] / rr + dr * drdx;
+  }
+}
+
+__device__ __forceinline__ float tanh_f(float x) { return tanhf(x); }
+
+// acc[ot] += sum_{s in [S0, S0+SN)} A(ot, s) * b(s), A read from the packed [ot][s][lane]
+// table with row length S.  Weight operands are prefetched PF steps ahead; the
+// scheduling barrier keeps the compiler from hoisting the whole table into registers.
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+
+// weight read: byte offset = voff (per lane) + soff (wave-uniform)
+__device__ __forceinline__ float bld(rsrc_t rs, int voff, int soff) {
+  return __builtin_bit_cast(float, (unsigned)__builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff, 0));
+}
+
+template <int OT, int S, int S0, int SN, typename BF>
+__device__ __forceinline__ void mfma_chain(rsrc_t rs, int sbase, int vl4, BF bfun, floatx4 (&acc)[OT]) {
+  constexpr int PF = 2;
+  float wb[PF + 1][OT];
+#pragma unroll
+  for (int p = 0; p < PF && p < SN; ++p)
+#pragma unroll
+    for (int ot = 0; ot < OT; ++ot) wb[p][ot] = bld(rs, vl4, sbase + (ot * S + S0 + p) * 256);
+#pragma unroll
+  for (int s = 0; s < SN; ++s) {
+    if (s + PF < SN) {
+#pragma unroll
+      for (int ot = 0; ot < OT; ++ot) wb[(s + PF) % (PF + 1)][ot] = bld(rs, vl4, sbase + (ot * S + S0 + s + PF) * 256);
+    }
+    const float b = bfun(S0 + s);
+#pragma unroll
+    for (int ot = 0; ot < OT; ++ot) acc[ot] = mfma4(wb[s % (PF + 1)][ot], b, acc[ot]);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// A wave-uniform zero the compiler cannot see through: adding it to the per-chunk
+// weight base keeps weight loads (and their addresses) inside the chunk loop.
+__device__ __forceinline__ int opaque_zero() {
+  int z;
+  asm volatile("s_mov_b32 %0, 0" : "=s"(z));
+  return z;
+}
+
+// f(integral_constant<G0>, integral_constant<size>) over tile groups [G0, G0+GS) < T
+template <int G0, int T, int GS, typename F>
+__device__ __forceinline__ void for_groups(F&& f) {
+  if constexpr (G0 < T) {
+    constexpr int SZ = (G0 + GS <= T) ? GS : (T - G0);
+    f(std::integral_constant<int, G0>(), std::integral_constant<int, SZ>());
+    for_groups<G0 + GS, T, GS>(f);
+  }
+}
+
+constexpr int kGroup = 2;  // layer-3 tiles processed together
+
+// Compact the skin row of neighbour type j to the pairs within rcut, in ascending
+// index order, into the wave's LDS list (float4: x, y, z, global index bits).
+__device__ __forceinline__ int compact_row(const EmbedArgs& a, int rep, int c, int j, float4 pc,
+                                           float4* list, int lane) {
+  const int gc = rep * a.natoms + c;
+  const int cnt = a.nbr_cnt[j][gc];
+  const int* row = a.nbr[j] + (size_t)gc * a.cap[j];
+  int m = 0;
+  for (int e0 = 0; e0 < cnt; e0 += 64) {
+    const int e = e0 + lane;
+    bool keep = false;
+    float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (e < cnt) {
+      const int idx = row[e];
+      const int g = rep * a.natoms + a.off[j] + idx;
+      const float4 pj = a.pos[g];
+      const float dx = minimg(pj.x - pc.x, a.box[0]);
+      const float dy = minimg(pj.y - pc.y, a.box[1]);
+      const float dz = minimg(pj.z - pc.z, a.box[2]);
+      const float r = pair_r(dx, dy, dz);
+      keep = (r > 1e-14f) && (r / a.rcut < 1.f);
+      out = make_float4(dx, dy, dz, __int_as_float(g));
+    }
+    const unsigned long long bal = __ballot(keep);
+    const int pre = __popcll(bal & ((1ull << lane) - 1ull));
+    if (keep) list[m + pre] = out;
+    m += __popcll(bal);
+  }
+  return m;
+}
+
+// ---------------------------------------------------------------------------------
+// Forward: T (10 x W) and G (A x W) per centre.  One wave per centre.
+
+template <int T1, int T2, int T3, int Z2, int Z3>
+__global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
+  extern __shared__ float smem[];
+  const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const rsrc_t wrs = __builtin_amdgcn_make_buffer_rsrc((void*)a.net, (short)0, a.net_bytes, 0x00020000);
+  const int q = lane >> 4, n = lane & 15;
+  constexpr int HS = 16 * T3 + 1;
+  float* wbase = smem + wv * a.lds_wave_floats;
+  float4* list = reinterpret_cast<float4*>(wbase);
+  float* Rl = wbase + 4 * a.list_cap;
+  float* Hl = Rl + 256;
+
+  const int rep = blockIdx.x / a.blocks_per_rep;
+  const int c = (blockIdx.x % a.blocks_per_rep) * 4 + wv;
+  if (c >= a.natoms) return;
+  const int gc = rep * a.natoms + c;
+  int ti = 0;
+  while (ti + 1 < a.ntypes && c >= a.off[ti + 1]) ++ti;
+  const float4 pc = a.pos[gc];
+
+  floatx4 Tacc[T3];
+#pragma unroll
+  for (int t = 0; t < T3; ++t) Tacc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+  int pairs = 0;
+
+  for (int j = 0; j < a.ntypes; ++j) {
+    const int m = compact_row(a, rep, c, j, pc, list, lane);
+    pairs += m;
+    wave_sync();
+    // byte offsets of the net's tables (layout of embed_net_offsets)
+    const int netb = (ti * a.ntypes + j) * a.net_stride * 4;
+    constexpr int O_K0 = 0, O_B0 = 16 * T1, O_B1 = 32 * T1, O_B2 = 32 * T1 + 16 * T2;
+    constexpr int P0 = ((16 * T1 * 2 + 16 * T2 + 16 * T3) + 63) / 64 * 64;
+    constexpr int O_W1A = P0, O_W2A = O_W1A + T2 * 4 * T1 * 64;
+    const int vl4 = lane * 4, vq16 = q * 16;
+    for (int ch = 0; ch < m; ch += 16) {
+      const int nb = netb + opaque_zero();
+      auto k0 = [&](int r) { return bld(wrs, vq16, nb + 4 * (O_K0 + 16 * (r / 4) + (r % 4))); };
+      auto b0 = [&](int r) { return bld(wrs, vq16, nb + 4 * (O_B0 + 16 * (r / 4) + (r % 4))); };
+      auto b1 = [&](int r) { return bld(wrs, vq16, nb + 4 * (O_B1 + 16 * (r / 4) + (r % 4))); };
+      auto b2 = [&](int r) { return bld(wrs, vq16, nb + 4 * (O_B2 + 16 * (r / 4) + (r % 4))); };
+      const bool valid = ch + n < m;
+      const float4 e = valid ? list[ch + n] : make_float4(0.f, 0.f, 0.f, 0.f);
+      Geo g;
+      float R[10];
+      env_forward(e, valid, a.rcut, a.mean[ti], a.stdv[ti], g, R);
+      if (q == 0) {
+#pragma unroll
+        for (int x = 0; x < 16; ++x) Rl[n * 16 + x] = x < 10 ? R[x] : 0.f;
+      }
+      float h1[4 * T1];
+#pragma unroll
+      for (int r = 0; r < 4 * T1; ++r) h1[r] = tanh_f(g.st * k0(r) + b0(r));
+      floatx4 acc2[T2];
+#pragma unroll
+      for (int t = 0; t < T2; ++t) acc2[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+      mfma_chain<T2, 4 * T1, 0, 4 * T1>(wrs, nb + 4 * O_W1A, vl4, [&](int s_) { return h1[s_]; }, acc2);
+      float h2[4 * T2];
+#pragma unroll
+      for (int r = 0; r < 4 * T2; ++r) {
+        float v = tanh_f(acc2[r / 4][r % 4] + b1(r));
+        if (r / Z2 < 4 * T1) v += h1[r / Z2];
+        h2[r] = v;
+      }
+      for_groups<0, T3, kGroup>([&](auto g0c, auto szc) {
+        constexpr int g0 = decltype(g0c)::value, SZ = decltype(szc)::value;
+        floatx4 acc3[SZ];
+#pragma unroll
+        for (int t = 0; t < SZ; ++t) acc3[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+        mfma_chain<SZ, 4 * T2, 0, 4 * T2>(wrs, nb + 4 * (O_W2A + g0 * 4 * T2 * 64), vl4,
+                                          [&](int s_) { return h2[s_]; }, acc3);
+#pragma unroll
+        for (int k = 0; k < SZ; ++k)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int r = 4 * (g0 + k) + i;
+            const int slot = 16 * (g0 + k) + 4 * q + i;
+            float v = tanh_f(acc3[k][i] + b2(r));
+            if (r / Z3 < 4 * T2) v += h2[r / Z3];
+            Hl[n * HS + slot] = v;
+          }
+      });
+      wave_sync();
+      // reducer: T[x][slot] += sum_nbr R[nbr][x] h3[nbr][slot]
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) {
+        const float av = Rl[(4 * kk + q) * 16 + n];
+#pragma unroll
+        for (int nt = 0; nt < T3; ++nt) Tacc[nt] = mfma4(av, Hl[(4 * kk + q) * HS + 16 * nt + n], Tacc[nt]);
+      }
+      wave_sync();
+    }
+  }
+  // T in logical order into LDS, then T and G to global
+  float* Tl = Hl;
+  const int W = a.W, A = a.A;
+#pragma unroll
+  for (int nt = 0; nt < T3; ++nt) {
+    const int qq = n / 4, ii = n % 4, r = 4 * nt + ii;
+    const int w = r < a.size3[qq] ? a.base3[qq] + r : -1;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int x = 4 * q + i;
+      if (x < 10 && w >= 0) Tl[x * W + w] = Tacc[nt][i] * a.inv_nnbrs;
+    }
+  }
+  wave_sync();
+  float* Tout = a.T + (size_t)gc * 10 * W;
+  for (int e = lane; e < 10 * W; e += 64) Tout[e] = Tl[e];
+  const int grow = a.rowbase[ti] + rep * a.count[ti] + (c - a.off[ti]);
+  float* Gout = a.G + (size_t)grow * A * W;
+  const float* tb = a.tbias;
+  const float r2 = 1.4142135623730951f;
+  for (int e = lane; e < A * W; e += 64) {
+    const int aa = e / W, w = e % W;
+    const float tww = Tl[w] + tb[w], twa = Tl[aa] + tb[aa];
+    float v = tww * twa;
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) v += Tl[(1 + cc) * W + w] * Tl[(1 + cc) * W + aa];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float t3 = Tl[(1 + k) * W + A + aa];
+      const float g2 = t3 * t3 + Tl[(4 + k) * W + A + aa];
+      v += g2 * Tl[(4 + k) * W + w];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float g2 = r2 * Tl[(1 + k) * W + A + aa] * Tl[(1 + (k + 2) % 3) * W + A + aa] +
+                       Tl[(7 + k) * W + A + aa];
+      v += g2 * Tl[(7 + k) * W + w];
+    }
+    Gout[e] = v;
+  }
+  if (lane == 0) a.paircnt[gc] = pairs;
+}
+
+// ---------------------------------------------------------------------------------
+// Backward: forces (scatter) and virial partials.  One wave per centre.
+
+template <int T1, int T2, int T3, int Z2, int Z3>
+__global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
+  extern __shared__ float smem[];
+  __shared__ double vir_s[4][9];
+  const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const rsrc_t wrs = __builtin_amdgcn_make_buffer_rsrc((void*)a.net, (short)0, a.net_bytes, 0x00020000);
+  const int q = lane >> 4, n = lane & 15;
+  constexpr int HS = 16 * T3 + 1;
+  const int W = a.W, A = a.A;
+  float* wbase = smem + wv * a.lds_wave_floats;
+  // chunk-phase layout
+  float* dTp = wbase;                    // 16 * HS
+  float* Rl = dTp + 16 * HS;             // 256
+  float* dRl = Rl + 256;                 // 256
+  float4* list = reinterpret_cast<float4*>(dRl + 256);
+  // prologue layout (aliases list region)
+  float* Tl = dRl + 256;                 // 10*W
+  float* dGl = Tl + 10 * W;              // A*W
+  float* Ul = dGl + A * W;               // 10*A
+
+  const int rep = blockIdx.x / a.blocks_per_rep;
+  const int c = (blockIdx.x % a.blocks_per_rep) * 4 + wv;
+  const bool active = c < a.natoms;
+  float vir[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) vir[k] = 0.f;
+
+  if (active) {
+    const int gc = rep * a.natoms + c;
+    int ti = 0;
+    while (ti + 1 < a.ntypes && c >= a.off[ti + 1]) ++ti;
+    const float4 pc = a.pos[gc];
+    const int grow = a.rowbase[ti] + rep * a.count[ti] + (c - a.off[ti]);
+    // ---- prologue: dT = d(G)/d(T)^T dG, scaled by 1/Nnbrs, into dTp (physical slots)
+    {
+      const float* Tg = a.T + (size_t)gc * 10 * W;
+      const float* dGg = a.dG + (size_t)grow * A * W;
+      for (int e = lane; e < 10 * W; e += 64) Tl[e] = Tg[e];
+      for (int e = lane; e < A * W; e += 64) dGl[e] = dGg[e];
+      for (int e = lane; e < 16 * HS; e += 64) dTp[e] = 0.f;
+      wave_sync();
+      for (int e = lane; e < W; e += 64) Tl[e] += a.tbias[e];  // row 0 becomes TW
+      wave_sync();
+      for (int e = lane; e < 10 * A; e += 64) {
+        const int x = e / A, aa = e % A;
+        float s = 0.f;
+        for (int w = 0; w < W; ++w) s += dGl[aa * W + w] * Tl[x * W + w];
+        Ul[x * A + aa] = s;
+      }
+      wave_sync();
+      const float r2 = 1.4142135623730951f;
+      for (int e = lane; e < 10 * W; e += 64) {
+        const int x = e / W, w = e % W;
+        float v = 0.f;
+        if (x < 4) {
+          for (int aa = 0; aa < A; ++aa) v += dGl[aa * W + w] * Tl[x * W + aa];
+          if (w < A) v += Ul[x * A + w];
+        } else {
+          const int k = x - 4;
+          for (int aa = 0; aa < A; ++aa) {
+            float g2;
+            if (k < 3) {
+              const float t3 = Tl[(1 + k) * W + A + aa];
+              g2 = t3 * t3 + Tl[(4 + k) * W + A + aa];
+            } else {
+              const int kp = k - 3;
+              g2 = r2 * Tl[(1 + kp) * W + A + aa] * Tl[(1 + (kp + 2) % 3) * W + A + aa] +
+                   Tl[(4 + k) * W + A + aa];
+            }
+            v += dGl[aa * W + w] * g2;
+          }
+        }
+        if (w >= A && w < 2 * A) {
+          const int aa = w - A;
+          if (x >= 4) v += Ul[x * A + aa];
+          if (x >= 1 && x <= 3) {
+            const int cc = x - 1;
+            v += 2.f * Tl[x * W + w] * Ul[(4 + cc) * A + aa];
+            v += r2 * Tl[(1 + (cc + 2) % 3) * W + w] * Ul[(7 + cc) * A + aa];
+            v += r2 * Tl[(1 + (cc + 1) % 3) * W + w] * Ul[(7 + (cc + 1) % 3) * A + aa];
+          }
+        }
+        // physical slot of logical feature w
+        int slot = -1;
+        for (int qq = 0; qq < 4; ++qq) {
+          const int rr = w - a.base3[qq];
+          if (rr >= 0 && rr < a.size3[qq]) slot = 16 * (rr / 4) + 4 * qq + (rr % 4);
+        }
+        if (slot >= 0) dTp[x * HS + slot] = v * a.inv_nnbrs;
+      }
+      wave_sync();
+    }
+    // ---- chunks
+    float fc[3] = {0.f, 0.f, 0.f};
+    for (int j = 0; j < a.ntypes; ++j) {
+      const int m = compact_row(a, rep, c, j, pc, list, lane);
+      wave_sync();
+      const int netb = (ti * a.ntypes + j) * a.net_stride * 4;
+      constexpr int O_K0 = 0, O_B0 = 16 * T1, O_B1 = 32 * T1, O_B2 = 32 * T1 + 16 * T2;
+      constexpr int P0 = ((16 * T1 * 2 + 16 * T2 + 16 * T3) + 63) / 64 * 64;
+      constexpr int O_W1A = P0, O_W2A = O_W1A + T2 * 4 * T1 * 64;
+      constexpr int O_W2B = O_W2A + T3 * 4 * T2 * 64, O_W1B = O_W2B + T2 * 4 * T3 * 64;
+      const int vl4 = lane * 4, vq16 = q * 16;
+      for (int ch = 0; ch < m; ch += 16) {
+        const int nb = netb + opaque_zero();
+        auto k0 = [&](int r) { return bld(wrs, vq16, nb + 4 * (O_K0 + 16 * (r / 4) + (r % 4))); };
+        auto b0 = [&](int r) { return bld(wrs, vq16, nb + 4 * (O_B0 + 16 * (r / 4) + (r % 4))); };
+        auto b1 = [&](int r) { return bld(wrs, vq16, nb + 4 * (O_B1 + 16 * (r / 4) + (r % 4))); };
+        auto b2 = [&](int r) { return bld(wrs, vq16, nb + 4 * (O_B2 + 16 * (r / 4) + (r % 4))); };
+        const bool valid = ch + n < m;
+        const float4 e = valid ? list[ch + n] : make_float4(0.f, 0.f, 0.f, 0.f);
+        Geo g;
+        float R[10];
+        env_forward(e, valid, a.rcut, a.mean[ti], a.stdv[ti], g, R);
+        if (q == 0) {
+#pragma unroll
+          for (int x = 0; x < 16; ++x) Rl[n * 16 + x] = x < 10 ? R[x] : 0.f;
+        }
+        float h1[4 * T1];
+#pragma unroll
+        for (int r = 0; r < 4 * T1; ++r) h1[r] = tanh_f(g.st * k0(r) + b0(r));
+        // h2 = t2 + h1[r/Z2] and h3 = t3 + h2[r/Z3] are re-formed where used
+        auto H2 = [&](int r, const float* t2v) -> float {
+          return t2v[r] + ((r / Z2 < 4 * T1) ? h1[r / Z2] : 0.f);
+        };
+        floatx4 acc2[T2];
+#pragma unroll
+        for (int t = 0; t < T2; ++t) acc2[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+        mfma_chain<T2, 4 * T1, 0, 4 * T1>(wrs, nb + 4 * O_W1A, vl4, [&](int s_) { return h1[s_]; }, acc2);
+        float t2[4 * T2];
+#pragma unroll
+        for (int r = 0; r < 4 * T2; ++r) t2[r] = tanh_f(acc2[r / 4][r % 4] + b1(r));
+        wave_sync();  // Rl visible to the dh3 products
+        floatx4 dr = floatx4{0.f, 0.f, 0.f, 0.f};
+        floatx4 dh2[T2];
+#pragma unroll
+        for (int t = 0; t < T2; ++t) dh2[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+        for_groups<0, T3, kGroup>([&](auto g0c, auto szc) {
+          constexpr int g0 = decltype(g0c)::value, SZ = decltype(szc)::value;
+          floatx4 acc3[SZ];
+#pragma unroll
+          for (int t = 0; t < SZ; ++t) acc3[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+          mfma_chain<SZ, 4 * T2, 0, 4 * T2>(wrs, nb + 4 * (O_W2A + g0 * 4 * T2 * 64), vl4,
+                                            [&](int s_) { return H2(s_, t2); }, acc3);
+          float t3[4 * SZ];
+#pragma unroll
+          for (int k = 0; k < SZ; ++k)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) t3[4 * k + i] = tanh_f(acc3[k][i] + b2(4 * (g0 + k) + i));
+          // dR^T[x][nbr] += sum_slot dT[x][slot] h3[slot][nbr] over this group's slots
+#pragma unroll
+          for (int k = 0; k < SZ; ++k)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              const int r = 4 * (g0 + k) + i;
+              const float h3v = t3[4 * k + i] + ((r / Z3 < 4 * T2) ? H2(r / Z3, t2) : 0.f);
+              dr = mfma4(dTp[n * HS + 16 * (g0 + k) + 4 * q + i], h3v, dr);
+            }
+          // dh3[slot][nbr] = sum_x dT[x][slot] R[nbr][x]
+          floatx4 dh3[SZ];
+#pragma unroll
+          for (int k = 0; k < SZ; ++k) {
+            floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kk = 0; kk < 3; ++kk)
+              acc = mfma4(dTp[(4 * kk + q) * HS + 16 * (g0 + k) + n], Rl[n * 16 + 4 * kk + q], acc);
+            dh3[k] = acc;
+          }
+          // repeat-residual: dh2[r/Z3] += dh3[r]; then dz3 = dh3 (1 - t3^2)
+#pragma unroll
+          for (int k = 0; k < SZ; ++k)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              const int r = 4 * (g0 + k) + i;
+              if (r / Z3 < 4 * T2) dh2[(r / Z3) / 4][(r / Z3) % 4] += dh3[k][i];
+              dh3[k][i] *= (1.f - t3[4 * k + i] * t3[4 * k + i]);
+            }
+          mfma_chain<T2, 4 * T3, 4 * g0, 4 * SZ>(wrs, nb + 4 * O_W2B, vl4,
+                                                [&](int s_) { return dh3[(s_ - 4 * g0) / 4][(s_ - 4 * g0) % 4]; }, dh2);
+        });
+#pragma unroll
+        for (int i = 0; i < 4; ++i) dRl[n * 16 + 4 * q + i] = dr[i];
+        // layer 2 backward
+        floatx4 dh1[T1];
+#pragma unroll
+        for (int at = 0; at < T1; ++at) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int r1 = 4 * at + i;
+            float v = 0.f;
+            if (Z2 == 2) {
+              if (2 * r1 < 4 * T2) v += dh2[(2 * r1) / 4][(2 * r1) % 4];
+              if (2 * r1 + 1 < 4 * T2) v += dh2[(2 * r1 + 1) / 4][(2 * r1 + 1) % 4];
+            } else {
+              if (r1 < 4 * T2) v += dh2[r1 / 4][r1 % 4];
+            }
+            dh1[at][i] = v;
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < 4 * T2; ++r) dh2[r / 4][r % 4] *= (1.f - t2[r] * t2[r]);
+        mfma_chain<T1, 4 * T2, 0, 4 * T2>(wrs, nb + 4 * O_W1B, vl4, [&](int s_) { return dh2[s_ / 4][s_ % 4]; }, dh1);
+        float dst = 0.f;
+#pragma unroll
+        for (int r1 = 0; r1 < 4 * T1; ++r1) dst += dh1[r1 / 4][r1 % 4] * (1.f - h1[r1] * h1[r1]) * k0(r1);
+        dst += __shfl_xor(dst, 16);
+        dst += __shfl_xor(dst, 32);
+        wave_sync();
+        float dR[10];
+#pragma unroll
+        for (int x = 0; x < 10; ++x) dR[x] = dRl[n * 16 + x];
+        float gx[3];
+        env_backward(g, dR, dst, a.stdv[ti], gx);
+        if (valid && q == 0) {
+          const int gj = __float_as_int(e.w);
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            atomicAdd(a.force + (size_t)gj * 3 + k, -gx[k]);
+            fc[k] += gx[k];
+          }
+#pragma unroll
+          for (int aa = 0; aa < 3; ++aa)
+#pragma unroll
+            for (int bb = 0; bb < 3; ++bb) vir[aa * 3 + bb] -= g.x[aa] * gx[bb];
+        }
+        wave_sync();
+      }
+    }
+    // centre force: reduce over the 16 lanes of group 0
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      float v = fc[k];
+      v += __shfl_xor(v, 1);
+      v += __shfl_xor(v, 2);
+      v += __shfl_xor(v, 4);
+      v += __shfl_xor(v, 8);
+      fc[k] = v;
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) atomicAdd(a.force + (size_t)gc * 3 + k, fc[k]);
+    }
+  }
+  // virial: wave reduce, block reduce in fp64, one partial per block
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    float v = vir[k];
+    v += __shfl_xor(v, 1);
+    v += __shfl_xor(v, 2);
+    v += __shfl_xor(v, 4);
+    v += __shfl_xor(v, 8);
+    vir[k] = v;
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) vir_s[wv][k] = (double)vir[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < 9) {
+    double s = vir_s[0][threadIdx.x] + vir_s[1][threadIdx.x] + vir_s[2][threadIdx.x] + vir_s[3][threadIdx.x];
+    a.partial[(size_t)blockIdx.x * 9 + threadIdx.x] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// Launchers
+
+static void fill_args(System* s, EmbedArgs& a) {
+  Model* M = s->model;
+  memset(&a, 0, sizeof(a));
+  a.pos = s->d_pos32;
+  for (int j = 0; j < s->ntypes; ++j) {
+    a.nbr[j] = s->d_nbr[j];
+    a.nbr_cnt[j] = s->d_nbr_cnt[j];
+    a.cap[j] = s->cap[j];
+    a.count[j] = s->count[j];
+    a.rowbase[j] = s->rowbase[j];
+    a.mean[j] = (float)M->mean[j];
+    a.stdv[j] = (float)(1.0 / M->istd[j]);
+  }
+  for (int j = 0; j <= s->ntypes; ++j) a.off[j] = s->off[j];
+  a.ntypes = s->ntypes;
+  a.natoms = s->natoms;
+  a.nrep = s->nrep;
+  a.blocks_per_rep = (s->natoms + 3) / 4;
+  for (int d = 0; d < 3; ++d) a.box[d] = (float)s->box[d];
+  a.rcut = (float)M->rcut;
+  a.net = M->d_emb;
+  a.net_stride = M->off.total;
+  a.net_bytes = M->off.total * 4 * M->ntypes * M->ntypes;
+  for (int q = 0; q < 4; ++q) {
+    a.base3[q] = M->lay.base[2][q];
+    a.size3[q] = M->lay.size[2][q];
+  }
+  a.W = M->wout;
+  a.A = M->axis;
+  a.inv_nnbrs = (float)(1.0 / M->nnbrs);
+  a.tbias = M->d_tbias;
+  a.T = s->d_T;
+  a.G = (float*)s->d_G;
+  a.dG = s->d_dG;
+  a.force = s->d_force;
+  a.partial = s->d_partial;
+  a.paircnt = reinterpret_cast<int*>(s->d_pairs);
+  int cmax = 1;
+  for (int j = 0; j < s->ntypes; ++j) cmax = std::max(cmax, s->cap[j]);
+  a.list_cap = (cmax + 3) / 4 * 4;
+}
+
+template <int T1, int T2, int T3, int Z2, int Z3>
+static int32_t launch_embed(System* s, bool backward) {
+  EmbedArgs a;
+  fill_args(s, a);
+  constexpr int HS = 16 * T3 + 1;
+  const int W = a.W, A = a.A;
+  const int nblocks = a.blocks_per_rep * s->nrep;
+  size_t wave_floats;
+  if (!backward) {
+    wave_floats = 4 * a.list_cap + 256 + 16 * HS;
+    wave_floats = std::max(wave_floats, (size_t)(4 * a.list_cap + 256 + 10 * W));
+  } else {
+    const size_t chunk = 16 * HS + 512 + 4 * a.list_cap;
+    const size_t pro = 16 * HS + 512 + 10 * W + A * W + 10 * A;
+    wave_floats = std::max(chunk, pro);
+  }
+  wave_floats = (wave_floats + 3) / 4 * 4;
+  a.lds_wave_floats = (int)wave_floats;
+  const size_t lds = 4 * wave_floats * sizeof(float);
+  if (lds > 160 * 1024) {
+    set_error("embedding kernel needs %zu bytes of LDS (neighbour capacity too large)", lds);
+    return DPMD_E_UNSUPPORTED;
+  }
+  if (!backward) {
+    hipLaunchKernelGGL((embed_fwd_kernel<T1, T2, T3, Z2, Z3>), dim3(nblocks), dim3(256), lds, s->stream, a);
+  } else {
+    if (s->partial_blocks < nblocks) {
+      set_error("virial partial buffer too small");
+      return DPMD_E_STATE;
+    }
+    hipLaunchKernelGGL((embed_bwd_kernel<T1, T2, T3, Z2, Z3>), dim3(nblocks), dim3(256), lds, s->stream, a);
+  }
+  DPMD_HIP(hipGetLastError());
+  return DPMD_OK;
+}
+
+int32_t embed_launch(System* s, bool backward) {
+  const EmbedLayout& L = s->model->lay;
+  if (L.tiles[0] == 2 && L.tiles[1] == 4 && L.tiles[2] == 7 && L.z[1] == 2 && L.z[2] == 2)
+    return launch_embed<2, 4, 7, 2, 2>(s, backward);
+  if (L.tiles[0] == 2 && L.tiles[1] == 2 && L.tiles[2] == 4 && L.z[1] == 1 && L.z[2] == 2)
+    return launch_embed<2, 2, 4, 1, 2>(s, backward);
+  set_error("embedding widths (%d,%d,%d) have no kernel instantiation", L.w[0], L.w[1], L.w[2]);
+  return DPMD_E_UNSUPPORTED;
+}
+
+}  // namespace dpmd

@@ -1,0 +1,173 @@
+// Internal declarations shared by the HIP translation units of libdpmd_hip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <vector>
+#include "dpmd.h"
+
+namespace dpmd {
+
+void set_error(const char* fmt, ...);
+
+#define DPMD_HIP(x)                                                                    \
+  do {                                                                                 \
+    hipError_t e_ = (x);                                                               \
+    if (e_ != hipSuccess) {                                                            \
+      ::dpmd::set_error("HIP error '%s' at %s:%d (%s)", hipGetErrorString(e_), __FILE__, \
+                        __LINE__, #x);                                                 \
+      return DPMD_E_HIP;                                                               \
+    }                                                                                  \
+  } while (0)
+
+#define DPMD_TRY(x)              \
+  do {                           \
+    int32_t r_ = (x);            \
+    if (r_ != DPMD_OK) return r_; \
+  } while (0)
+
+constexpr int kMaxT = DPMD_MAX_TYPES;
+constexpr int kWave = 64;
+
+// Embedding-net feature layout.  An MFMA 16x16 accumulator tile holds, in lane
+// (q = lane/16, n = lane%16), the rows 4q+i (i<4) of column n.  With features as rows
+// and neighbours as columns, lane group q owns registers r = 4*tile + i of every layer.
+// Register r of layer L in group q carries logical feature base[L][q] + r; groups take
+// contiguous runs of first-layer features and their repeat-residual children, so the
+// residual parent of register r is register r / Z in the same lane.
+struct EmbedLayout {
+  int w[3];         // widths of the three layers
+  int z[3];         // z[1] = w1/w0, z[2] = w2/w1 (1 or 2)
+  int tiles[3];     // 16-row MFMA tiles per layer
+  int base[3][4];
+  int size[3][4];
+  bool init(const int* widths);
+  // physical slot (16*tile + 4*q + i) -> logical feature or -1
+  int logical(int layer, int slot) const;
+};
+
+// Per-net device tables (fp32), see embed.hip for the meaning of each block.
+struct EmbedNetOffsets {
+  int k0, b0, b1, b2, w1a, w2a, w2b, w1b, total;
+};
+EmbedNetOffsets embed_net_offsets(const EmbedLayout& L);
+void embed_pack_net(const EmbedLayout& L, const float* K0, const float* B0, const float* K1,
+                    const float* B1, const float* K2, const float* B2, float* out);
+// bf16 copies of the A-operand tables (w1a, w2a, w2b, w1b), packed per MFMA step
+struct EmbedNetOffsetsH {
+  int w1a, w2a, w2b, w1b, total;
+};
+EmbedNetOffsetsH embed_net_offsets_h(const EmbedLayout& L);
+void embed_pack_net_h(const EmbedLayout& L, const float* packed_f32, uint16_t* out, bool fp16);
+
+struct Model {
+  int device = 0;
+  int ntypes = 0;
+  EmbedLayout lay;
+  int fitw[3] = {0, 0, 0};
+  int axis = 0;
+  int wout = 0;  // embedding output width W
+  double rcut = 0;
+  double mean[kMaxT], istd[kMaxT], ebias[kMaxT];
+  double nnbrs = 1, out_norm = 1;
+  float* d_tbias = nullptr;        // (W)
+  float* d_emb = nullptr;          // packed nets, ntypes^2 * off.total
+  uint16_t* d_emb_h[2] = {nullptr, nullptr};  // bf16 / fp16 operand tables
+  EmbedNetOffsets off;
+  EmbedNetOffsetsH offh;
+  // fitting weights per type: K (in,out) padded to out multiple of 16, transposes
+  float* d_fit = nullptr;
+  size_t fit_off[kMaxT][16];  // see api.hip
+  size_t fit_total = 0;
+  float fit_b3[kMaxT] = {0, 0, 0, 0};
+  uint16_t* d_fit_h[2] = {nullptr, nullptr};  // half-precision copies of kernels
+};
+
+struct Timer {
+  bool on = false;
+  double ms[8] = {0};
+  int64_t count[8] = {0};
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[8];
+};
+
+struct System {
+  Model* model = nullptr;
+  hipStream_t stream = nullptr;
+  int natoms = 0, nrep = 1, ntypes = 0;
+  int count[kMaxT], off[kMaxT + 1];
+  std::vector<int> perm;      // sorted position -> caller index
+  std::vector<int> type_sorted;
+  double box[3] = {0, 0, 0};
+  double ref_box[3] = {0, 0, 0};
+  // device state, type-sorted, (nrep, natoms, ...)
+  double* d_pos = nullptr;    // (R*N*3)
+  double* d_mom = nullptr;    // momenta
+  double* d_mass = nullptr;   // (N)
+  double* d_pos_save = nullptr;
+  double* d_mom_save = nullptr;
+  float4* d_pos32 = nullptr;  // wrapped fp32 mirror for the model
+  float* d_force = nullptr;   // (R*N*3)
+  // neighbour list
+  bool nl_ready = false;
+  bool nl_overflow = false;
+  double rc_skin = 0, buffer_ratio = 1.2;
+  int knbr[kMaxT];
+  int cap[kMaxT];             // knbr - 1 columns
+  int* d_nbr[kMaxT] = {nullptr};
+  int* d_nbr_cnt[kMaxT] = {nullptr};
+  float4* d_canon = nullptr;  // canonical coordinates of the last build (reference_position)
+  int ncell[3] = {0, 0, 0};
+  bool use_cells = false;
+  int* d_cell_count = nullptr;
+  int* d_cell_start = nullptr;
+  int* d_cell_fill = nullptr;
+  int* d_cell_atoms = nullptr;
+  int* d_atom_cell = nullptr;
+  int nl_lds_list = 512;
+  int* d_flags = nullptr;     // [0] rebuild flag, [1] drift bits, [2] error bits, [3] list overflow,
+                              // [4..4+kMaxT) max count per type, [8] measure-list overflow
+  // model work buffers
+  int rowbase[kMaxT + 1];     // fitting rows per type, padded to 64
+  int mrows = 0;
+  float* d_T = nullptr;       // (R*N, 10, W)
+  void* d_G = nullptr;        // (mrows, A*W) fp32 or half
+  float* d_dG = nullptr;      // (mrows, A*W)
+  float* d_fitbuf = nullptr;  // 7 activations (mrows, Fpad)
+  void* d_fitbuf_h = nullptr; // half copies of GEMM A operands
+  float* d_eatom = nullptr;   // (mrows)
+  double* d_partial = nullptr;  // per-block virial partials (nblocks, 9)
+  int partial_blocks = 0;
+  double* d_red = nullptr;    // reduction outputs
+  unsigned long long* d_pairs = nullptr;
+  double last_pairs = 0;
+  int fpad = 0;
+  // integrator
+  double* d_thermo = nullptr;  // per replica thermostat scalars
+  double* d_traj = nullptr;    // device trajectory staging (chunk)
+  size_t traj_cap = 0;
+  double* d_nm = nullptr;      // PIMD normal-mode matrix (P*P)
+  uint64_t rng_counter = 0;
+  bool force_valid = false;
+  Timer timer;
+  ~System();
+};
+
+// kernels / launchers (return DPMD_OK or error)
+int32_t nlist_build(System* s, bool measure, bool conditional);
+int32_t nlist_drift_check(System* s, float threshold);
+int32_t nlist_alloc_cells(System* s);
+int32_t model_forces(System* s, int precision);
+int32_t model_reduce(System* s, double* energy, double* virial, double* eatom_host);
+int32_t md_kick(System* s, double dt_half);
+int32_t md_drift(System* s, double dt);
+int32_t md_write_pos32(System* s);
+int32_t md_ke(System* s, double* ke);
+int32_t md_langevin_o(System* s, double dt, double gamma, double kT, uint64_t seed, uint64_t counter);
+int32_t md_pimd_free_ring(System* s, const double* cmat, const double* freqs, double dt_half);
+int32_t md_pimd_o(System* s, const double* cmat, const double* c1, const double* c2, uint64_t seed,
+                  uint64_t counter);
+int32_t md_check_nan(System* s);
+
+void timer_begin(System* s, int which, hipEvent_t* ev);
+void timer_end(System* s, int which, hipEvent_t ev);
+
+}  // namespace dpmd

@@ -1,0 +1,438 @@
+// Integrator and reduction kernels.  Positions and momenta are FP64 (type-sorted,
+// (nrep, natoms, 3)); forces arrive in FP32 from the model kernels.
+//
+// Restated integrator steps:
+//   velocity Verlet (jax_md simulate.nve, wrapped at deepmd_jax/routine.py:139-144):
+//     p += dt/2 F; x += dt p/m; F = F(x); p += dt/2 F
+//   BAOAB O step (deepmd_jax/routine.py:86-96): p = c1 p + c2 sqrt(m) xi,
+//     c1 = exp(-gamma dt), c2 = sqrt(kT (1-c1^2)); thermostat_work += dKE
+//   PIMD free ring + PILE-L (deepmd_jax/pimd.py:92-111,160-179,210-227)
+// Positions are kept unwrapped, which is what the reference's wrap-then-
+// min_image_unwrap (deepmd_jax/md_utils.py:162-169) produces.
+#include "internal.h"
+#include <algorithm>
+#include <string.h>
+
+namespace dpmd {
+
+__global__ void write_pos32_kernel(const double* pos, float4* p32, int total, double Lx, double Ly, double Lz) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  const double L[3] = {Lx, Ly, Lz};
+  float v[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double x = pos[(size_t)g * 3 + d];
+    double r = fmod(x, L[d]);
+    if (r < 0) r += L[d];
+    if (r >= L[d]) r -= L[d];
+    v[d] = (float)r;
+    if (v[d] >= (float)L[d]) v[d] = 0.f;
+  }
+  p32[g] = make_float4(v[0], v[1], v[2], 0.f);
+}
+
+__global__ void zero_kernel(float* x, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    x[i] = 0.f;
+}
+
+__global__ void kick_kernel(double* mom, const float* force, size_t n3, double h) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n3) mom[i] += h * (double)force[i];
+}
+
+// x += dt p / m, fp32 mirror, NaN/Inf test -> error bit 16
+__global__ void drift_kernel(double* pos, const double* mom, const double* mass, float4* p32, int natoms,
+                             int total, double dt, double Lx, double Ly, double Lz, int* flags) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  const double m = mass[g % natoms];
+  const double L[3] = {Lx, Ly, Lz};
+  float v[3];
+  bool bad = false;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double p = mom[(size_t)g * 3 + d];
+    const double x = pos[(size_t)g * 3 + d] + dt * p / m;
+    pos[(size_t)g * 3 + d] = x;
+    bad |= !isfinite(x) || !isfinite(p);
+    double r = fmod(x, L[d]);
+    if (r < 0) r += L[d];
+    if (r >= L[d]) r -= L[d];
+    v[d] = (float)r;
+    if (v[d] >= (float)L[d]) v[d] = 0.f;
+  }
+  p32[g] = make_float4(v[0], v[1], v[2], 0.f);
+  if (bad) atomicOr(&flags[2], 16);
+}
+
+// per-replica sum of per-row energies and of virial partials, FP64
+__global__ void reduce_kernel(const float* eatom, int nt, const int* rowbase, const int* count, int nrep,
+                              const double* partial, int blocks_per_rep, double* out) {
+  // out[rep*10 + 0] = energy, out[rep*10 + 1..9] = virial
+  __shared__ double red[256];
+  const int rep = blockIdx.x;
+  double s = 0.0;
+  for (int t = 0; t < nt; ++t) {
+    const int base = rowbase[t] + rep * count[t];
+    for (int i = threadIdx.x; i < count[t]; i += blockDim.x) s += (double)eatom[base + i];
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[rep * 10] = red[0];
+  __syncthreads();
+  for (int c = 0; c < 9; ++c) {
+    double v = 0.0;
+    for (int b = threadIdx.x; b < blocks_per_rep; b += blockDim.x) v += partial[(size_t)(rep * blocks_per_rep + b) * 9 + c];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+      if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[rep * 10 + 1 + c] = red[0];
+    __syncthreads();
+  }
+}
+
+__global__ void ke_kernel(const double* mom, const double* mass, int natoms, double* out) {
+  __shared__ double red[256];
+  const int rep = blockIdx.x;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < natoms; i += blockDim.x) {
+    const double* p = mom + ((size_t)rep * natoms + i) * 3;
+    s += (p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) / (2.0 * mass[i]);
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[rep] = red[0];
+}
+
+// --- counter-based normal variates: Philox4x32-10 (Salmon et al., SC'11) + Box-Muller
+__device__ __forceinline__ void philox(uint32_t ctr[4], uint32_t key[2]) {
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = (uint64_t)M0 * ctr[0];
+    const uint64_t p1 = (uint64_t)M1 * ctr[2];
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ ctr[1] ^ key[0];
+    const uint32_t n1 = (uint32_t)p1;
+    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ ctr[3] ^ key[1];
+    const uint32_t n3 = (uint32_t)p0;
+    ctr[0] = n0; ctr[1] = n1; ctr[2] = n2; ctr[3] = n3;
+    key[0] += W0; key[1] += W1;
+  }
+}
+
+__device__ __forceinline__ void normal4(uint64_t seed, uint64_t counter, uint32_t idx, uint32_t stream, double z[4]) {
+  uint32_t c[4] = {(uint32_t)counter, (uint32_t)(counter >> 32), idx, stream};
+  uint32_t k[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  philox(c, k);
+  const double two32 = 4294967296.0;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const double u1 = ((double)c[2 * h] + 1.0) / (two32 + 1.0);
+    const double u2 = (double)c[2 * h + 1] / two32;
+    const double r = sqrt(-2.0 * log(u1));
+    z[2 * h] = r * cos(6.283185307179586 * u2);
+    z[2 * h + 1] = r * sin(6.283185307179586 * u2);
+  }
+}
+
+// BAOAB O step, with thermostat work accumulated per replica
+__global__ void langevin_o_kernel(double* mom, const double* mass, int natoms, int total, double c1, double c2,
+                                  uint64_t seed, uint64_t counter, double* work) {
+  __shared__ double red[256];
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  double dke = 0.0;
+  int rep = (blockIdx.x * blockDim.x) / natoms;
+  if (g < total) {
+    const double m = mass[g % natoms];
+    double z[4];
+    normal4(seed, counter, (uint32_t)g, 0x4c414e47u, z);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double p0 = mom[(size_t)g * 3 + d];
+      const double p1 = c1 * p0 + c2 * sqrt(m) * z[d];
+      mom[(size_t)g * 3 + d] = p1;
+      dke += 0.5 * (p1 * p1 - p0 * p0) / m;
+    }
+  }
+  // blocks never straddle replicas: the launcher rounds each replica to whole blocks
+  red[threadIdx.x] = dke;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && rep < total / natoms) atomicAdd(&work[rep], red[0]);
+}
+
+__global__ void scale_mom_kernel(double* mom, int natoms, int total3, const double* scale) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < total3) mom[i] *= scale[(i / 3) / natoms];
+}
+
+// PIMD exact free ring-polymer step for one (atom, dim) per thread: unwrap beads onto
+// bead 0's image, rotate to normal modes, propagate, rotate back.
+__global__ void pimd_free_ring_kernel(double* pos, double* mom, const double* mass, int natoms, int P,
+                                      const double* C, const double* freqs, double h, double Lx, double Ly,
+                                      double Lz) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= natoms * 3) return;
+  const int i = t / 3, d = t % 3;
+  const double L = d == 0 ? Lx : (d == 1 ? Ly : Lz);
+  const double m = mass[i];
+  double q[64], p[64], qn[64], pn[64];
+  const double x0 = pos[(size_t)i * 3 + d];
+  for (int b = 0; b < P; ++b) {
+    const double x = pos[((size_t)b * natoms + i) * 3 + d];
+    double dl = x - x0;
+    dl -= L * rint(dl / L);
+    q[b] = x0 + dl;
+    p[b] = mom[((size_t)b * natoms + i) * 3 + d];
+  }
+  for (int k = 0; k < P; ++k) {
+    double sq = 0, sp = 0;
+    for (int b = 0; b < P; ++b) {
+      sq += C[b * P + k] * q[b];
+      sp += C[b * P + k] * p[b];
+    }
+    qn[k] = sq;
+    pn[k] = sp;
+  }
+  qn[0] += h * pn[0] / m;
+  for (int k = 1; k < P; ++k) {
+    const double w = freqs[k];
+    const double cs = cos(w * h), sn = sin(w * h);
+    const double qq = qn[k] * cs + pn[k] * sn / (m * w);
+    const double pp = pn[k] * cs - m * w * qn[k] * sn;
+    qn[k] = qq;
+    pn[k] = pp;
+  }
+  for (int b = 0; b < P; ++b) {
+    double sq = 0, sp = 0;
+    for (int k = 0; k < P; ++k) {
+      sq += C[b * P + k] * qn[k];
+      sp += C[b * P + k] * pn[k];
+    }
+    pos[((size_t)b * natoms + i) * 3 + d] = sq;
+    mom[((size_t)b * natoms + i) * 3 + d] = sp;
+  }
+}
+
+// PILE-L O step in normal modes; work[0] accumulates the thermostat work
+__global__ void pimd_o_kernel(double* mom, const double* mass, int natoms, int P, const double* C,
+                              const double* c1, const double* c2, uint64_t seed, uint64_t counter, double* work) {
+  __shared__ double red[256];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  double dke = 0.0;
+  if (t < natoms * 3) {
+    const int i = t / 3, d = t % 3;
+    const double m = mass[i];
+    double p[64], pn[64];
+    for (int b = 0; b < P; ++b) p[b] = mom[((size_t)b * natoms + i) * 3 + d];
+    for (int k = 0; k < P; ++k) {
+      double s = 0;
+      for (int b = 0; b < P; ++b) s += C[b * P + k] * p[b];
+      pn[k] = s;
+    }
+    for (int k = 0; k < P; k += 4) {
+      double z[4];
+      normal4(seed, counter, (uint32_t)t, 0x50494d00u + (uint32_t)k, z);
+      for (int u = 0; u < 4 && k + u < P; ++u) pn[k + u] = c1[k + u] * pn[k + u] + c2[k + u] * sqrt(m) * z[u];
+    }
+    for (int b = 0; b < P; ++b) {
+      double s = 0;
+      for (int k = 0; k < P; ++k) s += C[b * P + k] * pn[k];
+      dke += 0.5 * (s * s - p[b] * p[b]) / m;
+      mom[((size_t)b * natoms + i) * 3 + d] = s;
+    }
+  }
+  red[threadIdx.x] = dke;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(&work[0], red[0]);
+}
+
+// ---------------------------------------------------------------------------------
+
+int32_t md_write_pos32(System* s) {
+  const int RN = s->nrep * s->natoms;
+  hipLaunchKernelGGL(write_pos32_kernel, dim3((RN + 255) / 256), dim3(256), 0, s->stream, s->d_pos, s->d_pos32,
+                     RN, s->box[0], s->box[1], s->box[2]);
+  DPMD_HIP(hipGetLastError());
+  return DPMD_OK;
+}
+
+int32_t md_zero_force(System* s) {
+  const size_t n = (size_t)s->nrep * s->natoms * 3;
+  hipLaunchKernelGGL(zero_kernel, dim3(std::min<size_t>(2048, (n + 255) / 256)), dim3(256), 0, s->stream,
+                     s->d_force, n);
+  DPMD_HIP(hipGetLastError());
+  return DPMD_OK;
+}
+
+int32_t md_kick(System* s, double h) {
+  const size_t n3 = (size_t)s->nrep * s->natoms * 3;
+  hipLaunchKernelGGL(kick_kernel, dim3((n3 + 255) / 256), dim3(256), 0, s->stream, s->d_mom, s->d_force, n3, h);
+  DPMD_HIP(hipGetLastError());
+  return DPMD_OK;
+}
+
+int32_t md_drift(System* s, double dt) {
+  const int RN = s->nrep * s->natoms;
+  hipLaunchKernelGGL(drift_kernel, dim3((RN + 255) / 256), dim3(256), 0, s->stream, s->d_pos, s->d_mom,
+                     s->d_mass, s->d_pos32, s->natoms, RN, dt, s->box[0], s->box[1], s->box[2], s->d_flags);
+  DPMD_HIP(hipGetLastError());
+  return DPMD_OK;
+}
+
+int32_t md_reduce_launch(System* s, int* d_rowbase, int* d_count) {
+  hipLaunchKernelGGL(reduce_kernel, dim3(s->nrep), dim3(256), 0, s->stream, s->d_eatom, s->ntypes, d_rowbase,
+                     d_count, s->nrep, s->d_partial, (s->natoms + 3) / 4, s->d_red);
+  DPMD_HIP(hipGetLastError());
+  return DPMD_OK;
+}
+
+int32_t md_ke_launch(System* s, double* d_out) {
+  hipLaunchKernelGGL(ke_kernel, dim3(s->nrep), dim3(256), 0, s->stream, s->d_mom, s->d_mass, s->natoms, d_out);
+  DPMD_HIP(hipGetLastError());
+  return DPMD_OK;
+}
+
+int32_t md_langevin_o(System* s, double dt, double gamma, double kT, uint64_t seed, uint64_t counter) {
+  const double c1 = exp(-gamma * dt);
+  const double c2 = sqrt(std::max(kT * (1.0 - c1 * c1), 0.0));
+  if (s->natoms % 256 != 0 && s->nrep > 1) {
+    // blocks must not straddle replicas: launch per replica
+    for (int r = 0; r < s->nrep; ++r) {
+      hipLaunchKernelGGL(langevin_o_kernel, dim3((s->natoms + 255) / 256), dim3(256), 0, s->stream,
+                         s->d_mom + (size_t)r * s->natoms * 3, s->d_mass, s->natoms, s->natoms, c1, c2, seed,
+                         counter * 4096 + r, s->d_thermo + r);
+    }
+  } else {
+    const int RN = s->nrep * s->natoms;
+    hipLaunchKernelGGL(langevin_o_kernel, dim3((RN + 255) / 256), dim3(256), 0, s->stream, s->d_mom, s->d_mass,
+                       s->natoms, RN, c1, c2, seed, counter, s->d_thermo);
+  }
+  DPMD_HIP(hipGetLastError());
+  return DPMD_OK;
+}
+
+int32_t md_scale_mom(System* s, const double* d_scale) {
+  const int n3 = s->nrep * s->natoms * 3;
+  hipLaunchKernelGGL(scale_mom_kernel, dim3((n3 + 255) / 256), dim3(256), 0, s->stream, s->d_mom, s->natoms, n3,
+                     d_scale);
+  DPMD_HIP(hipGetLastError());
+  return DPMD_OK;
+}
+
+int32_t md_pimd_free_ring(System* s, const double* d_C, const double* d_freqs, double h) {
+  const int n = s->natoms * 3;
+  hipLaunchKernelGGL(pimd_free_ring_kernel, dim3((n + 127) / 128), dim3(128), 0, s->stream, s->d_pos, s->d_mom,
+                     s->d_mass, s->natoms, s->nrep, d_C, d_freqs, h, s->box[0], s->box[1], s->box[2]);
+  DPMD_HIP(hipGetLastError());
+  return DPMD_OK;
+}
+
+int32_t md_pimd_o(System* s, const double* d_C, const double* d_c1, const double* d_c2, uint64_t seed,
+                  uint64_t counter) {
+  const int n = s->natoms * 3;
+  hipLaunchKernelGGL(pimd_o_kernel, dim3((n + 255) / 256), dim3(256), 0, s->stream, s->d_mom, s->d_mass,
+                     s->natoms, s->nrep, d_C, d_c1, d_c2, seed, counter, s->d_thermo);
+  DPMD_HIP(hipGetLastError());
+  return DPMD_OK;
+}
+
+// Nose-Hoover chain half step (jax_md simulate.nose_hoover_chain with chain_steps = 1,
+// sy_steps = 1, as wrapped by deepmd_jax/routine.py:147-158), one thread per replica.
+// thermo layout per replica (64 doubles): [0] invariant term, [1] initialised,
+// [8..16) xi, [16..24) p_xi, [24..32) Q, [40] momentum scale.
+__global__ void nh_half_kernel(double* thermo, const double* ke, int nrep, int C, double kT, double dt, double tau,
+                               double dof) {
+  const int rep = threadIdx.x;
+  if (rep >= nrep) return;
+  double* th = thermo + rep * 64;
+  double* xi = th + 8;
+  double* px = th + 16;
+  double* Q = th + 24;
+  if (th[1] == 0.0) {
+    for (int k = 0; k < C; ++k) { xi[k] = 0.0; px[k] = 0.0; }
+    th[1] = 1.0;
+  }
+  for (int k = 0; k < C; ++k) Q[k] = kT * tau * tau;
+  Q[0] *= dof;
+  double KE = ke[rep];
+  const double d2 = dt / 2, d4 = d2 / 2, d8 = d4 / 2;
+  const int M = C - 1;
+  double G;
+  if (M > 0) {
+    G = px[M - 1] * px[M - 1] / Q[M - 1] - kT;
+    px[M] += d4 * G;
+    double pnew = px[M];
+    for (int m = M - 1; m > 0; --m) {
+      G = px[m - 1] * px[m - 1] / Q[m - 1] - kT;
+      const double sc = exp(-d8 * pnew / Q[m + 1]);
+      pnew = sc * (sc * px[m] + d4 * G);
+      px[m] = pnew;
+    }
+  }
+  G = 2.0 * KE - dof * kT;
+  {
+    const double sc = (M > 0) ? exp(-d8 * px[1] / Q[1]) : 1.0;
+    px[0] = sc * (sc * px[0] + d4 * G);
+  }
+  const double scale = exp(-d2 * px[0] / Q[0]);
+  KE *= scale * scale;
+  for (int k = 0; k < C; ++k) xi[k] += d2 * px[k] / Q[k];
+  G = 2.0 * KE - dof * kT;
+  for (int m = 0; m < M; ++m) {
+    const double sc = exp(-d8 * px[m + 1] / Q[m + 1]);
+    const double pu = sc * (sc * px[m] + d4 * G);
+    G = pu * pu / Q[m] - kT;
+    px[m] = pu;
+  }
+  if (M > 0) px[M] += d4 * G;
+  th[40] = scale;
+  double e = px[0] * px[0] / (2.0 * Q[0]) + dof * kT * xi[0];
+  for (int k = 1; k < C; ++k) e += px[k] * px[k] / (2.0 * Q[k]) + kT * xi[k];
+  th[0] = e;
+}
+
+__global__ void nh_scale_kernel(double* mom, const double* thermo, int natoms, int total3) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < total3) mom[i] *= thermo[((i / 3) / natoms) * 64 + 40];
+}
+
+int32_t md_ke_launch(System* s, double* d_out);
+
+int32_t md_nh_half(System* s, double kT, double dt, double tau, int chain) {
+  if (chain < 1 || chain > 8) {
+    set_error("Nose-Hoover chain length %d outside [1, 8]", chain);
+    return DPMD_E_ARG;
+  }
+  double* d_ke = s->d_red + 10 * s->nrep;
+  DPMD_TRY(md_ke_launch(s, d_ke));
+  hipLaunchKernelGGL(nh_half_kernel, dim3(1), dim3(64), 0, s->stream, s->d_thermo, d_ke, s->nrep, chain, kT, dt,
+                     tau, 3.0 * s->natoms);
+  const int n3 = s->nrep * s->natoms * 3;
+  hipLaunchKernelGGL(nh_scale_kernel, dim3((n3 + 255) / 256), dim3(256), 0, s->stream, s->d_mom, s->d_thermo,
+                     s->natoms, n3);
+  DPMD_HIP(hipGetLastError());
+  return DPMD_OK;
+}
+
+}  // namespace dpmd

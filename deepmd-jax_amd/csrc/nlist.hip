@@ -1,0 +1,374 @@
+// Typed neighbour list on the device: cell binning, 27-cell scan, per-row ascending
+// type-split output, overflow detection, and the per-step drift test that decides a
+// rebuild.
+//
+// Semantics restated from deepmd-jax (and the jax_md neighbour list it wraps):
+//   canonicalize   deepmd_jax/md_utils.py:364-373   c = ((f32(x) mod L) * scale) * f32(1-5e-7)
+//   pair test      jax_md space.periodic + square_distance (call site md_utils.py:384-415):
+//                  d = (ca - cb); d = ((d + L/2) mod L) - L/2; d2 = (dx^2 + dy^2) + dz^2 < cut^2,
+//                  self excluded
+//   rows           md_utils.py:431-459: per neighbour type, ascending type-local indices,
+//                  padded with N, knbr_j - 1 columns; overflow if a row needs knbr_j
+//   drift test     deepmd_jax/md.py:693-741: max |minimg((f32(x) mod L)*scale - c_ref)|
+//                  > ((rc+skin) - rc*scale)/2 - 1e-6 L_min
+// All float32 expressions are evaluated without contraction so the neighbour sets are
+// bit-exact with a float32 evaluation of the same formulas.
+#include "internal.h"
+#include <algorithm>
+#include <string.h>
+
+namespace dpmd {
+
+__device__ __forceinline__ void nl_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+#pragma clang fp contract(off)
+
+__device__ __forceinline__ float fmod_floor(float x, float y) {
+  float r = fmodf(x, y);
+  if (r != 0.f && ((r < 0.f) != (y < 0.f))) r = r + y;
+  return r;
+}
+
+__device__ __forceinline__ float pair_d2(float4 a, float4 b, float Lx, float Ly, float Lz) {
+  float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+  const float hx = Lx * 0.5f, hy = Ly * 0.5f, hz = Lz * 0.5f;
+  dx = fmod_floor(dx + hx, Lx) - hx;
+  dy = fmod_floor(dy + hy, Ly) - hy;
+  dz = fmod_floor(dz + hz, Lz) - hz;
+  const float sx = dx * dx, sy = dy * dy, sz = dz * dz;
+  return (sx + sy) + sz;
+}
+
+struct NlArgs {
+  const double* pos;   // (R*N*3) type-sorted
+  float4* canon;       // (R*N)
+  int* atom_cell;      // (R*N)
+  int* cell_count;     // (R*ncell)
+  int* cell_start;
+  int* cell_fill;
+  int* cell_atoms;     // (R*N)
+  int* nbr[kMaxT];
+  int* nbr_cnt[kMaxT];
+  int cap[kMaxT];
+  int off[kMaxT + 1];
+  int ntypes, natoms, nrep;
+  int nc[3], ncell;
+  int use_cells;
+  float L[3];
+  float scale[3];
+  float cut_sq;
+  int* flags;
+  int conditional;
+  int measure;
+  int lcap;
+};
+
+__global__ void nl_prepare_kernel(NlArgs a) {
+  if (a.conditional && a.flags[0] == 0) return;
+  const int tot = a.nrep * a.ncell;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += gridDim.x * blockDim.x) {
+    a.cell_count[i] = 0;
+    a.cell_fill[i] = 0;
+  }
+}
+
+__global__ void nl_canon_kernel(NlArgs a) {
+  if (a.conditional && a.flags[0] == 0) return;
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.nrep * a.natoms) return;
+  const float shrink = (float)(1.0 - 5e-7);
+  float c[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    float x = (float)a.pos[(size_t)g * 3 + d];
+    x = fmod_floor(x, a.L[d]);
+    x = x * a.scale[d];
+    c[d] = x * shrink;
+  }
+  a.canon[g] = make_float4(c[0], c[1], c[2], 0.f);
+  if (a.use_cells) {
+    int ci[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      int v = (int)(c[d] * ((float)a.nc[d] / a.L[d]));
+      ci[d] = v < 0 ? 0 : (v >= a.nc[d] ? a.nc[d] - 1 : v);
+    }
+    const int cell = (ci[2] * a.nc[1] + ci[1]) * a.nc[0] + ci[0];
+    const int rep = g / a.natoms;
+    a.atom_cell[g] = cell;
+    atomicAdd(&a.cell_count[rep * a.ncell + cell], 1);
+  }
+}
+
+// one block per replica: exclusive scan over cells
+__global__ void nl_scan_kernel(NlArgs a) {
+  if (a.conditional && a.flags[0] == 0) return;
+  __shared__ int part[1024];
+  const int rep = blockIdx.x;
+  const int* cnt = a.cell_count + rep * a.ncell;
+  int* st = a.cell_start + rep * a.ncell;
+  const int per = (a.ncell + blockDim.x - 1) / blockDim.x;
+  const int b = threadIdx.x * per;
+  int s = 0;
+  for (int k = 0; k < per; ++k)
+    if (b + k < a.ncell) s += cnt[b + k];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int d = 1; d < blockDim.x; d <<= 1) {
+    int v = threadIdx.x >= d ? part[threadIdx.x - d] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int run = threadIdx.x > 0 ? part[threadIdx.x - 1] : 0;
+  for (int k = 0; k < per; ++k)
+    if (b + k < a.ncell) {
+      st[b + k] = run;
+      run += cnt[b + k];
+    }
+}
+
+__global__ void nl_scatter_kernel(NlArgs a) {
+  if (a.conditional && a.flags[0] == 0) return;
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.nrep * a.natoms) return;
+  const int rep = g / a.natoms, i = g % a.natoms;
+  const int cell = a.atom_cell[g];
+  const int slot = a.cell_start[rep * a.ncell + cell] + atomicAdd(&a.cell_fill[rep * a.ncell + cell], 1);
+  a.cell_atoms[(size_t)rep * a.natoms + slot] = i;
+}
+
+// one wave per centre
+__global__ void __launch_bounds__(256) nl_build_kernel(NlArgs a) {
+  if (a.conditional && a.flags[0] == 0) return;
+  extern __shared__ int nl_smem[];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int* list = nl_smem + wv * 2 * a.lcap;
+  int* sorted = list + a.lcap;
+  const int g = blockIdx.x * 4 + wv;
+  if (g >= a.nrep * a.natoms) return;
+  const int rep = g / a.natoms, i = g % a.natoms;
+  const float4* cn = a.canon + (size_t)rep * a.natoms;
+  const float4 ci = cn[i];
+  const float Lx = a.L[0], Ly = a.L[1], Lz = a.L[2];
+  int cnt = 0;
+  auto consider = [&](int atom, bool in) {
+    bool keep = false;
+    if (in && atom != i) keep = pair_d2(ci, cn[atom], Lx, Ly, Lz) < a.cut_sq;
+    const unsigned long long bal = __ballot(keep);
+    const int pre = __popcll(bal & ((1ull << lane) - 1ull));
+    if (keep && cnt + pre < a.lcap) list[cnt + pre] = atom;
+    cnt += __popcll(bal);
+  };
+  if (a.use_cells) {
+    const int cell = a.atom_cell[g];
+    const int cx = cell % a.nc[0], cy = (cell / a.nc[0]) % a.nc[1], cz = cell / (a.nc[0] * a.nc[1]);
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int x = (cx + dx + a.nc[0]) % a.nc[0];
+          const int y = (cy + dy + a.nc[1]) % a.nc[1];
+          const int z = (cz + dz + a.nc[2]) % a.nc[2];
+          const int cc = (z * a.nc[1] + y) * a.nc[0] + x;
+          const int st = a.cell_start[rep * a.ncell + cc];
+          const int num = a.cell_count[rep * a.ncell + cc];
+          for (int b = 0; b < num; b += 64) {
+            const bool in = b + lane < num;
+            const int atom = in ? a.cell_atoms[(size_t)rep * a.natoms + st + b + lane] : 0;
+            consider(atom, in);
+          }
+        }
+  } else {
+    for (int b = 0; b < a.natoms; b += 64) {
+      const bool in = b + lane < a.natoms;
+      consider(in ? b + lane : 0, in);
+    }
+  }
+  if (cnt > a.lcap) {
+    if (lane == 0) atomicOr(&a.flags[8], 1);
+    cnt = a.lcap;
+  }
+  nl_wave_sync();
+  for (int e = lane; e < cnt; e += 64) {
+    const int v = list[e];
+    int rank = 0;
+    for (int k = 0; k < cnt; ++k) rank += list[k] < v ? 1 : 0;
+    sorted[rank] = v;
+  }
+  nl_wave_sync();
+  int start = 0;
+  for (int j = 0; j < a.ntypes; ++j) {
+    int c = 0;
+    for (int e = lane; e < cnt; e += 64) {
+      const int v = sorted[e];
+      c += (v >= a.off[j] && v < a.off[j + 1]) ? 1 : 0;
+    }
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) c += __shfl_xor(c, s);
+    if (a.measure) {
+      if (lane == 0) atomicMax(&a.flags[4 + j], c);
+    } else {
+      const int capj = a.cap[j];
+      if (c > capj && lane == 0) atomicOr(&a.flags[2], 2);
+      int* row = a.nbr[j] + (size_t)g * capj;
+      for (int p = lane; p < capj; p += 64) row[p] = p < c ? sorted[start + p] - a.off[j] : a.natoms;
+      if (lane == 0) a.nbr_cnt[j][g] = c < capj ? c : capj;
+    }
+    start += c;
+  }
+}
+
+// drift of every atom against the canonical coordinates of the last build
+__global__ void nl_drift_kernel(const double* pos, const float4* canon, int total, float Lx, float Ly,
+                                float Lz, float sx, float sy, float sz, int* flags) {
+  __shared__ float red[256];
+  float mx = 0.f;
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += gridDim.x * blockDim.x) {
+    const float4 r = canon[g];
+    float d[3];
+    const float L[3] = {Lx, Ly, Lz}, sc[3] = {sx, sy, sz}, rv[3] = {r.x, r.y, r.z};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      float x = (float)pos[(size_t)g * 3 + k];
+      x = fmod_floor(x, L[k]) * sc[k];
+      float dd = x - rv[k];
+      const float h = L[k] / 2.f;
+      dd = fmod_floor(dd - h, L[k]) - h;
+      d[k] = dd;
+    }
+    const float n = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    mx = fmaxf(mx, n);
+  }
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(&flags[1], __float_as_int(red[0]));
+}
+
+__global__ void nl_decide_kernel(int* flags, float threshold) {
+  const float mx = __int_as_float(flags[1]);
+  flags[0] = (mx > threshold) ? 1 : 0;
+  flags[1] = 0;
+}
+
+__global__ void nl_set_flag_kernel(int* flags, int v) { flags[0] = v; }
+
+__global__ void nl_clear_kernel(int* flags, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) flags[i] = 0;
+}
+
+#pragma clang fp contract(on)
+
+static void fill_nl(System* s, NlArgs& a) {
+  memset(&a, 0, sizeof(a));
+  a.pos = s->d_pos;
+  a.canon = s->d_canon;
+  a.atom_cell = s->d_atom_cell;
+  a.cell_count = s->d_cell_count;
+  a.cell_start = s->d_cell_start;
+  a.cell_fill = s->d_cell_fill;
+  a.cell_atoms = s->d_cell_atoms;
+  for (int j = 0; j < s->ntypes; ++j) {
+    a.nbr[j] = s->d_nbr[j];
+    a.nbr_cnt[j] = s->d_nbr_cnt[j];
+    a.cap[j] = s->cap[j];
+  }
+  for (int j = 0; j <= s->ntypes; ++j) a.off[j] = s->off[j];
+  a.ntypes = s->ntypes;
+  a.natoms = s->natoms;
+  a.nrep = s->nrep;
+  for (int d = 0; d < 3; ++d) {
+    a.nc[d] = s->ncell[d];
+    a.L[d] = (float)s->ref_box[d];
+    a.scale[d] = (float)s->ref_box[d] / (float)s->box[d];
+  }
+  a.ncell = s->ncell[0] * s->ncell[1] * s->ncell[2];
+  a.use_cells = s->use_cells ? 1 : 0;
+  const float cut = (float)s->rc_skin;
+  a.cut_sq = cut * cut;
+  a.flags = s->d_flags;
+  a.lcap = s->nl_lds_list;
+}
+
+int32_t nlist_alloc_cells(System* s) {
+  // cell edge >= cutoff with a margin; the 27-cell stencil needs >= 3 cells per axis
+  bool cells = true;
+  for (int d = 0; d < 3; ++d) {
+    int nc = (int)(s->ref_box[d] / (s->rc_skin * (1.0 + 1e-4) + 1e-3));
+    s->ncell[d] = nc;
+    if (nc < 3) cells = false;
+  }
+  s->use_cells = cells;
+  if (!cells) s->ncell[0] = s->ncell[1] = s->ncell[2] = 1;
+  const size_t ncell = (size_t)s->ncell[0] * s->ncell[1] * s->ncell[2];
+  const size_t RN = (size_t)s->nrep * s->natoms;
+  hipFree(s->d_cell_count);
+  hipFree(s->d_cell_start);
+  hipFree(s->d_cell_fill);
+  hipFree(s->d_cell_atoms);
+  hipFree(s->d_atom_cell);
+  hipFree(s->d_canon);
+  s->d_cell_count = s->d_cell_start = s->d_cell_fill = s->d_cell_atoms = s->d_atom_cell = nullptr;
+  s->d_canon = nullptr;
+  DPMD_HIP(hipMalloc(&s->d_cell_count, sizeof(int) * ncell * s->nrep));
+  DPMD_HIP(hipMalloc(&s->d_cell_start, sizeof(int) * ncell * s->nrep));
+  DPMD_HIP(hipMalloc(&s->d_cell_fill, sizeof(int) * ncell * s->nrep));
+  DPMD_HIP(hipMalloc(&s->d_cell_atoms, sizeof(int) * RN));
+  DPMD_HIP(hipMalloc(&s->d_atom_cell, sizeof(int) * RN));
+  DPMD_HIP(hipMalloc(&s->d_canon, sizeof(float4) * RN));
+  DPMD_HIP(hipMemsetAsync(s->d_atom_cell, 0, sizeof(int) * RN, s->stream));
+  return DPMD_OK;
+}
+
+// measure: record the per-type maximum counts in flags[4..]; conditional: every kernel
+// returns immediately unless flags[0] was set by the drift test.
+int32_t nlist_build(System* s, bool measure, bool conditional) {
+  NlArgs a;
+  fill_nl(s, a);
+  a.measure = measure ? 1 : 0;
+  a.conditional = conditional ? 1 : 0;
+  if (!conditional) hipLaunchKernelGGL(nl_set_flag_kernel, dim3(1), dim3(1), 0, s->stream, s->d_flags, 1);
+  const int RN = s->nrep * s->natoms;
+  hipEvent_t ev = nullptr;
+  timer_begin(s, 0, &ev);
+  if (s->use_cells)
+    hipLaunchKernelGGL(nl_prepare_kernel, dim3(std::max(1, std::min(1024, (a.ncell * s->nrep + 255) / 256))),
+                       dim3(256), 0, s->stream, a);
+  hipLaunchKernelGGL(nl_canon_kernel, dim3((RN + 255) / 256), dim3(256), 0, s->stream, a);
+  if (s->use_cells) {
+    hipLaunchKernelGGL(nl_scan_kernel, dim3(s->nrep), dim3(1024), 0, s->stream, a);
+    hipLaunchKernelGGL(nl_scatter_kernel, dim3((RN + 255) / 256), dim3(256), 0, s->stream, a);
+  }
+  const size_t lds = sizeof(int) * 2 * a.lcap * 4;
+  hipLaunchKernelGGL(nl_build_kernel, dim3((RN + 3) / 4), dim3(256), lds, s->stream, a);
+  timer_end(s, 0, ev);
+  DPMD_HIP(hipGetLastError());
+  return DPMD_OK;
+}
+
+int32_t nlist_drift_check(System* s, float threshold) {
+  const int RN = s->nrep * s->natoms;
+  const int blocks = std::min(1024, (RN + 255) / 256);
+  hipLaunchKernelGGL(nl_drift_kernel, dim3(blocks), dim3(256), 0, s->stream, s->d_pos, s->d_canon, RN,
+                     (float)s->ref_box[0], (float)s->ref_box[1], (float)s->ref_box[2],
+                     (float)s->ref_box[0] / (float)s->box[0], (float)s->ref_box[1] / (float)s->box[1],
+                     (float)s->ref_box[2] / (float)s->box[2], s->d_flags);
+  hipLaunchKernelGGL(nl_decide_kernel, dim3(1), dim3(1), 0, s->stream, s->d_flags, threshold);
+  DPMD_HIP(hipGetLastError());
+  return DPMD_OK;
+}
+
+int32_t nlist_clear_flags(System* s) {
+  hipLaunchKernelGGL(nl_clear_kernel, dim3(1), dim3(64), 0, s->stream, s->d_flags, 16);
+  DPMD_HIP(hipGetLastError());
+  return DPMD_OK;
+}
+
+}  // namespace dpmd

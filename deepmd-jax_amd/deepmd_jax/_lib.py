@@ -1,0 +1,283 @@
+"""ctypes binding of libdpmd_hip.so (C ABI declared in include/dpmd.h).
+
+The library is the only compute path: importing this module fails loudly when the
+shared object is missing, and every call raises on a non-zero status.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+MAXT = 4
+PREC = {'fp32': 0, 'bf16': 1, 'fp16': 2}
+ROUTINE = {'NVE': 0, 'NVT': 1, 'NVT_langevin': 2, 'PIMD': 3}
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, 'libdpmd_hip.so')
+
+c_i32, c_f64, c_vp = ctypes.c_int32, ctypes.c_double, ctypes.c_void_p
+
+
+class ModelDesc(ctypes.Structure):
+    _fields_ = [('ntypes', c_i32), ('embed_widths', c_i32 * 3), ('fit_widths', c_i32 * 3), ('axis', c_i32),
+                ('rcut', c_f64), ('sr_mean', c_f64 * MAXT), ('sr_std', c_f64 * MAXT), ('nnbrs', c_f64),
+                ('ebias', c_f64 * MAXT), ('out_norm', c_f64), ('tbias', c_vp),
+                ('emb_kernel', (c_vp * 3) * (MAXT * MAXT)), ('emb_bias', (c_vp * 3) * (MAXT * MAXT)),
+                ('fit_kernel', (c_vp * 4) * MAXT), ('fit_bias', (c_vp * 4) * MAXT), ('fit_dt', (c_vp * 3) * MAXT)]
+
+
+class MDParams(ctypes.Structure):
+    _fields_ = [('routine', c_i32), ('precision', c_i32), ('dt', c_f64), ('kT', c_f64), ('gamma', c_f64),
+                ('tau', c_f64), ('chain_length', c_i32), ('record_every', c_i32), ('seed', ctypes.c_uint64),
+                ('pimd_lambda', c_f64)]
+
+
+class DPMDError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libdpmd_hip.so once; raise if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise DPMDError('%s not found: build it with `python deepmd-jax_amd/build.py`' % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER
+    sig = {
+        'dpmd_last_error': (ctypes.c_char_p, []),
+        'dpmd_version': (c_i32, []),
+        'dpmd_model_create': (c_i32, [P(ModelDesc), c_i32, P(c_vp)]),
+        'dpmd_model_destroy': (c_i32, [c_vp]),
+        'dpmd_system_create': (c_i32, [c_vp, c_i32, c_i32, c_vp, P(c_vp)]),
+        'dpmd_system_destroy': (c_i32, [c_vp]),
+        'dpmd_system_set_box': (c_i32, [c_vp, c_vp]),
+        'dpmd_system_set_positions': (c_i32, [c_vp, c_vp]),
+        'dpmd_system_get_positions': (c_i32, [c_vp, c_vp]),
+        'dpmd_system_set_velocities': (c_i32, [c_vp, c_vp]),
+        'dpmd_system_get_velocities': (c_i32, [c_vp, c_vp]),
+        'dpmd_system_set_masses': (c_i32, [c_vp, c_vp]),
+        'dpmd_nlist_allocate': (c_i32, [c_vp, c_f64, c_f64, c_vp, c_vp]),
+        'dpmd_nlist_update': (c_i32, [c_vp, c_vp]),
+        'dpmd_nlist_export': (c_i32, [c_vp, c_i32, c_vp]),
+        'dpmd_nlist_capacity': (c_i32, [c_vp, c_vp]),
+        'dpmd_compute': (c_i32, [c_vp, c_i32, c_vp, c_vp, c_vp, c_vp]),
+        'dpmd_md_run': (c_i32, [c_vp, P(MDParams), c_i32, c_vp, c_vp, c_vp]),
+        'dpmd_pimd_setup': (c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp]),
+        'dpmd_kinetic_energy': (c_i32, [c_vp, c_vp]),
+        'dpmd_thermostat_energy': (c_i32, [c_vp, c_vp]),
+        'dpmd_timing_enable': (c_i32, [c_vp, c_i32]),
+        'dpmd_timing_get': (c_i32, [c_vp, c_i32, c_vp, c_vp]),
+        'dpmd_pair_count': (c_i32, [c_vp, c_vp]),
+        'dpmd_synchronize': (c_i32, [c_vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().dpmd_last_error().decode(errors='replace')
+        if rc == -1:
+            raise ValueError(msg)
+        if rc == -4:
+            raise NotImplementedError(msg)
+        raise DPMDError('libdpmd_hip error %d: %s' % (rc, msg))
+
+
+def ptr(a):
+    return a.ctypes.data_as(c_vp) if a is not None else None
+
+
+def exported_symbols():
+    """Names of the C-ABI entry points bound here (all declared in include/dpmd.h)."""
+    lib()
+    return ['dpmd_last_error', 'dpmd_version', 'dpmd_model_create', 'dpmd_model_destroy', 'dpmd_system_create',
+            'dpmd_system_destroy', 'dpmd_system_set_box', 'dpmd_system_set_positions', 'dpmd_system_get_positions',
+            'dpmd_system_set_velocities', 'dpmd_system_get_velocities', 'dpmd_system_set_masses',
+            'dpmd_nlist_allocate', 'dpmd_nlist_update', 'dpmd_nlist_export', 'dpmd_nlist_capacity', 'dpmd_compute',
+            'dpmd_md_run', 'dpmd_pimd_setup', 'dpmd_kinetic_energy', 'dpmd_thermostat_energy', 'dpmd_timing_enable',
+            'dpmd_timing_get', 'dpmd_pair_count', 'dpmd_synchronize']
+
+
+class DeviceModel:
+    """Model weights resident on one GPU (replaces load_model's device_put)."""
+
+    def __init__(self, params, variables, device=0):
+        P = variables['params']
+        d = ModelDesc()
+        nt = int(params['ntypes'])
+        if params.get('use_mp', False) or params.get('atomic', False):
+            raise NotImplementedError('only the se_a energy model (no message passing, no atomic output) is supported')
+        if not params.get('use_2nd', True):
+            raise NotImplementedError('use_2nd=False is not supported')
+        if params.get('is_compressed', False):
+            raise NotImplementedError('compressed (tabulated) embedding nets are not supported yet')
+        if list(params.get('valid_types', range(nt))) != list(range(nt)):
+            raise NotImplementedError('models with valid_types != range(ntypes) are not supported')
+        ew = list(params['embed_widths'])
+        fw = list(params['fit_widths'])
+        if len(ew) != 3 or len(fw) != 3:
+            raise NotImplementedError('kernels implement 3 embedding layers and 3 fitting layers')
+        d.ntypes = nt
+        for i in range(3):
+            d.embed_widths[i] = int(ew[i])
+            d.fit_widths[i] = int(fw[i])
+        d.axis = int(params['axis'])
+        d.rcut = float(params['rcut'])
+        for t in range(nt):
+            d.sr_mean[t] = float(np.asarray(params['sr_mean'])[t])
+            d.sr_std[t] = float(np.asarray(params['sr_std'])[t])
+            eb = params.get('Ebias')
+            d.ebias[t] = 0.0 if eb is None else float(np.asarray(eb).reshape(-1)[t])
+        d.nnbrs = float(params['Nnbrs'])
+        d.out_norm = float(params.get('out_norm', 1.0))
+        keep = []
+
+        def arr(x):
+            a = np.ascontiguousarray(np.asarray(x, np.float32))
+            keep.append(a)
+            return a.ctypes.data
+
+        d.tbias = arr(P['Tbias'])
+        for k in range(nt * nt):
+            net = P['embedding_net_%d' % k]
+            for l in range(3):
+                d.emb_kernel[k][l] = arr(net['Dense_%d' % l]['kernel'])
+                d.emb_bias[k][l] = arr(net['Dense_%d' % l]['bias'])
+        for t in range(nt):
+            net = P['fitting_net_%d' % t]
+            for l in range(4):
+                d.fit_kernel[t][l] = arr(net['Dense_%d' % l]['kernel'])
+                b = net['Dense_%d' % l].get('bias')
+                d.fit_bias[t][l] = arr(b) if b is not None else None
+            for l in (1, 2):
+                d.fit_dt[t][l] = arr(net['dt%d' % l]) if ('dt%d' % l) in net else None
+        h = c_vp()
+        check(lib().dpmd_model_create(ctypes.byref(d), int(device), ctypes.byref(h)))
+        self.handle = h
+        self.params = params
+        self.ntypes = nt
+        self.rcut = float(params['rcut'])
+
+    def __del__(self):
+        if getattr(self, 'handle', None) and _lib is not None:
+            _lib.dpmd_model_destroy(self.handle)
+            self.handle = None
+
+
+class DeviceSystem:
+    """natoms x nreplica atoms resident on the GPU, in the library's type-sorted layout."""
+
+    def __init__(self, model, type_idx, nreplica=1):
+        self.model = model
+        self.type_idx = np.ascontiguousarray(np.asarray(type_idx, np.int32))
+        self.natoms = len(self.type_idx)
+        self.nrep = int(nreplica)
+        h = c_vp()
+        check(lib().dpmd_system_create(model.handle, self.natoms, self.nrep, ptr(self.type_idx), ctypes.byref(h)))
+        self.handle = h
+        self.knbr = None
+
+    def __del__(self):
+        if getattr(self, 'handle', None) and _lib is not None:
+            _lib.dpmd_system_destroy(self.handle)
+            self.handle = None
+
+    def set_box(self, box3):
+        b = np.ascontiguousarray(np.asarray(box3, np.float64).reshape(3))
+        check(lib().dpmd_system_set_box(self.handle, ptr(b)))
+
+    def set_positions(self, pos):
+        p = np.ascontiguousarray(np.asarray(pos, np.float64).reshape(self.nrep, self.natoms, 3))
+        check(lib().dpmd_system_set_positions(self.handle, ptr(p)))
+
+    def get_positions(self):
+        p = np.empty((self.nrep, self.natoms, 3))
+        check(lib().dpmd_system_get_positions(self.handle, ptr(p)))
+        return p
+
+    def set_velocities(self, vel):
+        v = np.ascontiguousarray(np.asarray(vel, np.float64).reshape(self.nrep, self.natoms, 3))
+        check(lib().dpmd_system_set_velocities(self.handle, ptr(v)))
+
+    def get_velocities(self):
+        v = np.empty((self.nrep, self.natoms, 3))
+        check(lib().dpmd_system_get_velocities(self.handle, ptr(v)))
+        return v
+
+    def set_masses(self, mass):
+        m = np.ascontiguousarray(np.asarray(mass, np.float64).reshape(self.natoms))
+        check(lib().dpmd_system_set_masses(self.handle, ptr(m)))
+
+    def nlist_allocate(self, rc_skin, buffer_ratio=1.2, knbr_override=None):
+        out = np.zeros(MAXT, np.int32)
+        ko = None if knbr_override is None else np.ascontiguousarray(np.asarray(knbr_override, np.int32))
+        check(lib().dpmd_nlist_allocate(self.handle, float(rc_skin), float(buffer_ratio), ptr(ko), ptr(out)))
+        self.knbr = [int(k) for k in out[:self.model.ntypes]]
+        return self.knbr
+
+    def nlist_update(self):
+        ov = np.zeros(1, np.int32)
+        check(lib().dpmd_nlist_update(self.handle, ptr(ov)))
+        return bool(ov[0])
+
+    def nlist_export(self, j):
+        cap = self.knbr[j] - 1
+        out = np.empty((self.nrep, self.natoms, max(cap, 1)), np.int32)
+        check(lib().dpmd_nlist_export(self.handle, int(j), ptr(out)))
+        return out[..., :cap]
+
+    def compute(self, precision='fp32', want_atom=False):
+        e = np.zeros(self.nrep)
+        f = np.zeros((self.nrep, self.natoms, 3))
+        v = np.zeros((self.nrep, 9))
+        ea = np.zeros((self.nrep, self.natoms)) if want_atom else None
+        check(lib().dpmd_compute(self.handle, PREC[precision], ptr(e), ptr(f), ptr(v), ptr(ea)))
+        return e, f, v.reshape(self.nrep, 3, 3), ea
+
+    def md_run(self, params, nsteps, record=True):
+        tp = np.empty((nsteps, self.nrep, self.natoms, 3)) if record else None
+        tv = np.empty((nsteps, self.nrep, self.natoms, 3)) if record else None
+        err = np.zeros(1, np.uint32)
+        check(lib().dpmd_md_run(self.handle, ctypes.byref(params), int(nsteps), ptr(tp), ptr(tv), ptr(err)))
+        return tp, tv, int(err[0])
+
+    def pimd_setup(self, C, freqs, c1, c2):
+        a = [np.ascontiguousarray(np.asarray(x, np.float64)) for x in (C, freqs, c1, c2)]
+        check(lib().dpmd_pimd_setup(self.handle, *[ptr(x) for x in a]))
+
+    def kinetic_energy(self):
+        ke = np.zeros(self.nrep)
+        check(lib().dpmd_kinetic_energy(self.handle, ptr(ke)))
+        return ke
+
+    def thermostat_energy(self):
+        e = np.zeros(self.nrep)
+        check(lib().dpmd_thermostat_energy(self.handle, ptr(e)))
+        return e
+
+    def timing(self, on=None, which=None):
+        if on is not None:
+            check(lib().dpmd_timing_enable(self.handle, 1 if on else 0))
+            return None
+        ms = np.zeros(1)
+        cnt = np.zeros(1, np.int64)
+        check(lib().dpmd_timing_get(self.handle, int(which), ptr(ms), ptr(cnt)))
+        return float(ms[0]), int(cnt[0])
+
+    def pair_count(self):
+        p = np.zeros(1)
+        check(lib().dpmd_pair_count(self.handle, ptr(p)))
+        return float(p[0])
+
+    def synchronize(self):
+        check(lib().dpmd_synchronize(self.handle))

@@ -1,0 +1,80 @@
+"""Frame evaluation on the GPU engine.
+
+Boxes shorter than 2*rcut, which deepmd-jax evaluates on its dense lattice-image
+path (deepmd_jax/utils.py:66-78, deepmd_jax/data.py:347-364), are evaluated here as
+a k1 x k2 x k3 periodic supercell with every k_d*L_d >= 2*rcut: each atom's images
+have identical environments, so E = E_super / prod(k), the force on an atom is the
+force on its first image and the virial is W_super / prod(k).
+"""
+import numpy as np
+
+from . import _lib
+
+
+def canonical_box(box):
+    """(3,) orthorhombic box lengths from a scalar, (3,), (9,) or (3,3) box."""
+    b = np.asarray(box, np.float64)
+    if b.size == 1:
+        return np.full(3, float(b.reshape(())))
+    if b.size == 3:
+        return b.reshape(3).copy()
+    if b.size == 9:
+        m = b.reshape(3, 3)
+        if np.any(m - np.diag(np.diag(m))):
+            raise NotImplementedError('only orthorhombic boxes are supported')
+        return np.diag(m).copy()
+    raise ValueError('box must be a scalar, (3,), (9,) or (3,3)')
+
+
+def supercell_factors(box3, rcut):
+    return np.maximum(1, np.ceil(2.0 * rcut / np.asarray(box3) * (1 + 1e-12))).astype(int)
+
+
+def replicate(coord, box3, type_idx, k):
+    """Coordinates of the k[0] x k[1] x k[2] supercell; image 0 holds the original atoms."""
+    shifts = np.stack(np.meshgrid(np.arange(k[0]), np.arange(k[1]), np.arange(k[2]), indexing='ij'), -1).reshape(-1, 3)
+    c = (coord[None, :, :] + (shifts * box3)[:, None, :]).reshape(-1, 3)
+    return c, box3 * k, np.tile(type_idx, len(shifts))
+
+
+class Evaluator:
+    """Energies, forces and virials of orthorhombic frames for one model."""
+
+    def __init__(self, params, variables, device=0, model=None):
+        self.params = params
+        self.model = model if model is not None else _lib.DeviceModel(params, variables, device)
+        self.rcut = float(params['rcut'])
+        self._systems = {}
+
+    def _system(self, type_idx, nrep):
+        key = (tuple(np.asarray(type_idx).tolist()), nrep)
+        s = self._systems.get(key)
+        if s is None:
+            s = _lib.DeviceSystem(self.model, type_idx, nrep)
+            self._systems[key] = s
+        return s
+
+    def evaluate_batch(self, coords, box3, type_idx, precision='fp32'):
+        """coords (F,N,3) sharing one box -> (E (F,), F (F,N,3), virial (F,3,3))."""
+        coords = np.asarray(coords, np.float64)
+        box3 = canonical_box(box3)
+        type_idx = np.asarray(type_idx)
+        k = supercell_factors(box3, self.rcut)
+        nk = int(np.prod(k))
+        F, N = coords.shape[:2]
+        if nk > 1:
+            cs = [replicate(c, box3, type_idx, k) for c in coords]
+            sc = np.stack([c[0] for c in cs])
+            sbox, stype = cs[0][1], cs[0][2]
+        else:
+            sc, sbox, stype = coords, box3, type_idx
+        sys = self._system(stype, F)
+        sys.set_box(sbox)
+        sys.set_positions(sc)
+        sys.nlist_allocate(self.rcut, 1.2)
+        e, f, v, _ = sys.compute(precision)
+        return e / nk, f[:, :N], v / nk
+
+    def evaluate_frame(self, coord, box3, type_idx, precision='fp32'):
+        e, f, v = self.evaluate_batch(np.asarray(coord)[None], box3, type_idx, precision)
+        return e[0], f[0], v[0]

@@ -1,0 +1,347 @@
+"""`Simulation`: the MD driver surface of deepmd_jax/md.py:48-1090 on the GPU engine.
+
+The host loop follows the reference's run protocol (deepmd_jax/md.py:874-964):
+chunks of max(10, min(100, 100000 // (N P))) steps, each chunk one device call
+(dpmd_md_run: drift test, conditional neighbour rebuild, integrator, force
+evaluation per step, no host synchronisation inside), then the error-code check
+with overflow recovery and chunk re-run (deepmd_jax/md.py:644-676,906-916), the
+trajectory copy and the periodic report (deepmd_jax/md.py:473-595).
+
+Routines: 'NVE' (velocity Verlet), 'NVT' (Nose-Hoover chain; with n_bead > 1
+ring-polymer PILE-L), 'NVT_langevin' (BAOAB).  Positions and momenta are FP64 on
+the device.  Not provided: NPT, fixed atoms, model deviation, XYZ dumps, DPLR.
+"""
+import time
+
+import numpy as np
+
+from . import _lib
+from .constants import MASS_UNIT_CONVERSION, TEMP_UNIT_CONVERSION, PRESS_UNIT_CONVERSION
+from .evaluator import canonical_box
+from .pimd import normal_mode_transform, pile_l_factors, centroid
+from .utils import load_model
+
+
+def print_run_profile(steps, elapsed_time, natoms, dt):
+    """Per-run timing summary (deepmd_jax/md_utils.py:172-179)."""
+    steps_per_microsecond_per_atom = 1e-6 * steps * natoms / (elapsed_time + 1e-6)
+    nanosecond_per_day = 1e-6 * steps * dt * (86400 / elapsed_time)
+    print('# Finished %d steps in %dh %dm %ds.' %
+          (steps, elapsed_time // 3600, (elapsed_time % 3600) // 60, elapsed_time % 60))
+    print('# Performance: %.3f ns/day, %.3f step/μs/atom' % (nanosecond_per_day, steps_per_microsecond_per_atom))
+
+
+class Simulation:
+    step: int = 0
+
+    def __init__(self, model_path, box, type_idx, mass, routine, dt, initial_position, initial_velocity=None,
+                 report_interval=200, temperature=None, pressure=None, debug=False, seed=0,
+                 model_deviation_paths=[], use_neighbor_list_when_possible=True, neighbor_skin=None,
+                 neighbor_buffer_ratio=1.2, remove_com_motion=False, tau_t=100., tau_p=1000., chain_length_t=3,
+                 chain_length_p=3, chain_steps_t=1, chain_steps_p=1, sy_steps_t=1, sy_steps_p=1,
+                 fixed_indices=None, couple_axes=((0, 1, 2),), n_bead=1, type_symbols=None, pimd_lambda=1.0,
+                 pimd_print_invariant=False, precision='fp32', device=0, model=None):
+        self.report_interval = int(report_interval)
+        self.log = []
+        self.step = 0
+        self._n_bead = int(n_bead)
+        self._is_pimd = self._n_bead > 1
+        self._pimd_lambda = float(pimd_lambda)
+        if self._pimd_lambda < 0:
+            raise ValueError('pimd_lambda must be non-negative.')
+        if 'NPT' in routine:
+            raise NotImplementedError('NPT is not provided by this engine.')
+        if routine not in ('NVE', 'NVT', 'NVT_langevin'):
+            raise NotImplementedError("routine currently limited to 'NVE', 'NVT', 'NVT_langevin', 'NPT'")
+        if fixed_indices is not None:
+            raise NotImplementedError('fixed_indices is not provided by this engine.')
+        if len(model_deviation_paths) > 0:
+            raise NotImplementedError('model_deviation_paths is not provided by this engine.')
+        if remove_com_motion:
+            raise NotImplementedError('remove_com_motion is not provided by this engine.')
+        if not (isinstance(couple_axes, tuple) and couple_axes == ((0, 1, 2),)):
+            raise ValueError("couple_axes is only meaningful for routine='NPT'")
+        if chain_steps_t != 1 or sy_steps_t != 1:
+            raise NotImplementedError('Nose-Hoover chain_steps / sy_steps other than 1 are not provided.')
+        pos = np.asarray(initial_position, np.float64)
+        if self._is_pimd:
+            if routine != 'NVT':
+                raise NotImplementedError("Path-integral MD (n_bead > 1) currently supports routine='NVT' or 'NPT'.")
+            if temperature is None:
+                raise ValueError('PIMD needs a temperature.')
+            if pos.ndim == 2:
+                pos = np.broadcast_to(pos, (self._n_bead,) + pos.shape).copy()
+            elif pos.ndim == 3:
+                if pos.shape[0] != self._n_bead:
+                    raise ValueError('initial_position leading axis %d does not match n_bead=%d.'
+                                     % (pos.shape[0], self._n_bead))
+            else:
+                raise ValueError('initial_position must have shape (N,3) or (P,N,3).')
+            self._natoms = pos.shape[1]
+        else:
+            if pos.ndim != 2 or pos.shape[1] != 3:
+                raise ValueError('initial_position must have shape (N,3).')
+            self._natoms = pos.shape[0]
+            pos = pos[None]
+        self._temperature = temperature
+        self._dt = float(dt)
+        self._routine = routine
+        self._precision = precision
+        if model is None:
+            self._model, self._variables = load_model(model_path)
+            self._device_model = _lib.DeviceModel(self._model.params, self._variables, device)
+        else:
+            self._model, self._variables, self._device_model = model
+        params = self._model.params
+        type_idx_np = np.asarray(type_idx).astype(int)
+        ct = params.get('chemical_types')
+        if ct is not None:
+            unknown = set(type_idx_np.tolist()) - set(ct)
+            if unknown:
+                raise ValueError('Atomic numbers %s in type_idx are not in model.params["chemical_types"]=%s'
+                                 % (sorted(unknown), ct))
+            z2i = {z: i for i, z in enumerate(ct)}
+            type_idx_np = np.array([z2i[z] for z in type_idx_np], dtype=int)
+        if len(type_idx_np) != self._natoms:
+            raise ValueError('type_idx length does not match the number of atoms.')
+        if type_idx_np.min() < 0 or type_idx_np.max() >= params['ntypes']:
+            raise ValueError('type index out of trained scope of model (ntypes=%d)' % params['ntypes'])
+        self._type_idx = type_idx_np
+        self._mass_amu = np.asarray(mass, np.float64)[self._type_idx]
+        self._mass = self._mass_amu * MASS_UNIT_CONVERSION
+        self._type_symbols = type_symbols
+        self._debug = debug
+        self._step_chunk_size = max(10, min(100, 100000 // (self._natoms * self._n_bead)))
+        self._neighbor_skin = 0.3 if neighbor_skin is None else float(neighbor_skin)
+        self._neighbor_buffer_ratio = float(neighbor_buffer_ratio)
+        self._box = canonical_box(box)
+        self._rcut = float(params['rcut'])
+        if self._box.min() < 2 * self._rcut:
+            raise NotImplementedError('MD needs box lengths >= 2*rcut (%.3f); use a supercell' % (2 * self._rcut))
+        # The reference switches to its dense lattice path below 2*(rcut+skin); the
+        # neighbour list is exact for any skin with 2*(rcut+skin) <= L, so the skin is
+        # shrunk to fit instead.
+        if self._box.min() <= 2 * (self._rcut + self._neighbor_skin) or not use_neighbor_list_when_possible:
+            self._neighbor_skin = max(0.0, 0.5 * self._box.min() * (1 - 1e-9) - self._rcut)
+        self._sys = _lib.DeviceSystem(self._device_model, self._type_idx, self._n_bead)
+        self._sys.set_box(self._box)
+        self._sys.set_masses(self._mass)
+        self._sys.set_positions(pos)
+        self._allocate_nlist()
+        self._kT = (temperature if temperature is not None else 0.0) * TEMP_UNIT_CONVERSION
+        self._params = _lib.MDParams()
+        self._params.precision = _lib.PREC[precision]
+        self._params.dt = self._dt
+        self._params.kT = self._kT
+        self._params.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+        self._params.tau = float(tau_t)
+        self._params.chain_length = int(chain_length_t)
+        self._params.gamma = 1.0 / float(tau_t)
+        self._params.pimd_lambda = self._pimd_lambda
+        if self._is_pimd:
+            self._params.routine = _lib.ROUTINE['PIMD']
+            freqs, C = normal_mode_transform(self._n_bead, self._kT)
+            _, c1, c2 = pile_l_factors(freqs, self._dt, self._kT, self._n_bead, float(tau_t), self._pimd_lambda)
+            self._sys.pimd_setup(C, freqs, c1, c2)
+            self._nm = (freqs, C)
+        else:
+            self._params.routine = _lib.ROUTINE[routine]
+        # initial velocities: MB at kT (COM momentum removed), or the caller's
+        rng = np.random.default_rng(seed)
+        if initial_velocity is not None:
+            self.setVelocity(initial_velocity)
+        else:
+            kTP = self._kT * self._n_bead
+            p = rng.standard_normal((self._n_bead, self._natoms, 3)) * np.sqrt(self._mass * kTP)[None, :, None]
+            if not self._is_pimd:
+                p -= p.mean(1, keepdims=True)
+            self._sys.set_velocities(p / self._mass[None, :, None])
+        self._is_initial_state = True
+        self._error_code = 0
+        labels = {('NVE', False): 'NVE run (velocity-Verlet integrator)',
+                  ('NVT', False): 'NVT run (Nose-Hoover thermostat)',
+                  ('NVT_langevin', False): 'NVT Langevin run (BAOAB Langevin thermostat)',
+                  ('NVT', True): 'NVT PIMD run (PILE-L Langevin thermostat)'}
+        info = []
+        if temperature is not None and routine != 'NVE':
+            info.append('temperature=%g K' % temperature)
+        if self._is_pimd:
+            info += ['n_beads=%d' % self._n_bead, 'pimd_lambda=%g' % self._pimd_lambda]
+        print('# Initialized %s with %d atoms%s' % (labels[(routine, self._is_pimd)], self._natoms,
+                                                   (' (%s)' % ', '.join(info)) if info else ''))
+
+    # ---------------------------------------------------------------- internals
+    def _allocate_nlist(self, knbr=None):
+        rc_skin = self._rcut + self._neighbor_skin
+        self._sys.nlist_allocate(rc_skin, self._neighbor_buffer_ratio, knbr)
+        print('# Neighborlist allocated with size %s, rcut_plus_skin = %g, buffer_ratio = %g'
+              % (np.array(self._sys.knbr) - 1, rc_skin, self._neighbor_buffer_ratio))
+
+    def _state_pos(self):
+        return self._sys.get_positions()
+
+    def _report_values(self):
+        e, f, v, _ = self._sys.compute(self._precision)
+        ke = self._sys.kinetic_energy()
+        P = self._n_bead
+        pe = float(e.mean()) if self._is_pimd else float(e[0])
+        ke_tot = float(ke.sum())
+        N = self._natoms
+        temp = 2 * ke_tot / (3 * N * P) / TEMP_UNIT_CONVERSION / P
+        rep = {'Temperature': temp, 'KE': ke_tot / P ** 2, 'PE': pe}
+        if self._routine == 'NVE':
+            rep['Invariant'] = ke_tot + pe
+        elif self._routine == 'NVT_langevin':
+            rep['Invariant'] = ke_tot + pe - float(self._sys.thermostat_energy()[0])
+        elif self._routine == 'NVT' and not self._is_pimd:
+            rep['Invariant'] = ke_tot + pe + float(self._sys.thermostat_energy()[0])
+        self._last_force = f
+        self._last_virial = v
+        self._last_energy = e
+        return rep
+
+    def _print_report(self):
+        rep = self._report_values()
+        keys = list(rep.keys())
+        if self._is_initial_state:
+            print(' '.join(['%-8s' % 'Step'] + ['%-12s' % k for k in keys] + ['%-6s' % 'Time']))
+        now = time.time()
+        row = [self.step] + [rep[k] for k in keys] + [now - self._tic_between_report]
+        self.log.append(row)
+        print('%-8d ' % row[0] + ' '.join('%-12.3f' % x for x in row[1:-1]) + ' %-6.3f' % row[-1])
+        self._tic_between_report = now
+
+    def _initialize_run(self, steps):
+        print('# Running %d steps...' % steps)
+        traj_length = steps + int(self._is_initial_state)
+        self._offset = self.step - int(self._is_initial_state)
+        shape = (self._n_bead, self._natoms, 3) if self._is_pimd else (self._natoms, 3)
+        self._position_trajectory = np.zeros((traj_length,) + shape)
+        self._velocity_trajectory = np.zeros((traj_length,) + shape)
+        self._box_trajectory = np.zeros((traj_length, 3))
+        if self._is_pimd:
+            self._centroid_trajectory = np.zeros((traj_length, self._natoms, 3))
+        if self._is_initial_state:
+            self._position_trajectory[0] = self.getPosition()
+            self._velocity_trajectory[0] = self.getVelocity()
+            self._box_trajectory[0] = self.getBox()
+            if self._is_pimd:
+                self._centroid_trajectory[0] = self.getCentroidPosition()
+        self._tic_of_this_run = time.time()
+        self._tic_between_report = time.time()
+        self._error_code = 0
+        self._print_report()
+        self._is_initial_state = False
+
+    def _resolve_overflow(self):
+        """deepmd_jax/md.py:644-676: reallocate; grow the buffer ratio on a repeat."""
+        if self._error_code & 4:
+            self._neighbor_buffer_ratio += 0.05
+        self._allocate_nlist()
+        print('# Successfully resolved overflow code', self._error_code)
+        self._error_code = 4
+
+    def run(self, steps, dump_interval=None, dump_prefix=None, dump_content='positions', dump_mode='overwrite'):
+        if dump_prefix is not None:
+            raise NotImplementedError('XYZ dumps are not provided by this engine.')
+        self._initialize_run(steps)
+        remaining = steps
+        while remaining > 0:
+            nxt = min(self.report_interval - self.step % self.report_interval, self._step_chunk_size, remaining)
+            tp, tv, err = self._sys.md_run(self._params, nxt, record=True)
+            if err & 16:
+                print('# Warning: Nan or Inf encountered in simulation. Terminating.')
+                self._error_code |= 16
+                remaining = 0
+            if err & 2:
+                self._error_code |= 2
+                self._resolve_overflow()
+                continue
+            if not err & 16:
+                self._error_code &= ~4
+            lo = self.step - self._offset
+            if self._is_pimd:
+                self._position_trajectory[lo:lo + nxt] = tp
+                self._velocity_trajectory[lo:lo + nxt] = tv
+                for k in range(nxt):
+                    self._centroid_trajectory[lo + k] = centroid(tp[k], self._box)
+            else:
+                self._position_trajectory[lo:lo + nxt] = tp[:, 0]
+                self._velocity_trajectory[lo:lo + nxt] = tv[:, 0]
+            self._box_trajectory[lo:lo + nxt] = self._box
+            self.step += nxt
+            remaining -= nxt
+            if self.step % self.report_interval == 0 or remaining == 0:
+                self._print_report()
+        print_run_profile(steps, time.time() - self._tic_of_this_run, self._natoms * self._n_bead, self._dt)
+        traj = {'position': self._position_trajectory, 'velocity': self._velocity_trajectory,
+                'box': self._box_trajectory}
+        if self._is_pimd:
+            traj['centroid'] = self._centroid_trajectory
+        return traj
+
+    # ------------------------------------------------------------ getters/setters
+    def getPosition(self):
+        p = self._state_pos()
+        return p if self._is_pimd else p[0]
+
+    def getCentroidPosition(self):
+        p = self._state_pos()
+        return centroid(p, self._box) if self._is_pimd else p[0]
+
+    def setPosition(self, position, box=None):
+        if box is not None:
+            raise ValueError('Box can only be changed in NPT simulations.')
+        position = np.asarray(position, np.float64)
+        if self._is_pimd and position.ndim == 2:
+            position = np.broadcast_to(position, (self._n_bead,) + position.shape)
+        expect = (self._n_bead, self._natoms, 3) if self._is_pimd else (self._natoms, 3)
+        if position.shape != expect:
+            raise ValueError('Position must have the same shape as the initial position, or you have to create a '
+                             'new Simulation instance.')
+        self._sys.set_positions(position.reshape(self._n_bead, self._natoms, 3))
+        if self._sys.nlist_update():
+            self._allocate_nlist()
+
+    def getVelocity(self):
+        v = self._sys.get_velocities()
+        return v if self._is_pimd else v[0]
+
+    def setVelocity(self, velocity):
+        v = np.asarray(velocity, np.float64)
+        if self._is_pimd and v.ndim == 2:
+            v = np.broadcast_to(v, (self._n_bead,) + v.shape)
+        self._sys.set_velocities(v.reshape(self._n_bead, self._natoms, 3))
+
+    def setRandomVelocity(self, temperature, remove_com_motion=True, seed=None):
+        rng = np.random.default_rng(seed)
+        kT = temperature * TEMP_UNIT_CONVERSION
+        v = rng.standard_normal((self._n_bead, self._natoms, 3)) * np.sqrt(kT * self._n_bead / self._mass)[None, :, None]
+        if remove_com_motion:
+            m = self._mass[None, :, None]
+            v = v - (v * m).sum((0, 1)) / (m.sum() * self._n_bead)
+        self._sys.set_velocities(v)
+
+    def getBox(self):
+        return self._box.copy()
+
+    def getEnergy(self):
+        e, _, _, _ = self._sys.compute(self._precision)
+        return float(e.mean())
+
+    def getForce(self):
+        _, f, _, _ = self._sys.compute(self._precision)
+        return f if self._is_pimd else f[0]
+
+    def getVirial(self):
+        _, _, v, _ = self._sys.compute(self._precision)
+        return v if self._is_pimd else v[0]
+
+    def getPressure(self):
+        """Isotropic pressure (bar): (N kT - dU/deps / 3) / V, deepmd_jax/md_utils.py:139-159."""
+        if self._temperature is None:
+            raise ValueError('getPressure needs a temperature')
+        _, _, v, _ = self._sys.compute(self._precision)
+        dUdeps = -float(np.trace(v.mean(0)))
+        V = float(np.prod(self._box))
+        return (self._natoms * self._kT - dUdeps / 3.0) / V / PRESS_UNIT_CONVERSION

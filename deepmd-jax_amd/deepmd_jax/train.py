@@ -1,0 +1,71 @@
+"""`evaluate()` with the signature and return value of deepmd_jax/train.py:703-788.
+
+Training (deepmd_jax/train.py:15-520) is not part of this engine.
+"""
+import numpy as np
+
+from .evaluator import Evaluator, canonical_box
+from .utils import load_model
+
+
+def _boxes(box, nframes):
+    box = np.asarray(box, np.float64)
+    if box.ndim == 1:
+        return np.stack([canonical_box(b) for b in box])
+    if box.ndim == 2:
+        if box.shape[1] in (1, 3, 9):
+            return np.stack([canonical_box(b) for b in box])
+        raise ValueError
+    if box.ndim == 3 and box.shape[1:] == (3, 3):
+        return np.stack([canonical_box(b) for b in box])
+    raise ValueError
+
+
+def evaluate(model_path, coord, box, type_idx, batch_size=1, precision='fp32'):
+    """Evaluate a trained model on frames (no ground truth needed).
+
+    coord (F, N, 3); box (F,), (F,1), (F,3), (F,9) or (F,3,3); type_idx (N,).
+    Returns {'energy': (F,), 'force': (F, N, 3)} in the caller's atom order.
+    precision: 'fp32' (default), 'bf16' or 'fp16' embedding/fitting GEMMs.
+    """
+    try:
+        coord = np.asarray(coord)
+        type_idx = np.asarray(type_idx)
+        assert coord.ndim == 3 and coord.shape[2] == 3
+        assert type_idx.ndim == 1 and np.asarray(box).ndim in [1, 2, 3]
+        assert coord.shape[1] == type_idx.shape[0]
+        assert coord.shape[0] == np.asarray(box).shape[0]
+        boxes = _boxes(box, coord.shape[0])
+    except (AssertionError, ValueError):
+        raise ValueError('Input shapes are incorrect: \n' +
+                         'coord: (n_frames, n_atoms, 3) \n' +
+                         'box: (n_frames) + (,) or (1,) or (3,) or (9), or (3,3) \n' +
+                         'type_idx (n_atoms).')
+    model, variables = load_model(model_path, replicate=False)
+    params = model.params
+    if params.get('type', 'energy') not in ('energy',):
+        raise NotImplementedError('evaluate() supports energy models')
+    ct = params.get('chemical_types')
+    if ct is not None:
+        unknown = set(type_idx.tolist()) - set(ct)
+        if unknown:
+            raise ValueError('Atomic numbers %s in type_idx are not in model.params["chemical_types"]=%s'
+                             % (sorted(unknown), ct))
+        z2i = {z: i for i, z in enumerate(ct)}
+        type_idx = np.array([z2i[z] for z in type_idx.tolist()])
+    if type_idx.min() < 0 or type_idx.max() >= params['ntypes']:
+        raise ValueError('type index out of trained scope of model (ntypes=%d)' % params['ntypes'])
+    ev = Evaluator(params, variables)
+    F = coord.shape[0]
+    energy = np.zeros(F)
+    force = np.zeros(coord.shape)
+    i = 0
+    while i < F:
+        j = i + 1
+        while j < F and j - i < batch_size and np.array_equal(boxes[j], boxes[i]):
+            j += 1
+        e, f, _ = ev.evaluate_batch(coord[i:j], boxes[i], type_idx, precision)
+        energy[i:j] = e
+        force[i:j] = f
+        i = j
+    return {'energy': energy, 'force': force}

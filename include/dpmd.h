@@ -1,0 +1,168 @@
+/*
+ * dpmd.h -- C ABI of libdpmd_hip.so, the MI355X (gfx950) engine for deepmd-jax's
+ * Deep Potential se_a energy/force/virial evaluation and the MD step around it.
+ *
+ * Plain C types only: host pointers, sizes, int32 status codes.  Every entry point
+ * returns 0 on success and a negative DPMD_E* code on failure; dpmd_last_error()
+ * then holds a thread-local message.  Device-side conditions (neighbour-list
+ * overflow, NaN/Inf) are reported as the reference's bit-coded error word
+ * (deepmd_jax/md.py:616-642): 2 = neighbour buffer overflow, 16 = NaN or Inf.
+ *
+ * Arrays given in "caller order" follow the caller's atom order; the library keeps
+ * its device state in the reference's stable type-sorted order
+ * (deepmd_jax/utils.py:329-342) and permutes at the boundary.
+ *
+ * The reference is pure Python on JAX; it has no FFI.  Each entry point below names
+ * the reference function it replaces.  The Python host package
+ * (deepmd-jax_amd/deepmd_jax) binds them with ctypes.
+ */
+#ifndef DPMD_H
+#define DPMD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DPMD_MAX_TYPES 4
+
+#define DPMD_OK 0
+#define DPMD_E_ARG -1      /* invalid argument (ValueError in the reference) */
+#define DPMD_E_HIP -2      /* HIP runtime error */
+#define DPMD_E_STATE -3    /* call out of order (e.g. no neighbour list yet) */
+#define DPMD_E_UNSUPPORTED -4 /* model topology the kernels do not implement */
+
+/* Arithmetic of the embedding / fitting GEMMs.  Switch, environment matrix and
+ * descriptor contraction are always FP32; energy / virial reductions FP64. */
+#define DPMD_PREC_FP32 0
+#define DPMD_PREC_BF16 1
+#define DPMD_PREC_FP16 2
+
+/* Integrators for dpmd_md_run (deepmd_jax/md.py:313-383). */
+#define DPMD_NVE 0          /* velocity Verlet, deepmd_jax/routine.py:139-144 */
+#define DPMD_NVT_NH 1       /* Nose-Hoover chain, deepmd_jax/routine.py:147-158 */
+#define DPMD_NVT_LANGEVIN 2 /* BAOAB, deepmd_jax/routine.py:46-107 */
+#define DPMD_PIMD_PILE_L 3  /* ring-polymer BAOAB + PILE-L, deepmd_jax/pimd.py:114-207 */
+
+typedef struct dpmd_model* dpmd_model_t;
+typedef struct dpmd_system* dpmd_system_t;
+
+/* Host description of an se_a energy model, i.e. model.params + variables['params']
+ * of a deepmd-jax model.pkl (deepmd_jax/utils.py:225-237, deepmd_jax/train.py:288-304).
+ * Dense kernels are flax layout (in, out), row-major fp32.  Embedding net k = i*ntypes + j
+ * maps centre type i and neighbour type j (deepmd_jax/dpmodel.py:50-51). */
+typedef struct {
+  int32_t ntypes;
+  int32_t embed_widths[3];
+  int32_t fit_widths[3];
+  int32_t axis;
+  double rcut;
+  double sr_mean[DPMD_MAX_TYPES];
+  double sr_std[DPMD_MAX_TYPES];
+  double nnbrs;
+  double ebias[DPMD_MAX_TYPES];
+  double out_norm;
+  const float* tbias;                                              /* (W) */
+  const float* emb_kernel[DPMD_MAX_TYPES * DPMD_MAX_TYPES][3];
+  const float* emb_bias[DPMD_MAX_TYPES * DPMD_MAX_TYPES][3];
+  const float* fit_kernel[DPMD_MAX_TYPES][4];
+  const float* fit_bias[DPMD_MAX_TYPES][4];
+  const float* fit_dt[DPMD_MAX_TYPES][3];                          /* NULL where no residual */
+} dpmd_model_desc;
+
+/* Integrator parameters for dpmd_md_run. Units: fs, eV, Angstrom. */
+typedef struct {
+  int32_t routine;      /* DPMD_NVE, ... */
+  int32_t precision;    /* DPMD_PREC_* for the force evaluations */
+  double dt;
+  double kT;            /* thermostat temperature (eV) */
+  double gamma;         /* Langevin friction (1/fs); PIMD: centroid friction */
+  double tau;           /* Nose-Hoover relaxation time (fs) */
+  int32_t chain_length; /* Nose-Hoover chain length */
+  int32_t record_every; /* >0: store positions/velocities every step into traj buffers */
+  uint64_t seed;        /* stochastic integrators: counter-based normal stream */
+  double pimd_lambda;   /* PILE-L internal-mode damping scale */
+} dpmd_md_params;
+
+const char* dpmd_last_error(void);
+int32_t dpmd_version(void);
+
+/* Replaces load_model's device_put of variables (deepmd_jax/utils.py:230-237). */
+int32_t dpmd_model_create(const dpmd_model_desc* desc, int32_t device, dpmd_model_t* out);
+int32_t dpmd_model_destroy(dpmd_model_t model);
+
+/* One simulated system of natoms atoms x nreplica replicas (ring-polymer beads share
+ * the composition).  Replaces the state set-up of Simulation.__init__
+ * (deepmd_jax/md.py:61-283).  type_idx: (natoms) in [0, ntypes). */
+int32_t dpmd_system_create(dpmd_model_t model, int32_t natoms, int32_t nreplica,
+                           const int32_t* type_idx, dpmd_system_t* out);
+int32_t dpmd_system_destroy(dpmd_system_t sys);
+
+/* Orthorhombic box lengths (3). */
+int32_t dpmd_system_set_box(dpmd_system_t sys, const double* box3);
+/* (nreplica, natoms, 3) caller order.  setPosition / getPosition (deepmd_jax/md.py:969-1014). */
+int32_t dpmd_system_set_positions(dpmd_system_t sys, const double* pos);
+int32_t dpmd_system_get_positions(dpmd_system_t sys, double* pos);
+/* Velocities (Angstrom/fs); setVelocity / getVelocity (deepmd_jax/md.py:1016-1031). */
+int32_t dpmd_system_set_velocities(dpmd_system_t sys, const double* vel);
+int32_t dpmd_system_get_velocities(dpmd_system_t sys, double* vel);
+/* Per-atom masses (natoms) in eV fs^2 / Angstrom^2. */
+int32_t dpmd_system_set_masses(dpmd_system_t sys, const double* mass);
+
+/* Typed neighbour list at cutoff rc_skin.  Replaces typed_neighbor_list's allocate_fn
+ * (deepmd_jax/md_utils.py:376-421): measures the per-type maximum neighbour count,
+ * sizes each row to knbr_j = 1 + floor(k_j*ratio + max(20-k_j,0)*max(ratio-1.2,0)),
+ * builds.  knbr_override (ntypes) skips the measurement when non-NULL.  knbr_out (ntypes). */
+int32_t dpmd_nlist_allocate(dpmd_system_t sys, double rc_skin, double buffer_ratio,
+                            const int32_t* knbr_override, int32_t* knbr_out);
+/* Rebuild at the allocated capacity; update_fn (deepmd_jax/md_utils.py:424-429).
+ * *overflow = 1 when some row needed more than knbr_j - 1 entries. */
+int32_t dpmd_nlist_update(dpmd_system_t sys, int32_t* overflow);
+/* Export rows for neighbour type j as nbrs_nm (deepmd_jax/md_utils.py:431-459):
+ * out (nreplica, natoms, knbr_j - 1) int32, centres in type-sorted order, entries are
+ * ascending type-local indices padded with natoms. */
+int32_t dpmd_nlist_export(dpmd_system_t sys, int32_t j, int32_t* out);
+/* Current capacities knbr (ntypes). */
+int32_t dpmd_nlist_capacity(dpmd_system_t sys, int32_t* knbr);
+
+/* Energy, forces and virial of the current positions.  Replaces
+ * DPModel.energy_and_force (deepmd_jax/dpmodel.py:106-108) and the strain derivative
+ * of iso_pressure (deepmd_jax/md_utils.py:139-159).  Needs an up-to-date neighbour
+ * list (dpmd_nlist_allocate / _update).
+ * energy (nreplica); force (nreplica, natoms, 3) caller order; virial (nreplica, 9)
+ * row-major, virial = -sum_pairs x_ij (outer) dE/dx_ij; atom_energy (nreplica, natoms).
+ * Any output pointer may be NULL. */
+int32_t dpmd_compute(dpmd_system_t sys, int32_t precision, double* energy, double* force,
+                     double* virial, double* atom_energy);
+
+/* nsteps MD steps fully on the device: per step the drift test and conditional
+ * neighbour rebuild (deepmd_jax/md.py:693-741), the integrator step with one force
+ * evaluation, minimum-image unwrap (deepmd_jax/md_utils.py:162-169).  Replaces the
+ * jit(lax.scan(inner_step)) chunk of Simulation.run (deepmd_jax/md.py:759-804).
+ * traj_pos / traj_vel: (nsteps, nreplica, natoms, 3) caller order or NULL.
+ * *error_code accumulates the reference's bit codes. */
+int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps,
+                    double* traj_pos, double* traj_vel, uint32_t* error_code);
+/* Ring-polymer constants for DPMD_PIMD_PILE_L (deepmd_jax/pimd.py:22-44,92-111,160-179):
+ * C (P*P, row-major, primitive <- normal mode), freqs (P), and the per-mode O-step
+ * factors c1 = exp(-gamma_k dt), c2 = sqrt(P kT (1 - c1^2)) (P).  P = nreplica. */
+int32_t dpmd_pimd_setup(dpmd_system_t sys, const double* C, const double* freqs, const double* c1,
+                        const double* c2);
+/* Kinetic energy (nreplica) in FP64 from the device momenta. */
+int32_t dpmd_kinetic_energy(dpmd_system_t sys, double* ke);
+/* Thermostat work / chain state scalar used by the reference invariants (nreplica). */
+int32_t dpmd_thermostat_energy(dpmd_system_t sys, double* e);
+
+/* Profiling: enable HIP-event timing of each kernel class on the system's stream;
+ * which = 0 nlist, 1 embed_fwd, 2 fit, 3 embed_bwd, 4 integrator.  ms = summed event
+ * time, count = launches timed. */
+int32_t dpmd_timing_enable(dpmd_system_t sys, int32_t on);
+int32_t dpmd_timing_get(dpmd_system_t sys, int32_t which, double* ms, int64_t* count);
+/* Mean neighbours within rcut per centre of the last evaluation (algorithmic FLOP count). */
+int32_t dpmd_pair_count(dpmd_system_t sys, double* pairs_within_rcut);
+int32_t dpmd_synchronize(dpmd_system_t sys);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

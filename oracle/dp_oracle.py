@@ -1,0 +1,285 @@
+"""CPU oracle for the DP se_a energy/force/virial path -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import
+this module; the product path (deepmd-jax_amd/) never does.
+
+A restatement, in torch on the CPU (float64 or float32), of deepmd-jax's
+energy model and of the neighbour-list protocol it consumes.  Forces and the
+virial come from torch autograd, the analogue of the reference's `jax.grad`
+(deepmd_jax/dpmodel.py:106-108, deepmd_jax/md.py:292).
+
+Function -> reference file:line it follows:
+  shift_ortho           deepmd_jax/utils.py:20-23
+  sr                    deepmd_jax/utils.py:29-32
+  tensor_3to6           deepmd_jax/utils.py:51-52
+  relative_coord_nlist  deepmd_jax/utils.py:79-82
+  relative_coord_lattice deepmd_jax/utils.py:66-78
+  lattice_candidate     deepmd_jax/data.py:347-364
+  embedding_net         deepmd_jax/utils.py:100-125 (uncompressed branch)
+  fitting_net           deepmd_jax/utils.py:153-166
+  dp_energy             deepmd_jax/dpmodel.py:30-81,100
+  energy_force_virial   deepmd_jax/dpmodel.py:106-108; virial as the strain
+                        derivative of deepmd_jax/md_utils.py:139-159
+Pinning (tests/test_oracle.py): closed forms of the switch and of tensor_3to6,
+rotation / translation / permutation invariance, the T3=T6=0 outer-product
+reduction of G, all-zero-weight nets, autograd against central finite
+differences, tr(virial) against the finite-difference strain derivative, the
+lattice path against the neighbour-list path.  Every function here is pinned
+by at least one of those; the exact FP32 rounding order of XLA is not
+(tolerance-level parity only).
+"""
+import numpy as np
+import torch
+
+
+def shift_ortho(d, box3):
+    return torch.remainder(d - box3 / 2, box3) - box3 / 2
+
+
+def sr(r, rcut):
+    t = r / rcut
+    return (r > 1e-14) * (t < 1) / (r + 1e-15) * (1 - 3 * t ** 2 + 2 * t ** 3)
+
+
+def tensor_3to6(x, axis, bias=0.0):
+    return torch.cat([x ** 2 - bias, 2 ** 0.5 * x * torch.roll(x, shifts=1, dims=axis)], dim=axis)
+
+
+def _safe_norm(x, dim):
+    return torch.linalg.norm(torch.where(torch.abs(x) > 1e-15, x, torch.full_like(x, 1e-15)), dim=dim)
+
+
+def relative_coord_nlist(coord_i, coord_j, box3, nbrs):
+    """x (N_i,3,M), r (N_i,M) for one (centre type, neighbour type) block.
+    nbrs: (N_i, M) int64 type-local indices, padding >= N_j."""
+    Nj = coord_j.shape[0]
+    valid = nbrs < Nj
+    idx = torch.clamp(nbrs, max=max(Nj - 1, 0))
+    if Nj == 0:
+        x = torch.zeros(coord_i.shape[0], 3, nbrs.shape[1], dtype=coord_i.dtype)
+        return x, torch.zeros(coord_i.shape[0], nbrs.shape[1], dtype=coord_i.dtype)
+    x = shift_ortho(coord_j[idx] - coord_i[:, None], box3).permute(0, 2, 1)
+    r = _safe_norm(x, 1) * valid
+    x = x * valid[:, None]
+    return x, r
+
+
+def lattice_candidate(box33, rcut):
+    """deepmd_jax/data.py:347-364 for one box: (lattice_cand (X,3) int, lattice_max)."""
+    N = 2
+    box33 = np.asarray(box33, np.float64)
+    recp_norm = np.linalg.norm(np.linalg.inv(box33), axis=-1)
+    n = np.ceil(rcut * recp_norm - 0.5).astype(int)
+    cand = np.stack(np.meshgrid(range(-n[0], n[0] + 1), range(-n[1], n[1] + 1), range(-n[2], n[2] + 1),
+                                indexing='ij'), -1).reshape(-1, 3)
+    trial = np.stack(np.meshgrid(np.arange(-N, N + 1), np.arange(-N, N + 1), np.arange(-N, N + 1)),
+                     -1).reshape(-1, 3) / (2 * N)
+    is_nb = np.linalg.norm((cand[:, None] - trial) @ box33, axis=-1) < rcut
+    cand = cand[is_nb.any(1)]
+    lattice_max = int(is_nb.sum(0).max())
+    return cand, lattice_max
+
+
+def relative_coord_lattice(coord_i, coord_j, box33, cand, lattice_max):
+    """Dense all-pairs block with the lattice_max nearest images (orthorhombic box)."""
+    Ni, Nj = coord_i.shape[0], coord_j.shape[0]
+    box3 = torch.diagonal(box33)
+    x = shift_ortho(coord_j[None] - coord_i[:, None], box3).permute(0, 2, 1)  # (Ni,3,Nj)
+    X = len(cand)
+    if X > 1:
+        shifts = (torch.as_tensor(cand, dtype=box33.dtype) @ box33).T  # (3,X)
+        xX = x[..., None] - shifts[:, None]  # (Ni,3,Nj,X)
+        if X == lattice_max:
+            x = xX.reshape(Ni, 3, -1)
+        else:
+            rX = _safe_norm(xX, 1)  # (Ni,Nj,X)
+            idx = torch.argsort(rX, dim=-1)[..., :lattice_max]
+            x = torch.take_along_dim(xX, idx[:, None], dim=-1).reshape(Ni, 3, -1)
+    r = _safe_norm(x, 1)
+    return x, r
+
+
+def _dense(p, x):
+    return x @ torch.as_tensor(p['kernel'], dtype=x.dtype) + torch.as_tensor(p['bias'], dtype=x.dtype)
+
+
+def embedding_net(net, s, reducer):
+    """s (N,M,1) -> reducer (N,X,M) @ h (N,M,W); repeat-residual between layers."""
+    x = torch.tanh(_dense(net['Dense_0'], s))
+    li = 1
+    while 'Dense_%d' % li in net:
+        w_prev = x.shape[-1]
+        y = torch.tanh(_dense(net['Dense_%d' % li], x))
+        Z = y.shape[-1] // w_prev
+        x = y + torch.repeat_interleave(x, Z, dim=-1)
+        li += 1
+    return reducer @ x
+
+
+def fitting_net(net, x):
+    li = 0
+    while 'Dense_%d' % (li + 1) in net:
+        x_prev = x
+        x = torch.tanh(_dense(net['Dense_%d' % li], x))
+        if li > 0 and x.shape[-1] == x_prev.shape[-1]:
+            x = x * torch.as_tensor(net['dt%d' % li], dtype=x.dtype) + x_prev
+        li += 1
+    return _dense(net['Dense_%d' % li], x)
+
+
+def _type_sort(coord, type_idx):
+    perm = np.argsort(np.asarray(type_idx), kind='stable')
+    return coord[torch.as_tensor(perm)], perm
+
+
+def dp_energy(params, variables, coord, box3, type_idx, nbrs_nm=None, lattice=None, return_T=False,
+              strain=None):
+    """Energy of one frame.  coord: (N,3) torch tensor in caller order.
+    Either nbrs_nm[i][j] (type-local, padded) or lattice=(cand, lattice_max).
+    strain (3,3), if given, multiplies every minimum-image pair vector (the
+    perturbation the virial differentiates)."""
+    P = variables['params']
+    dtype = coord.dtype
+    ntypes = params['ntypes']
+    type_idx = np.asarray(type_idx)
+    type_count = np.bincount(type_idx, minlength=ntypes)
+    c, _ = _type_sort(coord, type_idx)
+    off = np.concatenate([[0], np.cumsum(type_count)])
+    cs = [c[off[t]:off[t + 1]] for t in range(ntypes)]
+    box3 = torch.as_tensor(np.asarray(box3, np.float64), dtype=dtype)
+    if box3.ndim == 2:
+        box3 = torch.diagonal(box3)
+    box33 = torch.diag(box3)
+    rcut, A = params['rcut'], params['axis']
+    mean = np.asarray(params['sr_mean'], np.float64)
+    std = np.asarray(params['sr_std'], np.float64)
+    Y = ntypes
+    T_list = []
+    for i in range(Y):
+        Ti = 0
+        for j in range(Y):
+            if nbrs_nm is not None:
+                x, r = relative_coord_nlist(cs[i], cs[j], box3, torch.as_tensor(np.asarray(nbrs_nm[i][j]), dtype=torch.long))
+            else:
+                x, r = relative_coord_lattice(cs[i], cs[j], box33, lattice[0], lattice[1])
+            if strain is not None:
+                x = torch.einsum('ab,nbm->nam', strain, x)
+                r = _safe_norm(x, 1) * (r > 0)
+            s = sr(r, rcut)
+            s_norm = s / std[i]
+            s_cn = (s - mean[i]) / std[i]
+            xn = x / (r + 1e-16)[:, None]
+            R3 = 3 ** 0.5 * s_norm[:, None] * xn
+            R6 = 3 * s_norm[:, None] * tensor_3to6(xn, axis=1, bias=1.0 / 3)
+            RX = torch.cat([s_norm[:, None], R3, R6], dim=1)
+            Ti = Ti + embedding_net(P['embedding_net_%d' % (i * Y + j)], s_cn[:, :, None], RX)
+        T_list.append(Ti)
+    T = torch.cat(T_list, 0) / params['Nnbrs']
+    TW = T[:, 0] + torch.as_tensor(P['Tbias'], dtype=dtype)
+    T3, T6 = T[:, 1:4], T[:, 4:]
+    G = TW[:, None] * TW[:, :A, None] + (T3[:, :, None] * T3[:, :, :A, None]).sum(1)
+    G2 = tensor_3to6(T3[:, :, A:2 * A], axis=1) + T6[:, :, A:2 * A]
+    G = G + (G2[..., None] * T6[:, :, None]).sum(1)
+    G = G.reshape(G.shape[0], -1)
+    e = []
+    for t in range(Y):
+        f = fitting_net(P['fitting_net_%d' % t], G[off[t]:off[t + 1]])
+        e.append(f[:, 0] + float(np.asarray(params['Ebias'])[t]))
+    e = torch.cat(e) * params['out_norm']
+    if return_T:
+        return e.sum(), e, T, G
+    return e.sum()
+
+
+def energy_force_virial(params, variables, coord, box3, type_idx, nbrs_nm=None, lattice=None,
+                        dtype=torch.float64):
+    """(E, F (N,3) caller order, virial (3,3)) with forces from autograd.
+
+    virial = -dE/d(strain) under x -> (1+eps) x, box -> (1+eps) box, as the
+    full tensor (deepmd sign: virial = sum_pairs x_ij (x) f_ij)."""
+    coord = torch.as_tensor(np.asarray(coord, np.float64), dtype=dtype).clone().requires_grad_(True)
+    box3 = np.asarray(box3, np.float64).reshape(-1)[:3]
+    eps = torch.zeros(3, 3, dtype=dtype, requires_grad=True)
+    strain = torch.eye(3, dtype=dtype) + eps
+    if isinstance(lattice, str) and lattice == 'auto':
+        lattice = lattice_candidate(np.diag(box3), params['rcut'])
+    E = dp_energy(params, variables, coord, box3, type_idx, nbrs_nm=nbrs_nm, lattice=lattice, strain=strain)
+    gx, ge = torch.autograd.grad(E, [coord, eps])
+    return E.detach(), -gx.detach(), -ge.detach().T
+
+
+# ---------------------------------------------------------------------------------
+# Neighbour list: deepmd_jax/md_utils.py:364-373 (canonicalize), jax_md periodic
+# displacement + d^2 < (rc+skin)^2 + self exclusion, :431-459 (get_nm).
+# Brute force over all pairs in float32, op order (Ra - Rb), ((dx^2+dy^2)+dz^2).
+
+def canonicalize_f32(coord, box, reference_box=None):
+    box = np.asarray(box, np.float32).reshape(-1)[:3]
+    ref = box if reference_box is None else np.asarray(reference_box, np.float32)
+    scale = (ref / box).astype(np.float32)
+    c = np.remainder(np.asarray(coord, np.float32), box).astype(np.float32) * scale
+    return (c * np.float32(1 - 5e-7)).astype(np.float32)
+
+
+def pair_d2_f32(ca, cb, side):
+    """jax_md space.periodic displacement then square_distance, float32."""
+    side = np.asarray(side, np.float32)
+    half = np.float32(0.5) * side
+    d = (ca - cb).astype(np.float32)
+    d = (np.remainder((d + half).astype(np.float32), side).astype(np.float32) - half).astype(np.float32)
+    sq = (d * d).astype(np.float32)
+    return ((sq[..., 0] + sq[..., 1]).astype(np.float32) + sq[..., 2]).astype(np.float32)
+
+
+def neighbor_sets(coord, box, type_idx, rc_skin, reference_box=None):
+    """Per centre (type-sorted order): sorted arrays of type-sorted neighbour indices.
+    Returns (sorted_coord_f32, list of index arrays, d2 matrix rows' borderline count)."""
+    type_idx = np.asarray(type_idx)
+    perm = np.argsort(type_idx, kind='stable')
+    c = canonicalize_f32(np.asarray(coord)[perm], box, reference_box)
+    side = np.asarray(box if reference_box is None else reference_box, np.float32).reshape(-1)[:3]
+    cut = np.float32(rc_skin)
+    cut_sq = np.float32(cut * cut)
+    N = len(c)
+    rows = []
+    border = 0
+    for i in range(N):
+        d2 = pair_d2_f32(c[i][None], c, side)
+        m = d2 < cut_sq
+        m[i] = False
+        rows.append(np.nonzero(m)[0])
+        border += int((np.abs(d2.astype(np.float64) - float(cut_sq)) < 4e-6 * float(cut_sq)).sum())
+    return c, rows, border
+
+
+def knbr_from_counts(kmax, buffer_ratio=1.2):
+    """deepmd_jax/md_utils.py:399-403."""
+    k = np.asarray(kmax)
+    kb = np.where(k == 0, 1, 1 + (k * buffer_ratio + np.maximum(20 - k, 0) * max(buffer_ratio - 1.2, 0)))
+    return [int(v) for v in kb.astype(int)]
+
+
+def typed_nlist(coord, box, type_idx, rc_skin, ntypes=2, knbr=None, buffer_ratio=1.2):
+    """nbrs_nm[i][j] (N_i, knbr_j-1) int32, padding N; overflow flag; knbr."""
+    type_idx = np.asarray(type_idx)
+    N = len(type_idx)
+    tc = np.bincount(type_idx, minlength=ntypes)
+    off = np.concatenate([[0], np.cumsum(tc)])
+    _, rows, _ = neighbor_sets(coord, box, type_idx, rc_skin)
+    counts = np.zeros((N, ntypes), int)
+    for a, r in enumerate(rows):
+        for j in range(ntypes):
+            counts[a, j] = ((r >= off[j]) & (r < off[j + 1])).sum()
+    if knbr is None:
+        knbr = knbr_from_counts(counts.max(0), buffer_ratio)
+    overflow = bool((counts >= np.asarray(knbr)[None]).any())
+    full = []
+    for j in range(ntypes):
+        arr = np.full((N, knbr[j] - 1), N, np.int32)
+        for a, r in enumerate(rows):
+            rj = r[(r >= off[j]) & (r < off[j + 1])] - off[j]
+            rj = rj[:knbr[j] - 1]
+            arr[a, :len(rj)] = rj
+        full.append(arr)
+    nbrs_nm = [[full[j][off[i]:off[i + 1]] for j in range(ntypes)] for i in range(ntypes)]
+    return nbrs_nm, overflow, knbr

@@ -1,0 +1,130 @@
+"""GPU parity of the HIP path against the CPU oracle, through the C ABI.
+
+Tolerances (FP32 device arithmetic against the FP64 oracle):
+  energy: |dE| / N <= 1e-5 eV/atom and |dE| / |E| <= 1e-5
+  force:  max |dF| <= 1e-4 * max(1, max |F|) eV/A  (FP32 path); RMS relative <= 1e-5
+  virial: max |dW| <= 1e-4 * max |W|
+Neighbour lists: bit-exact index sets (type-local, ascending, padded with N).
+"""
+import numpy as np
+import pytest
+
+from oracle import dp_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _system(device_model, coord, box, types, rc_skin, nrep=1):
+    from deepmd_jax import _lib
+    s = _lib.DeviceSystem(device_model, types, nrep)
+    s.set_box(box)
+    s.set_masses(np.ones(len(types)))
+    s.set_positions(np.broadcast_to(coord, (nrep,) + coord.shape))
+    s.nlist_allocate(rc_skin, 1.2)
+    return s
+
+
+def _f32(x):
+    return np.asarray(x, np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize('nmol,rc_skin', [(64, 6.2), (512, 7.0), (512, 6.3)])
+def test_nlist_bit_exact(device_model, nmol, rc_skin):
+    from deepmd_jax import synthetic as syn
+    print('case nlist', nmol, rc_skin, flush=True)
+    coord, box, types = syn.water_box(nmol, seed=5)
+    coord = _f32(coord)
+    ref, overflow, knbr = O.typed_nlist(coord, box, types, rc_skin)
+    s = _system(device_model, coord, box, types, rc_skin)
+    assert s.knbr == knbr
+    N = len(types)
+    off = np.concatenate([[0], np.cumsum(np.bincount(types))])
+    for j in range(2):
+        got = s.nlist_export(j)[0]
+        for i in range(2):
+            np.testing.assert_array_equal(got[off[i]:off[i + 1]], ref[i][j])
+    assert not s.nlist_update()
+
+
+@pytest.mark.parametrize('nmol', [64, 216])
+def test_energy_force_virial(device_model, model0, nmol):
+    from deepmd_jax import synthetic as syn
+    print('case efv', nmol, flush=True)
+    params, var = model0
+    coord, box, types = syn.water_box(nmol, seed=1)
+    coord = _f32(coord)
+    E, F, W = O.energy_force_virial(params, var, coord, box, types, lattice='auto')
+    E, F, W = E.item(), F.numpy(), W.numpy()
+    s = _system(device_model, coord, box, types, params['rcut'])
+    e, f, v, ea = s.compute('fp32', want_atom=True)
+    N = len(types)
+    assert abs(e[0] - E) / N <= 1e-5 and abs(e[0] - E) / abs(E) <= 1e-5, (e[0], E)
+    assert np.abs(f[0] - F).max() <= 1e-4 * max(1.0, np.abs(F).max()), np.abs(f[0] - F).max()
+    assert np.sqrt(((f[0] - F) ** 2).mean() / (F ** 2).mean()) <= 1e-5
+    assert np.abs(v[0] - W).max() <= 1e-4 * np.abs(W).max(), (v[0], W)
+    assert abs(ea[0].sum() - e[0]) <= 1e-6 * abs(E)
+
+
+def test_replicas_and_skin_independent(device_model, model0):
+    """Replicas of one system give identical results; the skin does not change E/F."""
+    from deepmd_jax import synthetic as syn
+    print('case replicas', flush=True)
+    coord, box, types = syn.water_box(216, seed=2)
+    s1 = _system(device_model, coord, box, types, 6.0)
+    e1, f1, v1, _ = s1.compute()
+    s3 = _system(device_model, coord, box, types, 7.0, nrep=3)
+    e3, f3, v3, _ = s3.compute()
+    for r in range(3):
+        assert abs(e3[r] - e1[0]) <= 1e-6 * abs(e1[0])
+        assert np.abs(f3[r] - f1[0]).max() <= 1e-5 * np.abs(f1).max()
+
+
+def test_nve_against_oracle_trajectory(device_model, model0):
+    """20 velocity-Verlet steps: device FP64 integrator + FP32 forces vs the oracle in FP64."""
+    from deepmd_jax import _lib, synthetic as syn
+    from deepmd_jax.constants import MASS_UNIT_CONVERSION
+    print('case nve', flush=True)
+    params, var = model0
+    coord, box, types = syn.water_box(64, seed=0)
+    vel = syn.maxwell_boltzmann(types, 300.0, seed=0)
+    m = np.array(syn.MASSES)[types] * MASS_UNIT_CONVERSION
+    s = _lib.DeviceSystem(device_model, types, 1)
+    s.set_box(box)
+    s.set_masses(m)
+    s.set_positions(coord)
+    s.set_velocities(vel)
+    s.nlist_allocate(6.2, 1.2)
+    p = _lib.MDParams()
+    p.routine, p.precision, p.dt = 0, 0, 0.5
+    nsteps = 20
+    tp, tv, err = s.md_run(p, nsteps)
+    assert err == 0
+    x, pm = coord.copy(), vel * m[:, None]
+    lat = O.lattice_candidate(np.diag(box), params['rcut'])
+    _, F, _ = O.energy_force_virial(params, var, x, box, types, lattice=lat)
+    F = F.numpy()
+    for k in range(nsteps):
+        pm += 0.25 * F
+        x += 0.5 * pm / m[:, None]
+        _, F, _ = O.energy_force_virial(params, var, x, box, types, lattice=lat)
+        F = F.numpy()
+        pm += 0.25 * F
+    assert np.abs(tp[-1, 0] - x).max() < 1e-5
+    assert np.abs(tv[-1, 0] - pm / m[:, None]).max() < 1e-5
+
+
+def test_overflow_is_reported_and_state_restored(device_model, model0):
+    from deepmd_jax import _lib, synthetic as syn
+    print('case overflow', flush=True)
+    coord, box, types = syn.water_box(216, seed=3)
+    s = _lib.DeviceSystem(device_model, types, 1)
+    s.set_box(box)
+    s.set_masses(np.ones(len(types)) * 100.0)
+    s.set_positions(coord)
+    s.nlist_allocate(7.0, 1.2, knbr_override=[10, 10])
+    x0 = s.get_positions()
+    p = _lib.MDParams()
+    p.routine, p.dt = 0, 0.5
+    _, _, err = s.md_run(p, 3, record=False)
+    assert err & 2
+    np.testing.assert_array_equal(s.get_positions(), x0)
