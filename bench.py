@@ -76,7 +76,19 @@ def main():
     ap.add_argument('--precision', default='fp32')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-sim', action='store_true')
+    ap.add_argument('--calibrate-copy', action='store_true',
+                    help='run a plain 256 MiB elementwise copy (read 256 MiB, write 256 MiB) five times, for '
+                         'scaling FETCH_SIZE / WRITE_SIZE counters')
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        x = torch.ones(64 * 1024 * 1024, device='cuda')
+        y = torch.empty_like(x)
+        for _ in range(5):
+            torch.mul(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        print(json.dumps({'copy_bytes_read': x.numel() * 4, 'copy_bytes_written': y.numel() * 4, 'launches': 5}))
+        return
     if args.gpus != 1:
         print('bench.py runs on one GPU; multi-GPU runs are not part of this build (--gpus %d requested)' % args.gpus,
               file=sys.stderr)
@@ -109,22 +121,30 @@ def main():
             continue
         done += k
     sys_.synchronize()
-    sys_.timing(on=True)
+
+    def run_steps(n):
+        done, realloc = 0, 0
+        while done < n:
+            k = min(100, n - done)
+            _, _, err = sys_.md_run(p, k, record=False)
+            if err & 16:
+                raise RuntimeError('NaN/Inf during the benchmark')
+            if err & 2:
+                realloc += 1
+                sys_.nlist_allocate(params['rcut'] + 1.0, 1.2 + 0.05 * realloc)
+                continue
+            done += k
+        sys_.synchronize()
+        return realloc
+
     t0 = time.perf_counter()
-    done, realloc = 0, 0
-    while done < args.steps:
-        k = min(100, args.steps - done)
-        _, _, err = sys_.md_run(p, k, record=False)
-        if err & 16:
-            raise RuntimeError('NaN/Inf during the benchmark')
-        if err & 2:
-            realloc += 1
-            sys_.nlist_allocate(params['rcut'] + 1.0, 1.2 + 0.05 * realloc)
-            continue
-        done += k
-    sys_.synchronize()
+    realloc = run_steps(args.steps)
     t1 = time.perf_counter()
     elapsed = t1 - t0
+    # per-kernel HIP-event timing on the library's stream, in a separate pass
+    ntimed = max(10, min(50, args.steps))
+    sys_.timing(on=True)
+    run_steps(ntimed)
     kt = {KERNELS[w]: sys_.timing(which=w) for w in KERNELS}
     sys_.timing(on=False)
     pairs = sys_.pair_count()
@@ -151,7 +171,7 @@ def main():
         'ns_per_day': steps_per_s * 0.5 * 86400 * 1e-6,
         'model_tflops': ftot * steps_per_s / 1e12,
         'pairs_within_rcut_per_atom': pairs,
-        'kernel_ms_per_step': {k: v[0] / args.steps for k, v in kt.items()},
+        'kernel_ms_per_step': {k: v[0] / ntimed for k, v in kt.items()},
         'nlist_reallocations': realloc,
         'roofline': roof,
     }
