@@ -67,6 +67,33 @@ def cpu_baseline(params, var, seconds_budget=30.0):
                       'median %.2f s, x8 atoms extrapolated to 98,304' % (len(times), t)}
 
 
+def accuracy(params, var, model):
+    """Energy/atom, force RMSE and virial error of fp32 / bf16 / fp16 against the FP64 CPU
+    oracle on one 12,288-atom water frame (4,096 H2O, seed 0)."""
+    import torch
+    sys.path.insert(0, ROOT)
+    from oracle import dp_oracle as O
+    from deepmd_jax import _lib, synthetic as syn
+    torch.set_num_threads(len(os.sched_getaffinity(0)))
+    coord, box, types = syn.water_box(4096, seed=0)
+    nb, _, _ = O.typed_nlist(coord, box, types, params['rcut'])
+    E, F, W = O.energy_force_virial(params, var, coord, box, types, nbrs_nm=nb, dtype=torch.float64)
+    E, F, W = E.item(), F.numpy(), W.numpy()
+    s = _lib.DeviceSystem(model, types, 1)
+    s.set_box(box)
+    s.set_positions(coord)
+    s.nlist_allocate(params['rcut'], 1.2)
+    res = {'frame': '4,096 H2O (12,288 atoms), seed 0'}
+    for prec in ('fp32', 'bf16', 'fp16'):
+        e, f, v, _ = s.compute(prec)
+        res[prec] = {'energy_per_atom_abs_err': abs(e[0] - E) / len(types),
+                     'force_rmse_eV_per_A': float(np.sqrt(((f[0] - F) ** 2).mean())),
+                     'force_max_abs_err': float(np.abs(f[0] - F).max()),
+                     'virial_max_rel_err': float(np.abs(v[0] - W).max() / np.abs(W).max())}
+    res['force_rms_eV_per_A'] = float(np.sqrt((F ** 2).mean()))
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -76,6 +103,8 @@ def main():
     ap.add_argument('--precision', default='fp32')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-sim', action='store_true')
+    ap.add_argument('--no-half', action='store_true')
+    ap.add_argument('--no-acc', action='store_true')
     ap.add_argument('--calibrate-copy', action='store_true',
                     help='run a plain 256 MiB elementwise copy (read 256 MiB, write 256 MiB) five times, for '
                          'scaling FETCH_SIZE / WRITE_SIZE counters')
@@ -175,6 +204,16 @@ def main():
         'nlist_reallocations': realloc,
         'roofline': roof,
     }
+    # precision forms: steps/s, same workload
+    if args.precision == 'fp32' and not args.no_half:
+        for prec in ('bf16', 'fp16'):
+            p.precision = _lib.PREC[prec]
+            run_steps(args.warmup)
+            t0 = time.perf_counter()
+            run_steps(args.steps)
+            out['%s_steps_per_s' % prec] = args.steps / (time.perf_counter() - t0)
+            out['%s_speedup_vs_fp32' % prec] = out['%s_steps_per_s' % prec] / steps_per_s
+        p.precision = _lib.PREC['fp32']
     if not args.no_sim:
         from deepmd_jax.md import Simulation
         from deepmd_jax.dpmodel import DPModel
@@ -194,6 +233,8 @@ def main():
         out['cpu_baseline'] = cpu_baseline(params, var)
     else:
         out['cpu_baseline'] = None
+    if not args.no_acc:
+        out['precision_errors_vs_fp64_oracle'] = accuracy(params, var, model)
     print(json.dumps(out))
 
 

@@ -20,7 +20,7 @@ void set_error(const char* fmt, ...) {
   va_end(ap);
 }
 
-int32_t embed_launch(System* s, bool backward);
+int32_t embed_launch(System* s, bool backward, int precision);
 int32_t fit_launch(System* s, int precision);
 int32_t md_zero_force(System* s);
 int32_t md_reduce_launch(System* s, int* d_rowbase, int* d_count);
@@ -77,6 +77,20 @@ System::~System() {
 
 enum { FK0T = 0, FK1T, FK2T, FK0, FK1, FK2, FB0, FB1, FB2, FDT1, FDT2, FK3, FB3, FNB };
 
+static uint16_t bf16_bits(float f) {  // round to nearest even
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+static uint16_t half_bits(float f) {
+  _Float16 h = (_Float16)f;
+  uint16_t u;
+  memcpy(&u, &h, 2);
+  return u;
+}
+
 static int32_t model_create(const dpmd_model_desc* d, int device, Model** out) {
   if (!d || !out) { set_error("null argument"); return DPMD_E_ARG; }
   if (d->ntypes < 1 || d->ntypes > kMaxT) { set_error("ntypes %d out of range [1,%d]", d->ntypes, kMaxT); return DPMD_E_ARG; }
@@ -120,15 +134,25 @@ static int32_t model_create(const dpmd_model_desc* d, int device, Model** out) {
   }
   DPMD_HIP(hipMalloc(&M->d_emb, sizeof(float) * emb.size()));
   DPMD_HIP(hipMemcpy(M->d_emb, emb.data(), sizeof(float) * emb.size(), hipMemcpyHostToDevice));
+  M->offh = embed_net_offsets_h(M->lay);
+  for (int h = 0; h < 2; ++h) {
+    std::vector<uint16_t> tab((size_t)nn * M->offh.total);
+    for (int k = 0; k < nn; ++k)
+      embed_pack_net_h(M->lay, emb.data() + (size_t)k * M->off.total, tab.data() + (size_t)k * M->offh.total, h == 1);
+    DPMD_HIP(hipMalloc(&M->d_emb_h[h], sizeof(uint16_t) * tab.size()));
+    DPMD_HIP(hipMemcpy(M->d_emb_h[h], tab.data(), sizeof(uint16_t) * tab.size(), hipMemcpyHostToDevice));
+  }
   DPMD_HIP(hipMalloc(&M->d_tbias, sizeof(float) * M->wout));
   if (d->tbias) DPMD_HIP(hipMemcpy(M->d_tbias, d->tbias, sizeof(float) * M->wout, hipMemcpyHostToDevice));
   else DPMD_HIP(hipMemset(M->d_tbias, 0, sizeof(float) * M->wout));
-  // fitting nets
+  // fitting nets: output width FN (multiple of 16), K / row stride FKS (multiple of 32)
   const int fw = d->fit_widths[0];
-  const int F = (fw + 15) / 16 * 16;
+  const int FN = (fw + 15) / 16 * 16;
+  const int FKS = (fw + 31) / 32 * 32;
   const int D = M->axis * M->wout;
-  const size_t sizes[FNB] = {(size_t)F * D, (size_t)F * F, (size_t)F * F, (size_t)D * F, (size_t)F * F, (size_t)F * F,
-                             (size_t)F, (size_t)F, (size_t)F, (size_t)F, (size_t)F, (size_t)F, 1};
+  if (D % 32) { set_error("axis*width %d must be a multiple of 32", D); delete M; return DPMD_E_UNSUPPORTED; }
+  const size_t sizes[FNB] = {(size_t)FN * D, (size_t)FN * FKS, (size_t)FN * FKS, (size_t)D * FKS, (size_t)FN * FKS,
+                             (size_t)FN * FKS, (size_t)FN, (size_t)FN, (size_t)FN, (size_t)FN, (size_t)FN, (size_t)FN, 1};
   size_t pos = 0;
   for (int t = 0; t < d->ntypes; ++t)
     for (int b = 0; b < FNB; ++b) {
@@ -148,15 +172,15 @@ static int32_t model_create(const dpmd_model_desc* d, int device, Model** out) {
     for (int b = 0; b < FNB; ++b) o[b] = fit.data() + M->fit_off[t][b];
     for (int k = 0; k < D; ++k)
       for (int n = 0; n < fw; ++n) {
-        o[FK0T][(size_t)n * D + k] = K0[(size_t)k * fw + n];
-        o[FK0][(size_t)k * F + n] = K0[(size_t)k * fw + n];
+        o[FK0T][(size_t)n * D + k] = K0[(size_t)k * fw + n];   // Bt[n][k] = K0[k][n]
+        o[FK0][(size_t)k * FKS + n] = K0[(size_t)k * fw + n];  // Bt[k_in][n_out] for dG = dz1 K0^T
       }
     for (int k = 0; k < fw; ++k)
       for (int n = 0; n < fw; ++n) {
-        o[FK1T][n * F + k] = K1[k * fw + n];
-        o[FK2T][n * F + k] = K2[k * fw + n];
-        o[FK1][k * F + n] = K1[k * fw + n];
-        o[FK2][k * F + n] = K2[k * fw + n];
+        o[FK1T][n * FKS + k] = K1[k * fw + n];
+        o[FK2T][n * FKS + k] = K2[k * fw + n];
+        o[FK1][k * FKS + n] = K1[k * fw + n];
+        o[FK2][k * FKS + n] = K2[k * fw + n];
       }
     for (int n = 0; n < fw; ++n) {
       o[FB0][n] = d->fit_bias[t][0] ? d->fit_bias[t][0][n] : 0.f;
@@ -170,8 +194,16 @@ static int32_t model_create(const dpmd_model_desc* d, int device, Model** out) {
     o[FB3][0] = (float)(b3 * d->out_norm);
     M->fit_b3[t] = (float)(b3 * d->out_norm);
   }
+  M->fit_n = FN;
+  M->fit_k = FKS;
   DPMD_HIP(hipMalloc(&M->d_fit, sizeof(float) * fit.size()));
   DPMD_HIP(hipMemcpy(M->d_fit, fit.data(), sizeof(float) * fit.size(), hipMemcpyHostToDevice));
+  for (int h = 0; h < 2; ++h) {
+    std::vector<uint16_t> fh(fit.size());
+    for (size_t i = 0; i < fit.size(); ++i) fh[i] = h ? half_bits(fit[i]) : bf16_bits(fit[i]);
+    DPMD_HIP(hipMalloc(&M->d_fit_h[h], sizeof(uint16_t) * fh.size()));
+    DPMD_HIP(hipMemcpy(M->d_fit_h[h], fh.data(), sizeof(uint16_t) * fh.size(), hipMemcpyHostToDevice));
+  }
   *out = M;
   return DPMD_OK;
 }
@@ -190,12 +222,14 @@ static int32_t sys_alloc_work(System* s) {
   }
   s->rowbase[s->ntypes] = rb;
   s->mrows = rb;
-  s->fpad = (M->fitw[0] + 15) / 16 * 16;
+  s->fpad = M->fit_n;
+  s->fstride = M->fit_k;
   DPMD_HIP(hipMalloc(&s->d_T, sizeof(float) * RN * 10 * W));
   DPMD_HIP(hipMalloc(&s->d_G, sizeof(float) * (size_t)s->mrows * D));
   DPMD_HIP(hipMemset(s->d_G, 0, sizeof(float) * (size_t)s->mrows * D));
   DPMD_HIP(hipMalloc(&s->d_dG, sizeof(float) * (size_t)s->mrows * D));
-  DPMD_HIP(hipMalloc(&s->d_fitbuf, sizeof(float) * 7 * (size_t)s->mrows * s->fpad));
+  DPMD_HIP(hipMalloc(&s->d_fitbuf, sizeof(float) * 7 * (size_t)s->mrows * s->fstride));
+  DPMD_HIP(hipMemset(s->d_fitbuf, 0, sizeof(float) * 7 * (size_t)s->mrows * s->fstride));
   DPMD_HIP(hipMalloc(&s->d_eatom, sizeof(float) * std::max(1, s->mrows)));
   s->partial_blocks = s->nrep * ((s->natoms + 3) / 4);
   DPMD_HIP(hipMalloc(&s->d_partial, sizeof(double) * 9 * s->partial_blocks));
@@ -258,13 +292,13 @@ static int32_t model_forces_impl(System* s, int precision) {
   s->force_valid = true;
   DPMD_TRY(md_zero_force(s));
   timer_begin(s, 1, &ev);
-  DPMD_TRY(embed_launch(s, false));
+  DPMD_TRY(embed_launch(s, false, precision));
   timer_end(s, 1, ev);
   timer_begin(s, 2, &ev);
   DPMD_TRY(fit_launch(s, precision));
   timer_end(s, 2, ev);
   timer_begin(s, 3, &ev);
-  DPMD_TRY(embed_launch(s, true));
+  DPMD_TRY(embed_launch(s, true, precision));
   timer_end(s, 3, ev);
   return DPMD_OK;
 }

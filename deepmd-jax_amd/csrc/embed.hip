@@ -173,6 +173,8 @@ struct EmbedArgs {
   int* paircnt;                // (R*N)
   int list_cap;                // LDS list entries per wave
   int net_bytes;
+  const void* hnet;            // 16-bit operand tables (bf16 or fp16), per net hnet_stride bytes
+  int hnet_stride, hnet_bytes;
   int lds_wave_floats;
 };
 
@@ -243,7 +245,48 @@ __device__ __forceinline__ void env_backward(const Geo& g, const float dR[10], f
   }
 }
 
-__device__ __forceinline__ float tanh_f(float x) { return tanhf(x); }
+// tanh(x) = 1 - 2 / (1 + 2^(2x log2 e)) with the hardware exp2 and reciprocal; the odd
+// Taylor polynomial below |x| < 2^-8 keeps relative accuracy where 1 - (...) cancels.
+// Absolute error ~2e-7 over the real line.
+__device__ __forceinline__ float tanh_f(float x) {
+  const float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);
+  const float t = 1.f - 2.f * __builtin_amdgcn_rcpf(1.f + e);
+  const float x2 = x * x;
+  return fabsf(x) < 0.00390625f ? x * (1.f - x2 * (1.f / 3.f)) : t;
+}
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+template <int PREC>
+struct H8;
+template <>
+struct H8<1> {
+  typedef bf16x8 T;
+  typedef __bf16 E;
+  static __device__ __forceinline__ floatx4 mfma(T a, T b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct H8<2> {
+  typedef halfx8 T;
+  typedef _Float16 E;
+  static __device__ __forceinline__ floatx4 mfma(T a, T b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+  }
+};
+
+// B operand of a 16x16x32 step from 8 consecutive fp32 registers r0..r0+7 (zero past n)
+template <int PREC, int NREG, typename GET>
+__device__ __forceinline__ typename H8<PREC>::T pack8(GET get, int s2) {
+  typedef typename H8<PREC>::E E;
+  typename H8<PREC>::T v;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) v[j] = (8 * s2 + j < NREG) ? (E)get(8 * s2 + j) : (E)0.f;
+  return v;
+}
 
 // acc[ot] += sum_{s in [S0, S0+SN)} A(ot, s) * b(s), A read from the packed [ot][s][lane]
 // table with row length S.  Weight operands are prefetched PF steps ahead; the
@@ -272,6 +315,33 @@ __device__ __forceinline__ void mfma_chain(rsrc_t rs, int sbase, int vl4, BF bfu
     const float b = bfun(S0 + s);
 #pragma unroll
     for (int ot = 0; ot < OT; ++ot) acc[ot] = mfma4(wb[s % (PF + 1)][ot], b, acc[ot]);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// 16-bit A tables are [ot][s2][lane][8]: one 16-byte load per lane per 16x16x32 MFMA.
+template <int PREC, int OT, int S2, int S0, int SN, typename BF>
+__device__ __forceinline__ void mfma_chain_h(rsrc_t rs, int sbase, int vl16, BF bfun, floatx4 (&acc)[OT]) {
+  typedef typename H8<PREC>::T HT;
+  constexpr int PF = 2;
+  HT wb[PF + 1][OT];
+#pragma unroll
+  for (int p = 0; p < PF && p < SN; ++p)
+#pragma unroll
+    for (int ot = 0; ot < OT; ++ot)
+      wb[p][ot] = __builtin_bit_cast(HT, (u32x4)__builtin_amdgcn_raw_buffer_load_b128(
+                                             rs, vl16, sbase + (ot * S2 + S0 + p) * 1024, 0));
+#pragma unroll
+  for (int s = 0; s < SN; ++s) {
+    if (s + PF < SN) {
+#pragma unroll
+      for (int ot = 0; ot < OT; ++ot)
+        wb[(s + PF) % (PF + 1)][ot] = __builtin_bit_cast(
+            HT, (u32x4)__builtin_amdgcn_raw_buffer_load_b128(rs, vl16, sbase + (ot * S2 + S0 + s + PF) * 1024, 0));
+    }
+    const HT b = bfun(S0 + s);
+#pragma unroll
+    for (int ot = 0; ot < OT; ++ot) acc[ot] = H8<PREC>::mfma(wb[s % (PF + 1)][ot], b, acc[ot]);
     __builtin_amdgcn_sched_barrier(0);
   }
 }
@@ -330,17 +400,22 @@ __device__ __forceinline__ int compact_row(const EmbedArgs& a, int rep, int c, i
 // ---------------------------------------------------------------------------------
 // Forward: T (10 x W) and G (A x W) per centre.  One wave per centre.
 
-template <int T1, int T2, int T3, int Z2, int Z3>
+template <int T1, int T2, int T3, int Z2, int Z3, int PREC>
 __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
   extern __shared__ float smem[];
   const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const rsrc_t wrs = __builtin_amdgcn_make_buffer_rsrc((void*)a.net, (short)0, a.net_bytes, 0x00020000);
+  const rsrc_t hrs = __builtin_amdgcn_make_buffer_rsrc((void*)(PREC ? a.hnet : a.net), (short)0,
+                                                       PREC ? a.hnet_bytes : a.net_bytes, 0x00020000);
+  constexpr int S21 = (4 * T1 + 7) / 8, S22 = (4 * T2 + 7) / 8, S23 = (4 * T3 + 7) / 8;
+  constexpr int H_W1A = 0, H_W2A = H_W1A + T2 * S21 * 1024;
+  (void)S23;
   const int q = lane >> 4, n = lane & 15;
-  constexpr int HS = 16 * T3 + 1;
   float* wbase = smem + wv * a.lds_wave_floats;
   float4* list = reinterpret_cast<float4*>(wbase);
   float* Rl = wbase + 4 * a.list_cap;
-  float* Hl = Rl + 256;
+  float* Hl = Rl + 256;  // one tile group of h3: [16 neighbours][HG]
+  constexpr int HG = 16 * kGroup + 1;
 
   const int rep = blockIdx.x / a.blocks_per_rep;
   const int c = (blockIdx.x % a.blocks_per_rep) * 4 + wv;
@@ -364,9 +439,11 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
     constexpr int O_K0 = 0, O_B0 = 16 * T1, O_B1 = 32 * T1, O_B2 = 32 * T1 + 16 * T2;
     constexpr int P0 = ((16 * T1 * 2 + 16 * T2 + 16 * T3) + 63) / 64 * 64;
     constexpr int O_W1A = P0, O_W2A = O_W1A + T2 * 4 * T1 * 64;
-    const int vl4 = lane * 4, vq16 = q * 16;
+    const int vl4 = lane * 4, vq16 = q * 16, vl16 = lane * 16;
+    const int hnetb = PREC ? (ti * a.ntypes + j) * a.hnet_stride : 0;
     for (int ch = 0; ch < m; ch += 16) {
       const int nb = netb + opaque_zero();
+      const int hb = hnetb + opaque_zero();
       auto k0 = [&](int r) { return bld(wrs, vq16, nb + 4 * (O_K0 + 16 * (r / 4) + (r % 4))); };
       auto b0 = [&](int r) { return bld(wrs, vq16, nb + 4 * (O_B0 + 16 * (r / 4) + (r % 4))); };
       auto b1 = [&](int r) { return bld(wrs, vq16, nb + 4 * (O_B1 + 16 * (r / 4) + (r % 4))); };
@@ -386,7 +463,11 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
       floatx4 acc2[T2];
 #pragma unroll
       for (int t = 0; t < T2; ++t) acc2[t] = floatx4{0.f, 0.f, 0.f, 0.f};
-      mfma_chain<T2, 4 * T1, 0, 4 * T1>(wrs, nb + 4 * O_W1A, vl4, [&](int s_) { return h1[s_]; }, acc2);
+      if constexpr (PREC == 0)
+        mfma_chain<T2, 4 * T1, 0, 4 * T1>(wrs, nb + 4 * O_W1A, vl4, [&](int s_) { return h1[s_]; }, acc2);
+      else
+        mfma_chain_h<PREC, T2, S21, 0, S21>(hrs, hb + H_W1A, vl16, [&](int s2) {
+          return pack8<PREC, 4 * T1>([&](int r) { return h1[r]; }, s2); }, acc2);
       float h2[4 * T2];
 #pragma unroll
       for (int r = 0; r < 4 * T2; ++r) {
@@ -399,32 +480,35 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
         floatx4 acc3[SZ];
 #pragma unroll
         for (int t = 0; t < SZ; ++t) acc3[t] = floatx4{0.f, 0.f, 0.f, 0.f};
-        mfma_chain<SZ, 4 * T2, 0, 4 * T2>(wrs, nb + 4 * (O_W2A + g0 * 4 * T2 * 64), vl4,
-                                          [&](int s_) { return h2[s_]; }, acc3);
+        if constexpr (PREC == 0)
+          mfma_chain<SZ, 4 * T2, 0, 4 * T2>(wrs, nb + 4 * (O_W2A + g0 * 4 * T2 * 64), vl4,
+                                            [&](int s_) { return h2[s_]; }, acc3);
+        else
+          mfma_chain_h<PREC, SZ, S22, 0, S22>(hrs, hb + H_W2A + g0 * S22 * 1024, vl16, [&](int s2) {
+            return pack8<PREC, 4 * T2>([&](int r) { return h2[r]; }, s2); }, acc3);
 #pragma unroll
         for (int k = 0; k < SZ; ++k)
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
             const int r = 4 * (g0 + k) + i;
-            const int slot = 16 * (g0 + k) + 4 * q + i;
             float v = tanh_f(acc3[k][i] + b2(r));
             if (r / Z3 < 4 * T2) v += h2[r / Z3];
-            Hl[n * HS + slot] = v;
+            Hl[n * HG + 16 * k + 4 * q + i] = v;
           }
+        wave_sync();
+        // reducer for this group: T[x][slot] += sum_nbr R[nbr][x] h3[nbr][slot]
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+          const float av = Rl[(4 * kk + q) * 16 + n];
+#pragma unroll
+          for (int k = 0; k < SZ; ++k) Tacc[g0 + k] = mfma4(av, Hl[(4 * kk + q) * HG + 16 * k + n], Tacc[g0 + k]);
+        }
+        wave_sync();
       });
-      wave_sync();
-      // reducer: T[x][slot] += sum_nbr R[nbr][x] h3[nbr][slot]
-#pragma unroll
-      for (int kk = 0; kk < 4; ++kk) {
-        const float av = Rl[(4 * kk + q) * 16 + n];
-#pragma unroll
-        for (int nt = 0; nt < T3; ++nt) Tacc[nt] = mfma4(av, Hl[(4 * kk + q) * HS + 16 * nt + n], Tacc[nt]);
-      }
-      wave_sync();
     }
   }
   // T in logical order into LDS, then T and G to global
-  float* Tl = Hl;
+  float* Tl = wbase;
   const int W = a.W, A = a.A;
 #pragma unroll
   for (int nt = 0; nt < T3; ++nt) {
@@ -469,12 +553,18 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
 // ---------------------------------------------------------------------------------
 // Backward: forces (scatter) and virial partials.  One wave per centre.
 
-template <int T1, int T2, int T3, int Z2, int Z3>
+template <int T1, int T2, int T3, int Z2, int Z3, int PREC>
 __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
   extern __shared__ float smem[];
   __shared__ double vir_s[4][9];
   const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const rsrc_t wrs = __builtin_amdgcn_make_buffer_rsrc((void*)a.net, (short)0, a.net_bytes, 0x00020000);
+  const rsrc_t hrs = __builtin_amdgcn_make_buffer_rsrc((void*)(PREC ? a.hnet : a.net), (short)0,
+                                                       PREC ? a.hnet_bytes : a.net_bytes, 0x00020000);
+  constexpr int S21 = (4 * T1 + 7) / 8, S22 = (4 * T2 + 7) / 8, S23 = (4 * T3 + 7) / 8;
+  constexpr int H_W1A = 0, H_W2A = H_W1A + T2 * S21 * 1024, H_W2B = H_W2A + T3 * S22 * 1024,
+                H_W1B = H_W2B + T2 * S23 * 1024;
+  static_assert(kGroup * 4 == 8, "16-bit backward assumes two layer-3 tiles per group");
   const int q = lane >> 4, n = lane & 15;
   constexpr int HS = 16 * T3 + 1;
   const int W = a.W, A = a.A;
@@ -484,10 +574,6 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
   float* Rl = dTp + 16 * HS;             // 256
   float* dRl = Rl + 256;                 // 256
   float4* list = reinterpret_cast<float4*>(dRl + 256);
-  // prologue layout (aliases list region)
-  float* Tl = dRl + 256;                 // 10*W
-  float* dGl = Tl + 10 * W;              // A*W
-  float* Ul = dGl + A * W;               // 10*A
 
   const int rep = blockIdx.x / a.blocks_per_rep;
   const int c = (blockIdx.x % a.blocks_per_rep) * 4 + wv;
@@ -501,63 +587,19 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
     int ti = 0;
     while (ti + 1 < a.ntypes && c >= a.off[ti + 1]) ++ti;
     const float4 pc = a.pos[gc];
-    const int grow = a.rowbase[ti] + rep * a.count[ti] + (c - a.off[ti]);
-    // ---- prologue: dT = d(G)/d(T)^T dG, scaled by 1/Nnbrs, into dTp (physical slots)
+    // ---- prologue: dE/dT (logical [x][w], from dT_kernel) into the physical slot layout
     {
-      const float* Tg = a.T + (size_t)gc * 10 * W;
-      const float* dGg = a.dG + (size_t)grow * A * W;
-      for (int e = lane; e < 10 * W; e += 64) Tl[e] = Tg[e];
-      for (int e = lane; e < A * W; e += 64) dGl[e] = dGg[e];
+      const float* dTg = a.T + (size_t)gc * 10 * W;
       for (int e = lane; e < 16 * HS; e += 64) dTp[e] = 0.f;
       wave_sync();
-      for (int e = lane; e < W; e += 64) Tl[e] += a.tbias[e];  // row 0 becomes TW
-      wave_sync();
-      for (int e = lane; e < 10 * A; e += 64) {
-        const int x = e / A, aa = e % A;
-        float s = 0.f;
-        for (int w = 0; w < W; ++w) s += dGl[aa * W + w] * Tl[x * W + w];
-        Ul[x * A + aa] = s;
-      }
-      wave_sync();
-      const float r2 = 1.4142135623730951f;
       for (int e = lane; e < 10 * W; e += 64) {
         const int x = e / W, w = e % W;
-        float v = 0.f;
-        if (x < 4) {
-          for (int aa = 0; aa < A; ++aa) v += dGl[aa * W + w] * Tl[x * W + aa];
-          if (w < A) v += Ul[x * A + w];
-        } else {
-          const int k = x - 4;
-          for (int aa = 0; aa < A; ++aa) {
-            float g2;
-            if (k < 3) {
-              const float t3 = Tl[(1 + k) * W + A + aa];
-              g2 = t3 * t3 + Tl[(4 + k) * W + A + aa];
-            } else {
-              const int kp = k - 3;
-              g2 = r2 * Tl[(1 + kp) * W + A + aa] * Tl[(1 + (kp + 2) % 3) * W + A + aa] +
-                   Tl[(4 + k) * W + A + aa];
-            }
-            v += dGl[aa * W + w] * g2;
-          }
-        }
-        if (w >= A && w < 2 * A) {
-          const int aa = w - A;
-          if (x >= 4) v += Ul[x * A + aa];
-          if (x >= 1 && x <= 3) {
-            const int cc = x - 1;
-            v += 2.f * Tl[x * W + w] * Ul[(4 + cc) * A + aa];
-            v += r2 * Tl[(1 + (cc + 2) % 3) * W + w] * Ul[(7 + cc) * A + aa];
-            v += r2 * Tl[(1 + (cc + 1) % 3) * W + w] * Ul[(7 + (cc + 1) % 3) * A + aa];
-          }
-        }
-        // physical slot of logical feature w
         int slot = -1;
         for (int qq = 0; qq < 4; ++qq) {
           const int rr = w - a.base3[qq];
           if (rr >= 0 && rr < a.size3[qq]) slot = 16 * (rr / 4) + 4 * qq + (rr % 4);
         }
-        if (slot >= 0) dTp[x * HS + slot] = v * a.inv_nnbrs;
+        if (slot >= 0) dTp[x * HS + slot] = dTg[e];
       }
       wave_sync();
     }
@@ -571,9 +613,11 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
       constexpr int P0 = ((16 * T1 * 2 + 16 * T2 + 16 * T3) + 63) / 64 * 64;
       constexpr int O_W1A = P0, O_W2A = O_W1A + T2 * 4 * T1 * 64;
       constexpr int O_W2B = O_W2A + T3 * 4 * T2 * 64, O_W1B = O_W2B + T2 * 4 * T3 * 64;
-      const int vl4 = lane * 4, vq16 = q * 16;
+      const int vl4 = lane * 4, vq16 = q * 16, vl16 = lane * 16;
+      const int hnetb = PREC ? (ti * a.ntypes + j) * a.hnet_stride : 0;
       for (int ch = 0; ch < m; ch += 16) {
         const int nb = netb + opaque_zero();
+        const int hb = hnetb + opaque_zero();
         auto k0 = [&](int r) { return bld(wrs, vq16, nb + 4 * (O_K0 + 16 * (r / 4) + (r % 4))); };
         auto b0 = [&](int r) { return bld(wrs, vq16, nb + 4 * (O_B0 + 16 * (r / 4) + (r % 4))); };
         auto b1 = [&](int r) { return bld(wrs, vq16, nb + 4 * (O_B1 + 16 * (r / 4) + (r % 4))); };
@@ -597,7 +641,11 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
         floatx4 acc2[T2];
 #pragma unroll
         for (int t = 0; t < T2; ++t) acc2[t] = floatx4{0.f, 0.f, 0.f, 0.f};
-        mfma_chain<T2, 4 * T1, 0, 4 * T1>(wrs, nb + 4 * O_W1A, vl4, [&](int s_) { return h1[s_]; }, acc2);
+        if constexpr (PREC == 0)
+          mfma_chain<T2, 4 * T1, 0, 4 * T1>(wrs, nb + 4 * O_W1A, vl4, [&](int s_) { return h1[s_]; }, acc2);
+        else
+          mfma_chain_h<PREC, T2, S21, 0, S21>(hrs, hb + H_W1A, vl16, [&](int s2) {
+            return pack8<PREC, 4 * T1>([&](int r) { return h1[r]; }, s2); }, acc2);
         float t2[4 * T2];
 #pragma unroll
         for (int r = 0; r < 4 * T2; ++r) t2[r] = tanh_f(acc2[r / 4][r % 4] + b1(r));
@@ -611,8 +659,12 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
           floatx4 acc3[SZ];
 #pragma unroll
           for (int t = 0; t < SZ; ++t) acc3[t] = floatx4{0.f, 0.f, 0.f, 0.f};
-          mfma_chain<SZ, 4 * T2, 0, 4 * T2>(wrs, nb + 4 * (O_W2A + g0 * 4 * T2 * 64), vl4,
-                                            [&](int s_) { return H2(s_, t2); }, acc3);
+          if constexpr (PREC == 0)
+            mfma_chain<SZ, 4 * T2, 0, 4 * T2>(wrs, nb + 4 * (O_W2A + g0 * 4 * T2 * 64), vl4,
+                                              [&](int s_) { return H2(s_, t2); }, acc3);
+          else
+            mfma_chain_h<PREC, SZ, S22, 0, S22>(hrs, hb + H_W2A + g0 * S22 * 1024, vl16, [&](int s2) {
+              return pack8<PREC, 4 * T2>([&](int r) { return H2(r, t2); }, s2); }, acc3);
           float t3[4 * SZ];
 #pragma unroll
           for (int k = 0; k < SZ; ++k)
@@ -646,8 +698,12 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
               if (r / Z3 < 4 * T2) dh2[(r / Z3) / 4][(r / Z3) % 4] += dh3[k][i];
               dh3[k][i] *= (1.f - t3[4 * k + i] * t3[4 * k + i]);
             }
-          mfma_chain<T2, 4 * T3, 4 * g0, 4 * SZ>(wrs, nb + 4 * O_W2B, vl4,
-                                                [&](int s_) { return dh3[(s_ - 4 * g0) / 4][(s_ - 4 * g0) % 4]; }, dh2);
+          if constexpr (PREC == 0)
+            mfma_chain<T2, 4 * T3, 4 * g0, 4 * SZ>(wrs, nb + 4 * O_W2B, vl4,
+                                                  [&](int s_) { return dh3[(s_ - 4 * g0) / 4][(s_ - 4 * g0) % 4]; }, dh2);
+          else
+            mfma_chain_h<PREC, T2, S23, g0 / 2, 1>(hrs, hb + H_W2B, vl16, [&](int s2) {
+              return pack8<PREC, 4 * SZ>([&](int r) { return dh3[r / 4][r % 4]; }, 0); }, dh2);
         });
 #pragma unroll
         for (int i = 0; i < 4; ++i) dRl[n * 16 + 4 * q + i] = dr[i];
@@ -670,7 +726,11 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
         }
 #pragma unroll
         for (int r = 0; r < 4 * T2; ++r) dh2[r / 4][r % 4] *= (1.f - t2[r] * t2[r]);
-        mfma_chain<T1, 4 * T2, 0, 4 * T2>(wrs, nb + 4 * O_W1B, vl4, [&](int s_) { return dh2[s_ / 4][s_ % 4]; }, dh1);
+        if constexpr (PREC == 0)
+          mfma_chain<T1, 4 * T2, 0, 4 * T2>(wrs, nb + 4 * O_W1B, vl4, [&](int s_) { return dh2[s_ / 4][s_ % 4]; }, dh1);
+        else
+          mfma_chain_h<PREC, T1, S22, 0, S22>(hrs, hb + H_W1B, vl16, [&](int s2) {
+            return pack8<PREC, 4 * T2>([&](int r) { return dh2[r / 4][r % 4]; }, s2); }, dh1);
         float dst = 0.f;
 #pragma unroll
         for (int r1 = 0; r1 < 4 * T1; ++r1) dst += dh1[r1 / 4][r1 % 4] * (1.f - h1[r1] * h1[r1]) * k0(r1);
@@ -734,6 +794,77 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
 }
 
 // ---------------------------------------------------------------------------------
+// dE/dT from dE/dG (descriptor backward, deepmd_jax/dpmodel.py:74-78), scaled by
+// 1/Nnbrs and written in place over T (logical [x][w] layout).  One wave per centre.
+//   G[a][w] = TW[w] TW[a] + sum_c T3[c][w] T3[c][a] + sum_k G2[k][a] T6[k][w]
+//   G2[k][a] = T3[k][A+a]^2 + T6[k][A+a] (k<3);  sqrt2 T3[k'][A+a] T3[k'-1][A+a] + T6[3+k'][A+a]
+
+__global__ void __launch_bounds__(256) dT_kernel(const EmbedArgs a) {
+  extern __shared__ float smem[];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int W = a.W, A = a.A;
+  float* Tl = smem + wv * a.lds_wave_floats;  // 10*W
+  float* dGl = Tl + 10 * W;                    // A*W
+  float* Ul = dGl + A * W;                     // 10*A
+  const int rep = blockIdx.x / a.blocks_per_rep;
+  const int c = (blockIdx.x % a.blocks_per_rep) * 4 + wv;
+  if (c >= a.natoms) return;
+  const int gc = rep * a.natoms + c;
+  int ti = 0;
+  while (ti + 1 < a.ntypes && c >= a.off[ti + 1]) ++ti;
+  const int grow = a.rowbase[ti] + rep * a.count[ti] + (c - a.off[ti]);
+  float* Tg = a.T + (size_t)gc * 10 * W;
+  const float* dGg = a.dG + (size_t)grow * A * W;
+  for (int e = lane; e < 10 * W; e += 64) Tl[e] = Tg[e] + (e < W ? a.tbias[e] : 0.f);  // row 0 becomes TW
+  for (int e = lane; e < A * W; e += 64) dGl[e] = dGg[e];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (int e = lane; e < 10 * A; e += 64) {
+    const int x = e / A, aa = e % A;
+    float s = 0.f;
+    for (int w = 0; w < W; ++w) s += dGl[aa * W + w] * Tl[x * W + w];
+    Ul[x * A + aa] = s;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const float r2 = 1.4142135623730951f;
+  for (int e = lane; e < 10 * W; e += 64) {
+    const int x = e / W, w = e % W;
+    float v = 0.f;
+    if (x < 4) {
+      for (int aa = 0; aa < A; ++aa) v += dGl[aa * W + w] * Tl[x * W + aa];
+      if (w < A) v += Ul[x * A + w];
+    } else {
+      const int k = x - 4;
+      for (int aa = 0; aa < A; ++aa) {
+        float g2;
+        if (k < 3) {
+          const float t3 = Tl[(1 + k) * W + A + aa];
+          g2 = t3 * t3 + Tl[(4 + k) * W + A + aa];
+        } else {
+          const int kp = k - 3;
+          g2 = r2 * Tl[(1 + kp) * W + A + aa] * Tl[(1 + (kp + 2) % 3) * W + A + aa] + Tl[(4 + k) * W + A + aa];
+        }
+        v += dGl[aa * W + w] * g2;
+      }
+    }
+    if (w >= A && w < 2 * A) {
+      const int aa = w - A;
+      if (x >= 4) v += Ul[x * A + aa];
+      if (x >= 1 && x <= 3) {
+        const int cc = x - 1;
+        v += 2.f * Tl[x * W + w] * Ul[(4 + cc) * A + aa];
+        v += r2 * Tl[(1 + (cc + 2) % 3) * W + w] * Ul[(7 + cc) * A + aa];
+        v += r2 * Tl[(1 + (cc + 1) % 3) * W + w] * Ul[(7 + (cc + 1) % 3) * A + aa];
+      }
+    }
+    Tg[e] = v * a.inv_nnbrs;
+  }
+}
+
+// ---------------------------------------------------------------------------------
 // Launchers
 
 static void fill_args(System* s, EmbedArgs& a) {
@@ -778,21 +909,33 @@ static void fill_args(System* s, EmbedArgs& a) {
   a.list_cap = (cmax + 3) / 4 * 4;
 }
 
-template <int T1, int T2, int T3, int Z2, int Z3>
+template <int T1, int T2, int T3, int Z2, int Z3, int PREC>
 static int32_t launch_embed(System* s, bool backward) {
   EmbedArgs a;
   fill_args(s, a);
+  if (PREC) {
+    Model* M = s->model;
+    a.hnet = M->d_emb_h[PREC - 1];
+    a.hnet_stride = M->offh.total * 2;
+    a.hnet_bytes = M->offh.total * 2 * M->ntypes * M->ntypes;
+    if (!a.hnet) {
+      set_error("16-bit embedding tables missing");
+      return DPMD_E_STATE;
+    }
+  }
   constexpr int HS = 16 * T3 + 1;
   const int W = a.W, A = a.A;
   const int nblocks = a.blocks_per_rep * s->nrep;
   size_t wave_floats;
   if (!backward) {
-    wave_floats = 4 * a.list_cap + 256 + 16 * HS;
-    wave_floats = std::max(wave_floats, (size_t)(4 * a.list_cap + 256 + 10 * W));
+    wave_floats = 4 * a.list_cap + 256 + 16 * (16 * kGroup + 1);
+    wave_floats = std::max(wave_floats, (size_t)(10 * W));
   } else {
-    const size_t chunk = 16 * HS + 512 + 4 * a.list_cap;
-    const size_t pro = 16 * HS + 512 + 10 * W + A * W + 10 * A;
-    wave_floats = std::max(chunk, pro);
+    wave_floats = 16 * HS + 512 + 4 * a.list_cap;
+    // dE/dG -> dE/dT first, in place over T
+    EmbedArgs b = a;
+    b.lds_wave_floats = (10 * W + A * W + 10 * A + 3) / 4 * 4;
+    hipLaunchKernelGGL(dT_kernel, dim3(nblocks), dim3(256), 4 * b.lds_wave_floats * sizeof(float), s->stream, b);
   }
   wave_floats = (wave_floats + 3) / 4 * 4;
   a.lds_wave_floats = (int)wave_floats;
@@ -802,26 +945,81 @@ static int32_t launch_embed(System* s, bool backward) {
     return DPMD_E_UNSUPPORTED;
   }
   if (!backward) {
-    hipLaunchKernelGGL((embed_fwd_kernel<T1, T2, T3, Z2, Z3>), dim3(nblocks), dim3(256), lds, s->stream, a);
+    hipLaunchKernelGGL((embed_fwd_kernel<T1, T2, T3, Z2, Z3, PREC>), dim3(nblocks), dim3(256), lds, s->stream, a);
   } else {
     if (s->partial_blocks < nblocks) {
       set_error("virial partial buffer too small");
       return DPMD_E_STATE;
     }
-    hipLaunchKernelGGL((embed_bwd_kernel<T1, T2, T3, Z2, Z3>), dim3(nblocks), dim3(256), lds, s->stream, a);
+    hipLaunchKernelGGL((embed_bwd_kernel<T1, T2, T3, Z2, Z3, PREC>), dim3(nblocks), dim3(256), lds, s->stream, a);
   }
   DPMD_HIP(hipGetLastError());
   return DPMD_OK;
 }
 
-int32_t embed_launch(System* s, bool backward) {
+template <int T1, int T2, int T3, int Z2, int Z3>
+static int32_t launch_prec(System* s, bool backward, int precision) {
+  if (precision == DPMD_PREC_BF16) return launch_embed<T1, T2, T3, Z2, Z3, 1>(s, backward);
+  if (precision == DPMD_PREC_FP16) return launch_embed<T1, T2, T3, Z2, Z3, 2>(s, backward);
+  return launch_embed<T1, T2, T3, Z2, Z3, 0>(s, backward);
+}
+
+int32_t embed_launch(System* s, bool backward, int precision) {
   const EmbedLayout& L = s->model->lay;
   if (L.tiles[0] == 2 && L.tiles[1] == 4 && L.tiles[2] == 7 && L.z[1] == 2 && L.z[2] == 2)
-    return launch_embed<2, 4, 7, 2, 2>(s, backward);
+    return launch_prec<2, 4, 7, 2, 2>(s, backward, precision);
   if (L.tiles[0] == 2 && L.tiles[1] == 2 && L.tiles[2] == 4 && L.z[1] == 1 && L.z[2] == 2)
-    return launch_embed<2, 2, 4, 1, 2>(s, backward);
+    return launch_prec<2, 2, 4, 1, 2>(s, backward, precision);
   set_error("embedding widths (%d,%d,%d) have no kernel instantiation", L.w[0], L.w[1], L.w[2]);
   return DPMD_E_UNSUPPORTED;
+}
+
+// 16-bit operand tables derived from the packed fp32 tables: [ot][s2][lane][8] with
+// element j of step s2 taken from fp32 register step 8*s2 + j (zero past the end).
+EmbedNetOffsetsH embed_net_offsets_h(const EmbedLayout& L) {
+  const int t1 = L.tiles[0], t2 = L.tiles[1], t3 = L.tiles[2];
+  const int s21 = (4 * t1 + 7) / 8, s22 = (4 * t2 + 7) / 8, s23 = (4 * t3 + 7) / 8;
+  EmbedNetOffsetsH o;
+  o.w1a = 0;
+  o.w2a = o.w1a + t2 * s21 * 512;
+  o.w2b = o.w2a + t3 * s22 * 512;
+  o.w1b = o.w2b + t2 * s23 * 512;
+  o.total = o.w1b + t1 * s22 * 512;
+  return o;
+}
+
+static uint16_t to_bf16(float f) {
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+static uint16_t to_fp16(float f) {
+  _Float16 h = (_Float16)f;
+  uint16_t u;
+  memcpy(&u, &h, 2);
+  return u;
+}
+
+void embed_pack_net_h(const EmbedLayout& L, const float* f32, uint16_t* out, bool fp16) {
+  const EmbedNetOffsets o = embed_net_offsets(L);
+  const EmbedNetOffsetsH oh = embed_net_offsets_h(L);
+  const int t1 = L.tiles[0], t2 = L.tiles[1], t3 = L.tiles[2];
+  struct Tab { int src, dst, ot, sregs; } tabs[4] = {
+      {o.w1a, oh.w1a, t2, 4 * t1}, {o.w2a, oh.w2a, t3, 4 * t2}, {o.w2b, oh.w2b, t2, 4 * t3}, {o.w1b, oh.w1b, t1, 4 * t2}};
+  memset(out, 0, sizeof(uint16_t) * oh.total);
+  for (const Tab& t : tabs) {
+    const int s2n = (t.sregs + 7) / 8;
+    for (int ot = 0; ot < t.ot; ++ot)
+      for (int s2 = 0; s2 < s2n; ++s2)
+        for (int lane = 0; lane < 64; ++lane)
+          for (int j = 0; j < 8; ++j) {
+            const int r = 8 * s2 + j;
+            const float v = r < t.sregs ? f32[t.src + (ot * t.sregs + r) * 64 + lane] : 0.f;
+            out[t.dst + ((ot * s2n + s2) * 64 + lane) * 8 + j] = fp16 ? to_fp16(v) : to_bf16(v);
+          }
+  }
 }
 
 }  // namespace dpmd

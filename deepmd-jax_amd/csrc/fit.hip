@@ -8,20 +8,26 @@
 //   dz3 = k3 dt2 (1-t3^2); dy2 = k3 + dz3 K2^T; dz2 = dy2 dt1 (1-t2^2)
 //   dy1 = dy2 + dz2 K1^T;  dz1 = dy1 (1-y1^2);  dG = dz1 K0^T
 // Rows are atoms of one type (padded to 64).  B is given transposed (N x K row-major).
+// FP32: v_mfma_f32_16x16x4f32, K staged in 16-wide slices.  BF16 / FP16: A converted
+// while staging into LDS, B pre-converted, v_mfma_f32_16x16x32_{bf16,f16}, FP32
+// accumulation and FP32 epilogues.
 #include "internal.h"
 #include <algorithm>
 #include <string.h>
+#include <type_traits>
 
 namespace dpmd {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
 
 enum FitEpi { EPI_TANH = 0, EPI_RES = 1, EPI_E3 = 2, EPI_BWD2 = 3, EPI_BWD1 = 4, EPI_STORE = 5 };
 
 struct GemmArgs {
-  const float* A;   // M x K, row stride lda
+  const float* A;   // M x K fp32, row stride lda
   int lda;
-  const float* Bt;  // N x K, row stride ldb
+  const void* Bt;   // N x K (fp32 or 16-bit), row stride ldb elements
   int ldb;
   int M, N, K;
   float* C;         // primary output, row stride ldc
@@ -36,76 +42,16 @@ struct GemmArgs {
   float* eout;      // per-row energy
 };
 
-constexpr int KC = 16;
-constexpr int LDS_STRIDE = KC + 4;
+__device__ __forceinline__ float tanh_fit(float x) {
+  const float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);
+  const float t = 1.f - 2.f * __builtin_amdgcn_rcpf(1.f + e);
+  return fabsf(x) < 0.00390625f ? x * (1.f - x * x * (1.f / 3.f)) : t;
+}
 
+// lane holds C[m0 + 16wv + 4q + i][n0 + 16nt + n]
 template <int NT, int EPI>
-__global__ void __launch_bounds__(256, 2) gemm_f32_kernel(const GemmArgs g) {
-  __shared__ float As[64 * LDS_STRIDE];
-  __shared__ float Bs[16 * NT * LDS_STRIDE];
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wv = tid >> 6;
-  const int q = lane >> 4, n = lane & 15;
-  const int m0 = blockIdx.x * 64;
-  const int n0 = blockIdx.y * 16 * NT;
-  constexpr int BN = 16 * NT;
-  constexpr int BV = BN * KC / 4;  // float4 per B chunk
-
-  floatx4 acc[NT];
-#pragma unroll
-  for (int t = 0; t < NT; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
-
-  // staging registers
-  float4 ra;
-  float4 rb[(BV + 255) / 256];
-  auto load_chunk = [&](int k0) {
-    {
-      const int row = tid / 4, c4 = tid % 4;
-      ra = *reinterpret_cast<const float4*>(g.A + (size_t)(m0 + row) * g.lda + k0 + 4 * c4);
-    }
-#pragma unroll
-    for (int v = 0; v < (BV + 255) / 256; ++v) {
-      const int idx = tid + 256 * v;
-      if (idx < BV) {
-        const int row = idx / 4, c4 = idx % 4;
-        rb[v] = *reinterpret_cast<const float4*>(g.Bt + (size_t)(n0 + row) * g.ldb + k0 + 4 * c4);
-      }
-    }
-  };
-  auto store_chunk = [&]() {
-    {
-      const int row = tid / 4, c4 = tid % 4;
-      *reinterpret_cast<float4*>(&As[row * LDS_STRIDE + 4 * c4]) = ra;
-    }
-#pragma unroll
-    for (int v = 0; v < (BV + 255) / 256; ++v) {
-      const int idx = tid + 256 * v;
-      if (idx < BV) {
-        const int row = idx / 4, c4 = idx % 4;
-        *reinterpret_cast<float4*>(&Bs[row * LDS_STRIDE + 4 * c4]) = rb[v];
-      }
-    }
-  };
-
-  load_chunk(0);
-  for (int k0 = 0; k0 < g.K; k0 += KC) {
-    __syncthreads();
-    store_chunk();
-    __syncthreads();
-    if (k0 + KC < g.K) load_chunk(k0 + KC);
-    // lane (row m = n, k-slot q) uses k = 4q + kk
-    const float4 av = *reinterpret_cast<const float4*>(&As[(16 * wv + n) * LDS_STRIDE + 4 * q]);
-#pragma unroll
-    for (int nt = 0; nt < NT; ++nt) {
-      const float4 bv = *reinterpret_cast<const float4*>(&Bs[(16 * nt + n) * LDS_STRIDE + 4 * q]);
-      acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, bv.x, acc[nt], 0, 0, 0);
-      acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, bv.y, acc[nt], 0, 0, 0);
-      acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.z, bv.z, acc[nt], 0, 0, 0);
-      acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.w, bv.w, acc[nt], 0, 0, 0);
-    }
-  }
-
-  // epilogue: lane holds C[m0 + 16wv + 4q + i][n0 + 16nt + n]
+__device__ __forceinline__ void epilogue(const GemmArgs& g, const floatx4 (&acc)[NT], int m0, int n0, int wv, int q,
+                                         int n) {
   float esum[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int nt = 0; nt < NT; ++nt) {
@@ -116,13 +62,13 @@ __global__ void __launch_bounds__(256, 2) gemm_f32_kernel(const GemmArgs g) {
       const float cv = acc[nt][i];
       const size_t o = row * g.ldc + col;
       if (EPI == EPI_TANH) {
-        g.C[o] = tanhf(cv + g.bias[col]);
+        g.C[o] = tanh_fit(cv + g.bias[col]);
       } else if (EPI == EPI_RES) {
-        const float t = tanhf(cv + g.bias[col]);
+        const float t = tanh_fit(cv + g.bias[col]);
         g.C2[o] = t;
         g.C[o] = t * g.dt[col] + g.aux1[o];
       } else if (EPI == EPI_E3) {
-        const float t = tanhf(cv + g.bias[col]);
+        const float t = tanh_fit(cv + g.bias[col]);
         const float y3 = t * g.dt[col] + g.aux1[o];
         esum[i] += y3 * g.k3[col];
         g.C[o] = g.k3[col] * g.dt[col] * (1.f - t * t);
@@ -153,98 +99,262 @@ __global__ void __launch_bounds__(256, 2) gemm_f32_kernel(const GemmArgs g) {
   }
 }
 
+// compile-time loop: f(integral_constant<V>) for V in [V0, V1)
+template <int V0, int V1>
+struct Unroll {
+  template <class F>
+  static __device__ __forceinline__ void run(F&& f) {
+    if constexpr (V0 < V1) {
+      f(std::integral_constant<int, V0>());
+      Unroll<V0 + 1, V1>::run(f);
+    }
+  }
+};
+
+constexpr int KC32 = 16;             // fp32 K slice
+constexpr int LS32 = KC32 + 4;       // fp32 LDS row stride (floats)
+constexpr int KC16 = 32;             // 16-bit K slice
+constexpr int LS16 = KC16 + 8;       // 16-bit LDS row stride (elements)
+
+template <int NT, int EPI>
+__global__ void __launch_bounds__(256, 2) gemm_f32_kernel(const GemmArgs g) {
+  __shared__ float As[64 * LS32];
+  __shared__ float Bs[16 * NT * LS32];
+  constexpr int BN = 16 * NT;
+  constexpr int BV = BN * KC32 / 4;  // float4 per B slice
+  constexpr int NB = (BV + 255) / 256;
+  static_assert(NB <= 4, "B staging unrolled for at most 4 pieces per thread");
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wv = tid >> 6;
+  const int q = lane >> 4, n = lane & 15;
+  const int m0 = blockIdx.x * 64;
+  const int n0 = blockIdx.y * BN;
+  const float* Bt = (const float*)g.Bt;
+  floatx4 acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const int arow = tid / 4, ac4 = tid % 4;
+  float4 ra = *reinterpret_cast<const float4*>(g.A + (size_t)(m0 + arow) * g.lda + 4 * ac4);
+  float4 rb[NB];
+if constexpr (0 < NB) { const int idx = tid + 256 * 0; if (0 + 1 < NB || idx < BV) rb[0] = *reinterpret_cast<const float4*>(Bt + (size_t)(n0 + idx / 4) * g.ldb + 4 * (idx % 4)); }
+    if constexpr (1 < NB) { const int idx = tid + 256 * 1; if (1 + 1 < NB || idx < BV) rb[1] = *reinterpret_cast<const float4*>(Bt + (size_t)(n0 + idx / 4) * g.ldb + 4 * (idx % 4)); }
+    if constexpr (2 < NB) { const int idx = tid + 256 * 2; if (2 + 1 < NB || idx < BV) rb[2] = *reinterpret_cast<const float4*>(Bt + (size_t)(n0 + idx / 4) * g.ldb + 4 * (idx % 4)); }
+    if constexpr (3 < NB) { const int idx = tid + 256 * 3; if (3 + 1 < NB || idx < BV) rb[3] = *reinterpret_cast<const float4*>(Bt + (size_t)(n0 + idx / 4) * g.ldb + 4 * (idx % 4)); }
+  for (int k0 = 0; k0 < g.K; k0 += KC32) {
+    __syncthreads();
+    *reinterpret_cast<float4*>(&As[arow * LS32 + 4 * ac4]) = ra;
+if constexpr (0 < NB) { const int idx = tid + 256 * 0; if (0 + 1 < NB || idx < BV) *reinterpret_cast<float4*>(&Bs[(idx / 4) * LS32 + 4 * (idx % 4)]) = rb[0]; }
+    if constexpr (1 < NB) { const int idx = tid + 256 * 1; if (1 + 1 < NB || idx < BV) *reinterpret_cast<float4*>(&Bs[(idx / 4) * LS32 + 4 * (idx % 4)]) = rb[1]; }
+    if constexpr (2 < NB) { const int idx = tid + 256 * 2; if (2 + 1 < NB || idx < BV) *reinterpret_cast<float4*>(&Bs[(idx / 4) * LS32 + 4 * (idx % 4)]) = rb[2]; }
+    if constexpr (3 < NB) { const int idx = tid + 256 * 3; if (3 + 1 < NB || idx < BV) *reinterpret_cast<float4*>(&Bs[(idx / 4) * LS32 + 4 * (idx % 4)]) = rb[3]; }
+    __syncthreads();
+    const int k1 = k0 + KC32;
+    if (k1 < g.K) {
+      ra = *reinterpret_cast<const float4*>(g.A + (size_t)(m0 + arow) * g.lda + k1 + 4 * ac4);
+if constexpr (0 < NB) { const int idx = tid + 256 * 0; if (0 + 1 < NB || idx < BV) rb[0] = *reinterpret_cast<const float4*>(Bt + (size_t)(n0 + idx / 4) * g.ldb + k1 + 4 * (idx % 4)); }
+    if constexpr (1 < NB) { const int idx = tid + 256 * 1; if (1 + 1 < NB || idx < BV) rb[1] = *reinterpret_cast<const float4*>(Bt + (size_t)(n0 + idx / 4) * g.ldb + k1 + 4 * (idx % 4)); }
+    if constexpr (2 < NB) { const int idx = tid + 256 * 2; if (2 + 1 < NB || idx < BV) rb[2] = *reinterpret_cast<const float4*>(Bt + (size_t)(n0 + idx / 4) * g.ldb + k1 + 4 * (idx % 4)); }
+    if constexpr (3 < NB) { const int idx = tid + 256 * 3; if (3 + 1 < NB || idx < BV) rb[3] = *reinterpret_cast<const float4*>(Bt + (size_t)(n0 + idx / 4) * g.ldb + k1 + 4 * (idx % 4)); }
+    }
+    const float4 av = *reinterpret_cast<const float4*>(&As[(16 * wv + n) * LS32 + 4 * q]);
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+      const float4 bv = *reinterpret_cast<const float4*>(&Bs[(16 * nt + n) * LS32 + 4 * q]);
+      acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, bv.x, acc[nt], 0, 0, 0);
+      acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, bv.y, acc[nt], 0, 0, 0);
+      acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.z, bv.z, acc[nt], 0, 0, 0);
+      acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.w, bv.w, acc[nt], 0, 0, 0);
+    }
+  }
+  epilogue<NT, EPI>(g, acc, m0, n0, wv, q, n);
+}
+
+template <int PREC>
+struct Half8;
+template <>
+struct Half8<1> {
+  typedef bf16x8 T;
+  typedef __bf16 E;
+  static __device__ __forceinline__ floatx4 mfma(T a, T b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Half8<2> {
+  typedef halfx8 T;
+  typedef _Float16 E;
+  static __device__ __forceinline__ floatx4 mfma(T a, T b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+  }
+};
+
+template <int NT, int EPI, int PREC>
+__global__ void __launch_bounds__(256, 2) gemm_h_kernel(const GemmArgs g) {
+  typedef typename Half8<PREC>::T HT;
+  typedef typename Half8<PREC>::E HE;
+  __shared__ __attribute__((aligned(16))) uint16_t As[64 * LS16];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[16 * NT * LS16];
+  constexpr int BN = 16 * NT;
+  constexpr int BV = BN * KC16 / 8;  // 16-byte pieces per B slice
+  constexpr int NB = (BV + 255) / 256;
+  static_assert(NB <= 4, "B staging unrolled for at most 4 pieces per thread");
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wv = tid >> 6;
+  const int q = lane >> 4, n = lane & 15;
+  const int m0 = blockIdx.x * 64;
+  const int n0 = blockIdx.y * BN;
+  const uint16_t* Bt = (const uint16_t*)g.Bt;
+  floatx4 acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const int arow = tid / 4, ac8 = tid % 4;  // 64 rows x 4 pieces of 8 elements
+  float4 ra0, ra1;
+  uint4 rb[NB];
+  {
+    const float* ap = g.A + (size_t)(m0 + arow) * g.lda + 8 * ac8;
+    ra0 = *reinterpret_cast<const float4*>(ap);
+    ra1 = *reinterpret_cast<const float4*>(ap + 4);
+  }
+if constexpr (0 < NB) { const int idx = tid + 256 * 0; if (0 + 1 < NB || idx < BV) rb[0] = *reinterpret_cast<const uint4*>(Bt + (size_t)(n0 + idx / 4) * g.ldb + 8 * (idx % 4)); }
+    if constexpr (1 < NB) { const int idx = tid + 256 * 1; if (1 + 1 < NB || idx < BV) rb[1] = *reinterpret_cast<const uint4*>(Bt + (size_t)(n0 + idx / 4) * g.ldb + 8 * (idx % 4)); }
+    if constexpr (2 < NB) { const int idx = tid + 256 * 2; if (2 + 1 < NB || idx < BV) rb[2] = *reinterpret_cast<const uint4*>(Bt + (size_t)(n0 + idx / 4) * g.ldb + 8 * (idx % 4)); }
+    if constexpr (3 < NB) { const int idx = tid + 256 * 3; if (3 + 1 < NB || idx < BV) rb[3] = *reinterpret_cast<const uint4*>(Bt + (size_t)(n0 + idx / 4) * g.ldb + 8 * (idx % 4)); }
+  for (int k0 = 0; k0 < g.K; k0 += KC16) {
+    __syncthreads();
+    {
+      HT h;
+      h[0] = (HE)ra0.x; h[1] = (HE)ra0.y; h[2] = (HE)ra0.z; h[3] = (HE)ra0.w;
+      h[4] = (HE)ra1.x; h[5] = (HE)ra1.y; h[6] = (HE)ra1.z; h[7] = (HE)ra1.w;
+      *reinterpret_cast<HT*>(&As[arow * LS16 + 8 * ac8]) = h;
+    }
+if constexpr (0 < NB) { const int idx = tid + 256 * 0; if (0 + 1 < NB || idx < BV) *reinterpret_cast<uint4*>(&Bs[(idx / 4) * LS16 + 8 * (idx % 4)]) = rb[0]; }
+    if constexpr (1 < NB) { const int idx = tid + 256 * 1; if (1 + 1 < NB || idx < BV) *reinterpret_cast<uint4*>(&Bs[(idx / 4) * LS16 + 8 * (idx % 4)]) = rb[1]; }
+    if constexpr (2 < NB) { const int idx = tid + 256 * 2; if (2 + 1 < NB || idx < BV) *reinterpret_cast<uint4*>(&Bs[(idx / 4) * LS16 + 8 * (idx % 4)]) = rb[2]; }
+    if constexpr (3 < NB) { const int idx = tid + 256 * 3; if (3 + 1 < NB || idx < BV) *reinterpret_cast<uint4*>(&Bs[(idx / 4) * LS16 + 8 * (idx % 4)]) = rb[3]; }
+    __syncthreads();
+    const int k1 = k0 + KC16;
+    if (k1 < g.K) {
+      const float* ap = g.A + (size_t)(m0 + arow) * g.lda + k1 + 8 * ac8;
+      ra0 = *reinterpret_cast<const float4*>(ap);
+      ra1 = *reinterpret_cast<const float4*>(ap + 4);
+if constexpr (0 < NB) { const int idx = tid + 256 * 0; if (0 + 1 < NB || idx < BV) rb[0] = *reinterpret_cast<const uint4*>(Bt + (size_t)(n0 + idx / 4) * g.ldb + k1 + 8 * (idx % 4)); }
+    if constexpr (1 < NB) { const int idx = tid + 256 * 1; if (1 + 1 < NB || idx < BV) rb[1] = *reinterpret_cast<const uint4*>(Bt + (size_t)(n0 + idx / 4) * g.ldb + k1 + 8 * (idx % 4)); }
+    if constexpr (2 < NB) { const int idx = tid + 256 * 2; if (2 + 1 < NB || idx < BV) rb[2] = *reinterpret_cast<const uint4*>(Bt + (size_t)(n0 + idx / 4) * g.ldb + k1 + 8 * (idx % 4)); }
+    if constexpr (3 < NB) { const int idx = tid + 256 * 3; if (3 + 1 < NB || idx < BV) rb[3] = *reinterpret_cast<const uint4*>(Bt + (size_t)(n0 + idx / 4) * g.ldb + k1 + 8 * (idx % 4)); }
+    }
+    const HT av = *reinterpret_cast<const HT*>(&As[(16 * wv + n) * LS16 + 8 * q]);
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+      const HT bv = *reinterpret_cast<const HT*>(&Bs[(16 * nt + n) * LS16 + 8 * q]);
+      acc[nt] = Half8<PREC>::mfma(av, bv, acc[nt]);
+    }
+  }
+  epilogue<NT, EPI>(g, acc, m0, n0, wv, q, n);
+}
+
+template <int NT, int EPI>
+static int32_t go(hipStream_t st, const GemmArgs& g, int prec) {
+  dim3 grid(g.M / 64, g.N / (16 * NT));
+  if (prec == DPMD_PREC_BF16) hipLaunchKernelGGL((gemm_h_kernel<NT, EPI, 1>), grid, dim3(256), 0, st, g);
+  else if (prec == DPMD_PREC_FP16) hipLaunchKernelGGL((gemm_h_kernel<NT, EPI, 2>), grid, dim3(256), 0, st, g);
+  else hipLaunchKernelGGL((gemm_f32_kernel<NT, EPI>), grid, dim3(256), 0, st, g);
+  DPMD_HIP(hipGetLastError());
+  return DPMD_OK;
+}
+
 template <int EPI>
-static int32_t gemm_dispatch(hipStream_t st, const GemmArgs& g) {
-  if (g.M % 64 || g.K % KC || g.N % 16) {
+static int32_t gemm_dispatch(hipStream_t st, const GemmArgs& g, int prec) {
+  const int kc = prec == DPMD_PREC_FP32 ? KC32 : KC16;
+  if (g.M % 64 || g.K % kc || g.N % 16) {
     set_error("gemm shape %d x %d x %d not tiled", g.M, g.N, g.K);
     return DPMD_E_STATE;
   }
   const int nt16 = g.N / 16;
-  // EPI_E3 needs the whole row in one block
-  auto go = [&](auto ntc) -> int32_t {
-    constexpr int NT = decltype(ntc)::value;
-    dim3 grid(g.M / 64, nt16 / NT);
-    hipLaunchKernelGGL((gemm_f32_kernel<NT, EPI>), grid, dim3(256), 0, st, g);
-    DPMD_HIP(hipGetLastError());
-    return DPMD_OK;
-  };
-  if (EPI == EPI_E3) {
-    if (nt16 == 15) return go(std::integral_constant<int, 15>());
-    if (nt16 == 8) return go(std::integral_constant<int, 8>());
-    if (nt16 == 16) return go(std::integral_constant<int, 16>());
+  if (EPI == EPI_E3) {  // the fused energy head needs the whole row in one block
+    if (nt16 == 15) return go<15, EPI>(st, g, prec);
+    if (nt16 == 8) return go<8, EPI>(st, g, prec);
+    if (nt16 == 16) return go<16, EPI>(st, g, prec);
     set_error("fitting width %d has no fused energy epilogue", g.N);
     return DPMD_E_UNSUPPORTED;
   }
-  if (nt16 % 15 == 0) return go(std::integral_constant<int, 15>());
-  if (nt16 % 10 == 0) return go(std::integral_constant<int, 10>());
-  if (nt16 % 8 == 0) return go(std::integral_constant<int, 8>());
-  if (nt16 % 12 == 0) return go(std::integral_constant<int, 12>());
-  if (nt16 % 16 == 0) return go(std::integral_constant<int, 16>());
+  if (nt16 % 15 == 0) return go<15, EPI>(st, g, prec);
+  if (nt16 % 10 == 0) return go<10, EPI>(st, g, prec);
+  if (nt16 % 8 == 0) return go<8, EPI>(st, g, prec);
+  if (nt16 % 12 == 0) return go<12, EPI>(st, g, prec);
+  if (nt16 % 16 == 0) return go<16, EPI>(st, g, prec);
   set_error("gemm width %d has no tile instantiation", g.N);
   return DPMD_E_UNSUPPORTED;
 }
 
-// Fitting-weight block offsets in Model::fit_off[t][*]
+// Fitting-weight block offsets in Model::fit_off[t][*] (elements)
 enum { FK0T = 0, FK1T, FK2T, FK0, FK1, FK2, FB0, FB1, FB2, FDT1, FDT2, FK3, FB3 };
 
 int32_t fit_launch(System* s, int precision) {
-  (void)precision;
   Model* M = s->model;
-  const int F = s->fpad;
+  const int FN = s->fpad;          // output width (multiple of 16)
+  const int FK = s->fstride;       // activation row stride = K of hidden GEMMs (multiple of 32)
   const int D = M->axis * M->wout;
   float* Y1 = s->d_fitbuf;
-  const size_t act = (size_t)s->mrows * F;
+  const size_t act = (size_t)s->mrows * FK;
   float* T2 = Y1 + act;
   float* Y2 = T2 + act;
   float* DZ3 = Y2 + act;
   float* DY2 = DZ3 + act;
   float* DZ2 = DY2 + act;
   float* DZ1 = DZ2 + act;
+  const float* Wf = M->d_fit;
+  const uint16_t* Wh = precision == DPMD_PREC_FP32 ? nullptr : M->d_fit_h[precision - 1];
+  if (precision != DPMD_PREC_FP32 && !Wh) {
+    set_error("16-bit fitting weights missing");
+    return DPMD_E_STATE;
+  }
   for (int t = 0; t < s->ntypes; ++t) {
     const int rows = s->rowbase[t + 1] - s->rowbase[t];
     if (rows == 0) continue;
     const size_t r0 = s->rowbase[t];
-    const float* W = M->d_fit;
     const size_t* o = M->fit_off[t];
+    auto B = [&](int blk) -> const void* {
+      return Wh ? (const void*)(Wh + o[blk]) : (const void*)(Wf + o[blk]);
+    };
     GemmArgs g;
     memset(&g, 0, sizeof(g));
     g.M = rows;
     // 1: y1 = tanh(G K0 + b0)
     g.A = (const float*)s->d_G + r0 * D; g.lda = D; g.K = D;
-    g.Bt = W + o[FK0T]; g.ldb = D; g.N = F;
-    g.C = Y1 + r0 * F; g.ldc = F; g.bias = W + o[FB0];
-    DPMD_TRY(gemm_dispatch<EPI_TANH>(s->stream, g));
+    g.Bt = B(FK0T); g.ldb = D; g.N = FN;
+    g.C = Y1 + r0 * FK; g.ldc = FK; g.bias = Wf + o[FB0];
+    DPMD_TRY(gemm_dispatch<EPI_TANH>(s->stream, g, precision));
     // 2: t2 = tanh(y1 K1 + b1), y2 = t2 dt1 + y1
-    g.A = Y1 + r0 * F; g.lda = F; g.K = F;
-    g.Bt = W + o[FK1T]; g.ldb = F;
-    g.C = Y2 + r0 * F; g.C2 = T2 + r0 * F; g.aux1 = Y1 + r0 * F;
-    g.bias = W + o[FB1]; g.dt = W + o[FDT1];
-    DPMD_TRY(gemm_dispatch<EPI_RES>(s->stream, g));
+    g.A = Y1 + r0 * FK; g.lda = FK; g.K = FK;
+    g.Bt = B(FK1T); g.ldb = FK;
+    g.C = Y2 + r0 * FK; g.C2 = T2 + r0 * FK; g.aux1 = Y1 + r0 * FK;
+    g.bias = Wf + o[FB1]; g.dt = Wf + o[FDT1];
+    DPMD_TRY(gemm_dispatch<EPI_RES>(s->stream, g, precision));
     // 3: t3, y3, e, dz3
-    g.A = Y2 + r0 * F;
-    g.Bt = W + o[FK2T];
-    g.C = DZ3 + r0 * F; g.C2 = nullptr; g.aux1 = Y2 + r0 * F;
-    g.bias = W + o[FB2]; g.dt = W + o[FDT2]; g.k3 = W + o[FK3];
+    g.A = Y2 + r0 * FK;
+    g.Bt = B(FK2T);
+    g.C = DZ3 + r0 * FK; g.C2 = nullptr; g.aux1 = Y2 + r0 * FK;
+    g.bias = Wf + o[FB2]; g.dt = Wf + o[FDT2]; g.k3 = Wf + o[FK3];
     g.b3 = M->fit_b3[t];
     g.eout = s->d_eatom + r0;
-    DPMD_TRY(gemm_dispatch<EPI_E3>(s->stream, g));
+    DPMD_TRY(gemm_dispatch<EPI_E3>(s->stream, g, precision));
     // 4: dy2 = k3 + dz3 K2^T, dz2 = dy2 dt1 (1 - t2^2)
-    g.A = DZ3 + r0 * F;
-    g.Bt = W + o[FK2];
-    g.C = DZ2 + r0 * F; g.C2 = DY2 + r0 * F; g.aux1 = T2 + r0 * F;
-    g.dt = W + o[FDT1];
-    DPMD_TRY(gemm_dispatch<EPI_BWD2>(s->stream, g));
+    g.A = DZ3 + r0 * FK;
+    g.Bt = B(FK2);
+    g.C = DZ2 + r0 * FK; g.C2 = DY2 + r0 * FK; g.aux1 = T2 + r0 * FK;
+    g.dt = Wf + o[FDT1];
+    DPMD_TRY(gemm_dispatch<EPI_BWD2>(s->stream, g, precision));
     // 5: dy1 = dy2 + dz2 K1^T, dz1 = dy1 (1 - y1^2)
-    g.A = DZ2 + r0 * F;
-    g.Bt = W + o[FK1];
-    g.C = DZ1 + r0 * F; g.C2 = nullptr; g.aux1 = DY2 + r0 * F; g.aux2 = Y1 + r0 * F;
-    DPMD_TRY(gemm_dispatch<EPI_BWD1>(s->stream, g));
+    g.A = DZ2 + r0 * FK;
+    g.Bt = B(FK1);
+    g.C = DZ1 + r0 * FK; g.C2 = nullptr; g.aux1 = DY2 + r0 * FK; g.aux2 = Y1 + r0 * FK;
+    DPMD_TRY(gemm_dispatch<EPI_BWD1>(s->stream, g, precision));
     // 6: dG = dz1 K0^T
-    g.A = DZ1 + r0 * F;
-    g.Bt = W + o[FK0]; g.ldb = F; g.K = F;
+    g.A = DZ1 + r0 * FK;
+    g.Bt = B(FK0); g.ldb = FK; g.K = FK;
     g.N = D;
     g.C = s->d_dG + r0 * D; g.ldc = D;
-    DPMD_TRY(gemm_dispatch<EPI_STORE>(s->stream, g));
+    DPMD_TRY(gemm_dispatch<EPI_STORE>(s->stream, g, precision));
   }
   return DPMD_OK;
 }

@@ -79,6 +79,7 @@ struct Model {
   size_t fit_off[kMaxT][16];  // see api.hip
   size_t fit_total = 0;
   float fit_b3[kMaxT] = {0, 0, 0, 0};
+  int fit_n = 0, fit_k = 0;  // output width (x16) and K stride (x32) of the hidden GEMMs
   uint16_t* d_fit_h[2] = {nullptr, nullptr};  // half-precision copies of kernels
 };
 
@@ -139,7 +140,8 @@ struct System {
   double* d_red = nullptr;    // reduction outputs
   unsigned long long* d_pairs = nullptr;
   double last_pairs = 0;
-  int fpad = 0;
+  int fpad = 0;       // fitting output width
+  int fstride = 0;    // fitting activation row stride
   // integrator
   double* d_thermo = nullptr;  // per replica thermostat scalars
   double* d_traj = nullptr;    // device trajectory staging (chunk)

@@ -316,17 +316,12 @@ int32_t md_ke_launch(System* s, double* d_out) {
 int32_t md_langevin_o(System* s, double dt, double gamma, double kT, uint64_t seed, uint64_t counter) {
   const double c1 = exp(-gamma * dt);
   const double c2 = sqrt(std::max(kT * (1.0 - c1 * c1), 0.0));
-  if (s->natoms % 256 != 0 && s->nrep > 1) {
-    // blocks must not straddle replicas: launch per replica
-    for (int r = 0; r < s->nrep; ++r) {
-      hipLaunchKernelGGL(langevin_o_kernel, dim3((s->natoms + 255) / 256), dim3(256), 0, s->stream,
-                         s->d_mom + (size_t)r * s->natoms * 3, s->d_mass, s->natoms, s->natoms, c1, c2, seed,
-                         counter * 4096 + r, s->d_thermo + r);
-    }
-  } else {
-    const int RN = s->nrep * s->natoms;
-    hipLaunchKernelGGL(langevin_o_kernel, dim3((RN + 255) / 256), dim3(256), 0, s->stream, s->d_mom, s->d_mass,
-                       s->natoms, RN, c1, c2, seed, counter, s->d_thermo);
+  // one launch per replica so that a block never straddles two replicas; the work
+  // accumulator of replica r is thermo[64 r]
+  for (int r = 0; r < s->nrep; ++r) {
+    hipLaunchKernelGGL(langevin_o_kernel, dim3((s->natoms + 255) / 256), dim3(256), 0, s->stream,
+                       s->d_mom + (size_t)r * s->natoms * 3, s->d_mass, s->natoms, s->natoms, c1, c2, seed,
+                       counter * 65536 + (uint64_t)r, s->d_thermo + 64 * r);
   }
   DPMD_HIP(hipGetLastError());
   return DPMD_OK;

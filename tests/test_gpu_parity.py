@@ -128,3 +128,58 @@ def test_overflow_is_reported_and_state_restored(device_model, model0):
     _, _, err = s.md_run(p, 3, record=False)
     assert err & 2
     np.testing.assert_array_equal(s.get_positions(), x0)
+
+
+@pytest.mark.parametrize('prec', ['bf16', 'fp16'])
+def test_half_precision_forms(device_model, model0, prec):
+    """BF16 / FP16 embedding and fitting GEMMs (FP32 accumulation; FP32 switch, environment,
+    contraction and descriptor): energy/atom and force RMSE against the FP64 oracle within
+    1e-3 eV/atom and 1e-3 eV/A, virial within 1e-2 relative."""
+    from deepmd_jax import synthetic as syn
+    print('case prec', prec, flush=True)
+    params, var = model0
+    coord, box, types = syn.water_box(216, seed=11)
+    E, F, W = O.energy_force_virial(params, var, coord, box, types, lattice='auto')
+    E, F, W = E.item(), F.numpy(), W.numpy()
+    s = _system(device_model, coord, box, types, params['rcut'])
+    e, f, v, _ = s.compute(prec)
+    N = len(types)
+    rmse = np.sqrt(((f[0] - F) ** 2).mean())
+    print(prec, 'dE/atom', abs(e[0] - E) / N, 'F rmse', rmse, 'W maxrel', np.abs(v[0] - W).max() / np.abs(W).max())
+    assert abs(e[0] - E) / N <= 1e-3
+    assert rmse <= 1e-3
+    assert np.abs(v[0] - W).max() <= 1e-2 * np.abs(W).max()
+    # the FP32 path of the same system is far tighter
+    e32, f32, _, _ = s.compute('fp32')
+    assert np.sqrt(((f32[0] - F) ** 2).mean()) < 0.1 * max(rmse, 1e-9) or np.sqrt(((f32[0] - F) ** 2).mean()) < 1e-6
+
+
+def test_nvt_langevin_and_nose_hoover_run(device_model, model0):
+    """Thermostatted runs stay finite; Nose-Hoover invariant is conserved to 1e-3 eV/atom over 100 steps."""
+    from deepmd_jax import _lib, synthetic as syn
+    from deepmd_jax.constants import MASS_UNIT_CONVERSION, TEMP_UNIT_CONVERSION
+    print('case nvt', flush=True)
+    coord, box, types = syn.water_box(216, seed=12)
+    vel = syn.maxwell_boltzmann(types, 300.0, seed=2)
+    m = np.array(syn.MASSES)[types] * MASS_UNIT_CONVERSION
+    for routine in (1, 2):
+        s = _lib.DeviceSystem(device_model, types, 1)
+        s.set_box(box)
+        s.set_masses(m)
+        s.set_positions(coord)
+        s.set_velocities(vel)
+        s.nlist_allocate(7.0, 1.2)
+        p = _lib.MDParams()
+        p.routine, p.dt, p.kT, p.gamma, p.tau, p.chain_length, p.seed = routine, 0.5, 300 * TEMP_UNIT_CONVERSION, 0.01, 50.0, 3, 7
+        e0, _, _, _ = s.compute()
+        inv0 = e0[0] + s.kinetic_energy()[0]
+        tp, tv, err = s.md_run(p, 100)
+        assert err == 0 and np.isfinite(tp).all()
+        e1, _, _, _ = s.compute()
+        th = s.thermostat_energy()[0]
+        if routine == 1:
+            inv1 = e1[0] + s.kinetic_energy()[0] + th
+            assert abs(inv1 - inv0) / len(types) < 1e-3, (inv0, inv1)
+        else:
+            inv1 = e1[0] + s.kinetic_energy()[0] - th
+            assert abs(inv1 - inv0) / len(types) < 2e-3, (inv0, inv1)
