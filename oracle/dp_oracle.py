@@ -283,3 +283,62 @@ def typed_nlist(coord, box, type_idx, rc_skin, ntypes=2, knbr=None, buffer_ratio
         full.append(arr)
     nbrs_nm = [[full[j][off[i]:off[i + 1]] for j in range(ntypes)] for i in range(ntypes)]
     return nbrs_nm, overflow, knbr
+
+
+# ---------------------------------------------------------------------------------
+# Integrators (FP64, NumPy): deepmd_jax/pimd.py:22-44 (normal modes), :114-207 (BAOAB
+# ring-polymer step with exact free-ring propagation and PILE-L O-step), :210-227
+# (unwrap across beads); jax_md velocity Verlet (deepmd_jax/routine.py:139-144).
+
+def vv_step(x, p, m, force_fn, dt, F):
+    p = p + 0.5 * dt * F
+    x = x + dt * p / m[..., None]
+    F = force_fn(x)
+    p = p + 0.5 * dt * F
+    return x, p, F
+
+
+def ring_modes(n_bead, kT, hbar=6.582119569e-1):
+    A = 2 * np.eye(n_bead) - np.roll(np.eye(n_bead), 1, axis=0) - np.roll(np.eye(n_bead), -1, axis=0)
+    eva, eve = np.linalg.eigh(A)
+    o = np.argsort(eva)
+    eva, eve = eva[o], eve[:, o]
+    eva[0] = 0.0
+    return n_bead * kT / hbar * np.sqrt(np.maximum(eva, 0.0)), eve
+
+
+def unwrap_beads(x, box):
+    d = x - x[0:1]
+    d = d - box * np.round(d / box)
+    return x[0:1] + d
+
+
+def free_ring(x, p, m, C, freqs, h, box):
+    """Exact propagation of the free ring polymer for time h (deepmd_jax/pimd.py:160-179)."""
+    x = unwrap_beads(x, box)
+    q = np.tensordot(C.T, x, axes=(1, 0))
+    pn = np.tensordot(C.T, p, axes=(1, 0))
+    mm = m[None, :, None]
+    w = freqs[:, None, None]
+    qn = q.copy()
+    qn[0] = q[0] + h * pn[0] / m[:, None]
+    cs, sn = np.cos(w[1:] * h), np.sin(w[1:] * h)
+    qn[1:] = q[1:] * cs + pn[1:] * sn / (mm * w[1:])
+    pnn = pn.copy()
+    pnn[1:] = pn[1:] * cs - mm * w[1:] * q[1:] * sn
+    return np.tensordot(C, qn, axes=(1, 0)), np.tensordot(C, pnn, axes=(1, 0))
+
+
+def pimd_step(x, p, m, force_fn, dt, C, freqs, c1, c2, F, box, noise=None):
+    """One BAOAB ring-polymer step; noise (P,N,3) standard normals or None (c2 ignored)."""
+    p = p + 0.5 * dt * F
+    x, p = free_ring(x, p, m, C, freqs, 0.5 * dt, box)
+    pn = np.tensordot(C.T, p, axes=(1, 0))
+    pn = c1[:, None, None] * pn
+    if noise is not None:
+        pn = pn + c2[:, None, None] * np.sqrt(m)[None, :, None] * noise
+    p = np.tensordot(C, pn, axes=(1, 0))
+    x, p = free_ring(x, p, m, C, freqs, 0.5 * dt, box)
+    F = force_fn(x)
+    p = p + 0.5 * dt * F
+    return x, p, F

@@ -210,3 +210,56 @@ def test_golden_fixture(model0):
     np.testing.assert_allclose(W.numpy(), g['virial'], atol=1e-9)
     np.testing.assert_allclose(params['sr_mean'], g['sr_mean'], rtol=1e-14)
     np.testing.assert_allclose(params['sr_std'], g['sr_std'], rtol=1e-14)
+
+
+def test_ring_modes_closed_form():
+    """Normal-mode frequencies 2 P kT/hbar sin(pi k / P) (as multiset) and C orthogonal."""
+    P, kT = 8, 0.025
+    f, C = O.ring_modes(P, kT)
+    hbar = 6.582119569e-1
+    expect = np.sort(2 * P * kT / hbar * np.sin(np.pi * np.arange(P) / P))
+    np.testing.assert_allclose(np.sort(f), expect, atol=1e-12)
+    np.testing.assert_allclose(C.T @ C, np.eye(P), atol=1e-12)
+    assert f[0] == 0
+
+
+def test_free_ring_matches_harmonic_solution():
+    """With no physical force the free-ring step is the exact harmonic ring solution:
+    energy 0.5 p^2/m + 0.5 m w_P^2 sum (x_b - x_{b-1})^2 is conserved exactly and
+    two half steps equal one full step."""
+    P, kT = 6, 0.03
+    f, C = O.ring_modes(P, kT)
+    rng = np.random.default_rng(0)
+    m = np.array([1.5, 16.0])
+    x = rng.standard_normal((P, 2, 3)) * 0.1 + 5.0
+    p = rng.standard_normal((P, 2, 3)) * 0.05
+    box = np.full(3, 50.0)
+    wP = P * kT / 6.582119569e-1
+
+    def H(x, p):
+        d = x - np.roll(x, 1, axis=0)
+        return 0.5 * (p ** 2 / m[None, :, None]).sum() + 0.5 * wP ** 2 * (m[None, :, None] * d ** 2).sum()
+    x1, p1 = O.free_ring(x, p, m, C, f, 3.0, box)
+    assert abs(H(x1, p1) - H(x, p)) < 1e-10 * H(x, p)
+    xa, pa = O.free_ring(*O.free_ring(x, p, m, C, f, 1.5, box), m, C, f, 1.5, box)
+    np.testing.assert_allclose(xa, x1, atol=1e-12)
+    np.testing.assert_allclose(pa, p1, atol=1e-12)
+
+
+def test_velocity_verlet_harmonic_oscillator():
+    """VV on a harmonic oscillator: energy error stays bounded (1%) over 1000 steps and
+    integrating back with -dt returns to the start."""
+    k, m, dt = 2.0, np.array([3.0]), 0.05
+    ff = lambda x: -k * x
+    x, p = np.array([[1.0, 0.0, 0.0]]), np.array([[0.0, 0.5, 0.0]])
+    F = ff(x)
+    xs, ps = x.copy(), p.copy()
+    for _ in range(1000):
+        x, p, F = O.vv_step(x, p, m, ff, dt, F)
+    E0 = 0.5 * (ps ** 2).sum() / m[0] + 0.5 * k * (xs ** 2).sum()
+    E1 = 0.5 * (p ** 2).sum() / m[0] + 0.5 * k * (x ** 2).sum()
+    assert abs(E1 - E0) < 1e-2 * E0
+    for _ in range(1000):
+        x, p, F = O.vv_step(x, p, m, ff, -dt, F)
+    np.testing.assert_allclose(x, xs, atol=1e-9)
+    np.testing.assert_allclose(p, ps, atol=1e-9)
