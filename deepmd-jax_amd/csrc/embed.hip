@@ -366,6 +366,17 @@ __device__ __forceinline__ void for_groups(F&& f) {
 
 constexpr int kGroup = 2;  // layer-3 tiles processed together
 
+// Per-net first-layer weights and the three bias vectors (physical slot order) of all
+// ntypes^2 nets, copied once per block into LDS: [net][k0 | b0 | b1 | b2].
+template <int T1, int T2, int T3>
+__device__ __forceinline__ float* load_small_tables(const EmbedArgs& a, float* smem) {
+  constexpr int SM = 16 * (2 * T1 + T2 + T3);
+  const int nn = a.ntypes * a.ntypes;
+  for (int i = threadIdx.x; i < nn * SM; i += blockDim.x) smem[i] = a.net[(size_t)(i / SM) * a.net_stride + i % SM];
+  __syncthreads();
+  return smem + (nn * SM + 3) / 4 * 4;
+}
+
 // Compact the skin row of neighbour type j to the pairs within rcut, in ascending
 // index order, into the wave's LDS list (float4: x, y, z, global index bits).
 __device__ __forceinline__ int compact_row(const EmbedArgs& a, int rep, int c, int j, float4 pc,
@@ -411,7 +422,8 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
   constexpr int H_W1A = 0, H_W2A = H_W1A + T2 * S21 * 1024;
   (void)S23;
   const int q = lane >> 4, n = lane & 15;
-  float* wbase = smem + wv * a.lds_wave_floats;
+  float* small = smem;
+  float* wbase = load_small_tables<T1, T2, T3>(a, smem) + wv * a.lds_wave_floats;
   float4* list = reinterpret_cast<float4*>(wbase);
   float* Rl = wbase + 4 * a.list_cap;
   float* Hl = Rl + 256;  // one tile group of h3: [16 neighbours][HG]
@@ -444,10 +456,11 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
     for (int ch = 0; ch < m; ch += 16) {
       const int nb = netb + opaque_zero();
       const int hb = hnetb + opaque_zero();
-      auto k0 = [&](int r) { return bld(wrs, vq16, nb + 4 * (O_K0 + 16 * (r / 4) + (r % 4))); };
-      auto b0 = [&](int r) { return bld(wrs, vq16, nb + 4 * (O_B0 + 16 * (r / 4) + (r % 4))); };
-      auto b1 = [&](int r) { return bld(wrs, vq16, nb + 4 * (O_B1 + 16 * (r / 4) + (r % 4))); };
-      auto b2 = [&](int r) { return bld(wrs, vq16, nb + 4 * (O_B2 + 16 * (r / 4) + (r % 4))); };
+      const float* sm = small + (ti * a.ntypes + j) * (16 * (2 * T1 + T2 + T3)) + 4 * q + (nb - netb);
+      auto k0 = [&](int r) { return sm[O_K0 + 16 * (r / 4) + (r % 4)]; };
+      auto b0 = [&](int r) { return sm[O_B0 + 16 * (r / 4) + (r % 4)]; };
+      auto b1 = [&](int r) { return sm[O_B1 + 16 * (r / 4) + (r % 4)]; };
+      auto b2 = [&](int r) { return sm[O_B2 + 16 * (r / 4) + (r % 4)]; };
       const bool valid = ch + n < m;
       const float4 e = valid ? list[ch + n] : make_float4(0.f, 0.f, 0.f, 0.f);
       Geo g;
@@ -568,7 +581,8 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
   const int q = lane >> 4, n = lane & 15;
   constexpr int HS = 16 * T3 + 1;
   const int W = a.W, A = a.A;
-  float* wbase = smem + wv * a.lds_wave_floats;
+  float* small = smem;
+  float* wbase = load_small_tables<T1, T2, T3>(a, smem) + wv * a.lds_wave_floats;
   // chunk-phase layout
   float* dTp = wbase;                    // 16 * HS
   float* Rl = dTp + 16 * HS;             // 256
@@ -587,19 +601,93 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
     int ti = 0;
     while (ti + 1 < a.ntypes && c >= a.off[ti + 1]) ++ti;
     const float4 pc = a.pos[gc];
-    // ---- prologue: dE/dT (logical [x][w], from dT_kernel) into the physical slot layout
+    const int grow = a.rowbase[ti] + rep * a.count[ti] + (c - a.off[ti]);
+    // ---- prologue: dE/dT = (dG/dT)^T dE/dG / Nnbrs into the physical slot layout dTp.
+    // G[a][w] = TW[w]TW[a] + sum_c T3[c][w]T3[c][a] + sum_k G2[k][a]T6[k][w] (dpmodel.py:74-78)
+    //   V[x][w] = sum_a dG[a][w] X[x][a],  X = (TW, T3 rows)[:, :A] for x<4, G2 for x>=4
+    //   U[x][a] = sum_w dG[a][w] Tx[x][w]   (row 0 of Tx is TW)
+    // plus the U terms of the factors T[:, :A] and of G2's arguments T[:, A:2A].
     {
-      const float* dTg = a.T + (size_t)gc * 10 * W;
+      const float* Tg = a.T + (size_t)gc * 10 * W;
+      const float* dGg = a.dG + (size_t)grow * A * W;
+      float* Tl = dTp + 16 * HS;   // 10*W   (aliases Rl, dRl, list)
+      float* Xl = Tl + 10 * W;     // 10*A
+      float* Ul = Xl + 10 * A;     // 10*A
+      for (int e = lane; e < 10 * W; e += 64) Tl[e] = Tg[e] + (e < W ? a.tbias[e] : 0.f);
       for (int e = lane; e < 16 * HS; e += 64) dTp[e] = 0.f;
       wave_sync();
-      for (int e = lane; e < 10 * W; e += 64) {
-        const int x = e / W, w = e % W;
+      const float r2 = 1.4142135623730951f;
+      for (int e = lane; e < 10 * A; e += 64) {
+        const int x = e / A, aa = e % A;
+        float v;
+        if (x < 4) {
+          v = Tl[x * W + aa];
+        } else if (x < 7) {
+          const int k = x - 4;
+          const float t3 = Tl[(1 + k) * W + A + aa];
+          v = t3 * t3 + Tl[x * W + A + aa];
+        } else {
+          const int kp = x - 7;
+          v = r2 * Tl[(1 + kp) * W + A + aa] * Tl[(1 + (kp + 2) % 3) * W + A + aa] + Tl[x * W + A + aa];
+        }
+        Xl[e] = v;
+      }
+      {
+        // U: lane (a = lane%16, wq = lane/16) sums w = wq mod 4, then the 4 lanes of a combine
+        const int aa = lane & 15, wq = lane >> 4;
+        float u[10];
+#pragma unroll
+        for (int x = 0; x < 10; ++x) u[x] = 0.f;
+        if (aa < A) {
+          for (int w = wq; w < W; w += 4) {
+            const float d = dGg[aa * W + w];
+#pragma unroll
+            for (int x = 0; x < 10; ++x) u[x] += d * Tl[x * W + w];
+          }
+        }
+#pragma unroll
+        for (int x = 0; x < 10; ++x) {
+          u[x] += __shfl_xor(u[x], 16);
+          u[x] += __shfl_xor(u[x], 32);
+        }
+        if (wq == 0 && aa < A) {
+#pragma unroll
+          for (int x = 0; x < 10; ++x) Ul[x * A + aa] = u[x];
+        }
+      }
+      wave_sync();
+      for (int w = lane; w < W; w += 64) {
+        float v[10];
+#pragma unroll
+        for (int x = 0; x < 10; ++x) v[x] = 0.f;
+        for (int aa = 0; aa < A; ++aa) {
+          const float d = dGg[aa * W + w];
+#pragma unroll
+          for (int x = 0; x < 10; ++x) v[x] += d * Xl[x * A + aa];
+        }
+        if (w < A) {
+#pragma unroll
+          for (int x = 0; x < 4; ++x) v[x] += Ul[x * A + w];
+        } else if (w < 2 * A) {
+          const int aa = w - A;
+#pragma unroll
+          for (int x = 4; x < 10; ++x) v[x] += Ul[x * A + aa];
+#pragma unroll
+          for (int cc = 0; cc < 3; ++cc) {
+            v[1 + cc] += 2.f * Tl[(1 + cc) * W + w] * Ul[(4 + cc) * A + aa] +
+                         r2 * Tl[(1 + (cc + 2) % 3) * W + w] * Ul[(7 + cc) * A + aa] +
+                         r2 * Tl[(1 + (cc + 1) % 3) * W + w] * Ul[(7 + (cc + 1) % 3) * A + aa];
+          }
+        }
         int slot = -1;
         for (int qq = 0; qq < 4; ++qq) {
           const int rr = w - a.base3[qq];
           if (rr >= 0 && rr < a.size3[qq]) slot = 16 * (rr / 4) + 4 * qq + (rr % 4);
         }
-        if (slot >= 0) dTp[x * HS + slot] = dTg[e];
+        if (slot >= 0) {
+#pragma unroll
+          for (int x = 0; x < 10; ++x) dTp[x * HS + slot] = v[x] * a.inv_nnbrs;
+        }
       }
       wave_sync();
     }
@@ -618,10 +706,11 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
       for (int ch = 0; ch < m; ch += 16) {
         const int nb = netb + opaque_zero();
         const int hb = hnetb + opaque_zero();
-        auto k0 = [&](int r) { return bld(wrs, vq16, nb + 4 * (O_K0 + 16 * (r / 4) + (r % 4))); };
-        auto b0 = [&](int r) { return bld(wrs, vq16, nb + 4 * (O_B0 + 16 * (r / 4) + (r % 4))); };
-        auto b1 = [&](int r) { return bld(wrs, vq16, nb + 4 * (O_B1 + 16 * (r / 4) + (r % 4))); };
-        auto b2 = [&](int r) { return bld(wrs, vq16, nb + 4 * (O_B2 + 16 * (r / 4) + (r % 4))); };
+        const float* sm = small + (ti * a.ntypes + j) * (16 * (2 * T1 + T2 + T3)) + 4 * q + (nb - netb);
+        auto k0 = [&](int r) { return sm[O_K0 + 16 * (r / 4) + (r % 4)]; };
+        auto b0 = [&](int r) { return sm[O_B0 + 16 * (r / 4) + (r % 4)]; };
+        auto b1 = [&](int r) { return sm[O_B1 + 16 * (r / 4) + (r % 4)]; };
+        auto b2 = [&](int r) { return sm[O_B2 + 16 * (r / 4) + (r % 4)]; };
         const bool valid = ch + n < m;
         const float4 e = valid ? list[ch + n] : make_float4(0.f, 0.f, 0.f, 0.f);
         Geo g;
@@ -794,77 +883,6 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
 }
 
 // ---------------------------------------------------------------------------------
-// dE/dT from dE/dG (descriptor backward, deepmd_jax/dpmodel.py:74-78), scaled by
-// 1/Nnbrs and written in place over T (logical [x][w] layout).  One wave per centre.
-//   G[a][w] = TW[w] TW[a] + sum_c T3[c][w] T3[c][a] + sum_k G2[k][a] T6[k][w]
-//   G2[k][a] = T3[k][A+a]^2 + T6[k][A+a] (k<3);  sqrt2 T3[k'][A+a] T3[k'-1][A+a] + T6[3+k'][A+a]
-
-__global__ void __launch_bounds__(256) dT_kernel(const EmbedArgs a) {
-  extern __shared__ float smem[];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int W = a.W, A = a.A;
-  float* Tl = smem + wv * a.lds_wave_floats;  // 10*W
-  float* dGl = Tl + 10 * W;                    // A*W
-  float* Ul = dGl + A * W;                     // 10*A
-  const int rep = blockIdx.x / a.blocks_per_rep;
-  const int c = (blockIdx.x % a.blocks_per_rep) * 4 + wv;
-  if (c >= a.natoms) return;
-  const int gc = rep * a.natoms + c;
-  int ti = 0;
-  while (ti + 1 < a.ntypes && c >= a.off[ti + 1]) ++ti;
-  const int grow = a.rowbase[ti] + rep * a.count[ti] + (c - a.off[ti]);
-  float* Tg = a.T + (size_t)gc * 10 * W;
-  const float* dGg = a.dG + (size_t)grow * A * W;
-  for (int e = lane; e < 10 * W; e += 64) Tl[e] = Tg[e] + (e < W ? a.tbias[e] : 0.f);  // row 0 becomes TW
-  for (int e = lane; e < A * W; e += 64) dGl[e] = dGg[e];
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  for (int e = lane; e < 10 * A; e += 64) {
-    const int x = e / A, aa = e % A;
-    float s = 0.f;
-    for (int w = 0; w < W; ++w) s += dGl[aa * W + w] * Tl[x * W + w];
-    Ul[x * A + aa] = s;
-  }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  const float r2 = 1.4142135623730951f;
-  for (int e = lane; e < 10 * W; e += 64) {
-    const int x = e / W, w = e % W;
-    float v = 0.f;
-    if (x < 4) {
-      for (int aa = 0; aa < A; ++aa) v += dGl[aa * W + w] * Tl[x * W + aa];
-      if (w < A) v += Ul[x * A + w];
-    } else {
-      const int k = x - 4;
-      for (int aa = 0; aa < A; ++aa) {
-        float g2;
-        if (k < 3) {
-          const float t3 = Tl[(1 + k) * W + A + aa];
-          g2 = t3 * t3 + Tl[(4 + k) * W + A + aa];
-        } else {
-          const int kp = k - 3;
-          g2 = r2 * Tl[(1 + kp) * W + A + aa] * Tl[(1 + (kp + 2) % 3) * W + A + aa] + Tl[(4 + k) * W + A + aa];
-        }
-        v += dGl[aa * W + w] * g2;
-      }
-    }
-    if (w >= A && w < 2 * A) {
-      const int aa = w - A;
-      if (x >= 4) v += Ul[x * A + aa];
-      if (x >= 1 && x <= 3) {
-        const int cc = x - 1;
-        v += 2.f * Tl[x * W + w] * Ul[(4 + cc) * A + aa];
-        v += r2 * Tl[(1 + (cc + 2) % 3) * W + w] * Ul[(7 + cc) * A + aa];
-        v += r2 * Tl[(1 + (cc + 1) % 3) * W + w] * Ul[(7 + (cc + 1) % 3) * A + aa];
-      }
-    }
-    Tg[e] = v * a.inv_nnbrs;
-  }
-}
-
-// ---------------------------------------------------------------------------------
 // Launchers
 
 static void fill_args(System* s, EmbedArgs& a) {
@@ -931,15 +949,12 @@ static int32_t launch_embed(System* s, bool backward) {
     wave_floats = 4 * a.list_cap + 256 + 16 * (16 * kGroup + 1);
     wave_floats = std::max(wave_floats, (size_t)(10 * W));
   } else {
-    wave_floats = 16 * HS + 512 + 4 * a.list_cap;
-    // dE/dG -> dE/dT first, in place over T
-    EmbedArgs b = a;
-    b.lds_wave_floats = (10 * W + A * W + 10 * A + 3) / 4 * 4;
-    hipLaunchKernelGGL(dT_kernel, dim3(nblocks), dim3(256), 4 * b.lds_wave_floats * sizeof(float), s->stream, b);
+    wave_floats = 16 * HS + std::max((size_t)(512 + 4 * a.list_cap), (size_t)(10 * W + 20 * A));
   }
   wave_floats = (wave_floats + 3) / 4 * 4;
   a.lds_wave_floats = (int)wave_floats;
-  const size_t lds = 4 * wave_floats * sizeof(float);
+  const size_t small = (size_t)(s->ntypes * s->ntypes * 16 * (2 * T1 + T2 + T3) + 3) / 4 * 4;
+  const size_t lds = (4 * wave_floats + small) * sizeof(float);
   if (lds > 160 * 1024) {
     set_error("embedding kernel needs %zu bytes of LDS (neighbour capacity too large)", lds);
     return DPMD_E_UNSUPPORTED;

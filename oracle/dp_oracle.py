@@ -342,3 +342,40 @@ def pimd_step(x, p, m, force_fn, dt, C, freqs, c1, c2, F, box, noise=None):
     F = force_fn(x)
     p = p + 0.5 * dt * F
     return x, p, F
+
+
+def local_fd_forces(params, variables, coord, box, type_idx, atoms, h=1e-4):
+    """FP64 central-difference forces on selected atoms of a large periodic box.
+
+    Moving atom i changes only the energies of atoms within rcut of i, whose
+    environments lie within 2*rcut of i; each derivative is taken on that local
+    cluster (minimum-image coordinates around i, non-periodic big box)."""
+    coord = np.asarray(coord, np.float64)
+    box = np.asarray(box, np.float64).reshape(3)
+    type_idx = np.asarray(type_idx)
+    rc = params['rcut']
+    out = np.zeros((len(atoms), 3))
+    for n, i in enumerate(atoms):
+        d = coord - coord[i]
+        d = d - box * np.round(d / box)
+        r = np.linalg.norm(d, axis=1)
+        clus = np.nonzero(r < 2 * rc + 0.5)[0]
+        inner = set(np.nonzero(r[clus] < rc + 0.5)[0].tolist())
+        cc = d[clus] + 50.0
+        tt = type_idx[clus]
+        big = np.full(3, 100.0 + 6 * rc)
+        ii = int(np.nonzero(clus == i)[0][0])
+        order = np.argsort(tt, kind='stable')
+        pos_in_sorted = np.empty(len(order), int)
+        pos_in_sorted[order] = np.arange(len(order))
+        sel = torch.as_tensor(sorted(pos_in_sorted[list(inner)]))
+        lat = lattice_candidate(np.diag(big), rc)
+        for dim in range(3):
+            es = []
+            for sgn in (1, -1):
+                c2 = cc.copy()
+                c2[ii, dim] += sgn * h
+                _, e, _, _ = dp_energy(params, variables, torch.as_tensor(c2), big, tt, lattice=lat, return_T=True)
+                es.append(e[sel].sum().item())
+            out[n, dim] = -(es[0] - es[1]) / (2 * h)
+    return out

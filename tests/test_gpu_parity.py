@@ -223,3 +223,21 @@ def test_pimd_step_against_oracle(device_model, model0):
         xo, po, F = O.pimd_step(xo, po, m, forces, dt, C, freqs, np.ones(P), np.zeros(P), F, box)
     assert np.abs(tp[-1] - xo).max() < 1e-5
     assert np.abs(tv[-1] - po / m[None, :, None]).max() < 1e-5
+
+
+def test_default_topology_model():
+    """The reference's default widths (embed 32/32/64, fit 128x3, axis 12) run on the
+    second kernel instantiation; FP32 parity with the FP64 oracle."""
+    from deepmd_jax import _lib, synthetic as syn
+    print('case default topology', flush=True)
+    params, var = syn.init_model(1, embed_widths=(32, 32, 64), fit_widths=(128, 128, 128), axis=12)
+    coord, box, types = syn.water_box(216, seed=14)
+    E, F, W = O.energy_force_virial(params, var, coord, box, types, lattice='auto')
+    E, F, W = E.item(), F.numpy(), W.numpy()
+    dm = _lib.DeviceModel(params, var)
+    s = _system(dm, coord, box, types, params['rcut'])
+    for prec, tol_e, tol_f in (('fp32', 1e-5, 1e-5), ('bf16', 1e-3, 1e-3)):
+        e, f, v, _ = s.compute(prec)
+        N = len(types)
+        assert abs(e[0] - E) / N <= tol_e, (prec, e[0], E)
+        assert np.sqrt(((f[0] - F) ** 2).mean()) <= tol_f * max(1.0, np.sqrt((F ** 2).mean())), prec

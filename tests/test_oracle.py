@@ -263,3 +263,15 @@ def test_velocity_verlet_harmonic_oscillator():
         x, p, F = O.vv_step(x, p, m, ff, -dt, F)
     np.testing.assert_allclose(x, xs, atol=1e-9)
     np.testing.assert_allclose(p, ps, atol=1e-9)
+
+
+def test_local_fd_matches_autograd(model0):
+    """The local-cluster central differences reproduce periodic autograd forces (box
+    larger than 4 rcut + 1, so a 2 rcut + 0.5 cluster around an atom has no image ambiguity)."""
+    params, var = model0
+    coord, box, types = _small(1000, seed=13)
+    nb, _, _ = O.typed_nlist(coord, box, types, params['rcut'])
+    _, F, _ = O.energy_force_virial(params, var, coord, box, types, nbrs_nm=nb)
+    atoms = [0, 5, 1001]
+    fd = O.local_fd_forces(params, var, coord, box, types, atoms)
+    np.testing.assert_allclose(fd, F.numpy()[atoms], atol=2e-6)
