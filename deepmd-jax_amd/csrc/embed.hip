@@ -584,8 +584,8 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
   float* small = smem;
   float* wbase = load_small_tables<T1, T2, T3>(a, smem) + wv * a.lds_wave_floats;
   // chunk-phase layout
-  float* dTp = wbase;                    // 16 * HS
-  float* Rl = dTp + 16 * HS;             // 256
+  float* dTp = wbase;                    // 10 * HS (rows x >= 10 of the MFMA operands are zero)
+  float* Rl = dTp + 10 * HS;             // 256
   float* dRl = Rl + 256;                 // 256
   float4* list = reinterpret_cast<float4*>(dRl + 256);
 
@@ -610,11 +610,11 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
     {
       const float* Tg = a.T + (size_t)gc * 10 * W;
       const float* dGg = a.dG + (size_t)grow * A * W;
-      float* Tl = dTp + 16 * HS;   // 10*W   (aliases Rl, dRl, list)
+      float* Tl = dTp + 10 * HS;   // 10*W   (aliases Rl, dRl, list)
       float* Xl = Tl + 10 * W;     // 10*A
       float* Ul = Xl + 10 * A;     // 10*A
       for (int e = lane; e < 10 * W; e += 64) Tl[e] = Tg[e] + (e < W ? a.tbias[e] : 0.f);
-      for (int e = lane; e < 16 * HS; e += 64) dTp[e] = 0.f;
+      for (int e = lane; e < 10 * HS; e += 64) dTp[e] = 0.f;
       wave_sync();
       const float r2 = 1.4142135623730951f;
       for (int e = lane; e < 10 * A; e += 64) {
@@ -766,7 +766,7 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
             for (int i = 0; i < 4; ++i) {
               const int r = 4 * (g0 + k) + i;
               const float h3v = t3[4 * k + i] + ((r / Z3 < 4 * T2) ? H2(r / Z3, t2) : 0.f);
-              dr = mfma4(dTp[n * HS + 16 * (g0 + k) + 4 * q + i], h3v, dr);
+              dr = mfma4(n < 10 ? dTp[n * HS + 16 * (g0 + k) + 4 * q + i] : 0.f, h3v, dr);
             }
           // dh3[slot][nbr] = sum_x dT[x][slot] R[nbr][x]
           floatx4 dh3[SZ];
@@ -775,7 +775,7 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
             floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int kk = 0; kk < 3; ++kk)
-              acc = mfma4(dTp[(4 * kk + q) * HS + 16 * (g0 + k) + n], Rl[n * 16 + 4 * kk + q], acc);
+              acc = mfma4(4 * kk + q < 10 ? dTp[(4 * kk + q) * HS + 16 * (g0 + k) + n] : 0.f, Rl[n * 16 + 4 * kk + q], acc);
             dh3[k] = acc;
           }
           // repeat-residual: dh2[r/Z3] += dh3[r]; then dz3 = dh3 (1 - t3^2)
@@ -949,7 +949,7 @@ static int32_t launch_embed(System* s, bool backward) {
     wave_floats = 4 * a.list_cap + 256 + 16 * (16 * kGroup + 1);
     wave_floats = std::max(wave_floats, (size_t)(10 * W));
   } else {
-    wave_floats = 16 * HS + std::max((size_t)(512 + 4 * a.list_cap), (size_t)(10 * W + 20 * A));
+    wave_floats = 10 * HS + std::max((size_t)(512 + 4 * a.list_cap), (size_t)(10 * W + 20 * A));
   }
   wave_floats = (wave_floats + 3) / 4 * 4;
   a.lds_wave_floats = (int)wave_floats;
