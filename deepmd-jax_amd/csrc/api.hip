@@ -20,14 +20,6 @@ void set_error(const char* fmt, ...) {
   va_end(ap);
 }
 
-int32_t embed_launch(System* s, bool backward, int precision);
-int32_t fit_launch(System* s, int precision);
-int32_t md_zero_force(System* s);
-int32_t md_reduce_launch(System* s, int* d_rowbase, int* d_count);
-int32_t md_ke_launch(System* s, double* d_out);
-int32_t md_scale_mom(System* s, const double* d_scale);
-int32_t nlist_clear_flags(System* s);
-int32_t md_nh_half(System* s, double kT, double dt, double tau, int chain);
 
 void timer_begin(System* s, int which, hipEvent_t* ev) {
   *ev = nullptr;
@@ -60,15 +52,9 @@ static void timer_harvest(System* s) {
 }
 
 System::~System() {
-  void* ptrs[] = {d_pos, d_mom, d_mass, d_pos_save, d_mom_save, d_pos32, d_force, d_canon, d_cell_count,
-                  d_cell_start, d_cell_fill, d_cell_atoms, d_atom_cell, d_flags, d_T, d_G, d_dG, d_fitbuf,
-                  d_fitbuf_h, d_eatom, d_partial, d_red, d_pairs, d_thermo, d_traj, d_nm};
-  for (void* p : ptrs)
-    if (p) hipFree(p);
-  for (int j = 0; j < kMaxT; ++j) {
-    if (d_nbr[j]) hipFree(d_nbr[j]);
-    if (d_nbr_cnt[j]) hipFree(d_nbr_cnt[j]);
-  }
+  for (auto& c : caps)
+    if (*c.first) hipFree(*c.first);
+  if (d_nm) hipFree(d_nm);
   if (stream) hipStreamDestroy(stream);
 }
 
@@ -209,79 +195,175 @@ static int32_t model_create(const dpmd_model_desc* d, int device, Model** out) {
 }
 
 // ------------------------------------------------------------------------------
-// system
+// system: segments, layout, grow-only buffers
 
-static int32_t sys_alloc_work(System* s) {
-  Model* M = s->model;
-  const size_t RN = (size_t)s->nrep * s->natoms;
-  const int W = M->wout, D = M->axis * M->wout;
-  int rb = 0;
-  for (int t = 0; t < s->ntypes; ++t) {
-    s->rowbase[t] = rb;
-    rb += (s->nrep * s->count[t] + 63) / 64 * 64;
+int32_t sys_ensure(System* s, void** p, size_t bytes) {
+  bytes = std::max<size_t>(bytes, 256);
+  for (auto& c : s->caps) {
+    if (c.first != p) continue;
+    if (*p && c.second >= bytes) return DPMD_OK;
+    if (*p) DPMD_HIP(hipFree(*p));
+    *p = nullptr;
+    const size_t nb = bytes + bytes / 4;  // regrowth keeps headroom for the next layout
+    DPMD_HIP(hipMalloc(p, nb));
+    c.second = nb;
+    return DPMD_OK;
   }
-  s->rowbase[s->ntypes] = rb;
-  s->mrows = rb;
-  s->fpad = M->fit_n;
-  s->fstride = M->fit_k;
-  DPMD_HIP(hipMalloc(&s->d_T, sizeof(float) * RN * 10 * W));
-  DPMD_HIP(hipMalloc(&s->d_G, sizeof(float) * (size_t)s->mrows * D));
-  DPMD_HIP(hipMemset(s->d_G, 0, sizeof(float) * (size_t)s->mrows * D));
-  DPMD_HIP(hipMalloc(&s->d_dG, sizeof(float) * (size_t)s->mrows * D));
-  DPMD_HIP(hipMalloc(&s->d_fitbuf, sizeof(float) * 7 * (size_t)s->mrows * s->fstride));
-  DPMD_HIP(hipMemset(s->d_fitbuf, 0, sizeof(float) * 7 * (size_t)s->mrows * s->fstride));
-  DPMD_HIP(hipMalloc(&s->d_eatom, sizeof(float) * std::max(1, s->mrows)));
-  s->partial_blocks = s->nrep * ((s->natoms + 3) / 4);
-  DPMD_HIP(hipMalloc(&s->d_partial, sizeof(double) * 9 * s->partial_blocks));
-  DPMD_HIP(hipMalloc(&s->d_red, sizeof(double) * (16 * s->nrep + 64)));
-  DPMD_HIP(hipMalloc(&s->d_pairs, sizeof(int) * RN));
-  DPMD_HIP(hipMemset(s->d_pairs, 0, sizeof(int) * RN));
+  DPMD_HIP(hipMalloc(p, bytes));
+  s->caps.push_back({p, bytes});
   return DPMD_OK;
 }
 
-static int32_t sys_create(Model* M, int natoms, int nrep, const int32_t* type_idx, System** out) {
-  if (!M || !out || !type_idx) { set_error("null argument"); return DPMD_E_ARG; }
-  if (natoms < 1 || nrep < 1) { set_error("natoms and nreplica must be positive"); return DPMD_E_ARG; }
-  if (nrep > 64) { set_error("nreplica %d above 64", nrep); return DPMD_E_ARG; }
-  for (int i = 0; i < natoms; ++i)
+// The stream is idle whenever host code changes a segment (every entry point ends with a
+// stream synchronisation), so a copy followed by a synchronisation is race-free.
+int32_t sys_upload_segments(System* s) {
+  if (!s->seg_dirty) return DPMD_OK;
+  DPMD_HIP(hipMemcpyAsync(s->d_seg, s->seg.data(), sizeof(SegInfo) * s->nseg, hipMemcpyHostToDevice, s->stream));
+  DPMD_HIP(hipStreamSynchronize(s->stream));
+  s->seg_dirty = false;
+  return DPMD_OK;
+}
+
+static bool same_types(const int32_t* a, const int32_t* b, int n) {
+  for (int i = 0; i < n; ++i)
+    if (a[i] != b[i]) return false;
+  return true;
+}
+
+// Lay out nseg segments (seg_natoms atoms each, caller order concatenated in type_idx):
+// stable type sort inside each segment, fitting rows grouped by type over all segments,
+// embedding blocks never straddling segments.  Buffers are grown, never shrunk.
+static int32_t sys_layout(System* s, int nseg, const int32_t* seg_natoms, const int32_t* type_idx) {
+  Model* M = s->model;
+  if (nseg < 1 || !seg_natoms || !type_idx) { set_error("need nseg >= 1, segment sizes and type indices"); return DPMD_E_ARG; }
+  int64_t tot = 0;
+  for (int r = 0; r < nseg; ++r) {
+    if (seg_natoms[r] < 1) { set_error("segment %d has %d atoms", r, seg_natoms[r]); return DPMD_E_ARG; }
+    tot += seg_natoms[r];
+  }
+  if (tot > (1 << 26)) { set_error("%lld atoms above the 2^26 limit of one system", (long long)tot); return DPMD_E_ARG; }
+  const int NA = (int)tot;
+  for (int i = 0; i < NA; ++i)
     if (type_idx[i] < 0 || type_idx[i] >= M->ntypes) {
       set_error("type index %d out of trained scope of model (ntypes=%d)", type_idx[i], M->ntypes);
       return DPMD_E_ARG;
     }
   DPMD_HIP(hipSetDevice(M->device));
+  if (s->stream) DPMD_HIP(hipStreamSynchronize(s->stream));
+  s->nseg = nseg;
+  s->NA = NA;
+  s->ntypes = M->ntypes;
+  s->seg.assign(nseg, SegInfo());
+  s->perm.resize(NA);
+  s->type_sorted.resize(NA);
+  s->uniform = true;
+  for (int t = 0; t < kMaxT; ++t) s->count[t] = 0;
+  int base = 0, blk = 0;
+  std::vector<int> atom_seg(NA);
+  s->blk_seg.clear();
+  for (int r = 0; r < nseg; ++r) {
+    SegInfo& S = s->seg[r];
+    memset(&S, 0, sizeof(S));
+    const int n = seg_natoms[r];
+    const int32_t* ty = type_idx + base;
+    if (r > 0 && !(n == seg_natoms[0] && same_types(ty, type_idx, n))) s->uniform = false;
+    S.base = base;
+    S.natoms = n;
+    int cnt[kMaxT] = {0, 0, 0, 0};
+    for (int i = 0; i < n; ++i) cnt[ty[i]]++;
+    S.off[0] = 0;
+    for (int t = 0; t < s->ntypes; ++t) S.off[t + 1] = S.off[t] + cnt[t];
+    for (int t = s->ntypes; t < kMaxT; ++t) S.off[t + 1] = S.off[t];
+    for (int t = 0; t < s->ntypes; ++t) {
+      S.row[t] = s->count[t];
+      s->count[t] += cnt[t];
+    }
+    std::vector<int> lp(n);
+    std::iota(lp.begin(), lp.end(), 0);
+    std::stable_sort(lp.begin(), lp.end(), [&](int x, int y) { return ty[x] < ty[y]; });
+    for (int i = 0; i < n; ++i) {
+      s->perm[base + i] = base + lp[i];
+      s->type_sorted[base + i] = ty[lp[i]];
+      atom_seg[base + i] = r;
+    }
+    S.blk0 = blk;
+    S.nblk = (n + 3) / 4;
+    for (int b = 0; b < S.nblk; ++b) s->blk_seg.push_back(r);
+    blk += S.nblk;
+    S.nimg = 1;
+    S.seed = (unsigned long long)r;
+    S.step = 0;
+    base += n;
+  }
+  s->nblocks = blk;
+  int rb = 0;
+  for (int t = 0; t < s->ntypes; ++t) {
+    s->rowbase[t] = rb;
+    rb += (s->count[t] + 63) / 64 * 64;
+  }
+  for (int t = s->ntypes; t <= kMaxT; ++t) s->rowbase[t] = rb;
+  s->mrows = rb;
+  s->fpad = M->fit_n;
+  s->fstride = M->fit_k;
+  s->mass.assign(NA, 1.0);
+  const size_t W = M->wout, D = (size_t)M->axis * M->wout;
+  const size_t mr = std::max(1, s->mrows);
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_seg, sizeof(SegInfo) * nseg));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_atom_seg, sizeof(int) * NA));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_blk_seg, sizeof(int) * s->nblocks));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_pos, sizeof(double) * NA * 3));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_mom, sizeof(double) * NA * 3));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_pos_save, sizeof(double) * NA * 3));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_mom_save, sizeof(double) * NA * 3));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_mass, sizeof(double) * NA));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_pos32, sizeof(float4) * NA));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_force, sizeof(float) * NA * 3));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_scratch, sizeof(double) * NA));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_flags, sizeof(int) * 16));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_thermo, sizeof(double) * 64 * nseg));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_T, sizeof(float) * (size_t)NA * 10 * W));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_G, sizeof(float) * mr * D));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_dG, sizeof(float) * mr * D));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_fitbuf, sizeof(float) * 7 * mr * s->fstride));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_eatom, sizeof(float) * mr));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_partial, sizeof(double) * 9 * s->nblocks));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_red, sizeof(double) * (32 * (size_t)nseg + 64)));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_pairs, sizeof(int) * NA));
+  hipStream_t st = s->stream;
+  DPMD_HIP(hipMemsetAsync(s->d_pos, 0, sizeof(double) * NA * 3, st));
+  DPMD_HIP(hipMemsetAsync(s->d_mom, 0, sizeof(double) * NA * 3, st));
+  DPMD_HIP(hipMemsetAsync(s->d_force, 0, sizeof(float) * NA * 3, st));
+  DPMD_HIP(hipMemsetAsync(s->d_flags, 0, sizeof(int) * 16, st));
+  DPMD_HIP(hipMemsetAsync(s->d_thermo, 0, sizeof(double) * 64 * nseg, st));
+  DPMD_HIP(hipMemsetAsync(s->d_pairs, 0, sizeof(int) * NA, st));
+  // padding rows of G and the K-padding columns of the fitting activations must read as zero
+  DPMD_HIP(hipMemsetAsync(s->d_G, 0, sizeof(float) * mr * D, st));
+  DPMD_HIP(hipMemsetAsync(s->d_fitbuf, 0, sizeof(float) * 7 * mr * s->fstride, st));
+  DPMD_HIP(hipMemcpyAsync(s->d_atom_seg, atom_seg.data(), sizeof(int) * NA, hipMemcpyHostToDevice, st));
+  DPMD_HIP(hipMemcpyAsync(s->d_blk_seg, s->blk_seg.data(), sizeof(int) * s->nblocks, hipMemcpyHostToDevice, st));
+  DPMD_HIP(hipMemcpyAsync(s->d_mass, s->mass.data(), sizeof(double) * NA, hipMemcpyHostToDevice, st));
+  DPMD_HIP(hipStreamSynchronize(st));
+  s->seg_dirty = true;
+  s->nl_ready = false;
+  s->nl_overflow = false;
+  s->force_valid = false;
+  s->rng_counter = 0;
+  return DPMD_OK;
+}
+
+static int32_t sys_create(Model* M, int nseg, const int32_t* seg_natoms, const int32_t* type_idx, System** out) {
+  if (!M || !out) { set_error("null argument"); return DPMD_E_ARG; }
+  DPMD_HIP(hipSetDevice(M->device));
   System* s = new System();
   s->model = M;
-  s->natoms = natoms;
-  s->nrep = nrep;
-  s->ntypes = M->ntypes;
-  for (int t = 0; t < kMaxT; ++t) { s->count[t] = 0; s->knbr[t] = 0; s->cap[t] = 0; }
-  for (int i = 0; i < natoms; ++i) s->count[type_idx[i]]++;
-  s->off[0] = 0;
-  for (int t = 0; t < s->ntypes; ++t) s->off[t + 1] = s->off[t] + s->count[t];
-  s->perm.resize(natoms);
-  std::iota(s->perm.begin(), s->perm.end(), 0);
-  std::stable_sort(s->perm.begin(), s->perm.end(), [&](int a, int b) { return type_idx[a] < type_idx[b]; });
-  s->type_sorted.resize(natoms);
-  for (int i = 0; i < natoms; ++i) s->type_sorted[i] = type_idx[s->perm[i]];
-  DPMD_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
-  const size_t RN = (size_t)natoms * nrep;
-  DPMD_HIP(hipMalloc(&s->d_pos, sizeof(double) * RN * 3));
-  DPMD_HIP(hipMalloc(&s->d_mom, sizeof(double) * RN * 3));
-  DPMD_HIP(hipMalloc(&s->d_pos_save, sizeof(double) * RN * 3));
-  DPMD_HIP(hipMalloc(&s->d_mom_save, sizeof(double) * RN * 3));
-  DPMD_HIP(hipMalloc(&s->d_mass, sizeof(double) * natoms));
-  DPMD_HIP(hipMalloc(&s->d_pos32, sizeof(float4) * RN));
-  DPMD_HIP(hipMalloc(&s->d_force, sizeof(float) * RN * 3));
-  DPMD_HIP(hipMalloc(&s->d_flags, sizeof(int) * 16));
-  DPMD_HIP(hipMalloc(&s->d_thermo, sizeof(double) * 64 * nrep));
-  DPMD_HIP(hipMemset(s->d_pos, 0, sizeof(double) * RN * 3));
-  DPMD_HIP(hipMemset(s->d_mom, 0, sizeof(double) * RN * 3));
-  DPMD_HIP(hipMemset(s->d_force, 0, sizeof(float) * RN * 3));
-  DPMD_HIP(hipMemset(s->d_flags, 0, sizeof(int) * 16));
-  DPMD_HIP(hipMemset(s->d_thermo, 0, sizeof(double) * 64 * nrep));
-  std::vector<double> ones(natoms, 1.0);
-  DPMD_HIP(hipMemcpy(s->d_mass, ones.data(), sizeof(double) * natoms, hipMemcpyHostToDevice));
-  int32_t rc = sys_alloc_work(s);
+  for (int t = 0; t < kMaxT; ++t) { s->knbr[t] = 0; s->cap[t] = 0; }
+  hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    set_error("HIP error '%s' creating a stream", hipGetErrorString(e));
+    delete s;
+    return DPMD_E_HIP;
+  }
+  int32_t rc = sys_layout(s, nseg, seg_natoms, type_idx);
   if (rc != DPMD_OK) { delete s; return rc; }
   *out = s;
   return DPMD_OK;
@@ -289,6 +371,7 @@ static int32_t sys_create(Model* M, int natoms, int nrep, const int32_t* type_id
 
 static int32_t model_forces_impl(System* s, int precision) {
   hipEvent_t ev;
+  DPMD_TRY(sys_upload_segments(s));
   s->force_valid = true;
   DPMD_TRY(md_zero_force(s));
   timer_begin(s, 1, &ev);
@@ -303,16 +386,6 @@ static int32_t model_forces_impl(System* s, int precision) {
   return DPMD_OK;
 }
 
-static int32_t nl_threshold(System* s, float* thr) {
-  const double rc = s->model->rcut;
-  float sc = 0.f;
-  for (int d = 0; d < 3; ++d) sc = std::max(sc, (float)s->ref_box[d] / (float)s->box[d]);
-  const float allowed = ((float)s->rc_skin - (float)rc * sc) / 2.f;
-  const double lmin = std::min(s->ref_box[0], std::min(s->ref_box[1], s->ref_box[2]));
-  *thr = allowed - (float)(lmin * 1e-6);
-  return DPMD_OK;
-}
-
 static int32_t nl_read_flags(System* s, int* h) {
   DPMD_HIP(hipMemcpyAsync(h, s->d_flags, sizeof(int) * 16, hipMemcpyDeviceToHost, s->stream));
   DPMD_HIP(hipStreamSynchronize(s->stream));
@@ -321,15 +394,24 @@ static int32_t nl_read_flags(System* s, int* h) {
 
 static int32_t nl_allocate(System* s, double rc_skin, double ratio, const int32_t* kover, int32_t* kout) {
   if (!(rc_skin > 0)) { set_error("rc_skin must be positive"); return DPMD_E_ARG; }
-  if (s->box[0] <= 0) { set_error("box not set"); return DPMD_E_STATE; }
-  for (int d = 0; d < 3; ++d)
-    if (s->box[d] < 2.0 * rc_skin) {
-      set_error("box length %g below 2*(rcut+skin)=%g: neighbour list needs the minimum-image regime", s->box[d], 2 * rc_skin);
-      return DPMD_E_ARG;
-    }
+  for (int r = 0; r < s->nseg; ++r) {
+    const SegInfo& S = s->seg[r];
+    if (!(S.boxd[0] > 0)) { set_error("box not set"); return DPMD_E_STATE; }
+    for (int d = 0; d < 3; ++d)
+      if (S.boxd[d] < 2.0 * rc_skin) {
+        set_error("box length %g below 2*(rcut+skin)=%g: neighbour list needs the minimum-image regime",
+                  S.boxd[d], 2 * rc_skin);
+        return DPMD_E_ARG;
+      }
+  }
   s->rc_skin = rc_skin;
   s->buffer_ratio = ratio;
-  for (int d = 0; d < 3; ++d) s->ref_box[d] = s->box[d];
+  for (SegInfo& S : s->seg)
+    for (int d = 0; d < 3; ++d) {
+      S.L[d] = (float)S.boxd[d];
+      S.scale[d] = 1.f;
+    }
+  s->seg_dirty = true;
   DPMD_TRY(nlist_alloc_cells(s));
   int h[16];
   int kn[kMaxT];
@@ -357,13 +439,8 @@ static int32_t nl_allocate(System* s, double rc_skin, double ratio, const int32_
     s->cap[j] = std::max(1, kn[j] - 1);
     ksum += kn[j];
     if (kout) kout[j] = kn[j];
-    if (s->d_nbr[j]) hipFree(s->d_nbr[j]);
-    if (s->d_nbr_cnt[j]) hipFree(s->d_nbr_cnt[j]);
-    s->d_nbr[j] = nullptr;
-    s->d_nbr_cnt[j] = nullptr;
-    const size_t RN = (size_t)s->nrep * s->natoms;
-    DPMD_HIP(hipMalloc(&s->d_nbr[j], sizeof(int) * RN * s->cap[j]));
-    DPMD_HIP(hipMalloc(&s->d_nbr_cnt[j], sizeof(int) * RN));
+    DPMD_TRY(sys_ensure(s, (void**)&s->d_nbr[j], sizeof(int) * (size_t)s->NA * s->cap[j]));
+    DPMD_TRY(sys_ensure(s, (void**)&s->d_nbr_cnt[j], sizeof(int) * (size_t)s->NA));
   }
   while (s->nl_lds_list < ksum + 64) s->nl_lds_list *= 2;
   DPMD_TRY(nlist_clear_flags(s));
@@ -376,18 +453,22 @@ static int32_t nl_allocate(System* s, double rc_skin, double ratio, const int32_
 }
 
 static void to_sorted(System* s, const double* in, std::vector<double>& out) {
-  const int N = s->natoms;
-  out.resize((size_t)s->nrep * N * 3);
-  for (int r = 0; r < s->nrep; ++r)
-    for (int i = 0; i < N; ++i)
-      for (int d = 0; d < 3; ++d) out[((size_t)r * N + i) * 3 + d] = in[((size_t)r * N + s->perm[i]) * 3 + d];
+  out.resize((size_t)s->NA * 3);
+  for (int g = 0; g < s->NA; ++g)
+    for (int d = 0; d < 3; ++d) out[(size_t)g * 3 + d] = in[(size_t)s->perm[g] * 3 + d];
 }
 
 static void from_sorted(System* s, const double* in, double* out) {
-  const int N = s->natoms;
-  for (int r = 0; r < s->nrep; ++r)
-    for (int i = 0; i < N; ++i)
-      for (int d = 0; d < 3; ++d) out[((size_t)r * N + s->perm[i]) * 3 + d] = in[((size_t)r * N + i) * 3 + d];
+  for (int g = 0; g < s->NA; ++g)
+    for (int d = 0; d < 3; ++d) out[(size_t)s->perm[g] * 3 + d] = in[(size_t)g * 3 + d];
+}
+
+static double seg_ebias(System* s, int r) {
+  const Model* M = s->model;
+  const SegInfo& S = s->seg[r];
+  double eb = 0;
+  for (int t = 0; t < s->ntypes; ++t) eb += M->ebias[t] * (S.off[t + 1] - S.off[t]) * M->out_norm;
+  return eb / S.nimg;
 }
 
 static int32_t compute_impl(System* s, int precision, double* energy, double* force, double* virial, double* eatom) {
@@ -395,51 +476,57 @@ static int32_t compute_impl(System* s, int precision, double* energy, double* fo
   Model* M = s->model;
   DPMD_TRY(md_write_pos32(s));
   DPMD_TRY(model_forces_impl(s, precision));
-  int meta[2 * kMaxT];
-  for (int t = 0; t < s->ntypes; ++t) { meta[t] = s->rowbase[t]; meta[kMaxT + t] = s->count[t]; }
-  int* d_meta = reinterpret_cast<int*>(s->d_red + 16 * s->nrep);
-  DPMD_HIP(hipMemcpyAsync(d_meta, meta, sizeof(meta), hipMemcpyHostToDevice, s->stream));
-  DPMD_TRY(md_reduce_launch(s, d_meta, d_meta + kMaxT));
-  std::vector<double> red((size_t)10 * s->nrep);
-  DPMD_HIP(hipMemcpyAsync(red.data(), s->d_red, sizeof(double) * 10 * s->nrep, hipMemcpyDeviceToHost, s->stream));
-  const size_t RN = (size_t)s->nrep * s->natoms;
+  DPMD_TRY(md_reduce_launch(s));
+  const size_t NA = s->NA;
+  std::vector<double> red((size_t)16 * s->nseg);
+  DPMD_HIP(hipMemcpyAsync(red.data(), s->d_red, sizeof(double) * red.size(), hipMemcpyDeviceToHost, s->stream));
   std::vector<float> f;
   if (force) {
-    f.resize(RN * 3);
-    DPMD_HIP(hipMemcpyAsync(f.data(), s->d_force, sizeof(float) * RN * 3, hipMemcpyDeviceToHost, s->stream));
+    f.resize(NA * 3);
+    DPMD_HIP(hipMemcpyAsync(f.data(), s->d_force, sizeof(float) * NA * 3, hipMemcpyDeviceToHost, s->stream));
   }
   std::vector<float> ea;
   if (eatom) {
     ea.resize(s->mrows);
     DPMD_HIP(hipMemcpyAsync(ea.data(), s->d_eatom, sizeof(float) * s->mrows, hipMemcpyDeviceToHost, s->stream));
   }
-  std::vector<int> pc(RN);
-  DPMD_HIP(hipMemcpyAsync(pc.data(), s->d_pairs, sizeof(int) * RN, hipMemcpyDeviceToHost, s->stream));
   DPMD_HIP(hipStreamSynchronize(s->stream));
-  double ps = 0;
-  for (size_t i = 0; i < RN; ++i) ps += pc[i];
-  s->last_pairs = ps / RN;
-  double eb = 0;
-  for (int t = 0; t < s->ntypes; ++t) eb += M->ebias[t] * s->count[t] * M->out_norm;
-  for (int r = 0; r < s->nrep; ++r) {
-    if (energy) energy[r] = red[r * 10] + eb;
+  for (int r = 0; r < s->nseg; ++r) {
+    if (energy) energy[r] = red[r * 16] + seg_ebias(s, r);
     if (virial)
-      for (int k = 0; k < 9; ++k) virial[r * 9 + k] = red[r * 10 + 1 + k];
+      for (int k = 0; k < 9; ++k) virial[r * 9 + k] = red[r * 16 + 1 + k];
   }
   if (force) {
-    std::vector<double> fd(RN * 3);
-    for (size_t i = 0; i < RN * 3; ++i) fd[i] = f[i];
+    std::vector<double> fd(NA * 3);
+    for (size_t i = 0; i < NA * 3; ++i) fd[i] = f[i];
     from_sorted(s, fd.data(), force);
   }
   if (eatom) {
-    const int N = s->natoms;
-    for (int r = 0; r < s->nrep; ++r)
-      for (int i = 0; i < N; ++i) {
-        const int t = s->type_sorted[i];
-        const int row = s->rowbase[t] + r * s->count[t] + (i - s->off[t]);
-        eatom[(size_t)r * N + s->perm[i]] = ea[row] + M->ebias[t] * M->out_norm;
+    for (int r = 0; r < s->nseg; ++r) {
+      const SegInfo& S = s->seg[r];
+      for (int i = 0; i < S.natoms; ++i) {
+        const int g = S.base + i;
+        const int t = s->type_sorted[g];
+        const int row = s->rowbase[t] + S.row[t] + (i - S.off[t]);
+        eatom[s->perm[g]] = ea[row] + M->ebias[t] * M->out_norm;
       }
+    }
   }
+  return DPMD_OK;
+}
+
+static int32_t set_seg_box(System* s, int r, const double* b) {
+  for (int d = 0; d < 3; ++d)
+    if (!(b[d] > 0)) { set_error("box lengths must be positive"); return DPMD_E_ARG; }
+  SegInfo& S = s->seg[r];
+  for (int d = 0; d < 3; ++d) {
+    S.boxd[d] = b[d];
+    S.box[d] = (float)b[d];
+    if (!s->nl_ready) S.L[d] = (float)b[d];
+    S.scale[d] = S.L[d] / (float)b[d];
+  }
+  s->seg_dirty = true;
+  s->force_valid = false;
   return DPMD_OK;
 }
 
@@ -447,10 +534,15 @@ static int32_t compute_impl(System* s, int precision, double* energy, double* fo
 
 using namespace dpmd;
 
+#define SYS(sys)                                                        \
+  System* s = reinterpret_cast<System*>(sys);                           \
+  if (!s) { set_error("null system handle"); return DPMD_E_ARG; }       \
+  DPMD_HIP(hipSetDevice(s->model->device))
+
 extern "C" {
 
 const char* dpmd_last_error(void) { return g_err; }
-int32_t dpmd_version(void) { return 1; }
+int32_t dpmd_version(void) { return 2; }
 
 int32_t dpmd_model_create(const dpmd_model_desc* desc, int32_t device, dpmd_model_t* out) {
   Model* m = nullptr;
@@ -475,10 +567,27 @@ int32_t dpmd_model_destroy(dpmd_model_t model) {
 
 int32_t dpmd_system_create(dpmd_model_t model, int32_t natoms, int32_t nreplica, const int32_t* type_idx,
                            dpmd_system_t* out) {
+  if (!type_idx || !out) { set_error("null argument"); return DPMD_E_ARG; }
+  if (natoms < 1 || nreplica < 1) { set_error("natoms and nreplica must be positive"); return DPMD_E_ARG; }
+  std::vector<int32_t> sn(nreplica, natoms), ty((size_t)natoms * nreplica);
+  for (int r = 0; r < nreplica; ++r) std::copy(type_idx, type_idx + natoms, ty.begin() + (size_t)r * natoms);
   System* s = nullptr;
-  int32_t rc = sys_create(reinterpret_cast<Model*>(model), natoms, nreplica, type_idx, &s);
+  int32_t rc = sys_create(reinterpret_cast<Model*>(model), nreplica, sn.data(), ty.data(), &s);
   if (rc == DPMD_OK) *out = reinterpret_cast<dpmd_system_t>(s);
   return rc;
+}
+
+int32_t dpmd_system_create_ragged(dpmd_model_t model, int32_t nseg, const int32_t* seg_natoms,
+                                  const int32_t* type_idx, dpmd_system_t* out) {
+  System* s = nullptr;
+  int32_t rc = sys_create(reinterpret_cast<Model*>(model), nseg, seg_natoms, type_idx, &s);
+  if (rc == DPMD_OK) *out = reinterpret_cast<dpmd_system_t>(s);
+  return rc;
+}
+
+int32_t dpmd_system_reshape(dpmd_system_t sys, int32_t nseg, const int32_t* seg_natoms, const int32_t* type_idx) {
+  SYS(sys);
+  return sys_layout(s, nseg, seg_natoms, type_idx);
 }
 
 int32_t dpmd_system_destroy(dpmd_system_t sys) {
@@ -492,21 +601,59 @@ int32_t dpmd_system_destroy(dpmd_system_t sys) {
 }
 
 int32_t dpmd_system_set_box(dpmd_system_t sys, const double* box3) {
-  System* s = reinterpret_cast<System*>(sys);
-  if (!s || !box3) { set_error("null argument"); return DPMD_E_ARG; }
-  for (int d = 0; d < 3; ++d)
-    if (!(box3[d] > 0)) { set_error("box lengths must be positive"); return DPMD_E_ARG; }
-  for (int d = 0; d < 3; ++d) s->box[d] = box3[d];
-  if (!s->nl_ready)
-    for (int d = 0; d < 3; ++d) s->ref_box[d] = box3[d];
+  SYS(sys);
+  if (!box3) { set_error("null argument"); return DPMD_E_ARG; }
+  for (int r = 0; r < s->nseg; ++r) DPMD_TRY(set_seg_box(s, r, box3));
+  return DPMD_OK;
+}
+
+int32_t dpmd_system_set_boxes(dpmd_system_t sys, const double* boxes) {
+  SYS(sys);
+  if (!boxes) { set_error("null argument"); return DPMD_E_ARG; }
+  for (int r = 0; r < s->nseg; ++r) DPMD_TRY(set_seg_box(s, r, boxes + 3 * r));
+  return DPMD_OK;
+}
+
+int32_t dpmd_system_set_images(dpmd_system_t sys, const int32_t* nimg) {
+  SYS(sys);
+  if (!nimg) { set_error("null argument"); return DPMD_E_ARG; }
+  for (int r = 0; r < s->nseg; ++r) {
+    SegInfo& S = s->seg[r];
+    if (nimg[r] < 1) { set_error("segment %d: image count %d", r, nimg[r]); return DPMD_E_ARG; }
+    for (int t = 0; t < s->ntypes; ++t)
+      if ((S.off[t + 1] - S.off[t]) % nimg[r]) {
+        set_error("segment %d: %d atoms of type %d are not %d whole images", r, S.off[t + 1] - S.off[t], t, nimg[r]);
+        return DPMD_E_ARG;
+      }
+    S.nimg = nimg[r];
+  }
+  s->seg_dirty = true;
+  return DPMD_OK;
+}
+
+int32_t dpmd_system_set_rng(dpmd_system_t sys, const uint64_t* seeds, const uint64_t* steps) {
+  SYS(sys);
+  for (int r = 0; r < s->nseg; ++r) {
+    if (seeds) s->seg[r].seed = seeds[r];
+    if (steps) s->seg[r].step = steps[r];
+  }
+  s->seg_dirty = true;
+  return DPMD_OK;
+}
+
+int32_t dpmd_system_get_rng(dpmd_system_t sys, uint64_t* seeds, uint64_t* steps) {
+  SYS(sys);
+  for (int r = 0; r < s->nseg; ++r) {
+    if (seeds) seeds[r] = s->seg[r].seed;
+    if (steps) steps[r] = s->seg[r].step;
+  }
   return DPMD_OK;
 }
 
 int32_t dpmd_system_set_positions(dpmd_system_t sys, const double* pos) {
-  System* s = reinterpret_cast<System*>(sys);
-  if (!s || !pos) { set_error("null argument"); return DPMD_E_ARG; }
-  if (s->box[0] <= 0) { set_error("box not set"); return DPMD_E_STATE; }
-  hipSetDevice(s->model->device);
+  SYS(sys);
+  if (!pos) { set_error("null argument"); return DPMD_E_ARG; }
+  if (!(s->seg[0].boxd[0] > 0)) { set_error("box not set"); return DPMD_E_STATE; }
   std::vector<double> v;
   to_sorted(s, pos, v);
   DPMD_HIP(hipMemcpyAsync(s->d_pos, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice, s->stream));
@@ -517,74 +664,61 @@ int32_t dpmd_system_set_positions(dpmd_system_t sys, const double* pos) {
 }
 
 int32_t dpmd_system_get_positions(dpmd_system_t sys, double* pos) {
-  System* s = reinterpret_cast<System*>(sys);
-  if (!s || !pos) { set_error("null argument"); return DPMD_E_ARG; }
-  std::vector<double> v((size_t)s->nrep * s->natoms * 3);
+  SYS(sys);
+  if (!pos) { set_error("null argument"); return DPMD_E_ARG; }
+  std::vector<double> v((size_t)s->NA * 3);
   DPMD_HIP(hipMemcpyAsync(v.data(), s->d_pos, sizeof(double) * v.size(), hipMemcpyDeviceToHost, s->stream));
   DPMD_HIP(hipStreamSynchronize(s->stream));
   from_sorted(s, v.data(), pos);
   return DPMD_OK;
 }
 
-static int32_t get_masses_sorted(System* s, std::vector<double>& m) {
-  m.resize(s->natoms);
-  DPMD_HIP(hipMemcpy(m.data(), s->d_mass, sizeof(double) * s->natoms, hipMemcpyDeviceToHost));
-  return DPMD_OK;
-}
-
 int32_t dpmd_system_set_velocities(dpmd_system_t sys, const double* vel) {
-  System* s = reinterpret_cast<System*>(sys);
-  if (!s || !vel) { set_error("null argument"); return DPMD_E_ARG; }
-  std::vector<double> v, m;
+  SYS(sys);
+  if (!vel) { set_error("null argument"); return DPMD_E_ARG; }
+  std::vector<double> v;
   to_sorted(s, vel, v);
-  DPMD_TRY(get_masses_sorted(s, m));
-  const int N = s->natoms;
-  for (int r = 0; r < s->nrep; ++r)
-    for (int i = 0; i < N; ++i)
-      for (int d = 0; d < 3; ++d) v[((size_t)r * N + i) * 3 + d] *= m[i];
+  for (int g = 0; g < s->NA; ++g)
+    for (int d = 0; d < 3; ++d) v[(size_t)g * 3 + d] *= s->mass[g];
   DPMD_HIP(hipMemcpyAsync(s->d_mom, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice, s->stream));
   DPMD_HIP(hipStreamSynchronize(s->stream));
   return DPMD_OK;
 }
 
 int32_t dpmd_system_get_velocities(dpmd_system_t sys, double* vel) {
-  System* s = reinterpret_cast<System*>(sys);
-  if (!s || !vel) { set_error("null argument"); return DPMD_E_ARG; }
-  std::vector<double> v((size_t)s->nrep * s->natoms * 3), m;
+  SYS(sys);
+  if (!vel) { set_error("null argument"); return DPMD_E_ARG; }
+  std::vector<double> v((size_t)s->NA * 3);
   DPMD_HIP(hipMemcpyAsync(v.data(), s->d_mom, sizeof(double) * v.size(), hipMemcpyDeviceToHost, s->stream));
   DPMD_HIP(hipStreamSynchronize(s->stream));
-  DPMD_TRY(get_masses_sorted(s, m));
-  const int N = s->natoms;
-  for (int r = 0; r < s->nrep; ++r)
-    for (int i = 0; i < N; ++i)
-      for (int d = 0; d < 3; ++d) v[((size_t)r * N + i) * 3 + d] /= m[i];
+  for (int g = 0; g < s->NA; ++g)
+    for (int d = 0; d < 3; ++d) v[(size_t)g * 3 + d] /= s->mass[g];
   from_sorted(s, v.data(), vel);
   return DPMD_OK;
 }
 
 int32_t dpmd_system_set_masses(dpmd_system_t sys, const double* mass) {
-  System* s = reinterpret_cast<System*>(sys);
-  if (!s || !mass) { set_error("null argument"); return DPMD_E_ARG; }
-  std::vector<double> m(s->natoms);
-  for (int i = 0; i < s->natoms; ++i) {
-    m[i] = mass[s->perm[i]];
-    if (!(m[i] > 0)) { set_error("masses must be positive"); return DPMD_E_ARG; }
+  SYS(sys);
+  if (!mass) { set_error("null argument"); return DPMD_E_ARG; }
+  std::vector<double> m(s->NA);
+  for (int g = 0; g < s->NA; ++g) {
+    m[g] = mass[s->perm[g]];
+    if (!(m[g] > 0)) { set_error("masses must be positive"); return DPMD_E_ARG; }
   }
-  DPMD_HIP(hipMemcpy(s->d_mass, m.data(), sizeof(double) * m.size(), hipMemcpyHostToDevice));
+  s->mass = m;
+  DPMD_HIP(hipMemcpyAsync(s->d_mass, s->mass.data(), sizeof(double) * m.size(), hipMemcpyHostToDevice, s->stream));
+  DPMD_HIP(hipStreamSynchronize(s->stream));
   return DPMD_OK;
 }
 
 int32_t dpmd_nlist_allocate(dpmd_system_t sys, double rc_skin, double buffer_ratio, const int32_t* knbr_override,
                             int32_t* knbr_out) {
-  System* s = reinterpret_cast<System*>(sys);
-  if (!s) { set_error("null argument"); return DPMD_E_ARG; }
-  hipSetDevice(s->model->device);
+  SYS(sys);
   return nl_allocate(s, rc_skin, buffer_ratio, knbr_override, knbr_out);
 }
 
 int32_t dpmd_nlist_update(dpmd_system_t sys, int32_t* overflow) {
-  System* s = reinterpret_cast<System*>(sys);
-  if (!s) { set_error("null argument"); return DPMD_E_ARG; }
+  SYS(sys);
   if (!s->nl_ready) { set_error("neighbour list not allocated"); return DPMD_E_STATE; }
   DPMD_TRY(nlist_clear_flags(s));
   DPMD_TRY(nlist_build(s, false, false));
@@ -597,39 +731,37 @@ int32_t dpmd_nlist_update(dpmd_system_t sys, int32_t* overflow) {
 }
 
 int32_t dpmd_nlist_export(dpmd_system_t sys, int32_t j, int32_t* out) {
-  System* s = reinterpret_cast<System*>(sys);
-  if (!s || !out) { set_error("null argument"); return DPMD_E_ARG; }
+  SYS(sys);
+  if (!out) { set_error("null argument"); return DPMD_E_ARG; }
   if (!s->nl_ready) { set_error("neighbour list not allocated"); return DPMD_E_STATE; }
   if (j < 0 || j >= s->ntypes) { set_error("type %d out of range", j); return DPMD_E_ARG; }
-  const size_t RN = (size_t)s->nrep * s->natoms;
-  DPMD_HIP(hipMemcpyAsync(out, s->d_nbr[j], sizeof(int) * RN * s->cap[j], hipMemcpyDeviceToHost, s->stream));
+  DPMD_HIP(hipMemcpyAsync(out, s->d_nbr[j], sizeof(int) * (size_t)s->NA * s->cap[j], hipMemcpyDeviceToHost,
+                          s->stream));
   DPMD_HIP(hipStreamSynchronize(s->stream));
   return DPMD_OK;
 }
 
 int32_t dpmd_nlist_capacity(dpmd_system_t sys, int32_t* knbr) {
-  System* s = reinterpret_cast<System*>(sys);
-  if (!s || !knbr) { set_error("null argument"); return DPMD_E_ARG; }
+  SYS(sys);
+  if (!knbr) { set_error("null argument"); return DPMD_E_ARG; }
   for (int j = 0; j < s->ntypes; ++j) knbr[j] = s->knbr[j];
   return DPMD_OK;
 }
 
 int32_t dpmd_compute(dpmd_system_t sys, int32_t precision, double* energy, double* force, double* virial,
                      double* atom_energy) {
-  System* s = reinterpret_cast<System*>(sys);
-  if (!s) { set_error("null argument"); return DPMD_E_ARG; }
+  SYS(sys);
   if (precision != DPMD_PREC_FP32 && precision != DPMD_PREC_BF16 && precision != DPMD_PREC_FP16) {
     set_error("unknown precision %d", precision);
     return DPMD_E_ARG;
   }
-  hipSetDevice(s->model->device);
   return compute_impl(s, precision, energy, force, virial, atom_energy);
 }
 
 int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps, double* traj_pos, double* traj_vel,
                     uint32_t* error_code) {
-  System* s = reinterpret_cast<System*>(sys);
-  if (!s || !p) { set_error("null argument"); return DPMD_E_ARG; }
+  SYS(sys);
+  if (!p) { set_error("null argument"); return DPMD_E_ARG; }
   if (!s->nl_ready) { set_error("no neighbour list: call dpmd_nlist_allocate first"); return DPMD_E_STATE; }
   if (nsteps < 0) { set_error("nsteps must be >= 0"); return DPMD_E_ARG; }
   if (p->routine != DPMD_NVE && p->routine != DPMD_NVT_LANGEVIN && p->routine != DPMD_NVT_NH &&
@@ -637,29 +769,28 @@ int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps, 
     set_error("unknown routine %d", p->routine);
     return DPMD_E_ARG;
   }
-  if (p->routine == DPMD_PIMD_PILE_L && !s->d_nm) { set_error("PIMD normal modes not set"); return DPMD_E_STATE; }
-  hipSetDevice(s->model->device);
-  const size_t RN = (size_t)s->nrep * s->natoms;
-  const bool rec = traj_pos || traj_vel;
-  if (rec) {
-    const size_t need = (size_t)nsteps * RN * 3 * 2;
-    if (need > s->traj_cap) {
-      if (s->d_traj) hipFree(s->d_traj);
-      s->d_traj = nullptr;
-      DPMD_HIP(hipMalloc(&s->d_traj, sizeof(double) * need));
-      s->traj_cap = need;
-    }
+  if (p->precision != DPMD_PREC_FP32 && p->precision != DPMD_PREC_BF16 && p->precision != DPMD_PREC_FP16) {
+    set_error("unknown precision %d", p->precision);
+    return DPMD_E_ARG;
   }
-  DPMD_HIP(hipMemcpyAsync(s->d_pos_save, s->d_pos, sizeof(double) * RN * 3, hipMemcpyDeviceToDevice, s->stream));
-  DPMD_HIP(hipMemcpyAsync(s->d_mom_save, s->d_mom, sizeof(double) * RN * 3, hipMemcpyDeviceToDevice, s->stream));
+  if (p->routine == DPMD_PIMD_PILE_L && !s->d_nm) { set_error("PIMD normal modes not set"); return DPMD_E_STATE; }
+  if (p->routine == DPMD_PIMD_PILE_L && !s->uniform) { set_error("PIMD needs identical beads"); return DPMD_E_STATE; }
+  const size_t NA = s->NA;
+  const bool rec = traj_pos || traj_vel;
+  if (rec) DPMD_TRY(sys_ensure(s, (void**)&s->d_traj, sizeof(double) * (size_t)nsteps * NA * 6));
+  DPMD_HIP(hipMemcpyAsync(s->d_pos_save, s->d_pos, sizeof(double) * NA * 3, hipMemcpyDeviceToDevice, s->stream));
+  DPMD_HIP(hipMemcpyAsync(s->d_mom_save, s->d_mom, sizeof(double) * NA * 3, hipMemcpyDeviceToDevice, s->stream));
   if (s->nl_overflow) {
     // the current list already lacks neighbours (deepmd_jax/md.py:637-641): report
     // bit 2 before stepping so the caller reallocates
     if (error_code) *error_code |= 2u;
+    DPMD_HIP(hipStreamSynchronize(s->stream));
     return DPMD_OK;
   }
-  float thr;
-  nl_threshold(s, &thr);
+  std::vector<double> thermo_save((size_t)64 * s->nseg);
+  DPMD_HIP(hipMemcpyAsync(thermo_save.data(), s->d_thermo, sizeof(double) * thermo_save.size(),
+                          hipMemcpyDeviceToHost, s->stream));
+  DPMD_TRY(sys_upload_segments(s));
   DPMD_TRY(nlist_clear_flags(s));
   const double dt = p->dt;
   const int prec = p->precision;
@@ -667,8 +798,9 @@ int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps, 
     DPMD_TRY(md_write_pos32(s));
     DPMD_TRY(model_forces_impl(s, prec));
   }
+  const uint64_t counter0 = s->rng_counter;
   for (int step = 0; step < nsteps; ++step) {
-    DPMD_TRY(nlist_drift_check(s, thr));
+    DPMD_TRY(nlist_drift_check(s));
     DPMD_TRY(nlist_build(s, false, true));
     hipEvent_t ev;
     if (p->routine == DPMD_NVE) {
@@ -683,7 +815,7 @@ int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps, 
     } else if (p->routine == DPMD_NVT_LANGEVIN) {
       DPMD_TRY(md_kick(s, 0.5 * dt));
       DPMD_TRY(md_drift(s, 0.5 * dt));
-      DPMD_TRY(md_langevin_o(s, dt, p->gamma, p->kT, p->seed, s->rng_counter++));
+      DPMD_TRY(md_langevin_o(s, dt, p->gamma, p->kT, p->seed, (uint64_t)step));
       DPMD_TRY(md_drift(s, 0.5 * dt));
       DPMD_TRY(model_forces_impl(s, prec));
       DPMD_TRY(md_kick(s, 0.5 * dt));
@@ -695,23 +827,23 @@ int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps, 
       DPMD_TRY(md_kick(s, 0.5 * dt));
       DPMD_TRY(md_nh_half(s, p->kT, dt, p->tau, p->chain_length));
     } else {
-      const int P = s->nrep;
+      const int P = s->nseg;
       const double* C = s->d_nm;
       const double* fr = s->d_nm + P * P;
       const double* c1 = fr + P;
       const double* c2 = c1 + P;
       DPMD_TRY(md_kick(s, 0.5 * dt));
       DPMD_TRY(md_pimd_free_ring(s, C, fr, 0.5 * dt));
-      DPMD_TRY(md_pimd_o(s, C, c1, c2, p->seed, s->rng_counter++));
+      DPMD_TRY(md_pimd_o(s, C, c1, c2, p->seed, counter0 + step));
       DPMD_TRY(md_pimd_free_ring(s, C, fr, 0.5 * dt));
       DPMD_TRY(md_write_pos32(s));
       DPMD_TRY(model_forces_impl(s, prec));
       DPMD_TRY(md_kick(s, 0.5 * dt));
     }
     if (rec) {
-      DPMD_HIP(hipMemcpyAsync(s->d_traj + (size_t)step * RN * 6, s->d_pos, sizeof(double) * RN * 3,
+      DPMD_HIP(hipMemcpyAsync(s->d_traj + (size_t)step * NA * 6, s->d_pos, sizeof(double) * NA * 3,
                               hipMemcpyDeviceToDevice, s->stream));
-      DPMD_HIP(hipMemcpyAsync(s->d_traj + (size_t)step * RN * 6 + RN * 3, s->d_mom, sizeof(double) * RN * 3,
+      DPMD_HIP(hipMemcpyAsync(s->d_traj + (size_t)step * NA * 6 + NA * 3, s->d_mom, sizeof(double) * NA * 3,
                               hipMemcpyDeviceToDevice, s->stream));
     }
   }
@@ -723,30 +855,31 @@ int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps, 
   if (code & 2) {
     // neighbour capacity exceeded during the chunk: restore the entry state so the caller
     // can reallocate and re-run (deepmd_jax/md.py:914-916)
-    DPMD_HIP(hipMemcpyAsync(s->d_pos, s->d_pos_save, sizeof(double) * RN * 3, hipMemcpyDeviceToDevice, s->stream));
-    DPMD_HIP(hipMemcpyAsync(s->d_mom, s->d_mom_save, sizeof(double) * RN * 3, hipMemcpyDeviceToDevice, s->stream));
+    DPMD_HIP(hipMemcpyAsync(s->d_pos, s->d_pos_save, sizeof(double) * NA * 3, hipMemcpyDeviceToDevice, s->stream));
+    DPMD_HIP(hipMemcpyAsync(s->d_mom, s->d_mom_save, sizeof(double) * NA * 3, hipMemcpyDeviceToDevice, s->stream));
+    DPMD_HIP(hipMemcpyAsync(s->d_thermo, thermo_save.data(), sizeof(double) * thermo_save.size(),
+                            hipMemcpyHostToDevice, s->stream));
     DPMD_TRY(md_write_pos32(s));
     DPMD_HIP(hipStreamSynchronize(s->stream));
     s->force_valid = false;
     s->nl_overflow = true;
     return DPMD_OK;
   }
+  s->rng_counter = counter0 + nsteps;
+  for (SegInfo& S : s->seg) S.step += nsteps;
+  s->seg_dirty = true;
   if (rec) {
-    std::vector<double> buf((size_t)nsteps * RN * 6);
+    std::vector<double> buf((size_t)nsteps * NA * 6);
     DPMD_HIP(hipMemcpyAsync(buf.data(), s->d_traj, sizeof(double) * buf.size(), hipMemcpyDeviceToHost, s->stream));
     DPMD_HIP(hipStreamSynchronize(s->stream));
-    std::vector<double> m;
-    DPMD_TRY(get_masses_sorted(s, m));
-    const int N = s->natoms;
     for (int k = 0; k < nsteps; ++k) {
-      double* bp = buf.data() + (size_t)k * RN * 6;
-      double* bm = bp + RN * 3;
-      if (traj_pos) from_sorted(s, bp, traj_pos + (size_t)k * RN * 3);
+      double* bp = buf.data() + (size_t)k * NA * 6;
+      double* bm = bp + NA * 3;
+      if (traj_pos) from_sorted(s, bp, traj_pos + (size_t)k * NA * 3);
       if (traj_vel) {
-        for (int r = 0; r < s->nrep; ++r)
-          for (int i = 0; i < N; ++i)
-            for (int d = 0; d < 3; ++d) bm[((size_t)r * N + i) * 3 + d] /= m[i];
-        from_sorted(s, bm, traj_vel + (size_t)k * RN * 3);
+        for (size_t g = 0; g < NA; ++g)
+          for (int d = 0; d < 3; ++d) bm[g * 3 + d] /= s->mass[g];
+        from_sorted(s, bm, traj_vel + (size_t)k * NA * 3);
       }
     }
   }
@@ -754,10 +887,12 @@ int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps, 
 }
 
 int32_t dpmd_pimd_setup(dpmd_system_t sys, const double* C, const double* freqs, const double* c1, const double* c2) {
-  System* s = reinterpret_cast<System*>(sys);
-  if (!s || !C || !freqs || !c1 || !c2) { set_error("null argument"); return DPMD_E_ARG; }
-  const int P = s->nrep;
+  SYS(sys);
+  if (!C || !freqs || !c1 || !c2) { set_error("null argument"); return DPMD_E_ARG; }
+  const int P = s->nseg;
   if (P < 2) { set_error("PIMD needs nreplica >= 2"); return DPMD_E_ARG; }
+  if (P > 64) { set_error("PIMD supports at most 64 beads"); return DPMD_E_ARG; }
+  if (!s->uniform) { set_error("PIMD needs identical beads"); return DPMD_E_STATE; }
   for (int k = 1; k < P; ++k)
     if (!(freqs[k] > 0)) { set_error("non-centroid normal-mode frequencies must be positive"); return DPMD_E_ARG; }
   std::vector<double> buf((size_t)P * P + 3 * P);
@@ -765,34 +900,52 @@ int32_t dpmd_pimd_setup(dpmd_system_t sys, const double* C, const double* freqs,
   std::copy(freqs, freqs + P, buf.begin() + P * P);
   std::copy(c1, c1 + P, buf.begin() + P * P + P);
   std::copy(c2, c2 + P, buf.begin() + P * P + 2 * P);
-  if (!s->d_nm) DPMD_HIP(hipMalloc(&s->d_nm, sizeof(double) * buf.size()));
+  if (s->d_nm) DPMD_HIP(hipFree(s->d_nm));
+  s->d_nm = nullptr;
+  DPMD_HIP(hipMalloc(&s->d_nm, sizeof(double) * buf.size()));
   DPMD_HIP(hipMemcpy(s->d_nm, buf.data(), sizeof(double) * buf.size(), hipMemcpyHostToDevice));
   return DPMD_OK;
 }
 
 int32_t dpmd_kinetic_energy(dpmd_system_t sys, double* ke) {
-  System* s = reinterpret_cast<System*>(sys);
-  if (!s || !ke) { set_error("null argument"); return DPMD_E_ARG; }
-  double* d_out = s->d_red + 10 * s->nrep;
+  SYS(sys);
+  if (!ke) { set_error("null argument"); return DPMD_E_ARG; }
+  DPMD_TRY(sys_upload_segments(s));
+  double* d_out = s->d_red + 16 * s->nseg;
   DPMD_TRY(md_ke_launch(s, d_out));
-  DPMD_HIP(hipMemcpyAsync(ke, d_out, sizeof(double) * s->nrep, hipMemcpyDeviceToHost, s->stream));
+  DPMD_HIP(hipMemcpyAsync(ke, d_out, sizeof(double) * s->nseg, hipMemcpyDeviceToHost, s->stream));
   DPMD_HIP(hipStreamSynchronize(s->stream));
   return DPMD_OK;
 }
 
 int32_t dpmd_thermostat_energy(dpmd_system_t sys, double* e) {
-  System* s = reinterpret_cast<System*>(sys);
-  if (!s || !e) { set_error("null argument"); return DPMD_E_ARG; }
-  std::vector<double> t((size_t)64 * s->nrep);
+  SYS(sys);
+  if (!e) { set_error("null argument"); return DPMD_E_ARG; }
+  std::vector<double> t((size_t)64 * s->nseg);
   DPMD_HIP(hipMemcpyAsync(t.data(), s->d_thermo, sizeof(double) * t.size(), hipMemcpyDeviceToHost, s->stream));
   DPMD_HIP(hipStreamSynchronize(s->stream));
-  for (int r = 0; r < s->nrep; ++r) e[r] = t[(size_t)r * 64];
+  for (int r = 0; r < s->nseg; ++r) e[r] = t[(size_t)r * 64];
+  return DPMD_OK;
+}
+
+int32_t dpmd_thermostat_state_get(dpmd_system_t sys, double* state) {
+  SYS(sys);
+  if (!state) { set_error("null argument"); return DPMD_E_ARG; }
+  DPMD_HIP(hipMemcpyAsync(state, s->d_thermo, sizeof(double) * 64 * s->nseg, hipMemcpyDeviceToHost, s->stream));
+  DPMD_HIP(hipStreamSynchronize(s->stream));
+  return DPMD_OK;
+}
+
+int32_t dpmd_thermostat_state_set(dpmd_system_t sys, const double* state) {
+  SYS(sys);
+  if (!state) { set_error("null argument"); return DPMD_E_ARG; }
+  DPMD_HIP(hipMemcpyAsync(s->d_thermo, state, sizeof(double) * 64 * s->nseg, hipMemcpyHostToDevice, s->stream));
+  DPMD_HIP(hipStreamSynchronize(s->stream));
   return DPMD_OK;
 }
 
 int32_t dpmd_timing_enable(dpmd_system_t sys, int32_t on) {
-  System* s = reinterpret_cast<System*>(sys);
-  if (!s) { set_error("null argument"); return DPMD_E_ARG; }
+  SYS(sys);
   timer_harvest(s);
   s->timer.on = on != 0;
   for (int w = 0; w < 8; ++w) { s->timer.ms[w] = 0; s->timer.count[w] = 0; }
@@ -800,8 +953,8 @@ int32_t dpmd_timing_enable(dpmd_system_t sys, int32_t on) {
 }
 
 int32_t dpmd_timing_get(dpmd_system_t sys, int32_t which, double* ms, int64_t* count) {
-  System* s = reinterpret_cast<System*>(sys);
-  if (!s || which < 0 || which >= 8) { set_error("bad argument"); return DPMD_E_ARG; }
+  SYS(sys);
+  if (which < 0 || which >= 8) { set_error("bad argument"); return DPMD_E_ARG; }
   timer_harvest(s);
   if (ms) *ms = s->timer.ms[which];
   if (count) *count = s->timer.count[which];
@@ -809,21 +962,20 @@ int32_t dpmd_timing_get(dpmd_system_t sys, int32_t which, double* ms, int64_t* c
 }
 
 int32_t dpmd_pair_count(dpmd_system_t sys, double* pairs) {
-  System* s = reinterpret_cast<System*>(sys);
-  if (!s || !pairs) { set_error("null argument"); return DPMD_E_ARG; }
-  const size_t RN = (size_t)s->nrep * s->natoms;
-  std::vector<int> pc(RN);
-  DPMD_HIP(hipMemcpyAsync(pc.data(), s->d_pairs, sizeof(int) * RN, hipMemcpyDeviceToHost, s->stream));
+  SYS(sys);
+  if (!pairs) { set_error("null argument"); return DPMD_E_ARG; }
+  const size_t NA = s->NA;
+  std::vector<int> pc(NA);
+  DPMD_HIP(hipMemcpyAsync(pc.data(), s->d_pairs, sizeof(int) * NA, hipMemcpyDeviceToHost, s->stream));
   DPMD_HIP(hipStreamSynchronize(s->stream));
   double t = 0;
-  for (size_t i = 0; i < RN; ++i) t += pc[i];
-  *pairs = t / RN;
+  for (size_t i = 0; i < NA; ++i) t += pc[i];
+  *pairs = t / NA;
   return DPMD_OK;
 }
 
 int32_t dpmd_synchronize(dpmd_system_t sys) {
-  System* s = reinterpret_cast<System*>(sys);
-  if (!s) { set_error("null argument"); return DPMD_E_ARG; }
+  SYS(sys);
   DPMD_HIP(hipStreamSynchronize(s->stream));
   return DPMD_OK;
 }

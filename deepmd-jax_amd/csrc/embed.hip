@@ -148,15 +148,14 @@ void embed_pack_net(const EmbedLayout& L, const float* K0, const float* B0, cons
 // Device helpers
 
 struct EmbedArgs {
-  const float4* pos;           // (R*N) wrapped fp32 positions, type-sorted
-  const int* nbr[kMaxT];       // (R*N, cap_j) type-local indices, padded with N
-  const int* nbr_cnt[kMaxT];   // (R*N)
+  const float4* pos;           // (NA) wrapped fp32 positions, type-sorted per segment
+  const SegInfo* seg;
+  const int* blk_seg;          // block -> segment
+  const int* nbr[kMaxT];       // (NA, cap_j) segment- and type-local indices, padded with N
+  const int* nbr_cnt[kMaxT];   // (NA)
   int cap[kMaxT];
-  int off[kMaxT + 1];
-  int count[kMaxT];
   int rowbase[kMaxT];
-  int ntypes, natoms, nrep, blocks_per_rep;
-  float box[3];
+  int ntypes;
   float rcut;
   float mean[kMaxT], stdv[kMaxT];
   const float* net;            // packed nets, index (i*ntypes+j)*net_stride
@@ -165,12 +164,12 @@ struct EmbedArgs {
   int W, A;
   float inv_nnbrs;
   const float* tbias;
-  float* T;                    // (R*N, 10, W)
+  float* T;                    // (NA, 10, W)
   float* G;                    // (mrows, A*W)
   const float* dG;             // (mrows, A*W)
   float* force;                // (R*N, 3)
   double* partial;             // (gridDim.x, 9) virial partials
-  int* paircnt;                // (R*N)
+  int* paircnt;                // (NA)
   int list_cap;                // LDS list entries per wave
   int net_bytes;
   const void* hnet;            // 16-bit operand tables (bf16 or fp16), per net hnet_stride bytes
@@ -379,9 +378,9 @@ __device__ __forceinline__ float* load_small_tables(const EmbedArgs& a, float* s
 
 // Compact the skin row of neighbour type j to the pairs within rcut, in ascending
 // index order, into the wave's LDS list (float4: x, y, z, global index bits).
-__device__ __forceinline__ int compact_row(const EmbedArgs& a, int rep, int c, int j, float4 pc,
+__device__ __forceinline__ int compact_row(const EmbedArgs& a, const SegInfo& S, int c, int j, float4 pc,
                                            float4* list, int lane) {
-  const int gc = rep * a.natoms + c;
+  const int gc = S.base + c;
   const int cnt = a.nbr_cnt[j][gc];
   const int* row = a.nbr[j] + (size_t)gc * a.cap[j];
   int m = 0;
@@ -391,11 +390,11 @@ __device__ __forceinline__ int compact_row(const EmbedArgs& a, int rep, int c, i
     float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
     if (e < cnt) {
       const int idx = row[e];
-      const int g = rep * a.natoms + a.off[j] + idx;
+      const int g = S.base + S.off[j] + idx;
       const float4 pj = a.pos[g];
-      const float dx = minimg(pj.x - pc.x, a.box[0]);
-      const float dy = minimg(pj.y - pc.y, a.box[1]);
-      const float dz = minimg(pj.z - pc.z, a.box[2]);
+      const float dx = minimg(pj.x - pc.x, S.box[0]);
+      const float dy = minimg(pj.y - pc.y, S.box[1]);
+      const float dz = minimg(pj.z - pc.z, S.box[2]);
       const float r = pair_r(dx, dy, dz);
       keep = (r > 1e-14f) && (r / a.rcut < 1.f);
       out = make_float4(dx, dy, dz, __int_as_float(g));
@@ -429,12 +428,12 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
   float* Hl = Rl + 256;  // one tile group of h3: [16 neighbours][HG]
   constexpr int HG = 16 * kGroup + 1;
 
-  const int rep = blockIdx.x / a.blocks_per_rep;
-  const int c = (blockIdx.x % a.blocks_per_rep) * 4 + wv;
-  if (c >= a.natoms) return;
-  const int gc = rep * a.natoms + c;
+  const SegInfo& S = a.seg[a.blk_seg[blockIdx.x]];
+  const int c = (blockIdx.x - S.blk0) * 4 + wv;
+  if (c >= S.natoms) return;
+  const int gc = S.base + c;
   int ti = 0;
-  while (ti + 1 < a.ntypes && c >= a.off[ti + 1]) ++ti;
+  while (ti + 1 < a.ntypes && c >= S.off[ti + 1]) ++ti;
   const float4 pc = a.pos[gc];
 
   floatx4 Tacc[T3];
@@ -443,7 +442,7 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
   int pairs = 0;
 
   for (int j = 0; j < a.ntypes; ++j) {
-    const int m = compact_row(a, rep, c, j, pc, list, lane);
+    const int m = compact_row(a, S, c, j, pc, list, lane);
     pairs += m;
     wave_sync();
     // byte offsets of the net's tables (layout of embed_net_offsets)
@@ -536,7 +535,7 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
   wave_sync();
   float* Tout = a.T + (size_t)gc * 10 * W;
   for (int e = lane; e < 10 * W; e += 64) Tout[e] = Tl[e];
-  const int grow = a.rowbase[ti] + rep * a.count[ti] + (c - a.off[ti]);
+  const int grow = a.rowbase[ti] + S.row[ti] + (c - S.off[ti]);
   float* Gout = a.G + (size_t)grow * A * W;
   const float* tb = a.tbias;
   const float r2 = 1.4142135623730951f;
@@ -589,19 +588,19 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
   float* dRl = Rl + 256;                 // 256
   float4* list = reinterpret_cast<float4*>(dRl + 256);
 
-  const int rep = blockIdx.x / a.blocks_per_rep;
-  const int c = (blockIdx.x % a.blocks_per_rep) * 4 + wv;
-  const bool active = c < a.natoms;
+  const SegInfo& S = a.seg[a.blk_seg[blockIdx.x]];
+  const int c = (blockIdx.x - S.blk0) * 4 + wv;
+  const bool active = c < S.natoms;
   float vir[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) vir[k] = 0.f;
 
   if (active) {
-    const int gc = rep * a.natoms + c;
+    const int gc = S.base + c;
     int ti = 0;
-    while (ti + 1 < a.ntypes && c >= a.off[ti + 1]) ++ti;
+    while (ti + 1 < a.ntypes && c >= S.off[ti + 1]) ++ti;
     const float4 pc = a.pos[gc];
-    const int grow = a.rowbase[ti] + rep * a.count[ti] + (c - a.off[ti]);
+    const int grow = a.rowbase[ti] + S.row[ti] + (c - S.off[ti]);
     // ---- prologue: dE/dT = (dG/dT)^T dE/dG / Nnbrs into the physical slot layout dTp.
     // G[a][w] = TW[w]TW[a] + sum_c T3[c][w]T3[c][a] + sum_k G2[k][a]T6[k][w] (dpmodel.py:74-78)
     //   V[x][w] = sum_a dG[a][w] X[x][a],  X = (TW, T3 rows)[:, :A] for x<4, G2 for x>=4
@@ -694,7 +693,7 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
     // ---- chunks
     float fc[3] = {0.f, 0.f, 0.f};
     for (int j = 0; j < a.ntypes; ++j) {
-      const int m = compact_row(a, rep, c, j, pc, list, lane);
+      const int m = compact_row(a, S, c, j, pc, list, lane);
       wave_sync();
       const int netb = (ti * a.ntypes + j) * a.net_stride * 4;
       constexpr int O_K0 = 0, O_B0 = 16 * T1, O_B1 = 32 * T1, O_B2 = 32 * T1 + 16 * T2;
@@ -889,21 +888,17 @@ static void fill_args(System* s, EmbedArgs& a) {
   Model* M = s->model;
   memset(&a, 0, sizeof(a));
   a.pos = s->d_pos32;
+  a.seg = s->d_seg;
+  a.blk_seg = s->d_blk_seg;
   for (int j = 0; j < s->ntypes; ++j) {
     a.nbr[j] = s->d_nbr[j];
     a.nbr_cnt[j] = s->d_nbr_cnt[j];
     a.cap[j] = s->cap[j];
-    a.count[j] = s->count[j];
     a.rowbase[j] = s->rowbase[j];
     a.mean[j] = (float)M->mean[j];
     a.stdv[j] = (float)(1.0 / M->istd[j]);
   }
-  for (int j = 0; j <= s->ntypes; ++j) a.off[j] = s->off[j];
   a.ntypes = s->ntypes;
-  a.natoms = s->natoms;
-  a.nrep = s->nrep;
-  a.blocks_per_rep = (s->natoms + 3) / 4;
-  for (int d = 0; d < 3; ++d) a.box[d] = (float)s->box[d];
   a.rcut = (float)M->rcut;
   a.net = M->d_emb;
   a.net_stride = M->off.total;
@@ -943,7 +938,7 @@ static int32_t launch_embed(System* s, bool backward) {
   }
   constexpr int HS = 16 * T3 + 1;
   const int W = a.W, A = a.A;
-  const int nblocks = a.blocks_per_rep * s->nrep;
+  const int nblocks = s->nblocks;
   size_t wave_floats;
   if (!backward) {
     wave_floats = 4 * a.list_cap + 256 + 16 * (16 * kGroup + 1);
@@ -962,10 +957,6 @@ static int32_t launch_embed(System* s, bool backward) {
   if (!backward) {
     hipLaunchKernelGGL((embed_fwd_kernel<T1, T2, T3, Z2, Z3, PREC>), dim3(nblocks), dim3(256), lds, s->stream, a);
   } else {
-    if (s->partial_blocks < nblocks) {
-      set_error("virial partial buffer too small");
-      return DPMD_E_STATE;
-    }
     hipLaunchKernelGGL((embed_bwd_kernel<T1, T2, T3, Z2, Z3, PREC>), dim3(nblocks), dim3(256), lds, s->stream, a);
   }
   DPMD_HIP(hipGetLastError());

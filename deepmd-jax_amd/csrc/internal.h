@@ -90,84 +90,123 @@ struct Timer {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[8];
 };
 
+// One independent periodic system inside a System batch (a request, a bead, a frame).
+// Atoms of all segments are concatenated; inside a segment they are in stable
+// type-sorted order.  Device-visible (copied to System::d_seg).
+struct SegInfo {
+  int base;            // first atom (global index)
+  int natoms;
+  int off[kMaxT + 1];  // type offsets inside the segment
+  int row[kMaxT];      // fitting row of the segment's first atom of each type (relative to rowbase)
+  int blk0, nblk;      // embedding blocks (4 centres per block)
+  int cell0;           // first cell of the segment's cell grid
+  int nc[3];
+  int use_cells;
+  int nimg;            // periodic images held (supercell of a small box), 1 otherwise
+  float thr;           // neighbour-list drift threshold
+  float L[3];          // neighbour-list reference box
+  float scale[3];      // reference box / box
+  float box[3];        // current box, model minimum image
+  double boxd[3];      // current box, FP64 wrap
+  unsigned long long seed;  // stochastic stream of the segment
+  unsigned long long step;  // integrator steps taken (counter of the stream)
+};
+
+struct RowBase {
+  int r[kMaxT];
+};
+
 struct System {
   Model* model = nullptr;
   hipStream_t stream = nullptr;
-  int natoms = 0, nrep = 1, ntypes = 0;
-  int count[kMaxT], off[kMaxT + 1];
-  std::vector<int> perm;      // sorted position -> caller index
-  std::vector<int> type_sorted;
-  double box[3] = {0, 0, 0};
-  double ref_box[3] = {0, 0, 0};
-  // device state, type-sorted, (nrep, natoms, ...)
-  double* d_pos = nullptr;    // (R*N*3)
-  double* d_mom = nullptr;    // momenta
-  double* d_mass = nullptr;   // (N)
+  int ntypes = 0;
+  int nseg = 0;
+  int NA = 0;                   // atoms over all segments
+  bool uniform = true;          // every segment has the same composition (replicas / beads)
+  std::vector<SegInfo> seg;
+  bool seg_dirty = true;        // host copy changed since the last upload
+  int count[kMaxT];             // type counts summed over segments
+  std::vector<int> perm;        // global sorted index -> global caller index
+  std::vector<int> type_sorted; // per global sorted index
+  std::vector<int> blk_seg;     // embedding block -> segment
+  std::vector<double> mass;     // per global sorted atom
+  // device state
+  SegInfo* d_seg = nullptr;
+  int* d_atom_seg = nullptr;    // (NA)
+  int* d_blk_seg = nullptr;     // (nblocks)
+  double* d_pos = nullptr;      // (NA*3), unwrapped
+  double* d_mom = nullptr;
+  double* d_mass = nullptr;     // (NA)
   double* d_pos_save = nullptr;
   double* d_mom_save = nullptr;
-  float4* d_pos32 = nullptr;  // wrapped fp32 mirror for the model
-  float* d_force = nullptr;   // (R*N*3)
+  float4* d_pos32 = nullptr;    // wrapped fp32 mirror for the model
+  float* d_force = nullptr;     // (NA*3)
+  double* d_scratch = nullptr;  // (NA) per-atom work
   // neighbour list
   bool nl_ready = false;
   bool nl_overflow = false;
   double rc_skin = 0, buffer_ratio = 1.2;
   int knbr[kMaxT];
-  int cap[kMaxT];             // knbr - 1 columns
+  int cap[kMaxT];               // knbr - 1 columns
   int* d_nbr[kMaxT] = {nullptr};
   int* d_nbr_cnt[kMaxT] = {nullptr};
-  float4* d_canon = nullptr;  // canonical coordinates of the last build (reference_position)
-  int ncell[3] = {0, 0, 0};
-  bool use_cells = false;
+  float4* d_canon = nullptr;    // canonical coordinates of the last build (reference_position)
+  int ncell_total = 0;
   int* d_cell_count = nullptr;
   int* d_cell_start = nullptr;
   int* d_cell_fill = nullptr;
   int* d_cell_atoms = nullptr;
   int* d_atom_cell = nullptr;
   int nl_lds_list = 512;
-  int* d_flags = nullptr;     // [0] rebuild flag, [1] drift bits, [2] error bits, [3] list overflow,
-                              // [4..4+kMaxT) max count per type, [8] measure-list overflow
+  int* d_flags = nullptr;       // [0] rebuild flag, [1] drift flag, [2] error bits,
+                                // [4..4+kMaxT) max count per type, [8] measure-list overflow
   // model work buffers
-  int rowbase[kMaxT + 1];     // fitting rows per type, padded to 64
+  int rowbase[kMaxT + 1];       // fitting rows per type (all segments), padded to 64
   int mrows = 0;
-  float* d_T = nullptr;       // (R*N, 10, W)
-  void* d_G = nullptr;        // (mrows, A*W) fp32 or half
-  float* d_dG = nullptr;      // (mrows, A*W)
-  float* d_fitbuf = nullptr;  // 7 activations (mrows, Fpad)
-  void* d_fitbuf_h = nullptr; // half copies of GEMM A operands
-  float* d_eatom = nullptr;   // (mrows)
+  int nblocks = 0;              // embedding blocks over all segments
+  float* d_T = nullptr;         // (NA, 10, W)
+  void* d_G = nullptr;          // (mrows, A*W)
+  float* d_dG = nullptr;        // (mrows, A*W)
+  float* d_fitbuf = nullptr;    // 7 activations (mrows, fstride)
+  float* d_eatom = nullptr;     // (mrows)
   double* d_partial = nullptr;  // per-block virial partials (nblocks, 9)
-  int partial_blocks = 0;
-  double* d_red = nullptr;    // reduction outputs
-  unsigned long long* d_pairs = nullptr;
-  double last_pairs = 0;
-  int fpad = 0;       // fitting output width
-  int fstride = 0;    // fitting activation row stride
-  // integrator
-  double* d_thermo = nullptr;  // per replica thermostat scalars
-  double* d_traj = nullptr;    // device trajectory staging (chunk)
-  size_t traj_cap = 0;
-  double* d_nm = nullptr;      // PIMD normal-mode matrix (P*P)
-  uint64_t rng_counter = 0;
+  double* d_red = nullptr;      // 16 doubles per segment
+  int* d_pairs = nullptr;       // (NA)
+  int fpad = 0;                 // fitting output width
+  int fstride = 0;              // fitting activation row stride
   bool force_valid = false;
+  // integrator
+  double* d_thermo = nullptr;   // 64 doubles per segment
+  double* d_traj = nullptr;     // device trajectory staging
+  double* d_nm = nullptr;       // PIMD normal modes
+  uint64_t rng_counter = 0;
   Timer timer;
+  std::vector<std::pair<void**, size_t>> caps;  // allocation capacities (grow-only buffers)
   ~System();
 };
 
+// grow-only device allocation: keeps the buffer when it is large enough
+int32_t sys_ensure(System* s, void** p, size_t bytes);
+int32_t sys_upload_segments(System* s);
+
 // kernels / launchers (return DPMD_OK or error)
 int32_t nlist_build(System* s, bool measure, bool conditional);
-int32_t nlist_drift_check(System* s, float threshold);
+int32_t nlist_drift_check(System* s);
 int32_t nlist_alloc_cells(System* s);
-int32_t model_forces(System* s, int precision);
-int32_t model_reduce(System* s, double* energy, double* virial, double* eatom_host);
+int32_t nlist_clear_flags(System* s);
+int32_t embed_launch(System* s, bool backward, int precision);
+int32_t fit_launch(System* s, int precision);
+int32_t md_zero_force(System* s);
 int32_t md_kick(System* s, double dt_half);
 int32_t md_drift(System* s, double dt);
 int32_t md_write_pos32(System* s);
-int32_t md_ke(System* s, double* ke);
-int32_t md_langevin_o(System* s, double dt, double gamma, double kT, uint64_t seed, uint64_t counter);
+int32_t md_reduce_launch(System* s);
+int32_t md_ke_launch(System* s, double* d_out);
+int32_t md_langevin_o(System* s, double dt, double gamma, double kT, uint64_t seed, uint64_t step);
+int32_t md_nh_half(System* s, double kT, double dt, double tau, int chain);
 int32_t md_pimd_free_ring(System* s, const double* cmat, const double* freqs, double dt_half);
 int32_t md_pimd_o(System* s, const double* cmat, const double* c1, const double* c2, uint64_t seed,
                   uint64_t counter);
-int32_t md_check_nan(System* s);
 
 void timer_begin(System* s, int which, hipEvent_t* ev);
 void timer_end(System* s, int which, hipEvent_t ev);

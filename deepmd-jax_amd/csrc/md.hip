@@ -1,5 +1,8 @@
-// Integrator and reduction kernels.  Positions and momenta are FP64 (type-sorted,
-// (nrep, natoms, 3)); forces arrive in FP32 from the model kernels.
+// Integrator and reduction kernels.  Positions and momenta are FP64 ((NA, 3), segments
+// concatenated, type-sorted inside each segment); forces arrive in FP32 from the model
+// kernels.  Per-segment quantities (energy, virial, kinetic energy, thermostat state) are
+// reported per periodic image: a segment that holds nimg images of a small box divides
+// its sums by nimg.
 //
 // Restated integrator steps:
 //   velocity Verlet (jax_md simulate.nve, wrapped at deepmd_jax/routine.py:139-144):
@@ -15,21 +18,22 @@
 
 namespace dpmd {
 
-__global__ void write_pos32_kernel(const double* pos, float4* p32, int total, double Lx, double Ly, double Lz) {
+__device__ __forceinline__ float wrap32(double x, double L) {
+  double r = fmod(x, L);
+  if (r < 0) r += L;
+  if (r >= L) r -= L;
+  float v = (float)r;
+  if (v >= (float)L) v = 0.f;
+  return v;
+}
+
+__global__ void write_pos32_kernel(const double* pos, float4* p32, const SegInfo* seg, const int* atom_seg,
+                                   int total) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= total) return;
-  const double L[3] = {Lx, Ly, Lz};
-  float v[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    double x = pos[(size_t)g * 3 + d];
-    double r = fmod(x, L[d]);
-    if (r < 0) r += L[d];
-    if (r >= L[d]) r -= L[d];
-    v[d] = (float)r;
-    if (v[d] >= (float)L[d]) v[d] = 0.f;
-  }
-  p32[g] = make_float4(v[0], v[1], v[2], 0.f);
+  const SegInfo& S = seg[atom_seg[g]];
+  p32[g] = make_float4(wrap32(pos[(size_t)g * 3], S.boxd[0]), wrap32(pos[(size_t)g * 3 + 1], S.boxd[1]),
+                       wrap32(pos[(size_t)g * 3 + 2], S.boxd[2]), 0.f);
 }
 
 __global__ void zero_kernel(float* x, size_t n) {
@@ -43,12 +47,12 @@ __global__ void kick_kernel(double* mom, const float* force, size_t n3, double h
 }
 
 // x += dt p / m, fp32 mirror, NaN/Inf test -> error bit 16
-__global__ void drift_kernel(double* pos, const double* mom, const double* mass, float4* p32, int natoms,
-                             int total, double dt, double Lx, double Ly, double Lz, int* flags) {
+__global__ void drift_kernel(double* pos, const double* mom, const double* mass, float4* p32, const SegInfo* seg,
+                             const int* atom_seg, int total, double dt, int* flags) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= total) return;
-  const double m = mass[g % natoms];
-  const double L[3] = {Lx, Ly, Lz};
+  const SegInfo& S = seg[atom_seg[g]];
+  const double m = mass[g];
   float v[3];
   bool bad = false;
 #pragma unroll
@@ -57,64 +61,69 @@ __global__ void drift_kernel(double* pos, const double* mom, const double* mass,
     const double x = pos[(size_t)g * 3 + d] + dt * p / m;
     pos[(size_t)g * 3 + d] = x;
     bad |= !isfinite(x) || !isfinite(p);
-    double r = fmod(x, L[d]);
-    if (r < 0) r += L[d];
-    if (r >= L[d]) r -= L[d];
-    v[d] = (float)r;
-    if (v[d] >= (float)L[d]) v[d] = 0.f;
+    v[d] = wrap32(x, S.boxd[d]);
   }
   p32[g] = make_float4(v[0], v[1], v[2], 0.f);
   if (bad) atomicOr(&flags[2], 16);
 }
 
-// per-replica sum of per-row energies and of virial partials, FP64
-__global__ void reduce_kernel(const float* eatom, int nt, const int* rowbase, const int* count, int nrep,
-                              const double* partial, int blocks_per_rep, double* out) {
-  // out[rep*10 + 0] = energy, out[rep*10 + 1..9] = virial
-  __shared__ double red[256];
-  const int rep = blockIdx.x;
-  double s = 0.0;
-  for (int t = 0; t < nt; ++t) {
-    const int base = rowbase[t] + rep * count[t];
-    for (int i = threadIdx.x; i < count[t]; i += blockDim.x) s += (double)eatom[base + i];
-  }
-  red[threadIdx.x] = s;
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  red[threadIdx.x] = v;
   __syncthreads();
-  for (int k = 128; k > 0; k >>= 1) {
+  for (int k = blockDim.x / 2; k > 0; k >>= 1) {
     if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[rep * 10] = red[0];
+  const double r = red[0];
   __syncthreads();
+  return r;
+}
+
+// per-segment energy (sum of per-row energies) and virial (sum of the segment's block
+// partials), FP64, per periodic image: out[16 sid + 0] energy, out[16 sid + 1..9] virial
+__global__ void reduce_kernel(const float* eatom, int nt, RowBase rb, const SegInfo* seg, const double* partial,
+                              double* out) {
+  __shared__ double red[256];
+  const SegInfo& S = seg[blockIdx.x];
+  double s = 0.0;
+  for (int t = 0; t < nt; ++t) {
+    const int base = rb.r[t] + S.row[t];
+    const int cnt = S.off[t + 1] - S.off[t];
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x) s += (double)eatom[base + i];
+  }
+  const double inv = 1.0 / S.nimg;
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) out[blockIdx.x * 16] = s * inv;
   for (int c = 0; c < 9; ++c) {
     double v = 0.0;
-    for (int b = threadIdx.x; b < blocks_per_rep; b += blockDim.x) v += partial[(size_t)(rep * blocks_per_rep + b) * 9 + c];
-    red[threadIdx.x] = v;
-    __syncthreads();
-    for (int k = 128; k > 0; k >>= 1) {
-      if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) out[rep * 10 + 1 + c] = red[0];
-    __syncthreads();
+    for (int b = threadIdx.x; b < S.nblk; b += blockDim.x) v += partial[(size_t)(S.blk0 + b) * 9 + c];
+    v = block_sum(v, red);
+    if (threadIdx.x == 0) out[blockIdx.x * 16 + 1 + c] = v * inv;
   }
 }
 
-__global__ void ke_kernel(const double* mom, const double* mass, int natoms, double* out) {
+// kinetic energy per segment (per image)
+__global__ void ke_kernel(const double* mom, const double* mass, const SegInfo* seg, double* out) {
   __shared__ double red[256];
-  const int rep = blockIdx.x;
+  const SegInfo& S = seg[blockIdx.x];
   double s = 0.0;
-  for (int i = threadIdx.x; i < natoms; i += blockDim.x) {
-    const double* p = mom + ((size_t)rep * natoms + i) * 3;
-    s += (p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) / (2.0 * mass[i]);
+  for (int i = threadIdx.x; i < S.natoms; i += blockDim.x) {
+    const size_t g = (size_t)S.base + i;
+    const double* p = mom + g * 3;
+    s += (p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) / (2.0 * mass[g]);
   }
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int k = 128; k > 0; k >>= 1) {
-    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[rep] = red[0];
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) out[blockIdx.x] = s / S.nimg;
+}
+
+// per-segment sum of a per-atom scratch array, added to thermo[64 sid + slot] (per image)
+__global__ void seg_accumulate_kernel(const double* x, const SegInfo* seg, double* thermo, int slot) {
+  __shared__ double red[256];
+  const SegInfo& S = seg[blockIdx.x];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < S.natoms; i += blockDim.x) s += x[(size_t)S.base + i];
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) thermo[blockIdx.x * 64 + slot] += s / S.nimg;
 }
 
 // --- counter-based normal variates: Philox4x32-10 (Salmon et al., SC'11) + Box-Muller
@@ -148,38 +157,42 @@ __device__ __forceinline__ void normal4(uint64_t seed, uint64_t counter, uint32_
   }
 }
 
-// BAOAB O step, with thermostat work accumulated per replica
-__global__ void langevin_o_kernel(double* mom, const double* mass, int natoms, int total, double c1, double c2,
-                                  uint64_t seed, uint64_t counter, double* work) {
-  __shared__ double red[256];
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  double dke = 0.0;
-  int rep = (blockIdx.x * blockDim.x) / natoms;
-  if (g < total) {
-    const double m = mass[g % natoms];
-    double z[4];
-    normal4(seed, counter, (uint32_t)g, 0x4c414e47u, z);
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const double p0 = mom[(size_t)g * 3 + d];
-      const double p1 = c1 * p0 + c2 * sqrt(m) * z[d];
-      mom[(size_t)g * 3 + d] = p1;
-      dke += 0.5 * (p1 * p1 - p0 * p0) / m;
-    }
-  }
-  // blocks never straddle replicas: the launcher rounds each replica to whole blocks
-  red[threadIdx.x] = dke;
-  __syncthreads();
-  for (int k = 128; k > 0; k >>= 1) {
-    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0 && rep < total / natoms) atomicAdd(&work[rep], red[0]);
+// Index of an atom's primary copy inside its segment.  A segment holding nimg images of a
+// small box lists its atoms image-major before the stable type sort, so inside type block
+// j the copy of primary atom k of image m sits at off[j] + m n_j + k.
+__device__ __forceinline__ int primary_index(const SegInfo& S, int t) {
+  if (S.nimg == 1) return t;
+  int j = 0;
+  while (j + 1 < kMaxT && t >= S.off[j + 1]) ++j;
+  const int nj = (S.off[j + 1] - S.off[j]) / S.nimg;
+  return S.off[j] / S.nimg + (t - S.off[j]) % nj;
 }
 
-__global__ void scale_mom_kernel(double* mom, int natoms, int total3, const double* scale) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < total3) mom[i] *= scale[(i / 3) / natoms];
+__device__ __forceinline__ uint64_t seg_key(uint64_t seed, uint64_t seg_seed) {
+  return seed ^ (seg_seed * 0x9E3779B97F4A7C15ull);
+}
+
+// BAOAB O step.  The normals of an atom depend only on (seed, segment seed, segment step,
+// primary index), so a segment draws the same stream alone or batched with others, and
+// all images of a small box receive the same kicks.  Per-atom kinetic-energy change ->
+// scratch for the per-segment thermostat work.
+__global__ void langevin_o_kernel(double* mom, const double* mass, const SegInfo* seg, const int* atom_seg,
+                                  int total, double c1, double c2, uint64_t seed, uint64_t step, double* dke_out) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  const SegInfo& S = seg[atom_seg[g]];
+  const double m = mass[g];
+  double z[4];
+  normal4(seg_key(seed, S.seed), S.step + step, (uint32_t)primary_index(S, g - S.base), 0x4c414e47u, z);
+  double dke = 0.0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double p0 = mom[(size_t)g * 3 + d];
+    const double p1 = c1 * p0 + c2 * sqrt(m) * z[d];
+    mom[(size_t)g * 3 + d] = p1;
+    dke += 0.5 * (p1 * p1 - p0 * p0) / m;
+  }
+  dke_out[g] = dke;
 }
 
 // PIMD exact free ring-polymer step for one (atom, dim) per thread: unwrap beads onto
@@ -270,15 +283,16 @@ __global__ void pimd_o_kernel(double* mom, const double* mass, int natoms, int P
 // ---------------------------------------------------------------------------------
 
 int32_t md_write_pos32(System* s) {
-  const int RN = s->nrep * s->natoms;
-  hipLaunchKernelGGL(write_pos32_kernel, dim3((RN + 255) / 256), dim3(256), 0, s->stream, s->d_pos, s->d_pos32,
-                     RN, s->box[0], s->box[1], s->box[2]);
+  DPMD_TRY(sys_upload_segments(s));
+  const int NA = s->NA;
+  hipLaunchKernelGGL(write_pos32_kernel, dim3((NA + 255) / 256), dim3(256), 0, s->stream, s->d_pos, s->d_pos32,
+                     s->d_seg, s->d_atom_seg, NA);
   DPMD_HIP(hipGetLastError());
   return DPMD_OK;
 }
 
 int32_t md_zero_force(System* s) {
-  const size_t n = (size_t)s->nrep * s->natoms * 3;
+  const size_t n = (size_t)s->NA * 3;
   hipLaunchKernelGGL(zero_kernel, dim3(std::min<size_t>(2048, (n + 255) / 256)), dim3(256), 0, s->stream,
                      s->d_force, n);
   DPMD_HIP(hipGetLastError());
@@ -286,146 +300,140 @@ int32_t md_zero_force(System* s) {
 }
 
 int32_t md_kick(System* s, double h) {
-  const size_t n3 = (size_t)s->nrep * s->natoms * 3;
+  const size_t n3 = (size_t)s->NA * 3;
   hipLaunchKernelGGL(kick_kernel, dim3((n3 + 255) / 256), dim3(256), 0, s->stream, s->d_mom, s->d_force, n3, h);
   DPMD_HIP(hipGetLastError());
   return DPMD_OK;
 }
 
 int32_t md_drift(System* s, double dt) {
-  const int RN = s->nrep * s->natoms;
-  hipLaunchKernelGGL(drift_kernel, dim3((RN + 255) / 256), dim3(256), 0, s->stream, s->d_pos, s->d_mom,
-                     s->d_mass, s->d_pos32, s->natoms, RN, dt, s->box[0], s->box[1], s->box[2], s->d_flags);
+  const int NA = s->NA;
+  hipLaunchKernelGGL(drift_kernel, dim3((NA + 255) / 256), dim3(256), 0, s->stream, s->d_pos, s->d_mom, s->d_mass,
+                     s->d_pos32, s->d_seg, s->d_atom_seg, NA, dt, s->d_flags);
   DPMD_HIP(hipGetLastError());
   return DPMD_OK;
 }
 
-int32_t md_reduce_launch(System* s, int* d_rowbase, int* d_count) {
-  hipLaunchKernelGGL(reduce_kernel, dim3(s->nrep), dim3(256), 0, s->stream, s->d_eatom, s->ntypes, d_rowbase,
-                     d_count, s->nrep, s->d_partial, (s->natoms + 3) / 4, s->d_red);
+int32_t md_reduce_launch(System* s) {
+  RowBase rb;
+  for (int t = 0; t < kMaxT; ++t) rb.r[t] = s->rowbase[t];
+  hipLaunchKernelGGL(reduce_kernel, dim3(s->nseg), dim3(256), 0, s->stream, s->d_eatom, s->ntypes, rb, s->d_seg,
+                     s->d_partial, s->d_red);
   DPMD_HIP(hipGetLastError());
   return DPMD_OK;
 }
 
 int32_t md_ke_launch(System* s, double* d_out) {
-  hipLaunchKernelGGL(ke_kernel, dim3(s->nrep), dim3(256), 0, s->stream, s->d_mom, s->d_mass, s->natoms, d_out);
+  hipLaunchKernelGGL(ke_kernel, dim3(s->nseg), dim3(256), 0, s->stream, s->d_mom, s->d_mass, s->d_seg, d_out);
   DPMD_HIP(hipGetLastError());
   return DPMD_OK;
 }
 
-int32_t md_langevin_o(System* s, double dt, double gamma, double kT, uint64_t seed, uint64_t counter) {
+// BAOAB O step for every segment; thermostat work of segment r is thermo[64 r]
+int32_t md_langevin_o(System* s, double dt, double gamma, double kT, uint64_t seed, uint64_t step) {
   const double c1 = exp(-gamma * dt);
   const double c2 = sqrt(std::max(kT * (1.0 - c1 * c1), 0.0));
-  // one launch per replica so that a block never straddles two replicas; the work
-  // accumulator of replica r is thermo[64 r]
-  for (int r = 0; r < s->nrep; ++r) {
-    hipLaunchKernelGGL(langevin_o_kernel, dim3((s->natoms + 255) / 256), dim3(256), 0, s->stream,
-                       s->d_mom + (size_t)r * s->natoms * 3, s->d_mass, s->natoms, s->natoms, c1, c2, seed,
-                       counter * 65536 + (uint64_t)r, s->d_thermo + 64 * r);
-  }
-  DPMD_HIP(hipGetLastError());
-  return DPMD_OK;
-}
-
-int32_t md_scale_mom(System* s, const double* d_scale) {
-  const int n3 = s->nrep * s->natoms * 3;
-  hipLaunchKernelGGL(scale_mom_kernel, dim3((n3 + 255) / 256), dim3(256), 0, s->stream, s->d_mom, s->natoms, n3,
-                     d_scale);
+  const int NA = s->NA;
+  hipLaunchKernelGGL(langevin_o_kernel, dim3((NA + 255) / 256), dim3(256), 0, s->stream, s->d_mom, s->d_mass,
+                     s->d_seg, s->d_atom_seg, NA, c1, c2, seed, step, s->d_scratch);
+  hipLaunchKernelGGL(seg_accumulate_kernel, dim3(s->nseg), dim3(256), 0, s->stream, s->d_scratch, s->d_seg,
+                     s->d_thermo, 0);
   DPMD_HIP(hipGetLastError());
   return DPMD_OK;
 }
 
 int32_t md_pimd_free_ring(System* s, const double* d_C, const double* d_freqs, double h) {
-  const int n = s->natoms * 3;
+  const SegInfo& S = s->seg[0];
+  const int n = S.natoms * 3;
   hipLaunchKernelGGL(pimd_free_ring_kernel, dim3((n + 127) / 128), dim3(128), 0, s->stream, s->d_pos, s->d_mom,
-                     s->d_mass, s->natoms, s->nrep, d_C, d_freqs, h, s->box[0], s->box[1], s->box[2]);
+                     s->d_mass, S.natoms, s->nseg, d_C, d_freqs, h, S.boxd[0], S.boxd[1], S.boxd[2]);
   DPMD_HIP(hipGetLastError());
   return DPMD_OK;
 }
 
 int32_t md_pimd_o(System* s, const double* d_C, const double* d_c1, const double* d_c2, uint64_t seed,
                   uint64_t counter) {
-  const int n = s->natoms * 3;
+  const int n = s->seg[0].natoms * 3;
   hipLaunchKernelGGL(pimd_o_kernel, dim3((n + 255) / 256), dim3(256), 0, s->stream, s->d_mom, s->d_mass,
-                     s->natoms, s->nrep, d_C, d_c1, d_c2, seed, counter, s->d_thermo);
+                     s->seg[0].natoms, s->nseg, d_C, d_c1, d_c2, seed, counter, s->d_thermo);
   DPMD_HIP(hipGetLastError());
   return DPMD_OK;
 }
 
 // Nose-Hoover chain half step (jax_md simulate.nose_hoover_chain with chain_steps = 1,
-// sy_steps = 1, as wrapped by deepmd_jax/routine.py:147-158), one thread per replica.
-// thermo layout per replica (64 doubles): [0] invariant term, [1] initialised,
-// [8..16) xi, [16..24) p_xi, [24..32) Q, [40] momentum scale.
-__global__ void nh_half_kernel(double* thermo, const double* ke, int nrep, int C, double kT, double dt, double tau,
-                               double dof) {
-  const int rep = threadIdx.x;
-  if (rep >= nrep) return;
-  double* th = thermo + rep * 64;
-  double* xi = th + 8;
-  double* px = th + 16;
-  double* Q = th + 24;
-  if (th[1] == 0.0) {
-    for (int k = 0; k < C; ++k) { xi[k] = 0.0; px[k] = 0.0; }
-    th[1] = 1.0;
-  }
-  for (int k = 0; k < C; ++k) Q[k] = kT * tau * tau;
-  Q[0] *= dof;
-  double KE = ke[rep];
-  const double d2 = dt / 2, d4 = d2 / 2, d8 = d4 / 2;
-  const int M = C - 1;
-  double G;
-  if (M > 0) {
-    G = px[M - 1] * px[M - 1] / Q[M - 1] - kT;
-    px[M] += d4 * G;
-    double pnew = px[M];
-    for (int m = M - 1; m > 0; --m) {
-      G = px[m - 1] * px[m - 1] / Q[m - 1] - kT;
-      const double sc = exp(-d8 * pnew / Q[m + 1]);
-      pnew = sc * (sc * px[m] + d4 * G);
-      px[m] = pnew;
+// sy_steps = 1, as wrapped by deepmd_jax/routine.py:147-158), one thread per segment.
+// thermo layout per segment (64 doubles): [0] invariant term, [1] initialised,
+// [8..16) xi, [16..24) p_xi, [24..32) Q, [40] momentum scale.  Kinetic energy and degrees
+// of freedom are per periodic image.
+__global__ void nh_half_kernel(double* thermo, const double* ke, const SegInfo* seg, int nseg, int C, double kT,
+                               double dt, double tau) {
+  for (int rep = threadIdx.x; rep < nseg; rep += blockDim.x) {
+    const double dof = 3.0 * (seg[rep].natoms / seg[rep].nimg);
+    double* th = thermo + rep * 64;
+    double* xi = th + 8;
+    double* px = th + 16;
+    double* Q = th + 24;
+    if (th[1] == 0.0) {
+      for (int k = 0; k < C; ++k) { xi[k] = 0.0; px[k] = 0.0; }
+      th[1] = 1.0;
     }
+    for (int k = 0; k < C; ++k) Q[k] = kT * tau * tau;
+    Q[0] *= dof;
+    double KE = ke[rep];
+    const double d2 = dt / 2, d4 = d2 / 2, d8 = d4 / 2;
+    const int M = C - 1;
+    double G;
+    if (M > 0) {
+      G = px[M - 1] * px[M - 1] / Q[M - 1] - kT;
+      px[M] += d4 * G;
+      double pnew = px[M];
+      for (int m = M - 1; m > 0; --m) {
+        G = px[m - 1] * px[m - 1] / Q[m - 1] - kT;
+        const double sc = exp(-d8 * pnew / Q[m + 1]);
+        pnew = sc * (sc * px[m] + d4 * G);
+        px[m] = pnew;
+      }
+    }
+    G = 2.0 * KE - dof * kT;
+    {
+      const double sc = (M > 0) ? exp(-d8 * px[1] / Q[1]) : 1.0;
+      px[0] = sc * (sc * px[0] + d4 * G);
+    }
+    const double scale = exp(-d2 * px[0] / Q[0]);
+    KE *= scale * scale;
+    for (int k = 0; k < C; ++k) xi[k] += d2 * px[k] / Q[k];
+    G = 2.0 * KE - dof * kT;
+    for (int m = 0; m < M; ++m) {
+      const double sc = exp(-d8 * px[m + 1] / Q[m + 1]);
+      const double pu = sc * (sc * px[m] + d4 * G);
+      G = pu * pu / Q[m] - kT;
+      px[m] = pu;
+    }
+    if (M > 0) px[M] += d4 * G;
+    th[40] = scale;
+    double e = px[0] * px[0] / (2.0 * Q[0]) + dof * kT * xi[0];
+    for (int k = 1; k < C; ++k) e += px[k] * px[k] / (2.0 * Q[k]) + kT * xi[k];
+    th[0] = e;
   }
-  G = 2.0 * KE - dof * kT;
-  {
-    const double sc = (M > 0) ? exp(-d8 * px[1] / Q[1]) : 1.0;
-    px[0] = sc * (sc * px[0] + d4 * G);
-  }
-  const double scale = exp(-d2 * px[0] / Q[0]);
-  KE *= scale * scale;
-  for (int k = 0; k < C; ++k) xi[k] += d2 * px[k] / Q[k];
-  G = 2.0 * KE - dof * kT;
-  for (int m = 0; m < M; ++m) {
-    const double sc = exp(-d8 * px[m + 1] / Q[m + 1]);
-    const double pu = sc * (sc * px[m] + d4 * G);
-    G = pu * pu / Q[m] - kT;
-    px[m] = pu;
-  }
-  if (M > 0) px[M] += d4 * G;
-  th[40] = scale;
-  double e = px[0] * px[0] / (2.0 * Q[0]) + dof * kT * xi[0];
-  for (int k = 1; k < C; ++k) e += px[k] * px[k] / (2.0 * Q[k]) + kT * xi[k];
-  th[0] = e;
 }
 
-__global__ void nh_scale_kernel(double* mom, const double* thermo, int natoms, int total3) {
+__global__ void nh_scale_kernel(double* mom, const double* thermo, const int* atom_seg, int total3) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < total3) mom[i] *= thermo[((i / 3) / natoms) * 64 + 40];
+  if (i < total3) mom[i] *= thermo[atom_seg[i / 3] * 64 + 40];
 }
-
-int32_t md_ke_launch(System* s, double* d_out);
 
 int32_t md_nh_half(System* s, double kT, double dt, double tau, int chain) {
   if (chain < 1 || chain > 8) {
     set_error("Nose-Hoover chain length %d outside [1, 8]", chain);
     return DPMD_E_ARG;
   }
-  double* d_ke = s->d_red + 10 * s->nrep;
+  double* d_ke = s->d_red + 16 * s->nseg;
   DPMD_TRY(md_ke_launch(s, d_ke));
-  hipLaunchKernelGGL(nh_half_kernel, dim3(1), dim3(64), 0, s->stream, s->d_thermo, d_ke, s->nrep, chain, kT, dt,
-                     tau, 3.0 * s->natoms);
-  const int n3 = s->nrep * s->natoms * 3;
+  hipLaunchKernelGGL(nh_half_kernel, dim3(1), dim3(256), 0, s->stream, s->d_thermo, d_ke, s->d_seg, s->nseg, chain,
+                     kT, dt, tau);
+  const int n3 = s->NA * 3;
   hipLaunchKernelGGL(nh_scale_kernel, dim3((n3 + 255) / 256), dim3(256), 0, s->stream, s->d_mom, s->d_thermo,
-                     s->natoms, n3);
+                     s->d_atom_seg, n3);
   DPMD_HIP(hipGetLastError());
   return DPMD_OK;
 }

@@ -44,22 +44,19 @@ __device__ __forceinline__ float pair_d2(float4 a, float4 b, float Lx, float Ly,
 }
 
 struct NlArgs {
-  const double* pos;   // (R*N*3) type-sorted
-  float4* canon;       // (R*N)
-  int* atom_cell;      // (R*N)
-  int* cell_count;     // (R*ncell)
+  const double* pos;   // (NA*3) type-sorted per segment
+  const SegInfo* seg;
+  const int* atom_seg; // (NA)
+  float4* canon;       // (NA)
+  int* atom_cell;      // (NA) cell inside the segment's grid
+  int* cell_count;     // (ncell_total)
   int* cell_start;
   int* cell_fill;
-  int* cell_atoms;     // (R*N)
+  int* cell_atoms;     // (NA) segment-local atom indices grouped by cell
   int* nbr[kMaxT];
   int* nbr_cnt[kMaxT];
   int cap[kMaxT];
-  int off[kMaxT + 1];
-  int ntypes, natoms, nrep;
-  int nc[3], ncell;
-  int use_cells;
-  float L[3];
-  float scale[3];
+  int ntypes, NA, ncell_total;
   float cut_sq;
   int* flags;
   int conditional;
@@ -69,8 +66,7 @@ struct NlArgs {
 
 __global__ void nl_prepare_kernel(NlArgs a) {
   if (a.conditional && a.flags[0] == 0) return;
-  const int tot = a.nrep * a.ncell;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += gridDim.x * blockDim.x) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.ncell_total; i += gridDim.x * blockDim.x) {
     a.cell_count[i] = 0;
     a.cell_fill[i] = 0;
   }
@@ -79,43 +75,45 @@ __global__ void nl_prepare_kernel(NlArgs a) {
 __global__ void nl_canon_kernel(NlArgs a) {
   if (a.conditional && a.flags[0] == 0) return;
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= a.nrep * a.natoms) return;
+  if (g >= a.NA) return;
+  const SegInfo& S = a.seg[a.atom_seg[g]];
   const float shrink = (float)(1.0 - 5e-7);
   float c[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     float x = (float)a.pos[(size_t)g * 3 + d];
-    x = fmod_floor(x, a.L[d]);
-    x = x * a.scale[d];
+    x = fmod_floor(x, S.L[d]);
+    x = x * S.scale[d];
     c[d] = x * shrink;
   }
   a.canon[g] = make_float4(c[0], c[1], c[2], 0.f);
-  if (a.use_cells) {
+  if (S.use_cells) {
     int ci[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      int v = (int)(c[d] * ((float)a.nc[d] / a.L[d]));
-      ci[d] = v < 0 ? 0 : (v >= a.nc[d] ? a.nc[d] - 1 : v);
+      int v = (int)(c[d] * ((float)S.nc[d] / S.L[d]));
+      ci[d] = v < 0 ? 0 : (v >= S.nc[d] ? S.nc[d] - 1 : v);
     }
-    const int cell = (ci[2] * a.nc[1] + ci[1]) * a.nc[0] + ci[0];
-    const int rep = g / a.natoms;
+    const int cell = (ci[2] * S.nc[1] + ci[1]) * S.nc[0] + ci[0];
     a.atom_cell[g] = cell;
-    atomicAdd(&a.cell_count[rep * a.ncell + cell], 1);
+    atomicAdd(&a.cell_count[S.cell0 + cell], 1);
   }
 }
 
-// one block per replica: exclusive scan over cells
+// one block per segment: exclusive scan over the segment's cells
 __global__ void nl_scan_kernel(NlArgs a) {
   if (a.conditional && a.flags[0] == 0) return;
   __shared__ int part[1024];
-  const int rep = blockIdx.x;
-  const int* cnt = a.cell_count + rep * a.ncell;
-  int* st = a.cell_start + rep * a.ncell;
-  const int per = (a.ncell + blockDim.x - 1) / blockDim.x;
+  const SegInfo& S = a.seg[blockIdx.x];
+  if (!S.use_cells) return;
+  const int ncell = S.nc[0] * S.nc[1] * S.nc[2];
+  const int* cnt = a.cell_count + S.cell0;
+  int* st = a.cell_start + S.cell0;
+  const int per = (ncell + blockDim.x - 1) / blockDim.x;
   const int b = threadIdx.x * per;
   int s = 0;
   for (int k = 0; k < per; ++k)
-    if (b + k < a.ncell) s += cnt[b + k];
+    if (b + k < ncell) s += cnt[b + k];
   part[threadIdx.x] = s;
   __syncthreads();
   for (int d = 1; d < blockDim.x; d <<= 1) {
@@ -126,7 +124,7 @@ __global__ void nl_scan_kernel(NlArgs a) {
   }
   int run = threadIdx.x > 0 ? part[threadIdx.x - 1] : 0;
   for (int k = 0; k < per; ++k)
-    if (b + k < a.ncell) {
+    if (b + k < ncell) {
       st[b + k] = run;
       run += cnt[b + k];
     }
@@ -135,11 +133,12 @@ __global__ void nl_scan_kernel(NlArgs a) {
 __global__ void nl_scatter_kernel(NlArgs a) {
   if (a.conditional && a.flags[0] == 0) return;
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= a.nrep * a.natoms) return;
-  const int rep = g / a.natoms, i = g % a.natoms;
-  const int cell = a.atom_cell[g];
-  const int slot = a.cell_start[rep * a.ncell + cell] + atomicAdd(&a.cell_fill[rep * a.ncell + cell], 1);
-  a.cell_atoms[(size_t)rep * a.natoms + slot] = i;
+  if (g >= a.NA) return;
+  const SegInfo& S = a.seg[a.atom_seg[g]];
+  if (!S.use_cells) return;
+  const int cell = S.cell0 + a.atom_cell[g];
+  const int slot = a.cell_start[cell] + atomicAdd(&a.cell_fill[cell], 1);
+  a.cell_atoms[S.base + slot] = g - S.base;
 }
 
 // one wave per centre
@@ -150,11 +149,13 @@ __global__ void __launch_bounds__(256) nl_build_kernel(NlArgs a) {
   int* list = nl_smem + wv * 2 * a.lcap;
   int* sorted = list + a.lcap;
   const int g = blockIdx.x * 4 + wv;
-  if (g >= a.nrep * a.natoms) return;
-  const int rep = g / a.natoms, i = g % a.natoms;
-  const float4* cn = a.canon + (size_t)rep * a.natoms;
+  if (g >= a.NA) return;
+  const int sid = __builtin_amdgcn_readfirstlane(a.atom_seg[g]);
+  const SegInfo& S = a.seg[sid];
+  const int N = S.natoms, i = g - S.base;
+  const float4* cn = a.canon + S.base;
   const float4 ci = cn[i];
-  const float Lx = a.L[0], Ly = a.L[1], Lz = a.L[2];
+  const float Lx = S.L[0], Ly = S.L[1], Lz = S.L[2];
   int cnt = 0;
   auto consider = [&](int atom, bool in) {
     bool keep = false;
@@ -164,27 +165,29 @@ __global__ void __launch_bounds__(256) nl_build_kernel(NlArgs a) {
     if (keep && cnt + pre < a.lcap) list[cnt + pre] = atom;
     cnt += __popcll(bal);
   };
-  if (a.use_cells) {
+  if (S.use_cells) {
     const int cell = a.atom_cell[g];
-    const int cx = cell % a.nc[0], cy = (cell / a.nc[0]) % a.nc[1], cz = cell / (a.nc[0] * a.nc[1]);
+    const int nx = S.nc[0], ny = S.nc[1], nz = S.nc[2];
+    const int cx = cell % nx, cy = (cell / nx) % ny, cz = cell / (nx * ny);
+    const int* catoms = a.cell_atoms + S.base;
     for (int dz = -1; dz <= 1; ++dz)
       for (int dy = -1; dy <= 1; ++dy)
         for (int dx = -1; dx <= 1; ++dx) {
-          const int x = (cx + dx + a.nc[0]) % a.nc[0];
-          const int y = (cy + dy + a.nc[1]) % a.nc[1];
-          const int z = (cz + dz + a.nc[2]) % a.nc[2];
-          const int cc = (z * a.nc[1] + y) * a.nc[0] + x;
-          const int st = a.cell_start[rep * a.ncell + cc];
-          const int num = a.cell_count[rep * a.ncell + cc];
+          const int x = (cx + dx + nx) % nx;
+          const int y = (cy + dy + ny) % ny;
+          const int z = (cz + dz + nz) % nz;
+          const int cc = S.cell0 + (z * ny + y) * nx + x;
+          const int st = a.cell_start[cc];
+          const int num = a.cell_count[cc];
           for (int b = 0; b < num; b += 64) {
             const bool in = b + lane < num;
-            const int atom = in ? a.cell_atoms[(size_t)rep * a.natoms + st + b + lane] : 0;
+            const int atom = in ? catoms[st + b + lane] : 0;
             consider(atom, in);
           }
         }
   } else {
-    for (int b = 0; b < a.natoms; b += 64) {
-      const bool in = b + lane < a.natoms;
+    for (int b = 0; b < N; b += 64) {
+      const bool in = b + lane < N;
       consider(in ? b + lane : 0, in);
     }
   }
@@ -202,10 +205,11 @@ __global__ void __launch_bounds__(256) nl_build_kernel(NlArgs a) {
   nl_wave_sync();
   int start = 0;
   for (int j = 0; j < a.ntypes; ++j) {
+    const int o0 = S.off[j], o1 = S.off[j + 1];
     int c = 0;
     for (int e = lane; e < cnt; e += 64) {
       const int v = sorted[e];
-      c += (v >= a.off[j] && v < a.off[j + 1]) ? 1 : 0;
+      c += (v >= o0 && v < o1) ? 1 : 0;
     }
 #pragma unroll
     for (int s = 1; s < 64; s <<= 1) c += __shfl_xor(c, s);
@@ -215,46 +219,40 @@ __global__ void __launch_bounds__(256) nl_build_kernel(NlArgs a) {
       const int capj = a.cap[j];
       if (c > capj && lane == 0) atomicOr(&a.flags[2], 2);
       int* row = a.nbr[j] + (size_t)g * capj;
-      for (int p = lane; p < capj; p += 64) row[p] = p < c ? sorted[start + p] - a.off[j] : a.natoms;
+      for (int p = lane; p < capj; p += 64) row[p] = p < c ? sorted[start + p] - o0 : N;
       if (lane == 0) a.nbr_cnt[j][g] = c < capj ? c : capj;
     }
     start += c;
   }
 }
 
-// drift of every atom against the canonical coordinates of the last build
-__global__ void nl_drift_kernel(const double* pos, const float4* canon, int total, float Lx, float Ly,
-                                float Lz, float sx, float sy, float sz, int* flags) {
-  __shared__ float red[256];
-  float mx = 0.f;
+// drift of every atom against the canonical coordinates of the last build; sets
+// flags[1] when any atom of any segment moved beyond its segment's threshold
+__global__ void nl_drift_kernel(const double* pos, const float4* canon, const SegInfo* seg, const int* atom_seg,
+                                int total, int* flags) {
+  bool over = false;
   for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += gridDim.x * blockDim.x) {
+    const SegInfo& S = seg[atom_seg[g]];
     const float4 r = canon[g];
+    const float rv[3] = {r.x, r.y, r.z};
     float d[3];
-    const float L[3] = {Lx, Ly, Lz}, sc[3] = {sx, sy, sz}, rv[3] = {r.x, r.y, r.z};
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       float x = (float)pos[(size_t)g * 3 + k];
-      x = fmod_floor(x, L[k]) * sc[k];
+      x = fmod_floor(x, S.L[k]) * S.scale[k];
       float dd = x - rv[k];
-      const float h = L[k] / 2.f;
-      dd = fmod_floor(dd - h, L[k]) - h;
+      const float h = S.L[k] / 2.f;
+      dd = fmod_floor(dd - h, S.L[k]) - h;
       d[k] = dd;
     }
     const float n = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-    mx = fmaxf(mx, n);
+    over |= n > S.thr;
   }
-  red[threadIdx.x] = mx;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) atomicMax(&flags[1], __float_as_int(red[0]));
+  if (__ballot(over) && (threadIdx.x & 63) == 0) atomicOr(&flags[1], 1);
 }
 
-__global__ void nl_decide_kernel(int* flags, float threshold) {
-  const float mx = __int_as_float(flags[1]);
-  flags[0] = (mx > threshold) ? 1 : 0;
+__global__ void nl_decide_kernel(int* flags) {
+  flags[0] = flags[1] != 0 ? 1 : 0;
   flags[1] = 0;
 }
 
@@ -269,6 +267,8 @@ __global__ void nl_clear_kernel(int* flags, int n) {
 static void fill_nl(System* s, NlArgs& a) {
   memset(&a, 0, sizeof(a));
   a.pos = s->d_pos;
+  a.seg = s->d_seg;
+  a.atom_seg = s->d_atom_seg;
   a.canon = s->d_canon;
   a.atom_cell = s->d_atom_cell;
   a.cell_count = s->d_cell_count;
@@ -280,87 +280,81 @@ static void fill_nl(System* s, NlArgs& a) {
     a.nbr_cnt[j] = s->d_nbr_cnt[j];
     a.cap[j] = s->cap[j];
   }
-  for (int j = 0; j <= s->ntypes; ++j) a.off[j] = s->off[j];
   a.ntypes = s->ntypes;
-  a.natoms = s->natoms;
-  a.nrep = s->nrep;
-  for (int d = 0; d < 3; ++d) {
-    a.nc[d] = s->ncell[d];
-    a.L[d] = (float)s->ref_box[d];
-    a.scale[d] = (float)s->ref_box[d] / (float)s->box[d];
-  }
-  a.ncell = s->ncell[0] * s->ncell[1] * s->ncell[2];
-  a.use_cells = s->use_cells ? 1 : 0;
+  a.NA = s->NA;
+  a.ncell_total = s->ncell_total;
   const float cut = (float)s->rc_skin;
   a.cut_sq = cut * cut;
   a.flags = s->d_flags;
   a.lcap = s->nl_lds_list;
 }
 
+// Per-segment cell grids (edge >= cutoff with a margin; the 27-cell stencil needs >= 3
+// cells per axis, otherwise the segment is scanned brute force) and the drift thresholds.
 int32_t nlist_alloc_cells(System* s) {
-  // cell edge >= cutoff with a margin; the 27-cell stencil needs >= 3 cells per axis
-  bool cells = true;
-  for (int d = 0; d < 3; ++d) {
-    int nc = (int)(s->ref_box[d] / (s->rc_skin * (1.0 + 1e-4) + 1e-3));
-    s->ncell[d] = nc;
-    if (nc < 3) cells = false;
+  const double rc = s->model->rcut;
+  int cell0 = 0;
+  for (SegInfo& S : s->seg) {
+    bool cells = true;
+    for (int d = 0; d < 3; ++d) {
+      const int nc = (int)(S.L[d] / (s->rc_skin * (1.0 + 1e-4) + 1e-3));
+      S.nc[d] = nc;
+      if (nc < 3) cells = false;
+    }
+    S.use_cells = cells ? 1 : 0;
+    if (!cells) S.nc[0] = S.nc[1] = S.nc[2] = 1;
+    S.cell0 = cell0;
+    cell0 += cells ? S.nc[0] * S.nc[1] * S.nc[2] : 0;
+    // deepmd_jax/md.py:693-741: ((rc+skin) - rc*scale)/2 - 1e-6 L_min
+    float sc = 0.f;
+    for (int d = 0; d < 3; ++d) sc = std::max(sc, S.scale[d]);
+    const float allowed = ((float)s->rc_skin - (float)rc * sc) / 2.f;
+    const double lmin = std::min((double)S.L[0], std::min((double)S.L[1], (double)S.L[2]));
+    S.thr = allowed - (float)(lmin * 1e-6);
   }
-  s->use_cells = cells;
-  if (!cells) s->ncell[0] = s->ncell[1] = s->ncell[2] = 1;
-  const size_t ncell = (size_t)s->ncell[0] * s->ncell[1] * s->ncell[2];
-  const size_t RN = (size_t)s->nrep * s->natoms;
-  hipFree(s->d_cell_count);
-  hipFree(s->d_cell_start);
-  hipFree(s->d_cell_fill);
-  hipFree(s->d_cell_atoms);
-  hipFree(s->d_atom_cell);
-  hipFree(s->d_canon);
-  s->d_cell_count = s->d_cell_start = s->d_cell_fill = s->d_cell_atoms = s->d_atom_cell = nullptr;
-  s->d_canon = nullptr;
-  DPMD_HIP(hipMalloc(&s->d_cell_count, sizeof(int) * ncell * s->nrep));
-  DPMD_HIP(hipMalloc(&s->d_cell_start, sizeof(int) * ncell * s->nrep));
-  DPMD_HIP(hipMalloc(&s->d_cell_fill, sizeof(int) * ncell * s->nrep));
-  DPMD_HIP(hipMalloc(&s->d_cell_atoms, sizeof(int) * RN));
-  DPMD_HIP(hipMalloc(&s->d_atom_cell, sizeof(int) * RN));
-  DPMD_HIP(hipMalloc(&s->d_canon, sizeof(float4) * RN));
-  DPMD_HIP(hipMemsetAsync(s->d_atom_cell, 0, sizeof(int) * RN, s->stream));
+  s->ncell_total = std::max(1, cell0);
+  s->seg_dirty = true;
+  const size_t NA = (size_t)s->NA;
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_cell_count, sizeof(int) * s->ncell_total));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_cell_start, sizeof(int) * s->ncell_total));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_cell_fill, sizeof(int) * s->ncell_total));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_cell_atoms, sizeof(int) * NA));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_atom_cell, sizeof(int) * NA));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_canon, sizeof(float4) * NA));
+  DPMD_HIP(hipMemsetAsync(s->d_atom_cell, 0, sizeof(int) * NA, s->stream));
   return DPMD_OK;
 }
 
 // measure: record the per-type maximum counts in flags[4..]; conditional: every kernel
 // returns immediately unless flags[0] was set by the drift test.
 int32_t nlist_build(System* s, bool measure, bool conditional) {
+  DPMD_TRY(sys_upload_segments(s));
   NlArgs a;
   fill_nl(s, a);
   a.measure = measure ? 1 : 0;
   a.conditional = conditional ? 1 : 0;
   if (!conditional) hipLaunchKernelGGL(nl_set_flag_kernel, dim3(1), dim3(1), 0, s->stream, s->d_flags, 1);
-  const int RN = s->nrep * s->natoms;
+  const int NA = s->NA;
   hipEvent_t ev = nullptr;
   timer_begin(s, 0, &ev);
-  if (s->use_cells)
-    hipLaunchKernelGGL(nl_prepare_kernel, dim3(std::max(1, std::min(1024, (a.ncell * s->nrep + 255) / 256))),
-                       dim3(256), 0, s->stream, a);
-  hipLaunchKernelGGL(nl_canon_kernel, dim3((RN + 255) / 256), dim3(256), 0, s->stream, a);
-  if (s->use_cells) {
-    hipLaunchKernelGGL(nl_scan_kernel, dim3(s->nrep), dim3(1024), 0, s->stream, a);
-    hipLaunchKernelGGL(nl_scatter_kernel, dim3((RN + 255) / 256), dim3(256), 0, s->stream, a);
-  }
+  hipLaunchKernelGGL(nl_prepare_kernel, dim3(std::max(1, std::min(1024, (a.ncell_total + 255) / 256))), dim3(256),
+                     0, s->stream, a);
+  hipLaunchKernelGGL(nl_canon_kernel, dim3((NA + 255) / 256), dim3(256), 0, s->stream, a);
+  hipLaunchKernelGGL(nl_scan_kernel, dim3(s->nseg), dim3(1024), 0, s->stream, a);
+  hipLaunchKernelGGL(nl_scatter_kernel, dim3((NA + 255) / 256), dim3(256), 0, s->stream, a);
   const size_t lds = sizeof(int) * 2 * a.lcap * 4;
-  hipLaunchKernelGGL(nl_build_kernel, dim3((RN + 3) / 4), dim3(256), lds, s->stream, a);
+  hipLaunchKernelGGL(nl_build_kernel, dim3((NA + 3) / 4), dim3(256), lds, s->stream, a);
   timer_end(s, 0, ev);
   DPMD_HIP(hipGetLastError());
   return DPMD_OK;
 }
 
-int32_t nlist_drift_check(System* s, float threshold) {
-  const int RN = s->nrep * s->natoms;
-  const int blocks = std::min(1024, (RN + 255) / 256);
-  hipLaunchKernelGGL(nl_drift_kernel, dim3(blocks), dim3(256), 0, s->stream, s->d_pos, s->d_canon, RN,
-                     (float)s->ref_box[0], (float)s->ref_box[1], (float)s->ref_box[2],
-                     (float)s->ref_box[0] / (float)s->box[0], (float)s->ref_box[1] / (float)s->box[1],
-                     (float)s->ref_box[2] / (float)s->box[2], s->d_flags);
-  hipLaunchKernelGGL(nl_decide_kernel, dim3(1), dim3(1), 0, s->stream, s->d_flags, threshold);
+int32_t nlist_drift_check(System* s) {
+  const int NA = s->NA;
+  const int blocks = std::min(1024, (NA + 255) / 256);
+  hipLaunchKernelGGL(nl_drift_kernel, dim3(blocks), dim3(256), 0, s->stream, s->d_pos, s->d_canon, s->d_seg,
+                     s->d_atom_seg, NA, s->d_flags);
+  hipLaunchKernelGGL(nl_decide_kernel, dim3(1), dim3(1), 0, s->stream, s->d_flags);
   DPMD_HIP(hipGetLastError());
   return DPMD_OK;
 }

@@ -54,8 +54,14 @@ def lib():
         'dpmd_model_create': (c_i32, [P(ModelDesc), c_i32, P(c_vp)]),
         'dpmd_model_destroy': (c_i32, [c_vp]),
         'dpmd_system_create': (c_i32, [c_vp, c_i32, c_i32, c_vp, P(c_vp)]),
+        'dpmd_system_create_ragged': (c_i32, [c_vp, c_i32, c_vp, c_vp, P(c_vp)]),
+        'dpmd_system_reshape': (c_i32, [c_vp, c_i32, c_vp, c_vp]),
         'dpmd_system_destroy': (c_i32, [c_vp]),
         'dpmd_system_set_box': (c_i32, [c_vp, c_vp]),
+        'dpmd_system_set_boxes': (c_i32, [c_vp, c_vp]),
+        'dpmd_system_set_images': (c_i32, [c_vp, c_vp]),
+        'dpmd_system_set_rng': (c_i32, [c_vp, c_vp, c_vp]),
+        'dpmd_system_get_rng': (c_i32, [c_vp, c_vp, c_vp]),
         'dpmd_system_set_positions': (c_i32, [c_vp, c_vp]),
         'dpmd_system_get_positions': (c_i32, [c_vp, c_vp]),
         'dpmd_system_set_velocities': (c_i32, [c_vp, c_vp]),
@@ -70,6 +76,8 @@ def lib():
         'dpmd_pimd_setup': (c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp]),
         'dpmd_kinetic_energy': (c_i32, [c_vp, c_vp]),
         'dpmd_thermostat_energy': (c_i32, [c_vp, c_vp]),
+        'dpmd_thermostat_state_get': (c_i32, [c_vp, c_vp]),
+        'dpmd_thermostat_state_set': (c_i32, [c_vp, c_vp]),
         'dpmd_timing_enable': (c_i32, [c_vp, c_i32]),
         'dpmd_timing_get': (c_i32, [c_vp, c_i32, c_vp, c_vp]),
         'dpmd_pair_count': (c_i32, [c_vp, c_vp]),
@@ -101,10 +109,13 @@ def exported_symbols():
     """Names of the C-ABI entry points bound here (all declared in include/dpmd.h)."""
     lib()
     return ['dpmd_last_error', 'dpmd_version', 'dpmd_model_create', 'dpmd_model_destroy', 'dpmd_system_create',
-            'dpmd_system_destroy', 'dpmd_system_set_box', 'dpmd_system_set_positions', 'dpmd_system_get_positions',
+            'dpmd_system_create_ragged', 'dpmd_system_reshape', 'dpmd_system_destroy', 'dpmd_system_set_box',
+            'dpmd_system_set_boxes', 'dpmd_system_set_images', 'dpmd_system_set_rng', 'dpmd_system_get_rng',
+            'dpmd_system_set_positions', 'dpmd_system_get_positions',
             'dpmd_system_set_velocities', 'dpmd_system_get_velocities', 'dpmd_system_set_masses',
             'dpmd_nlist_allocate', 'dpmd_nlist_update', 'dpmd_nlist_export', 'dpmd_nlist_capacity', 'dpmd_compute',
-            'dpmd_md_run', 'dpmd_pimd_setup', 'dpmd_kinetic_energy', 'dpmd_thermostat_energy', 'dpmd_timing_enable',
+            'dpmd_md_run', 'dpmd_pimd_setup', 'dpmd_kinetic_energy', 'dpmd_thermostat_energy',
+            'dpmd_thermostat_state_get', 'dpmd_thermostat_state_set', 'dpmd_timing_enable',
             'dpmd_timing_get', 'dpmd_pair_count', 'dpmd_synchronize']
 
 
@@ -175,17 +186,57 @@ class DeviceModel:
 
 
 class DeviceSystem:
-    """natoms x nreplica atoms resident on the GPU, in the library's type-sorted layout."""
+    """Atoms resident on the GPU in the library's type-sorted layout.
 
-    def __init__(self, model, type_idx, nreplica=1):
+    Two forms: ``DeviceSystem(model, type_idx, nreplica)`` holds nreplica copies of one
+    composition (replicas, ring-polymer beads; arrays shaped (nreplica, natoms, ...)), and
+    ``DeviceSystem(model, segments=[type_idx_0, type_idx_1, ...])`` holds independent
+    systems of uneven size (arrays flat over all atoms, segments concatenated).
+    """
+
+    def __init__(self, model, type_idx=None, nreplica=1, segments=None):
         self.model = model
-        self.type_idx = np.ascontiguousarray(np.asarray(type_idx, np.int32))
-        self.natoms = len(self.type_idx)
-        self.nrep = int(nreplica)
         h = c_vp()
-        check(lib().dpmd_system_create(model.handle, self.natoms, self.nrep, ptr(self.type_idx), ctypes.byref(h)))
+        if segments is None:
+            self.ragged = False
+            self.type_idx = np.ascontiguousarray(np.asarray(type_idx, np.int32))
+            self.natoms = len(self.type_idx)
+            self.nrep = int(nreplica)
+            self.seg_natoms = [self.natoms] * self.nrep
+            check(lib().dpmd_system_create(model.handle, self.natoms, self.nrep, ptr(self.type_idx),
+                                           ctypes.byref(h)))
+        else:
+            self.handle = None
+            self._layout(segments)
+            check(lib().dpmd_system_create_ragged(model.handle, self.nrep, ptr(np.asarray(self.seg_natoms, np.int32)),
+                                                  ptr(self.type_idx), ctypes.byref(h)))
         self.handle = h
         self.knbr = None
+
+    def _layout(self, segments):
+        segs = [np.asarray(t, np.int32).reshape(-1) for t in segments]
+        if not segs:
+            raise ValueError('need at least one segment')
+        self.ragged = True
+        self.seg_natoms = [len(t) for t in segs]
+        self.type_idx = np.ascontiguousarray(np.concatenate(segs))
+        self.natoms = len(self.type_idx)
+        self.nrep = len(segs)
+        self.seg_offsets = np.concatenate([[0], np.cumsum(self.seg_natoms)]).astype(np.int64)
+
+    def reshape(self, segments):
+        """Lay the system out for new segments, reusing device buffers (all state is reset)."""
+        self._layout(segments)
+        check(lib().dpmd_system_reshape(self.handle, self.nrep, ptr(np.asarray(self.seg_natoms, np.int32)),
+                                        ptr(self.type_idx)))
+        self.knbr = None
+
+    def split(self, arr):
+        """Per-segment views of a flat (natoms_total, ...) array of a ragged system."""
+        return [arr[self.seg_offsets[k]:self.seg_offsets[k + 1]] for k in range(self.nrep)]
+
+    def _shape(self, *tail):
+        return ((self.natoms,) if self.ragged else (self.nrep, self.natoms)) + tail
 
     def __del__(self):
         if getattr(self, 'handle', None) and _lib is not None:
@@ -196,26 +247,49 @@ class DeviceSystem:
         b = np.ascontiguousarray(np.asarray(box3, np.float64).reshape(3))
         check(lib().dpmd_system_set_box(self.handle, ptr(b)))
 
+    def set_boxes(self, boxes):
+        b = np.ascontiguousarray(np.asarray(boxes, np.float64).reshape(self.nrep, 3))
+        check(lib().dpmd_system_set_boxes(self.handle, ptr(b)))
+
+    def set_images(self, nimg):
+        n = np.ascontiguousarray(np.asarray(nimg, np.int32).reshape(self.nrep))
+        check(lib().dpmd_system_set_images(self.handle, ptr(n)))
+
+    def set_rng(self, seeds=None, steps=None):
+        sd = None if seeds is None else np.ascontiguousarray(np.asarray(seeds, np.uint64).reshape(self.nrep))
+        st = None if steps is None else np.ascontiguousarray(np.asarray(steps, np.uint64).reshape(self.nrep))
+        check(lib().dpmd_system_set_rng(self.handle, ptr(sd), ptr(st)))
+
+    def get_rng(self):
+        sd = np.zeros(self.nrep, np.uint64)
+        st = np.zeros(self.nrep, np.uint64)
+        check(lib().dpmd_system_get_rng(self.handle, ptr(sd), ptr(st)))
+        return sd, st
+
     def set_positions(self, pos):
-        p = np.ascontiguousarray(np.asarray(pos, np.float64).reshape(self.nrep, self.natoms, 3))
+        p = np.ascontiguousarray(np.asarray(pos, np.float64).reshape(self._shape(3)))
         check(lib().dpmd_system_set_positions(self.handle, ptr(p)))
 
     def get_positions(self):
-        p = np.empty((self.nrep, self.natoms, 3))
+        p = np.empty(self._shape(3))
         check(lib().dpmd_system_get_positions(self.handle, ptr(p)))
         return p
 
     def set_velocities(self, vel):
-        v = np.ascontiguousarray(np.asarray(vel, np.float64).reshape(self.nrep, self.natoms, 3))
+        v = np.ascontiguousarray(np.asarray(vel, np.float64).reshape(self._shape(3)))
         check(lib().dpmd_system_set_velocities(self.handle, ptr(v)))
 
     def get_velocities(self):
-        v = np.empty((self.nrep, self.natoms, 3))
+        v = np.empty(self._shape(3))
         check(lib().dpmd_system_get_velocities(self.handle, ptr(v)))
         return v
 
     def set_masses(self, mass):
-        m = np.ascontiguousarray(np.asarray(mass, np.float64).reshape(self.natoms))
+        """Per-atom masses: (natoms) of one replica for the replica form, all atoms otherwise."""
+        m = np.asarray(mass, np.float64).reshape(-1)
+        if not self.ragged:
+            m = np.tile(m.reshape(self.natoms), self.nrep)
+        m = np.ascontiguousarray(m.reshape(self.nrep * self.natoms if not self.ragged else self.natoms))
         check(lib().dpmd_system_set_masses(self.handle, ptr(m)))
 
     def nlist_allocate(self, rc_skin, buffer_ratio=1.2, knbr_override=None):
@@ -232,21 +306,21 @@ class DeviceSystem:
 
     def nlist_export(self, j):
         cap = self.knbr[j] - 1
-        out = np.empty((self.nrep, self.natoms, max(cap, 1)), np.int32)
+        out = np.empty(self._shape(max(cap, 1)), np.int32)
         check(lib().dpmd_nlist_export(self.handle, int(j), ptr(out)))
         return out[..., :cap]
 
     def compute(self, precision='fp32', want_atom=False):
         e = np.zeros(self.nrep)
-        f = np.zeros((self.nrep, self.natoms, 3))
+        f = np.zeros(self._shape(3))
         v = np.zeros((self.nrep, 9))
-        ea = np.zeros((self.nrep, self.natoms)) if want_atom else None
+        ea = np.zeros(self._shape()) if want_atom else None
         check(lib().dpmd_compute(self.handle, PREC[precision], ptr(e), ptr(f), ptr(v), ptr(ea)))
         return e, f, v.reshape(self.nrep, 3, 3), ea
 
     def md_run(self, params, nsteps, record=True):
-        tp = np.empty((nsteps, self.nrep, self.natoms, 3)) if record else None
-        tv = np.empty((nsteps, self.nrep, self.natoms, 3)) if record else None
+        tp = np.empty((nsteps,) + self._shape(3)) if record else None
+        tv = np.empty((nsteps,) + self._shape(3)) if record else None
         err = np.zeros(1, np.uint32)
         check(lib().dpmd_md_run(self.handle, ctypes.byref(params), int(nsteps), ptr(tp), ptr(tv), ptr(err)))
         return tp, tv, int(err[0])
@@ -264,6 +338,16 @@ class DeviceSystem:
         e = np.zeros(self.nrep)
         check(lib().dpmd_thermostat_energy(self.handle, ptr(e)))
         return e
+
+    def thermostat_state(self, state=None):
+        """Get (state=None) or set the 64-double per-segment thermostat state."""
+        if state is None:
+            st = np.zeros((self.nrep, 64))
+            check(lib().dpmd_thermostat_state_get(self.handle, ptr(st)))
+            return st
+        st = np.ascontiguousarray(np.asarray(state, np.float64).reshape(self.nrep, 64))
+        check(lib().dpmd_thermostat_state_set(self.handle, ptr(st)))
+        return None
 
     def timing(self, on=None, which=None):
         if on is not None:

@@ -93,20 +93,47 @@ int32_t dpmd_model_destroy(dpmd_model_t model);
 
 /* One simulated system of natoms atoms x nreplica replicas (ring-polymer beads share
  * the composition).  Replaces the state set-up of Simulation.__init__
- * (deepmd_jax/md.py:61-283).  type_idx: (natoms) in [0, ntypes). */
+ * (deepmd_jax/md.py:61-283).  type_idx: (natoms) in [0, ntypes).  Replica r is segment r
+ * of the ragged layout below. */
 int32_t dpmd_system_create(dpmd_model_t model, int32_t natoms, int32_t nreplica,
                            const int32_t* type_idx, dpmd_system_t* out);
+/* nseg independent periodic systems ("segments") of uneven size evaluated and integrated
+ * together: every kernel launch covers all segments.  seg_natoms (nseg); type_idx
+ * (sum seg_natoms) caller order, segments concatenated.  Generalises the per-frame batch
+ * of evaluate() (deepmd_jax/train.py:703-788) to frames of different sizes. */
+int32_t dpmd_system_create_ragged(dpmd_model_t model, int32_t nseg, const int32_t* seg_natoms,
+                                  const int32_t* type_idx, dpmd_system_t* out);
+/* Re-lay the system out for a new set of segments, reusing its device buffers when they
+ * are large enough (grow-only).  All state (positions, velocities, masses, boxes, thermostat,
+ * neighbour list) is reset and must be set again.  No reference counterpart: the reference
+ * re-traces its jit functions per shape. */
+int32_t dpmd_system_reshape(dpmd_system_t sys, int32_t nseg, const int32_t* seg_natoms,
+                            const int32_t* type_idx);
 int32_t dpmd_system_destroy(dpmd_system_t sys);
 
-/* Orthorhombic box lengths (3). */
+/* Orthorhombic box lengths (3), applied to every segment. */
 int32_t dpmd_system_set_box(dpmd_system_t sys, const double* box3);
-/* (nreplica, natoms, 3) caller order.  setPosition / getPosition (deepmd_jax/md.py:969-1014). */
+/* Per-segment box lengths (nseg, 3). */
+int32_t dpmd_system_set_boxes(dpmd_system_t sys, const double* boxes);
+/* Per-segment count of periodic images held (nseg): a segment that holds the k^3 images
+ * of a box smaller than 2 rc (the supercell equivalent of the reference's lattice-image
+ * branch, deepmd_jax/utils.py:66-78) lists its atoms image-major; energies, virials,
+ * kinetic energies and thermostat work are then reported per image and all images draw
+ * the same thermostat noise. */
+int32_t dpmd_system_set_images(dpmd_system_t sys, const int32_t* nimg);
+/* Per-segment stochastic stream (nseg each, either may be NULL): the Langevin normals of
+ * a segment depend only on (params.seed, seed, step, atom), so a segment draws the same
+ * stream alone or in any batch.  steps counts integrator steps already taken. */
+int32_t dpmd_system_set_rng(dpmd_system_t sys, const uint64_t* seeds, const uint64_t* steps);
+int32_t dpmd_system_get_rng(dpmd_system_t sys, uint64_t* seeds, uint64_t* steps);
+/* (sum seg_natoms, 3) caller order.  setPosition / getPosition (deepmd_jax/md.py:969-1014). */
 int32_t dpmd_system_set_positions(dpmd_system_t sys, const double* pos);
 int32_t dpmd_system_get_positions(dpmd_system_t sys, double* pos);
 /* Velocities (Angstrom/fs); setVelocity / getVelocity (deepmd_jax/md.py:1016-1031). */
 int32_t dpmd_system_set_velocities(dpmd_system_t sys, const double* vel);
 int32_t dpmd_system_get_velocities(dpmd_system_t sys, double* vel);
-/* Per-atom masses (natoms) in eV fs^2 / Angstrom^2. */
+/* Per-atom masses (sum seg_natoms) in eV fs^2 / Angstrom^2, caller order.  Set masses
+ * before velocities (momenta are stored). */
 int32_t dpmd_system_set_masses(dpmd_system_t sys, const double* mass);
 
 /* Typed neighbour list at cutoff rc_skin.  Replaces typed_neighbor_list's allocate_fn
@@ -119,8 +146,9 @@ int32_t dpmd_nlist_allocate(dpmd_system_t sys, double rc_skin, double buffer_rat
  * *overflow = 1 when some row needed more than knbr_j - 1 entries. */
 int32_t dpmd_nlist_update(dpmd_system_t sys, int32_t* overflow);
 /* Export rows for neighbour type j as nbrs_nm (deepmd_jax/md_utils.py:431-459):
- * out (nreplica, natoms, knbr_j - 1) int32, centres in type-sorted order, entries are
- * ascending type-local indices padded with natoms. */
+ * out (sum seg_natoms, knbr_j - 1) int32, centres in type-sorted order per segment,
+ * entries are ascending type-local indices (inside the segment) padded with the
+ * segment's atom count. */
 int32_t dpmd_nlist_export(dpmd_system_t sys, int32_t j, int32_t* out);
 /* Current capacities knbr (ntypes). */
 int32_t dpmd_nlist_capacity(dpmd_system_t sys, int32_t* knbr);
@@ -129,8 +157,8 @@ int32_t dpmd_nlist_capacity(dpmd_system_t sys, int32_t* knbr);
  * DPModel.energy_and_force (deepmd_jax/dpmodel.py:106-108) and the strain derivative
  * of iso_pressure (deepmd_jax/md_utils.py:139-159).  Needs an up-to-date neighbour
  * list (dpmd_nlist_allocate / _update).
- * energy (nreplica); force (nreplica, natoms, 3) caller order; virial (nreplica, 9)
- * row-major, virial = -sum_pairs x_ij (outer) dE/dx_ij; atom_energy (nreplica, natoms).
+ * energy (nseg); force (sum seg_natoms, 3) caller order; virial (nseg, 9)
+ * row-major, virial = -sum_pairs x_ij (outer) dE/dx_ij; atom_energy (sum seg_natoms).
  * Any output pointer may be NULL. */
 int32_t dpmd_compute(dpmd_system_t sys, int32_t precision, double* energy, double* force,
                      double* virial, double* atom_energy);
@@ -139,7 +167,7 @@ int32_t dpmd_compute(dpmd_system_t sys, int32_t precision, double* energy, doubl
  * neighbour rebuild (deepmd_jax/md.py:693-741), the integrator step with one force
  * evaluation, minimum-image unwrap (deepmd_jax/md_utils.py:162-169).  Replaces the
  * jit(lax.scan(inner_step)) chunk of Simulation.run (deepmd_jax/md.py:759-804).
- * traj_pos / traj_vel: (nsteps, nreplica, natoms, 3) caller order or NULL.
+ * traj_pos / traj_vel: (nsteps, sum seg_natoms, 3) caller order or NULL.
  * *error_code accumulates the reference's bit codes. */
 int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps,
                     double* traj_pos, double* traj_vel, uint32_t* error_code);
@@ -148,10 +176,16 @@ int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps,
  * factors c1 = exp(-gamma_k dt), c2 = sqrt(P kT (1 - c1^2)) (P).  P = nreplica. */
 int32_t dpmd_pimd_setup(dpmd_system_t sys, const double* C, const double* freqs, const double* c1,
                         const double* c2);
-/* Kinetic energy (nreplica) in FP64 from the device momenta. */
+/* Kinetic energy (nseg) in FP64 from the device momenta. */
 int32_t dpmd_kinetic_energy(dpmd_system_t sys, double* ke);
-/* Thermostat work / chain state scalar used by the reference invariants (nreplica). */
+/* Thermostat work / chain state scalar used by the reference invariants (nseg):
+ * thermostat_work of NVTLangevinStateWithWork (deepmd_jax/routine.py:27-107), the chain
+ * energy of nvt_nose_hoover_invariant (deepmd_jax/routine.py:9-25). */
 int32_t dpmd_thermostat_energy(dpmd_system_t sys, double* e);
+/* Whole thermostat state, 64 doubles per segment, to carry a segment into another system
+ * (no reference counterpart: the reference keeps it in its scan carry). */
+int32_t dpmd_thermostat_state_get(dpmd_system_t sys, double* state);
+int32_t dpmd_thermostat_state_set(dpmd_system_t sys, const double* state);
 
 /* Profiling: enable HIP-event timing of each kernel class on the system's stream;
  * which = 0 nlist, 1 embed_fwd, 2 fit, 3 embed_bwd, 4 integrator.  ms = summed event
