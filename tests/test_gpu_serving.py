@@ -4,7 +4,7 @@ Tolerances:
   same request alone vs in a batch: energy |dE| <= 1e-6 |E| + 1e-6 eV (the per-segment
   energy reduction does not depend on the batch), forces max |dF| <= 1e-5 max |F|
   (FP32 force atomics accumulate in a launch-dependent order), virial <= 1e-5 max |W|;
-  MD end state after 30 Langevin steps: positions <= 1e-4 A, velocities <= 1e-4 A/fs.
+  MD end state after 40-90 Langevin steps: positions <= 1e-4 A, velocities <= 1e-4 A/fs.
   small boxes held as supercells vs the FP64 oracle's lattice-image sum: the plain
   FP32 tolerances of test_gpu_parity.py.
 Neighbour lists of a segment: bit-exact against the same box alone.
@@ -89,13 +89,13 @@ def test_small_box_images_vs_oracle(device_model, model0):
 def _requests(seed=7):
     from deepmd_jax import synthetic as syn
     from deepmd_jax.serving import Request
-    spec = [('eval', 40), ('md', 40), ('eval', 90), ('md', 120), ('eval', 200), ('md', 75)]
+    spec = [('eval', 40, 0), ('md', 40, 60), ('eval', 90, 0), ('md', 120, 90), ('eval', 200, 0), ('md', 75, 40)]
     reqs = []
-    for i, (kind, n) in enumerate(spec):
+    for i, (kind, n, steps) in enumerate(spec):
         c, b, t = syn.water_box(n, seed=seed + i, permute_sites=True)
         v = syn.maxwell_boltzmann(t, 300.0, seed=seed + i) if kind == 'md' else None
-        reqs.append(Request(rid=i, kind=kind, coord=c, box=b, type_idx=t, nsteps=30 if kind == 'md' else 0,
-                            velocity=v, seed=100 + i, arrival=0.0))
+        reqs.append(Request(rid=i, kind=kind, coord=c, box=b, type_idx=t, nsteps=steps, velocity=v,
+                            seed=100 + i, arrival=0.0))
     return reqs
 
 
@@ -120,16 +120,19 @@ def test_serving_alone_vs_mix(device_model):
     reqs = _requests()
     srv = Server(device_model, md_chunk=10)
     try:
-        # staggered: the second and third MD requests join a batch that is already running
-        for r in reqs:
+        for r in reqs[:-1]:
             srv.submit(r)
-            time.sleep(0.02)
+        # the last MD request joins a batch that is already running
+        t_end = time.monotonic() + 60
+        while srv.stats['md_chunks'] < 2 and time.monotonic() < t_end:
+            time.sleep(0.001)
+        srv.submit(reqs[-1])
         assert srv.wait(len(reqs), timeout=300)
     finally:
         srv.close()
     mixed = dict(srv.results)
-    assert srv.stats['max_md_segments'] >= 2
     print('stats', srv.stats, flush=True)
+    assert srv.stats['max_md_segments'] >= 2 and srv.stats['md_relayouts'] >= 2
     for r in _requests():
         one = Server(device_model, md_chunk=10)
         try:
