@@ -94,6 +94,40 @@ def accuracy(params, var, model):
     return res
 
 
+def serving(model, rates, precisions, n_requests=256, md_chunk=20, budget_gib=32):
+    """The serving stream: 256 water-box requests (n log-uniform in [32, 8192] H2O, first n
+    sites of a seeded grid permutation), 70 % single E/F/virial evaluations, 30 % 200-step
+    NVT Langevin segments at 300 K, dt 0.5 fs; Poisson arrivals at each rate (requests/s).
+    Reports p50/p99 latency (arrival -> result) and atoms*steps/s (evaluation = 1 step)."""
+    from deepmd_jax.serving import Server, request_stream, replay, summarize
+    reqs = request_stream(n_requests, seed=4, rate=1.0)
+    out = {'stream': '%d requests, n log-uniform [32, 8192] H2O (96-24,576 atoms), 70%% E/F/virial, '
+                     '30%% 200-step NVT Langevin 300 K dt 0.5 fs; Poisson arrivals' % n_requests,
+           'hbm_budget_GiB': budget_gib, 'md_chunk_steps': md_chunk,
+           'atoms_total': int(sum(r.natoms for r in reqs)),
+           'atoms_steps_total': int(sum(r.natoms * max(1, r.nsteps) for r in reqs))}
+    for prec in precisions:
+        warm = Server(model, precision=prec, md_chunk=md_chunk, hbm_budget=budget_gib << 30)
+        replay(warm, reqs[:8], time_scale=0.0)
+        warm.close()
+        res = {}
+        for rate in rates:
+            srv = Server(model, precision=prec, md_chunk=md_chunk, hbm_budget=budget_gib << 30)
+            t0 = time.perf_counter()
+            results = replay(srv, reqs, time_scale=1.0 / rate)
+            wall = time.perf_counter() - t0
+            srv.close()
+            sm = summarize(results)
+            sm['wall_s'] = wall
+            sm['stats'] = srv.stats
+            res['%g' % rate] = sm
+            print('# serving %s rate %g/s: p50 %.3f s p99 %.3f s, %.3g atoms*steps/s' %
+                  (prec, rate, sm['p50_latency_s'], sm['p99_latency_s'], sm['atoms_steps_per_s']),
+                  file=sys.stderr, flush=True)
+        out[prec] = res
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -105,6 +139,9 @@ def main():
     ap.add_argument('--no-sim', action='store_true')
     ap.add_argument('--no-half', action='store_true')
     ap.add_argument('--no-acc', action='store_true')
+    ap.add_argument('--no-serving', action='store_true')
+    ap.add_argument('--serving-rates', default='20', help='comma-separated Poisson arrival rates (requests/s)')
+    ap.add_argument('--serving-requests', type=int, default=256)
     ap.add_argument('--calibrate-copy', action='store_true',
                     help='run a plain 256 MiB elementwise copy (read 256 MiB, write 256 MiB) five times, for '
                          'scaling FETCH_SIZE / WRITE_SIZE counters')
@@ -241,6 +278,9 @@ def main():
         out['cpu_baseline'] = None
     if not args.no_acc:
         out['precision_errors_vs_fp64_oracle'] = accuracy(params, var, model)
+    if not args.no_serving:
+        rates = [float(x) for x in args.serving_rates.split(',') if x]
+        out['serving'] = serving(model, rates, ('fp32', 'bf16'), args.serving_requests)
     print(json.dumps(out))
 
 

@@ -27,11 +27,12 @@ def water_box_length(n_molecules, density=1.0):
     return (vol_cm3 * 1e24) ** (1.0 / 3.0)
 
 
-def _haar_rotation(rng):
-    q, r = np.linalg.qr(rng.standard_normal((3, 3)))
-    q = q * np.sign(np.diag(r))[None, :]
-    if np.linalg.det(q) < 0:
-        q[:, 0] = -q[:, 0]
+def _haar_rotations(rng, n):
+    """n Haar-random rotations; draws the same normals as n successive (3,3) draws."""
+    q, r = np.linalg.qr(rng.standard_normal((n, 3, 3)))
+    q = q * np.sign(np.diagonal(r, axis1=1, axis2=2))[:, None, :]
+    neg = np.linalg.det(q) < 0
+    q[neg, :, 0] = -q[neg, :, 0]
     return q
 
 
@@ -53,11 +54,11 @@ def water_box(n_molecules, seed=0, jitter=0.1, density=1.0, permute_sites=False)
     half = np.deg2rad(HOH_ANGLE_DEG / 2)
     local = np.array([[OH_BOND * np.sin(half), OH_BOND * np.cos(half), 0.0],
                       [-OH_BOND * np.sin(half), OH_BOND * np.cos(half), 0.0]])
-    coord = np.empty((3 * n_molecules, 3))
-    for m in range(n_molecules):
-        R = _haar_rotation(rng)
-        coord[3 * m] = sites[m]
-        coord[3 * m + 1:3 * m + 3] = sites[m] + local @ R.T
+    R = _haar_rotations(rng, n_molecules)
+    coord = np.empty((n_molecules, 3, 3))
+    coord[:, 0] = sites
+    coord[:, 1:] = sites[:, None, :] + np.matmul(local[None], R.transpose(0, 2, 1))
+    coord = coord.reshape(3 * n_molecules, 3)
     coord += rng.uniform(-jitter, jitter, coord.shape)
     coord = np.mod(coord, L)
     type_idx = np.tile(np.array([0, 1, 1]), n_molecules)
