@@ -94,6 +94,83 @@ def accuracy(params, var, model):
     return res
 
 
+def pimd(model, params, var, steps=100, warmup=10, precisions=('fp32', 'bf16'), acc_beads=4):
+    """Ring-polymer NVT (PILE-L, 300 K, dt 0.25 fs): 32 beads x 4,096 H2O (12,288 atoms per
+    bead, L = 49.7 A), per-bead +-0.05 A uniform jitter from seed 3, all beads in one
+    system (one launch per kernel for all 393,216 atoms).  Device-resident bead*steps/s per
+    precision, and the centroid-force error against the FP64 oracle over the first
+    `acc_beads` beads of the initial configuration."""
+    from deepmd_jax import _lib, synthetic as syn
+    from deepmd_jax.constants import MASS_UNIT_CONVERSION, TEMP_UNIT_CONVERSION
+    from deepmd_jax.pimd import normal_mode_transform, pile_l_factors
+    P, dt = 32, 0.25
+    coord, box, types = syn.water_box(4096, seed=3)
+    rng = np.random.default_rng(3)
+    x = coord[None] + rng.uniform(-0.05, 0.05, (P,) + coord.shape)
+    m = np.array(syn.MASSES)[types] * MASS_UNIT_CONVERSION
+    kT = 300 * TEMP_UNIT_CONVERSION
+    v = rng.standard_normal(x.shape) * np.sqrt(kT * P / m)[None, :, None]
+    freqs, C = normal_mode_transform(P, kT)
+    _, c1, c2 = pile_l_factors(freqs, dt, kT, P, 100.0)
+    s = _lib.DeviceSystem(model, types, P)
+    s.set_box(box)
+    s.set_masses(m)
+    s.pimd_setup(C, freqs, c1, c2)
+    p = _lib.MDParams()
+    p.routine, p.dt, p.kT, p.seed, p.gamma, p.tau = _lib.ROUTINE['PIMD'], dt, kT, 3, 0.01, 100.0
+    N = len(types)
+    out = {'workload': '32 beads x 4,096 H2O (12,288 atoms per bead, L=%.2f A), PILE-L 300 K, dt 0.25 fs, '
+                       'skin 1 A' % box[0], 'atoms_total': P * N}
+
+    def reset():
+        s.set_positions(x)
+        s.set_velocities(v)
+        s.nlist_allocate(params['rcut'] + 1.0, 1.2)
+
+    def run(n):
+        done = 0
+        while done < n:
+            k = min(50, n - done)
+            _, _, err = s.md_run(p, k, record=False)
+            if err & 2:
+                s.nlist_allocate(params['rcut'] + 1.0, 1.25)
+                continue
+            if err & 16:
+                raise RuntimeError('non-finite state in the PIMD run')
+            done += k
+    for prec in precisions:
+        p.precision = _lib.PREC[prec]
+        reset()
+        run(warmup)
+        s.synchronize()
+        t0 = time.perf_counter()
+        run(steps)
+        s.synchronize()
+        el = time.perf_counter() - t0
+        out[prec] = {'steps_per_s': steps / el, 'bead_steps_per_s': P * steps / el,
+                     'atoms_steps_per_s': P * N * steps / el, 'ms_per_step': 1e3 * el / steps, 'steps': steps}
+    if acc_beads:
+        import torch
+        sys.path.insert(0, ROOT)
+        from oracle import dp_oracle as O
+        torch.set_num_threads(len(os.sched_getaffinity(0)))
+        reset()
+        Fo = []
+        for b in range(acc_beads):
+            nb, _, _ = O.typed_nlist(x[b], box, types, params['rcut'])
+            Fo.append(O.energy_force_virial(params, var, x[b], box, types, nbrs_nm=nb,
+                                            dtype=torch.float64)[1].numpy())
+        Fc = np.mean(Fo, 0)
+        out['centroid_force'] = {'beads': acc_beads, 'oracle': 'FP64, per bead',
+                                 'centroid_force_rms_eV_per_A': float(np.sqrt((Fc ** 2).mean()))}
+        for prec in precisions:
+            _, f, _, _ = s.compute(prec)
+            fc = f[:acc_beads].mean(0)
+            out['centroid_force'][prec] = {'rmse_eV_per_A': float(np.sqrt(((fc - Fc) ** 2).mean())),
+                                           'max_abs_err': float(np.abs(fc - Fc).max())}
+    return out
+
+
 def serving(model, rates, precisions, n_requests=256, md_chunk=20, budget_gib=32):
     """The serving stream: 256 water-box requests (n log-uniform in [32, 8192] H2O, first n
     sites of a seeded grid permutation), 70 % single E/F/virial evaluations, 30 % 200-step
@@ -140,6 +217,9 @@ def main():
     ap.add_argument('--no-half', action='store_true')
     ap.add_argument('--no-acc', action='store_true')
     ap.add_argument('--no-serving', action='store_true')
+    ap.add_argument('--no-pimd', action='store_true')
+    ap.add_argument('--pimd-steps', type=int, default=100)
+    ap.add_argument('--pimd-acc-beads', type=int, default=2)
     ap.add_argument('--serving-rates', default='20', help='comma-separated Poisson arrival rates (requests/s)')
     ap.add_argument('--serving-requests', type=int, default=256)
     ap.add_argument('--calibrate-copy', action='store_true',
@@ -278,6 +358,8 @@ def main():
         out['cpu_baseline'] = None
     if not args.no_acc:
         out['precision_errors_vs_fp64_oracle'] = accuracy(params, var, model)
+    if not args.no_pimd:
+        out['pimd'] = pimd(model, params, var, steps=args.pimd_steps, acc_beads=0 if args.no_acc else args.pimd_acc_beads)
     if not args.no_serving:
         rates = [float(x) for x in args.serving_rates.split(',') if x]
         out['serving'] = serving(model, rates, ('fp32', 'bf16'), args.serving_requests)
