@@ -289,7 +289,7 @@ class DeviceSystem:
         m = np.asarray(mass, np.float64).reshape(-1)
         if not self.ragged:
             m = np.tile(m.reshape(self.natoms), self.nrep)
-        m = np.ascontiguousarray(m.reshape(self.nrep * self.natoms if not self.ragged else self.natoms))
+        m = np.ascontiguousarray(m)
         check(lib().dpmd_system_set_masses(self.handle, ptr(m)))
 
     def nlist_allocate(self, rc_skin, buffer_ratio=1.2, knbr_override=None):
@@ -297,6 +297,7 @@ class DeviceSystem:
         ko = None if knbr_override is None else np.ascontiguousarray(np.asarray(knbr_override, np.int32))
         check(lib().dpmd_nlist_allocate(self.handle, float(rc_skin), float(buffer_ratio), ptr(ko), ptr(out)))
         self.knbr = [int(k) for k in out[:self.model.ntypes]]
+        self.rc_skin_last = float(rc_skin)
         return self.knbr
 
     def nlist_update(self):

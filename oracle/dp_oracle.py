@@ -349,7 +349,9 @@ def local_fd_forces(params, variables, coord, box, type_idx, atoms, h=1e-4):
 
     Moving atom i changes only the energies of atoms within rcut of i, whose
     environments lie within 2*rcut of i; each derivative is taken on that local
-    cluster (minimum-image coordinates around i, non-periodic big box)."""
+    cluster (minimum-image coordinates around i, non-periodic big box), with a
+    neighbour list at rcut + 0.2 built once per cluster (a displacement of h cannot
+    bring a pair from beyond rcut + 0.2 inside rcut)."""
     coord = np.asarray(coord, np.float64)
     box = np.asarray(box, np.float64).reshape(3)
     type_idx = np.asarray(type_idx)
@@ -369,13 +371,13 @@ def local_fd_forces(params, variables, coord, box, type_idx, atoms, h=1e-4):
         pos_in_sorted = np.empty(len(order), int)
         pos_in_sorted[order] = np.arange(len(order))
         sel = torch.as_tensor(sorted(pos_in_sorted[list(inner)]))
-        lat = lattice_candidate(np.diag(big), rc)
+        nb, _, _ = typed_nlist(cc, big, tt, rc + 0.2, ntypes=params['ntypes'])
         for dim in range(3):
             es = []
             for sgn in (1, -1):
                 c2 = cc.copy()
                 c2[ii, dim] += sgn * h
-                _, e, _, _ = dp_energy(params, variables, torch.as_tensor(c2), big, tt, lattice=lat, return_T=True)
+                _, e, _, _ = dp_energy(params, variables, torch.as_tensor(c2), big, tt, nbrs_nm=nb, return_T=True)
                 es.append(e[sel].sum().item())
             out[n, dim] = -(es[0] - es[1]) / (2 * h)
     return out

@@ -241,3 +241,41 @@ def test_default_topology_model():
         N = len(types)
         assert abs(e[0] - E) / N <= tol_e, (prec, e[0], E)
         assert np.sqrt(((f[0] - F) ** 2).mean()) <= tol_f * max(1.0, np.sqrt((F ** 2).mean())), prec
+
+
+def test_forces_98k_box_against_fd(device_model, model0):
+    """Full-size box (32,768 H2O, 98,304 atoms): device FP32 forces on 64 random atoms
+    against FP64 central differences (h = 1e-4 A) of the oracle energy.
+    Tolerance: max |dF| <= 1e-4 * max(1, max |F|)."""
+    from deepmd_jax import _lib, synthetic as syn
+    print('case fd 98k', flush=True)
+    params, var = model0
+    coord, box, types = syn.water_box(32768, seed=0)
+    coord = _f32(coord)
+    s = _system(device_model, coord, box, types, params['rcut'] + 1.0)
+    _, f, _, _ = s.compute('fp32')
+    atoms = np.sort(np.random.default_rng(11).choice(len(types), 64, replace=False))
+    fd = O.local_fd_forces(params, var, coord, box, types, atoms)
+    err = np.abs(f[0][atoms] - fd).max()
+    print('fd max err', err, 'max |F|', np.abs(fd).max(), flush=True)
+    assert err <= 1e-4 * max(1.0, np.abs(fd).max())
+
+
+def test_autograd_boundary(device_model, model0):
+    """torch.autograd through the device energy: dE/dx = -F, and the energy change
+    along a random displacement matches the gradient (first order)."""
+    import torch
+    from deepmd_jax import synthetic as syn
+    from deepmd_jax.autodiff import DeviceEnergy, energy_and_force
+    print('case autograd', flush=True)
+    params, var = model0
+    coord, box, types = syn.water_box(64, seed=2)
+    s = _system(device_model, coord, box, types, params['rcut'] + 0.2)
+    e, f = energy_and_force(s, coord)
+    _, f2, _, _ = s.compute('fp32')
+    np.testing.assert_allclose(f, f2[0], rtol=0, atol=1e-12)
+    d = np.random.default_rng(0).standard_normal(coord.shape) * 1e-3
+    c = torch.as_tensor(coord + d)
+    e2 = float(DeviceEnergy.apply(c, s, 'fp32'))
+    de_lin = -(f * d).sum()
+    assert abs((e2 - e) - de_lin) <= 0.05 * abs(de_lin) + 1e-4
