@@ -92,6 +92,7 @@ def accuracy(params, var, model, frames=1):
         s.set_positions(coord)
         s.nlist_allocate(params['rcut'], 1.2)
         frms.append(float(np.sqrt((F ** 2).mean())))
+        print('# accuracy frame %d done' % fr, file=sys.stderr, flush=True)
         for prec in per:
             e, f, v, _ = s.compute(prec)
             vals = (abs(e[0] - E) / len(types), float(np.sqrt(((f[0] - F) ** 2).mean())),
@@ -143,6 +144,8 @@ def nve_drift(model, params, coord, box, types, vel, steps, sample=100, precisio
             if err & 16:
                 break
             step += sample
+            if step % 1000 == 0:
+                print('# drift %s step %d H %.4f' % (prec, step, H[-1]), file=sys.stderr, flush=True)
         t, H = np.array(t), np.array(H)
         k = t <= 1.0 + 1e-9
         out[prec] = {'drift_eV_per_atom_per_ps': float(np.polyfit(t, H, 1)[0] / N),

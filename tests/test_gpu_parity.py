@@ -273,7 +273,8 @@ def test_autograd_boundary(device_model, model0):
     s = _system(device_model, coord, box, types, params['rcut'] + 0.2)
     e, f = energy_and_force(s, coord)
     _, f2, _, _ = s.compute('fp32')
-    np.testing.assert_allclose(f, f2[0], rtol=0, atol=1e-12)
+    # two evaluations differ only by the order of the FP32 force atomics
+    assert np.abs(f - f2[0]).max() <= 1e-5 * np.abs(f2[0]).max()
     d = np.random.default_rng(0).standard_normal(coord.shape) * 1e-3
     c = torch.as_tensor(coord + d)
     e2 = float(DeviceEnergy.apply(c, s, 'fp32'))
