@@ -19,6 +19,7 @@
 namespace dpmd {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef unsigned int uintx4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
 
@@ -134,32 +135,38 @@ __global__ void __launch_bounds__(256, 2) gemm_f32_kernel(const GemmArgs g) {
 #pragma unroll
   for (int t = 0; t < NT; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
   const int arow = tid / 4, ac4 = tid % 4;
-  float4 ra = *reinterpret_cast<const float4*>(g.A + (size_t)(m0 + arow) * g.lda + 4 * ac4);
-  float4 rb[NB];
-if constexpr (0 < NB) { const int idx = tid + 256 * 0; if (0 + 1 < NB || idx < BV) rb[0] = *reinterpret_cast<const float4*>(Bt + (size_t)(n0 + idx / 4) * g.ldb + 4 * (idx % 4)); }
-    if constexpr (1 < NB) { const int idx = tid + 256 * 1; if (1 + 1 < NB || idx < BV) rb[1] = *reinterpret_cast<const float4*>(Bt + (size_t)(n0 + idx / 4) * g.ldb + 4 * (idx % 4)); }
-    if constexpr (2 < NB) { const int idx = tid + 256 * 2; if (2 + 1 < NB || idx < BV) rb[2] = *reinterpret_cast<const float4*>(Bt + (size_t)(n0 + idx / 4) * g.ldb + 4 * (idx % 4)); }
-    if constexpr (3 < NB) { const int idx = tid + 256 * 3; if (3 + 1 < NB || idx < BV) rb[3] = *reinterpret_cast<const float4*>(Bt + (size_t)(n0 + idx / 4) * g.ldb + 4 * (idx % 4)); }
+  // staging registers are named vectors (an array of them was placed in scratch)
+  floatx4 ra, rb0, rb1, rb2, rb3;
+  auto bsrc = [&](int piece, int k) {
+    const int idx = tid + 256 * piece;
+    return reinterpret_cast<const floatx4*>(Bt + (size_t)(n0 + idx / 4) * g.ldb + k + 4 * (idx % 4));
+  };
+  auto bdst = [&](int piece) {
+    const int idx = tid + 256 * piece;
+    return reinterpret_cast<floatx4*>(&Bs[(idx / 4) * LS32 + 4 * (idx % 4)]);
+  };
+  auto live = [&](int piece) { return piece + 1 < NB || tid + 256 * piece < BV; };
+  auto fetch = [&](int k) {
+    ra = *reinterpret_cast<const floatx4*>(g.A + (size_t)(m0 + arow) * g.lda + k + 4 * ac4);
+    if constexpr (NB > 0) { if (live(0)) rb0 = *bsrc(0, k); }
+    if constexpr (NB > 1) { if (live(1)) rb1 = *bsrc(1, k); }
+    if constexpr (NB > 2) { if (live(2)) rb2 = *bsrc(2, k); }
+    if constexpr (NB > 3) { if (live(3)) rb3 = *bsrc(3, k); }
+  };
+  fetch(0);
   for (int k0 = 0; k0 < g.K; k0 += KC32) {
     __syncthreads();
-    *reinterpret_cast<float4*>(&As[arow * LS32 + 4 * ac4]) = ra;
-if constexpr (0 < NB) { const int idx = tid + 256 * 0; if (0 + 1 < NB || idx < BV) *reinterpret_cast<float4*>(&Bs[(idx / 4) * LS32 + 4 * (idx % 4)]) = rb[0]; }
-    if constexpr (1 < NB) { const int idx = tid + 256 * 1; if (1 + 1 < NB || idx < BV) *reinterpret_cast<float4*>(&Bs[(idx / 4) * LS32 + 4 * (idx % 4)]) = rb[1]; }
-    if constexpr (2 < NB) { const int idx = tid + 256 * 2; if (2 + 1 < NB || idx < BV) *reinterpret_cast<float4*>(&Bs[(idx / 4) * LS32 + 4 * (idx % 4)]) = rb[2]; }
-    if constexpr (3 < NB) { const int idx = tid + 256 * 3; if (3 + 1 < NB || idx < BV) *reinterpret_cast<float4*>(&Bs[(idx / 4) * LS32 + 4 * (idx % 4)]) = rb[3]; }
+    *reinterpret_cast<floatx4*>(&As[arow * LS32 + 4 * ac4]) = ra;
+    if constexpr (NB > 0) { if (live(0)) *bdst(0) = rb0; }
+    if constexpr (NB > 1) { if (live(1)) *bdst(1) = rb1; }
+    if constexpr (NB > 2) { if (live(2)) *bdst(2) = rb2; }
+    if constexpr (NB > 3) { if (live(3)) *bdst(3) = rb3; }
     __syncthreads();
-    const int k1 = k0 + KC32;
-    if (k1 < g.K) {
-      ra = *reinterpret_cast<const float4*>(g.A + (size_t)(m0 + arow) * g.lda + k1 + 4 * ac4);
-if constexpr (0 < NB) { const int idx = tid + 256 * 0; if (0 + 1 < NB || idx < BV) rb[0] = *reinterpret_cast<const float4*>(Bt + (size_t)(n0 + idx / 4) * g.ldb + k1 + 4 * (idx % 4)); }
-    if constexpr (1 < NB) { const int idx = tid + 256 * 1; if (1 + 1 < NB || idx < BV) rb[1] = *reinterpret_cast<const float4*>(Bt + (size_t)(n0 + idx / 4) * g.ldb + k1 + 4 * (idx % 4)); }
-    if constexpr (2 < NB) { const int idx = tid + 256 * 2; if (2 + 1 < NB || idx < BV) rb[2] = *reinterpret_cast<const float4*>(Bt + (size_t)(n0 + idx / 4) * g.ldb + k1 + 4 * (idx % 4)); }
-    if constexpr (3 < NB) { const int idx = tid + 256 * 3; if (3 + 1 < NB || idx < BV) rb[3] = *reinterpret_cast<const float4*>(Bt + (size_t)(n0 + idx / 4) * g.ldb + k1 + 4 * (idx % 4)); }
-    }
-    const float4 av = *reinterpret_cast<const float4*>(&As[(16 * wv + n) * LS32 + 4 * q]);
+    if (k0 + KC32 < g.K) fetch(k0 + KC32);
+    const floatx4 av = *reinterpret_cast<const floatx4*>(&As[(16 * wv + n) * LS32 + 4 * q]);
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) {
-      const float4 bv = *reinterpret_cast<const float4*>(&Bs[(16 * nt + n) * LS32 + 4 * q]);
+      const floatx4 bv = *reinterpret_cast<const floatx4*>(&Bs[(16 * nt + n) * LS32 + 4 * q]);
       acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, bv.x, acc[nt], 0, 0, 0);
       acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, bv.y, acc[nt], 0, 0, 0);
       acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.z, bv.z, acc[nt], 0, 0, 0);
@@ -208,17 +215,27 @@ __global__ void __launch_bounds__(256, 2) gemm_h_kernel(const GemmArgs g) {
 #pragma unroll
   for (int t = 0; t < NT; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
   const int arow = tid / 4, ac8 = tid % 4;  // 64 rows x 4 pieces of 8 elements
-  float4 ra0, ra1;
-  uint4 rb[NB];
-  {
-    const float* ap = g.A + (size_t)(m0 + arow) * g.lda + 8 * ac8;
-    ra0 = *reinterpret_cast<const float4*>(ap);
-    ra1 = *reinterpret_cast<const float4*>(ap + 4);
-  }
-if constexpr (0 < NB) { const int idx = tid + 256 * 0; if (0 + 1 < NB || idx < BV) rb[0] = *reinterpret_cast<const uint4*>(Bt + (size_t)(n0 + idx / 4) * g.ldb + 8 * (idx % 4)); }
-    if constexpr (1 < NB) { const int idx = tid + 256 * 1; if (1 + 1 < NB || idx < BV) rb[1] = *reinterpret_cast<const uint4*>(Bt + (size_t)(n0 + idx / 4) * g.ldb + 8 * (idx % 4)); }
-    if constexpr (2 < NB) { const int idx = tid + 256 * 2; if (2 + 1 < NB || idx < BV) rb[2] = *reinterpret_cast<const uint4*>(Bt + (size_t)(n0 + idx / 4) * g.ldb + 8 * (idx % 4)); }
-    if constexpr (3 < NB) { const int idx = tid + 256 * 3; if (3 + 1 < NB || idx < BV) rb[3] = *reinterpret_cast<const uint4*>(Bt + (size_t)(n0 + idx / 4) * g.ldb + 8 * (idx % 4)); }
+  floatx4 ra0, ra1;
+  uintx4 rb0, rb1, rb2, rb3;
+  auto bsrc = [&](int piece, int k) {
+    const int idx = tid + 256 * piece;
+    return reinterpret_cast<const uintx4*>(Bt + (size_t)(n0 + idx / 4) * g.ldb + k + 8 * (idx % 4));
+  };
+  auto bdst = [&](int piece) {
+    const int idx = tid + 256 * piece;
+    return reinterpret_cast<uintx4*>(&Bs[(idx / 4) * LS16 + 8 * (idx % 4)]);
+  };
+  auto live = [&](int piece) { return piece + 1 < NB || tid + 256 * piece < BV; };
+  auto fetch = [&](int k) {
+    const float* ap = g.A + (size_t)(m0 + arow) * g.lda + k + 8 * ac8;
+    ra0 = *reinterpret_cast<const floatx4*>(ap);
+    ra1 = *reinterpret_cast<const floatx4*>(ap + 4);
+    if constexpr (NB > 0) { if (live(0)) rb0 = *bsrc(0, k); }
+    if constexpr (NB > 1) { if (live(1)) rb1 = *bsrc(1, k); }
+    if constexpr (NB > 2) { if (live(2)) rb2 = *bsrc(2, k); }
+    if constexpr (NB > 3) { if (live(3)) rb3 = *bsrc(3, k); }
+  };
+  fetch(0);
   for (int k0 = 0; k0 < g.K; k0 += KC16) {
     __syncthreads();
     {
@@ -227,21 +244,12 @@ if constexpr (0 < NB) { const int idx = tid + 256 * 0; if (0 + 1 < NB || idx < B
       h[4] = (HE)ra1.x; h[5] = (HE)ra1.y; h[6] = (HE)ra1.z; h[7] = (HE)ra1.w;
       *reinterpret_cast<HT*>(&As[arow * LS16 + 8 * ac8]) = h;
     }
-if constexpr (0 < NB) { const int idx = tid + 256 * 0; if (0 + 1 < NB || idx < BV) *reinterpret_cast<uint4*>(&Bs[(idx / 4) * LS16 + 8 * (idx % 4)]) = rb[0]; }
-    if constexpr (1 < NB) { const int idx = tid + 256 * 1; if (1 + 1 < NB || idx < BV) *reinterpret_cast<uint4*>(&Bs[(idx / 4) * LS16 + 8 * (idx % 4)]) = rb[1]; }
-    if constexpr (2 < NB) { const int idx = tid + 256 * 2; if (2 + 1 < NB || idx < BV) *reinterpret_cast<uint4*>(&Bs[(idx / 4) * LS16 + 8 * (idx % 4)]) = rb[2]; }
-    if constexpr (3 < NB) { const int idx = tid + 256 * 3; if (3 + 1 < NB || idx < BV) *reinterpret_cast<uint4*>(&Bs[(idx / 4) * LS16 + 8 * (idx % 4)]) = rb[3]; }
+    if constexpr (NB > 0) { if (live(0)) *bdst(0) = rb0; }
+    if constexpr (NB > 1) { if (live(1)) *bdst(1) = rb1; }
+    if constexpr (NB > 2) { if (live(2)) *bdst(2) = rb2; }
+    if constexpr (NB > 3) { if (live(3)) *bdst(3) = rb3; }
     __syncthreads();
-    const int k1 = k0 + KC16;
-    if (k1 < g.K) {
-      const float* ap = g.A + (size_t)(m0 + arow) * g.lda + k1 + 8 * ac8;
-      ra0 = *reinterpret_cast<const float4*>(ap);
-      ra1 = *reinterpret_cast<const float4*>(ap + 4);
-if constexpr (0 < NB) { const int idx = tid + 256 * 0; if (0 + 1 < NB || idx < BV) rb[0] = *reinterpret_cast<const uint4*>(Bt + (size_t)(n0 + idx / 4) * g.ldb + k1 + 8 * (idx % 4)); }
-    if constexpr (1 < NB) { const int idx = tid + 256 * 1; if (1 + 1 < NB || idx < BV) rb[1] = *reinterpret_cast<const uint4*>(Bt + (size_t)(n0 + idx / 4) * g.ldb + k1 + 8 * (idx % 4)); }
-    if constexpr (2 < NB) { const int idx = tid + 256 * 2; if (2 + 1 < NB || idx < BV) rb[2] = *reinterpret_cast<const uint4*>(Bt + (size_t)(n0 + idx / 4) * g.ldb + k1 + 8 * (idx % 4)); }
-    if constexpr (3 < NB) { const int idx = tid + 256 * 3; if (3 + 1 < NB || idx < BV) rb[3] = *reinterpret_cast<const uint4*>(Bt + (size_t)(n0 + idx / 4) * g.ldb + k1 + 8 * (idx % 4)); }
-    }
+    if (k0 + KC16 < g.K) fetch(k0 + KC16);
     const HT av = *reinterpret_cast<const HT*>(&As[(16 * wv + n) * LS16 + 8 * q]);
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) {
