@@ -244,14 +244,13 @@ __device__ __forceinline__ void env_backward(const Geo& g, const float dR[10], f
   }
 }
 
-// tanh(x) = 1 - 2 / (1 + 2^(2x log2 e)) with the hardware exp2 and reciprocal; the odd
-// Taylor polynomial below |x| < 2^-8 keeps relative accuracy where 1 - (...) cancels.
-// Absolute error ~2e-7 over the real line.
+// tanh(x) = 1 - 2 / (1 + 2^(2x log2 e)) with the hardware exp2 and reciprocal, branch
+// free (five instructions, two of them transcendental).  Absolute error ~2e-7 over the
+// real line, saturating to +-1; near 0 the error stays absolute (no relative accuracy
+// is needed: every activation enters a sum with O(1) terms).
 __device__ __forceinline__ float tanh_f(float x) {
   const float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);
-  const float t = 1.f - 2.f * __builtin_amdgcn_rcpf(1.f + e);
-  const float x2 = x * x;
-  return fabsf(x) < 0.00390625f ? x * (1.f - x2 * (1.f / 3.f)) : t;
+  return __builtin_fmaf(-2.f, __builtin_amdgcn_rcpf(1.f + e), 1.f);
 }
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -410,8 +409,13 @@ __device__ __forceinline__ int compact_row(const EmbedArgs& a, const SegInfo& S,
 // ---------------------------------------------------------------------------------
 // Forward: T (10 x W) and G (A x W) per centre.  One wave per centre.
 
-template <int T1, int T2, int T3, int Z2, int Z3, int PREC>
+// V1: registers of layer 1 that hold a feature in some lane group (the rest are padding
+// in every group); layer 2 then has V2 = Z2 V1 such registers.  MFMA k-steps and tanh
+// evaluations over registers that are padding everywhere are skipped.
+template <int T1, int T2, int T3, int Z2, int Z3, int PREC, int V1>
 __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
+  constexpr int V2 = Z2 * V1;
+  static_assert(V1 <= 4 * T1 && V2 <= 4 * T2, "valid registers exceed the tiles");
   extern __shared__ float smem[];
   const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const rsrc_t wrs = __builtin_amdgcn_make_buffer_rsrc((void*)a.net, (short)0, a.net_bytes, 0x00020000);
@@ -446,12 +450,12 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
     pairs += m;
     wave_sync();
     // byte offsets of the net's tables (layout of embed_net_offsets)
-    const int netb = (ti * a.ntypes + j) * a.net_stride * 4;
+    const int netb = __builtin_amdgcn_readfirstlane((ti * a.ntypes + j) * a.net_stride * 4);
     constexpr int O_K0 = 0, O_B0 = 16 * T1, O_B1 = 32 * T1, O_B2 = 32 * T1 + 16 * T2;
     constexpr int P0 = ((16 * T1 * 2 + 16 * T2 + 16 * T3) + 63) / 64 * 64;
     constexpr int O_W1A = P0, O_W2A = O_W1A + T2 * 4 * T1 * 64;
     const int vl4 = lane * 4, vq16 = q * 16, vl16 = lane * 16;
-    const int hnetb = PREC ? (ti * a.ntypes + j) * a.hnet_stride : 0;
+    const int hnetb = PREC ? __builtin_amdgcn_readfirstlane((ti * a.ntypes + j) * a.hnet_stride) : 0;
     for (int ch = 0; ch < m; ch += 16) {
       const int nb = netb + opaque_zero();
       const int hb = hnetb + opaque_zero();
@@ -471,20 +475,23 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
       }
       float h1[4 * T1];
 #pragma unroll
-      for (int r = 0; r < 4 * T1; ++r) h1[r] = tanh_f(g.st * k0(r) + b0(r));
+      for (int r = 0; r < 4 * T1; ++r) h1[r] = r < V1 ? tanh_f(g.st * k0(r) + b0(r)) : 0.f;
       floatx4 acc2[T2];
 #pragma unroll
       for (int t = 0; t < T2; ++t) acc2[t] = floatx4{0.f, 0.f, 0.f, 0.f};
       if constexpr (PREC == 0)
-        mfma_chain<T2, 4 * T1, 0, 4 * T1>(wrs, nb + 4 * O_W1A, vl4, [&](int s_) { return h1[s_]; }, acc2);
+        mfma_chain<T2, 4 * T1, 0, V1>(wrs, nb + 4 * O_W1A, vl4, [&](int s_) { return h1[s_]; }, acc2);
       else
         mfma_chain_h<PREC, T2, S21, 0, S21>(hrs, hb + H_W1A, vl16, [&](int s2) {
           return pack8<PREC, 4 * T1>([&](int r) { return h1[r]; }, s2); }, acc2);
       float h2[4 * T2];
 #pragma unroll
       for (int r = 0; r < 4 * T2; ++r) {
-        float v = tanh_f(acc2[r / 4][r % 4] + b1(r));
-        if (r / Z2 < 4 * T1) v += h1[r / Z2];
+        float v = 0.f;
+        if (r < V2) {
+          v = tanh_f(acc2[r / 4][r % 4] + b1(r));
+          if (r / Z2 < 4 * T1) v += h1[r / Z2];
+        }
         h2[r] = v;
       }
       for_groups<0, T3, kGroup>([&](auto g0c, auto szc) {
@@ -493,8 +500,8 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
 #pragma unroll
         for (int t = 0; t < SZ; ++t) acc3[t] = floatx4{0.f, 0.f, 0.f, 0.f};
         if constexpr (PREC == 0)
-          mfma_chain<SZ, 4 * T2, 0, 4 * T2>(wrs, nb + 4 * (O_W2A + g0 * 4 * T2 * 64), vl4,
-                                            [&](int s_) { return h2[s_]; }, acc3);
+          mfma_chain<SZ, 4 * T2, 0, V2>(wrs, nb + 4 * (O_W2A + g0 * 4 * T2 * 64), vl4,
+                                        [&](int s_) { return h2[s_]; }, acc3);
         else
           mfma_chain_h<PREC, SZ, S22, 0, S22>(hrs, hb + H_W2A + g0 * S22 * 1024, vl16, [&](int s2) {
             return pack8<PREC, 4 * T2>([&](int r) { return h2[r]; }, s2); }, acc3);
@@ -565,8 +572,10 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
 // ---------------------------------------------------------------------------------
 // Backward: forces (scatter) and virial partials.  One wave per centre.
 
-template <int T1, int T2, int T3, int Z2, int Z3, int PREC>
+template <int T1, int T2, int T3, int Z2, int Z3, int PREC, int V1>
 __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
+  constexpr int V2 = Z2 * V1;
+  static_assert(V1 <= 4 * T1 && V2 <= 4 * T2, "valid registers exceed the tiles");
   extern __shared__ float smem[];
   __shared__ double vir_s[4][9];
   const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -695,13 +704,13 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
     for (int j = 0; j < a.ntypes; ++j) {
       const int m = compact_row(a, S, c, j, pc, list, lane);
       wave_sync();
-      const int netb = (ti * a.ntypes + j) * a.net_stride * 4;
+      const int netb = __builtin_amdgcn_readfirstlane((ti * a.ntypes + j) * a.net_stride * 4);
       constexpr int O_K0 = 0, O_B0 = 16 * T1, O_B1 = 32 * T1, O_B2 = 32 * T1 + 16 * T2;
       constexpr int P0 = ((16 * T1 * 2 + 16 * T2 + 16 * T3) + 63) / 64 * 64;
       constexpr int O_W1A = P0, O_W2A = O_W1A + T2 * 4 * T1 * 64;
       constexpr int O_W2B = O_W2A + T3 * 4 * T2 * 64, O_W1B = O_W2B + T2 * 4 * T3 * 64;
       const int vl4 = lane * 4, vq16 = q * 16, vl16 = lane * 16;
-      const int hnetb = PREC ? (ti * a.ntypes + j) * a.hnet_stride : 0;
+      const int hnetb = PREC ? __builtin_amdgcn_readfirstlane((ti * a.ntypes + j) * a.hnet_stride) : 0;
       for (int ch = 0; ch < m; ch += 16) {
         const int nb = netb + opaque_zero();
         const int hb = hnetb + opaque_zero();
@@ -721,7 +730,7 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
         }
         float h1[4 * T1];
 #pragma unroll
-        for (int r = 0; r < 4 * T1; ++r) h1[r] = tanh_f(g.st * k0(r) + b0(r));
+        for (int r = 0; r < 4 * T1; ++r) h1[r] = r < V1 ? tanh_f(g.st * k0(r) + b0(r)) : 0.f;
         // h2 = t2 + h1[r/Z2] and h3 = t3 + h2[r/Z3] are re-formed where used
         auto H2 = [&](int r, const float* t2v) -> float {
           return t2v[r] + ((r / Z2 < 4 * T1) ? h1[r / Z2] : 0.f);
@@ -730,13 +739,13 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
 #pragma unroll
         for (int t = 0; t < T2; ++t) acc2[t] = floatx4{0.f, 0.f, 0.f, 0.f};
         if constexpr (PREC == 0)
-          mfma_chain<T2, 4 * T1, 0, 4 * T1>(wrs, nb + 4 * O_W1A, vl4, [&](int s_) { return h1[s_]; }, acc2);
+          mfma_chain<T2, 4 * T1, 0, V1>(wrs, nb + 4 * O_W1A, vl4, [&](int s_) { return h1[s_]; }, acc2);
         else
           mfma_chain_h<PREC, T2, S21, 0, S21>(hrs, hb + H_W1A, vl16, [&](int s2) {
             return pack8<PREC, 4 * T1>([&](int r) { return h1[r]; }, s2); }, acc2);
         float t2[4 * T2];
 #pragma unroll
-        for (int r = 0; r < 4 * T2; ++r) t2[r] = tanh_f(acc2[r / 4][r % 4] + b1(r));
+        for (int r = 0; r < 4 * T2; ++r) t2[r] = r < V2 ? tanh_f(acc2[r / 4][r % 4] + b1(r)) : 0.f;
         wave_sync();  // Rl visible to the dh3 products
         floatx4 dr = floatx4{0.f, 0.f, 0.f, 0.f};
         floatx4 dh2[T2];
@@ -748,8 +757,8 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
 #pragma unroll
           for (int t = 0; t < SZ; ++t) acc3[t] = floatx4{0.f, 0.f, 0.f, 0.f};
           if constexpr (PREC == 0)
-            mfma_chain<SZ, 4 * T2, 0, 4 * T2>(wrs, nb + 4 * (O_W2A + g0 * 4 * T2 * 64), vl4,
-                                              [&](int s_) { return H2(s_, t2); }, acc3);
+            mfma_chain<SZ, 4 * T2, 0, V2>(wrs, nb + 4 * (O_W2A + g0 * 4 * T2 * 64), vl4,
+                                          [&](int s_) { return H2(s_, t2); }, acc3);
           else
             mfma_chain_h<PREC, SZ, S22, 0, S22>(hrs, hb + H_W2A + g0 * S22 * 1024, vl16, [&](int s2) {
               return pack8<PREC, 4 * T2>([&](int r) { return H2(r, t2); }, s2); }, acc3);
@@ -815,7 +824,7 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
 #pragma unroll
         for (int r = 0; r < 4 * T2; ++r) dh2[r / 4][r % 4] *= (1.f - t2[r] * t2[r]);
         if constexpr (PREC == 0)
-          mfma_chain<T1, 4 * T2, 0, 4 * T2>(wrs, nb + 4 * O_W1B, vl4, [&](int s_) { return dh2[s_ / 4][s_ % 4]; }, dh1);
+          mfma_chain<T1, 4 * T2, 0, V2>(wrs, nb + 4 * O_W1B, vl4, [&](int s_) { return dh2[s_ / 4][s_ % 4]; }, dh1);
         else
           mfma_chain_h<PREC, T1, S22, 0, S22>(hrs, hb + H_W1B, vl16, [&](int s2) {
             return pack8<PREC, 4 * T2>([&](int r) { return dh2[r / 4][r % 4]; }, s2); }, dh1);
@@ -922,7 +931,7 @@ static void fill_args(System* s, EmbedArgs& a) {
   a.list_cap = (cmax + 3) / 4 * 4;
 }
 
-template <int T1, int T2, int T3, int Z2, int Z3, int PREC>
+template <int T1, int T2, int T3, int Z2, int Z3, int PREC, int V1>
 static int32_t launch_embed(System* s, bool backward) {
   EmbedArgs a;
   fill_args(s, a);
@@ -955,27 +964,27 @@ static int32_t launch_embed(System* s, bool backward) {
     return DPMD_E_UNSUPPORTED;
   }
   if (!backward) {
-    hipLaunchKernelGGL((embed_fwd_kernel<T1, T2, T3, Z2, Z3, PREC>), dim3(nblocks), dim3(256), lds, s->stream, a);
+    hipLaunchKernelGGL((embed_fwd_kernel<T1, T2, T3, Z2, Z3, PREC, V1>), dim3(nblocks), dim3(256), lds, s->stream, a);
   } else {
-    hipLaunchKernelGGL((embed_bwd_kernel<T1, T2, T3, Z2, Z3, PREC>), dim3(nblocks), dim3(256), lds, s->stream, a);
+    hipLaunchKernelGGL((embed_bwd_kernel<T1, T2, T3, Z2, Z3, PREC, V1>), dim3(nblocks), dim3(256), lds, s->stream, a);
   }
   DPMD_HIP(hipGetLastError());
   return DPMD_OK;
 }
 
-template <int T1, int T2, int T3, int Z2, int Z3>
+template <int T1, int T2, int T3, int Z2, int Z3, int V1>
 static int32_t launch_prec(System* s, bool backward, int precision) {
-  if (precision == DPMD_PREC_BF16) return launch_embed<T1, T2, T3, Z2, Z3, 1>(s, backward);
-  if (precision == DPMD_PREC_FP16) return launch_embed<T1, T2, T3, Z2, Z3, 2>(s, backward);
-  return launch_embed<T1, T2, T3, Z2, Z3, 0>(s, backward);
+  if (precision == DPMD_PREC_BF16) return launch_embed<T1, T2, T3, Z2, Z3, 1, V1>(s, backward);
+  if (precision == DPMD_PREC_FP16) return launch_embed<T1, T2, T3, Z2, Z3, 2, V1>(s, backward);
+  return launch_embed<T1, T2, T3, Z2, Z3, 0, V1>(s, backward);
 }
 
 int32_t embed_launch(System* s, bool backward, int precision) {
   const EmbedLayout& L = s->model->lay;
   if (L.tiles[0] == 2 && L.tiles[1] == 4 && L.tiles[2] == 7 && L.z[1] == 2 && L.z[2] == 2)
-    return launch_prec<2, 4, 7, 2, 2>(s, backward, precision);
+    return launch_prec<2, 4, 7, 2, 2, 7>(s, backward, precision);  // 7 layer-3 tiles imply v1 <= 7
   if (L.tiles[0] == 2 && L.tiles[1] == 2 && L.tiles[2] == 4 && L.z[1] == 1 && L.z[2] == 2)
-    return launch_prec<2, 2, 4, 1, 2>(s, backward, precision);
+    return launch_prec<2, 2, 4, 1, 2, 8>(s, backward, precision);
   set_error("embedding widths (%d,%d,%d) have no kernel instantiation", L.w[0], L.w[1], L.w[2]);
   return DPMD_E_UNSUPPORTED;
 }
