@@ -156,8 +156,8 @@ struct EmbedArgs {
   int cap[kMaxT];
   int rowbase[kMaxT];
   int ntypes;
-  float rcut;
-  float mean[kMaxT], stdv[kMaxT];
+  float rcut, inv_rcut;
+  float mean[kMaxT], inv_std[kMaxT];
   const float* net;            // packed nets, index (i*ntypes+j)*net_stride
   int net_stride;
   int base3[4], size3[4];
@@ -177,56 +177,65 @@ struct EmbedArgs {
   int lds_wave_floats;
 };
 
-// floor-mod shift to [-L/2, L/2): ((d - L/2) mod L) - L/2, as jnp.remainder computes it
+// Minimum image of d = x_j - x_i for positions wrapped into [0, L) (so |d| < L): the
+// representative in [-L/2, L/2), as ((d + L/2) mod L) - L/2 gives it, branch free.
 __device__ __forceinline__ float minimg(float d, float L) {
   const float h = L * 0.5f;
-  float r = fmodf(d - h, L);
-  if (r != 0.f && ((r < 0.f) != (L < 0.f))) r += L;
-  return r - h;
+  d = d >= h ? d - L : d;
+  return d < -h ? d + L : d;
 }
 
 __device__ __forceinline__ float safe_c(float x) { return fabsf(x) > 1e-15f ? x : 1e-15f; }
 
+// |x| with components below 1e-15 replaced by 1e-15 (deepmd_jax/utils.py:81)
 __device__ __forceinline__ float pair_r(float x, float y, float z) {
   float a = safe_c(x), b = safe_c(y), c = safe_c(z);
-  return sqrtf(a * a + b * b + c * c);
+  return __builtin_amdgcn_sqrtf(a * a + b * b + c * c);
 }
 
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
 struct Geo {
-  float x[3], u[3], r, s, sh, st, dsdr;
+  float x[3], u[3], r, inv_rr, sh, st, dsdr;
 };
 
-__device__ __forceinline__ void env_forward(const float4 e, bool valid, float rcut, float mean,
-                                            float stdv, Geo& g, float R[10]) {
+// Switch and environment row (deepmd_jax/utils.py:29-32, dpmodel.py:37-47) with
+// reciprocals in place of divisions (1-ulp hardware rcp / sqrt).
+__device__ __forceinline__ void env_forward(const float4 e, bool valid, float inv_rcut, float mean,
+                                            float inv_std, Geo& g, float R[10]) {
   g.x[0] = valid ? e.x : 0.f;
   g.x[1] = valid ? e.y : 0.f;
   g.x[2] = valid ? e.z : 0.f;
-  float r = valid ? pair_r(g.x[0], g.x[1], g.x[2]) : 0.f;
-  float t = r / rcut;
-  float sw = 1.f - 3.f * t * t + 2.f * t * t * t;
-  bool on = (r > 1e-14f) && (t < 1.f);
-  float inv = 1.f / (r + 1e-15f);
-  float s = on ? inv * sw : 0.f;
+  const float r = valid ? pair_r(g.x[0], g.x[1], g.x[2]) : 0.f;
+  const float t = r * inv_rcut;
+  const float sw = 1.f - 3.f * t * t + 2.f * t * t * t;
+  const bool on = (r > 1e-14f) && (t < 1.f);
+  const float inv = rcp(r + 1e-15f);
+  const float s = on ? inv * sw : 0.f;
   g.r = r;
-  g.s = s;
-  g.dsdr = on ? ((-6.f * t + 6.f * t * t) / rcut) * inv - sw * inv * inv : 0.f;
-  g.sh = s / stdv;
-  g.st = valid ? (s - mean) / stdv : 0.f;
-  const float rr = r + 1e-16f;
-  for (int c = 0; c < 3; ++c) g.u[c] = g.x[c] / rr;
+  g.dsdr = on ? ((-6.f * t + 6.f * t * t) * inv_rcut) * inv - sw * inv * inv : 0.f;
+  g.sh = s * inv_std;
+  g.st = valid ? (s - mean) * inv_std : 0.f;
+  g.inv_rr = rcp(r + 1e-16f);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) g.u[c] = g.x[c] * g.inv_rr;
   const float s3 = 1.7320508075688772f * g.sh, s33 = 3.f * g.sh;
   R[0] = g.sh;
+#pragma unroll
   for (int c = 0; c < 3; ++c) R[1 + c] = s3 * g.u[c];
+#pragma unroll
   for (int c = 0; c < 3; ++c) R[4 + c] = s33 * (g.u[c] * g.u[c] - (1.f / 3.f));
+#pragma unroll
   for (int c = 0; c < 3; ++c) R[7 + c] = s33 * (1.4142135623730951f * g.u[c] * g.u[(c + 2) % 3]);
 }
 
 // dE/dx of one pair given dE/dR (10) and dE/dst.
-__device__ __forceinline__ void env_backward(const Geo& g, const float dR[10], float dst, float stdv,
+__device__ __forceinline__ void env_backward(const Geo& g, const float dR[10], float dst, float inv_std,
                                              float out[3]) {
   const float s3 = 1.7320508075688772f, r2 = 1.4142135623730951f;
   float dsh = dR[0];
   float du[3];
+#pragma unroll
   for (int c = 0; c < 3; ++c) {
     const int cm = (c + 2) % 3, cp = (c + 1) % 3;
     dsh += s3 * dR[1 + c] * g.u[c] + 3.f * dR[4 + c] * (g.u[c] * g.u[c] - (1.f / 3.f)) +
@@ -234,13 +243,16 @@ __device__ __forceinline__ void env_backward(const Geo& g, const float dR[10], f
     du[c] = s3 * g.sh * dR[1 + c] + 6.f * g.sh * g.u[c] * dR[4 + c] +
             3.f * r2 * g.sh * (dR[7 + c] * g.u[cm] + dR[7 + cp] * g.u[cp]);
   }
-  const float ds = (dsh + dst) / stdv;
-  const float rr = g.r + 1e-16f;
+  const float ds = (dsh + dst) * inv_std;
+  const float irr = g.inv_rr;
   float dr = ds * g.dsdr;
-  for (int c = 0; c < 3; ++c) dr -= du[c] * g.x[c] / (rr * rr);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) dr -= du[c] * g.x[c] * (irr * irr);
+  const float inv_r = g.r > 0.f ? rcp(g.r) : 0.f;
+#pragma unroll
   for (int c = 0; c < 3; ++c) {
-    const float drdx = (fabsf(g.x[c]) > 1e-15f && g.r > 0.f) ? g.x[c] / g.r : 0.f;
-    out[c] = du[c] / rr + dr * drdx;
+    const float drdx = fabsf(g.x[c]) > 1e-15f ? g.x[c] * inv_r : 0.f;
+    out[c] = du[c] * irr + dr * drdx;
   }
 }
 
@@ -468,7 +480,7 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
       const float4 e = valid ? list[ch + n] : make_float4(0.f, 0.f, 0.f, 0.f);
       Geo g;
       float R[10];
-      env_forward(e, valid, a.rcut, a.mean[ti], a.stdv[ti], g, R);
+      env_forward(e, valid, a.inv_rcut, a.mean[ti], a.inv_std[ti], g, R);
       if (q == 0) {
 #pragma unroll
         for (int x = 0; x < 16; ++x) Rl[n * 16 + x] = x < 10 ? R[x] : 0.f;
@@ -723,7 +735,7 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
         const float4 e = valid ? list[ch + n] : make_float4(0.f, 0.f, 0.f, 0.f);
         Geo g;
         float R[10];
-        env_forward(e, valid, a.rcut, a.mean[ti], a.stdv[ti], g, R);
+        env_forward(e, valid, a.inv_rcut, a.mean[ti], a.inv_std[ti], g, R);
         if (q == 0) {
 #pragma unroll
           for (int x = 0; x < 16; ++x) Rl[n * 16 + x] = x < 10 ? R[x] : 0.f;
@@ -838,7 +850,7 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
 #pragma unroll
         for (int x = 0; x < 10; ++x) dR[x] = dRl[n * 16 + x];
         float gx[3];
-        env_backward(g, dR, dst, a.stdv[ti], gx);
+        env_backward(g, dR, dst, a.inv_std[ti], gx);
         if (valid && q == 0) {
           const int gj = __float_as_int(e.w);
 #pragma unroll
@@ -905,10 +917,11 @@ static void fill_args(System* s, EmbedArgs& a) {
     a.cap[j] = s->cap[j];
     a.rowbase[j] = s->rowbase[j];
     a.mean[j] = (float)M->mean[j];
-    a.stdv[j] = (float)(1.0 / M->istd[j]);
+    a.inv_std[j] = (float)M->istd[j];
   }
   a.ntypes = s->ntypes;
   a.rcut = (float)M->rcut;
+  a.inv_rcut = (float)(1.0 / M->rcut);
   a.net = M->d_emb;
   a.net_stride = M->off.total;
   a.net_bytes = M->off.total * 4 * M->ntypes * M->ntypes;
