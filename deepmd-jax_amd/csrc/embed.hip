@@ -375,6 +375,9 @@ __device__ __forceinline__ void for_groups(F&& f) {
 }
 
 constexpr int kGroup = 2;  // layer-3 tiles processed together
+// Row stride (floats) of the per-chunk [16 neighbours][env row] LDS tiles: odd, so the
+// 16 neighbours of a column read or write land in distinct banks.
+constexpr int kRS = 17;
 
 // Per-net first-layer weights and the three bias vectors (physical slot order) of all
 // ntypes^2 nets, copied once per block into LDS: [net][k0 | b0 | b1 | b2].
@@ -440,8 +443,8 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
   float* small = smem;
   float* wbase = load_small_tables<T1, T2, T3>(a, smem) + wv * a.lds_wave_floats;
   float4* list = reinterpret_cast<float4*>(wbase);
-  float* Rl = wbase + 4 * a.list_cap;
-  float* Hl = Rl + 256;  // one tile group of h3: [16 neighbours][HG]
+  float* Rl = wbase + 4 * a.list_cap;  // [16 neighbours][kRS]
+  float* Hl = Rl + 16 * kRS;            // one tile group of h3: [16 neighbours][HG]
   constexpr int HG = 16 * kGroup + 1;
 
   const SegInfo& S = a.seg[a.blk_seg[blockIdx.x]];
@@ -481,9 +484,13 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
       Geo g;
       float R[10];
       env_forward(e, valid, a.inv_rcut, a.mean[ti], a.inv_std[ti], g, R);
-      if (q == 0) {
 #pragma unroll
-        for (int x = 0; x < 16; ++x) Rl[n * 16 + x] = x < 10 ? R[x] : 0.f;
+      for (int i = 0; i < 4; ++i) {
+        const int x = 4 * q + i;  // lane group q stores env rows 4q..4q+3 of its neighbour
+        float v = 0.f;
+#pragma unroll
+        for (int y = 0; y < 10; ++y) v = x == y ? R[y] : v;
+        Rl[n * kRS + x] = v;
       }
       float h1[4 * T1];
 #pragma unroll
@@ -530,7 +537,7 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
         // reducer for this group: T[x][slot] += sum_nbr R[nbr][x] h3[nbr][slot]
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
-          const float av = Rl[(4 * kk + q) * 16 + n];
+          const float av = Rl[(4 * kk + q) * kRS + n];
 #pragma unroll
           for (int k = 0; k < SZ; ++k) Tacc[g0 + k] = mfma4(av, Hl[(4 * kk + q) * HG + 16 * k + n], Tacc[g0 + k]);
         }
@@ -604,10 +611,11 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
   float* small = smem;
   float* wbase = load_small_tables<T1, T2, T3>(a, smem) + wv * a.lds_wave_floats;
   // chunk-phase layout
+  constexpr int ODT = (10 * HS + 3) / 4 * 4;
   float* dTp = wbase;                    // 10 * HS (rows x >= 10 of the MFMA operands are zero)
-  float* Rl = dTp + 10 * HS;             // 256
-  float* dRl = Rl + 256;                 // 256
-  float4* list = reinterpret_cast<float4*>(dRl + 256);
+  float* Rl = dTp + ODT;                 // [16 neighbours][kRS]
+  float* dRl = Rl + 16 * kRS;            // [16 neighbours][kRS]
+  float4* list = reinterpret_cast<float4*>(dRl + 16 * kRS);  // 16-byte aligned
 
   const SegInfo& S = a.seg[a.blk_seg[blockIdx.x]];
   const int c = (blockIdx.x - S.blk0) * 4 + wv;
@@ -630,7 +638,7 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
     {
       const float* Tg = a.T + (size_t)gc * 10 * W;
       const float* dGg = a.dG + (size_t)grow * A * W;
-      float* Tl = dTp + 10 * HS;   // 10*W   (aliases Rl, dRl, list)
+      float* Tl = dTp + ODT;       // 10*W   (aliases Rl, dRl, list)
       float* Xl = Tl + 10 * W;     // 10*A
       float* Ul = Xl + 10 * A;     // 10*A
       for (int e = lane; e < 10 * W; e += 64) Tl[e] = Tg[e] + (e < W ? a.tbias[e] : 0.f);
@@ -736,9 +744,13 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
         Geo g;
         float R[10];
         env_forward(e, valid, a.inv_rcut, a.mean[ti], a.inv_std[ti], g, R);
-        if (q == 0) {
 #pragma unroll
-          for (int x = 0; x < 16; ++x) Rl[n * 16 + x] = x < 10 ? R[x] : 0.f;
+        for (int i = 0; i < 4; ++i) {
+          const int x = 4 * q + i;
+          float v = 0.f;
+#pragma unroll
+          for (int y = 0; y < 10; ++y) v = x == y ? R[y] : v;
+          Rl[n * kRS + x] = v;
         }
         float h1[4 * T1];
 #pragma unroll
@@ -795,7 +807,7 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
             floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int kk = 0; kk < 3; ++kk)
-              acc = mfma4(4 * kk + q < 10 ? dTp[(4 * kk + q) * HS + 16 * (g0 + k) + n] : 0.f, Rl[n * 16 + 4 * kk + q], acc);
+              acc = mfma4(4 * kk + q < 10 ? dTp[(4 * kk + q) * HS + 16 * (g0 + k) + n] : 0.f, Rl[n * kRS + 4 * kk + q], acc);
             dh3[k] = acc;
           }
           // repeat-residual: dh2[r/Z3] += dh3[r]; then dz3 = dh3 (1 - t3^2)
@@ -815,7 +827,7 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
               return pack8<PREC, 4 * SZ>([&](int r) { return dh3[r / 4][r % 4]; }, 0); }, dh2);
         });
 #pragma unroll
-        for (int i = 0; i < 4; ++i) dRl[n * 16 + 4 * q + i] = dr[i];
+        for (int i = 0; i < 4; ++i) dRl[n * kRS + 4 * q + i] = dr[i];
         // layer 2 backward
         floatx4 dh1[T1];
 #pragma unroll
@@ -848,7 +860,7 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
         wave_sync();
         float dR[10];
 #pragma unroll
-        for (int x = 0; x < 10; ++x) dR[x] = dRl[n * 16 + x];
+        for (int x = 0; x < 10; ++x) dR[x] = dRl[n * kRS + x];
         float gx[3];
         env_backward(g, dR, dst, a.inv_std[ti], gx);
         if (valid && q == 0) {
@@ -963,10 +975,10 @@ static int32_t launch_embed(System* s, bool backward) {
   const int nblocks = s->nblocks;
   size_t wave_floats;
   if (!backward) {
-    wave_floats = 4 * a.list_cap + 256 + 16 * (16 * kGroup + 1);
+    wave_floats = 4 * a.list_cap + 16 * kRS + 16 * (16 * kGroup + 1);
     wave_floats = std::max(wave_floats, (size_t)(10 * W));
   } else {
-    wave_floats = 10 * HS + std::max((size_t)(512 + 4 * a.list_cap), (size_t)(10 * W + 20 * A));
+    wave_floats = (10 * HS + 3) / 4 * 4 + std::max((size_t)(2 * 16 * kRS + 4 * a.list_cap), (size_t)(10 * W + 20 * A));
   }
   wave_floats = (wave_floats + 3) / 4 * 4;
   a.lds_wave_floats = (int)wave_floats;
