@@ -24,6 +24,12 @@ PEAK_FP32_MFMA = 157.3e12
 PEAK_BF16_MFMA = 2.5e15
 PEAK_HBM = 8.0e12
 KERNELS = {0: 'nlist', 1: 'embed_fwd', 2: 'fit', 3: 'embed_bwd', 4: 'integrator'}
+# HBM bytes per launch of the FP32 embed_bwd kernel on the 98,304-atom box, from two
+# rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE in KiB per dispatch) scaled by what the
+# same counters read for a 256 MiB elementwise copy (x2.0 and x1.0): 1,076.2 MiB read +
+# 457.4 MiB written (profiles/embed_fit_pmc_water98k.md).  Counters cannot be read
+# without the profiler, so the figure is carried here and re-measured with the kernel.
+EMBED_BWD_TRAFFIC_FP32 = (1076.2 + 457.4) * 2 ** 20
 
 
 def flops_per_step(N, pairs, A=16, W=100, emb=(25, 50, 100), fit=240):
@@ -373,7 +379,9 @@ def main():
     avg_s = ms_dom / 1e3 / max(cnt_dom, 1)
     peak = PEAK_FP32_MFMA if args.precision == 'fp32' else PEAK_BF16_MFMA
     roof = {'bound': 'mfma', 'kernel': dom, 'achieved': fk[dom] / avg_s / 1e12, 'peak': peak / 1e12,
-            'unit': 'TFLOP/s', 'frac': fk[dom] / avg_s / peak, 'traffic': None,
+            'unit': 'TFLOP/s', 'frac': fk[dom] / avg_s / peak,
+            'traffic': EMBED_BWD_TRAFFIC_FP32 if (dom == 'embed_bwd' and args.precision == 'fp32'
+                                                  and args.nmol == 32768) else None,
             'launch_ms': avg_s * 1e3}
     out = {
         'metric': 'MD steps/s, 98k-atom water DP, fp32 vs BF16 at stated force err; p99 latency',
