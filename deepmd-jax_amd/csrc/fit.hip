@@ -49,44 +49,47 @@ __device__ __forceinline__ float tanh_fit(float x) {
   return __builtin_fmaf(-2.f, __builtin_amdgcn_rcpf(1.f + e), 1.f);
 }
 
+// One output element: C[row][col] from the accumulator value cv.  Returns the energy-head
+// contribution y3 k3 for EPI_E3 (0 otherwise).
+template <int EPI>
+__device__ __forceinline__ float epi_one(const GemmArgs& g, size_t row, int col, float cv) {
+  const size_t o = row * g.ldc + col;
+  if (EPI == EPI_TANH) {
+    g.C[o] = tanh_fit(cv + g.bias[col]);
+  } else if (EPI == EPI_RES) {
+    const float t = tanh_fit(cv + g.bias[col]);
+    g.C2[o] = t;
+    g.C[o] = t * g.dt[col] + g.aux1[o];
+  } else if (EPI == EPI_E3) {
+    const float t = tanh_fit(cv + g.bias[col]);
+    const float y3 = t * g.dt[col] + g.aux1[o];
+    g.C[o] = g.k3[col] * g.dt[col] * (1.f - t * t);
+    return y3 * g.k3[col];
+  } else if (EPI == EPI_BWD2) {
+    const float dy2 = g.k3[col] + cv;
+    const float t2 = g.aux1[o];
+    g.C2[o] = dy2;
+    g.C[o] = dy2 * g.dt[col] * (1.f - t2 * t2);
+  } else if (EPI == EPI_BWD1) {
+    const float dy1 = g.aux1[o] + cv;
+    const float y1 = g.aux2[o];
+    g.C[o] = dy1 * (1.f - y1 * y1);
+  } else {
+    g.C[o] = cv;
+  }
+  return 0.f;
+}
+
 // lane holds C[m0 + 16wv + 4q + i][n0 + 16nt + n]
 template <int NT, int EPI>
 __device__ __forceinline__ void epilogue(const GemmArgs& g, const floatx4 (&acc)[NT], int m0, int n0, int wv, int q,
                                          int n) {
   float esum[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int nt = 0; nt < NT; ++nt) {
-    const int col = n0 + 16 * nt + n;
+  for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const size_t row = (size_t)(m0 + 16 * wv + 4 * q + i);
-      const float cv = acc[nt][i];
-      const size_t o = row * g.ldc + col;
-      if (EPI == EPI_TANH) {
-        g.C[o] = tanh_fit(cv + g.bias[col]);
-      } else if (EPI == EPI_RES) {
-        const float t = tanh_fit(cv + g.bias[col]);
-        g.C2[o] = t;
-        g.C[o] = t * g.dt[col] + g.aux1[o];
-      } else if (EPI == EPI_E3) {
-        const float t = tanh_fit(cv + g.bias[col]);
-        const float y3 = t * g.dt[col] + g.aux1[o];
-        esum[i] += y3 * g.k3[col];
-        g.C[o] = g.k3[col] * g.dt[col] * (1.f - t * t);
-      } else if (EPI == EPI_BWD2) {
-        const float dy2 = g.k3[col] + cv;
-        const float t2 = g.aux1[o];
-        g.C2[o] = dy2;
-        g.C[o] = dy2 * g.dt[col] * (1.f - t2 * t2);
-      } else if (EPI == EPI_BWD1) {
-        const float dy1 = g.aux1[o] + cv;
-        const float y1 = g.aux2[o];
-        g.C[o] = dy1 * (1.f - y1 * y1);
-      } else {
-        g.C[o] = cv;
-      }
-    }
-  }
+    for (int i = 0; i < 4; ++i)
+      esum[i] += epi_one<EPI>(g, (size_t)(m0 + 16 * wv + 4 * q + i), n0 + 16 * nt + n, acc[nt][i]);
   if (EPI == EPI_E3) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -195,25 +198,33 @@ struct Half8<2> {
   }
 };
 
+// 16-bit GEMM: block tile 64 rows x 16 NT columns, four waves as 2 x 2, each owning 32
+// rows (two 16-row tiles) and half of the column tiles, so every B fragment read from LDS
+// feeds two MFMAs (the LDS read rate, not the MFMA rate, bounds a 16-row wave tile).
 template <int NT, int EPI, int PREC>
 __global__ void __launch_bounds__(256, 2) gemm_h_kernel(const GemmArgs g) {
   typedef typename Half8<PREC>::T HT;
   typedef typename Half8<PREC>::E HE;
   __shared__ __attribute__((aligned(16))) uint16_t As[64 * LS16];
   __shared__ __attribute__((aligned(16))) uint16_t Bs[16 * NT * LS16];
+  __shared__ float Es[64];
   constexpr int BN = 16 * NT;
+  constexpr int NH = (NT + 1) / 2;   // column tiles per wave
   constexpr int BV = BN * KC16 / 8;  // 16-byte pieces per B slice
   constexpr int NB = (BV + 255) / 256;
   static_assert(NB <= 4, "B staging unrolled for at most 4 pieces per thread");
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
+  const int wm = wv & 1, wn = wv >> 1;
   const int q = lane >> 4, n = lane & 15;
   const int m0 = blockIdx.x * 64;
   const int n0 = blockIdx.y * BN;
   const uint16_t* Bt = (const uint16_t*)g.Bt;
-  floatx4 acc[NT];
+  floatx4 acc[2][NH];
 #pragma unroll
-  for (int t = 0; t < NT; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+  for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+    for (int t = 0; t < NH; ++t) acc[mi][t] = floatx4{0.f, 0.f, 0.f, 0.f};
   const int arow = tid / 4, ac8 = tid % 4;  // 64 rows x 4 pieces of 8 elements
   floatx4 ra0, ra1;
   uintx4 rb0, rb1, rb2, rb3;
@@ -250,14 +261,66 @@ __global__ void __launch_bounds__(256, 2) gemm_h_kernel(const GemmArgs g) {
     if constexpr (NB > 3) { if (live(3)) *bdst(3) = rb3; }
     __syncthreads();
     if (k0 + KC16 < g.K) fetch(k0 + KC16);
-    const HT av = *reinterpret_cast<const HT*>(&As[(16 * wv + n) * LS16 + 8 * q]);
+    const HT a0 = *reinterpret_cast<const HT*>(&As[(32 * wm + n) * LS16 + 8 * q]);
+    const HT a1 = *reinterpret_cast<const HT*>(&As[(32 * wm + 16 + n) * LS16 + 8 * q]);
 #pragma unroll
-    for (int nt = 0; nt < NT; ++nt) {
-      const HT bv = *reinterpret_cast<const HT*>(&Bs[(16 * nt + n) * LS16 + 8 * q]);
-      acc[nt] = Half8<PREC>::mfma(av, bv, acc[nt]);
+    for (int nt = 0; nt < NH; ++nt) {
+      const int tile = wn * NH + nt;
+      if (NT % 2 == 0 || tile < NT) {
+        const HT bv = *reinterpret_cast<const HT*>(&Bs[(16 * tile + n) * LS16 + 8 * q]);
+        acc[0][nt] = Half8<PREC>::mfma(a0, bv, acc[0][nt]);
+        acc[1][nt] = Half8<PREC>::mfma(a1, bv, acc[1][nt]);
+      }
     }
   }
-  epilogue<NT, EPI>(g, acc, m0, n0, wv, q, n);
+  // epilogue: lane holds C[m0 + 32 wm + 16 mi + 4q + i][n0 + 16 (wn NH + nt) + n]
+  float esum[2][4];
+#pragma unroll
+  for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) esum[mi][i] = 0.f;
+#pragma unroll
+  for (int nt = 0; nt < NH; ++nt) {
+    const int tile = wn * NH + nt;
+    if (NT % 2 == 0 || tile < NT) {
+#pragma unroll
+      for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          esum[mi][i] += epi_one<EPI>(g, (size_t)(m0 + 32 * wm + 16 * mi + 4 * q + i), n0 + 16 * tile + n,
+                                      acc[mi][nt][i]);
+    }
+  }
+  if (EPI == EPI_E3) {
+    // row sums: over the 16 columns of a lane group, then over the two column halves
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        float v = esum[mi][i];
+        v += __shfl_xor(v, 1);
+        v += __shfl_xor(v, 2);
+        v += __shfl_xor(v, 4);
+        v += __shfl_xor(v, 8);
+        esum[mi][i] = v;
+      }
+    if (wn == 1 && n == 0) {
+#pragma unroll
+      for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) Es[32 * wm + 16 * mi + 4 * q + i] = esum[mi][i];
+    }
+    __syncthreads();
+    if (wn == 0 && n == 0) {
+#pragma unroll
+      for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int r = 32 * wm + 16 * mi + 4 * q + i;
+          g.eout[m0 + r] = (esum[mi][i] + Es[r]) + g.b3;
+        }
+    }
+  }
 }
 
 template <int NT, int EPI>

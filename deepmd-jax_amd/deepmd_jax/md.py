@@ -31,6 +31,76 @@ def print_run_profile(steps, elapsed_time, natoms, dt):
     print('# Performance: %.3f ns/day, %.3f step/μs/atom' % (nanosecond_per_day, steps_per_microsecond_per_atom))
 
 
+class _ImageSystem:
+    """A DeviceSystem seen through its primary cell: with k = (1, 1, 1) a plain pass-through;
+    otherwise each replica holds the prod(k) images of the box (image-major), which the
+    device integrates together (same masses, velocities and thermostat noise), and every
+    getter returns the first image.  Energies, virials and kinetic energies come back per
+    image from the library."""
+
+    def __init__(self, model, type_idx, nrep, box, k):
+        from .evaluator import replicate
+        self.k = np.asarray(k, int)
+        self.nimg = int(np.prod(self.k))
+        self.N = len(type_idx)
+        self.box = np.asarray(box, np.float64)
+        self._shift = replicate(np.zeros((self.N, 3)), self.box, type_idx, self.k)[0]
+        types = np.tile(np.asarray(type_idx), self.nimg)
+        self.dev = _lib.DeviceSystem(model, types, nrep)
+        self.dev.set_box(self.box * self.k)
+        if self.nimg > 1:
+            self.dev.set_images(np.full(nrep, self.nimg))
+
+    def _up(self, x):  # (P, N, 3) -> (P, nimg N, 3)
+        x = np.asarray(x, np.float64)
+        return x if self.nimg == 1 else np.tile(x, (1, self.nimg, 1))
+
+    def set_positions(self, pos):
+        p = np.asarray(pos, np.float64)
+        self.dev.set_positions(p if self.nimg == 1 else self._up(p) + self._shift[None])
+
+    def get_positions(self):
+        return self.dev.get_positions()[:, :self.N]
+
+    def set_velocities(self, vel):
+        self.dev.set_velocities(self._up(vel))
+
+    def get_velocities(self):
+        return self.dev.get_velocities()[:, :self.N]
+
+    def set_masses(self, mass):
+        self.dev.set_masses(np.tile(np.asarray(mass, np.float64), self.nimg))
+
+    def compute(self, precision):
+        e, f, v, ea = self.dev.compute(precision)
+        return e, f[:, :self.N], v, ea
+
+    def md_run(self, params, nsteps, record=True):
+        tp, tv, err = self.dev.md_run(params, nsteps, record)
+        if record:
+            tp, tv = tp[:, :, :self.N], tv[:, :, :self.N]
+        return tp, tv, err
+
+    @property
+    def knbr(self):
+        return self.dev.knbr
+
+    def nlist_allocate(self, rc_skin, ratio, knbr=None):
+        return self.dev.nlist_allocate(rc_skin, ratio, knbr)
+
+    def nlist_update(self):
+        return self.dev.nlist_update()
+
+    def pimd_setup(self, *a):
+        return self.dev.pimd_setup(*a)
+
+    def kinetic_energy(self):
+        return self.dev.kinetic_energy()
+
+    def thermostat_energy(self):
+        return self.dev.thermostat_energy()
+
+
 class Simulation:
     step: int = 0
 
@@ -116,15 +186,18 @@ class Simulation:
         self._neighbor_buffer_ratio = float(neighbor_buffer_ratio)
         self._box = canonical_box(box)
         self._rcut = float(params['rcut'])
+        # The reference switches to its dense lattice-image path below 2*(rcut+skin)
+        # (deepmd_jax/utils.py:66-78).  Here the neighbour list is exact for any skin with
+        # 2*(rcut+skin) <= L, so the skin is shrunk to fit; a box below 2*rcut is held as a
+        # k1 x k2 x k3 periodic supercell whose images move together (the lattice-image sum).
+        k = np.ones(3, int)
         if self._box.min() < 2 * self._rcut:
-            raise NotImplementedError('MD needs box lengths >= 2*rcut (%.3f); use a supercell' % (2 * self._rcut))
-        # The reference switches to its dense lattice path below 2*(rcut+skin); the
-        # neighbour list is exact for any skin with 2*(rcut+skin) <= L, so the skin is
-        # shrunk to fit instead.
-        if self._box.min() <= 2 * (self._rcut + self._neighbor_skin) or not use_neighbor_list_when_possible:
+            if self._is_pimd:
+                raise NotImplementedError('path-integral MD needs box lengths >= 2*rcut (%.3f)' % (2 * self._rcut))
+            k = np.maximum(1, np.ceil(2 * (self._rcut + self._neighbor_skin) / self._box * (1 + 1e-9))).astype(int)
+        elif self._box.min() <= 2 * (self._rcut + self._neighbor_skin) or not use_neighbor_list_when_possible:
             self._neighbor_skin = max(0.0, 0.5 * self._box.min() * (1 - 1e-9) - self._rcut)
-        self._sys = _lib.DeviceSystem(self._device_model, self._type_idx, self._n_bead)
-        self._sys.set_box(self._box)
+        self._sys = _ImageSystem(self._device_model, self._type_idx, self._n_bead, self._box, k)
         self._sys.set_masses(self._mass)
         self._sys.set_positions(pos)
         self._allocate_nlist()

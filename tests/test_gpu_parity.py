@@ -280,3 +280,39 @@ def test_autograd_boundary(device_model, model0):
     e2 = float(DeviceEnergy.apply(c, s, 'fp32'))
     de_lin = -(f * d).sum()
     assert abs((e2 - e) - de_lin) <= 0.05 * abs(de_lin) + 1e-4
+
+
+def test_simulation_small_box_images(device_model, model0):
+    """Simulation on a box below 2 rcut (32 H2O, L = 9.86 A): held as a 2x2x2 image
+    supercell on the device; 20 NVE steps against the FP64 oracle's lattice-image VV, and
+    the reported energy is the primary cell's."""
+    import contextlib
+    import io
+    from deepmd_jax import synthetic as syn
+    from deepmd_jax.constants import MASS_UNIT_CONVERSION
+    from deepmd_jax.dpmodel import DPModel
+    from deepmd_jax.md import Simulation
+    print('case simulation small box', flush=True)
+    params, var = model0
+    coord, box, types = syn.water_box(32, seed=4, permute_sites=True)
+    assert box[0] < 2 * params['rcut']
+    vel = syn.maxwell_boltzmann(types, 300.0, seed=4)
+    with contextlib.redirect_stdout(io.StringIO()):
+        sim = Simulation(None, box, types, syn.MASSES, 'NVE', 0.5, coord, initial_velocity=vel,
+                         report_interval=10 ** 6, neighbor_skin=0.5, model=(DPModel(params), var, device_model))
+        assert sim._sys.nimg == 8
+        E0 = sim.getEnergy()
+        traj = sim.run(20)
+    lat = O.lattice_candidate(np.diag(box), params['rcut'])
+    m = np.array(syn.MASSES)[types] * MASS_UNIT_CONVERSION
+    Eo, F, _ = O.energy_force_virial(params, var, coord, box, types, lattice=lat)
+    assert abs(E0 - Eo.item()) <= 1e-5 * len(types)
+    x, pm, F = coord.copy(), vel * m[:, None], F.numpy()
+    for k in range(20):
+        pm += 0.25 * F
+        x += 0.5 * pm / m[:, None]
+        _, F, _ = O.energy_force_virial(params, var, x, box, types, lattice=lat)
+        F = F.numpy()
+        pm += 0.25 * F
+    assert np.abs(traj['position'][-1] - x).max() < 1e-5
+    assert np.abs(traj['velocity'][-1] - pm / m[:, None]).max() < 1e-5
