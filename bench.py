@@ -241,6 +241,81 @@ def pimd(model, params, var, steps=100, warmup=10, precisions=('fp32', 'bf16'), 
     return out
 
 
+def compressed(params, var, coord, box, types, vel, steps, warmup, acc=True):
+    """The same NVE workload with the model after compress_model (1,024 intervals, rmin
+    0.6 A, the training defaults): tabulated embedding nets (FP32) with FP32 and BF16
+    fitting GEMMs.  Device-resident steps/s, per-kernel ms per step, and (acc) the errors
+    against the FP64 oracle of the same tabulated model on a 12,288-atom frame."""
+    import contextlib
+    import copy
+    import io
+    from deepmd_jax import _lib, synthetic as syn
+    from deepmd_jax.constants import MASS_UNIT_CONVERSION
+    from deepmd_jax.dpmodel import DPModel
+    from deepmd_jax.utils import compress_model
+    pc, vc = copy.deepcopy(params), copy.deepcopy(var)
+    with contextlib.redirect_stdout(io.StringIO()):
+        m, vc = compress_model(DPModel(pc), vc, 1024, 0.6)
+    pc = m.params
+    model = _lib.DeviceModel(pc, vc)
+    s = _lib.DeviceSystem(model, types, 1)
+    s.set_box(box)
+    s.set_masses(np.array(syn.MASSES)[types] * MASS_UNIT_CONVERSION)
+    p = _lib.MDParams()
+    p.routine, p.dt = 0, 0.5
+    out = {'model': 'seeded model after compress_model(Ngrids=1024, rmin=0.6)'}
+
+    def run(n):
+        done = 0
+        while done < n:
+            k = min(100, n - done)
+            _, _, err = s.md_run(p, k, record=False)
+            if err & 16:
+                raise RuntimeError('NaN/Inf in the compressed-model run')
+            if err & 2:
+                s.nlist_allocate(pc['rcut'] + 1.0, 1.3)
+                continue
+            done += k
+        s.synchronize()
+    for prec in ('fp32', 'bf16'):
+        p.precision = _lib.PREC[prec]
+        s.set_positions(coord)
+        s.set_velocities(vel)
+        s.nlist_allocate(pc['rcut'] + 1.0, 1.2)
+        run(warmup)
+        t0 = time.perf_counter()
+        run(steps)
+        el = time.perf_counter() - t0
+        s.timing(on=True)
+        run(20)
+        kt = {KERNELS[w]: s.timing(which=w)[0] / 20 for w in KERNELS}
+        s.timing(on=False)
+        out[prec] = {'steps_per_s': steps / el, 'ms_per_step': 1e3 * el / steps,
+                     'atoms_steps_per_s': steps * len(types) / el, 'kernel_ms_per_step': kt}
+        print('# compressed %s: %.1f steps/s' % (prec, steps / el), file=sys.stderr, flush=True)
+    if acc:
+        import torch
+        sys.path.insert(0, ROOT)
+        from oracle import dp_oracle as O
+        torch.set_num_threads(len(os.sched_getaffinity(0)))
+        c2, b2, t2 = syn.water_box(4096, seed=0)
+        nb, _, _ = O.typed_nlist(c2, b2, t2, pc['rcut'])
+        E, F, W = O.energy_force_virial(pc, vc, c2, b2, t2, nbrs_nm=nb, dtype=torch.float64)
+        E, F, W = E.item(), F.numpy(), W.numpy()
+        s2 = _lib.DeviceSystem(model, t2, 1)
+        s2.set_box(b2)
+        s2.set_positions(c2)
+        s2.nlist_allocate(pc['rcut'], 1.2)
+        out['errors_vs_fp64_oracle'] = {'frame': '4,096 H2O (12,288 atoms), seed 0, same tabulated model'}
+        for prec in ('fp32', 'bf16'):
+            e, f, v, _ = s2.compute(prec)
+            out['errors_vs_fp64_oracle'][prec] = {
+                'energy_per_atom_abs_err': abs(e[0] - E) / len(t2),
+                'force_rmse_eV_per_A': float(np.sqrt(((f[0] - F) ** 2).mean())),
+                'virial_max_rel_err': float(np.abs(v[0] - W).max() / np.abs(W).max())}
+    return out
+
+
 def serving(model, rates, precisions, n_requests=256, md_chunk=20, budget_gib=32):
     """The serving stream: 256 water-box requests (n log-uniform in [32, 8192] H2O, first n
     sites of a seeded grid permutation), 70 % single E/F/virial evaluations, 30 % 200-step
@@ -288,6 +363,7 @@ def main():
     ap.add_argument('--no-acc', action='store_true')
     ap.add_argument('--no-serving', action='store_true')
     ap.add_argument('--no-pimd', action='store_true')
+    ap.add_argument('--no-compressed', action='store_true')
     ap.add_argument('--frames', type=int, default=1, help='seeded frames for the precision errors')
     ap.add_argument('--drift-steps', type=int, default=0, help='NVE drift run length per precision (0: skip)')
     ap.add_argument('--pimd-steps', type=int, default=100)
@@ -434,6 +510,9 @@ def main():
         out['precision_errors_vs_fp64_oracle'] = accuracy(params, var, model, args.frames)
     if args.drift_steps:
         out['nve_drift'] = nve_drift(model, params, coord, box, types, vel, args.drift_steps)
+    if not args.no_compressed:
+        out['compressed'] = compressed(params, var, coord, box, types, vel, args.steps, args.warmup,
+                                       acc=not args.no_acc)
     if not args.no_pimd:
         out['pimd'] = pimd(model, params, var, steps=args.pimd_steps, acc_beads=0 if args.no_acc else args.pimd_acc_beads)
     if not args.no_serving:

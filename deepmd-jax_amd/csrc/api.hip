@@ -111,12 +111,45 @@ static int32_t model_create(const dpmd_model_desc* d, int device, Model** out) {
   // embedding nets
   M->off = embed_net_offsets(M->lay);
   const int nn = d->ntypes * d->ntypes;
-  std::vector<float> emb((size_t)nn * M->off.total);
+  std::vector<float> emb((size_t)nn * M->off.total, 0.f);
   for (int k = 0; k < nn; ++k) {
-    for (int l = 0; l < 3; ++l)
-      if (!d->emb_kernel[k][l] || !d->emb_bias[k][l]) { set_error("embedding net %d layer %d missing", k, l); delete M; return DPMD_E_ARG; }
+    bool have = true;
+    for (int l = 0; l < 3; ++l) have = have && d->emb_kernel[k][l] && d->emb_bias[k][l];
+    if (!have) {
+      if (d->tab_ngrids > 0) continue;  // tabulated nets need no MLP weights
+      set_error("embedding net %d weights missing", k);
+      delete M;
+      return DPMD_E_ARG;
+    }
     embed_pack_net(M->lay, d->emb_kernel[k][0], d->emb_bias[k][0], d->emb_kernel[k][1], d->emb_bias[k][1],
                    d->emb_kernel[k][2], d->emb_bias[k][2], emb.data() + (size_t)k * M->off.total);
+  }
+  if (d->tab_ngrids < 0 || d->tab_ngrids > (1 << 20)) {
+    set_error("tab_ngrids %d out of range [0, 2^20]", d->tab_ngrids);
+    delete M;
+    return DPMD_E_ARG;
+  }
+  if (d->tab_ngrids > 0) {
+    // delta in float32 as the reference forms it: (srmax - srmin) / Ngrids
+    M->tab_n = d->tab_ngrids;
+    M->tab_stride = d->tab_ngrids * 6 * 16 * M->lay.tiles[2];
+    if ((size_t)nn * M->tab_stride * sizeof(float) >= (size_t)1 << 31) {
+      set_error("tabulated embedding tables must stay below 2 GiB (%d intervals)", d->tab_ngrids);
+      delete M;
+      return DPMD_E_UNSUPPORTED;
+    }
+    std::vector<float> tab((size_t)nn * M->tab_stride);
+    for (int k = 0; k < nn; ++k) {
+      if (!d->tab_coeff[k]) { set_error("tabulated embedding net %d missing", k); delete M; return DPMD_E_ARG; }
+      const float lo = d->tab_srmin[k], hi = d->tab_srmax[k];
+      const float delta = (hi - lo) / (float)d->tab_ngrids;
+      if (!(delta > 0.f)) { set_error("tabulated net %d: srmax must exceed srmin", k); delete M; return DPMD_E_ARG; }
+      M->tab_srmin[k] = lo;
+      M->tab_delta[k] = delta;
+      embed_pack_table(M->lay, d->tab_ngrids, d->tab_coeff[k], tab.data() + (size_t)k * M->tab_stride);
+    }
+    DPMD_HIP(hipMalloc(&M->d_tab, sizeof(float) * tab.size()));
+    DPMD_HIP(hipMemcpy(M->d_tab, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice));
   }
   DPMD_HIP(hipMalloc(&M->d_emb, sizeof(float) * emb.size()));
   DPMD_HIP(hipMemcpy(M->d_emb, emb.data(), sizeof(float) * emb.size(), hipMemcpyHostToDevice));
@@ -557,6 +590,7 @@ int32_t dpmd_model_destroy(dpmd_model_t model) {
   if (M->d_emb) hipFree(M->d_emb);
   if (M->d_tbias) hipFree(M->d_tbias);
   if (M->d_fit) hipFree(M->d_fit);
+  if (M->d_tab) hipFree(M->d_tab);
   for (int k = 0; k < 2; ++k) {
     if (M->d_emb_h[k]) hipFree(M->d_emb_h[k]);
     if (M->d_fit_h[k]) hipFree(M->d_fit_h[k]);

@@ -175,6 +175,9 @@ struct EmbedArgs {
   const void* hnet;            // 16-bit operand tables (bf16 or fp16), per net hnet_stride bytes
   int hnet_stride, hnet_bytes;
   int lds_wave_floats;
+  const float* tab;            // tabulated nets [net][interval][6][16*T3] (compressed model)
+  int tab_n, tab_stride, tab_bytes;
+  float tab_srmin[kMaxT * kMaxT], tab_delta[kMaxT * kMaxT];
 };
 
 // Minimum image of d = x_j - x_i for positions wrapped into [0, L) (so |d| < L): the
@@ -265,6 +268,18 @@ __device__ __forceinline__ float tanh_f(float x) {
   return __builtin_fmaf(-2.f, __builtin_amdgcn_rcpf(1.f + e), 1.f);
 }
 
+// Interval of a tabulated net (deepmd_jax/utils.py:130-135): x = s~ - srmin,
+// idx = trunc(x / delta), x0 = x - idx*delta; the coefficient row is idx brought into
+// [0, n) by jax's gather rule (negative wraps once, then clamps) while x0 keeps the
+// unclamped idx.  The quotient is bounded before the conversion (no overflow).
+__device__ __forceinline__ int tab_interval(float st, float srmin, float delta, int n, float& x0) {
+  const float x = st - srmin;
+  const float qf = fminf(fmaxf(x / delta, -1073741824.f), 1073741824.f);
+  const int idx = (int)qf;
+  x0 = x - (float)idx * delta;
+  return idx < 0 ? max(idx + n, 0) : min(idx, n - 1);
+}
+
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
@@ -306,6 +321,11 @@ typedef __amdgpu_buffer_rsrc_t rsrc_t;
 // weight read: byte offset = voff (per lane) + soff (wave-uniform)
 __device__ __forceinline__ float bld(rsrc_t rs, int voff, int soff) {
   return __builtin_bit_cast(float, (unsigned)__builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff, 0));
+}
+
+// tabulated-net coefficients: four consecutive slots of one power of one interval
+__device__ __forceinline__ floatx4 tab_load(rsrc_t rs, int voff, int soff) {
+  return __builtin_bit_cast(floatx4, (u32x4)__builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, 0));
 }
 
 template <int OT, int S, int S0, int SN, typename BF>
@@ -427,7 +447,7 @@ __device__ __forceinline__ int compact_row(const EmbedArgs& a, const SegInfo& S,
 // V1: registers of layer 1 that hold a feature in some lane group (the rest are padding
 // in every group); layer 2 then has V2 = Z2 V1 such registers.  MFMA k-steps and tanh
 // evaluations over registers that are padding everywhere are skipped.
-template <int T1, int T2, int T3, int Z2, int Z3, int PREC, int V1>
+template <int T1, int T2, int T3, int Z2, int Z3, int PREC, int V1, bool TAB>
 __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
   constexpr int V2 = Z2 * V1;
   static_assert(V1 <= 4 * T1 && V2 <= 4 * T2, "valid registers exceed the tiles");
@@ -492,49 +512,11 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
         for (int y = 0; y < 10; ++y) v = x == y ? R[y] : v;
         Rl[n * kRS + x] = v;
       }
-      float h1[4 * T1];
-#pragma unroll
-      for (int r = 0; r < 4 * T1; ++r) h1[r] = r < V1 ? tanh_f(g.st * k0(r) + b0(r)) : 0.f;
-      floatx4 acc2[T2];
-#pragma unroll
-      for (int t = 0; t < T2; ++t) acc2[t] = floatx4{0.f, 0.f, 0.f, 0.f};
-      if constexpr (PREC == 0)
-        mfma_chain<T2, 4 * T1, 0, V1>(wrs, nb + 4 * O_W1A, vl4, [&](int s_) { return h1[s_]; }, acc2);
-      else
-        mfma_chain_h<PREC, T2, S21, 0, S21>(hrs, hb + H_W1A, vl16, [&](int s2) {
-          return pack8<PREC, 4 * T1>([&](int r) { return h1[r]; }, s2); }, acc2);
-      float h2[4 * T2];
-#pragma unroll
-      for (int r = 0; r < 4 * T2; ++r) {
-        float v = 0.f;
-        if (r < V2) {
-          v = tanh_f(acc2[r / 4][r % 4] + b1(r));
-          if (r / Z2 < 4 * T1) v += h1[r / Z2];
-        }
-        h2[r] = v;
-      }
-      for_groups<0, T3, kGroup>([&](auto g0c, auto szc) {
+      // reducer for one group of layer-3 tiles held in Hl:
+      // T[x][slot] += sum_nbr R[nbr][x] h3[nbr][slot]
+      auto reduce_group = [&](auto g0c, auto szc) {
         constexpr int g0 = decltype(g0c)::value, SZ = decltype(szc)::value;
-        floatx4 acc3[SZ];
-#pragma unroll
-        for (int t = 0; t < SZ; ++t) acc3[t] = floatx4{0.f, 0.f, 0.f, 0.f};
-        if constexpr (PREC == 0)
-          mfma_chain<SZ, 4 * T2, 0, V2>(wrs, nb + 4 * (O_W2A + g0 * 4 * T2 * 64), vl4,
-                                        [&](int s_) { return h2[s_]; }, acc3);
-        else
-          mfma_chain_h<PREC, SZ, S22, 0, S22>(hrs, hb + H_W2A + g0 * S22 * 1024, vl16, [&](int s2) {
-            return pack8<PREC, 4 * T2>([&](int r) { return h2[r]; }, s2); }, acc3);
-#pragma unroll
-        for (int k = 0; k < SZ; ++k)
-#pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const int r = 4 * (g0 + k) + i;
-            float v = tanh_f(acc3[k][i] + b2(r));
-            if (r / Z3 < 4 * T2) v += h2[r / Z3];
-            Hl[n * HG + 16 * k + 4 * q + i] = v;
-          }
         wave_sync();
-        // reducer for this group: T[x][slot] += sum_nbr R[nbr][x] h3[nbr][slot]
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
           const float av = Rl[(4 * kk + q) * kRS + n];
@@ -542,7 +524,77 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
           for (int k = 0; k < SZ; ++k) Tacc[g0 + k] = mfma4(av, Hl[(4 * kk + q) * HG + 16 * k + n], Tacc[g0 + k]);
         }
         wave_sync();
-      });
+      };
+      if constexpr (TAB) {
+        // tabulated net: h3 = sum_c coeff[row][c] x0^(5-c) (Horner), four slots per load
+        const int net = ti * a.ntypes + j;
+        float x0;
+        const int row = tab_interval(g.st, a.tab_srmin[net], a.tab_delta[net], a.tab_n, x0);
+        const int voff = row * (6 * 64 * T3) + vq16;
+        const int tb = __builtin_amdgcn_readfirstlane(net * a.tab_stride * 4) + opaque_zero();
+        const rsrc_t trs = __builtin_amdgcn_make_buffer_rsrc((void*)a.tab, (short)0, a.tab_bytes, 0x00020000);
+        for_groups<0, T3, kGroup>([&](auto g0c, auto szc) {
+          constexpr int g0 = decltype(g0c)::value, SZ = decltype(szc)::value;
+#pragma unroll
+          for (int k = 0; k < SZ; ++k) {
+            floatx4 cf[6];
+#pragma unroll
+            for (int cc = 0; cc < 6; ++cc) cf[cc] = tab_load(trs, voff + 64 * (g0 + k), tb + cc * 64 * T3);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              float v = cf[0][i];
+#pragma unroll
+              for (int cc = 1; cc < 6; ++cc) v = __builtin_fmaf(v, x0, cf[cc][i]);
+              Hl[n * HG + 16 * k + 4 * q + i] = v;
+            }
+          }
+          reduce_group(g0c, szc);
+        });
+      } else {
+        float h1[4 * T1];
+#pragma unroll
+        for (int r = 0; r < 4 * T1; ++r) h1[r] = r < V1 ? tanh_f(g.st * k0(r) + b0(r)) : 0.f;
+        floatx4 acc2[T2];
+#pragma unroll
+        for (int t = 0; t < T2; ++t) acc2[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+        if constexpr (PREC == 0)
+          mfma_chain<T2, 4 * T1, 0, V1>(wrs, nb + 4 * O_W1A, vl4, [&](int s_) { return h1[s_]; }, acc2);
+        else
+          mfma_chain_h<PREC, T2, S21, 0, S21>(hrs, hb + H_W1A, vl16, [&](int s2) {
+            return pack8<PREC, 4 * T1>([&](int r) { return h1[r]; }, s2); }, acc2);
+        float h2[4 * T2];
+#pragma unroll
+        for (int r = 0; r < 4 * T2; ++r) {
+          float v = 0.f;
+          if (r < V2) {
+            v = tanh_f(acc2[r / 4][r % 4] + b1(r));
+            if (r / Z2 < 4 * T1) v += h1[r / Z2];
+          }
+          h2[r] = v;
+        }
+        for_groups<0, T3, kGroup>([&](auto g0c, auto szc) {
+          constexpr int g0 = decltype(g0c)::value, SZ = decltype(szc)::value;
+          floatx4 acc3[SZ];
+#pragma unroll
+          for (int t = 0; t < SZ; ++t) acc3[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+          if constexpr (PREC == 0)
+            mfma_chain<SZ, 4 * T2, 0, V2>(wrs, nb + 4 * (O_W2A + g0 * 4 * T2 * 64), vl4,
+                                          [&](int s_) { return h2[s_]; }, acc3);
+          else
+            mfma_chain_h<PREC, SZ, S22, 0, S22>(hrs, hb + H_W2A + g0 * S22 * 1024, vl16, [&](int s2) {
+              return pack8<PREC, 4 * T2>([&](int r) { return h2[r]; }, s2); }, acc3);
+#pragma unroll
+          for (int k = 0; k < SZ; ++k)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              const int r = 4 * (g0 + k) + i;
+              float v = tanh_f(acc3[k][i] + b2(r));
+              if (r / Z3 < 4 * T2) v += h2[r / Z3];
+              Hl[n * HG + 16 * k + 4 * q + i] = v;
+            }
+          reduce_group(g0c, szc);
+        });
+      }
     }
   }
   // T in logical order into LDS, then T and G to global
@@ -591,7 +643,7 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
 // ---------------------------------------------------------------------------------
 // Backward: forces (scatter) and virial partials.  One wave per centre.
 
-template <int T1, int T2, int T3, int Z2, int Z3, int PREC, int V1>
+template <int T1, int T2, int T3, int Z2, int Z3, int PREC, int V1, bool TAB>
 __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
   constexpr int V2 = Z2 * V1;
   static_assert(V1 <= 4 * T1 && V2 <= 4 * T2, "valid registers exceed the tiles");
@@ -752,111 +804,168 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
           for (int y = 0; y < 10; ++y) v = x == y ? R[y] : v;
           Rl[n * kRS + x] = v;
         }
-        float h1[4 * T1];
+        float dst;
+        if constexpr (TAB) {
+          // tabulated net: h3 = p(x0) and dh3/ds~ = p'(x0) from one read of the interval's
+          // coefficients; dE/ds~ = sum_slot dh3[slot] p'[slot] (no gradient through idx)
+          const int net = ti * a.ntypes + j;
+          float x0;
+          const int row = tab_interval(g.st, a.tab_srmin[net], a.tab_delta[net], a.tab_n, x0);
+          const int voff = row * (6 * 64 * T3) + vq16;
+          const int tb = __builtin_amdgcn_readfirstlane(net * a.tab_stride * 4) + opaque_zero();
+          const rsrc_t trs = __builtin_amdgcn_make_buffer_rsrc((void*)a.tab, (short)0, a.tab_bytes, 0x00020000);
+          wave_sync();  // Rl visible to the dh3 products
+          floatx4 dr = floatx4{0.f, 0.f, 0.f, 0.f};
+          float ds = 0.f;
+          for_groups<0, T3, kGroup>([&](auto g0c, auto szc) {
+            constexpr int g0 = decltype(g0c)::value, SZ = decltype(szc)::value;
+            float h3[4 * SZ], dp[4 * SZ];
 #pragma unroll
-        for (int r = 0; r < 4 * T1; ++r) h1[r] = r < V1 ? tanh_f(g.st * k0(r) + b0(r)) : 0.f;
-        // h2 = t2 + h1[r/Z2] and h3 = t3 + h2[r/Z3] are re-formed where used
-        auto H2 = [&](int r, const float* t2v) -> float {
-          return t2v[r] + ((r / Z2 < 4 * T1) ? h1[r / Z2] : 0.f);
-        };
-        floatx4 acc2[T2];
+            for (int k = 0; k < SZ; ++k) {
+              floatx4 cf[6];
 #pragma unroll
-        for (int t = 0; t < T2; ++t) acc2[t] = floatx4{0.f, 0.f, 0.f, 0.f};
-        if constexpr (PREC == 0)
-          mfma_chain<T2, 4 * T1, 0, V1>(wrs, nb + 4 * O_W1A, vl4, [&](int s_) { return h1[s_]; }, acc2);
-        else
-          mfma_chain_h<PREC, T2, S21, 0, S21>(hrs, hb + H_W1A, vl16, [&](int s2) {
-            return pack8<PREC, 4 * T1>([&](int r) { return h1[r]; }, s2); }, acc2);
-        float t2[4 * T2];
+              for (int cc = 0; cc < 6; ++cc) cf[cc] = tab_load(trs, voff + 64 * (g0 + k), tb + cc * 64 * T3);
 #pragma unroll
-        for (int r = 0; r < 4 * T2; ++r) t2[r] = r < V2 ? tanh_f(acc2[r / 4][r % 4] + b1(r)) : 0.f;
-        wave_sync();  // Rl visible to the dh3 products
-        floatx4 dr = floatx4{0.f, 0.f, 0.f, 0.f};
-        floatx4 dh2[T2];
+              for (int i = 0; i < 4; ++i) {
+                float v = cf[0][i], d = 5.f * cf[0][i];
 #pragma unroll
-        for (int t = 0; t < T2; ++t) dh2[t] = floatx4{0.f, 0.f, 0.f, 0.f};
-        for_groups<0, T3, kGroup>([&](auto g0c, auto szc) {
-          constexpr int g0 = decltype(g0c)::value, SZ = decltype(szc)::value;
-          floatx4 acc3[SZ];
+                for (int cc = 1; cc < 5; ++cc) {
+                  v = __builtin_fmaf(v, x0, cf[cc][i]);
+                  d = __builtin_fmaf(d, x0, (float)(5 - cc) * cf[cc][i]);
+                }
+                h3[4 * k + i] = __builtin_fmaf(v, x0, cf[5][i]);
+                dp[4 * k + i] = d;
+              }
+            }
+            // dR^T[x][nbr] += sum_slot dT[x][slot] h3[slot][nbr]
 #pragma unroll
-          for (int t = 0; t < SZ; ++t) acc3[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+            for (int k = 0; k < SZ; ++k)
+#pragma unroll
+              for (int i = 0; i < 4; ++i)
+                dr = mfma4(n < 10 ? dTp[n * HS + 16 * (g0 + k) + 4 * q + i] : 0.f, h3[4 * k + i], dr);
+            // dh3[slot][nbr] = sum_x dT[x][slot] R[nbr][x]
+#pragma unroll
+            for (int k = 0; k < SZ; ++k) {
+              floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+              for (int kk = 0; kk < 3; ++kk)
+                acc = mfma4(4 * kk + q < 10 ? dTp[(4 * kk + q) * HS + 16 * (g0 + k) + n] : 0.f, Rl[n * kRS + 4 * kk + q], acc);
+#pragma unroll
+              for (int i = 0; i < 4; ++i) ds = __builtin_fmaf(acc[i], dp[4 * k + i], ds);
+            }
+          });
+#pragma unroll
+          for (int i = 0; i < 4; ++i) dRl[n * kRS + 4 * q + i] = dr[i];
+          ds += __shfl_xor(ds, 16);
+          ds += __shfl_xor(ds, 32);
+          dst = ds;
+        } else {
+          float h1[4 * T1];
+#pragma unroll
+          for (int r = 0; r < 4 * T1; ++r) h1[r] = r < V1 ? tanh_f(g.st * k0(r) + b0(r)) : 0.f;
+          // h2 = t2 + h1[r/Z2] and h3 = t3 + h2[r/Z3] are re-formed where used
+          auto H2 = [&](int r, const float* t2v) -> float {
+            return t2v[r] + ((r / Z2 < 4 * T1) ? h1[r / Z2] : 0.f);
+          };
+          floatx4 acc2[T2];
+#pragma unroll
+          for (int t = 0; t < T2; ++t) acc2[t] = floatx4{0.f, 0.f, 0.f, 0.f};
           if constexpr (PREC == 0)
-            mfma_chain<SZ, 4 * T2, 0, V2>(wrs, nb + 4 * (O_W2A + g0 * 4 * T2 * 64), vl4,
-                                          [&](int s_) { return H2(s_, t2); }, acc3);
+            mfma_chain<T2, 4 * T1, 0, V1>(wrs, nb + 4 * O_W1A, vl4, [&](int s_) { return h1[s_]; }, acc2);
           else
-            mfma_chain_h<PREC, SZ, S22, 0, S22>(hrs, hb + H_W2A + g0 * S22 * 1024, vl16, [&](int s2) {
-              return pack8<PREC, 4 * T2>([&](int r) { return H2(r, t2); }, s2); }, acc3);
-          float t3[4 * SZ];
+            mfma_chain_h<PREC, T2, S21, 0, S21>(hrs, hb + H_W1A, vl16, [&](int s2) {
+              return pack8<PREC, 4 * T1>([&](int r) { return h1[r]; }, s2); }, acc2);
+          float t2[4 * T2];
 #pragma unroll
-          for (int k = 0; k < SZ; ++k)
+          for (int r = 0; r < 4 * T2; ++r) t2[r] = r < V2 ? tanh_f(acc2[r / 4][r % 4] + b1(r)) : 0.f;
+          wave_sync();  // Rl visible to the dh3 products
+          floatx4 dr = floatx4{0.f, 0.f, 0.f, 0.f};
+          floatx4 dh2[T2];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) t3[4 * k + i] = tanh_f(acc3[k][i] + b2(4 * (g0 + k) + i));
-          // dR^T[x][nbr] += sum_slot dT[x][slot] h3[slot][nbr] over this group's slots
+          for (int t = 0; t < T2; ++t) dh2[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+          for_groups<0, T3, kGroup>([&](auto g0c, auto szc) {
+            constexpr int g0 = decltype(g0c)::value, SZ = decltype(szc)::value;
+            floatx4 acc3[SZ];
 #pragma unroll
-          for (int k = 0; k < SZ; ++k)
+            for (int t = 0; t < SZ; ++t) acc3[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+            if constexpr (PREC == 0)
+              mfma_chain<SZ, 4 * T2, 0, V2>(wrs, nb + 4 * (O_W2A + g0 * 4 * T2 * 64), vl4,
+                                            [&](int s_) { return H2(s_, t2); }, acc3);
+            else
+              mfma_chain_h<PREC, SZ, S22, 0, S22>(hrs, hb + H_W2A + g0 * S22 * 1024, vl16, [&](int s2) {
+                return pack8<PREC, 4 * T2>([&](int r) { return H2(r, t2); }, s2); }, acc3);
+            float t3[4 * SZ];
+#pragma unroll
+            for (int k = 0; k < SZ; ++k)
+#pragma unroll
+              for (int i = 0; i < 4; ++i) t3[4 * k + i] = tanh_f(acc3[k][i] + b2(4 * (g0 + k) + i));
+            // dR^T[x][nbr] += sum_slot dT[x][slot] h3[slot][nbr] over this group's slots
+#pragma unroll
+            for (int k = 0; k < SZ; ++k)
+#pragma unroll
+              for (int i = 0; i < 4; ++i) {
+                const int r = 4 * (g0 + k) + i;
+                const float h3v = t3[4 * k + i] + ((r / Z3 < 4 * T2) ? H2(r / Z3, t2) : 0.f);
+                dr = mfma4(n < 10 ? dTp[n * HS + 16 * (g0 + k) + 4 * q + i] : 0.f, h3v, dr);
+              }
+            // dh3[slot][nbr] = sum_x dT[x][slot] R[nbr][x]
+            floatx4 dh3[SZ];
+#pragma unroll
+            for (int k = 0; k < SZ; ++k) {
+              floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+              for (int kk = 0; kk < 3; ++kk)
+                acc = mfma4(4 * kk + q < 10 ? dTp[(4 * kk + q) * HS + 16 * (g0 + k) + n] : 0.f, Rl[n * kRS + 4 * kk + q], acc);
+              dh3[k] = acc;
+            }
+            // repeat-residual: dh2[r/Z3] += dh3[r]; then dz3 = dh3 (1 - t3^2)
+#pragma unroll
+            for (int k = 0; k < SZ; ++k)
+#pragma unroll
+              for (int i = 0; i < 4; ++i) {
+                const int r = 4 * (g0 + k) + i;
+                if (r / Z3 < 4 * T2) dh2[(r / Z3) / 4][(r / Z3) % 4] += dh3[k][i];
+                dh3[k][i] *= (1.f - t3[4 * k + i] * t3[4 * k + i]);
+              }
+            if constexpr (PREC == 0)
+              mfma_chain<T2, 4 * T3, 4 * g0, 4 * SZ>(wrs, nb + 4 * O_W2B, vl4,
+                                                    [&](int s_) { return dh3[(s_ - 4 * g0) / 4][(s_ - 4 * g0) % 4]; }, dh2);
+            else
+              mfma_chain_h<PREC, T2, S23, g0 / 2, 1>(hrs, hb + H_W2B, vl16, [&](int s2) {
+                return pack8<PREC, 4 * SZ>([&](int r) { return dh3[r / 4][r % 4]; }, 0); }, dh2);
+          });
+#pragma unroll
+          for (int i = 0; i < 4; ++i) dRl[n * kRS + 4 * q + i] = dr[i];
+          // layer 2 backward
+          floatx4 dh1[T1];
+#pragma unroll
+          for (int at = 0; at < T1; ++at) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-              const int r = 4 * (g0 + k) + i;
-              const float h3v = t3[4 * k + i] + ((r / Z3 < 4 * T2) ? H2(r / Z3, t2) : 0.f);
-              dr = mfma4(n < 10 ? dTp[n * HS + 16 * (g0 + k) + 4 * q + i] : 0.f, h3v, dr);
+              const int r1 = 4 * at + i;
+              float v = 0.f;
+              if (Z2 == 2) {
+                if (2 * r1 < 4 * T2) v += dh2[(2 * r1) / 4][(2 * r1) % 4];
+                if (2 * r1 + 1 < 4 * T2) v += dh2[(2 * r1 + 1) / 4][(2 * r1 + 1) % 4];
+              } else {
+                if (r1 < 4 * T2) v += dh2[r1 / 4][r1 % 4];
+              }
+              dh1[at][i] = v;
             }
-          // dh3[slot][nbr] = sum_x dT[x][slot] R[nbr][x]
-          floatx4 dh3[SZ];
-#pragma unroll
-          for (int k = 0; k < SZ; ++k) {
-            floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int kk = 0; kk < 3; ++kk)
-              acc = mfma4(4 * kk + q < 10 ? dTp[(4 * kk + q) * HS + 16 * (g0 + k) + n] : 0.f, Rl[n * kRS + 4 * kk + q], acc);
-            dh3[k] = acc;
           }
-          // repeat-residual: dh2[r/Z3] += dh3[r]; then dz3 = dh3 (1 - t3^2)
 #pragma unroll
-          for (int k = 0; k < SZ; ++k)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-              const int r = 4 * (g0 + k) + i;
-              if (r / Z3 < 4 * T2) dh2[(r / Z3) / 4][(r / Z3) % 4] += dh3[k][i];
-              dh3[k][i] *= (1.f - t3[4 * k + i] * t3[4 * k + i]);
-            }
+          for (int r = 0; r < 4 * T2; ++r) dh2[r / 4][r % 4] *= (1.f - t2[r] * t2[r]);
           if constexpr (PREC == 0)
-            mfma_chain<T2, 4 * T3, 4 * g0, 4 * SZ>(wrs, nb + 4 * O_W2B, vl4,
-                                                  [&](int s_) { return dh3[(s_ - 4 * g0) / 4][(s_ - 4 * g0) % 4]; }, dh2);
+            mfma_chain<T1, 4 * T2, 0, V2>(wrs, nb + 4 * O_W1B, vl4, [&](int s_) { return dh2[s_ / 4][s_ % 4]; }, dh1);
           else
-            mfma_chain_h<PREC, T2, S23, g0 / 2, 1>(hrs, hb + H_W2B, vl16, [&](int s2) {
-              return pack8<PREC, 4 * SZ>([&](int r) { return dh3[r / 4][r % 4]; }, 0); }, dh2);
-        });
+            mfma_chain_h<PREC, T1, S22, 0, S22>(hrs, hb + H_W1B, vl16, [&](int s2) {
+              return pack8<PREC, 4 * T2>([&](int r) { return dh2[r / 4][r % 4]; }, s2); }, dh1);
+          dst = 0.f;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) dRl[n * kRS + 4 * q + i] = dr[i];
-        // layer 2 backward
-        floatx4 dh1[T1];
-#pragma unroll
-        for (int at = 0; at < T1; ++at) {
-#pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const int r1 = 4 * at + i;
-            float v = 0.f;
-            if (Z2 == 2) {
-              if (2 * r1 < 4 * T2) v += dh2[(2 * r1) / 4][(2 * r1) % 4];
-              if (2 * r1 + 1 < 4 * T2) v += dh2[(2 * r1 + 1) / 4][(2 * r1 + 1) % 4];
-            } else {
-              if (r1 < 4 * T2) v += dh2[r1 / 4][r1 % 4];
-            }
-            dh1[at][i] = v;
-          }
+          for (int r1 = 0; r1 < 4 * T1; ++r1) dst += dh1[r1 / 4][r1 % 4] * (1.f - h1[r1] * h1[r1]) * k0(r1);
+          dst += __shfl_xor(dst, 16);
+          dst += __shfl_xor(dst, 32);
         }
-#pragma unroll
-        for (int r = 0; r < 4 * T2; ++r) dh2[r / 4][r % 4] *= (1.f - t2[r] * t2[r]);
-        if constexpr (PREC == 0)
-          mfma_chain<T1, 4 * T2, 0, V2>(wrs, nb + 4 * O_W1B, vl4, [&](int s_) { return dh2[s_ / 4][s_ % 4]; }, dh1);
-        else
-          mfma_chain_h<PREC, T1, S22, 0, S22>(hrs, hb + H_W1B, vl16, [&](int s2) {
-            return pack8<PREC, 4 * T2>([&](int r) { return dh2[r / 4][r % 4]; }, s2); }, dh1);
-        float dst = 0.f;
-#pragma unroll
-        for (int r1 = 0; r1 < 4 * T1; ++r1) dst += dh1[r1 / 4][r1 % 4] * (1.f - h1[r1] * h1[r1]) * k0(r1);
-        dst += __shfl_xor(dst, 16);
-        dst += __shfl_xor(dst, 32);
         wave_sync();
         float dR[10];
 #pragma unroll
@@ -954,9 +1063,19 @@ static void fill_args(System* s, EmbedArgs& a) {
   int cmax = 1;
   for (int j = 0; j < s->ntypes; ++j) cmax = std::max(cmax, s->cap[j]);
   a.list_cap = (cmax + 3) / 4 * 4;
+  if (M->tab_n > 0) {
+    a.tab = M->d_tab;
+    a.tab_n = M->tab_n;
+    a.tab_stride = M->tab_stride;
+    a.tab_bytes = (int)((size_t)M->tab_stride * 4 * M->ntypes * M->ntypes);
+    for (int k = 0; k < M->ntypes * M->ntypes; ++k) {
+      a.tab_srmin[k] = M->tab_srmin[k];
+      a.tab_delta[k] = M->tab_delta[k];
+    }
+  }
 }
 
-template <int T1, int T2, int T3, int Z2, int Z3, int PREC, int V1>
+template <int T1, int T2, int T3, int Z2, int Z3, int PREC, int V1, bool TAB>
 static int32_t launch_embed(System* s, bool backward) {
   EmbedArgs a;
   fill_args(s, a);
@@ -989,19 +1108,22 @@ static int32_t launch_embed(System* s, bool backward) {
     return DPMD_E_UNSUPPORTED;
   }
   if (!backward) {
-    hipLaunchKernelGGL((embed_fwd_kernel<T1, T2, T3, Z2, Z3, PREC, V1>), dim3(nblocks), dim3(256), lds, s->stream, a);
+    hipLaunchKernelGGL((embed_fwd_kernel<T1, T2, T3, Z2, Z3, PREC, V1, TAB>), dim3(nblocks), dim3(256), lds, s->stream, a);
   } else {
-    hipLaunchKernelGGL((embed_bwd_kernel<T1, T2, T3, Z2, Z3, PREC, V1>), dim3(nblocks), dim3(256), lds, s->stream, a);
+    hipLaunchKernelGGL((embed_bwd_kernel<T1, T2, T3, Z2, Z3, PREC, V1, TAB>), dim3(nblocks), dim3(256), lds, s->stream, a);
   }
   DPMD_HIP(hipGetLastError());
   return DPMD_OK;
 }
 
+// A tabulated (compressed) model has no embedding GEMMs: its kernel is FP32 whatever the
+// precision of the fitting GEMMs.
 template <int T1, int T2, int T3, int Z2, int Z3, int V1>
 static int32_t launch_prec(System* s, bool backward, int precision) {
-  if (precision == DPMD_PREC_BF16) return launch_embed<T1, T2, T3, Z2, Z3, 1, V1>(s, backward);
-  if (precision == DPMD_PREC_FP16) return launch_embed<T1, T2, T3, Z2, Z3, 2, V1>(s, backward);
-  return launch_embed<T1, T2, T3, Z2, Z3, 0, V1>(s, backward);
+  if (s->model->tab_n > 0) return launch_embed<T1, T2, T3, Z2, Z3, 0, V1, true>(s, backward);
+  if (precision == DPMD_PREC_BF16) return launch_embed<T1, T2, T3, Z2, Z3, 1, V1, false>(s, backward);
+  if (precision == DPMD_PREC_FP16) return launch_embed<T1, T2, T3, Z2, Z3, 2, V1, false>(s, backward);
+  return launch_embed<T1, T2, T3, Z2, Z3, 0, V1, false>(s, backward);
 }
 
 int32_t embed_launch(System* s, bool backward, int precision) {
@@ -1026,6 +1148,18 @@ EmbedNetOffsetsH embed_net_offsets_h(const EmbedLayout& L) {
   o.w1b = o.w2b + t2 * s23 * 512;
   o.total = o.w1b + t1 * s22 * 512;
   return o;
+}
+
+// Tabulated net coefficients (ngrids, 6, W) -> [interval][power][16*tiles3] in the
+// layer-3 physical slot order (padding slots zero).
+void embed_pack_table(const EmbedLayout& L, int ngrids, const float* coeff, float* out) {
+  const int S = 16 * L.tiles[2], W = L.w[2];
+  for (int g = 0; g < ngrids; ++g)
+    for (int c = 0; c < 6; ++c)
+      for (int slot = 0; slot < S; ++slot) {
+        const int w = L.logical(2, slot);
+        out[((size_t)g * 6 + c) * S + slot] = w >= 0 ? coeff[((size_t)g * 6 + c) * W + w] : 0.f;
+      }
 }
 
 static uint16_t to_bf16(float f) {

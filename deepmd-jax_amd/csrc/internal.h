@@ -81,7 +81,14 @@ struct Model {
   float fit_b3[kMaxT] = {0, 0, 0, 0};
   int fit_n = 0, fit_k = 0;  // output width (x16) and K stride (x32) of the hidden GEMMs
   uint16_t* d_fit_h[2] = {nullptr, nullptr};  // half-precision copies of kernels
+  // tabulated embedding nets (compressed model): per net [interval][6 powers][16*tiles3]
+  // in the layer-3 physical slot order, so one float4 load fills four accumulator rows
+  int tab_n = 0;                   // intervals; 0 = evaluate the MLPs
+  int tab_stride = 0;              // floats per net
+  float tab_srmin[kMaxT * kMaxT], tab_delta[kMaxT * kMaxT];
+  float* d_tab = nullptr;
 };
+void embed_pack_table(const EmbedLayout& L, int ngrids, const float* coeff, float* out);
 
 struct Timer {
   bool on = false;

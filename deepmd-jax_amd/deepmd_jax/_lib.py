@@ -23,7 +23,9 @@ class ModelDesc(ctypes.Structure):
                 ('rcut', c_f64), ('sr_mean', c_f64 * MAXT), ('sr_std', c_f64 * MAXT), ('nnbrs', c_f64),
                 ('ebias', c_f64 * MAXT), ('out_norm', c_f64), ('tbias', c_vp),
                 ('emb_kernel', (c_vp * 3) * (MAXT * MAXT)), ('emb_bias', (c_vp * 3) * (MAXT * MAXT)),
-                ('fit_kernel', (c_vp * 4) * MAXT), ('fit_bias', (c_vp * 4) * MAXT), ('fit_dt', (c_vp * 3) * MAXT)]
+                ('fit_kernel', (c_vp * 4) * MAXT), ('fit_bias', (c_vp * 4) * MAXT), ('fit_dt', (c_vp * 3) * MAXT),
+                ('tab_ngrids', c_i32), ('tab_srmin', ctypes.c_float * (MAXT * MAXT)),
+                ('tab_srmax', ctypes.c_float * (MAXT * MAXT)), ('tab_coeff', c_vp * (MAXT * MAXT))]
 
 
 class MDParams(ctypes.Structure):
@@ -130,8 +132,6 @@ class DeviceModel:
             raise NotImplementedError('only the se_a energy model (no message passing, no atomic output) is supported')
         if not params.get('use_2nd', True):
             raise NotImplementedError('use_2nd=False is not supported')
-        if params.get('is_compressed', False):
-            raise NotImplementedError('compressed (tabulated) embedding nets are not supported yet')
         if list(params.get('valid_types', range(nt))) != list(range(nt)):
             raise NotImplementedError('models with valid_types != range(ntypes) are not supported')
         ew = list(params['embed_widths'])
@@ -159,11 +159,36 @@ class DeviceModel:
             return a.ctypes.data
 
         d.tbias = arr(P['Tbias'])
+        compressed = bool(params.get('is_compressed', False))
         for k in range(nt * nt):
-            net = P['embedding_net_%d' % k]
+            net = P.get('embedding_net_%d' % k)
+            if net is None:
+                if not compressed:
+                    raise ValueError('embedding_net_%d missing from the model variables' % k)
+                continue
             for l in range(3):
                 d.emb_kernel[k][l] = arr(net['Dense_%d' % l]['kernel'])
                 d.emb_bias[k][l] = arr(net['Dense_%d' % l]['bias'])
+        if compressed:
+            # tabulated nets of compress_model (deepmd_jax/utils.py:250-297)
+            cvs = variables.get('compress_var')
+            if cvs is None:
+                raise ValueError("model.params['is_compressed'] is set but variables has no 'compress_var'")
+            ngrids = None
+            for k in range(nt * nt):
+                cv = cvs['embedding_net_%d' % k]
+                c = np.asarray(cv['poly_coeff'], np.float32)
+                if c.ndim != 3 or c.shape[1] != 6 or c.shape[2] != int(ew[-1]):
+                    raise ValueError('poly_coeff of embedding_net_%d has shape %s, expected (Ngrids, 6, %d)'
+                                     % (k, c.shape, int(ew[-1])))
+                if ngrids is None:
+                    ngrids = c.shape[0]
+                elif c.shape[0] != ngrids:
+                    raise NotImplementedError('tabulated nets with different interval counts')
+                d.tab_srmin[k] = float(np.float32(cv['srmin']))
+                d.tab_srmax[k] = float(np.float32(cv['srmax']))
+                d.tab_coeff[k] = arr(c)
+            d.tab_ngrids = int(ngrids)
         for t in range(nt):
             net = P['fitting_net_%d' % t]
             for l in range(4):

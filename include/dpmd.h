@@ -68,6 +68,15 @@ typedef struct {
   const float* fit_kernel[DPMD_MAX_TYPES][4];
   const float* fit_bias[DPMD_MAX_TYPES][4];
   const float* fit_dt[DPMD_MAX_TYPES][3];                          /* NULL where no residual */
+  /* Compressed (tabulated) embedding nets, variables['compress_var'] of a model that went
+   * through compress_model (deepmd_jax/utils.py:250-297).  tab_ngrids > 0 selects them:
+   * net k then evaluates the piecewise quintic tab_coeff[k] (tab_ngrids, 6, W), row c the
+   * coefficient of x0^(5-c), over [tab_srmin[k], tab_srmax[k]] (deepmd_jax/utils.py:126-151),
+   * and emb_kernel / emb_bias may be NULL.  tab_ngrids = 0: the MLPs above are evaluated. */
+  int32_t tab_ngrids;
+  float tab_srmin[DPMD_MAX_TYPES * DPMD_MAX_TYPES];
+  float tab_srmax[DPMD_MAX_TYPES * DPMD_MAX_TYPES];
+  const float* tab_coeff[DPMD_MAX_TYPES * DPMD_MAX_TYPES];
 } dpmd_model_desc;
 
 /* Integrator parameters for dpmd_md_run. Units: fs, eV, Angstrom. */

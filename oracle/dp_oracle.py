@@ -16,6 +16,9 @@ Function -> reference file:line it follows:
   relative_coord_lattice deepmd_jax/utils.py:66-78
   lattice_candidate     deepmd_jax/data.py:347-364
   embedding_net         deepmd_jax/utils.py:100-125 (uncompressed branch)
+  embedding_net_compressed deepmd_jax/utils.py:126-151 (tabulated branch)
+  compress_net          deepmd_jax/utils.py:250-286 (derivatives by autograd, the
+                        quintic Hermite by scipy's BPoly.from_derivatives, as there)
   fitting_net           deepmd_jax/utils.py:153-166
   dp_energy             deepmd_jax/dpmodel.py:30-81,100
   energy_force_virial   deepmd_jax/dpmodel.py:106-108; virial as the strain
@@ -116,6 +119,48 @@ def embedding_net(net, s, reducer):
     return reducer @ x
 
 
+def embedding_net_compressed(cv, s, reducer):
+    """Tabulated embedding net: s (N,M,1) -> reducer (N,X,M) @ p(s) (N,M,W).
+
+    Interval rule as written there: x = s - srmin, idx = int(x / delta) (truncation),
+    x0 = x - idx*delta, delta = (srmax - srmin)/Ngrids in the dtype the table is stored in
+    (float32 after compress_model); the coefficient gather clamps idx into range
+    (jax's out-of-bounds gather rule) while x0 keeps the unclamped idx.  No gradient flows
+    through idx."""
+    dtype = s.dtype
+    srmin, srmax = np.asarray(cv['srmin'])[()], np.asarray(cv['srmax'])[()]
+    coeff = torch.as_tensor(np.asarray(cv['poly_coeff'])).to(dtype)
+    n = coeff.shape[0]
+    delta = float((srmax - srmin) / n)  # in the stored dtype (float32 after compress_model)
+    x = s[..., 0] - float(srmin)
+    idx = torch.trunc((x / delta).detach()).to(torch.long)
+    x0 = x - idx.to(dtype) * delta
+    c = coeff[torch.clamp(idx, 0, n - 1)]  # (N,M,6,W)
+    embed = sum(c[..., i, :] * x0[..., None] ** (5 - i) for i in range(6))
+    return reducer @ embed
+
+
+def compress_net(net, sr_range, Ngrids):
+    """poly_coeff (Ngrids, 6, W) of one embedding net over s~ in sr_range, following
+    deepmd_jax/utils.py:273-286 in FP64: value, first and second derivative at Ngrids+1
+    nodes (here torch.func.jacfwd, there jax.jacfwd), then the piecewise quintic from
+    scipy.interpolate.BPoly.from_derivatives converted by PPoly.from_bernstein_basis,
+    coefficients transposed to (interval, power 5..0, W)."""
+    from scipy.interpolate import BPoly, PPoly
+    r = np.linspace(sr_range[0], sr_range[1], Ngrids + 1)
+    red = torch.ones(1, 1, 1, dtype=torch.float64)
+
+    def f(x):  # scalar -> (W,)
+        return embedding_net(net, x.reshape(1, 1, 1), red)[0, 0]
+
+    rt = torch.as_tensor(r)
+    f0 = torch.stack([f(x) for x in rt]).numpy()
+    f1 = torch.func.vmap(torch.func.jacfwd(f))(rt).numpy()
+    f2 = torch.func.vmap(torch.func.jacfwd(torch.func.jacfwd(f)))(rt).numpy()
+    poly = PPoly.from_bernstein_basis(BPoly.from_derivatives(r, np.stack([f0, f1, f2], axis=1), orders=None))
+    return poly.c.transpose(1, 0, 2), r
+
+
 def fitting_net(net, x):
     li = 0
     while 'Dense_%d' % (li + 1) in net:
@@ -172,7 +217,11 @@ def dp_energy(params, variables, coord, box3, type_idx, nbrs_nm=None, lattice=No
             R3 = 3 ** 0.5 * s_norm[:, None] * xn
             R6 = 3 * s_norm[:, None] * tensor_3to6(xn, axis=1, bias=1.0 / 3)
             RX = torch.cat([s_norm[:, None], R3, R6], dim=1)
-            Ti = Ti + embedding_net(P['embedding_net_%d' % (i * Y + j)], s_cn[:, :, None], RX)
+            if params.get('is_compressed', False):
+                Ti = Ti + embedding_net_compressed(variables['compress_var']['embedding_net_%d' % (i * Y + j)],
+                                                   s_cn[:, :, None], RX)
+            else:
+                Ti = Ti + embedding_net(P['embedding_net_%d' % (i * Y + j)], s_cn[:, :, None], RX)
         T_list.append(Ti)
     T = torch.cat(T_list, 0) / params['Nnbrs']
     TW = T[:, 0] + torch.as_tensor(P['Tbias'], dtype=dtype)

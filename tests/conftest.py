@@ -38,3 +38,21 @@ def model_pkl(tmp_path_factory, model0):
 def device_model(model0):
     from deepmd_jax import _lib
     return _lib.DeviceModel(model0[0], model0[1])
+
+
+@pytest.fixture(scope='session')
+def model0c(model0):
+    """The seeded model after compress_model (1,024 intervals, rmin 0.6 A: the defaults of
+    deepmd_jax/train.py:49-50)."""
+    import copy
+    from deepmd_jax.dpmodel import DPModel
+    from deepmd_jax.utils import compress_model
+    params, var = copy.deepcopy(model0)
+    m, var = compress_model(DPModel(params), var, 1024, 0.6)
+    return m.params, var
+
+
+@pytest.fixture(scope='session')
+def device_model_c(model0c):
+    from deepmd_jax import _lib
+    return _lib.DeviceModel(model0c[0], model0c[1])
