@@ -55,7 +55,32 @@ System::~System() {
   for (auto& c : caps)
     if (*c.first) hipFree(*c.first);
   if (d_nm) hipFree(d_nm);
+  for (hipEvent_t e : ev_frame) hipEventDestroy(e);
+  for (hipEvent_t e : ev_copied) hipEventDestroy(e);
+  if (h_traj) hipHostFree(h_traj);
+  if (copy_stream) hipStreamDestroy(copy_stream);
   if (stream) hipStreamDestroy(stream);
+}
+
+// Pinned staging for nsteps recorded frames and one pair of events per frame (grow-only).
+static int32_t traj_staging(System* s, int nsteps) {
+  const size_t bytes = sizeof(double) * (size_t)nsteps * s->NA * 6;
+  if (!s->copy_stream) DPMD_HIP(hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking));
+  if (s->h_traj_bytes < bytes) {
+    if (s->h_traj) DPMD_HIP(hipHostFree(s->h_traj));
+    s->h_traj = nullptr;
+    s->h_traj_bytes = 0;
+    DPMD_HIP(hipHostMalloc((void**)&s->h_traj, bytes, hipHostMallocDefault));
+    s->h_traj_bytes = bytes;
+  }
+  while ((int)s->ev_frame.size() < nsteps) {
+    hipEvent_t a, b;
+    DPMD_HIP(hipEventCreateWithFlags(&a, hipEventDisableTiming));
+    s->ev_frame.push_back(a);
+    DPMD_HIP(hipEventCreateWithFlags(&b, hipEventDisableTiming));
+    s->ev_copied.push_back(b);
+  }
+  return DPMD_OK;
 }
 
 // ------------------------------------------------------------------------------
@@ -362,6 +387,7 @@ static int32_t sys_layout(System* s, int nseg, const int32_t* seg_natoms, const 
   DPMD_TRY(sys_ensure(s, (void**)&s->d_partial, sizeof(double) * 9 * s->nblocks));
   DPMD_TRY(sys_ensure(s, (void**)&s->d_red, sizeof(double) * (32 * (size_t)nseg + 64)));
   DPMD_TRY(sys_ensure(s, (void**)&s->d_pairs, sizeof(int) * NA));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_perm, sizeof(int) * NA));
   hipStream_t st = s->stream;
   DPMD_HIP(hipMemsetAsync(s->d_pos, 0, sizeof(double) * NA * 3, st));
   DPMD_HIP(hipMemsetAsync(s->d_mom, 0, sizeof(double) * NA * 3, st));
@@ -375,6 +401,7 @@ static int32_t sys_layout(System* s, int nseg, const int32_t* seg_natoms, const 
   DPMD_HIP(hipMemcpyAsync(s->d_atom_seg, atom_seg.data(), sizeof(int) * NA, hipMemcpyHostToDevice, st));
   DPMD_HIP(hipMemcpyAsync(s->d_blk_seg, s->blk_seg.data(), sizeof(int) * s->nblocks, hipMemcpyHostToDevice, st));
   DPMD_HIP(hipMemcpyAsync(s->d_mass, s->mass.data(), sizeof(double) * NA, hipMemcpyHostToDevice, st));
+  DPMD_HIP(hipMemcpyAsync(s->d_perm, s->perm.data(), sizeof(int) * NA, hipMemcpyHostToDevice, st));
   DPMD_HIP(hipStreamSynchronize(st));
   s->seg_dirty = true;
   s->nl_ready = false;
@@ -811,7 +838,10 @@ int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps, 
   if (p->routine == DPMD_PIMD_PILE_L && !s->uniform) { set_error("PIMD needs identical beads"); return DPMD_E_STATE; }
   const size_t NA = s->NA;
   const bool rec = traj_pos || traj_vel;
-  if (rec) DPMD_TRY(sys_ensure(s, (void**)&s->d_traj, sizeof(double) * (size_t)nsteps * NA * 6));
+  if (rec) {
+    DPMD_TRY(sys_ensure(s, (void**)&s->d_traj, sizeof(double) * (size_t)nsteps * NA * 6));
+    DPMD_TRY(traj_staging(s, nsteps));
+  }
   DPMD_HIP(hipMemcpyAsync(s->d_pos_save, s->d_pos, sizeof(double) * NA * 3, hipMemcpyDeviceToDevice, s->stream));
   DPMD_HIP(hipMemcpyAsync(s->d_mom_save, s->d_mom, sizeof(double) * NA * 3, hipMemcpyDeviceToDevice, s->stream));
   if (s->nl_overflow) {
@@ -875,10 +905,32 @@ int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps, 
       DPMD_TRY(md_kick(s, 0.5 * dt));
     }
     if (rec) {
-      DPMD_HIP(hipMemcpyAsync(s->d_traj + (size_t)step * NA * 6, s->d_pos, sizeof(double) * NA * 3,
-                              hipMemcpyDeviceToDevice, s->stream));
-      DPMD_HIP(hipMemcpyAsync(s->d_traj + (size_t)step * NA * 6 + NA * 3, s->d_mom, sizeof(double) * NA * 3,
-                              hipMemcpyDeviceToDevice, s->stream));
+      // frame -> caller order on the device, then to pinned memory on the copy stream
+      double* frame = s->d_traj + (size_t)step * NA * 6;
+      DPMD_TRY(md_traj_record(s, frame, frame + NA * 3));
+      DPMD_HIP(hipEventRecord(s->ev_frame[step], s->stream));
+      DPMD_HIP(hipStreamWaitEvent(s->copy_stream, s->ev_frame[step], 0));
+      DPMD_HIP(hipMemcpyAsync(s->h_traj + (size_t)step * NA * 6, frame, sizeof(double) * NA * 6,
+                              hipMemcpyDeviceToHost, s->copy_stream));
+      DPMD_HIP(hipEventRecord(s->ev_copied[step], s->copy_stream));
+    }
+  }
+  if (rec) {
+    // copy each frame out of pinned memory while the device computes the later steps;
+    // on an overflow the frames are discarded by the caller's re-run of the chunk.  The
+    // destination pages are touched first (a fresh trajectory array is mapped lazily), so
+    // the page faults fall while the first steps compute.
+    const size_t n3 = (size_t)nsteps * NA * 3, stride = 4096 / sizeof(double);
+    for (double* dst : {traj_pos, traj_vel}) {
+      if (!dst || n3 == 0) continue;
+      for (size_t i = 0; i < n3; i += stride) dst[i] = 0.0;
+      dst[n3 - 1] = 0.0;
+    }
+    for (int k = 0; k < nsteps; ++k) {
+      DPMD_HIP(hipEventSynchronize(s->ev_copied[k]));
+      const double* hp = s->h_traj + (size_t)k * NA * 6;
+      if (traj_pos) memcpy(traj_pos + (size_t)k * NA * 3, hp, sizeof(double) * NA * 3);
+      if (traj_vel) memcpy(traj_vel + (size_t)k * NA * 3, hp + NA * 3, sizeof(double) * NA * 3);
     }
   }
   int h[16];
@@ -902,21 +954,6 @@ int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps, 
   s->rng_counter = counter0 + nsteps;
   for (SegInfo& S : s->seg) S.step += nsteps;
   s->seg_dirty = true;
-  if (rec) {
-    std::vector<double> buf((size_t)nsteps * NA * 6);
-    DPMD_HIP(hipMemcpyAsync(buf.data(), s->d_traj, sizeof(double) * buf.size(), hipMemcpyDeviceToHost, s->stream));
-    DPMD_HIP(hipStreamSynchronize(s->stream));
-    for (int k = 0; k < nsteps; ++k) {
-      double* bp = buf.data() + (size_t)k * NA * 6;
-      double* bm = bp + NA * 3;
-      if (traj_pos) from_sorted(s, bp, traj_pos + (size_t)k * NA * 3);
-      if (traj_vel) {
-        for (size_t g = 0; g < NA; ++g)
-          for (int d = 0; d < 3; ++d) bm[g * 3 + d] /= s->mass[g];
-        from_sorted(s, bm, traj_vel + (size_t)k * NA * 3);
-      }
-    }
-  }
   return DPMD_OK;
 }
 

@@ -126,6 +126,11 @@ struct RowBase {
 struct System {
   Model* model = nullptr;
   hipStream_t stream = nullptr;
+  // trajectory frames leave the device on their own stream while later steps compute
+  hipStream_t copy_stream = nullptr;
+  double* h_traj = nullptr;     // pinned staging of recorded frames
+  size_t h_traj_bytes = 0;
+  std::vector<hipEvent_t> ev_frame, ev_copied;
   int ntypes = 0;
   int nseg = 0;
   int NA = 0;                   // atoms over all segments
@@ -184,7 +189,9 @@ struct System {
   bool force_valid = false;
   // integrator
   double* d_thermo = nullptr;   // 64 doubles per segment
-  double* d_traj = nullptr;     // device trajectory staging
+  double* d_traj = nullptr;     // device trajectory staging: positions then velocities,
+                                // (steps, NA, 3) each, caller order
+  int* d_perm = nullptr;        // (NA) sorted index -> caller index
   double* d_nm = nullptr;       // PIMD normal modes
   uint64_t rng_counter = 0;
   Timer timer;
@@ -207,6 +214,7 @@ int32_t md_zero_force(System* s);
 int32_t md_kick(System* s, double dt_half);
 int32_t md_drift(System* s, double dt);
 int32_t md_write_pos32(System* s);
+int32_t md_traj_record(System* s, double* tpos, double* tvel);
 int32_t md_reduce_launch(System* s);
 int32_t md_ke_launch(System* s, double* d_out);
 int32_t md_langevin_o(System* s, double dt, double gamma, double kT, uint64_t seed, uint64_t step);

@@ -291,6 +291,26 @@ int32_t md_write_pos32(System* s) {
   return DPMD_OK;
 }
 
+// One recorded trajectory frame in the caller's atom order: positions and velocities
+// (momentum / mass), as Simulation.run returns them (deepmd_jax/md.py:806-964).
+__global__ void traj_record_kernel(const double* pos, const double* mom, const double* mass, const int* perm,
+                                   double* tpos, double* tvel, int n3) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n3) return;
+  const int g = i / 3, d = i - 3 * g;
+  const size_t o = (size_t)perm[g] * 3 + d;
+  tpos[o] = pos[i];
+  tvel[o] = mom[i] / mass[g];
+}
+
+int32_t md_traj_record(System* s, double* tpos, double* tvel) {
+  const int n3 = s->NA * 3;
+  hipLaunchKernelGGL(traj_record_kernel, dim3((n3 + 255) / 256), dim3(256), 0, s->stream, s->d_pos, s->d_mom,
+                     s->d_mass, s->d_perm, tpos, tvel, n3);
+  DPMD_HIP(hipGetLastError());
+  return DPMD_OK;
+}
+
 int32_t md_zero_force(System* s) {
   const size_t n = (size_t)s->NA * 3;
   hipLaunchKernelGGL(zero_kernel, dim3(std::min<size_t>(2048, (n + 255) / 256)), dim3(256), 0, s->stream,

@@ -344,9 +344,19 @@ class DeviceSystem:
         check(lib().dpmd_compute(self.handle, PREC[precision], ptr(e), ptr(f), ptr(v), ptr(ea)))
         return e, f, v.reshape(self.nrep, 3, 3), ea
 
-    def md_run(self, params, nsteps, record=True):
-        tp = np.empty((nsteps,) + self._shape(3)) if record else None
-        tv = np.empty((nsteps,) + self._shape(3)) if record else None
+    def md_run(self, params, nsteps, record=True, out=None):
+        """Run nsteps device steps; with record, the per-step positions and velocities in
+        caller order, written into `out` = (pos, vel) when given (C-contiguous float64 arrays
+        of shape (nsteps,) + the system's (.., 3) shape, e.g. slices of a trajectory)."""
+        shape = (nsteps,) + self._shape(3)
+        if record and out is not None:
+            tp, tv = out
+            for a in (tp, tv):
+                if a.shape != shape or a.dtype != np.float64 or not a.flags.c_contiguous:
+                    raise ValueError('md_run out arrays must be C-contiguous float64 of shape %s' % (shape,))
+        else:
+            tp = np.empty(shape) if record else None
+            tv = np.empty(shape) if record else None
         err = np.zeros(1, np.uint32)
         check(lib().dpmd_md_run(self.handle, ctypes.byref(params), int(nsteps), ptr(tp), ptr(tv), ptr(err)))
         return tp, tv, int(err[0])

@@ -75,10 +75,18 @@ class _ImageSystem:
         e, f, v, ea = self.dev.compute(precision)
         return e, f[:, :self.N], v, ea
 
-    def md_run(self, params, nsteps, record=True):
+    def md_run(self, params, nsteps, record=True, out=None):
+        """out = (pos, vel) arrays of shape (nsteps, nrep, N, 3) receive the frames; with
+        images held, the primary cell is copied out of the device frames."""
+        if self.nimg == 1:
+            return self.dev.md_run(params, nsteps, record, out)
         tp, tv, err = self.dev.md_run(params, nsteps, record)
         if record:
             tp, tv = tp[:, :, :self.N], tv[:, :, :self.N]
+            if out is not None:
+                out[0][...] = tp
+                out[1][...] = tv
+                tp, tv = out
         return tp, tv, err
 
     @property
@@ -321,7 +329,12 @@ class Simulation:
         remaining = steps
         while remaining > 0:
             nxt = min(self.report_interval - self.step % self.report_interval, self._step_chunk_size, remaining)
-            tp, tv, err = self._sys.md_run(self._params, nxt, record=True)
+            lo = self.step - self._offset
+            # the device writes the chunk's frames straight into the trajectory arrays
+            shp = (nxt, self._n_bead, self._natoms, 3)
+            out = (self._position_trajectory[lo:lo + nxt].reshape(shp),
+                   self._velocity_trajectory[lo:lo + nxt].reshape(shp))
+            tp, tv, err = self._sys.md_run(self._params, nxt, record=True, out=out)
             if err & 16:
                 print('# Warning: Nan or Inf encountered in simulation. Terminating.')
                 self._error_code |= 16
@@ -332,15 +345,9 @@ class Simulation:
                 continue
             if not err & 16:
                 self._error_code &= ~4
-            lo = self.step - self._offset
             if self._is_pimd:
-                self._position_trajectory[lo:lo + nxt] = tp
-                self._velocity_trajectory[lo:lo + nxt] = tv
                 for k in range(nxt):
                     self._centroid_trajectory[lo + k] = centroid(tp[k], self._box)
-            else:
-                self._position_trajectory[lo:lo + nxt] = tp[:, 0]
-                self._velocity_trajectory[lo:lo + nxt] = tv[:, 0]
             self._box_trajectory[lo:lo + nxt] = self._box
             self.step += nxt
             remaining -= nxt

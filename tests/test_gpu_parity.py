@@ -316,3 +316,41 @@ def test_simulation_small_box_images(device_model, model0):
         pm += 0.25 * F
     assert np.abs(traj['position'][-1] - x).max() < 1e-5
     assert np.abs(traj['velocity'][-1] - pm / m[:, None]).max() < 1e-5
+
+
+def test_simulation_trajectory_frames(device_model, model0):
+    """Simulation.run on a box that needs no images (216 H2O, L = 18.6 A): the device writes
+    each chunk's frames (caller order, velocities = momenta / mass) straight into the
+    returned trajectory; 25 NVE steps (three chunks of at most 10) against the FP64
+    oracle's velocity Verlet, frame 0 the initial state."""
+    import contextlib
+    import io
+    from deepmd_jax import synthetic as syn
+    from deepmd_jax.constants import MASS_UNIT_CONVERSION
+    from deepmd_jax.dpmodel import DPModel
+    from deepmd_jax.md import Simulation
+    print('case simulation trajectory frames', flush=True)
+    params, var = model0
+    coord, box, types = syn.water_box(216, seed=6, permute_sites=True)
+    vel = syn.maxwell_boltzmann(types, 300.0, seed=6)
+    with contextlib.redirect_stdout(io.StringIO()):
+        sim = Simulation(None, box, types, syn.MASSES, 'NVE', 0.5, coord, initial_velocity=vel,
+                         report_interval=10 ** 6, neighbor_skin=1.0, model=(DPModel(params), var, device_model))
+        assert sim._sys.nimg == 1
+        sim._step_chunk_size = 10
+        traj = sim.run(25)
+    assert traj['position'].shape == (26, len(types), 3)
+    np.testing.assert_allclose(traj['position'][0], coord, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(traj['velocity'][0], vel, rtol=0, atol=1e-12)
+    m = np.array(syn.MASSES)[types] * MASS_UNIT_CONVERSION
+    nb, _, _ = O.typed_nlist(coord, box, types, params['rcut'] + 1.0)
+    x, pm = coord.copy(), vel * m[:, None]
+    F = O.energy_force_virial(params, var, x, box, types, nbrs_nm=nb)[1].numpy()
+    for k in range(25):
+        pm += 0.25 * F
+        x += 0.5 * pm / m[:, None]
+        F = O.energy_force_virial(params, var, x, box, types, nbrs_nm=nb)[1].numpy()
+        pm += 0.25 * F
+        if k in (9, 19, 24):
+            assert np.abs(traj['position'][k + 1] - x).max() < 1e-5, k
+            assert np.abs(traj['velocity'][k + 1] - pm / m[:, None]).max() < 1e-5, k
