@@ -388,6 +388,7 @@ static int32_t sys_layout(System* s, int nseg, const int32_t* seg_natoms, const 
   DPMD_TRY(sys_ensure(s, (void**)&s->d_red, sizeof(double) * (32 * (size_t)nseg + 64)));
   DPMD_TRY(sys_ensure(s, (void**)&s->d_pairs, sizeof(int) * NA));
   DPMD_TRY(sys_ensure(s, (void**)&s->d_perm, sizeof(int) * NA));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_npt, sizeof(double) * 128));
   hipStream_t st = s->stream;
   DPMD_HIP(hipMemsetAsync(s->d_pos, 0, sizeof(double) * NA * 3, st));
   DPMD_HIP(hipMemsetAsync(s->d_mom, 0, sizeof(double) * NA * 3, st));
@@ -408,6 +409,7 @@ static int32_t sys_layout(System* s, int nseg, const int32_t* seg_natoms, const 
   s->nl_overflow = false;
   s->force_valid = false;
   s->rng_counter = 0;
+  s->npt_ready = false;
   return DPMD_OK;
 }
 
@@ -819,6 +821,56 @@ int32_t dpmd_compute(dpmd_system_t sys, int32_t precision, double* energy, doubl
   return compute_impl(s, precision, energy, force, virial, atom_energy);
 }
 
+// NPT parameters from dpmd_md_params; couple_axes rules of deepmd_jax/routine.py:196-223
+static int32_t npt_args(System* s, const dpmd_md_params* p, NptArgs& a) {
+  if (s->nseg != 1 || s->seg[0].nimg != 1) {
+    set_error("NPT runs one system held without periodic images");
+    return DPMD_E_UNSUPPORTED;
+  }
+  if (p->chain_length_p < 1 || p->chain_length_p > 8) {
+    set_error("barostat chain length %d outside [1, 8]", p->chain_length_p);
+    return DPMD_E_ARG;
+  }
+  if (!(p->tau_p > 0) || !(p->kT > 0)) {
+    set_error("NPT needs tau_p > 0 and a temperature");
+    return DPMD_E_ARG;
+  }
+  int G = 0;
+  for (int d = 0; d < 3; ++d) {
+    const int g = p->couple[d];
+    if (g < -1 || g > 2) { set_error("couple[%d] = %d outside [-1, 2]", d, g); return DPMD_E_ARG; }
+    a.gid[d] = g;
+    if (g >= 0) {
+      a.nper[g] += 1;
+      G = std::max(G, g + 1);
+    }
+  }
+  if (G == 0) { set_error("couple_axes must contain at least one group; use NVT instead"); return DPMD_E_ARG; }
+  for (int g = 0; g < G; ++g)
+    if (a.nper[g] == 0) { set_error("coupled groups must be numbered 0..G-1 without gaps"); return DPMD_E_ARG; }
+  a.G = G;
+  a.C = p->chain_length_p;
+  a.pressure = p->pressure;
+  a.kT = p->kT;
+  a.dt = p->dt;
+  a.tau_p = p->tau_p;
+  a.rc = s->model->rcut;
+  a.rc_skin = s->rc_skin;
+  a.natoms = s->NA;
+  return DPMD_OK;
+}
+
+// drift threshold with the NPT safety factor on the host copy of the segment
+static void npt_threshold(System* s, const NptArgs& a) {
+  SegInfo& S = s->seg[0];
+  float sc = 0.f;
+  for (int d = 0; d < 3; ++d) sc = std::max(sc, S.scale[d]);
+  const float allowed = ((float)a.rc_skin - (float)a.rc * (float)((double)sc * 1.02)) / 2.f;
+  const double lmin = std::min((double)S.L[0], std::min((double)S.L[1], (double)S.L[2]));
+  S.thr = allowed - (float)(lmin * 1e-6);
+  s->seg_dirty = true;
+}
+
 int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps, double* traj_pos, double* traj_vel,
                     uint32_t* error_code) {
   SYS(sys);
@@ -826,10 +878,14 @@ int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps, 
   if (!s->nl_ready) { set_error("no neighbour list: call dpmd_nlist_allocate first"); return DPMD_E_STATE; }
   if (nsteps < 0) { set_error("nsteps must be >= 0"); return DPMD_E_ARG; }
   if (p->routine != DPMD_NVE && p->routine != DPMD_NVT_LANGEVIN && p->routine != DPMD_NVT_NH &&
-      p->routine != DPMD_PIMD_PILE_L) {
+      p->routine != DPMD_PIMD_PILE_L && p->routine != DPMD_NPT_NH) {
     set_error("unknown routine %d", p->routine);
     return DPMD_E_ARG;
   }
+  const bool npt = p->routine == DPMD_NPT_NH;
+  NptArgs na;
+  memset(&na, 0, sizeof(na));
+  if (npt) DPMD_TRY(npt_args(s, p, na));
   if (p->precision != DPMD_PREC_FP32 && p->precision != DPMD_PREC_BF16 && p->precision != DPMD_PREC_FP16) {
     set_error("unknown precision %d", p->precision);
     return DPMD_E_ARG;
@@ -854,13 +910,28 @@ int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps, 
   std::vector<double> thermo_save((size_t)64 * s->nseg);
   DPMD_HIP(hipMemcpyAsync(thermo_save.data(), s->d_thermo, sizeof(double) * thermo_save.size(),
                           hipMemcpyDeviceToHost, s->stream));
+  if (npt) {
+    // the drift threshold carries the NPT safety factor from the first step on
+    npt_threshold(s, na);
+    DPMD_HIP(hipMemcpyAsync(s->d_npt + 64, s->d_npt, sizeof(double) * 64, hipMemcpyDeviceToDevice, s->stream));
+  }
   DPMD_TRY(sys_upload_segments(s));
   DPMD_TRY(nlist_clear_flags(s));
   const double dt = p->dt;
   const int prec = p->precision;
-  if (!s->force_valid) {
+  const bool fresh_forces = !s->force_valid;
+  if (fresh_forces) {
     DPMD_TRY(md_write_pos32(s));
     DPMD_TRY(model_forces_impl(s, prec));
+  }
+  if (npt && (!s->npt_ready || fresh_forces)) {
+    // init_fn (deepmd_jax/routine.py:344-378), or dU/deps of recomputed forces
+    DPMD_TRY(md_npt_init(s, na, s->npt_ready));
+    s->npt_ready = true;
+  }
+  if (rec) {
+    DPMD_TRY(sys_ensure(s, (void**)&s->d_box_traj, sizeof(double) * 3 * (size_t)s->nseg * std::max(nsteps, 1)));
+    s->box_traj_steps = 0;
   }
   const uint64_t counter0 = s->rng_counter;
   for (int step = 0; step < nsteps; ++step) {
@@ -890,6 +961,17 @@ int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps, 
       DPMD_TRY(model_forces_impl(s, prec));
       DPMD_TRY(md_kick(s, 0.5 * dt));
       DPMD_TRY(md_nh_half(s, p->kT, dt, p->tau, p->chain_length));
+    } else if (npt) {
+      // apply_fn of deepmd_jax/routine.py:396-423 around inner_step :365-394
+      DPMD_TRY(md_npt_baro_half(s, na));
+      DPMD_TRY(md_nh_half(s, p->kT, dt, p->tau, p->chain_length));
+      DPMD_TRY(md_npt_box(s, na, 0));
+      DPMD_TRY(md_npt_particles(s, dt, true));
+      DPMD_TRY(model_forces_impl(s, prec));
+      DPMD_TRY(md_npt_particles(s, dt, false));
+      DPMD_TRY(md_npt_box(s, na, 1));
+      DPMD_TRY(md_nh_half(s, p->kT, dt, p->tau, p->chain_length));
+      DPMD_TRY(md_npt_baro_half(s, na));
     } else {
       const int P = s->nseg;
       const double* C = s->d_nm;
@@ -913,6 +995,7 @@ int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps, 
       DPMD_HIP(hipMemcpyAsync(s->h_traj + (size_t)step * NA * 6, frame, sizeof(double) * NA * 6,
                               hipMemcpyDeviceToHost, s->copy_stream));
       DPMD_HIP(hipEventRecord(s->ev_copied[step], s->copy_stream));
+      DPMD_TRY(md_box_record(s, s->d_box_traj + (size_t)step * 3 * s->nseg));
     }
   }
   if (rec) {
@@ -945,12 +1028,24 @@ int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps, 
     DPMD_HIP(hipMemcpyAsync(s->d_mom, s->d_mom_save, sizeof(double) * NA * 3, hipMemcpyDeviceToDevice, s->stream));
     DPMD_HIP(hipMemcpyAsync(s->d_thermo, thermo_save.data(), sizeof(double) * thermo_save.size(),
                             hipMemcpyHostToDevice, s->stream));
+    if (npt) {
+      // barostat state and the box of the chunk entry (the host copy of the segments)
+      DPMD_HIP(hipMemcpyAsync(s->d_npt, s->d_npt + 64, sizeof(double) * 64, hipMemcpyDeviceToDevice, s->stream));
+      s->seg_dirty = true;
+      DPMD_TRY(sys_upload_segments(s));
+    }
     DPMD_TRY(md_write_pos32(s));
     DPMD_HIP(hipStreamSynchronize(s->stream));
     s->force_valid = false;
     s->nl_overflow = true;
     return DPMD_OK;
   }
+  if (npt) {
+    // the box moved on the device: the host copy follows before any upload
+    DPMD_HIP(hipMemcpyAsync(s->seg.data(), s->d_seg, sizeof(SegInfo) * s->nseg, hipMemcpyDeviceToHost, s->stream));
+    DPMD_HIP(hipStreamSynchronize(s->stream));
+  }
+  if (rec) s->box_traj_steps = nsteps;
   s->rng_counter = counter0 + nsteps;
   for (SegInfo& S : s->seg) S.step += nsteps;
   s->seg_dirty = true;
@@ -996,6 +1091,39 @@ int32_t dpmd_thermostat_energy(dpmd_system_t sys, double* e) {
   DPMD_HIP(hipMemcpyAsync(t.data(), s->d_thermo, sizeof(double) * t.size(), hipMemcpyDeviceToHost, s->stream));
   DPMD_HIP(hipStreamSynchronize(s->stream));
   for (int r = 0; r < s->nseg; ++r) e[r] = t[(size_t)r * 64];
+  if (s->npt_ready) {
+    // NPT: barostat chain, P V and box kinetic energy join the invariant
+    double b[64];
+    DPMD_HIP(hipMemcpy(b, s->d_npt, sizeof(b), hipMemcpyDeviceToHost));
+    e[0] += b[43];
+  }
+  return DPMD_OK;
+}
+
+int32_t dpmd_system_get_boxes(dpmd_system_t sys, double* boxes) {
+  SYS(sys);
+  if (!boxes) { set_error("null argument"); return DPMD_E_ARG; }
+  for (int r = 0; r < s->nseg; ++r)
+    for (int d = 0; d < 3; ++d) boxes[r * 3 + d] = s->seg[r].boxd[d];
+  return DPMD_OK;
+}
+
+int32_t dpmd_md_box_trajectory(dpmd_system_t sys, double* boxes) {
+  SYS(sys);
+  if (!boxes) { set_error("null argument"); return DPMD_E_ARG; }
+  if (s->box_traj_steps == 0) return DPMD_OK;
+  DPMD_HIP(hipMemcpyAsync(boxes, s->d_box_traj, sizeof(double) * 3 * (size_t)s->nseg * s->box_traj_steps,
+                          hipMemcpyDeviceToHost, s->stream));
+  DPMD_HIP(hipStreamSynchronize(s->stream));
+  return DPMD_OK;
+}
+
+int32_t dpmd_npt_state_get(dpmd_system_t sys, double* state) {
+  SYS(sys);
+  if (!state) { set_error("null argument"); return DPMD_E_ARG; }
+  if (!s->npt_ready) { set_error("no NPT state: run an NPT chunk first"); return DPMD_E_STATE; }
+  DPMD_HIP(hipMemcpyAsync(state, s->d_npt, sizeof(double) * 64, hipMemcpyDeviceToHost, s->stream));
+  DPMD_HIP(hipStreamSynchronize(s->stream));
   return DPMD_OK;
 }
 

@@ -318,6 +318,14 @@ __device__ __forceinline__ typename H8<PREC>::T pack8(GET get, int s2) {
 // scheduling barrier keeps the compiler from hoisting the whole table into registers.
 typedef __amdgpu_buffer_rsrc_t rsrc_t;
 
+// MFMA steps by which weight operands run ahead of their use
+#ifndef DPMD_WEIGHT_PREFETCH
+#define DPMD_WEIGHT_PREFETCH 2
+#endif
+#ifndef DPMD_WEIGHT_PREFETCH_H
+#define DPMD_WEIGHT_PREFETCH_H 2
+#endif
+
 // weight read: byte offset = voff (per lane) + soff (wave-uniform)
 __device__ __forceinline__ float bld(rsrc_t rs, int voff, int soff) {
   return __builtin_bit_cast(float, (unsigned)__builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff, 0));
@@ -330,7 +338,7 @@ __device__ __forceinline__ floatx4 tab_load(rsrc_t rs, int voff, int soff) {
 
 template <int OT, int S, int S0, int SN, typename BF>
 __device__ __forceinline__ void mfma_chain(rsrc_t rs, int sbase, int vl4, BF bfun, floatx4 (&acc)[OT]) {
-  constexpr int PF = 2;
+  constexpr int PF = DPMD_WEIGHT_PREFETCH;
   float wb[PF + 1][OT];
 #pragma unroll
   for (int p = 0; p < PF && p < SN; ++p)
@@ -353,7 +361,7 @@ __device__ __forceinline__ void mfma_chain(rsrc_t rs, int sbase, int vl4, BF bfu
 template <int PREC, int OT, int S2, int S0, int SN, typename BF>
 __device__ __forceinline__ void mfma_chain_h(rsrc_t rs, int sbase, int vl16, BF bfun, floatx4 (&acc)[OT]) {
   typedef typename H8<PREC>::T HT;
-  constexpr int PF = 2;
+  constexpr int PF = DPMD_WEIGHT_PREFETCH_H;
   HT wb[PF + 1][OT];
 #pragma unroll
   for (int p = 0; p < PF && p < SN; ++p)

@@ -123,6 +123,15 @@ struct RowBase {
   int r[kMaxT];
 };
 
+// NPT step parameters (one segment): target pressure, thermostat temperature, step,
+// barostat time, cutoffs for the drift threshold, barostat chain length, group count,
+// atoms, per-axis group (-1 fixed) and axes per group.
+struct NptArgs {
+  double pressure, kT, dt, tau_p, rc, rc_skin;
+  int C, G, natoms;
+  int gid[3], nper[3];
+};
+
 struct System {
   Model* model = nullptr;
   hipStream_t stream = nullptr;
@@ -193,6 +202,10 @@ struct System {
                                 // (steps, NA, 3) each, caller order
   int* d_perm = nullptr;        // (NA) sorted index -> caller index
   double* d_nm = nullptr;       // PIMD normal modes
+  double* d_npt = nullptr;      // NPT barostat state (64 doubles) and its chunk-entry copy
+  bool npt_ready = false;       // NPT state initialised for the current layout
+  double* d_box_traj = nullptr; // per-step boxes of the last recorded chunk (steps, nseg, 3)
+  int box_traj_steps = 0;
   uint64_t rng_counter = 0;
   Timer timer;
   std::vector<std::pair<void**, size_t>> caps;  // allocation capacities (grow-only buffers)
@@ -215,6 +228,11 @@ int32_t md_kick(System* s, double dt_half);
 int32_t md_drift(System* s, double dt);
 int32_t md_write_pos32(System* s);
 int32_t md_traj_record(System* s, double* tpos, double* tvel);
+int32_t md_npt_init(System* s, const NptArgs& a, bool refresh_only);
+int32_t md_npt_baro_half(System* s, const NptArgs& a);
+int32_t md_npt_box(System* s, const NptArgs& a, int phase);
+int32_t md_npt_particles(System* s, double dt, bool move);
+int32_t md_box_record(System* s, double* out);
 int32_t md_reduce_launch(System* s);
 int32_t md_ke_launch(System* s, double* d_out);
 int32_t md_langevin_o(System* s, double dt, double gamma, double kT, uint64_t seed, uint64_t step);

@@ -385,6 +385,49 @@ int32_t md_pimd_o(System* s, const double* d_C, const double* d_c1, const double
 // thermo layout per segment (64 doubles): [0] invariant term, [1] initialised,
 // [8..16) xi, [16..24) p_xi, [24..32) Q, [40] momentum scale.  Kinetic energy and degrees
 // of freedom are per periodic image.
+// One Nose-Hoover chain half step (jax_md simulate.nose_hoover_chain with chain_steps = 1,
+// sy_steps = 1): advances the chain (xi, p_xi) of masses Q that couples to kinetic energy KE
+// of dof degrees of freedom at temperature kT over dt/2, and returns the factor that
+// scales the coupled momenta.
+__device__ double nhc_half_step(double* xi, double* px, const double* Q, int C, double KE, double dof, double kT,
+                                double dt) {
+  const double d2 = dt / 2, d4 = d2 / 2, d8 = d4 / 2;
+  const int M = C - 1;
+  double G;
+  if (M > 0) {
+    G = px[M - 1] * px[M - 1] / Q[M - 1] - kT;
+    px[M] += d4 * G;
+    double pnew = px[M];
+    for (int m = M - 1; m > 0; --m) {
+      G = px[m - 1] * px[m - 1] / Q[m - 1] - kT;
+      const double sc = exp(-d8 * pnew / Q[m + 1]);
+      pnew = sc * (sc * px[m] + d4 * G);
+      px[m] = pnew;
+    }
+  }
+  G = 2.0 * KE - dof * kT;
+  {
+    const double sc = (M > 0) ? exp(-d8 * px[1] / Q[1]) : 1.0;
+    px[0] = sc * (sc * px[0] + d4 * G);
+  }
+  const double scale = exp(-d2 * px[0] / Q[0]);
+  KE *= scale * scale;
+  for (int k = 0; k < C; ++k) xi[k] += d2 * px[k] / Q[k];
+  G = 2.0 * KE - dof * kT;
+  for (int m = 0; m < M; ++m) {
+    const double sc = exp(-d8 * px[m + 1] / Q[m + 1]);
+    const double pu = sc * (sc * px[m] + d4 * G);
+    G = pu * pu / Q[m] - kT;
+    px[m] = pu;
+  }
+  if (M > 0) px[M] += d4 * G;
+  return scale;
+}
+
+// Particle thermostat half step, one thread per segment (deepmd_jax/routine.py:147-158).
+// thermo layout per segment (64 doubles): [0] invariant term, [1] initialised,
+// [8..16) xi, [16..24) p_xi, [24..32) Q, [40] momentum scale.  Kinetic energy and degrees
+// of freedom are per periodic image.
 __global__ void nh_half_kernel(double* thermo, const double* ke, const SegInfo* seg, int nseg, int C, double kT,
                                double dt, double tau) {
   for (int rep = threadIdx.x; rep < nseg; rep += blockDim.x) {
@@ -399,38 +442,7 @@ __global__ void nh_half_kernel(double* thermo, const double* ke, const SegInfo* 
     }
     for (int k = 0; k < C; ++k) Q[k] = kT * tau * tau;
     Q[0] *= dof;
-    double KE = ke[rep];
-    const double d2 = dt / 2, d4 = d2 / 2, d8 = d4 / 2;
-    const int M = C - 1;
-    double G;
-    if (M > 0) {
-      G = px[M - 1] * px[M - 1] / Q[M - 1] - kT;
-      px[M] += d4 * G;
-      double pnew = px[M];
-      for (int m = M - 1; m > 0; --m) {
-        G = px[m - 1] * px[m - 1] / Q[m - 1] - kT;
-        const double sc = exp(-d8 * pnew / Q[m + 1]);
-        pnew = sc * (sc * px[m] + d4 * G);
-        px[m] = pnew;
-      }
-    }
-    G = 2.0 * KE - dof * kT;
-    {
-      const double sc = (M > 0) ? exp(-d8 * px[1] / Q[1]) : 1.0;
-      px[0] = sc * (sc * px[0] + d4 * G);
-    }
-    const double scale = exp(-d2 * px[0] / Q[0]);
-    KE *= scale * scale;
-    for (int k = 0; k < C; ++k) xi[k] += d2 * px[k] / Q[k];
-    G = 2.0 * KE - dof * kT;
-    for (int m = 0; m < M; ++m) {
-      const double sc = exp(-d8 * px[m + 1] / Q[m + 1]);
-      const double pu = sc * (sc * px[m] + d4 * G);
-      G = pu * pu / Q[m] - kT;
-      px[m] = pu;
-    }
-    if (M > 0) px[M] += d4 * G;
-    th[40] = scale;
+    th[40] = nhc_half_step(xi, px, Q, C, ke[rep], dof, kT, dt);
     double e = px[0] * px[0] / (2.0 * Q[0]) + dof * kT * xi[0];
     for (int k = 1; k < C; ++k) e += px[k] * px[k] / (2.0 * Q[k]) + kT * xi[k];
     th[0] = e;
@@ -454,6 +466,207 @@ int32_t md_nh_half(System* s, double kT, double dt, double tau, int chain) {
   const int n3 = s->NA * 3;
   hipLaunchKernelGGL(nh_scale_kernel, dim3((n3 + 255) / 256), dim3(256), 0, s->stream, s->d_mom, s->d_thermo,
                      s->d_atom_seg, n3);
+  DPMD_HIP(hipGetLastError());
+  return DPMD_OK;
+}
+
+// ------------------------------------------------------------------------------
+// NPT: MTK Nose-Hoover barostat over coupled axis groups (deepmd_jax/routine.py:161-423,
+// adapted from jax_md simulate.npt_nose_hoover), one segment.  Axis a belongs to group
+// gid[a] (or is held fixed, gid = -1); a group's axes share one log-scale R_b.  Device
+// state (64 doubles): [0..3) R_b, [3..6) P_b, [6..9) M_b, [9..12) dU/deps per group,
+// [12..15) reference box, [15] initialised, [16..24) barostat chain xi, [24..32) p_xi,
+// [32..40) Q, [40..43) per-axis 2 KE, [43] barostat terms of the invariant,
+// [45..48) momentum rates, [48..51) position exponents.
+
+__device__ __forceinline__ double sinhx_x(double x) {
+  const double x2 = x * x;
+  return 1 + x2 / 6 + x2 * x2 / 120 + x2 * x2 * x2 / 5040 + x2 * x2 * x2 * x2 / 362880 +
+         x2 * x2 * x2 * x2 * x2 / 39916800;
+}
+
+// per-axis sum of p^2 / m over the segment's atoms -> npt[40..43)
+__global__ void npt_axis_ke_kernel(const double* mom, const double* mass, int natoms, double* npt) {
+  __shared__ double red[256];
+  double k[3] = {0.0, 0.0, 0.0};
+  for (int i = threadIdx.x; i < natoms; i += blockDim.x) {
+    const double im = 1.0 / mass[i];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) k[d] += mom[(size_t)i * 3 + d] * mom[(size_t)i * 3 + d] * im;
+  }
+  for (int d = 0; d < 3; ++d) {
+    const double v = block_sum(k[d], red);
+    if (threadIdx.x == 0) npt[40 + d] = v;
+  }
+}
+
+__device__ __forceinline__ void npt_set_box(const NptArgs& a, double* npt, SegInfo& S) {
+  double sc = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    const int g = a.gid[d];
+    const double L = g >= 0 ? npt[12 + d] * exp(npt[g]) : npt[12 + d];
+    S.boxd[d] = L;
+    S.box[d] = (float)L;
+    S.scale[d] = S.L[d] / (float)L;
+    sc = fmax(sc, (double)S.scale[d]);
+  }
+  // drift threshold with the NPT safety factor (deepmd_jax/md.py:724-728)
+  const float allowed = ((float)a.rc_skin - (float)a.rc * (float)(sc * 1.02)) / 2.f;
+  const double lmin = fmin((double)S.L[0], fmin((double)S.L[1], (double)S.L[2]));
+  S.thr = allowed - (float)(lmin * 1e-6);
+}
+
+// dU/deps of each group from the virial diagonal (virial = -dU/dstrain): red[1 + 4 a] = W_aa
+__device__ __forceinline__ void npt_dudv(const NptArgs& a, const double* red, double* npt) {
+  for (int g = 0; g < 3; ++g) npt[9 + g] = 0.0;
+  for (int d = 0; d < 3; ++d)
+    if (a.gid[d] >= 0) npt[9 + a.gid[d]] -= red[1 + 4 * d];
+}
+
+// set-up (init_fn): zero box position / momentum, reference box, chain at rest, dU/deps
+// of the current forces.  refresh only: dU/deps of freshly computed forces.
+__global__ void npt_init_kernel(NptArgs a, double* npt, SegInfo* seg, const double* red, int refresh_only) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (!refresh_only || npt[15] == 0.0) {
+    for (int k = 0; k < 64; ++k) npt[k] = 0.0;
+    for (int d = 0; d < 3; ++d) npt[12 + d] = seg[0].boxd[d];
+    npt[15] = 1.0;
+    npt_set_box(a, npt, seg[0]);
+  }
+  npt_dudv(a, red, npt);
+}
+
+// Barostat chain half step on the box momenta, with the masses of update_mass /
+// update_box_mass: Q = kT tau_p^2 (G, 1, 1, ...), M_b = n_g (N + 1) kT tau_p^2.  Also
+// forms the barostat part of the invariant: chain terms + P V + KE_box.
+__global__ void npt_baro_half_kernel(NptArgs a, double* npt, const SegInfo* seg) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double tp2 = a.kT * a.tau_p * a.tau_p;
+  double* xi = npt + 16;
+  double* px = npt + 24;
+  double* Q = npt + 32;
+  for (int k = 0; k < a.C; ++k) Q[k] = tp2;
+  Q[0] *= a.G;
+  double ke = 0.0;
+  for (int g = 0; g < a.G; ++g) {
+    npt[6 + g] = a.nper[g] * (a.natoms + 1.0) * tp2;
+    ke += npt[3 + g] * npt[3 + g] / (2.0 * npt[6 + g]);
+  }
+  const double sc = nhc_half_step(xi, px, Q, a.C, ke, (double)a.G, a.kT, a.dt);
+  double e = 0.0;
+  for (int g = 0; g < a.G; ++g) {
+    npt[3 + g] *= sc;
+    e += npt[3 + g] * npt[3 + g] / (2.0 * npt[6 + g]);
+  }
+  for (int k = 0; k < a.C; ++k) e += px[k] * px[k] / (2.0 * Q[k]) + a.kT * xi[k];
+  const SegInfo& S = seg[0];
+  e += a.pressure * S.boxd[0] * S.boxd[1] * S.boxd[2];
+  npt[43] = e;
+}
+
+// Box force G_g = 2KE_g + (n_g / N_f) 2KE - dU/deps_g - P V n_g and the half kick of P_b.
+// phase 0 (first kick) then sets the per-axis rates of exp(iL2) and exponents of exp(iL1)
+// and moves the box: R_b += dt P_b / M_b.  phase 1 takes dU/deps from the new virial first.
+__global__ void npt_box_kernel(NptArgs a, double* npt, SegInfo* seg, const double* red, int phase) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  SegInfo& S = seg[0];
+  if (phase == 1) npt_dudv(a, red, npt);
+  const double Nf = 3.0 * a.natoms;
+  const double V = S.boxd[0] * S.boxd[1] * S.boxd[2];
+  const double k2tot = npt[40] + npt[41] + npt[42];
+  double k2g[3] = {0.0, 0.0, 0.0};
+  for (int d = 0; d < 3; ++d)
+    if (a.gid[d] >= 0) k2g[a.gid[d]] += npt[40 + d];
+  for (int g = 0; g < a.G; ++g) {
+    const double Ge = k2g[g] + (a.nper[g] / Nf) * k2tot - npt[9 + g] - a.pressure * V * a.nper[g];
+    npt[3 + g] += 0.5 * a.dt * Ge;
+  }
+  if (phase == 0) {
+    double vb[3], tr = 0.0;
+    for (int g = 0; g < a.G; ++g) {
+      vb[g] = npt[3 + g] / npt[6 + g];
+      tr += a.nper[g] * vb[g];
+    }
+    for (int d = 0; d < 3; ++d) {
+      const double v = a.gid[d] >= 0 ? vb[a.gid[d]] : 0.0;
+      npt[45 + d] = v + tr / Nf;
+      npt[48 + d] = v * a.dt;
+    }
+    for (int g = 0; g < a.G; ++g) npt[g] += vb[g] * a.dt;
+    npt_set_box(a, npt, S);
+  }
+}
+
+// exp(iL2): p = p e^{-x} + dt/2 F sinhx_x(x/2) e^{-x/2}, x = rate dt/2; with move, then
+// exp(iL1): r = r e^{y} + dt (p/m) e^{y/2} sinhx_x(y/2), y = exponent, and the fp32
+// mirror in the new box (NaN/Inf -> error bit 16).
+__global__ void npt_particle_kernel(double* pos, double* mom, const double* mass, const float* force, float4* p32,
+                                    const double* npt, const SegInfo* seg, int natoms, double dt, int move,
+                                    int* flags) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= natoms) return;
+  const SegInfo& S = seg[0];
+  const double m = mass[i];
+  float v[3];
+  bool bad = false;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double x = npt[45 + d] * dt * 0.5;
+    double p = mom[(size_t)i * 3 + d];
+    p = p * exp(-x) + 0.5 * dt * (double)force[(size_t)i * 3 + d] * sinhx_x(0.5 * x) * exp(-0.5 * x);
+    mom[(size_t)i * 3 + d] = p;
+    if (move) {
+      const double y = npt[48 + d];
+      const double r = pos[(size_t)i * 3 + d] * exp(y) + dt * (p / m) * exp(0.5 * y) * sinhx_x(0.5 * y);
+      pos[(size_t)i * 3 + d] = r;
+      bad |= !isfinite(r) || !isfinite(p);
+      v[d] = wrap32(r, S.boxd[d]);
+    }
+  }
+  if (move) {
+    p32[i] = make_float4(v[0], v[1], v[2], 0.f);
+    if (bad) atomicOr(&flags[2], 16);
+  }
+}
+
+__global__ void box_record_kernel(const SegInfo* seg, int nseg, double* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 3 * nseg) out[i] = seg[i / 3].boxd[i % 3];
+}
+
+int32_t md_npt_init(System* s, const NptArgs& a, bool refresh_only) {
+  DPMD_TRY(md_reduce_launch(s));
+  hipLaunchKernelGGL(npt_init_kernel, dim3(1), dim3(64), 0, s->stream, a, s->d_npt, s->d_seg, s->d_red,
+                     refresh_only ? 1 : 0);
+  DPMD_HIP(hipGetLastError());
+  return DPMD_OK;
+}
+
+int32_t md_npt_baro_half(System* s, const NptArgs& a) {
+  hipLaunchKernelGGL(npt_baro_half_kernel, dim3(1), dim3(64), 0, s->stream, a, s->d_npt, s->d_seg);
+  DPMD_HIP(hipGetLastError());
+  return DPMD_OK;
+}
+
+int32_t md_npt_box(System* s, const NptArgs& a, int phase) {
+  hipLaunchKernelGGL(npt_axis_ke_kernel, dim3(1), dim3(256), 0, s->stream, s->d_mom, s->d_mass, s->NA, s->d_npt);
+  if (phase == 1) DPMD_TRY(md_reduce_launch(s));
+  hipLaunchKernelGGL(npt_box_kernel, dim3(1), dim3(64), 0, s->stream, a, s->d_npt, s->d_seg, s->d_red, phase);
+  DPMD_HIP(hipGetLastError());
+  return DPMD_OK;
+}
+
+int32_t md_npt_particles(System* s, double dt, bool move) {
+  const int NA = s->NA;
+  hipLaunchKernelGGL(npt_particle_kernel, dim3((NA + 255) / 256), dim3(256), 0, s->stream, s->d_pos, s->d_mom,
+                     s->d_mass, s->d_force, s->d_pos32, s->d_npt, s->d_seg, NA, dt, move ? 1 : 0, s->d_flags);
+  DPMD_HIP(hipGetLastError());
+  return DPMD_OK;
+}
+
+int32_t md_box_record(System* s, double* out) {
+  hipLaunchKernelGGL(box_record_kernel, dim3((3 * s->nseg + 63) / 64), dim3(64), 0, s->stream, s->d_seg, s->nseg,
+                     out);
   DPMD_HIP(hipGetLastError());
   return DPMD_OK;
 }

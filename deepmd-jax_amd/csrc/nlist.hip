@@ -3,14 +3,15 @@
 // rebuild.
 //
 // Semantics restated from deepmd-jax (and the jax_md neighbour list it wraps):
-//   canonicalize   deepmd_jax/md_utils.py:364-373   c = ((f32(x) mod L) * scale) * f32(1-5e-7)
+//   canonicalize   deepmd_jax/md_utils.py:364-373   c = ((f32(x) mod box) * scale) * f32(1-5e-7),
+//                  scale = L / box (L: reference box of the list, box: current box)
 //   pair test      jax_md space.periodic + square_distance (call site md_utils.py:384-415):
 //                  d = (ca - cb); d = ((d + L/2) mod L) - L/2; d2 = (dx^2 + dy^2) + dz^2 < cut^2,
 //                  self excluded
 //   rows           md_utils.py:431-459: per neighbour type, ascending type-local indices,
 //                  padded with N, knbr_j - 1 columns; overflow if a row needs knbr_j
-//   drift test     deepmd_jax/md.py:693-741: max |minimg((f32(x) mod L)*scale - c_ref)|
-//                  > ((rc+skin) - rc*scale)/2 - 1e-6 L_min
+//   drift test     deepmd_jax/md.py:693-741: max |minimg_L((f32(x) mod box)*scale - c_ref)|
+//                  > ((rc+skin) - rc*max(scale)*(1.02 under NPT))/2 - 1e-6 L_min
 // All float32 expressions are evaluated without contraction so the neighbour sets are
 // bit-exact with a float32 evaluation of the same formulas.
 #include "internal.h"
@@ -82,7 +83,7 @@ __global__ void nl_canon_kernel(NlArgs a) {
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     float x = (float)a.pos[(size_t)g * 3 + d];
-    x = fmod_floor(x, S.L[d]);
+    x = fmod_floor(x, S.box[d]);  // wrap in the current box, then scale to the reference box
     x = x * S.scale[d];
     c[d] = x * shrink;
   }
@@ -239,7 +240,7 @@ __global__ void nl_drift_kernel(const double* pos, const float4* canon, const Se
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       float x = (float)pos[(size_t)g * 3 + k];
-      x = fmod_floor(x, S.L[k]) * S.scale[k];
+      x = fmod_floor(x, S.box[k]) * S.scale[k];
       float dd = x - rv[k];
       const float h = S.L[k] / 2.f;
       dd = fmod_floor(dd - h, S.L[k]) - h;

@@ -10,7 +10,7 @@ import numpy as np
 
 MAXT = 4
 PREC = {'fp32': 0, 'bf16': 1, 'fp16': 2}
-ROUTINE = {'NVE': 0, 'NVT': 1, 'NVT_langevin': 2, 'PIMD': 3}
+ROUTINE = {'NVE': 0, 'NVT': 1, 'NVT_langevin': 2, 'PIMD': 3, 'NPT': 4}
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, 'libdpmd_hip.so')
@@ -31,7 +31,8 @@ class ModelDesc(ctypes.Structure):
 class MDParams(ctypes.Structure):
     _fields_ = [('routine', c_i32), ('precision', c_i32), ('dt', c_f64), ('kT', c_f64), ('gamma', c_f64),
                 ('tau', c_f64), ('chain_length', c_i32), ('record_every', c_i32), ('seed', ctypes.c_uint64),
-                ('pimd_lambda', c_f64)]
+                ('pimd_lambda', c_f64), ('pressure', c_f64), ('tau_p', c_f64), ('chain_length_p', c_i32),
+                ('couple', c_i32 * 3)]
 
 
 class DPMDError(RuntimeError):
@@ -84,6 +85,9 @@ def lib():
         'dpmd_timing_get': (c_i32, [c_vp, c_i32, c_vp, c_vp]),
         'dpmd_pair_count': (c_i32, [c_vp, c_vp]),
         'dpmd_synchronize': (c_i32, [c_vp]),
+        'dpmd_system_get_boxes': (c_i32, [c_vp, c_vp]),
+        'dpmd_md_box_trajectory': (c_i32, [c_vp, c_vp]),
+        'dpmd_npt_state_get': (c_i32, [c_vp, c_vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -118,7 +122,8 @@ def exported_symbols():
             'dpmd_nlist_allocate', 'dpmd_nlist_update', 'dpmd_nlist_export', 'dpmd_nlist_capacity', 'dpmd_compute',
             'dpmd_md_run', 'dpmd_pimd_setup', 'dpmd_kinetic_energy', 'dpmd_thermostat_energy',
             'dpmd_thermostat_state_get', 'dpmd_thermostat_state_set', 'dpmd_timing_enable',
-            'dpmd_timing_get', 'dpmd_pair_count', 'dpmd_synchronize']
+            'dpmd_timing_get', 'dpmd_pair_count', 'dpmd_synchronize', 'dpmd_system_get_boxes',
+            'dpmd_md_box_trajectory', 'dpmd_npt_state_get']
 
 
 class DeviceModel:
@@ -360,6 +365,25 @@ class DeviceSystem:
         err = np.zeros(1, np.uint32)
         check(lib().dpmd_md_run(self.handle, ctypes.byref(params), int(nsteps), ptr(tp), ptr(tv), ptr(err)))
         return tp, tv, int(err[0])
+
+    def get_boxes(self):
+        """Current box lengths (nrep, 3); they move under NPT."""
+        b = np.zeros((self.nrep, 3))
+        check(lib().dpmd_system_get_boxes(self.handle, ptr(b)))
+        return b
+
+    def box_trajectory(self, nsteps):
+        """Per-step boxes (nsteps, nrep, 3) of the last recorded md_run chunk."""
+        b = np.zeros((nsteps, self.nrep, 3))
+        check(lib().dpmd_md_box_trajectory(self.handle, ptr(b)))
+        return b
+
+    def npt_state(self):
+        """The 64-double NPT barostat state (log box scales, box momenta and masses, dU/deps,
+        reference box, barostat chain; layout in md.hip)."""
+        st = np.zeros(64)
+        check(lib().dpmd_npt_state_get(self.handle, ptr(st)))
+        return st
 
     def pimd_setup(self, C, freqs, c1, c2):
         a = [np.ascontiguousarray(np.asarray(x, np.float64)) for x in (C, freqs, c1, c2)]

@@ -8,8 +8,10 @@ with overflow recovery and chunk re-run (deepmd_jax/md.py:644-676,906-916), the
 trajectory copy and the periodic report (deepmd_jax/md.py:473-595).
 
 Routines: 'NVE' (velocity Verlet), 'NVT' (Nose-Hoover chain; with n_bead > 1
-ring-polymer PILE-L), 'NVT_langevin' (BAOAB).  Positions and momenta are FP64 on
-the device.  Not provided: NPT, fixed atoms, model deviation, XYZ dumps, DPLR.
+ring-polymer PILE-L), 'NVT_langevin' (BAOAB), 'NPT' (MTK Nose-Hoover barostat over
+couple_axes groups with a Nose-Hoover chain thermostat, deepmd_jax/routine.py:161-456,
+one system in the neighbour-list regime).  Positions and momenta are FP64 on the device.
+Not provided: path-integral NPT, fixed atoms, model deviation, XYZ dumps, DPLR.
 """
 import time
 
@@ -29,6 +31,30 @@ def print_run_profile(steps, elapsed_time, natoms, dt):
     print('# Finished %d steps in %dh %dm %ds.' %
           (steps, elapsed_time // 3600, (elapsed_time % 3600) // 60, elapsed_time % 60))
     print('# Performance: %.3f ns/day, %.3f step/μs/atom' % (nanosecond_per_day, steps_per_microsecond_per_atom))
+
+
+def parse_couple_axes(couple_axes):
+    """Validate couple_axes (deepmd_jax/routine.py:196-223): a tuple of tuples partitioning
+    a subset of {0,1,2} into coupled groups; unlisted axes are fixed.  Returns the groups
+    and the per-axis group id (-1 fixed)."""
+    try:
+        groups = tuple(tuple(int(a) for a in g) for g in couple_axes)
+    except TypeError:
+        raise ValueError('couple_axes must be a tuple of tuples of ints; got %s' % (couple_axes,))
+    axes = [a for g in groups for a in g]
+    if any(a not in (0, 1, 2) for a in axes):
+        raise ValueError('couple_axes axes must be in {0,1,2}; got %s' % (couple_axes,))
+    if len(set(axes)) != len(axes):
+        raise ValueError('couple_axes must be a partition (no repeats); got %s' % (couple_axes,))
+    if any(len(g) == 0 for g in groups):
+        raise ValueError('couple_axes groups must be non-empty; got %s' % (couple_axes,))
+    if len(groups) == 0:
+        raise ValueError('couple_axes must contain at least one group; use NVT instead')
+    gid = np.full(3, -1, np.int32)
+    for g, grp in enumerate(groups):
+        for a in grp:
+            gid[a] = g
+    return groups, gid
 
 
 class _ImageSystem:
@@ -127,9 +153,7 @@ class Simulation:
         self._pimd_lambda = float(pimd_lambda)
         if self._pimd_lambda < 0:
             raise ValueError('pimd_lambda must be non-negative.')
-        if 'NPT' in routine:
-            raise NotImplementedError('NPT is not provided by this engine.')
-        if routine not in ('NVE', 'NVT', 'NVT_langevin'):
+        if routine not in ('NVE', 'NVT', 'NVT_langevin', 'NPT'):
             raise NotImplementedError("routine currently limited to 'NVE', 'NVT', 'NVT_langevin', 'NPT'")
         if fixed_indices is not None:
             raise NotImplementedError('fixed_indices is not provided by this engine.')
@@ -137,10 +161,20 @@ class Simulation:
             raise NotImplementedError('model_deviation_paths is not provided by this engine.')
         if remove_com_motion:
             raise NotImplementedError('remove_com_motion is not provided by this engine.')
-        if not (isinstance(couple_axes, tuple) and couple_axes == ((0, 1, 2),)):
+        if not (isinstance(couple_axes, tuple) and couple_axes == ((0, 1, 2),)) and routine != 'NPT':
             raise ValueError("couple_axes is only meaningful for routine='NPT'")
-        if chain_steps_t != 1 or sy_steps_t != 1:
+        if chain_steps_t != 1 or sy_steps_t != 1 or chain_steps_p != 1 or sy_steps_p != 1:
             raise NotImplementedError('Nose-Hoover chain_steps / sy_steps other than 1 are not provided.')
+        self._couple_axes = ((0, 1, 2),)
+        if routine == 'NPT':
+            if pressure is None:
+                raise ValueError("Missing argument 'pressure' (in bar) for routine 'NPT'")
+            if temperature is None:
+                raise ValueError("Missing argument 'temperature' for routine 'NPT'")
+            if self._is_pimd:
+                raise NotImplementedError('path-integral NPT is not provided by this engine.')
+            self._couple_axes, self._group_ids = parse_couple_axes(couple_axes)
+            self._pressure = float(pressure)
         pos = np.asarray(initial_position, np.float64)
         if self._is_pimd:
             if routine != 'NVT':
@@ -190,7 +224,7 @@ class Simulation:
         self._type_symbols = type_symbols
         self._debug = debug
         self._step_chunk_size = max(10, min(100, 100000 // (self._natoms * self._n_bead)))
-        self._neighbor_skin = 0.3 if neighbor_skin is None else float(neighbor_skin)
+        self._neighbor_skin = (0.5 if routine == 'NPT' else 0.3) if neighbor_skin is None else float(neighbor_skin)
         self._neighbor_buffer_ratio = float(neighbor_buffer_ratio)
         self._box = canonical_box(box)
         self._rcut = float(params['rcut'])
@@ -199,6 +233,12 @@ class Simulation:
         # 2*(rcut+skin) <= L, so the skin is shrunk to fit; a box below 2*rcut is held as a
         # k1 x k2 x k3 periodic supercell whose images move together (the lattice-image sum).
         k = np.ones(3, int)
+        if routine == 'NPT' and (self._box.min() <= 2 * (self._rcut + self._neighbor_skin) * 1.02
+                                 or not use_neighbor_list_when_possible):
+            # the reference's NPT condition for its neighbour list (deepmd_jax/md.py:450-456)
+            raise NotImplementedError('NPT needs box lengths above 2*(rcut+skin)*1.02 = %.3f A (neighbour-list '
+                                      'regime); smaller NPT boxes are not provided by this engine.'
+                                      % (2 * (self._rcut + self._neighbor_skin) * 1.02))
         if self._box.min() < 2 * self._rcut:
             if self._is_pimd:
                 raise NotImplementedError('path-integral MD needs box lengths >= 2*rcut (%.3f)' % (2 * self._rcut))
@@ -219,6 +259,13 @@ class Simulation:
         self._params.chain_length = int(chain_length_t)
         self._params.gamma = 1.0 / float(tau_t)
         self._params.pimd_lambda = self._pimd_lambda
+        if routine == 'NPT':
+            # deepmd_jax/md.py:357-366: pressure in eV/A^3, barostat chain of tau_p
+            self._params.pressure = self._pressure * PRESS_UNIT_CONVERSION
+            self._params.tau_p = float(tau_p)
+            self._params.chain_length_p = int(chain_length_p)
+            for a in range(3):
+                self._params.couple[a] = int(self._group_ids[a])
         if self._is_pimd:
             self._params.routine = _lib.ROUTINE['PIMD']
             freqs, C = normal_mode_transform(self._n_bead, self._kT)
@@ -242,10 +289,13 @@ class Simulation:
         labels = {('NVE', False): 'NVE run (velocity-Verlet integrator)',
                   ('NVT', False): 'NVT run (Nose-Hoover thermostat)',
                   ('NVT_langevin', False): 'NVT Langevin run (BAOAB Langevin thermostat)',
-                  ('NVT', True): 'NVT PIMD run (PILE-L Langevin thermostat)'}
+                  ('NVT', True): 'NVT PIMD run (PILE-L Langevin thermostat)',
+                  ('NPT', False): 'NPT run (Nose-Hoover thermostat/barostat)'}
         info = []
         if temperature is not None and routine != 'NVE':
             info.append('temperature=%g K' % temperature)
+        if routine == 'NPT':
+            info.append('pressure=%g bar' % self._pressure)
         if self._is_pimd:
             info += ['n_beads=%d' % self._n_bead, 'pimd_lambda=%g' % self._pimd_lambda]
         print('# Initialized %s with %d atoms%s' % (labels[(routine, self._is_pimd)], self._natoms,
@@ -275,6 +325,13 @@ class Simulation:
         elif self._routine == 'NVT_langevin':
             rep['Invariant'] = ke_tot + pe - float(self._sys.thermostat_energy()[0])
         elif self._routine == 'NVT' and not self._is_pimd:
+            rep['Invariant'] = ke_tot + pe + float(self._sys.thermostat_energy()[0])
+        elif self._routine == 'NPT':
+            # deepmd_jax/md.py:523-538: pressure, moving box lengths, invariant
+            box = self.getBox()
+            rep['Pressure'] = (N * self._kT + float(np.trace(v[0])) / 3.0) / float(np.prod(box)) / PRESS_UNIT_CONVERSION
+            for a in sorted(a for g in self._couple_axes for a in g):
+                rep['box_' + 'xyz'[a]] = float(box[a])
             rep['Invariant'] = ke_tot + pe + float(self._sys.thermostat_energy()[0])
         self._last_force = f
         self._last_virial = v
@@ -348,7 +405,10 @@ class Simulation:
             if self._is_pimd:
                 for k in range(nxt):
                     self._centroid_trajectory[lo + k] = centroid(tp[k], self._box)
-            self._box_trajectory[lo:lo + nxt] = self._box
+            if self._routine == 'NPT':
+                self._box_trajectory[lo:lo + nxt] = self._sys.dev.box_trajectory(nxt)[:, 0]
+            else:
+                self._box_trajectory[lo:lo + nxt] = self._box
             self.step += nxt
             remaining -= nxt
             if self.step % self.report_interval == 0 or remaining == 0:
@@ -403,6 +463,8 @@ class Simulation:
         self._sys.set_velocities(v)
 
     def getBox(self):
+        if self._routine == 'NPT':
+            return self._sys.dev.get_boxes()[0]
         return self._box.copy()
 
     def getEnergy(self):
@@ -423,5 +485,5 @@ class Simulation:
             raise ValueError('getPressure needs a temperature')
         _, _, v, _ = self._sys.compute(self._precision)
         dUdeps = -float(np.trace(v.mean(0)))
-        V = float(np.prod(self._box))
+        V = float(np.prod(self.getBox()))
         return (self._natoms * self._kT - dUdeps / 3.0) / V / PRESS_UNIT_CONVERSION

@@ -43,6 +43,7 @@ extern "C" {
 #define DPMD_NVT_NH 1       /* Nose-Hoover chain, deepmd_jax/routine.py:147-158 */
 #define DPMD_NVT_LANGEVIN 2 /* BAOAB, deepmd_jax/routine.py:46-107 */
 #define DPMD_PIMD_PILE_L 3  /* ring-polymer BAOAB + PILE-L, deepmd_jax/pimd.py:114-207 */
+#define DPMD_NPT_NH 4       /* MTK Nose-Hoover barostat + thermostat, deepmd_jax/routine.py:161-423 */
 
 typedef struct dpmd_model* dpmd_model_t;
 typedef struct dpmd_system* dpmd_system_t;
@@ -91,6 +92,12 @@ typedef struct {
   int32_t record_every; /* >0: store positions/velocities every step into traj buffers */
   uint64_t seed;        /* stochastic integrators: counter-based normal stream */
   double pimd_lambda;   /* PILE-L internal-mode damping scale */
+  /* DPMD_NPT_NH only (deepmd_jax/md.py:357-366, deepmd_jax/routine.py:161-423): */
+  double pressure;      /* target pressure (eV / Angstrom^3) */
+  double tau_p;         /* barostat relaxation time (fs) */
+  int32_t chain_length_p; /* barostat chain length */
+  int32_t couple[3];    /* coupled group of each axis (0, 1, 2; groups numbered from 0 in
+                           first-appearance order) or -1 for a fixed axis (couple_axes) */
 } dpmd_md_params;
 
 const char* dpmd_last_error(void);
@@ -189,7 +196,9 @@ int32_t dpmd_pimd_setup(dpmd_system_t sys, const double* C, const double* freqs,
 int32_t dpmd_kinetic_energy(dpmd_system_t sys, double* ke);
 /* Thermostat work / chain state scalar used by the reference invariants (nseg):
  * thermostat_work of NVTLangevinStateWithWork (deepmd_jax/routine.py:27-107), the chain
- * energy of nvt_nose_hoover_invariant (deepmd_jax/routine.py:9-25). */
+ * energy of nvt_nose_hoover_invariant (deepmd_jax/routine.py:9-25); after NPT steps also
+ * the barostat chain, P V and box kinetic energy of npt_nose_hoover_invariant
+ * (deepmd_jax/routine.py:426-456), so that PE + KE + this is conserved. */
 int32_t dpmd_thermostat_energy(dpmd_system_t sys, double* e);
 /* Whole thermostat state, 64 doubles per segment, to carry a segment into another system
  * (no reference counterpart: the reference keeps it in its scan carry). */
@@ -204,6 +213,18 @@ int32_t dpmd_timing_get(dpmd_system_t sys, int32_t which, double* ms, int64_t* c
 /* Mean neighbours within rcut per centre of the last evaluation (algorithmic FLOP count). */
 int32_t dpmd_pair_count(dpmd_system_t sys, double* pairs_within_rcut);
 int32_t dpmd_synchronize(dpmd_system_t sys);
+
+/* Current box lengths (nseg, 3): state.box of the reference NPT state
+ * (deepmd_jax/routine.py:189-194), equal to the set box outside NPT. */
+int32_t dpmd_system_get_boxes(dpmd_system_t sys, double* boxes);
+/* Per-step boxes (steps, nseg, 3) of the last dpmd_md_run that recorded a trajectory: the
+ * box trajectory of Simulation.run (deepmd_jax/md.py:846,924). */
+int32_t dpmd_md_box_trajectory(dpmd_system_t sys, double* boxes);
+/* NPT barostat state (64 doubles): [0..3) log box scale per group, [3..6) box momenta,
+ * [6..9) box masses, [9..12) dU/deps per group, [12..15) reference box, [16..24) chain
+ * positions, [24..32) chain momenta, [32..40) chain masses, [43] barostat invariant terms
+ * (CachedNPTNoseHooverState, deepmd_jax/routine.py:161-194). */
+int32_t dpmd_npt_state_get(dpmd_system_t sys, double* state);
 
 #ifdef __cplusplus
 }

@@ -23,6 +23,9 @@ Function -> reference file:line it follows:
   dp_energy             deepmd_jax/dpmodel.py:30-81,100
   energy_force_virial   deepmd_jax/dpmodel.py:106-108; virial as the strain
                         derivative of deepmd_jax/md_utils.py:139-159
+  NPTOracle             deepmd_jax/routine.py:161-456 (MTK barostat, coupled axes);
+  nhc_half_step         jax_md simulate.nose_hoover_chain as wrapped there (parity
+                        unpinned: jax_md is absent; pinned by invariant conservation)
 Pinning (tests/test_oracle.py): closed forms of the switch and of tensor_3to6,
 rotation / translation / permutation invariance, the T3=T6=0 outer-product
 reduction of G, all-zero-weight nets, autograd against central finite
@@ -430,3 +433,141 @@ def local_fd_forces(params, variables, coord, box, type_idx, atoms, h=1e-4):
                 es.append(e[sel].sum().item())
             out[n, dim] = -(es[0] - es[1]) / (2 * h)
     return out
+
+
+# ---------------------------------------------------------------------------------
+# NPT (deepmd_jax/routine.py:161-456, adapted there from jax_md simulate.npt_nose_hoover):
+# MTK barostat over coupled axis groups plus Nose-Hoover chains on the particles and on
+# the box momenta.  The chain half step restates jax_md's simulate.nose_hoover_chain
+# (chain_steps = 1, sy_steps = 1); jax_md is not in the reference tree, so that part is
+# pinned only through conservation of the invariant (tests/test_oracle.py).
+
+def nhc_half_step(xi, px, Q, KE, dof, kT, dt):
+    """Advance chain (xi, px) (arrays, in place) over dt/2; return the momentum scale."""
+    C = len(xi)
+    d2, d4, d8 = dt / 2, dt / 4, dt / 8
+    M = C - 1
+    if M > 0:
+        G = px[M - 1] ** 2 / Q[M - 1] - kT
+        px[M] += d4 * G
+        pnew = px[M]
+        for m in range(M - 1, 0, -1):
+            G = px[m - 1] ** 2 / Q[m - 1] - kT
+            sc = np.exp(-d8 * pnew / Q[m + 1])
+            pnew = sc * (sc * px[m] + d4 * G)
+            px[m] = pnew
+    G = 2 * KE - dof * kT
+    sc = np.exp(-d8 * px[1] / Q[1]) if M > 0 else 1.0
+    px[0] = sc * (sc * px[0] + d4 * G)
+    scale = np.exp(-d2 * px[0] / Q[0])
+    KE = KE * scale ** 2
+    xi += d2 * px / Q
+    G = 2 * KE - dof * kT
+    for m in range(M):
+        sc = np.exp(-d8 * px[m + 1] / Q[m + 1])
+        pu = sc * (sc * px[m] + d4 * G)
+        G = pu ** 2 / Q[m] - kT
+        px[m] = pu
+    if M > 0:
+        px[M] += d4 * G
+    return scale
+
+
+def _sinhx_x(x):
+    return 1 + x ** 2 / 6 + x ** 4 / 120 + x ** 6 / 5040 + x ** 8 / 362_880 + x ** 10 / 39_916_800
+
+
+class NPTOracle:
+    """State and step of the reference NPT integrator in FP64.
+
+    force_fn(x, box) -> (F (N,3), dUdeps_axis (3,)), dUdeps_axis[a] the derivative of the
+    energy under x_a -> (1+e) x_a, box_a -> (1+e) box_a (minus the virial diagonal)."""
+
+    def __init__(self, x, p, m, box, force_fn, dt, kT, pressure, tau_t, tau_p, chain_t=3, chain_p=3,
+                 couple_axes=((0, 1, 2),)):
+        self.x, self.p, self.m = np.array(x, float), np.array(p, float), np.asarray(m, float)
+        self.force_fn, self.dt, self.kT, self.P = force_fn, dt, kT, pressure
+        self.tau_t, self.tau_p = tau_t, tau_p
+        self.gid = np.full(3, -1)
+        for g, grp in enumerate(couple_axes):
+            for a in grp:
+                self.gid[a] = g
+        self.G = len(couple_axes)
+        self.nper = np.array([len(g) for g in couple_axes], float)
+        self.ref = np.asarray(box, float).copy()
+        self.Rb = np.zeros(self.G)
+        self.Pb = np.zeros(self.G)
+        N = len(self.m)
+        self.Mb = self.nper * (N + 1) * kT * tau_p ** 2
+        self.txi, self.tpx = np.zeros(chain_t), np.zeros(chain_t)
+        self.bxi, self.bpx = np.zeros(chain_p), np.zeros(chain_p)
+        self.F, d = force_fn(self.x, self.box)
+        self.dUdV = self._group(d)
+
+    def _group(self, axis_vals):
+        out = np.zeros(self.G)
+        for a in range(3):
+            if self.gid[a] >= 0:
+                out[self.gid[a]] += axis_vals[a]
+        return out
+
+    @property
+    def box(self):
+        return np.array([self.ref[a] * (np.exp(self.Rb[self.gid[a]]) if self.gid[a] >= 0 else 1.0)
+                         for a in range(3)])
+
+    def _ke(self):
+        return 0.5 * (self.p ** 2 / self.m[:, None]).sum()
+
+    def _box_force(self):
+        N_f = self.p.size
+        V = np.prod(self.box)
+        k2 = (self.p ** 2 / self.m[:, None]).sum(0)
+        return self._group(k2) + self.nper / N_f * k2.sum() - self.dUdV - self.P * V * self.nper
+
+    def _rates(self):
+        vb = self.Pb / self.Mb
+        per_axis = np.array([vb[self.gid[a]] if self.gid[a] >= 0 else 0.0 for a in range(3)])
+        return per_axis, per_axis + (self.nper * vb).sum() / self.p.size
+
+    def _exp_iL2(self, rate):
+        x = rate * self.dt / 2
+        self.p = self.p * np.exp(-x) + self.dt / 2 * self.F * _sinhx_x(x / 2) * np.exp(-x / 2)
+
+    def _chains(self):
+        kT, N = self.kT, len(self.m)
+        Qt = np.full(len(self.txi), kT * self.tau_t ** 2)
+        Qt[0] *= 3 * N
+        Qb = np.full(len(self.bxi), kT * self.tau_p ** 2)
+        Qb[0] *= self.G
+        self.Mb = self.nper * (N + 1) * kT * self.tau_p ** 2
+        return Qt, Qb
+
+    def step(self):
+        kT, dt = self.kT, self.dt
+        Qt, Qb = self._chains()
+        self.Pb = self.Pb * nhc_half_step(self.bxi, self.bpx, Qb, (self.Pb ** 2 / (2 * self.Mb)).sum(), self.G, kT, dt)
+        self.p = self.p * nhc_half_step(self.txi, self.tpx, Qt, self._ke(), self.p.size, kT, dt)
+        self.Pb = self.Pb + dt / 2 * self._box_force()
+        vb_axis, rate = self._rates()
+        self._exp_iL2(rate)
+        self.Rb = self.Rb + self.Pb / self.Mb * dt
+        y = vb_axis * dt
+        self.x = self.x * np.exp(y) + dt * (self.p / self.m[:, None]) * np.exp(y / 2) * _sinhx_x(y / 2)
+        self.F, d = self.force_fn(self.x, self.box)
+        self.dUdV = self._group(d)
+        self._exp_iL2(rate)
+        self.Pb = self.Pb + dt / 2 * self._box_force()
+        self.p = self.p * nhc_half_step(self.txi, self.tpx, Qt, self._ke(), self.p.size, kT, dt)
+        self.Pb = self.Pb * nhc_half_step(self.bxi, self.bpx, Qb, (self.Pb ** 2 / (2 * self.Mb)).sum(), self.G, kT, dt)
+
+    def invariant(self, PE):
+        """npt_nose_hoover_invariant (deepmd_jax/routine.py:426-456) given the potential energy."""
+        kT, N = self.kT, len(self.m)
+        Qt, Qb = self._chains()
+        E = PE + self._ke()
+        E += self.tpx[0] ** 2 / (2 * Qt[0]) + 3 * N * kT * self.txi[0]
+        E += (self.tpx[1:] ** 2 / (2 * Qt[1:]) + kT * self.txi[1:]).sum()
+        E += (self.bpx ** 2 / (2 * Qb) + kT * self.bxi).sum()
+        E += self.P * np.prod(self.box) + (self.Pb ** 2 / (2 * self.Mb)).sum()
+        return E
