@@ -316,6 +316,48 @@ def compressed(params, var, coord, box, types, vel, steps, warmup, acc=True):
     return out
 
 
+def npt(model, params, coord, box, types, vel, steps, warmup):
+    """NPT (MTK Nose-Hoover barostat + Nose-Hoover chain thermostat, isotropic coupling) at
+    300 K and 1 bar on the same 98,304-atom box, tau_t 100 fs, tau_p 1000 fs, skin 1 A:
+    device-resident steps/s and the relative box change over the timed steps."""
+    from deepmd_jax import _lib, synthetic as syn
+    from deepmd_jax.constants import MASS_UNIT_CONVERSION, PRESS_UNIT_CONVERSION, TEMP_UNIT_CONVERSION
+    s = _lib.DeviceSystem(model, types, 1)
+    s.set_box(box)
+    s.set_masses(np.array(syn.MASSES)[types] * MASS_UNIT_CONVERSION)
+    s.set_positions(coord)
+    s.set_velocities(vel)
+    s.nlist_allocate(params['rcut'] + 1.0, 1.2)
+    p = _lib.MDParams()
+    p.routine, p.dt, p.kT, p.tau, p.chain_length = _lib.ROUTINE['NPT'], 0.5, 300 * TEMP_UNIT_CONVERSION, 100.0, 3
+    p.pressure, p.tau_p, p.chain_length_p = 1.0 * PRESS_UNIT_CONVERSION, 1000.0, 3
+    for a in range(3):
+        p.couple[a] = 0
+    out = {'workload': 'NPT 300 K, 1 bar, isotropic, 98,304-atom water, tau_t 100 fs, tau_p 1000 fs, skin 1 A'}
+
+    def run(n):
+        done = 0
+        while done < n:
+            k = min(100, n - done)
+            _, _, err = s.md_run(p, k, record=False)
+            if err & 16:
+                raise RuntimeError('NaN/Inf in the NPT run')
+            if err & 2:
+                s.nlist_allocate(params['rcut'] + 1.0, 1.3)
+                continue
+            done += k
+        s.synchronize()
+    run(warmup)
+    b0 = s.get_boxes()[0]
+    t0 = time.perf_counter()
+    run(steps)
+    el = time.perf_counter() - t0
+    out['fp32'] = {'steps_per_s': steps / el, 'ms_per_step': 1e3 * el / steps,
+                   'box_rel_change': float(s.get_boxes()[0][0] / b0[0] - 1.0)}
+    print('# npt fp32: %.1f steps/s' % (steps / el), file=sys.stderr, flush=True)
+    return out
+
+
 def serving(model, rates, precisions, n_requests=256, md_chunk=20, budget_gib=32):
     """The serving stream: 256 water-box requests (n log-uniform in [32, 8192] H2O, first n
     sites of a seeded grid permutation), 70 % single E/F/virial evaluations, 30 % 200-step
@@ -364,6 +406,7 @@ def main():
     ap.add_argument('--no-serving', action='store_true')
     ap.add_argument('--no-pimd', action='store_true')
     ap.add_argument('--no-compressed', action='store_true')
+    ap.add_argument('--no-npt', action='store_true')
     ap.add_argument('--frames', type=int, default=1, help='seeded frames for the precision errors')
     ap.add_argument('--drift-steps', type=int, default=0, help='NVE drift run length per precision (0: skip)')
     ap.add_argument('--pimd-steps', type=int, default=100)
@@ -492,11 +535,11 @@ def main():
         from deepmd_jax.dpmodel import DPModel
         import io
         import contextlib
-        sim = Simulation(None, box, types, syn.MASSES, 'NVE', 0.5, coord, initial_velocity=vel,
-                         report_interval=10 ** 9, neighbor_skin=1.0, precision=args.precision,
-                         model=(DPModel(params), var, model))
         ns = max(20, args.steps // 4)
         with contextlib.redirect_stdout(io.StringIO()):
+            sim = Simulation(None, box, types, syn.MASSES, 'NVE', 0.5, coord, initial_velocity=vel,
+                             report_interval=10 ** 9, neighbor_skin=1.0, precision=args.precision,
+                             model=(DPModel(params), var, model))
             sim.run(10)
             t0 = time.perf_counter()
             sim.run(ns)
@@ -513,6 +556,8 @@ def main():
     if not args.no_compressed:
         out['compressed'] = compressed(params, var, coord, box, types, vel, args.steps, args.warmup,
                                        acc=not args.no_acc)
+    if not args.no_npt:
+        out['npt'] = npt(model, params, coord, box, types, vel, args.steps, args.warmup)
     if not args.no_pimd:
         out['pimd'] = pimd(model, params, var, steps=args.pimd_steps, acc_beads=0 if args.no_acc else args.pimd_acc_beads)
     if not args.no_serving:

@@ -389,6 +389,7 @@ static int32_t sys_layout(System* s, int nseg, const int32_t* seg_natoms, const 
   DPMD_TRY(sys_ensure(s, (void**)&s->d_pairs, sizeof(int) * NA));
   DPMD_TRY(sys_ensure(s, (void**)&s->d_perm, sizeof(int) * NA));
   DPMD_TRY(sys_ensure(s, (void**)&s->d_npt, sizeof(double) * 128));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_kepart, sizeof(double) * 3 * kKeBlocks * (size_t)nseg));
   hipStream_t st = s->stream;
   DPMD_HIP(hipMemsetAsync(s->d_pos, 0, sizeof(double) * NA * 3, st));
   DPMD_HIP(hipMemsetAsync(s->d_mom, 0, sizeof(double) * NA * 3, st));

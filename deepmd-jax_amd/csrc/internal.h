@@ -26,6 +26,7 @@ void set_error(const char* fmt, ...);
   } while (0)
 
 constexpr int kMaxT = DPMD_MAX_TYPES;
+constexpr int kKeBlocks = 64;   // most blocks per segment of the kinetic-energy reductions
 constexpr int kWave = 64;
 
 // Embedding-net feature layout.  An MFMA 16x16 accumulator tile holds, in lane
@@ -202,6 +203,7 @@ struct System {
                                 // (steps, NA, 3) each, caller order
   int* d_perm = nullptr;        // (NA) sorted index -> caller index
   double* d_nm = nullptr;       // PIMD normal modes
+  double* d_kepart = nullptr;   // kinetic-energy partial sums (nseg, kKeBlocks, 3)
   double* d_npt = nullptr;      // NPT barostat state (64 doubles) and its chunk-entry copy
   bool npt_ready = false;       // NPT state initialised for the current layout
   double* d_box_traj = nullptr; // per-step boxes of the last recorded chunk (steps, nseg, 3)

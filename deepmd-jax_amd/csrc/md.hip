@@ -102,18 +102,36 @@ __global__ void reduce_kernel(const float* eatom, int nt, RowBase rb, const SegI
   }
 }
 
-// kinetic energy per segment (per image)
-__global__ void ke_kernel(const double* mom, const double* mass, const SegInfo* seg, double* out) {
+// Kinetic energy per segment (per image) in two passes: block (kb, segment) sums
+// p_a^2 / m per axis over its strided share of the segment's atoms into part, then one
+// thread per segment adds the KB partials in a fixed order (deterministic).  Optionally
+// the per-axis sums (2 KE per axis) of segment 0 go to axis_out.
+__global__ void ke_part_kernel(const double* mom, const double* mass, const SegInfo* seg, int KB, double* part) {
   __shared__ double red[256];
-  const SegInfo& S = seg[blockIdx.x];
-  double s = 0.0;
-  for (int i = threadIdx.x; i < S.natoms; i += blockDim.x) {
+  const SegInfo& S = seg[blockIdx.y];
+  double k[3] = {0.0, 0.0, 0.0};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < S.natoms; i += KB * blockDim.x) {
     const size_t g = (size_t)S.base + i;
-    const double* p = mom + g * 3;
-    s += (p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) / (2.0 * mass[g]);
+    const double im = 1.0 / mass[g];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) k[d] += mom[g * 3 + d] * mom[g * 3 + d] * im;
   }
-  s = block_sum(s, red);
-  if (threadIdx.x == 0) out[blockIdx.x] = s / S.nimg;
+  for (int d = 0; d < 3; ++d) {
+    const double v = block_sum(k[d], red);
+    if (threadIdx.x == 0) part[((size_t)blockIdx.y * KB + blockIdx.x) * 3 + d] = v;
+  }
+}
+
+__global__ void ke_final_kernel(const double* part, const SegInfo* seg, int nseg, int KB, double* out,
+                                double* axis_out) {
+  const int sid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sid >= nseg) return;
+  double k[3] = {0.0, 0.0, 0.0};
+  for (int b = 0; b < KB; ++b)
+    for (int d = 0; d < 3; ++d) k[d] += part[((size_t)sid * KB + b) * 3 + d];
+  if (out) out[sid] = 0.5 * (k[0] + k[1] + k[2]) / seg[sid].nimg;
+  if (axis_out && sid == 0)
+    for (int d = 0; d < 3; ++d) axis_out[d] = k[d];
 }
 
 // per-segment sum of a per-atom scratch array, added to thermo[64 sid + slot] (per image)
@@ -343,11 +361,24 @@ int32_t md_reduce_launch(System* s) {
   return DPMD_OK;
 }
 
-int32_t md_ke_launch(System* s, double* d_out) {
-  hipLaunchKernelGGL(ke_kernel, dim3(s->nseg), dim3(256), 0, s->stream, s->d_mom, s->d_mass, s->d_seg, d_out);
+// blocks per segment for the kinetic-energy reductions: about 2,048 atoms per block
+static int ke_blocks(System* s) {
+  int mx = 1;
+  for (const SegInfo& S : s->seg) mx = std::max(mx, S.natoms);
+  return std::max(1, std::min(kKeBlocks, (mx + 2047) / 2048));
+}
+
+static int32_t ke_launch(System* s, double* d_out, double* d_axis) {
+  const int KB = ke_blocks(s);
+  hipLaunchKernelGGL(ke_part_kernel, dim3(KB, s->nseg), dim3(256), 0, s->stream, s->d_mom, s->d_mass, s->d_seg, KB,
+                     s->d_kepart);
+  hipLaunchKernelGGL(ke_final_kernel, dim3((s->nseg + 63) / 64), dim3(64), 0, s->stream, s->d_kepart, s->d_seg,
+                     s->nseg, KB, d_out, d_axis);
   DPMD_HIP(hipGetLastError());
   return DPMD_OK;
 }
+
+int32_t md_ke_launch(System* s, double* d_out) { return ke_launch(s, d_out, nullptr); }
 
 // BAOAB O step for every segment; thermostat work of segment r is thermo[64 r]
 int32_t md_langevin_o(System* s, double dt, double gamma, double kT, uint64_t seed, uint64_t step) {
@@ -483,21 +514,6 @@ __device__ __forceinline__ double sinhx_x(double x) {
   const double x2 = x * x;
   return 1 + x2 / 6 + x2 * x2 / 120 + x2 * x2 * x2 / 5040 + x2 * x2 * x2 * x2 / 362880 +
          x2 * x2 * x2 * x2 * x2 / 39916800;
-}
-
-// per-axis sum of p^2 / m over the segment's atoms -> npt[40..43)
-__global__ void npt_axis_ke_kernel(const double* mom, const double* mass, int natoms, double* npt) {
-  __shared__ double red[256];
-  double k[3] = {0.0, 0.0, 0.0};
-  for (int i = threadIdx.x; i < natoms; i += blockDim.x) {
-    const double im = 1.0 / mass[i];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) k[d] += mom[(size_t)i * 3 + d] * mom[(size_t)i * 3 + d] * im;
-  }
-  for (int d = 0; d < 3; ++d) {
-    const double v = block_sum(k[d], red);
-    if (threadIdx.x == 0) npt[40 + d] = v;
-  }
 }
 
 __device__ __forceinline__ void npt_set_box(const NptArgs& a, double* npt, SegInfo& S) {
@@ -649,7 +665,8 @@ int32_t md_npt_baro_half(System* s, const NptArgs& a) {
 }
 
 int32_t md_npt_box(System* s, const NptArgs& a, int phase) {
-  hipLaunchKernelGGL(npt_axis_ke_kernel, dim3(1), dim3(256), 0, s->stream, s->d_mom, s->d_mass, s->NA, s->d_npt);
+  // per-axis sum of p^2 / m -> npt[40..43)
+  DPMD_TRY(ke_launch(s, nullptr, s->d_npt + 40));
   if (phase == 1) DPMD_TRY(md_reduce_launch(s));
   hipLaunchKernelGGL(npt_box_kernel, dim3(1), dim3(64), 0, s->stream, a, s->d_npt, s->d_seg, s->d_red, phase);
   DPMD_HIP(hipGetLastError());
