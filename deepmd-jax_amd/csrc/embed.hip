@@ -42,6 +42,9 @@ __device__ __forceinline__ void wave_sync() {
 // ---------------------------------------------------------------------------------
 // Host-side layout and packing
 
+// 2 log2(e): forward weight tables and biases are stored multiplied by it (see tanh_s)
+constexpr float kTanhScale = 2.8853900817779268f;
+
 bool EmbedLayout::init(const int* widths) {
   for (int l = 0; l < 3; ++l) w[l] = widths[l];
   if (w[0] <= 0 || w[0] > 32) return false;
@@ -100,17 +103,19 @@ void embed_pack_net(const EmbedLayout& L, const float* K0, const float* B0, cons
   EmbedNetOffsets o = embed_net_offsets(L);
   memset(out, 0, sizeof(float) * o.total);
   int t1 = L.tiles[0], t2 = L.tiles[1], t3 = L.tiles[2];
+  // forward operands (first-layer weights, biases, w1a, w2a) carry the tanh scale; the
+  // backward tables (w2b, w1b) multiply gradients and stay unscaled
   for (int s = 0; s < 16 * t1; ++s) {
     int a = L.logical(0, s);
-    if (a >= 0) { out[o.k0 + s] = K0[a]; out[o.b0 + s] = B0[a]; }
+    if (a >= 0) { out[o.k0 + s] = kTanhScale * K0[a]; out[o.b0 + s] = kTanhScale * B0[a]; }
   }
   for (int s = 0; s < 16 * t2; ++s) {
     int c = L.logical(1, s);
-    if (c >= 0) out[o.b1 + s] = B1[c];
+    if (c >= 0) out[o.b1 + s] = kTanhScale * B1[c];
   }
   for (int s = 0; s < 16 * t3; ++s) {
     int w = L.logical(2, s);
-    if (w >= 0) out[o.b2 + s] = B2[w];
+    if (w >= 0) out[o.b2 + s] = kTanhScale * B2[w];
   }
   const int w1 = L.w[1], w2 = L.w[2];
   for (int lane = 0; lane < 64; ++lane) {
@@ -120,13 +125,13 @@ void embed_pack_net(const EmbedLayout& L, const float* K0, const float* B0, cons
       for (int st = 0; st < 4 * t1; ++st) {
         int a = L.logical(0, 16 * (st / 4) + 4 * q + st % 4);
         int c = L.logical(1, 16 * ot + m);
-        out[o.w1a + (ot * 4 * t1 + st) * 64 + lane] = (a >= 0 && c >= 0) ? K1[a * w1 + c] : 0.f;
+        out[o.w1a + (ot * 4 * t1 + st) * 64 + lane] = (a >= 0 && c >= 0) ? kTanhScale * K1[a * w1 + c] : 0.f;
       }
     for (int ot = 0; ot < t3; ++ot)
       for (int st = 0; st < 4 * t2; ++st) {
         int c = L.logical(1, 16 * (st / 4) + 4 * q + st % 4);
         int w = L.logical(2, 16 * ot + m);
-        out[o.w2a + (ot * 4 * t2 + st) * 64 + lane] = (c >= 0 && w >= 0) ? K2[c * w2 + w] : 0.f;
+        out[o.w2a + (ot * 4 * t2 + st) * 64 + lane] = (c >= 0 && w >= 0) ? kTanhScale * K2[c * w2 + w] : 0.f;
       }
     // backward: A[m=in slot][k=out slot]
     for (int ct = 0; ct < t2; ++ct)
@@ -259,12 +264,14 @@ __device__ __forceinline__ void env_backward(const Geo& g, const float dR[10], f
   }
 }
 
-// tanh(x) = 1 - 2 / (1 + 2^(2x log2 e)) with the hardware exp2 and reciprocal, branch
-// free (five instructions, two of them transcendental).  Absolute error ~2e-7 over the
-// real line, saturating to +-1; near 0 the error stays absolute (no relative accuracy
-// is needed: every activation enters a sum with O(1) terms).
-__device__ __forceinline__ float tanh_f(float x) {
-  const float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);
+// The forward weight tables and biases are stored multiplied by kTanhScale = 2 log2(e)
+// (embed_pack_net), so a pre-activation arrives as y = 2 log2(e) z and
+// tanh(z) = 1 - 2 / (1 + 2^y) with the hardware exp2 and reciprocal, branch free (four
+// instructions, two of them transcendental).  Absolute error ~2e-7 over the real line,
+// saturating to +-1; near 0 the error stays absolute (no relative accuracy is needed:
+// every activation enters a sum with O(1) terms).
+__device__ __forceinline__ float tanh_s(float y) {
+  const float e = __builtin_amdgcn_exp2f(y);
   return __builtin_fmaf(-2.f, __builtin_amdgcn_rcpf(1.f + e), 1.f);
 }
 
@@ -561,7 +568,7 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
       } else {
         float h1[4 * T1];
 #pragma unroll
-        for (int r = 0; r < 4 * T1; ++r) h1[r] = r < V1 ? tanh_f(g.st * k0(r) + b0(r)) : 0.f;
+        for (int r = 0; r < 4 * T1; ++r) h1[r] = r < V1 ? tanh_s(g.st * k0(r) + b0(r)) : 0.f;
         floatx4 acc2[T2];
 #pragma unroll
         for (int t = 0; t < T2; ++t) acc2[t] = floatx4{0.f, 0.f, 0.f, 0.f};
@@ -575,7 +582,7 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
         for (int r = 0; r < 4 * T2; ++r) {
           float v = 0.f;
           if (r < V2) {
-            v = tanh_f(acc2[r / 4][r % 4] + b1(r));
+            v = tanh_s(acc2[r / 4][r % 4] + b1(r));
             if (r / Z2 < 4 * T1) v += h1[r / Z2];
           }
           h2[r] = v;
@@ -596,7 +603,7 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
               const int r = 4 * (g0 + k) + i;
-              float v = tanh_f(acc3[k][i] + b2(r));
+              float v = tanh_s(acc3[k][i] + b2(r));
               if (r / Z3 < 4 * T2) v += h2[r / Z3];
               Hl[n * HG + 16 * k + 4 * q + i] = v;
             }
@@ -671,7 +678,9 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
   float* small = smem;
   float* wbase = load_small_tables<T1, T2, T3>(a, smem) + wv * a.lds_wave_floats;
   // chunk-phase layout
-  constexpr int ODT = (10 * HS + 3) / 4 * 4;
+  // dE/dT rows 0..9 and a zero row 10: MFMA operand rows beyond the ten environment rows
+  // read the zero row instead of being selected away per product
+  constexpr int ODT = (11 * HS + 3) / 4 * 4;
   float* dTp = wbase;                    // 10 * HS (rows x >= 10 of the MFMA operands are zero)
   float* Rl = dTp + ODT;                 // [16 neighbours][kRS]
   float* dRl = Rl + 16 * kRS;            // [16 neighbours][kRS]
@@ -702,7 +711,7 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
       float* Xl = Tl + 10 * W;     // 10*A
       float* Ul = Xl + 10 * A;     // 10*A
       for (int e = lane; e < 10 * W; e += 64) Tl[e] = Tg[e] + (e < W ? a.tbias[e] : 0.f);
-      for (int e = lane; e < 10 * HS; e += 64) dTp[e] = 0.f;
+      for (int e = lane; e < 11 * HS; e += 64) dTp[e] = 0.f;
       wave_sync();
       const float r2 = 1.4142135623730951f;
       for (int e = lane; e < 10 * A; e += 64) {
@@ -780,6 +789,10 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
       wave_sync();
     }
     // ---- chunks
+    // per-lane dE/dT operand rows: dR products use row n, dh3 products rows 4 kk + q
+    // (only kk = 2 can pass row 9); rows past 9 read the zero row
+    const float* dTr = dTp + (n < 10 ? n : 10) * HS + 4 * q;
+    const float* dTc2 = dTp + (8 + q < 10 ? 8 + q : 10) * HS + n;
     float fc[3] = {0.f, 0.f, 0.f};
     for (int j = 0; j < a.ntypes; ++j) {
       const int m = compact_row(a, S, c, j, pc, list, lane);
@@ -870,11 +883,11 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
         } else {
           float h1[4 * T1];
 #pragma unroll
-          for (int r = 0; r < 4 * T1; ++r) h1[r] = r < V1 ? tanh_f(g.st * k0(r) + b0(r)) : 0.f;
-          // h2 = t2 + h1[r/Z2] and h3 = t3 + h2[r/Z3] are re-formed where used
-          auto H2 = [&](int r, const float* t2v) -> float {
-            return t2v[r] + ((r / Z2 < 4 * T1) ? h1[r / Z2] : 0.f);
-          };
+          for (int r = 0; r < 4 * T1; ++r) h1[r] = r < V1 ? tanh_s(g.st * k0(r) + b0(r)) : 0.f;
+          // h2 = t2 + h1[r/Z2] is formed once (t2 is kept for tanh'); h3 = t3 + h2[r/Z3] is
+          // re-formed where used
+          float h2[4 * T2];
+          auto H2 = [&](int r, const float*) -> float { return h2[r]; };
           floatx4 acc2[T2];
 #pragma unroll
           for (int t = 0; t < T2; ++t) acc2[t] = floatx4{0.f, 0.f, 0.f, 0.f};
@@ -885,7 +898,10 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
               return pack8<PREC, 4 * T1>([&](int r) { return h1[r]; }, s2); }, acc2);
           float t2[4 * T2];
 #pragma unroll
-          for (int r = 0; r < 4 * T2; ++r) t2[r] = r < V2 ? tanh_f(acc2[r / 4][r % 4] + b1(r)) : 0.f;
+          for (int r = 0; r < 4 * T2; ++r) {
+            t2[r] = r < V2 ? tanh_s(acc2[r / 4][r % 4] + b1(r)) : 0.f;
+            h2[r] = t2[r] + ((r / Z2 < 4 * T1) ? h1[r / Z2] : 0.f);
+          }
           wave_sync();  // Rl visible to the dh3 products
           floatx4 dr = floatx4{0.f, 0.f, 0.f, 0.f};
           floatx4 dh2[T2];
@@ -906,7 +922,7 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
 #pragma unroll
             for (int k = 0; k < SZ; ++k)
 #pragma unroll
-              for (int i = 0; i < 4; ++i) t3[4 * k + i] = tanh_f(acc3[k][i] + b2(4 * (g0 + k) + i));
+              for (int i = 0; i < 4; ++i) t3[4 * k + i] = tanh_s(acc3[k][i] + b2(4 * (g0 + k) + i));
             // dR^T[x][nbr] += sum_slot dT[x][slot] h3[slot][nbr] over this group's slots
 #pragma unroll
             for (int k = 0; k < SZ; ++k)
@@ -914,7 +930,7 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
               for (int i = 0; i < 4; ++i) {
                 const int r = 4 * (g0 + k) + i;
                 const float h3v = t3[4 * k + i] + ((r / Z3 < 4 * T2) ? H2(r / Z3, t2) : 0.f);
-                dr = mfma4(n < 10 ? dTp[n * HS + 16 * (g0 + k) + 4 * q + i] : 0.f, h3v, dr);
+                dr = mfma4(dTr[16 * (g0 + k) + i], h3v, dr);
               }
             // dh3[slot][nbr] = sum_x dT[x][slot] R[nbr][x]
             floatx4 dh3[SZ];
@@ -923,7 +939,7 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
               floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
               for (int kk = 0; kk < 3; ++kk)
-                acc = mfma4(4 * kk + q < 10 ? dTp[(4 * kk + q) * HS + 16 * (g0 + k) + n] : 0.f, Rl[n * kRS + 4 * kk + q], acc);
+                acc = mfma4(kk < 2 ? dTp[(4 * kk + q) * HS + 16 * (g0 + k) + n] : dTc2[16 * (g0 + k)], Rl[n * kRS + 4 * kk + q], acc);
               dh3[k] = acc;
             }
             // repeat-residual: dh2[r/Z3] += dh3[r]; then dz3 = dh3 (1 - t3^2)
@@ -971,6 +987,7 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
           dst = 0.f;
 #pragma unroll
           for (int r1 = 0; r1 < 4 * T1; ++r1) dst += dh1[r1 / 4][r1 % 4] * (1.f - h1[r1] * h1[r1]) * k0(r1);
+          dst *= 1.f / kTanhScale;  // k0 is stored scaled
           dst += __shfl_xor(dst, 16);
           dst += __shfl_xor(dst, 32);
         }
@@ -1105,7 +1122,7 @@ static int32_t launch_embed(System* s, bool backward) {
     wave_floats = 4 * a.list_cap + 16 * kRS + 16 * (16 * kGroup + 1);
     wave_floats = std::max(wave_floats, (size_t)(10 * W));
   } else {
-    wave_floats = (10 * HS + 3) / 4 * 4 + std::max((size_t)(2 * 16 * kRS + 4 * a.list_cap), (size_t)(10 * W + 20 * A));
+    wave_floats = (11 * HS + 3) / 4 * 4 + std::max((size_t)(2 * 16 * kRS + 4 * a.list_cap), (size_t)(10 * W + 20 * A));
   }
   wave_floats = (wave_floats + 3) / 4 * 4;
   a.lds_wave_floats = (int)wave_floats;
