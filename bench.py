@@ -113,16 +113,43 @@ def accuracy(params, var, model, frames=1):
     return res
 
 
-def nve_drift(model, params, coord, box, types, vel, steps, sample=100, precisions=('fp32', 'bf16')):
+def nve_drift(model, params, coord, box, types, vel, steps, sample=100, precisions=('fp32', 'bf16'), equil_steps=0):
     """Total-energy drift of NVE velocity Verlet (dt 0.5 fs) from the same initial state
     per precision: KE + PE sampled every `sample` steps (PE evaluated in the same
     precision), least-squares slope in eV/atom/ps over the whole run and over the first
-    2,000 steps."""
+    2,000 steps.  equil_steps > 0: the initial state is first equilibrated in FP32 by NVT
+    Langevin (BAOAB, 300 K, tau_t 100 fs) for that many steps, so the NVE runs start from
+    a state at 300 K under the seeded potential instead of the water grid (whose excess
+    potential energy under the random-init potential otherwise turns into heat)."""
     from deepmd_jax import _lib, synthetic as syn
-    from deepmd_jax.constants import MASS_UNIT_CONVERSION
+    from deepmd_jax.constants import MASS_UNIT_CONVERSION, TEMP_UNIT_CONVERSION
     m = np.array(syn.MASSES)[types] * MASS_UNIT_CONVERSION
     N = len(types)
-    out = {'steps': steps, 'sample_every': sample, 'dt_fs': 0.5}
+    out = {'steps': steps, 'sample_every': sample, 'dt_fs': 0.5, 'equilibration_steps': equil_steps}
+    if equil_steps:
+        s = _lib.DeviceSystem(model, types, 1)
+        s.set_box(box)
+        s.set_masses(m)
+        s.set_positions(coord)
+        s.set_velocities(vel)
+        s.nlist_allocate(params['rcut'] + 1.0, 1.2)
+        p = _lib.MDParams()
+        p.routine, p.dt, p.kT, p.gamma, p.seed = _lib.ROUTINE['NVT_langevin'], 0.5, 300 * TEMP_UNIT_CONVERSION, 0.01, 11
+        done, Teq = 0, []
+        while done < equil_steps:
+            _, _, err = s.md_run(p, 100, record=False)
+            if err & 2:
+                s.nlist_allocate(params['rcut'] + 1.0, 1.3)
+                continue
+            if err & 16:
+                raise RuntimeError('NaN/Inf during the NVT equilibration')
+            done += 100
+            if done % 1000 == 0:
+                Teq.append(2 * float(s.kinetic_energy()[0]) / (3 * N) / TEMP_UNIT_CONVERSION)
+                print('# drift equilibration step %d T %.1f K' % (done, Teq[-1]), file=sys.stderr, flush=True)
+        coord, vel = s.get_positions()[0], s.get_velocities()[0]
+        out['equilibration_temperature_K'] = Teq
+        del s
     for prec in precisions:
         s = _lib.DeviceSystem(model, types, 1)
         s.set_box(box)
@@ -409,6 +436,8 @@ def main():
     ap.add_argument('--no-npt', action='store_true')
     ap.add_argument('--frames', type=int, default=1, help='seeded frames for the precision errors')
     ap.add_argument('--drift-steps', type=int, default=0, help='NVE drift run length per precision (0: skip)')
+    ap.add_argument('--drift-equil-steps', type=int, default=0,
+                    help='NVT Langevin equilibration steps before the NVE drift runs (0: start from the grid)')
     ap.add_argument('--pimd-steps', type=int, default=100)
     ap.add_argument('--pimd-acc-beads', type=int, default=2)
     ap.add_argument('--serving-rates', default='20', help='comma-separated Poisson arrival rates (requests/s)')
@@ -552,7 +581,8 @@ def main():
     if not args.no_acc:
         out['precision_errors_vs_fp64_oracle'] = accuracy(params, var, model, args.frames)
     if args.drift_steps:
-        out['nve_drift'] = nve_drift(model, params, coord, box, types, vel, args.drift_steps)
+        out['nve_drift'] = nve_drift(model, params, coord, box, types, vel, args.drift_steps,
+                                     equil_steps=args.drift_equil_steps)
     if not args.no_compressed:
         out['compressed'] = compressed(params, var, coord, box, types, vel, args.steps, args.warmup,
                                        acc=not args.no_acc)
