@@ -78,3 +78,14 @@ class Evaluator:
     def evaluate_frame(self, coord, box3, type_idx, precision='fp32'):
         e, f, v = self.evaluate_batch(np.asarray(coord)[None], box3, type_idx, precision)
         return e[0], f[0], v[0]
+
+
+def energy_forces_stress(evaluator, positions, cell, type_idx, precision='fp32'):
+    """Energy, forces and stress of one frame as the reference's ASE calculator returns them
+    (deepmd_jax/ase_calc.py:79-143): stress = (1/V) dU/d(strain) in Voigt order
+    (xx, yy, zz, yz, xz, xy), eV/A^3 -- the negative of jax_md.quantity.stress.  With the
+    engine's virial W = -dU/d(strain) that is -W / V."""
+    box3 = canonical_box(cell)
+    e, f, w = evaluator.evaluate_frame(positions, box3, type_idx, precision)
+    s = -np.asarray(w) / float(np.prod(box3))
+    return float(e), f, np.array([s[0, 0], s[1, 1], s[2, 2], s[1, 2], s[0, 2], s[0, 1]])

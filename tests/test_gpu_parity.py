@@ -354,3 +354,24 @@ def test_simulation_trajectory_frames(device_model, model0):
         if k in (9, 19, 24):
             assert np.abs(traj['position'][k + 1] - x).max() < 1e-5, k
             assert np.abs(traj['velocity'][k + 1] - pm / m[:, None]).max() < 1e-5, k
+
+
+@pytest.mark.parametrize('nmol', [32, 216])
+def test_ase_energy_forces_stress(device_model, model0, nmol):
+    """The ASE calculator's computation (deepmd_jax/ase_calc.py:79-143): stress = (1/V) dU/d(strain)
+    in Voigt order against the FP64 oracle's strain derivative, within 1e-4 of its largest
+    component; energy and forces at the FP32 tolerances.  32 H2O (L = 9.9 A) runs as a 2x2x2
+    supercell, 216 H2O on the neighbour list."""
+    from deepmd_jax import synthetic as syn
+    from deepmd_jax.evaluator import Evaluator, energy_forces_stress
+    print('case ase stress', nmol, flush=True)
+    params, var = model0
+    coord, box, types = syn.water_box(nmol, seed=15)
+    E, F, W = O.energy_force_virial(params, var, coord, box, types, lattice='auto')
+    S = -W.numpy() / np.prod(box)
+    ref = np.array([S[0, 0], S[1, 1], S[2, 2], S[1, 2], S[0, 2], S[0, 1]])
+    e, f, s = energy_forces_stress(Evaluator(params, var, model=device_model), coord, np.diag(box), types)
+    N = len(types)
+    assert abs(e - E.item()) / N <= 1e-5
+    assert np.abs(f - F.numpy()).max() <= 1e-4 * max(1.0, np.abs(F.numpy()).max())
+    assert np.abs(s - ref).max() <= 1e-4 * np.abs(ref).max(), (s, ref)
