@@ -434,6 +434,8 @@ def main():
     ap.add_argument('--no-pimd', action='store_true')
     ap.add_argument('--no-compressed', action='store_true')
     ap.add_argument('--no-npt', action='store_true')
+    ap.add_argument('--nve-run-steps', type=int, default=2000,
+                    help='length of the full NVE run reported beside the timed steps (0: skip)')
     ap.add_argument('--frames', type=int, default=1, help='seeded frames for the precision errors')
     ap.add_argument('--drift-steps', type=int, default=0, help='NVE drift run length per precision (0: skip)')
     ap.add_argument('--drift-equil-steps', type=int, default=0,
@@ -548,6 +550,21 @@ def main():
         'nlist_reallocations': realloc,
         'roofline': roof,
     }
+    # the full configured run: 2,000 NVE steps from the initial state, rebuilds included
+    if args.nve_run_steps and args.nmol == 32768:
+        out['nve_run'] = {'steps': args.nve_run_steps}
+        for prec in (('fp32', 'bf16') if args.precision == 'fp32' and not args.no_half else (args.precision,)):
+            p.precision = _lib.PREC[prec]
+            reset()
+            sys_.synchronize()
+            t0 = time.perf_counter()
+            ra = run_steps(args.nve_run_steps)
+            el = time.perf_counter() - t0
+            out['nve_run'][prec] = {'steps_per_s': args.nve_run_steps / el, 'wall_s': el,
+                                    'nlist_rebuilds': sys_.nlist_rebuilds(), 'nlist_reallocations': ra}
+            print('# nve run %s: %d steps at %.1f steps/s, %d rebuilds' % (
+                prec, args.nve_run_steps, args.nve_run_steps / el, sys_.nlist_rebuilds()), file=sys.stderr, flush=True)
+        p.precision = _lib.PREC[args.precision]
     # precision forms: steps/s, same workload
     if args.precision == 'fp32' and not args.no_half:
         for prec in ('bf16', 'fp16'):

@@ -469,6 +469,7 @@ static int32_t nl_allocate(System* s, double rc_skin, double ratio, const int32_
   }
   s->rc_skin = rc_skin;
   s->buffer_ratio = ratio;
+  s->nl_rebuilds = 0;
   for (SegInfo& S : s->seg)
     for (int d = 0; d < 3; ++d) {
       S.L[d] = (float)S.boxd[d];
@@ -1019,6 +1020,7 @@ int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps, 
   }
   int h[16];
   DPMD_TRY(nl_read_flags(s, h));
+  s->nl_rebuilds += h[9];
   uint32_t code = (uint32_t)(h[2] & (2 | 16));
   if (h[8]) code |= 2;
   if (error_code) *error_code |= code;
@@ -1158,6 +1160,13 @@ int32_t dpmd_timing_get(dpmd_system_t sys, int32_t which, double* ms, int64_t* c
   timer_harvest(s);
   if (ms) *ms = s->timer.ms[which];
   if (count) *count = s->timer.count[which];
+  return DPMD_OK;
+}
+
+int32_t dpmd_nlist_rebuilds(dpmd_system_t sys, int64_t* count) {
+  SYS(sys);
+  if (!count) { set_error("null argument"); return DPMD_E_ARG; }
+  *count = s->nl_rebuilds;
   return DPMD_OK;
 }
 

@@ -181,7 +181,9 @@ struct System {
   int* d_atom_cell = nullptr;
   int nl_lds_list = 512;
   int* d_flags = nullptr;       // [0] rebuild flag, [1] drift flag, [2] error bits,
-                                // [4..4+kMaxT) max count per type, [8] measure-list overflow
+                                // [4..4+kMaxT) max count per type, [8] measure-list overflow,
+                                // [9] drift-triggered rebuilds since the last clear
+  int64_t nl_rebuilds = 0;      // drift-triggered rebuilds since the list was allocated
   // model work buffers
   int rowbase[kMaxT + 1];       // fitting rows per type (all segments), padded to 64
   int mrows = 0;

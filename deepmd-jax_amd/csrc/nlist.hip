@@ -252,8 +252,10 @@ __global__ void nl_drift_kernel(const double* pos, const float4* canon, const Se
   if (__ballot(over) && (threadIdx.x & 63) == 0) atomicOr(&flags[1], 1);
 }
 
+// flags[9] counts the rebuilds the drift test triggered since the flags were cleared
 __global__ void nl_decide_kernel(int* flags) {
   flags[0] = flags[1] != 0 ? 1 : 0;
+  flags[9] += flags[0];
   flags[1] = 0;
 }
 

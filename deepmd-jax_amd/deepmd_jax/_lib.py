@@ -88,6 +88,7 @@ def lib():
         'dpmd_system_get_boxes': (c_i32, [c_vp, c_vp]),
         'dpmd_md_box_trajectory': (c_i32, [c_vp, c_vp]),
         'dpmd_npt_state_get': (c_i32, [c_vp, c_vp]),
+        'dpmd_nlist_rebuilds': (c_i32, [c_vp, c_vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -123,7 +124,7 @@ def exported_symbols():
             'dpmd_md_run', 'dpmd_pimd_setup', 'dpmd_kinetic_energy', 'dpmd_thermostat_energy',
             'dpmd_thermostat_state_get', 'dpmd_thermostat_state_set', 'dpmd_timing_enable',
             'dpmd_timing_get', 'dpmd_pair_count', 'dpmd_synchronize', 'dpmd_system_get_boxes',
-            'dpmd_md_box_trajectory', 'dpmd_npt_state_get']
+            'dpmd_md_box_trajectory', 'dpmd_npt_state_get', 'dpmd_nlist_rebuilds']
 
 
 class DeviceModel:
@@ -377,6 +378,12 @@ class DeviceSystem:
         b = np.zeros((nsteps, self.nrep, 3))
         check(lib().dpmd_md_box_trajectory(self.handle, ptr(b)))
         return b
+
+    def nlist_rebuilds(self):
+        """Drift-triggered neighbour-list rebuilds inside md_run since the last allocation."""
+        c = ctypes.c_int64()
+        check(lib().dpmd_nlist_rebuilds(self.handle, ctypes.byref(c)))
+        return int(c.value)
 
     def npt_state(self):
         """The 64-double NPT barostat state (log box scales, box momenta and masses, dU/deps,
