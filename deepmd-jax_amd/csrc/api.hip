@@ -389,7 +389,8 @@ static int32_t sys_layout(System* s, int nseg, const int32_t* seg_natoms, const 
   DPMD_TRY(sys_ensure(s, (void**)&s->d_pairs, sizeof(int) * NA));
   DPMD_TRY(sys_ensure(s, (void**)&s->d_perm, sizeof(int) * NA));
   DPMD_TRY(sys_ensure(s, (void**)&s->d_npt, sizeof(double) * 128));
-  DPMD_TRY(sys_ensure(s, (void**)&s->d_kepart, sizeof(double) * 3 * kKeBlocks * (size_t)nseg));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_kepart, sizeof(double) * kKeBlocks * std::max<size_t>(3 * (size_t)nseg, 4)));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_mobile, sizeof(float) * NA));
   hipStream_t st = s->stream;
   DPMD_HIP(hipMemsetAsync(s->d_pos, 0, sizeof(double) * NA * 3, st));
   DPMD_HIP(hipMemsetAsync(s->d_mom, 0, sizeof(double) * NA * 3, st));
@@ -411,6 +412,8 @@ static int32_t sys_layout(System* s, int nseg, const int32_t* seg_natoms, const 
   s->force_valid = false;
   s->rng_counter = 0;
   s->npt_ready = false;
+  s->has_fixed = false;
+  s->n_mobile = NA;
   return DPMD_OK;
 }
 
@@ -446,6 +449,8 @@ static int32_t model_forces_impl(System* s, int precision) {
   timer_begin(s, 3, &ev);
   DPMD_TRY(embed_launch(s, true, precision));
   timer_end(s, 3, ev);
+  // fixed atoms: their coordinates carry no gradient (deepmd_jax/md.py:428-430)
+  if (s->has_fixed) DPMD_TRY(md_mask_forces(s));
   return DPMD_OK;
 }
 
@@ -884,6 +889,10 @@ int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps, 
     set_error("unknown routine %d", p->routine);
     return DPMD_E_ARG;
   }
+  if (s->has_fixed && (p->routine == DPMD_PIMD_PILE_L || p->routine == DPMD_NPT_NH)) {
+    set_error("fixed atoms are supported with NVE, NVT and NVT_langevin (deepmd_jax/md.py:228-229)");
+    return DPMD_E_UNSUPPORTED;
+  }
   const bool npt = p->routine == DPMD_NPT_NH;
   NptArgs na;
   memset(&na, 0, sizeof(na));
@@ -893,6 +902,10 @@ int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps, 
     return DPMD_E_ARG;
   }
   if (p->routine == DPMD_PIMD_PILE_L && !s->d_nm) { set_error("PIMD normal modes not set"); return DPMD_E_STATE; }
+  if (p->remove_com && (p->routine == DPMD_PIMD_PILE_L || s->nseg != 1)) {
+    set_error("per-step centre-of-mass removal runs one system without beads");
+    return DPMD_E_UNSUPPORTED;
+  }
   if (p->routine == DPMD_PIMD_PILE_L && !s->uniform) { set_error("PIMD needs identical beads"); return DPMD_E_STATE; }
   const size_t NA = s->NA;
   const bool rec = traj_pos || traj_vel;
@@ -948,6 +961,7 @@ int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps, 
       DPMD_TRY(model_forces_impl(s, prec));
       timer_begin(s, 4, &ev);
       DPMD_TRY(md_kick(s, 0.5 * dt));
+      if (s->has_fixed) DPMD_TRY(md_mask_momenta(s));
       timer_end(s, 4, ev);
     } else if (p->routine == DPMD_NVT_LANGEVIN) {
       DPMD_TRY(md_kick(s, 0.5 * dt));
@@ -963,6 +977,7 @@ int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps, 
       DPMD_TRY(model_forces_impl(s, prec));
       DPMD_TRY(md_kick(s, 0.5 * dt));
       DPMD_TRY(md_nh_half(s, p->kT, dt, p->tau, p->chain_length));
+      if (s->has_fixed) DPMD_TRY(md_mask_momenta(s));
     } else if (npt) {
       // apply_fn of deepmd_jax/routine.py:396-423 around inner_step :365-394
       DPMD_TRY(md_npt_baro_half(s, na));
@@ -988,6 +1003,8 @@ int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps, 
       DPMD_TRY(model_forces_impl(s, prec));
       DPMD_TRY(md_kick(s, 0.5 * dt));
     }
+    // per-step centre-of-mass removal over the mobile atoms (deepmd_jax/md.py:786-790)
+    if (p->remove_com) DPMD_TRY(md_remove_com(s));
     if (rec) {
       // frame -> caller order on the device, then to pinned memory on the copy stream
       double* frame = s->d_traj + (size_t)step * NA * 6;
@@ -1160,6 +1177,29 @@ int32_t dpmd_timing_get(dpmd_system_t sys, int32_t which, double* ms, int64_t* c
   timer_harvest(s);
   if (ms) *ms = s->timer.ms[which];
   if (count) *count = s->timer.count[which];
+  return DPMD_OK;
+}
+
+int32_t dpmd_system_set_fixed(dpmd_system_t sys, const int32_t* fixed) {
+  SYS(sys);
+  if (fixed && s->nseg != 1) { set_error("fixed atoms need a system of one segment"); return DPMD_E_UNSUPPORTED; }
+  const int NA = s->NA;
+  std::vector<float> mob(NA, 1.f);
+  int nm = NA;
+  if (fixed) {
+    nm = 0;
+    for (int g = 0; g < NA; ++g) {
+      mob[g] = fixed[s->perm[g]] ? 0.f : 1.f;
+      nm += mob[g] != 0.f;
+    }
+  }
+  if (nm == 0) { set_error("every atom is fixed"); return DPMD_E_ARG; }
+  s->has_fixed = nm < NA;
+  s->n_mobile = nm;
+  DPMD_HIP(hipMemcpyAsync(s->d_mobile, mob.data(), sizeof(float) * NA, hipMemcpyHostToDevice, s->stream));
+  if (s->has_fixed) DPMD_TRY(md_mask_momenta(s));
+  DPMD_HIP(hipStreamSynchronize(s->stream));
+  s->force_valid = false;
   return DPMD_OK;
 }
 

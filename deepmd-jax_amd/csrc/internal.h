@@ -206,6 +206,9 @@ struct System {
   int* d_perm = nullptr;        // (NA) sorted index -> caller index
   double* d_nm = nullptr;       // PIMD normal modes
   double* d_kepart = nullptr;   // kinetic-energy partial sums (nseg, kKeBlocks, 3)
+  float* d_mobile = nullptr;    // (NA) 1 mobile, 0 fixed (fixed_indices)
+  bool has_fixed = false;
+  int n_mobile = 0;
   double* d_npt = nullptr;      // NPT barostat state (64 doubles) and its chunk-entry copy
   bool npt_ready = false;       // NPT state initialised for the current layout
   double* d_box_traj = nullptr; // per-step boxes of the last recorded chunk (steps, nseg, 3)
@@ -237,6 +240,9 @@ int32_t md_npt_baro_half(System* s, const NptArgs& a);
 int32_t md_npt_box(System* s, const NptArgs& a, int phase);
 int32_t md_npt_particles(System* s, double dt, bool move);
 int32_t md_box_record(System* s, double* out);
+int32_t md_mask_forces(System* s);
+int32_t md_mask_momenta(System* s);
+int32_t md_remove_com(System* s);
 int32_t md_reduce_launch(System* s);
 int32_t md_ke_launch(System* s, double* d_out);
 int32_t md_langevin_o(System* s, double dt, double gamma, double kT, uint64_t seed, uint64_t step);

@@ -48,11 +48,12 @@ __global__ void kick_kernel(double* mom, const float* force, size_t n3, double h
 
 // x += dt p / m, fp32 mirror, NaN/Inf test -> error bit 16
 __global__ void drift_kernel(double* pos, const double* mom, const double* mass, float4* p32, const SegInfo* seg,
-                             const int* atom_seg, int total, double dt, int* flags) {
+                             const int* atom_seg, int total, double dt, int* flags, const float* mobile) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= total) return;
   const SegInfo& S = seg[atom_seg[g]];
   const double m = mass[g];
+  if (mobile && mobile[g] == 0.f) dt = 0.0;  // fixed atoms keep their positions
   float v[3];
   bool bad = false;
 #pragma unroll
@@ -195,13 +196,15 @@ __device__ __forceinline__ uint64_t seg_key(uint64_t seed, uint64_t seg_seed) {
 // all images of a small box receive the same kicks.  Per-atom kinetic-energy change ->
 // scratch for the per-segment thermostat work.
 __global__ void langevin_o_kernel(double* mom, const double* mass, const SegInfo* seg, const int* atom_seg,
-                                  int total, double c1, double c2, uint64_t seed, uint64_t step, double* dke_out) {
+                                  int total, double c1, double c2, uint64_t seed, uint64_t step, double* dke_out,
+                                  const float* mobile) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= total) return;
   const SegInfo& S = seg[atom_seg[g]];
   const double m = mass[g];
   double z[4];
   normal4(seg_key(seed, S.seed), S.step + step, (uint32_t)primary_index(S, g - S.base), 0x4c414e47u, z);
+  if (mobile && mobile[g] == 0.f) z[0] = z[1] = z[2] = 0.0;  // no noise on fixed atoms (routine.py:93-94)
   double dke = 0.0;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -347,7 +350,7 @@ int32_t md_kick(System* s, double h) {
 int32_t md_drift(System* s, double dt) {
   const int NA = s->NA;
   hipLaunchKernelGGL(drift_kernel, dim3((NA + 255) / 256), dim3(256), 0, s->stream, s->d_pos, s->d_mom, s->d_mass,
-                     s->d_pos32, s->d_seg, s->d_atom_seg, NA, dt, s->d_flags);
+                     s->d_pos32, s->d_seg, s->d_atom_seg, NA, dt, s->d_flags, s->has_fixed ? s->d_mobile : nullptr);
   DPMD_HIP(hipGetLastError());
   return DPMD_OK;
 }
@@ -386,7 +389,8 @@ int32_t md_langevin_o(System* s, double dt, double gamma, double kT, uint64_t se
   const double c2 = sqrt(std::max(kT * (1.0 - c1 * c1), 0.0));
   const int NA = s->NA;
   hipLaunchKernelGGL(langevin_o_kernel, dim3((NA + 255) / 256), dim3(256), 0, s->stream, s->d_mom, s->d_mass,
-                     s->d_seg, s->d_atom_seg, NA, c1, c2, seed, step, s->d_scratch);
+                     s->d_seg, s->d_atom_seg, NA, c1, c2, seed, step, s->d_scratch,
+                     s->has_fixed ? s->d_mobile : nullptr);
   hipLaunchKernelGGL(seg_accumulate_kernel, dim3(s->nseg), dim3(256), 0, s->stream, s->d_scratch, s->d_seg,
                      s->d_thermo, 0);
   DPMD_HIP(hipGetLastError());
@@ -460,9 +464,10 @@ __device__ double nhc_half_step(double* xi, double* px, const double* Q, int C, 
 // [8..16) xi, [16..24) p_xi, [24..32) Q, [40] momentum scale.  Kinetic energy and degrees
 // of freedom are per periodic image.
 __global__ void nh_half_kernel(double* thermo, const double* ke, const SegInfo* seg, int nseg, int C, double kT,
-                               double dt, double tau) {
+                               double dt, double tau, double dof_mobile) {
   for (int rep = threadIdx.x; rep < nseg; rep += blockDim.x) {
-    const double dof = 3.0 * (seg[rep].natoms / seg[rep].nimg);
+    // degrees of freedom of the mobile atoms (with_fixed_atoms, deepmd_jax/routine.py:110-136)
+    const double dof = dof_mobile > 0 ? dof_mobile : 3.0 * (seg[rep].natoms / seg[rep].nimg);
     double* th = thermo + rep * 64;
     double* xi = th + 8;
     double* px = th + 16;
@@ -493,7 +498,7 @@ int32_t md_nh_half(System* s, double kT, double dt, double tau, int chain) {
   double* d_ke = s->d_red + 16 * s->nseg;
   DPMD_TRY(md_ke_launch(s, d_ke));
   hipLaunchKernelGGL(nh_half_kernel, dim3(1), dim3(256), 0, s->stream, s->d_thermo, d_ke, s->d_seg, s->nseg, chain,
-                     kT, dt, tau);
+                     kT, dt, tau, s->has_fixed ? 3.0 * s->n_mobile / s->seg[0].nimg : 0.0);
   const int n3 = s->NA * 3;
   hipLaunchKernelGGL(nh_scale_kernel, dim3((n3 + 255) / 256), dim3(256), 0, s->stream, s->d_mom, s->d_thermo,
                      s->d_atom_seg, n3);
@@ -684,6 +689,76 @@ int32_t md_npt_particles(System* s, double dt, bool move) {
 int32_t md_box_record(System* s, double* out) {
   hipLaunchKernelGGL(box_record_kernel, dim3((3 * s->nseg + 63) / 64), dim3(64), 0, s->stream, s->d_seg, s->nseg,
                      out);
+  DPMD_HIP(hipGetLastError());
+  return DPMD_OK;
+}
+
+// ------------------------------------------------------------------------------
+// Fixed atoms and centre-of-mass motion (deepmd_jax/md.py:221-234,428-430,744-757,786-790)
+
+// rows of fixed atoms set to zero: forces (float) after each evaluation, momenta (double)
+// after each NVE / Nose-Hoover step (with_fixed_atoms, deepmd_jax/routine.py:130-134)
+__global__ void mask_force_kernel(float* f, const float* mobile, int n3) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n3 && mobile[i / 3] == 0.f) f[i] = 0.f;
+}
+
+__global__ void mask_mom_kernel(double* p, const float* mobile, int n3) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n3 && mobile[i / 3] == 0.f) p[i] = 0.0;
+}
+
+// pass 1: per block, sum of mobile momenta (3) and mobile mass over a strided share of all atoms
+__global__ void com_part_kernel(const double* mom, const double* mass, const float* mobile, int natoms, int KB,
+                                double* part) {
+  __shared__ double red[256];
+  double k[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < natoms; i += KB * blockDim.x) {
+    const double w = mobile ? (double)mobile[i] : 1.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) k[d] += w * mom[(size_t)i * 3 + d];
+    k[3] += w * mass[i];
+  }
+  for (int d = 0; d < 4; ++d) {
+    const double v = block_sum(k[d], red);
+    if (threadIdx.x == 0) part[blockIdx.x * 4 + d] = v;
+  }
+}
+
+// pass 2: every thread adds the partials in the same order and removes m v_com from its atom
+__global__ void com_apply_kernel(double* mom, const double* mass, const double* part, int KB, int natoms) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= natoms) return;
+  double k[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int b = 0; b < KB; ++b)
+    for (int d = 0; d < 4; ++d) k[d] += part[b * 4 + d];
+  const double m = mass[i];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) mom[(size_t)i * 3 + d] -= m * k[d] / k[3];
+}
+
+int32_t md_mask_forces(System* s) {
+  const int n3 = s->NA * 3;
+  hipLaunchKernelGGL(mask_force_kernel, dim3((n3 + 255) / 256), dim3(256), 0, s->stream, s->d_force, s->d_mobile, n3);
+  DPMD_HIP(hipGetLastError());
+  return DPMD_OK;
+}
+
+int32_t md_mask_momenta(System* s) {
+  const int n3 = s->NA * 3;
+  hipLaunchKernelGGL(mask_mom_kernel, dim3((n3 + 255) / 256), dim3(256), 0, s->stream, s->d_mom, s->d_mobile, n3);
+  DPMD_HIP(hipGetLastError());
+  return DPMD_OK;
+}
+
+// v -= sum(v m_mobile) / sum(m_mobile) for every atom of the system (all beads together)
+int32_t md_remove_com(System* s) {
+  const int NA = s->NA;
+  const int KB = std::max(1, std::min(kKeBlocks, (NA + 2047) / 2048));
+  hipLaunchKernelGGL(com_part_kernel, dim3(KB), dim3(256), 0, s->stream, s->d_mom, s->d_mass,
+                     s->has_fixed ? s->d_mobile : nullptr, NA, KB, s->d_kepart);
+  hipLaunchKernelGGL(com_apply_kernel, dim3((NA + 255) / 256), dim3(256), 0, s->stream, s->d_mom, s->d_mass,
+                     s->d_kepart, KB, NA);
   DPMD_HIP(hipGetLastError());
   return DPMD_OK;
 }

@@ -32,7 +32,7 @@ class MDParams(ctypes.Structure):
     _fields_ = [('routine', c_i32), ('precision', c_i32), ('dt', c_f64), ('kT', c_f64), ('gamma', c_f64),
                 ('tau', c_f64), ('chain_length', c_i32), ('record_every', c_i32), ('seed', ctypes.c_uint64),
                 ('pimd_lambda', c_f64), ('pressure', c_f64), ('tau_p', c_f64), ('chain_length_p', c_i32),
-                ('couple', c_i32 * 3)]
+                ('couple', c_i32 * 3), ('remove_com', c_i32)]
 
 
 class DPMDError(RuntimeError):
@@ -89,6 +89,7 @@ def lib():
         'dpmd_md_box_trajectory': (c_i32, [c_vp, c_vp]),
         'dpmd_npt_state_get': (c_i32, [c_vp, c_vp]),
         'dpmd_nlist_rebuilds': (c_i32, [c_vp, c_vp]),
+        'dpmd_system_set_fixed': (c_i32, [c_vp, c_vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -124,7 +125,7 @@ def exported_symbols():
             'dpmd_md_run', 'dpmd_pimd_setup', 'dpmd_kinetic_energy', 'dpmd_thermostat_energy',
             'dpmd_thermostat_state_get', 'dpmd_thermostat_state_set', 'dpmd_timing_enable',
             'dpmd_timing_get', 'dpmd_pair_count', 'dpmd_synchronize', 'dpmd_system_get_boxes',
-            'dpmd_md_box_trajectory', 'dpmd_npt_state_get', 'dpmd_nlist_rebuilds']
+            'dpmd_md_box_trajectory', 'dpmd_npt_state_get', 'dpmd_nlist_rebuilds', 'dpmd_system_set_fixed']
 
 
 class DeviceModel:
@@ -378,6 +379,11 @@ class DeviceSystem:
         b = np.zeros((nsteps, self.nrep, 3))
         check(lib().dpmd_md_box_trajectory(self.handle, ptr(b)))
         return b
+
+    def set_fixed(self, fixed):
+        """Fixed-atom mask in caller order (flat over all atoms of all replicas), or None."""
+        f = None if fixed is None else np.ascontiguousarray(np.asarray(fixed).reshape(-1), np.int32)
+        check(lib().dpmd_system_set_fixed(self.handle, ptr(f)))
 
     def nlist_rebuilds(self):
         """Drift-triggered neighbour-list rebuilds inside md_run since the last allocation."""

@@ -11,7 +11,9 @@ Routines: 'NVE' (velocity Verlet), 'NVT' (Nose-Hoover chain; with n_bead > 1
 ring-polymer PILE-L), 'NVT_langevin' (BAOAB), 'NPT' (MTK Nose-Hoover barostat over
 couple_axes groups with a Nose-Hoover chain thermostat, deepmd_jax/routine.py:161-456,
 one system in the neighbour-list regime).  Positions and momenta are FP64 on the device.
-Not provided: path-integral NPT, fixed atoms, model deviation, XYZ dumps, DPLR.
+Fixed atoms (fixed_indices) and per-step centre-of-mass removal (remove_com_motion) run
+on the device (deepmd_jax/md.py:221-234,786-790, deepmd_jax/routine.py:110-136).
+Not provided: path-integral NPT, model deviation, XYZ dumps, DPLR.
 """
 import time
 
@@ -97,6 +99,9 @@ class _ImageSystem:
     def set_masses(self, mass):
         self.dev.set_masses(np.tile(np.asarray(mass, np.float64), self.nimg))
 
+    def set_fixed(self, fixed):
+        self.dev.set_fixed(None if fixed is None else np.tile(np.asarray(fixed, np.int32), self.nimg))
+
     def compute(self, precision):
         e, f, v, ea = self.dev.compute(precision)
         return e, f[:, :self.N], v, ea
@@ -155,12 +160,16 @@ class Simulation:
             raise ValueError('pimd_lambda must be non-negative.')
         if routine not in ('NVE', 'NVT', 'NVT_langevin', 'NPT'):
             raise NotImplementedError("routine currently limited to 'NVE', 'NVT', 'NVT_langevin', 'NPT'")
-        if fixed_indices is not None:
-            raise NotImplementedError('fixed_indices is not provided by this engine.')
+        if fixed_indices is not None and routine not in ('NVT', 'NVE', 'NVT_langevin'):
+            # deepmd_jax/md.py:228-229
+            raise NotImplementedError('Fixing atoms currently limited to NVE, NVT, and NVT_langevin routines.')
+        if fixed_indices is not None and self._is_pimd:
+            raise NotImplementedError('fixed atoms with path-integral MD are not provided by this engine.')
         if len(model_deviation_paths) > 0:
             raise NotImplementedError('model_deviation_paths is not provided by this engine.')
-        if remove_com_motion:
-            raise NotImplementedError('remove_com_motion is not provided by this engine.')
+        if remove_com_motion and self._is_pimd:
+            raise NotImplementedError('per-step remove_com_motion with path-integral MD is not provided by this engine.')
+        self._remove_com_motion = bool(remove_com_motion)
         if not (isinstance(couple_axes, tuple) and couple_axes == ((0, 1, 2),)) and routine != 'NPT':
             raise ValueError("couple_axes is only meaningful for routine='NPT'")
         if chain_steps_t != 1 or sy_steps_t != 1 or chain_steps_p != 1 or sy_steps_p != 1:
@@ -247,6 +256,16 @@ class Simulation:
             self._neighbor_skin = max(0.0, 0.5 * self._box.min() * (1 - 1e-9) - self._rcut)
         self._sys = _ImageSystem(self._device_model, self._type_idx, self._n_bead, self._box, k)
         self._sys.set_masses(self._mass)
+        # mobile-atom mask (deepmd_jax/md.py:220-226); the device zeroes the forces on fixed
+        # atoms, keeps their positions and masks their momenta after each NVE / NVT step
+        self._mobile = None
+        if fixed_indices is not None:
+            mobile = np.ones(self._natoms, bool)
+            mobile[np.asarray(fixed_indices)] = False
+            if not mobile.any():
+                raise ValueError('fixed_indices fixes every atom.')
+            self._mobile = mobile
+            self._sys.set_fixed(~mobile)
         self._sys.set_positions(pos)
         self._allocate_nlist()
         self._kT = (temperature if temperature is not None else 0.0) * TEMP_UNIT_CONVERSION
@@ -259,6 +278,7 @@ class Simulation:
         self._params.chain_length = int(chain_length_t)
         self._params.gamma = 1.0 / float(tau_t)
         self._params.pimd_lambda = self._pimd_lambda
+        self._params.remove_com = int(self._remove_com_motion)
         if routine == 'NPT':
             # deepmd_jax/md.py:357-366: pressure in eV/A^3, barostat chain of tau_p
             self._params.pressure = self._pressure * PRESS_UNIT_CONVERSION
@@ -283,6 +303,8 @@ class Simulation:
             p = rng.standard_normal((self._n_bead, self._natoms, 3)) * np.sqrt(self._mass * kTP)[None, :, None]
             if not self._is_pimd:
                 p -= p.mean(1, keepdims=True)
+            if self._mobile is not None:
+                p *= self._mobile[None, :, None]
             self._sys.set_velocities(p / self._mass[None, :, None])
         self._is_initial_state = True
         self._error_code = 0
@@ -318,7 +340,9 @@ class Simulation:
         pe = float(e.mean()) if self._is_pimd else float(e[0])
         ke_tot = float(ke.sum())
         N = self._natoms
-        temp = 2 * ke_tot / (3 * N * P) / TEMP_UNIT_CONVERSION / P
+        # temperature over the mobile degrees of freedom (deepmd_jax/md.py:478)
+        dof_scale = N / int(self._mobile.sum()) if self._mobile is not None else 1
+        temp = 2 * ke_tot / (3 * N * P) / TEMP_UNIT_CONVERSION / P * dof_scale
         rep = {'Temperature': temp, 'KE': ke_tot / P ** 2, 'PE': pe}
         if self._routine == 'NVE':
             rep['Invariant'] = ke_tot + pe
@@ -451,16 +475,26 @@ class Simulation:
         v = np.asarray(velocity, np.float64)
         if self._is_pimd and v.ndim == 2:
             v = np.broadcast_to(v, (self._n_bead,) + v.shape)
+        if self._mobile is not None:
+            v = v * self._mobile[:, None]
         self._sys.set_velocities(v.reshape(self._n_bead, self._natoms, 3))
+
+    def remove_com_motion(self):
+        """Remove the centre-of-mass velocity (deepmd_jax/md.py:743-757): over all beads and atoms
+        under PIMD, else over the mobile atoms, subtracted from every atom."""
+        v = self._sys.get_velocities()
+        w = np.broadcast_to(self._mass[None, :, None], v.shape[:2] + (1,))
+        if self._mobile is not None:
+            w = w * self._mobile[None, :, None]
+        self._sys.set_velocities(v - (v * w).sum((0, 1)) / w.sum())
 
     def setRandomVelocity(self, temperature, remove_com_motion=True, seed=None):
         rng = np.random.default_rng(seed)
         kT = temperature * TEMP_UNIT_CONVERSION
         v = rng.standard_normal((self._n_bead, self._natoms, 3)) * np.sqrt(kT * self._n_bead / self._mass)[None, :, None]
+        self.setVelocity(v if self._is_pimd else v[0])
         if remove_com_motion:
-            m = self._mass[None, :, None]
-            v = v - (v * m).sum((0, 1)) / (m.sum() * self._n_bead)
-        self._sys.set_velocities(v)
+            self.remove_com_motion()
 
     def getBox(self):
         if self._routine == 'NPT':

@@ -98,6 +98,8 @@ typedef struct {
   int32_t chain_length_p; /* barostat chain length */
   int32_t couple[3];    /* coupled group of each axis (0, 1, 2; groups numbered from 0 in
                            first-appearance order) or -1 for a fixed axis (couple_axes) */
+  int32_t remove_com;   /* nonzero: remove the centre-of-mass velocity of the mobile atoms after
+                           every step (Simulation(remove_com_motion=True), deepmd_jax/md.py:786-790) */
 } dpmd_md_params;
 
 const char* dpmd_last_error(void);
@@ -151,6 +153,12 @@ int32_t dpmd_system_get_velocities(dpmd_system_t sys, double* vel);
 /* Per-atom masses (sum seg_natoms) in eV fs^2 / Angstrom^2, caller order.  Set masses
  * before velocities (momenta are stored). */
 int32_t dpmd_system_set_masses(dpmd_system_t sys, const double* mass);
+/* Fixed atoms (fixed_indices of deepmd_jax/md.py:221-234): fixed (sum seg_natoms) caller order,
+ * nonzero = fixed, NULL = none.  Fixed atoms get zero force (their coordinates carry no
+ * gradient, :428-430), keep their positions, have their momenta zeroed after each NVE / NVT
+ * step and receive no Langevin noise; the Nose-Hoover chain counts 3 x mobile degrees of
+ * freedom (deepmd_jax/routine.py:110-136).  NVE, NVT and NVT_langevin only. */
+int32_t dpmd_system_set_fixed(dpmd_system_t sys, const int32_t* fixed);
 
 /* Typed neighbour list at cutoff rc_skin.  Replaces typed_neighbor_list's allocate_fn
  * (deepmd_jax/md_utils.py:376-421): measures the per-type maximum neighbour count,

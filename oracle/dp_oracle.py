@@ -26,6 +26,12 @@ Function -> reference file:line it follows:
   NPTOracle             deepmd_jax/routine.py:161-456 (MTK barostat, coupled axes);
   nhc_half_step         jax_md simulate.nose_hoover_chain as wrapped there (parity
                         unpinned: jax_md is absent; pinned by invariant conservation)
+  vv_step(mobile=)      deepmd_jax/routine.py:110-144 (with_fixed_atoms around jax_md
+                        nve), deepmd_jax/md.py:231-233 (fixed positions)
+  nvt_nh_step           deepmd_jax/routine.py:110-136,147-158 (jax_md nvt_nose_hoover,
+                        chain masses from the mobile degrees of freedom)
+  remove_com            deepmd_jax/md.py:786-790
+  energy_force_virial(mobile=) deepmd_jax/md.py:428-430 (stop-gradient on fixed atoms)
 Pinning (tests/test_oracle.py): closed forms of the switch and of tensor_3to6,
 rotation / translation / permutation invariance, the T3=T6=0 outer-product
 reduction of G, all-zero-weight nets, autograd against central finite
@@ -244,19 +250,24 @@ def dp_energy(params, variables, coord, box3, type_idx, nbrs_nm=None, lattice=No
 
 
 def energy_force_virial(params, variables, coord, box3, type_idx, nbrs_nm=None, lattice=None,
-                        dtype=torch.float64):
+                        dtype=torch.float64, mobile=None):
     """(E, F (N,3) caller order, virial (3,3)) with forces from autograd.
 
     virial = -dE/d(strain) under x -> (1+eps) x, box -> (1+eps) box, as the
-    full tensor (deepmd sign: virial = sum_pairs x_ij (x) f_ij)."""
-    coord = torch.as_tensor(np.asarray(coord, np.float64), dtype=dtype).clone().requires_grad_(True)
+    full tensor (deepmd sign: virial = sum_pairs x_ij (x) f_ij).  mobile (N,) bool:
+    the coordinates of the other atoms are cut from the graph before the strain is
+    applied, as the reference does for fixed atoms (deepmd_jax/md.py:428-432)."""
+    leaf = torch.as_tensor(np.asarray(coord, np.float64), dtype=dtype).clone().requires_grad_(True)
+    coord = leaf
+    if mobile is not None:
+        coord = torch.where(torch.as_tensor(np.asarray(mobile, bool))[:, None], leaf, leaf.detach())
     box3 = np.asarray(box3, np.float64).reshape(-1)[:3]
     eps = torch.zeros(3, 3, dtype=dtype, requires_grad=True)
     strain = torch.eye(3, dtype=dtype) + eps
     if isinstance(lattice, str) and lattice == 'auto':
         lattice = lattice_candidate(np.diag(box3), params['rcut'])
     E = dp_energy(params, variables, coord, box3, type_idx, nbrs_nm=nbrs_nm, lattice=lattice, strain=strain)
-    gx, ge = torch.autograd.grad(E, [coord, eps])
+    gx, ge = torch.autograd.grad(E, [leaf, eps])
     return E.detach(), -gx.detach(), -ge.detach().T
 
 
@@ -342,12 +353,52 @@ def typed_nlist(coord, box, type_idx, rc_skin, ntypes=2, knbr=None, buffer_ratio
 # ring-polymer step with exact free-ring propagation and PILE-L O-step), :210-227
 # (unwrap across beads); jax_md velocity Verlet (deepmd_jax/routine.py:139-144).
 
-def vv_step(x, p, m, force_fn, dt, F):
+def vv_step(x, p, m, force_fn, dt, F, mobile=None):
+    """Velocity Verlet; with mobile (N,) bool the fixed atoms keep their positions and their
+    momenta are zeroed after the step (force_fn then returns zero force on them)."""
     p = p + 0.5 * dt * F
-    x = x + dt * p / m[..., None]
+    dx = dt * p / m[..., None]
+    x = x + (dx if mobile is None else dx * mobile[:, None])
     F = force_fn(x)
     p = p + 0.5 * dt * F
+    if mobile is not None:
+        p = p * mobile[:, None]
     return x, p, F
+
+
+def nvt_nh_step(x, p, m, force_fn, dt, F, xi, px, kT, tau, mobile=None):
+    """One Nose-Hoover chain NVT step (chain half step, velocity Verlet, chain half step) with
+    chain masses Q0 = dof kT tau^2, Q_k = kT tau^2 over dof = 3 x (mobile atoms).  The chain
+    (xi, px) is advanced in place; returns (x, p, F)."""
+    dof = 3 * (len(m) if mobile is None else int(np.sum(mobile)))
+    Q = np.full(len(xi), kT * tau ** 2)
+    Q[0] *= dof
+    ke = lambda q: 0.5 * (q ** 2 / m[:, None]).sum()
+    p = p * nhc_half_step(xi, px, Q, ke(p), dof, kT, dt)
+    p = p + 0.5 * dt * F
+    dx = dt * p / m[:, None]
+    x = x + (dx if mobile is None else dx * mobile[:, None])
+    F = force_fn(x)
+    p = p + 0.5 * dt * F
+    p = p * nhc_half_step(xi, px, Q, ke(p), dof, kT, dt)
+    if mobile is not None:
+        p = p * mobile[:, None]
+    return x, p, F
+
+
+def nh_chain_energy(xi, px, kT, tau, dof):
+    """Chain part of the Nose-Hoover invariant: sum px^2 / 2Q + dof kT xi_0 + kT sum_k>0 xi_k."""
+    Q = np.full(len(xi), kT * tau ** 2)
+    Q[0] *= dof
+    return float((px ** 2 / (2 * Q)).sum() + dof * kT * xi[0] + kT * xi[1:].sum())
+
+
+def remove_com(p, m, mobile=None):
+    """Subtract the mobile-mass-weighted centre-of-mass velocity from every atom (momenta in,
+    momenta out)."""
+    v = p / m[:, None]
+    w = m if mobile is None else m * mobile
+    return (v - (v * w[:, None]).sum(0) / w.sum()) * m[:, None]
 
 
 def ring_modes(n_bead, kT, hbar=6.582119569e-1):
