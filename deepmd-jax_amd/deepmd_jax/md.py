@@ -13,7 +13,8 @@ couple_axes groups with a Nose-Hoover chain thermostat, deepmd_jax/routine.py:16
 one system in the neighbour-list regime).  Positions and momenta are FP64 on the device.
 Fixed atoms (fixed_indices) and per-step centre-of-mass removal (remove_com_motion) run
 on the device (deepmd_jax/md.py:221-234,786-790, deepmd_jax/routine.py:110-136).
-Not provided: path-integral NPT, model deviation, XYZ dumps, DPLR.
+XYZ dumps (run(dump_prefix=...)) stream from a writer thread (deepmd_jax/dumps.py).
+Not provided: path-integral NPT, model deviation, DPLR.
 """
 import time
 
@@ -21,6 +22,7 @@ import numpy as np
 
 from . import _lib
 from .constants import MASS_UNIT_CONVERSION, TEMP_UNIT_CONVERSION, PRESS_UNIT_CONVERSION
+from .dumps import CHEMICAL_SYMBOLS, DumpWriter, prepare_dumps, target_label
 from .evaluator import canonical_box
 from .pimd import normal_mode_transform, pile_l_factors, centroid
 from .utils import load_model
@@ -230,7 +232,13 @@ class Simulation:
         self._type_idx = type_idx_np
         self._mass_amu = np.asarray(mass, np.float64)[self._type_idx]
         self._mass = self._mass_amu * MASS_UNIT_CONVERSION
-        self._type_symbols = type_symbols
+        # symbols for XYZ dumps: given, or from the model's atomic numbers (deepmd_jax/md.py:186-194)
+        if type_symbols is None and ct is not None:
+            type_symbols = [CHEMICAL_SYMBOLS[int(z)] for z in ct]
+        if type_symbols is not None and len(type_symbols) < int(self._type_idx.max()) + 1:
+            raise ValueError('type_symbols has length %d but type_idx references types up to %d.'
+                             % (len(type_symbols), int(self._type_idx.max())))
+        self._type_symbols = list(type_symbols) if type_symbols is not None else None
         self._debug = debug
         self._step_chunk_size = max(10, min(100, 100000 // (self._natoms * self._n_bead)))
         self._neighbor_skin = (0.5 if routine == 'NPT' else 0.3) if neighbor_skin is None else float(neighbor_skin)
@@ -373,22 +381,31 @@ class Simulation:
         print('%-8d ' % row[0] + ' '.join('%-12.3f' % x for x in row[1:-1]) + ' %-6.3f' % row[-1])
         self._tic_between_report = now
 
-    def _initialize_run(self, steps):
-        print('# Running %d steps...' % steps)
+    def _initialize_run(self, steps, writer=None, label=None):
+        print('# Running %d steps%s...' % (steps, (' and dumping to %s' % label) if writer else ''))
         traj_length = steps + int(self._is_initial_state)
         self._offset = self.step - int(self._is_initial_state)
         shape = (self._n_bead, self._natoms, 3) if self._is_pimd else (self._natoms, 3)
-        self._position_trajectory = np.zeros((traj_length,) + shape)
-        self._velocity_trajectory = np.zeros((traj_length,) + shape)
-        self._box_trajectory = np.zeros((traj_length, 3))
-        if self._is_pimd:
-            self._centroid_trajectory = np.zeros((traj_length, self._natoms, 3))
-        if self._is_initial_state:
-            self._position_trajectory[0] = self.getPosition()
-            self._velocity_trajectory[0] = self.getVelocity()
-            self._box_trajectory[0] = self.getBox()
+        if writer is None:
+            self._position_trajectory = np.zeros((traj_length,) + shape)
+            self._velocity_trajectory = np.zeros((traj_length,) + shape)
+            self._box_trajectory = np.zeros((traj_length, 3))
             if self._is_pimd:
-                self._centroid_trajectory[0] = self.getCentroidPosition()
+                self._centroid_trajectory = np.zeros((traj_length, self._natoms, 3))
+            if self._is_initial_state:
+                self._position_trajectory[0] = self.getPosition()
+                self._velocity_trajectory[0] = self.getVelocity()
+                self._box_trajectory[0] = self.getBox()
+                if self._is_pimd:
+                    self._centroid_trajectory[0] = self.getCentroidPosition()
+        else:
+            # frames stream to the dump files; the device writes each chunk into one reused buffer
+            n = self._step_chunk_size
+            self._chunk_pos = np.empty((n, self._n_bead, self._natoms, 3))
+            self._chunk_vel = np.empty((n, self._n_bead, self._natoms, 3))
+            if self._is_initial_state and self.step % self._dump_interval == 0:
+                self._put_frame(writer, self._sys.get_positions(), self._sys.get_velocities(), self.getBox(),
+                                self.step)
         self._tic_of_this_run = time.time()
         self._tic_between_report = time.time()
         self._error_code = 0
@@ -403,18 +420,43 @@ class Simulation:
         print('# Successfully resolved overflow code', self._error_code)
         self._error_code = 4
 
+    def _put_frame(self, writer, pos, vel, box, step):
+        """Queue one frame ((P,N,3) positions and velocities) for the dump files, as float32
+        copies (the reference's default frame dtype)."""
+        kinds = {d[0] for d in writer.dumps}
+        p32 = np.array(pos if self._is_pimd else pos[0], np.float32) if 'position' in kinds else None
+        v32 = np.array(vel if self._is_pimd else vel[0], np.float32) if 'velocity' in kinds else None
+        c32 = np.array(centroid(pos, box), np.float32) if 'centroid' in kinds else None
+        writer.put(p32, v32, c32, np.array(box, np.float32), step)
+
     def run(self, steps, dump_interval=None, dump_prefix=None, dump_content='positions', dump_mode='overwrite'):
-        if dump_prefix is not None:
-            raise NotImplementedError('XYZ dumps are not provided by this engine.')
-        self._initialize_run(steps)
+        """Run `steps` steps; returns the trajectory dict, or None when frames stream to XYZ files
+        (deepmd_jax/md.py:874-964)."""
+        dumps, symbols, self._dump_interval = prepare_dumps(self._type_idx, self._type_symbols, self._n_bead,
+                                                            self.report_interval, dump_interval, dump_prefix,
+                                                            dump_content, dump_mode)
+        writer = DumpWriter(dumps, symbols) if dumps else None
+        try:
+            return self._run(steps, writer, target_label(dumps, dump_prefix) if dumps else None)
+        finally:
+            if writer is not None:
+                writer.close()
+                self._chunk_pos = self._chunk_vel = None
+
+    def _run(self, steps, writer, label):
+        self._initialize_run(steps, writer, label)
         remaining = steps
         while remaining > 0:
             nxt = min(self.report_interval - self.step % self.report_interval, self._step_chunk_size, remaining)
             lo = self.step - self._offset
-            # the device writes the chunk's frames straight into the trajectory arrays
+            # the device writes the chunk's frames straight into the trajectory arrays (or the
+            # reused chunk buffer when streaming to dump files)
             shp = (nxt, self._n_bead, self._natoms, 3)
-            out = (self._position_trajectory[lo:lo + nxt].reshape(shp),
-                   self._velocity_trajectory[lo:lo + nxt].reshape(shp))
+            if writer is None:
+                out = (self._position_trajectory[lo:lo + nxt].reshape(shp),
+                       self._velocity_trajectory[lo:lo + nxt].reshape(shp))
+            else:
+                out = (self._chunk_pos[:nxt], self._chunk_vel[:nxt])
             tp, tv, err = self._sys.md_run(self._params, nxt, record=True, out=out)
             if err & 16:
                 print('# Warning: Nan or Inf encountered in simulation. Terminating.')
@@ -426,18 +468,24 @@ class Simulation:
                 continue
             if not err & 16:
                 self._error_code &= ~4
-            if self._is_pimd:
+            boxes = self._sys.dev.box_trajectory(nxt)[:, 0] if self._routine == 'NPT' else None
+            if writer is not None:
                 for k in range(nxt):
-                    self._centroid_trajectory[lo + k] = centroid(tp[k], self._box)
-            if self._routine == 'NPT':
-                self._box_trajectory[lo:lo + nxt] = self._sys.dev.box_trajectory(nxt)[:, 0]
+                    if (self.step + k + 1) % self._dump_interval == 0:
+                        self._put_frame(writer, tp[k], tv[k], self._box if boxes is None else boxes[k],
+                                        self.step + k + 1)
             else:
-                self._box_trajectory[lo:lo + nxt] = self._box
+                if self._is_pimd:
+                    for k in range(nxt):
+                        self._centroid_trajectory[lo + k] = centroid(tp[k], self._box)
+                self._box_trajectory[lo:lo + nxt] = self._box if boxes is None else boxes
             self.step += nxt
             remaining -= nxt
             if self.step % self.report_interval == 0 or remaining == 0:
                 self._print_report()
         print_run_profile(steps, time.time() - self._tic_of_this_run, self._natoms * self._n_bead, self._dt)
+        if writer is not None:
+            return None
         traj = {'position': self._position_trajectory, 'velocity': self._velocity_trajectory,
                 'box': self._box_trajectory}
         if self._is_pimd:
