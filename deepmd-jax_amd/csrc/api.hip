@@ -830,11 +830,20 @@ int32_t dpmd_compute(dpmd_system_t sys, int32_t precision, double* energy, doubl
 
 // NPT parameters from dpmd_md_params; couple_axes rules of deepmd_jax/routine.py:196-223
 static int32_t npt_args(System* s, const dpmd_md_params* p, NptArgs& a) {
-  if (s->nseg != 1 || s->seg[0].nimg != 1) {
+  const bool pimd = p->routine == DPMD_PIMD_NPT;
+  if (!pimd && (s->nseg != 1 || s->seg[0].nimg != 1)) {
     set_error("NPT runs one system held without periodic images");
     return DPMD_E_UNSUPPORTED;
   }
-  if (p->chain_length_p < 1 || p->chain_length_p > 8) {
+  if (pimd) {
+    if (s->nseg < 2 || !s->uniform || !s->d_nm) {
+      set_error("path-integral NPT needs identical beads and dpmd_pimd_setup");
+      return DPMD_E_STATE;
+    }
+    for (const SegInfo& S : s->seg)
+      if (S.nimg != 1) { set_error("path-integral NPT runs beads held without periodic images"); return DPMD_E_UNSUPPORTED; }
+  }
+  if (!pimd && (p->chain_length_p < 1 || p->chain_length_p > 8)) {
     set_error("barostat chain length %d outside [1, 8]", p->chain_length_p);
     return DPMD_E_ARG;
   }
@@ -863,18 +872,19 @@ static int32_t npt_args(System* s, const dpmd_md_params* p, NptArgs& a) {
   a.tau_p = p->tau_p;
   a.rc = s->model->rcut;
   a.rc_skin = s->rc_skin;
-  a.natoms = s->NA;
+  a.natoms = s->seg[0].natoms;
   return DPMD_OK;
 }
 
-// drift threshold with the NPT safety factor on the host copy of the segment
+// drift threshold with the NPT safety factor on the host copy of the segments
 static void npt_threshold(System* s, const NptArgs& a) {
-  SegInfo& S = s->seg[0];
-  float sc = 0.f;
-  for (int d = 0; d < 3; ++d) sc = std::max(sc, S.scale[d]);
-  const float allowed = ((float)a.rc_skin - (float)a.rc * (float)((double)sc * 1.02)) / 2.f;
-  const double lmin = std::min((double)S.L[0], std::min((double)S.L[1], (double)S.L[2]));
-  S.thr = allowed - (float)(lmin * 1e-6);
+  for (SegInfo& S : s->seg) {
+    float sc = 0.f;
+    for (int d = 0; d < 3; ++d) sc = std::max(sc, S.scale[d]);
+    const float allowed = ((float)a.rc_skin - (float)a.rc * (float)((double)sc * 1.02)) / 2.f;
+    const double lmin = std::min((double)S.L[0], std::min((double)S.L[1], (double)S.L[2]));
+    S.thr = allowed - (float)(lmin * 1e-6);
+  }
   s->seg_dirty = true;
 }
 
@@ -885,24 +895,25 @@ int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps, 
   if (!s->nl_ready) { set_error("no neighbour list: call dpmd_nlist_allocate first"); return DPMD_E_STATE; }
   if (nsteps < 0) { set_error("nsteps must be >= 0"); return DPMD_E_ARG; }
   if (p->routine != DPMD_NVE && p->routine != DPMD_NVT_LANGEVIN && p->routine != DPMD_NVT_NH &&
-      p->routine != DPMD_PIMD_PILE_L && p->routine != DPMD_NPT_NH) {
+      p->routine != DPMD_PIMD_PILE_L && p->routine != DPMD_NPT_NH && p->routine != DPMD_PIMD_NPT) {
     set_error("unknown routine %d", p->routine);
     return DPMD_E_ARG;
   }
-  if (s->has_fixed && (p->routine == DPMD_PIMD_PILE_L || p->routine == DPMD_NPT_NH)) {
+  if (s->has_fixed && p->routine != DPMD_NVE && p->routine != DPMD_NVT_NH && p->routine != DPMD_NVT_LANGEVIN) {
     set_error("fixed atoms are supported with NVE, NVT and NVT_langevin (deepmd_jax/md.py:228-229)");
     return DPMD_E_UNSUPPORTED;
   }
   const bool npt = p->routine == DPMD_NPT_NH;
+  const bool pnpt = p->routine == DPMD_PIMD_NPT;
   NptArgs na;
   memset(&na, 0, sizeof(na));
-  if (npt) DPMD_TRY(npt_args(s, p, na));
+  if (npt || pnpt) DPMD_TRY(npt_args(s, p, na));
   if (p->precision != DPMD_PREC_FP32 && p->precision != DPMD_PREC_BF16 && p->precision != DPMD_PREC_FP16) {
     set_error("unknown precision %d", p->precision);
     return DPMD_E_ARG;
   }
   if (p->routine == DPMD_PIMD_PILE_L && !s->d_nm) { set_error("PIMD normal modes not set"); return DPMD_E_STATE; }
-  if (p->remove_com && (p->routine == DPMD_PIMD_PILE_L || s->nseg != 1)) {
+  if (p->remove_com && (p->routine == DPMD_PIMD_PILE_L || pnpt || s->nseg != 1)) {
     set_error("per-step centre-of-mass removal runs one system without beads");
     return DPMD_E_UNSUPPORTED;
   }
@@ -925,7 +936,7 @@ int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps, 
   std::vector<double> thermo_save((size_t)64 * s->nseg);
   DPMD_HIP(hipMemcpyAsync(thermo_save.data(), s->d_thermo, sizeof(double) * thermo_save.size(),
                           hipMemcpyDeviceToHost, s->stream));
-  if (npt) {
+  if (npt || pnpt) {
     // the drift threshold carries the NPT safety factor from the first step on
     npt_threshold(s, na);
     DPMD_HIP(hipMemcpyAsync(s->d_npt + 64, s->d_npt, sizeof(double) * 64, hipMemcpyDeviceToDevice, s->stream));
@@ -939,10 +950,19 @@ int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps, 
     DPMD_TRY(md_write_pos32(s));
     DPMD_TRY(model_forces_impl(s, prec));
   }
+  if ((npt || pnpt) && s->npt_ready && s->npt_kind != p->routine) s->npt_ready = false;
   if (npt && (!s->npt_ready || fresh_forces)) {
     // init_fn (deepmd_jax/routine.py:344-378), or dU/deps of recomputed forces
     DPMD_TRY(md_npt_init(s, na, s->npt_ready));
     s->npt_ready = true;
+    s->npt_kind = DPMD_NPT_NH;
+  }
+  if (pnpt && (!s->npt_ready || fresh_forces)) {
+    // init_fn (deepmd_jax/pimd.py:402-428): box at rest, then the centroid-virial dU/deps
+    if (!s->npt_ready) DPMD_TRY(md_pnpt_init(s, na));
+    DPMD_TRY(md_pnpt_dudv(s, na));
+    s->npt_ready = true;
+    s->npt_kind = DPMD_PIMD_NPT;
   }
   if (rec) {
     DPMD_TRY(sys_ensure(s, (void**)&s->d_box_traj, sizeof(double) * 3 * (size_t)s->nseg * std::max(nsteps, 1)));
@@ -989,6 +1009,25 @@ int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps, 
       DPMD_TRY(md_npt_box(s, na, 1));
       DPMD_TRY(md_nh_half(s, p->kT, dt, p->tau, p->chain_length));
       DPMD_TRY(md_npt_baro_half(s, na));
+    } else if (pnpt) {
+      // step_fn of deepmd_jax/pimd.py:430-516
+      const int P = s->nseg;
+      const double* C = s->d_nm;
+      const double* fr = s->d_nm + P * P;
+      const double* c1 = fr + P;
+      const double* c2 = c1 + P;
+      DPMD_TRY(md_pnpt_kick(s, na));
+      DPMD_TRY(md_pnpt_il2(s, na));
+      DPMD_TRY(md_pnpt_move(s, na, 0.5 * dt));
+      DPMD_TRY(md_pnpt_drift_ring(s, na, C, fr, 0.5 * dt));
+      DPMD_TRY(md_pimd_o(s, C, c1, c2, p->seed, counter0 + step));
+      DPMD_TRY(md_pnpt_box_o(s, na, p->gamma, p->seed, counter0 + step));
+      DPMD_TRY(md_pnpt_move(s, na, 0.5 * dt));
+      DPMD_TRY(md_pnpt_drift_ring(s, na, C, fr, 0.5 * dt));
+      DPMD_TRY(model_forces_impl(s, prec));
+      DPMD_TRY(md_pnpt_dudv(s, na));
+      DPMD_TRY(md_pnpt_il2(s, na));
+      DPMD_TRY(md_pnpt_kick(s, na));
     } else {
       const int P = s->nseg;
       const double* C = s->d_nm;
@@ -1048,7 +1087,7 @@ int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps, 
     DPMD_HIP(hipMemcpyAsync(s->d_mom, s->d_mom_save, sizeof(double) * NA * 3, hipMemcpyDeviceToDevice, s->stream));
     DPMD_HIP(hipMemcpyAsync(s->d_thermo, thermo_save.data(), sizeof(double) * thermo_save.size(),
                             hipMemcpyHostToDevice, s->stream));
-    if (npt) {
+    if (npt || pnpt) {
       // barostat state and the box of the chunk entry (the host copy of the segments)
       DPMD_HIP(hipMemcpyAsync(s->d_npt, s->d_npt + 64, sizeof(double) * 64, hipMemcpyDeviceToDevice, s->stream));
       s->seg_dirty = true;
@@ -1060,7 +1099,7 @@ int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps, 
     s->nl_overflow = true;
     return DPMD_OK;
   }
-  if (npt) {
+  if (npt || pnpt) {
     // the box moved on the device: the host copy follows before any upload
     DPMD_HIP(hipMemcpyAsync(s->seg.data(), s->d_seg, sizeof(SegInfo) * s->nseg, hipMemcpyDeviceToHost, s->stream));
     DPMD_HIP(hipStreamSynchronize(s->stream));
@@ -1111,7 +1150,7 @@ int32_t dpmd_thermostat_energy(dpmd_system_t sys, double* e) {
   DPMD_HIP(hipMemcpyAsync(t.data(), s->d_thermo, sizeof(double) * t.size(), hipMemcpyDeviceToHost, s->stream));
   DPMD_HIP(hipStreamSynchronize(s->stream));
   for (int r = 0; r < s->nseg; ++r) e[r] = t[(size_t)r * 64];
-  if (s->npt_ready) {
+  if (s->npt_ready && s->npt_kind == DPMD_NPT_NH) {
     // NPT: barostat chain, P V and box kinetic energy join the invariant
     double b[64];
     DPMD_HIP(hipMemcpy(b, s->d_npt, sizeof(b), hipMemcpyDeviceToHost));

@@ -211,6 +211,7 @@ struct System {
   int n_mobile = 0;
   double* d_npt = nullptr;      // NPT barostat state (64 doubles) and its chunk-entry copy
   bool npt_ready = false;       // NPT state initialised for the current layout
+  int npt_kind = 0;             // routine that initialised it (DPMD_NPT_NH or DPMD_PIMD_NPT)
   double* d_box_traj = nullptr; // per-step boxes of the last recorded chunk (steps, nseg, 3)
   int box_traj_steps = 0;
   uint64_t rng_counter = 0;
@@ -240,6 +241,13 @@ int32_t md_npt_baro_half(System* s, const NptArgs& a);
 int32_t md_npt_box(System* s, const NptArgs& a, int phase);
 int32_t md_npt_particles(System* s, double dt, bool move);
 int32_t md_box_record(System* s, double* out);
+int32_t md_pnpt_init(System* s, const NptArgs& a);
+int32_t md_pnpt_dudv(System* s, const NptArgs& a);
+int32_t md_pnpt_kick(System* s, const NptArgs& a);
+int32_t md_pnpt_move(System* s, const NptArgs& a, double h);
+int32_t md_pnpt_box_o(System* s, const NptArgs& a, double gamma, uint64_t seed, uint64_t counter);
+int32_t md_pnpt_il2(System* s, const NptArgs& a);
+int32_t md_pnpt_drift_ring(System* s, const NptArgs& a, const double* C, const double* freqs, double h);
 int32_t md_mask_forces(System* s);
 int32_t md_mask_momenta(System* s);
 int32_t md_remove_com(System* s);

@@ -650,6 +650,224 @@ __global__ void npt_particle_kernel(double* pos, double* mom, const double* mass
   }
 }
 
+// ------------------------------------------------------------------------------
+// Path-integral NPT (deepmd_jax/pimd.py:297-518, npt_langevin_pimd): the centroid follows
+// the MTK barostat step (exp(iL2) on the centroid momentum, exp(iL1) on the centroid
+// position), the internal ring modes propagate exactly, PILE-L acts on the bead momenta and
+// a Langevin O step on the box momenta.  The beads are the P segments (identical, no
+// images) and share one box.  Same 64-double block as NPT, with [40..43) the per-axis
+// centroid 2KE sum_i p_c^2 / m, [43] KE_box + P V - box work (the barostat part of the
+// reference's invariant), [44] box thermostat work, [51..54) the internal virial
+// sum_{b,i} (x_b - x_c) F_b per axis.
+
+__global__ void pnpt_init_kernel(NptArgs a, double* npt, SegInfo* seg, int nseg) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int k = 0; k < 64; ++k) npt[k] = 0.0;
+  for (int d = 0; d < 3; ++d) npt[12 + d] = seg[0].boxd[d];
+  for (int g = 0; g < a.G; ++g) npt[6 + g] = a.nper[g] * (a.natoms + 1.0) * a.kT * a.tau_p * a.tau_p;
+  npt[15] = 1.0;
+  for (int r = 0; r < nseg; ++r) npt_set_box(a, npt, seg[r]);
+}
+
+// Pass 1 of a per-axis sum over the (atom, axis) pairs of the ring polymer, a strided share
+// per block: mode 0 the centroid 2KE (p_c = mean over beads), mode 1 the internal virial
+// with the beads unwrapped onto bead 0's image (pimd.py:210-227).  part[3 block + axis].
+__global__ void pnpt_part_kernel(const double* pos, const double* mom, const double* mass, const float* force,
+                                 const SegInfo* seg, int natoms, int P, int mode, int KB, double* part) {
+  __shared__ double red[256];
+  double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+  const double invP = 1.0 / P;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < natoms * 3; t += KB * blockDim.x) {
+    const int i = t / 3, d = t - 3 * i;
+    double v = 0.0;
+    if (mode == 0) {
+      double pc = 0.0;
+      for (int b = 0; b < P; ++b) pc += mom[((size_t)b * natoms + i) * 3 + d];
+      pc *= invP;
+      v = pc * pc / mass[i];
+    } else {
+      const double L = seg[0].boxd[d];
+      const double x0 = pos[(size_t)i * 3 + d];
+      double sq = 0.0;
+      for (int b = 0; b < P; ++b) {
+        double dl = pos[((size_t)b * natoms + i) * 3 + d] - x0;
+        dl -= L * rint(dl / L);
+        sq += x0 + dl;
+      }
+      const double xc = sq * invP;
+      for (int b = 0; b < P; ++b) {
+        double dl = pos[((size_t)b * natoms + i) * 3 + d] - x0;
+        dl -= L * rint(dl / L);
+        v += (x0 + dl - xc) * (double)force[((size_t)b * natoms + i) * 3 + d];
+      }
+    }
+    acc0 += d == 0 ? v : 0.0;
+    acc1 += d == 1 ? v : 0.0;
+    acc2 += d == 2 ? v : 0.0;
+  }
+  const double s0 = block_sum(acc0, red);
+  const double s1 = block_sum(acc1, red);
+  const double s2 = block_sum(acc2, red);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x * 3 + 0] = s0;
+    part[blockIdx.x * 3 + 1] = s1;
+    part[blockIdx.x * 3 + 2] = s2;
+  }
+}
+
+// pass 2: the KB partials of each axis added in a fixed order
+__global__ void pnpt_sum3_kernel(const double* part, int KB, double* out) {
+  const int d = threadIdx.x;
+  if (d >= 3 || blockIdx.x != 0) return;
+  double v = 0.0;
+  for (int b = 0; b < KB; ++b) v += part[b * 3 + d];
+  out[d] = v;
+}
+
+// dU/deps per group, the centroid-virial form of force_stress_fn (pimd.py:336-353):
+// (sum over beads of -W_aa + internal virial_a) / P over the group's axes
+__global__ void pnpt_dudv_kernel(NptArgs a, double* npt, const double* red, int P) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int g = 0; g < 3; ++g) npt[9 + g] = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    if (a.gid[d] < 0) continue;
+    double v = npt[51 + d];
+    for (int r = 0; r < P; ++r) v -= red[16 * r + 1 + 4 * d];
+    npt[9 + a.gid[d]] += v / P;
+  }
+}
+
+// half kick of the box momenta, G_g = 2KE_c,g + (n_g / N_f) 2KE_c - dU/deps_g - P V n_g
+// (box_force_cv), then the barostat part of the invariant
+__global__ void pnpt_kick_kernel(NptArgs a, double* npt, const SegInfo* seg) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const SegInfo& S = seg[0];
+  const double Nf = 3.0 * a.natoms;
+  const double V = S.boxd[0] * S.boxd[1] * S.boxd[2];
+  const double k2tot = npt[40] + npt[41] + npt[42];
+  double k2g[3] = {0.0, 0.0, 0.0};
+  for (int d = 0; d < 3; ++d)
+    if (a.gid[d] >= 0) k2g[a.gid[d]] += npt[40 + d];
+  double e = a.pressure * V - npt[44];
+  for (int g = 0; g < a.G; ++g) {
+    const double Ge = k2g[g] + (a.nper[g] / Nf) * k2tot - npt[9 + g] - a.pressure * V * a.nper[g];
+    npt[3 + g] += 0.5 * a.dt * Ge;
+    e += npt[3 + g] * npt[3 + g] / (2.0 * npt[6 + g]);
+  }
+  npt[43] = e;
+}
+
+// R_b += h P_b / M_b and the new box on every bead
+__global__ void pnpt_move_kernel(NptArgs a, double* npt, SegInfo* seg, int nseg, double h) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int g = 0; g < a.G; ++g) npt[g] += h * npt[3 + g] / npt[6 + g];
+  for (int r = 0; r < nseg; ++r) npt_set_box(a, npt, seg[r]);
+}
+
+// Langevin O step of the box momenta, c1 = exp(-gamma dt), c2 = sqrt(kT (1 - c1^2)); the
+// change of box kinetic energy accumulates in the box work
+__global__ void pnpt_box_o_kernel(NptArgs a, double* npt, double c1, double c2, uint64_t seed, uint64_t counter) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double z[4];
+  normal4(seed, counter, 0u, 0x424f5800u, z);
+  for (int g = 0; g < a.G; ++g) {
+    const double Mb = npt[6 + g], p0 = npt[3 + g];
+    const double p1 = c1 * p0 + c2 * sqrt(Mb) * z[g];
+    npt[44] += 0.5 * (p1 * p1 - p0 * p0) / Mb;
+    npt[3 + g] = p1;
+  }
+}
+
+// exp(iL2) on the centroid momentum with the per-axis rate v_b,a + tr(v_b) / N_f, a plain
+// kick of the internal part (exp_iL2_pimd)
+__global__ void pnpt_il2_kernel(double* mom, const float* force, const double* npt, NptArgs a, int natoms, int P,
+                                double dt) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= natoms * 3) return;
+  const int i = t / 3, d = t - 3 * i;
+  double tr = 0.0, vd = 0.0;
+  for (int g = 0; g < a.G; ++g) {
+    const double vb = npt[3 + g] / npt[6 + g];
+    tr += a.nper[g] * vb;
+    if (g == a.gid[d]) vd = vb;
+  }
+  const double x = (vd + tr / (3.0 * a.natoms)) * dt * 0.5;
+  double pc = 0.0, fc = 0.0;
+  for (int b = 0; b < P; ++b) {
+    pc += mom[((size_t)b * natoms + i) * 3 + d];
+    fc += (double)force[((size_t)b * natoms + i) * 3 + d];
+  }
+  pc /= P;
+  fc /= P;
+  const double pcn = pc * exp(-x) + 0.5 * dt * fc * sinhx_x(0.5 * x) * exp(-0.5 * x);
+  for (int b = 0; b < P; ++b) {
+    const size_t o = ((size_t)b * natoms + i) * 3 + d;
+    mom[o] = pcn + (mom[o] - pc) + 0.5 * dt * ((double)force[o] - fc);
+  }
+}
+
+// exp(iL1) on the centroid (every bead shifted by dx_c = x_c (e^y - 1) + h v_c e^{y/2}
+// sinhx_x(y/2), y = v_b,a h) and the exact propagation of the internal normal modes over h
+// (exp_iL1_centroid_pimd, exact_internal_ring_step), beads unwrapped onto bead 0's image.
+__global__ void pnpt_drift_ring_kernel(double* pos, double* mom, const double* mass, const double* npt, NptArgs a,
+                                       const SegInfo* seg, int natoms, int P, const double* C, const double* freqs,
+                                       double h, int* flags) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= natoms * 3) return;
+  const int i = t / 3, d = t - 3 * i;
+  const double L = seg[0].boxd[d];
+  const double m = mass[i];
+  const int g = a.gid[d];
+  double vd = 0.0;
+  for (int k = 0; k < a.G; ++k)
+    if (k == g) vd = npt[3 + k] / npt[6 + k];
+  const double y = vd * h;
+  double q[64], p[64], qn[64], pn[64];
+  const double x0 = pos[(size_t)i * 3 + d];
+  double xc = 0.0, vc = 0.0;
+  for (int b = 0; b < P; ++b) {
+    double dl = pos[((size_t)b * natoms + i) * 3 + d] - x0;
+    dl -= L * rint(dl / L);
+    q[b] = x0 + dl;
+    p[b] = mom[((size_t)b * natoms + i) * 3 + d];
+    xc += q[b];
+    vc += p[b];
+  }
+  xc /= P;
+  vc /= P * m;
+  const double dxc = xc * (exp(y) - 1.0) + h * vc * exp(0.5 * y) * sinhx_x(0.5 * y);
+  for (int b = 0; b < P; ++b) q[b] += dxc;
+  for (int k = 0; k < P; ++k) {
+    double sq = 0, sp = 0;
+    for (int b = 0; b < P; ++b) {
+      sq += C[b * P + k] * q[b];
+      sp += C[b * P + k] * p[b];
+    }
+    qn[k] = sq;
+    pn[k] = sp;
+  }
+  for (int k = 1; k < P; ++k) {
+    const double w = freqs[k];
+    const double cs = cos(w * h), sn = sin(w * h);
+    const double qq = qn[k] * cs + pn[k] * sn / (m * w);
+    const double pp = pn[k] * cs - m * w * qn[k] * sn;
+    qn[k] = qq;
+    pn[k] = pp;
+  }
+  bool bad = false;
+  for (int b = 0; b < P; ++b) {
+    double sq = 0, sp = 0;
+    for (int k = 0; k < P; ++k) {
+      sq += C[b * P + k] * qn[k];
+      sp += C[b * P + k] * pn[k];
+    }
+    pos[((size_t)b * natoms + i) * 3 + d] = sq;
+    mom[((size_t)b * natoms + i) * 3 + d] = sp;
+    bad |= !isfinite(sq) || !isfinite(sp);
+  }
+  if (bad) atomicOr(&flags[2], 16);
+}
+
 __global__ void box_record_kernel(const SegInfo* seg, int nseg, double* out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < 3 * nseg) out[i] = seg[i / 3].boxd[i % 3];
@@ -682,6 +900,73 @@ int32_t md_npt_particles(System* s, double dt, bool move) {
   const int NA = s->NA;
   hipLaunchKernelGGL(npt_particle_kernel, dim3((NA + 255) / 256), dim3(256), 0, s->stream, s->d_pos, s->d_mom,
                      s->d_mass, s->d_force, s->d_pos32, s->d_npt, s->d_seg, NA, dt, move ? 1 : 0, s->d_flags);
+  DPMD_HIP(hipGetLastError());
+  return DPMD_OK;
+}
+
+// blocks for the per-axis ring-polymer sums: about 2,048 (atom, axis) pairs per block
+static int pnpt_blocks(int n3) { return std::max(1, std::min(kKeBlocks, (n3 + 2047) / 2048)); }
+
+int32_t md_pnpt_init(System* s, const NptArgs& a) {
+  hipLaunchKernelGGL(pnpt_init_kernel, dim3(1), dim3(64), 0, s->stream, a, s->d_npt, s->d_seg, s->nseg);
+  DPMD_HIP(hipGetLastError());
+  return DPMD_OK;
+}
+
+static int32_t pnpt_sum(System* s, int mode, double* out) {
+  const int N = s->seg[0].natoms, KB = pnpt_blocks(N * 3);
+  hipLaunchKernelGGL(pnpt_part_kernel, dim3(KB), dim3(256), 0, s->stream, s->d_pos, s->d_mom, s->d_mass, s->d_force,
+                     s->d_seg, N, s->nseg, mode, KB, s->d_kepart);
+  hipLaunchKernelGGL(pnpt_sum3_kernel, dim3(1), dim3(64), 0, s->stream, s->d_kepart, KB, out);
+  DPMD_HIP(hipGetLastError());
+  return DPMD_OK;
+}
+
+int32_t md_pnpt_dudv(System* s, const NptArgs& a) {
+  DPMD_TRY(md_reduce_launch(s));
+  DPMD_TRY(pnpt_sum(s, 1, s->d_npt + 51));
+  hipLaunchKernelGGL(pnpt_dudv_kernel, dim3(1), dim3(64), 0, s->stream, a, s->d_npt, s->d_red, s->nseg);
+  DPMD_HIP(hipGetLastError());
+  return DPMD_OK;
+}
+
+int32_t md_pnpt_kick(System* s, const NptArgs& a) {
+  DPMD_TRY(pnpt_sum(s, 0, s->d_npt + 40));
+  hipLaunchKernelGGL(pnpt_kick_kernel, dim3(1), dim3(64), 0, s->stream, a, s->d_npt, s->d_seg);
+  DPMD_HIP(hipGetLastError());
+  return DPMD_OK;
+}
+
+int32_t md_pnpt_move(System* s, const NptArgs& a, double h) {
+  hipLaunchKernelGGL(pnpt_move_kernel, dim3(1), dim3(64), 0, s->stream, a, s->d_npt, s->d_seg, s->nseg, h);
+  DPMD_HIP(hipGetLastError());
+  return DPMD_OK;
+}
+
+int32_t md_pnpt_box_o(System* s, const NptArgs& a, double gamma, uint64_t seed, uint64_t counter) {
+  const double c1 = exp(-gamma * a.dt);
+  const double c2 = sqrt(std::max(a.kT * (1.0 - c1 * c1), 0.0));
+  hipLaunchKernelGGL(pnpt_box_o_kernel, dim3(1), dim3(64), 0, s->stream, a, s->d_npt, c1, c2, seed, counter);
+  DPMD_HIP(hipGetLastError());
+  return DPMD_OK;
+}
+
+int32_t md_pnpt_il2(System* s, const NptArgs& a) {
+  const int N = s->seg[0].natoms;
+  hipLaunchKernelGGL(pnpt_il2_kernel, dim3((N * 3 + 255) / 256), dim3(256), 0, s->stream, s->d_mom, s->d_force,
+                     s->d_npt, a, N, s->nseg, a.dt);
+  DPMD_HIP(hipGetLastError());
+  return DPMD_OK;
+}
+
+int32_t md_pnpt_drift_ring(System* s, const NptArgs& a, const double* C, const double* freqs, double h) {
+  const int N = s->seg[0].natoms;
+  hipLaunchKernelGGL(pnpt_drift_ring_kernel, dim3((N * 3 + 127) / 128), dim3(128), 0, s->stream, s->d_pos,
+                     s->d_mom, s->d_mass, s->d_npt, a, s->d_seg, N, s->nseg, C, freqs, h, s->d_flags);
+  // the fp32 mirror in the box the device just set (no upload of the host segments)
+  const int NA = s->NA;
+  hipLaunchKernelGGL(write_pos32_kernel, dim3((NA + 255) / 256), dim3(256), 0, s->stream, s->d_pos, s->d_pos32,
+                     s->d_seg, s->d_atom_seg, NA);
   DPMD_HIP(hipGetLastError());
   return DPMD_OK;
 }

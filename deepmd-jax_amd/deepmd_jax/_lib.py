@@ -10,7 +10,7 @@ import numpy as np
 
 MAXT = 4
 PREC = {'fp32': 0, 'bf16': 1, 'fp16': 2}
-ROUTINE = {'NVE': 0, 'NVT': 1, 'NVT_langevin': 2, 'PIMD': 3, 'NPT': 4}
+ROUTINE = {'NVE': 0, 'NVT': 1, 'NVT_langevin': 2, 'PIMD': 3, 'NPT': 4, 'PIMD_NPT': 5}
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, 'libdpmd_hip.so')

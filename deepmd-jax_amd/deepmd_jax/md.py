@@ -9,12 +9,14 @@ trajectory copy and the periodic report (deepmd_jax/md.py:473-595).
 
 Routines: 'NVE' (velocity Verlet), 'NVT' (Nose-Hoover chain; with n_bead > 1
 ring-polymer PILE-L), 'NVT_langevin' (BAOAB), 'NPT' (MTK Nose-Hoover barostat over
-couple_axes groups with a Nose-Hoover chain thermostat, deepmd_jax/routine.py:161-456,
-one system in the neighbour-list regime).  Positions and momenta are FP64 on the device.
+couple_axes groups with a Nose-Hoover chain thermostat, deepmd_jax/routine.py:161-456;
+with n_bead > 1 MTK on the centroid with PILE-L and a Langevin barostat,
+deepmd_jax/pimd.py:297-518), in the neighbour-list regime.  Positions and momenta are
+FP64 on the device.
 Fixed atoms (fixed_indices) and per-step centre-of-mass removal (remove_com_motion) run
 on the device (deepmd_jax/md.py:221-234,786-790, deepmd_jax/routine.py:110-136).
 XYZ dumps (run(dump_prefix=...)) stream from a writer thread (deepmd_jax/dumps.py).
-Not provided: path-integral NPT, model deviation, DPLR.
+Not provided: model deviation, DPLR.
 """
 import time
 
@@ -24,7 +26,8 @@ from . import _lib
 from .constants import MASS_UNIT_CONVERSION, TEMP_UNIT_CONVERSION, PRESS_UNIT_CONVERSION
 from .dumps import CHEMICAL_SYMBOLS, DumpWriter, prepare_dumps, target_label
 from .evaluator import canonical_box
-from .pimd import normal_mode_transform, pile_l_factors, centroid
+from .constants import HBAR
+from .pimd import normal_mode_transform, pile_l_factors, centroid, spring_energy
 from .utils import load_model
 
 
@@ -182,13 +185,11 @@ class Simulation:
                 raise ValueError("Missing argument 'pressure' (in bar) for routine 'NPT'")
             if temperature is None:
                 raise ValueError("Missing argument 'temperature' for routine 'NPT'")
-            if self._is_pimd:
-                raise NotImplementedError('path-integral NPT is not provided by this engine.')
             self._couple_axes, self._group_ids = parse_couple_axes(couple_axes)
             self._pressure = float(pressure)
         pos = np.asarray(initial_position, np.float64)
         if self._is_pimd:
-            if routine != 'NVT':
+            if routine not in ('NVT', 'NPT'):
                 raise NotImplementedError("Path-integral MD (n_bead > 1) currently supports routine='NVT' or 'NPT'.")
             if temperature is None:
                 raise ValueError('PIMD needs a temperature.')
@@ -207,6 +208,7 @@ class Simulation:
             self._natoms = pos.shape[0]
             pos = pos[None]
         self._temperature = temperature
+        self._pimd_print_invariant = bool(pimd_print_invariant)
         self._dt = float(dt)
         self._routine = routine
         self._precision = precision
@@ -295,7 +297,11 @@ class Simulation:
             for a in range(3):
                 self._params.couple[a] = int(self._group_ids[a])
         if self._is_pimd:
-            self._params.routine = _lib.ROUTINE['PIMD']
+            # PILE-L of the bead momenta (deepmd_jax/md.py:341-351,368-381); path-integral NPT
+            # adds a Langevin barostat of friction 1 / tau_p (deepmd_jax/md.py:374)
+            self._params.routine = _lib.ROUTINE['PIMD_NPT' if routine == 'NPT' else 'PIMD']
+            if routine == 'NPT':
+                self._params.gamma = 1.0 / float(tau_p)
             freqs, C = normal_mode_transform(self._n_bead, self._kT)
             _, c1, c2 = pile_l_factors(freqs, self._dt, self._kT, self._n_bead, float(tau_t), self._pimd_lambda)
             self._sys.pimd_setup(C, freqs, c1, c2)
@@ -320,7 +326,8 @@ class Simulation:
                   ('NVT', False): 'NVT run (Nose-Hoover thermostat)',
                   ('NVT_langevin', False): 'NVT Langevin run (BAOAB Langevin thermostat)',
                   ('NVT', True): 'NVT PIMD run (PILE-L Langevin thermostat)',
-                  ('NPT', False): 'NPT run (Nose-Hoover thermostat/barostat)'}
+                  ('NPT', False): 'NPT run (Nose-Hoover thermostat/barostat)',
+                  ('NPT', True): 'NPT PIMD run (PILE-L thermostat, Langevin barostat)'}
         info = []
         if temperature is not None and routine != 'NVE':
             info.append('temperature=%g K' % temperature)
@@ -358,17 +365,41 @@ class Simulation:
             rep['Invariant'] = ke_tot + pe - float(self._sys.thermostat_energy()[0])
         elif self._routine == 'NVT' and not self._is_pimd:
             rep['Invariant'] = ke_tot + pe + float(self._sys.thermostat_energy()[0])
+        elif self._routine == 'NVT' and self._pimd_print_invariant:
+            rep['Invariant'] = self._pimd_invariant(e, ke_tot, self._box)
         elif self._routine == 'NPT':
             # deepmd_jax/md.py:523-538: pressure, moving box lengths, invariant
             box = self.getBox()
-            rep['Pressure'] = (N * self._kT + float(np.trace(v[0])) / 3.0) / float(np.prod(box)) / PRESS_UNIT_CONVERSION
+            # iso_pressure of the (bead-averaged) energy (deepmd_jax/md_utils.py:139-159)
+            rep['Pressure'] = (N * self._kT + float(np.trace(v.mean(0))) / 3.0) / float(np.prod(box)) / PRESS_UNIT_CONVERSION
             for a in sorted(a for g in self._couple_axes for a in g):
                 rep['box_' + 'xyz'[a]] = float(box[a])
-            rep['Invariant'] = ke_tot + pe + float(self._sys.thermostat_energy()[0])
+            if not self._is_pimd:
+                rep['Invariant'] = ke_tot + pe + float(self._sys.thermostat_energy()[0])
+            elif self._pimd_print_invariant:
+                # deepmd_jax/md.py:540-557: the barostat part KE_box + P V - box work is kept
+                # on the device
+                rep['Invariant'] = self._pimd_invariant(e, ke_tot, box) + self._barostat_terms(box)
         self._last_force = f
         self._last_virial = v
         self._last_energy = e
         return rep
+
+    def _barostat_terms(self, box):
+        """KE_box + P V - box work of path-integral NPT, kept on the device; P V before the
+        first step (box at rest)."""
+        try:
+            return float(self._sys.dev.npt_state()[43])
+        except _lib.DPMDError:
+            return self._params.pressure * float(np.prod(box))
+
+    def _pimd_invariant(self, e_beads, ke_tot, box):
+        """(H_rp - PILE-L work) / P with H_rp = KE + sum of bead energies + spring energy
+        (deepmd_jax/md.py:512-521)."""
+        P = self._n_bead
+        omega_p = P * self._kT / HBAR
+        h_rp = ke_tot + float(np.sum(e_beads)) + spring_energy(self._sys.get_positions(), self._mass, box, omega_p)
+        return (h_rp - float(self._sys.thermostat_energy()[0])) / P
 
     def _print_report(self):
         rep = self._report_values()
@@ -477,7 +508,7 @@ class Simulation:
             else:
                 if self._is_pimd:
                     for k in range(nxt):
-                        self._centroid_trajectory[lo + k] = centroid(tp[k], self._box)
+                        self._centroid_trajectory[lo + k] = centroid(tp[k], self._box if boxes is None else boxes[k])
                 self._box_trajectory[lo:lo + nxt] = self._box if boxes is None else boxes
             self.step += nxt
             remaining -= nxt
@@ -499,7 +530,7 @@ class Simulation:
 
     def getCentroidPosition(self):
         p = self._state_pos()
-        return centroid(p, self._box) if self._is_pimd else p[0]
+        return centroid(p, self.getBox()) if self._is_pimd else p[0]
 
     def setPosition(self, position, box=None):
         if box is not None:

@@ -43,3 +43,11 @@ def unwrap_across_beads(coord, box):
 
 def centroid(coord, box):
     return unwrap_across_beads(coord, box).mean(axis=0)
+
+
+def spring_energy(coord, mass, box, omega_p):
+    """Ring-polymer spring energy 0.5 m omega_P^2 sum_b (x_b - x_{b-1})^2 over the beads
+    unwrapped onto bead 0's image (deepmd_jax/pimd.py:82-89)."""
+    xu = unwrap_across_beads(coord, box)
+    dx = xu - np.roll(xu, 1, axis=0)
+    return 0.5 * omega_p ** 2 * float((np.asarray(mass)[None, :, None] * dx ** 2).sum())

@@ -44,6 +44,8 @@ extern "C" {
 #define DPMD_NVT_LANGEVIN 2 /* BAOAB, deepmd_jax/routine.py:46-107 */
 #define DPMD_PIMD_PILE_L 3  /* ring-polymer BAOAB + PILE-L, deepmd_jax/pimd.py:114-207 */
 #define DPMD_NPT_NH 4       /* MTK Nose-Hoover barostat + thermostat, deepmd_jax/routine.py:161-423 */
+#define DPMD_PIMD_NPT 5     /* path-integral NPT: MTK on the centroid, exact internal ring modes,
+                               PILE-L and a Langevin barostat, deepmd_jax/pimd.py:297-518 */
 
 typedef struct dpmd_model* dpmd_model_t;
 typedef struct dpmd_system* dpmd_system_t;
@@ -86,13 +88,15 @@ typedef struct {
   int32_t precision;    /* DPMD_PREC_* for the force evaluations */
   double dt;
   double kT;            /* thermostat temperature (eV) */
-  double gamma;         /* Langevin friction (1/fs); PIMD: centroid friction */
+  double gamma;         /* Langevin friction (1/fs) of NVT_langevin; DPMD_PIMD_NPT: of the box
+                           momenta (1 / tau_p in deepmd_jax/md.py:374) */
   double tau;           /* Nose-Hoover relaxation time (fs) */
   int32_t chain_length; /* Nose-Hoover chain length */
   int32_t record_every; /* >0: store positions/velocities every step into traj buffers */
   uint64_t seed;        /* stochastic integrators: counter-based normal stream */
   double pimd_lambda;   /* PILE-L internal-mode damping scale */
-  /* DPMD_NPT_NH only (deepmd_jax/md.py:357-366, deepmd_jax/routine.py:161-423): */
+  /* DPMD_NPT_NH (deepmd_jax/md.py:357-366, deepmd_jax/routine.py:161-423) and DPMD_PIMD_NPT
+   * (deepmd_jax/md.py:368-381; the chain length is not used there): */
   double pressure;      /* target pressure (eV / Angstrom^3) */
   double tau_p;         /* barostat relaxation time (fs) */
   int32_t chain_length_p; /* barostat chain length */
@@ -195,7 +199,7 @@ int32_t dpmd_compute(dpmd_system_t sys, int32_t precision, double* energy, doubl
  * *error_code accumulates the reference's bit codes. */
 int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps,
                     double* traj_pos, double* traj_vel, uint32_t* error_code);
-/* Ring-polymer constants for DPMD_PIMD_PILE_L (deepmd_jax/pimd.py:22-44,92-111,160-179):
+/* Ring-polymer constants for DPMD_PIMD_PILE_L and DPMD_PIMD_NPT (deepmd_jax/pimd.py:22-44,92-111,160-179):
  * C (P*P, row-major, primitive <- normal mode), freqs (P), and the per-mode O-step
  * factors c1 = exp(-gamma_k dt), c2 = sqrt(P kT (1 - c1^2)) (P).  P = nreplica. */
 int32_t dpmd_pimd_setup(dpmd_system_t sys, const double* C, const double* freqs, const double* c1,
@@ -234,7 +238,10 @@ int32_t dpmd_md_box_trajectory(dpmd_system_t sys, double* boxes);
 /* NPT barostat state (64 doubles): [0..3) log box scale per group, [3..6) box momenta,
  * [6..9) box masses, [9..12) dU/deps per group, [12..15) reference box, [16..24) chain
  * positions, [24..32) chain momenta, [32..40) chain masses, [43] barostat invariant terms
- * (CachedNPTNoseHooverState, deepmd_jax/routine.py:161-194). */
+ * (CachedNPTNoseHooverState, deepmd_jax/routine.py:161-194).  After DPMD_PIMD_NPT steps
+ * (PIMDNPTLangevinState, deepmd_jax/pimd.py:255-290): [9..12) the centroid-virial dU/deps,
+ * [40..43) the per-axis centroid 2KE, [43] KE_box + P V - box work, [44] box work; the
+ * PILE-L work is dpmd_thermostat_energy of segment 0. */
 int32_t dpmd_npt_state_get(dpmd_system_t sys, double* state);
 
 #ifdef __cplusplus

@@ -31,6 +31,8 @@ Function -> reference file:line it follows:
   nvt_nh_step           deepmd_jax/routine.py:110-136,147-158 (jax_md nvt_nose_hoover,
                         chain masses from the mobile degrees of freedom)
   remove_com            deepmd_jax/md.py:786-790
+  PIMDNPTOracle         deepmd_jax/pimd.py:297-518 (npt_langevin_pimd), invariant
+                        deepmd_jax/md.py:540-557 (pinned by its conservation)
   energy_force_virial(mobile=) deepmd_jax/md.py:428-430 (stop-gradient on fixed atoms)
 Pinning (tests/test_oracle.py): closed forms of the switch and of tensor_3to6,
 rotation / translation / permutation invariance, the T3=T6=0 outer-product
@@ -622,3 +624,121 @@ class NPTOracle:
         E += (self.bpx ** 2 / (2 * Qb) + kT * self.bxi).sum()
         E += self.P * np.prod(self.box) + (self.Pb ** 2 / (2 * self.Mb)).sum()
         return E
+
+
+class PIMDNPTOracle:
+    """Path-integral NPT step of deepmd_jax/pimd.py:297-518 (npt_langevin_pimd) in FP64 for an
+    orthorhombic box: Parrinello-Rahman / MTK on the centroid, exact free propagation of the
+    internal ring modes, PILE-L O step on the beads and a Langevin O step on the box momenta.
+
+    x, p (P, N, 3); force_fn(x_bead, box) -> (F (N,3), dU/deps per axis (3,)) for one bead.
+    c1, c2 (P,) PILE-L factors; gamma_box the box friction.  noise=None in step() switches the
+    stochastic terms off (the O steps then only damp by c1)."""
+
+    def __init__(self, x, p, m, box, force_fn, dt, kT, pressure, C, freqs, c1, c2, gamma_box, tau_p,
+                 couple_axes=((0, 1, 2),)):
+        self.x, self.p, self.m = np.array(x, float), np.array(p, float), np.asarray(m, float)
+        self.force_fn, self.dt, self.kT, self.P = force_fn, dt, kT, pressure
+        self.C, self.freqs, self.c1, self.c2 = C, np.asarray(freqs), np.asarray(c1), np.asarray(c2)
+        self.c1b = np.exp(-gamma_box * dt)
+        self.c2b = np.sqrt(max(kT * (1 - self.c1b ** 2), 0.0))
+        self.gid = np.full(3, -1)
+        for g, grp in enumerate(couple_axes):
+            for a in grp:
+                self.gid[a] = g
+        self.G = len(couple_axes)
+        self.nper = np.array([len(g) for g in couple_axes], float)
+        self.ref = np.asarray(box, float).copy()
+        self.Rb, self.Pb = np.zeros(self.G), np.zeros(self.G)
+        self.nb, self.N = self.x.shape[0], self.x.shape[1]
+        self.Mb = self.nper * (self.N + 1) * kT * tau_p ** 2
+        self.work = self.box_work = 0.0
+        self.F, self.dUdV = self._force_stress()
+
+    def _group(self, axis_vals):
+        out = np.zeros(self.G)
+        for a in range(3):
+            if self.gid[a] >= 0:
+                out[self.gid[a]] += axis_vals[a]
+        return out
+
+    def _axis(self, group_vals):
+        return np.array([group_vals[self.gid[a]] if self.gid[a] >= 0 else 0.0 for a in range(3)])
+
+    @property
+    def box(self):
+        return self.ref * np.exp(self._axis(self.Rb))
+
+    def _force_stress(self):
+        """Bead forces and the centroid-virial dU/deps per group (pimd.py:336-353)."""
+        F, d = [], np.zeros(3)
+        for b in range(self.nb):
+            Fb, db = self.force_fn(self.x[b], self.box)
+            F.append(Fb)
+            d += db
+        F = np.stack(F)
+        xu = unwrap_beads(self.x, self.box)
+        internal = ((xu - xu.mean(0)) * F).sum((0, 1))
+        return F, self._group(d + internal) / self.nb
+
+    def _kick_box(self):
+        pc = self.p.mean(0)
+        k2 = (pc ** 2 / self.m[:, None]).sum(0)
+        Nf = 3.0 * self.N
+        G = self._group(k2) + self.nper / Nf * k2.sum() - self.dUdV - self.P * np.prod(self.box) * self.nper
+        self.Pb = self.Pb + 0.5 * self.dt * G
+
+    def _iL2(self):
+        vb = self.Pb / self.Mb
+        rate = self._axis(vb) + (self.nper * vb).sum() / (3.0 * self.N)
+        x = rate * self.dt / 2
+        pc, fc = self.p.mean(0), self.F.mean(0)
+        pc_new = pc * np.exp(-x) + self.dt / 2 * fc * _sinhx_x(x / 2) * np.exp(-x / 2)
+        self.p = pc_new[None] + (self.p - pc[None]) + self.dt / 2 * (self.F - fc[None])
+
+    def _drift(self, h):
+        self.Rb = self.Rb + h * self.Pb / self.Mb
+        box = self.box
+        y = self._axis(self.Pb / self.Mb) * h
+        rc = unwrap_beads(self.x, box).mean(0)
+        vc = (self.p / self.m[None, :, None]).mean(0)
+        self.x = self.x + (rc * (np.exp(y) - 1) + h * vc * np.exp(y / 2) * _sinhx_x(y / 2))[None]
+        # internal ring modes only (the centroid moved above)
+        xu = unwrap_beads(self.x, box)
+        q = np.tensordot(self.C.T, xu, axes=(1, 0))
+        pn = np.tensordot(self.C.T, self.p, axes=(1, 0))
+        mm, w = self.m[None, :, None], self.freqs[1:, None, None]
+        cs, sn = np.cos(w * h), np.sin(w * h)
+        q1 = q[1:] * cs + pn[1:] * sn / (mm * w)
+        pn[1:] = pn[1:] * cs - mm * w * q[1:] * sn
+        q[1:] = q1
+        self.x = np.tensordot(self.C, q, axes=(1, 0))
+        self.p = np.tensordot(self.C, pn, axes=(1, 0))
+
+    def step(self, noise=None, noise_box=None):
+        self._kick_box()
+        self._iL2()
+        self._drift(self.dt / 2)
+        p_old = self.p
+        pn = self.c1[:, None, None] * np.tensordot(self.C.T, self.p, axes=(1, 0))
+        if noise is not None:
+            pn = pn + self.c2[:, None, None] * np.sqrt(self.m)[None, :, None] * noise
+        self.p = np.tensordot(self.C, pn, axes=(1, 0))
+        self.work += 0.5 * ((self.p ** 2 - p_old ** 2) / self.m[None, :, None]).sum()
+        pb_old = self.Pb
+        self.Pb = self.c1b * self.Pb + (self.c2b * np.sqrt(self.Mb) * noise_box if noise_box is not None else 0.0)
+        self.box_work += 0.5 * ((self.Pb ** 2 - pb_old ** 2) / self.Mb).sum()
+        self._drift(self.dt / 2)
+        self.F, self.dUdV = self._force_stress()
+        self._iL2()
+        self._kick_box()
+
+    def invariant(self, PE_beads, omega_P):
+        """deepmd_jax/md.py:540-557: (H_rp - work) / P + KE_box + P V - box work, with
+        H_rp = KE + sum of bead energies + ring spring energy."""
+        xu = unwrap_beads(self.x, self.box)
+        spring = 0.5 * omega_P ** 2 * (self.m[None, :, None] * (xu - np.roll(xu, 1, axis=0)) ** 2).sum()
+        ke = 0.5 * (self.p ** 2 / self.m[None, :, None]).sum()
+        H = ke + np.sum(PE_beads) + spring
+        return ((H - self.work) / self.nb + (self.Pb ** 2 / (2 * self.Mb)).sum() + self.P * np.prod(self.box)
+                - self.box_work)

@@ -94,3 +94,57 @@ def test_parse_couple_axes_groups():
     assert groups == ((0, 1), (2,)) and list(gid) == [0, 0, 1]
     _, gid = parse_couple_axes(((1,),))
     assert list(gid) == [-1, 0, -1]
+
+
+def _pimd_start(nb=4, seed=3, kT=0.5, hbar=0.25):
+    x, p, m, box, kT = _start(seed, kT=kT)
+    rng = np.random.default_rng(seed + 10)
+    xb = x[None] + rng.normal(0, 0.05, (nb,) + x.shape)
+    pb = rng.standard_normal((nb,) + x.shape) * np.sqrt(kT * nb)
+    freqs, C = O.ring_modes(nb, kT, hbar)
+    return xb, pb, m, box, kT, freqs, C, nb * kT / hbar
+
+
+@pytest.mark.parametrize('couple', [((0, 1, 2),), ((0, 1), (2,))])
+def test_pimd_npt_invariant_conserved(couple):
+    """deepmd_jax/pimd.py:297-518 with the O steps switched off (c1 = 1, no noise): the reported
+    invariant (H_rp / P + KE_box + P V) is conserved while the volume moves by > 5 %; a dropped
+    internal-virial term, centroid-only coupling or sign breaks it."""
+    xb, pb, m, box, kT, freqs, C, wP = _pimd_start()
+    nb = len(xb)
+
+    def ff(xx, b):
+        _, F, d = soft_spheres(xx, b)
+        return F, d
+    o = O.PIMDNPTOracle(xb, pb, m, box, ff, 0.004, kT, 1.0, C, freqs, np.ones(nb), np.zeros(nb), 0.0, 2.0,
+                        couple_axes=couple)
+    H, V = [], []
+    for s in range(1500):
+        o.step()
+        if s % 25 == 0:
+            pe = [soft_spheres(o.x[b], o.box)[0] for b in range(nb)]
+            H.append(o.invariant(pe, wP))
+            V.append(np.prod(o.box))
+    H, V = np.array(H), np.array(V)
+    assert V.max() / V.min() > 1.05, V.max() / V.min()
+    assert abs(H[-1] - H[0]) < 0.05 and H.std() < 0.02, (H[-1] - H[0], H.std())
+
+
+def test_pimd_npt_one_bead_is_classical_mtk_without_chains():
+    """With one bead the centroid is the particle: the step reduces to the MTK barostat step of
+    NPTOracle with its thermostat and barostat chains absent (c1 = 1, gamma_box = 0)."""
+    x, p, m, box, kT = _start(4)
+
+    def ff(xx, b):
+        _, F, d = soft_spheres(xx, b)
+        return F, d
+    o1 = O.PIMDNPTOracle(x[None], p[None], m, box, ff, 0.004, kT, 1.0, np.ones((1, 1)), np.zeros(1), np.ones(1),
+                         np.zeros(1), 0.0, 2.0)
+    o2 = O.NPTOracle(x, p, m, box, ff, 0.004, kT, 1.0, 1e12, 2.0)
+    o2._chains = lambda: (np.full(3, 1e300), np.full(3, 1e300))
+    for _ in range(50):
+        o1.step()
+        o2.step()
+    np.testing.assert_allclose(o1.box, o2.box, rtol=1e-12)
+    np.testing.assert_allclose(o1.x[0], o2.x, atol=1e-10)
+    np.testing.assert_allclose(o1.p[0], o2.p, atol=1e-10)
