@@ -16,7 +16,8 @@ FP64 on the device.
 Fixed atoms (fixed_indices) and per-step centre-of-mass removal (remove_com_motion) run
 on the device (deepmd_jax/md.py:221-234,786-790, deepmd_jax/routine.py:110-136).
 XYZ dumps (run(dump_prefix=...)) stream from a writer thread (deepmd_jax/dumps.py).
-Not provided: model deviation, DPLR.
+Model deviation (model_deviation_paths) evaluates each listed model on its own device
+system at report time.  Not provided: DPLR.
 """
 import time
 
@@ -170,8 +171,9 @@ class Simulation:
             raise NotImplementedError('Fixing atoms currently limited to NVE, NVT, and NVT_langevin routines.')
         if fixed_indices is not None and self._is_pimd:
             raise NotImplementedError('fixed atoms with path-integral MD are not provided by this engine.')
-        if len(model_deviation_paths) > 0:
-            raise NotImplementedError('model_deviation_paths is not provided by this engine.')
+        if len(model_deviation_paths) > 0 and self._is_pimd:
+            # deepmd_jax/md.py:148-149
+            raise NotImplementedError('model_deviation_paths is not supported with n_bead > 1.')
         if remove_com_motion and self._is_pimd:
             raise NotImplementedError('per-step remove_com_motion with path-integral MD is not provided by this engine.')
         self._remove_com_motion = bool(remove_com_motion)
@@ -278,6 +280,18 @@ class Simulation:
             self._sys.set_fixed(~mobile)
         self._sys.set_positions(pos)
         self._allocate_nlist()
+        # models of the deviation column (deepmd_jax/md.py:299-302,559-567): each on its own
+        # device system over the same atoms, evaluated at report time
+        self._deviation = []
+        for path in model_deviation_paths:
+            dmodel, dvar = load_model(path)
+            ds = _ImageSystem(_lib.DeviceModel(dmodel.params, dvar, device), self._type_idx, 1, self._box, k)
+            ds.set_masses(self._mass)
+            if self._mobile is not None:
+                ds.set_fixed(~self._mobile)
+            ds.set_positions(pos)
+            ds.nlist_allocate(float(dmodel.params['rcut']) + self._neighbor_skin, self._neighbor_buffer_ratio)
+            self._deviation.append((ds, float(dmodel.params['rcut'])))
         self._kT = (temperature if temperature is not None else 0.0) * TEMP_UNIT_CONVERSION
         self._params = _lib.MDParams()
         self._params.precision = _lib.PREC[precision]
@@ -380,10 +394,30 @@ class Simulation:
                 # deepmd_jax/md.py:540-557: the barostat part KE_box + P V - box work is kept
                 # on the device
                 rep['Invariant'] = self._pimd_invariant(e, ke_tot, box) + self._barostat_terms(box)
+        if self._deviation:
+            rep['Model_Devi'] = self._model_deviation()
         self._last_force = f
         self._last_virial = v
         self._last_energy = e
         return rep
+
+    def _model_deviation(self):
+        """max over atoms and components of the standard deviation of the deviation models'
+        forces at the current positions (deepmd_jax/md.py:559-567)."""
+        pos = self._sys.get_positions()
+        box = self.getBox()
+        forces = []
+        for ds, rc in self._deviation:
+            if self._routine == 'NPT':
+                ds.dev.set_box(box)
+                ds.set_positions(pos)
+                ds.nlist_allocate(rc + self._neighbor_skin, self._neighbor_buffer_ratio)
+            else:
+                ds.set_positions(pos)
+                if ds.nlist_update():
+                    ds.nlist_allocate(rc + self._neighbor_skin, self._neighbor_buffer_ratio)
+            forces.append(ds.compute(self._precision)[1][0])
+        return float(np.std(np.stack(forces), axis=0).max())
 
     def _barostat_terms(self, box):
         """KE_box + P V - box work of path-integral NPT, kept on the device; P V before the
