@@ -153,3 +153,26 @@ def init_model(seed=0, coord=None, box=None, type_idx=None, rcut=6.0,
         'sr_mean': mean, 'sr_std': std, 'Nnbrs': nnbrs,
     }
     return params, {'params': P}
+
+
+def init_wannier_model(seed=0, coord=None, box=None, type_idx=None, rcut=6.0, embed_widths=(25, 50, 100),
+                       fit_widths=(100, 100, 100), axis=16, ntypes=2, nsel=(0,), fit_dt=0.1, out_norm=0.05):
+    """Seeded random-init atomic vector model (type 'atomic', a Wannier-centroid model as
+    deepmd_jax/train.py:121-150 builds it): embedding nets for the centre types in nsel
+    (net k = position in nsel * ntypes + neighbour type), one fitting net of hidden layers
+    per nsel type with fit_widths[-1] == embed_widths[-1], outputs scaled by out_norm (the
+    displacements stay a fraction of an Angstrom)."""
+    if fit_widths[-1] != embed_widths[-1]:
+        raise ValueError('For atomic models, embed_widths[-1] must equal fit_widths[-1].')
+    params, var = init_model(seed, coord, box, type_idx, rcut, embed_widths, fit_widths, axis, ntypes, fit_dt)
+    P = var['params']
+    out = {'Tbias': P['Tbias']}
+    for ci, i in enumerate(nsel):
+        for j in range(ntypes):
+            out['embedding_net_%d' % (ci * ntypes + j)] = P['embedding_net_%d' % (i * ntypes + j)]
+        net = dict(P['fitting_net_%d' % i])
+        net.pop('Dense_%d' % len(fit_widths))
+        out['fitting_net_%d' % ci] = net
+    params.update({'type': 'atomic', 'atomic': True, 'nsel': list(nsel), 'out_norm': float(out_norm),
+                   'atomic_data_prefix': 'atomic_dipole'})
+    return params, {'params': out}

@@ -34,6 +34,10 @@ Function -> reference file:line it follows:
   PIMDNPTOracle         deepmd_jax/pimd.py:297-518 (npt_langevin_pimd), invariant
                         deepmd_jax/md.py:540-557 (pinned by its conservation)
   energy_force_virial(mobile=) deepmd_jax/md.py:428-430 (stop-gradient on fixed atoms)
+  dp_atomic             deepmd_jax/dpmodel.py:83-98 (atomic vector model, Wannier centroids)
+  p3m_energy            deepmd_jax/utils.py:174-222 (PPPM, TSC assignment); pinned against
+                        the direct Ewald reciprocal sum (ewald_reciprocal)
+  dplr_energy_force     deepmd_jax/md.py:404-415, deepmd_jax/utils.py:355-366
 Pinning (tests/test_oracle.py): closed forms of the switch and of tensor_3to6,
 rotation / translation / permutation invariance, the T3=T6=0 outer-product
 reduction of G, all-zero-weight nets, autograd against central finite
@@ -188,12 +192,10 @@ def _type_sort(coord, type_idx):
     return coord[torch.as_tensor(perm)], perm
 
 
-def dp_energy(params, variables, coord, box3, type_idx, nbrs_nm=None, lattice=None, return_T=False,
-              strain=None):
-    """Energy of one frame.  coord: (N,3) torch tensor in caller order.
-    Either nbrs_nm[i][j] (type-local, padded) or lattice=(cand, lattice_max).
-    strain (3,3), if given, multiplies every minimum-image pair vector (the
-    perturbation the virial differentiates)."""
+def _descriptor(params, variables, coord, box3, type_idx, nbrs_nm, lattice, strain, centres, net_of):
+    """Per-atom T (rows, 10, W) and G (rows, A*W) for the centres of the listed types, rows
+    grouped by type in that order (stable within a type), plus the type-sorted offsets.
+    net_of(ci, j): embedding-net index of the ci-th listed centre type and neighbour type j."""
     P = variables['params']
     dtype = coord.dtype
     ntypes = params['ntypes']
@@ -211,7 +213,7 @@ def dp_energy(params, variables, coord, box3, type_idx, nbrs_nm=None, lattice=No
     std = np.asarray(params['sr_std'], np.float64)
     Y = ntypes
     T_list = []
-    for i in range(Y):
+    for ci, i in enumerate(centres):
         Ti = 0
         for j in range(Y):
             if nbrs_nm is not None:
@@ -228,11 +230,12 @@ def dp_energy(params, variables, coord, box3, type_idx, nbrs_nm=None, lattice=No
             R3 = 3 ** 0.5 * s_norm[:, None] * xn
             R6 = 3 * s_norm[:, None] * tensor_3to6(xn, axis=1, bias=1.0 / 3)
             RX = torch.cat([s_norm[:, None], R3, R6], dim=1)
+            k = net_of(ci, j)
             if params.get('is_compressed', False):
-                Ti = Ti + embedding_net_compressed(variables['compress_var']['embedding_net_%d' % (i * Y + j)],
+                Ti = Ti + embedding_net_compressed(variables['compress_var']['embedding_net_%d' % k],
                                                    s_cn[:, :, None], RX)
             else:
-                Ti = Ti + embedding_net(P['embedding_net_%d' % (i * Y + j)], s_cn[:, :, None], RX)
+                Ti = Ti + embedding_net(P['embedding_net_%d' % k], s_cn[:, :, None], RX)
         T_list.append(Ti)
     T = torch.cat(T_list, 0) / params['Nnbrs']
     TW = T[:, 0] + torch.as_tensor(P['Tbias'], dtype=dtype)
@@ -240,7 +243,20 @@ def dp_energy(params, variables, coord, box3, type_idx, nbrs_nm=None, lattice=No
     G = TW[:, None] * TW[:, :A, None] + (T3[:, :, None] * T3[:, :, :A, None]).sum(1)
     G2 = tensor_3to6(T3[:, :, A:2 * A], axis=1) + T6[:, :, A:2 * A]
     G = G + (G2[..., None] * T6[:, :, None]).sum(1)
-    G = G.reshape(G.shape[0], -1)
+    return T, G.reshape(G.shape[0], -1), type_count
+
+
+def dp_energy(params, variables, coord, box3, type_idx, nbrs_nm=None, lattice=None, return_T=False,
+              strain=None):
+    """Energy of one frame.  coord: (N,3) torch tensor in caller order.
+    Either nbrs_nm[i][j] (type-local, padded) or lattice=(cand, lattice_max).
+    strain (3,3), if given, multiplies every minimum-image pair vector (the
+    perturbation the virial differentiates)."""
+    P = variables['params']
+    Y = params['ntypes']
+    T, G, type_count = _descriptor(params, variables, coord, box3, type_idx, nbrs_nm, lattice, strain,
+                                   list(range(Y)), lambda ci, j: ci * Y + j)
+    off = np.concatenate([[0], np.cumsum(type_count)])
     e = []
     for t in range(Y):
         f = fitting_net(P['fitting_net_%d' % t], G[off[t]:off[t + 1]])
@@ -249,6 +265,128 @@ def dp_energy(params, variables, coord, box3, type_idx, nbrs_nm=None, lattice=No
     if return_T:
         return e.sum(), e, T, G
     return e.sum()
+
+
+def fitting_hidden(net, x):
+    """fitting_net(use_final=False): the hidden layers only (deepmd_jax/utils.py:153-166)."""
+    li = 0
+    while 'Dense_%d' % li in net:
+        x_prev = x
+        x = torch.tanh(_dense(net['Dense_%d' % li], x))
+        if li > 0 and x.shape[-1] == x_prev.shape[-1]:
+            x = x * torch.as_tensor(net['dt%d' % li], dtype=x.dtype) + x_prev
+        li += 1
+    return x
+
+
+def dp_atomic(params, variables, coord, box3, type_idx, nbrs_nm=None, lattice=None):
+    """Atomic vector model (params['type'] == 'atomic', deepmd_jax/dpmodel.py:83-98): for each
+    atom of the selected types params['nsel'], out_norm * sum_w f_w T3[:, w], f the last hidden
+    layer of the fitting net of its type; rows in raw atom order among the selected atoms
+    (atomic_inverse_perm, deepmd_jax/utils.py:380-390).  Embedding net k = (position of the
+    centre type in nsel) * ntypes + neighbour type; fitting net t = position in nsel."""
+    P = variables['params']
+    Y = params['ntypes']
+    nsel = list(params['nsel'])
+    T, G, type_count = _descriptor(params, variables, coord, box3, type_idx, nbrs_nm, lattice, None, nsel,
+                                   lambda ci, j: ci * Y + j)
+    cnt = [int(type_count[t]) for t in nsel]
+    off = np.concatenate([[0], np.cumsum(cnt)])
+    T3 = T[:, 1:4]
+    pred = []
+    for ci in range(len(nsel)):
+        f = fitting_hidden(P['fitting_net_%d' % ci], G[off[ci]:off[ci + 1]])
+        pred.append((f[:, None, :] * T3[off[ci]:off[ci + 1]]).sum(-1))
+    pred = torch.cat(pred) * params['out_norm']
+    type_idx = np.asarray(type_idx)
+    order = np.concatenate([np.where(type_idx == t)[0] for t in nsel])
+    return pred[torch.as_tensor(np.argsort(order, kind='stable'))]
+
+
+def p3m_energy(coord, q, box3, beta, M):
+    """PPPM long-range energy (deepmd_jax/utils.py:177-222): triangular-shaped-cloud charge
+    assignment onto an M[0] x M[1] x M[2] grid, FFT, E = sum_k K(k) |S(k)|^2 with
+    K = (14.399645 * 2 pi / V) exp(-k^2 / 4 beta^2) w(k)^2 / (S(k) k^2) (K(0) = 0),
+    w = prod sinc^3 of k h / 2, S = (prod (1 - sin^2 + 2/15 sin^4))^2.  Differentiable in
+    coord (torch, float64)."""
+    dtype = torch.float64
+    coord = coord.to(dtype)
+    q = torch.as_tensor(np.asarray(q, np.float64), dtype=dtype)
+    box3 = torch.as_tensor(np.asarray(box3, np.float64), dtype=dtype)
+    M3 = torch.as_tensor(np.asarray(M), dtype=dtype)
+    kint = torch.stack(torch.meshgrid(*[torch.arange(m, dtype=dtype) for m in M], indexing='ij'))
+    MM = M3.reshape(3, 1, 1, 1)
+    kint = torch.remainder(kint - MM / 2, MM) - MM / 2
+    kgrid = 2 * np.pi / box3[:, None, None, None] * kint
+    k2 = (kgrid ** 2).sum(0)
+    z = kgrid * (box3 / M3)[:, None, None, None] / 2
+    zs = torch.where(z != 0, z, torch.ones_like(z))
+    w3k = torch.prod(torch.where(z != 0, (torch.sin(zs) / zs) ** 3, torch.ones_like(z)), 0)
+    sz = torch.sin(z)
+    Sk = torch.prod(1 - sz ** 2 + 2.0 / 15 * sz ** 4, 0) ** 2
+    k2s = torch.where(k2 != 0, k2, torch.ones_like(k2))
+    kf = (14.399645 * 2 * np.pi / torch.prod(box3)) * torch.exp(-k2 / (4 * beta ** 2)) * w3k ** 2 / (Sk * k2s)
+    kf = torch.where(k2 != 0, kf, torch.zeros_like(kf))
+    u = (torch.remainder(coord, box3) / box3 * M3).T                 # (3, N)
+    ci = torch.round(u.detach())
+    r = u - ci
+    fr = torch.stack([(r - 0.5) ** 2 / 2, 0.75 - r ** 2, (r + 0.5) ** 2 / 2])   # (3 offsets, 3 dims, N)
+    Mi = [int(m) for m in M]
+    grid = torch.zeros(Mi[0] * Mi[1] * Mi[2], dtype=dtype)
+    cidx = ci.to(torch.long)
+    for a in range(3):
+        for b in range(3):
+            for c in range(3):
+                w = fr[a, 0] * fr[b, 1] * fr[c, 2] * q
+                ix = torch.remainder(cidx[0] + a - 1, Mi[0])
+                iy = torch.remainder(cidx[1] + b - 1, Mi[1])
+                iz = torch.remainder(cidx[2] + c - 1, Mi[2])
+                grid = grid.index_add(0, (ix * Mi[1] + iy) * Mi[2] + iz, w)
+    S = torch.fft.fftn(grid.reshape(Mi))
+    return (kf * (S * S.conj()).real).sum()
+
+
+def p3m_grid(box3, beta, resolution=5):
+    """Grid size of get_p3mlr_grid_size (deepmd_jax/utils.py:174-175)."""
+    return tuple(int(v) for v in (np.asarray(box3, np.float64) * beta * resolution).astype(int))
+
+
+def ewald_reciprocal(coord, q, box3, beta, kmax):
+    """Direct Ewald reciprocal-space energy (14.399645 * 2 pi / V) sum_{0<|n|<=kmax}
+    exp(-k^2 / 4 beta^2) / k^2 |sum_i q_i exp(i k.r_i)|^2, k = 2 pi n / L: the limit the PPPM
+    energy approaches as its grid is refined (a closed-form check of p3m_energy)."""
+    coord = np.asarray(coord, np.float64)
+    q = np.asarray(q, np.float64)
+    box3 = np.asarray(box3, np.float64)
+    n = np.arange(-kmax, kmax + 1)
+    nv = np.stack(np.meshgrid(n, n, n, indexing='ij'), -1).reshape(-1, 3)
+    nv = nv[np.any(nv != 0, 1)]
+    k = 2 * np.pi * nv / box3
+    k2 = (k ** 2).sum(1)
+    S = (q[None] * np.exp(1j * k @ coord.T)).sum(1)
+    return float((14.399645 * 2 * np.pi / np.prod(box3)) * (np.exp(-k2 / (4 * beta ** 2)) / k2 * np.abs(S) ** 2).sum())
+
+
+def dplr_energy_force(params, variables, wc_params, wc_variables, coord, box3, type_idx, q_atoms, q_wc, beta, M,
+                      nbrs_nm=None, wc_nbrs_nm=None, lattice=None):
+    """DPLR energy (deepmd_jax/md.py:404-415): the short-range model's energy plus the PPPM
+    energy of the atom charges q_atoms[type] at the atoms and q_wc[position of the type in
+    nsel] at the Wannier centroids coord + dp_atomic(coord) (dplr_charges,
+    deepmd_jax/utils.py:355-366); forces by autograd through the centroid model."""
+    x = torch.as_tensor(np.asarray(coord, np.float64)).clone().requires_grad_(True)
+    type_idx = np.asarray(type_idx)
+    E_sr = dp_energy(params, variables, x, box3, type_idx, nbrs_nm=nbrs_nm, lattice=lattice)
+    nsel = list(wc_params['nsel'])
+    sel = np.isin(type_idx, nsel)
+    wc = x[torch.as_tensor(sel)] + dp_atomic(wc_params, wc_variables, x, box3, type_idx, nbrs_nm=wc_nbrs_nm,
+                                             lattice=lattice)
+    qa = np.asarray(q_atoms, np.float64)[type_idx]
+    pos = {t: k for k, t in enumerate(nsel)}
+    qw = np.asarray(q_wc, np.float64)[[pos[t] for t in type_idx[sel]]]
+    E_lr = p3m_energy(torch.cat([x, wc]), np.concatenate([qa, qw]), box3, beta, M)
+    E = E_sr + E_lr
+    g, = torch.autograd.grad(E, [x])
+    return E.item(), E_lr.item(), -g.numpy(), wc.detach().numpy()
 
 
 def energy_force_virial(params, variables, coord, box3, type_idx, nbrs_nm=None, lattice=None,
