@@ -13,7 +13,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, 'csrc')
 OUT_DIR = os.path.join(HERE, 'deepmd_jax')
 BUILD_DIR = os.path.join(HERE, '_build')
-SOURCES = ['api.hip', 'embed.hip', 'fit.hip', 'nlist.hip', 'md.hip']
+SOURCES = ['api.hip', 'embed.hip', 'fit.hip', 'nlist.hip', 'md.hip', 'pppm.hip']
 LIB_NAME = 'libdpmd_hip.so'
 
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
@@ -40,7 +40,8 @@ def build(jobs=4, verbose=True):
         objs = list(ex.map(_compile, SOURCES))
     lib = os.path.join(OUT_DIR, LIB_NAME)
     tmp = lib + '.tmp'
-    cmd = [HIPCC, '-shared', '--offload-arch=gfx950', '-o', tmp] + objs
+    cmd = [HIPCC, '-shared', '--offload-arch=gfx950', '-o', tmp] + objs + ['-L/opt/rocm/lib', '-lhipfft',
+                                                                            '-Wl,-rpath,/opt/rocm/lib']
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError('link failed:\n%s\n%s' % (' '.join(cmd), r.stderr[-8000:]))

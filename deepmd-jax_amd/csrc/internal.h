@@ -5,9 +5,34 @@
 #include <vector>
 #include "dpmd.h"
 
+struct hipfftHandle_t;
+
 namespace dpmd {
 
 void set_error(const char* fmt, ...);
+
+// PPPM long-range electrostatics on one device (pppm.hip): grid, half spectrum, influence
+// function and hipFFT plans for a grid size; set-up recomputes the influence function when
+// the box or beta change and reallocates when the grid size changes.
+struct Pppm {
+  int M[3] = {0, 0, 0};
+  double box[3] = {0.0, 0.0, 0.0};
+  double beta = 0.0;
+  hipfftHandle_t* fwd = nullptr;
+  hipfftHandle_t* inv = nullptr;
+  double* grid = nullptr;   // M0 M1 M2 charges, then the potential gradient phi
+  void* spec = nullptr;     // M0 M1 (M2/2+1) complex doubles
+  double* kfac = nullptr;   // influence function over the half spectrum
+  double* part = nullptr;   // energy partials
+  Pppm() = default;
+  Pppm(const Pppm&) = delete;
+  Pppm& operator=(const Pppm&) = delete;
+  ~Pppm();
+  void release();
+  int32_t setup(hipStream_t st, const int* grid_size, const double* box3, double beta);
+  // energy (device scalar) of n charges xyz (n x 3) q (n); -dE/dx added to force (n x 3)
+  int32_t compute(hipStream_t st, const double* xyz, const double* q, int n, double* d_energy, double* force);
+};
 
 #define DPMD_HIP(x)                                                                    \
   do {                                                                                 \

@@ -90,6 +90,7 @@ def lib():
         'dpmd_npt_state_get': (c_i32, [c_vp, c_vp]),
         'dpmd_nlist_rebuilds': (c_i32, [c_vp, c_vp]),
         'dpmd_system_set_fixed': (c_i32, [c_vp, c_vp]),
+        'dpmd_pppm_compute': (c_i32, [c_i32, c_vp, c_f64, c_vp, c_i32, c_vp, c_vp, c_vp, c_vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -125,7 +126,23 @@ def exported_symbols():
             'dpmd_md_run', 'dpmd_pimd_setup', 'dpmd_kinetic_energy', 'dpmd_thermostat_energy',
             'dpmd_thermostat_state_get', 'dpmd_thermostat_state_set', 'dpmd_timing_enable',
             'dpmd_timing_get', 'dpmd_pair_count', 'dpmd_synchronize', 'dpmd_system_get_boxes',
-            'dpmd_md_box_trajectory', 'dpmd_npt_state_get', 'dpmd_nlist_rebuilds', 'dpmd_system_set_fixed']
+            'dpmd_md_box_trajectory', 'dpmd_npt_state_get', 'dpmd_nlist_rebuilds', 'dpmd_system_set_fixed',
+            'dpmd_pppm_compute']
+
+
+def pppm_compute(box3, beta, grid, xyz, q, device=0):
+    """PPPM energy (eV) and forces (n, 3) of point charges q (e) at xyz (Angstrom) in an
+    orthorhombic box on the GPU (get_p3mlr_fn, deepmd_jax/utils.py:177-222)."""
+    b = np.ascontiguousarray(np.asarray(box3, np.float64).reshape(3))
+    g = np.ascontiguousarray(np.asarray(grid, np.int32).reshape(3))
+    x = np.ascontiguousarray(np.asarray(xyz, np.float64).reshape(-1, 3))
+    qq = np.ascontiguousarray(np.asarray(q, np.float64).reshape(-1))
+    if len(qq) != len(x):
+        raise ValueError('xyz and q disagree in length')
+    e = np.zeros(1)
+    f = np.zeros_like(x)
+    check(lib().dpmd_pppm_compute(int(device), ptr(b), float(beta), ptr(g), len(x), ptr(x), ptr(qq), ptr(e), ptr(f)))
+    return float(e[0]), f
 
 
 class DeviceModel:

@@ -235,6 +235,12 @@ int32_t dpmd_system_get_boxes(dpmd_system_t sys, double* boxes);
 /* Per-step boxes (steps, nseg, 3) of the last dpmd_md_run that recorded a trajectory: the
  * box trajectory of Simulation.run (deepmd_jax/md.py:846,924). */
 int32_t dpmd_md_box_trajectory(dpmd_system_t sys, double* boxes);
+/* PPPM long-range energy and forces of n point charges in an orthorhombic periodic box, the
+ * get_p3mlr_fn of deepmd_jax/utils.py:177-222 (TSC assignment onto grid[0] x grid[1] x grid[2],
+ * E = sum_k K(k)|S(k)|^2 in eV for charges in e and lengths in Angstrom), evaluated on GPU
+ * `device` in FP64.  xyz (n x 3), q (n); force (n x 3, -dE/dx) may be NULL. */
+int32_t dpmd_pppm_compute(int32_t device, const double* box3, double beta, const int32_t* grid, int32_t n,
+                          const double* xyz, const double* q, double* energy, double* force);
 /* NPT barostat state (64 doubles): [0..3) log box scale per group, [3..6) box momenta,
  * [6..9) box masses, [9..12) dU/deps per group, [12..15) reference box, [16..24) chain
  * positions, [24..32) chain momenta, [32..40) chain masses, [43] barostat invariant terms
