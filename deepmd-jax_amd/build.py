@@ -13,7 +13,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, 'csrc')
 OUT_DIR = os.path.join(HERE, 'deepmd_jax')
 BUILD_DIR = os.path.join(HERE, '_build')
-SOURCES = ['api.hip', 'embed.hip', 'fit.hip', 'nlist.hip', 'md.hip', 'pppm.hip']
+SOURCES = ['api.hip', 'embed.hip', 'fit.hip', 'nlist.hip', 'md.hip', 'pppm.hip', 'dplr.hip']
 LIB_NAME = 'libdpmd_hip.so'
 
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')

@@ -58,6 +58,7 @@ System::~System() {
   for (hipEvent_t e : ev_frame) hipEventDestroy(e);
   for (hipEvent_t e : ev_copied) hipEventDestroy(e);
   if (h_traj) hipHostFree(h_traj);
+  delete pppm;
   if (copy_stream) hipStreamDestroy(copy_stream);
   if (stream) hipStreamDestroy(stream);
 }
@@ -205,13 +206,21 @@ static int32_t model_create(const dpmd_model_desc* d, int device, Model** out) {
     }
   M->fit_total = pos;
   std::vector<float> fit(pos, 0.f);
+  M->atomic = d->atomic != 0;
+  M->nsel_mask = d->nsel_mask;
+  if (M->atomic && (d->fit_widths[2] != M->wout || d->fit_widths[0] != d->fit_widths[2])) {
+    set_error("atomic models need fit_widths[2] == embed_widths[2] (and equal hidden widths)");
+    delete M;
+    return DPMD_E_UNSUPPORTED;
+  }
+  const std::vector<float> no_head(fw, 0.f);  // atomic models have no energy head
   for (int t = 0; t < d->ntypes; ++t) {
-    for (int l = 0; l < 4; ++l)
+    for (int l = 0; l < (M->atomic ? 3 : 4); ++l)
       if (!d->fit_kernel[t][l]) { set_error("fitting net %d layer %d missing", t, l); delete M; return DPMD_E_ARG; }
     const float* K0 = d->fit_kernel[t][0];
     const float* K1 = d->fit_kernel[t][1];
     const float* K2 = d->fit_kernel[t][2];
-    const float* K3 = d->fit_kernel[t][3];
+    const float* K3 = M->atomic ? no_head.data() : d->fit_kernel[t][3];
     float* o[FNB];
     for (int b = 0; b < FNB; ++b) o[b] = fit.data() + M->fit_off[t][b];
     for (int k = 0; k < D; ++k)
@@ -234,7 +243,7 @@ static int32_t model_create(const dpmd_model_desc* d, int device, Model** out) {
       o[FDT2][n] = d->fit_dt[t][2] ? d->fit_dt[t][2][n] : 1.f;
       o[FK3][n] = (float)(K3[n] * d->out_norm);
     }
-    const float b3 = d->fit_bias[t][3] ? d->fit_bias[t][3][0] : 0.f;
+    const float b3 = !M->atomic && d->fit_bias[t][3] ? d->fit_bias[t][3][0] : 0.f;
     o[FB3][0] = (float)(b3 * d->out_norm);
     M->fit_b3[t] = (float)(b3 * d->out_norm);
   }
@@ -414,6 +423,7 @@ static int32_t sys_layout(System* s, int nseg, const int32_t* seg_natoms, const 
   s->npt_ready = false;
   s->has_fixed = false;
   s->n_mobile = NA;
+  s->wc_model = nullptr;  // DPLR maps belong to the old layout
   return DPMD_OK;
 }
 
@@ -449,6 +459,8 @@ static int32_t model_forces_impl(System* s, int precision) {
   timer_begin(s, 3, &ev);
   DPMD_TRY(embed_launch(s, true, precision));
   timer_end(s, 3, ev);
+  // DPLR long-range term (centroid model FP32 whatever the energy model's precision)
+  if (s->wc_model) DPMD_TRY(dplr_forces(s));
   // fixed atoms: their coordinates carry no gradient (deepmd_jax/md.py:428-430)
   if (s->has_fixed) DPMD_TRY(md_mask_forces(s));
   return DPMD_OK;
@@ -563,7 +575,7 @@ static int32_t compute_impl(System* s, int precision, double* energy, double* fo
   for (int r = 0; r < s->nseg; ++r) {
     if (energy) energy[r] = red[r * 16] + seg_ebias(s, r);
     if (virial)
-      for (int k = 0; k < 9; ++k) virial[r * 9 + k] = red[r * 16 + 1 + k];
+      for (int k = 0; k < 9; ++k) virial[r * 9 + k] = s->wc_model ? NAN : red[r * 16 + 1 + k];
   }
   if (force) {
     std::vector<double> fd(NA * 3);
@@ -905,6 +917,10 @@ int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps, 
   }
   const bool npt = p->routine == DPMD_NPT_NH;
   const bool pnpt = p->routine == DPMD_PIMD_NPT;
+  if (s->wc_model && (npt || pnpt)) {
+    set_error("DPLR forms no virial: NPT is not supported with it");
+    return DPMD_E_UNSUPPORTED;
+  }
   NptArgs na;
   memset(&na, 0, sizeof(na));
   if (npt || pnpt) DPMD_TRY(npt_args(s, p, na));

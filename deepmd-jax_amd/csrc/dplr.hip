@@ -1,0 +1,257 @@
+// DPLR: the long-range electrostatics of deepmd_jax/md.py:404-415 on the device.
+//
+// A Wannier-centroid model (an atomic vector model, deepmd_jax/dpmodel.py:83-98) predicts
+// for each atom of its selected types a displacement dx = out_norm sum_w f_w T3[:, w]; the
+// PPPM energy (pppm.hip) of charge q_atom at every atom and q_wc at every centroid x + dx is
+// added to the energy.  Forces: the PPPM force on an atom's charge and on its centroid's
+// charge act on the atom directly (d(x + dx)/dx = 1 + d(dx)/dx), and the remaining term
+// -(d dx / dx)^T g with g = dE/d(centroid) is the backward of the centroid model for the
+// upstream gradient g: through the hidden fitting layers into dE/dG, plus the direct
+// dE/dT3 = out_norm g f, into the embedding backward (which accumulates the forces).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "internal.h"
+
+namespace dpmd {
+
+namespace {
+
+struct RowTypes {
+  int rowbase[kMaxT + 1];
+  int ntypes;
+  const float* dt2[kMaxT];
+};
+
+// charge positions: the atoms, then the centroids x + out_norm sum_w f_w T3[c][w]
+__global__ void dplr_charges_kernel(const double* pos, const float* T, const float* Y3, int ldy, int W,
+                                    float out_norm, const int* wc_atom, const int* wc_row, int NA, int nwc,
+                                    double* qxyz) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < NA * 3) {
+    qxyz[t] = pos[t];
+    return;
+  }
+  const int u = t - NA * 3;
+  if (u >= nwc * 3) return;
+  const int k = u / 3, c = u - 3 * k;
+  const int g = wc_atom[k];
+  const float* y = Y3 + (size_t)wc_row[k] * ldy;
+  const float* t3 = T + ((size_t)g * 10 + 1 + c) * W;
+  float s = 0.f;
+  for (int w = 0; w < W; ++w) s += y[w] * t3[w];
+  qxyz[(size_t)(NA + k) * 3 + c] = pos[(size_t)g * 3 + c] + (double)(out_norm * s);
+}
+
+// forces of the atom and centroid charges onto the atoms; g = dE/d(centroid) per atom
+__global__ void dplr_scatter_kernel(const double* qf, const int* atom_wc, int NA, float* force, float* wc_g) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= NA * 3) return;
+  const int g = t / 3, c = t - 3 * g;
+  const int k = atom_wc[g];
+  double f = qf[t];
+  float gw = 0.f;
+  if (k >= 0) {
+    const double fw = qf[(size_t)(NA + k) * 3 + c];
+    f += fw;
+    gw = (float)(-fw);
+  }
+  force[t] += (float)f;
+  wc_g[t] = gw;
+}
+
+// dE/df[row][w] = out_norm sum_c g[c] T3[c][w] (DY3) and DZ3 = DY3 dt2 (1 - t3^2) for the
+// last (residual) hidden layer
+__global__ void dplr_dy3_kernel(const float* wc_g, const float* T, const float* T3l, int ldy, int W, float out_norm,
+                                const int* wc_atom, const int* wc_row, int nwc, RowTypes rt, float* DY3, float* DZ3) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nwc * W) return;
+  const int k = t / W, w = t - W * k;
+  const int g = wc_atom[k], row = wc_row[k];
+  int ty = 0;
+  while (ty + 1 < rt.ntypes && row >= rt.rowbase[ty + 1]) ++ty;
+  const float* gg = wc_g + (size_t)g * 3;
+  const float* Tg = T + (size_t)g * 10 * W;
+  const float d = out_norm * (gg[0] * Tg[W + w] + gg[1] * Tg[2 * W + w] + gg[2] * Tg[3 * W + w]);
+  const size_t o = (size_t)row * ldy + w;
+  const float th = T3l[o];
+  DY3[o] = d;
+  DZ3[o] = d * rt.dt2[ty][w] * (1.f - th * th);
+}
+
+__global__ void dplr_add_energy_kernel(double* red, const double* lr) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) red[0] += lr[0];
+}
+
+}  // namespace
+
+int32_t dplr_add_energy(System* s) {
+  hipLaunchKernelGGL(dplr_add_energy_kernel, dim3(1), dim3(64), 0, s->stream, s->d_red, s->d_lr);
+  DPMD_HIP(hipGetLastError());
+  return DPMD_OK;
+}
+
+static int32_t dplr_run(System* s, Model* M) {
+  const int NA = s->NA, nwc = s->n_wc, W = M->wout;
+  const hipStream_t st = s->stream;
+  // centroid model: descriptor, hidden layers, centroids
+  DPMD_TRY(embed_launch(s, false, DPMD_PREC_FP32));
+  AtomicFitBufs b;
+  DPMD_TRY(fit_atomic_buffers(s, M, b));
+  DPMD_TRY(fit_atomic_fwd(s, b));
+  const int nq = NA + nwc;
+  hipLaunchKernelGGL(dplr_charges_kernel, dim3((nq * 3 + 255) / 256), dim3(256), 0, st, s->d_pos, s->d_T, b.Y3,
+                     b.FK, W, (float)M->out_norm, s->d_wc_atom, s->d_wc_row, NA, nwc, s->d_qxyz);
+  // long-range energy and the forces on every charge
+  DPMD_TRY(s->pppm->setup(st, s->dplr_grid, s->seg[0].boxd, s->dplr_beta));
+  DPMD_HIP(hipMemsetAsync(s->d_qf, 0, sizeof(double) * 3 * nq, st));
+  DPMD_TRY(s->pppm->compute(st, s->d_qxyz, s->d_q, nq, s->d_lr, s->d_qf));
+  hipLaunchKernelGGL(dplr_scatter_kernel, dim3((NA * 3 + 255) / 256), dim3(256), 0, st, s->d_qf, s->d_atom_wc, NA,
+                     s->d_force, s->d_wc_g);
+  // backward of the centroid model for the upstream gradient g
+  RowTypes rt;
+  memset(&rt, 0, sizeof(rt));
+  rt.ntypes = s->ntypes;
+  for (int t = 0; t <= s->ntypes; ++t) rt.rowbase[t] = s->rowbase[t];
+  for (int t = 0; t < s->ntypes; ++t) rt.dt2[t] = M->d_fit + M->fit_off[t][10];  // FDT2 block
+  if (nwc > 0)
+    hipLaunchKernelGGL(dplr_dy3_kernel, dim3((nwc * W + 255) / 256), dim3(256), 0, st, s->d_wc_g, s->d_T, b.T3, b.FK,
+                       W, (float)M->out_norm, s->d_wc_atom, s->d_wc_row, nwc, rt, b.DY3, b.DZ3);
+  DPMD_HIP(hipGetLastError());
+  DPMD_TRY(fit_atomic_bwd(s, b));
+  s->wc_g = s->d_wc_g;
+  s->wc_y3 = b.Y3;
+  s->wc_ldy = b.FK;
+  s->wc_norm = (float)M->out_norm;
+  s->partial_out = s->d_partial_wc;
+  const int32_t rc = embed_launch(s, true, DPMD_PREC_FP32);
+  s->wc_g = nullptr;
+  s->wc_y3 = nullptr;
+  s->partial_out = nullptr;
+  return rc;
+}
+
+int32_t dplr_forces(System* s) {
+  Model* sr = s->model;
+  s->model = s->wc_model;
+  const int32_t rc = dplr_run(s, s->wc_model);
+  s->model = sr;
+  s->wc_g = nullptr;
+  s->wc_y3 = nullptr;
+  s->partial_out = nullptr;
+  return rc;
+}
+
+}  // namespace dpmd
+
+using namespace dpmd;
+
+#define SYS(h)                                             \
+  System* s = reinterpret_cast<System*>(h);                \
+  if (!s) {                                                \
+    set_error("null system");                              \
+    return DPMD_E_ARG;                                     \
+  }                                                        \
+  DPMD_HIP(hipSetDevice(s->model->device));
+
+extern "C" int32_t dpmd_system_set_dplr(dpmd_system_t sys, dpmd_model_t wc_model, const double* q_atoms,
+                                        const double* q_wc, double beta, const int32_t* grid) {
+  SYS(sys);
+  s->force_valid = false;
+  if (!wc_model) {
+    s->wc_model = nullptr;
+    return DPMD_OK;
+  }
+  Model* W = reinterpret_cast<Model*>(wc_model);
+  Model* M = s->model;
+  if (!q_atoms || !q_wc || !grid) { set_error("null argument"); return DPMD_E_ARG; }
+  if (!W->atomic) { set_error("the DPLR centroid model must be an atomic model"); return DPMD_E_ARG; }
+  if (W->device != M->device) { set_error("the centroid model lives on another device"); return DPMD_E_ARG; }
+  if (W->ntypes != M->ntypes || W->wout != M->wout || W->axis != M->axis || W->lay.tiles[0] != M->lay.tiles[0] ||
+      W->lay.tiles[1] != M->lay.tiles[1] || W->lay.tiles[2] != M->lay.tiles[2] || W->lay.z[1] != M->lay.z[1] ||
+      W->lay.z[2] != M->lay.z[2]) {
+    set_error("the centroid model needs the energy model's types, embedding widths and axis");
+    return DPMD_E_UNSUPPORTED;
+  }
+  if (W->tab_n > 0) { set_error("tabulated centroid models are not supported"); return DPMD_E_UNSUPPORTED; }
+  if (W->rcut > M->rcut) {
+    set_error("the centroid model's cutoff %.3f exceeds the energy model's %.3f (shared neighbour list)", W->rcut,
+              M->rcut);
+    return DPMD_E_UNSUPPORTED;
+  }
+  if (s->nseg != 1 || s->seg[0].nimg != 1) {
+    set_error("DPLR runs one system held without periodic images");
+    return DPMD_E_UNSUPPORTED;
+  }
+  if (!(beta > 0)) { set_error("DPLR needs beta > 0"); return DPMD_E_ARG; }
+  for (int d = 0; d < 3; ++d)
+    if (grid[d] < 2 || grid[d] > 4096) { set_error("PPPM grid size %d outside [2, 4096]", grid[d]); return DPMD_E_ARG; }
+  const int NA = s->NA;
+  const SegInfo& S = s->seg[0];
+  std::vector<int> wc_atom, wc_row, atom_wc(NA, -1);
+  std::vector<double> q;
+  for (int g = 0; g < NA; ++g) q.push_back(q_atoms[s->type_sorted[g]]);
+  for (int g = 0; g < NA; ++g) {
+    const int t = s->type_sorted[g];
+    if (!(W->nsel_mask >> t & 1)) continue;
+    atom_wc[g] = (int)wc_atom.size();
+    wc_atom.push_back(g);
+    wc_row.push_back(s->rowbase[t] + S.row[t] + (g - S.base - S.off[t]));
+    q.push_back(q_wc[t]);
+  }
+  const int nwc = (int)wc_atom.size();
+  // centroid k -> rank among the selected atoms in caller order (wc_predict's order)
+  std::vector<int> order(nwc);
+  for (int k = 0; k < nwc; ++k) order[k] = k;
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return s->perm[wc_atom[a]] < s->perm[wc_atom[b]]; });
+  s->wc_perm.assign(nwc, 0);
+  for (int r = 0; r < nwc; ++r) s->wc_perm[order[r]] = r;
+  const int nq = NA + nwc;
+  const size_t nn = std::max(nwc, 1);
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_wc_atom, sizeof(int) * nn));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_wc_row, sizeof(int) * nn));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_atom_wc, sizeof(int) * NA));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_qxyz, sizeof(double) * 3 * nq));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_q, sizeof(double) * nq));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_qf, sizeof(double) * 3 * nq));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_lr, sizeof(double)));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_wc_g, sizeof(float) * 3 * NA));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_partial_wc, sizeof(double) * 9 * (size_t)std::max(s->nblocks, 1)));
+  const size_t fitbytes = sizeof(float) * 10 * (size_t)std::max(s->mrows, 1) * W->fit_k;
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_wc_fit, fitbytes));
+  const hipStream_t st = s->stream;
+  DPMD_HIP(hipMemsetAsync(s->d_wc_fit, 0, fitbytes, st));
+  if (nwc > 0) {
+    DPMD_HIP(hipMemcpyAsync(s->d_wc_atom, wc_atom.data(), sizeof(int) * nwc, hipMemcpyHostToDevice, st));
+    DPMD_HIP(hipMemcpyAsync(s->d_wc_row, wc_row.data(), sizeof(int) * nwc, hipMemcpyHostToDevice, st));
+  }
+  DPMD_HIP(hipMemcpyAsync(s->d_atom_wc, atom_wc.data(), sizeof(int) * NA, hipMemcpyHostToDevice, st));
+  DPMD_HIP(hipMemcpyAsync(s->d_q, q.data(), sizeof(double) * nq, hipMemcpyHostToDevice, st));
+  DPMD_HIP(hipMemsetAsync(s->d_lr, 0, sizeof(double), st));
+  if (!s->pppm) s->pppm = new Pppm();
+  for (int d = 0; d < 3; ++d) s->dplr_grid[d] = grid[d];
+  s->dplr_beta = beta;
+  DPMD_TRY(s->pppm->setup(st, s->dplr_grid, S.boxd, beta));
+  DPMD_HIP(hipStreamSynchronize(st));
+  s->n_wc = nwc;
+  s->wc_model = W;
+  return DPMD_OK;
+}
+
+extern "C" int32_t dpmd_dplr_get(dpmd_system_t sys, double* e_lr, double* centroids) {
+  SYS(sys);
+  if (!s->wc_model) { set_error("DPLR is not set on this system"); return DPMD_E_STATE; }
+  const int NA = s->NA, nwc = s->n_wc;
+  if (e_lr) DPMD_HIP(hipMemcpyAsync(e_lr, s->d_lr, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  std::vector<double> c((size_t)3 * std::max(nwc, 1));
+  if (centroids && nwc > 0)
+    DPMD_HIP(hipMemcpyAsync(c.data(), s->d_qxyz + (size_t)NA * 3, sizeof(double) * 3 * nwc, hipMemcpyDeviceToHost,
+                            s->stream));
+  DPMD_HIP(hipStreamSynchronize(s->stream));
+  if (centroids)
+    for (int k = 0; k < nwc; ++k)
+      for (int d = 0; d < 3; ++d) centroids[(size_t)s->wc_perm[k] * 3 + d] = c[(size_t)k * 3 + d];
+  return DPMD_OK;
+}

@@ -183,6 +183,11 @@ struct EmbedArgs {
   const float* tab;            // tabulated nets [net][interval][6][16*T3] (compressed model)
   int tab_n, tab_stride, tab_bytes;
   float tab_srmin[kMaxT * kMaxT], tab_delta[kMaxT * kMaxT];
+  // backward of an atomic (centroid) model: dE/dT3[c][w] += wc_norm wc_g[atom][c] wc_y3[row][w]
+  const float* wc_g;           // (NA, 3) or null
+  const float* wc_y3;          // (mrows, wc_ldy)
+  int wc_ldy;
+  float wc_norm;
 };
 
 // Minimum image of d = x_j - x_i for positions wrapped into [0, L) (so |d| < L): the
@@ -781,6 +786,14 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
           const int rr = w - a.base3[qq];
           if (rr >= 0 && rr < a.size3[qq]) slot = 16 * (rr / 4) + 4 * qq + (rr % 4);
         }
+        if (a.wc_g) {
+          // the centroid output out_norm sum_w f_w T3[c][w] depends on T3 directly
+          const float y = a.wc_norm * a.wc_y3[(size_t)grow * a.wc_ldy + w];
+          const float* gg = a.wc_g + (size_t)gc * 3;
+          v[1] += gg[0] * y;
+          v[2] += gg[1] * y;
+          v[3] += gg[2] * y;
+        }
         if (slot >= 0) {
 #pragma unroll
           for (int x = 0; x < 10; ++x) dTp[x * HS + slot] = v[x] * a.inv_nnbrs;
@@ -1083,7 +1096,11 @@ static void fill_args(System* s, EmbedArgs& a) {
   a.G = (float*)s->d_G;
   a.dG = s->d_dG;
   a.force = s->d_force;
-  a.partial = s->d_partial;
+  a.partial = s->partial_out ? s->partial_out : s->d_partial;
+  a.wc_g = s->wc_g;
+  a.wc_y3 = s->wc_y3;
+  a.wc_ldy = s->wc_ldy;
+  a.wc_norm = s->wc_norm;
   a.paircnt = reinterpret_cast<int*>(s->d_pairs);
   int cmax = 1;
   for (int j = 0; j < s->ntypes; ++j) cmax = std::max(cmax, s->cap[j]);

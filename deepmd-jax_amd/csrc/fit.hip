@@ -23,7 +23,7 @@ typedef unsigned int uintx4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
 
-enum FitEpi { EPI_TANH = 0, EPI_RES = 1, EPI_E3 = 2, EPI_BWD2 = 3, EPI_BWD1 = 4, EPI_STORE = 5 };
+enum FitEpi { EPI_TANH = 0, EPI_RES = 1, EPI_E3 = 2, EPI_BWD2 = 3, EPI_BWD1 = 4, EPI_STORE = 5, EPI_BWD2A = 6 };
 
 struct GemmArgs {
   const float* A;   // M x K fp32, row stride lda
@@ -67,6 +67,12 @@ __device__ __forceinline__ float epi_one(const GemmArgs& g, size_t row, int col,
     return y3 * g.k3[col];
   } else if (EPI == EPI_BWD2) {
     const float dy2 = g.k3[col] + cv;
+    const float t2 = g.aux1[o];
+    g.C2[o] = dy2;
+    g.C[o] = dy2 * g.dt[col] * (1.f - t2 * t2);
+  } else if (EPI == EPI_BWD2A) {
+    // as EPI_BWD2 with a per-row upstream gradient aux2 in place of the energy head k3
+    const float dy2 = g.aux2[o] + cv;
     const float t2 = g.aux1[o];
     g.C2[o] = dy2;
     g.C[o] = dy2 * g.dt[col] * (1.f - t2 * t2);
@@ -358,6 +364,7 @@ static int32_t gemm_dispatch(hipStream_t st, const GemmArgs& g, int prec) {
   if (nt16 % 8 == 0) return go<8, EPI>(st, g, prec);
   if (nt16 % 12 == 0) return go<12, EPI>(st, g, prec);
   if (nt16 % 16 == 0) return go<16, EPI>(st, g, prec);
+  if (nt16 % 7 == 0) return go<7, EPI>(st, g, prec);
   set_error("gemm width %d has no tile instantiation", g.N);
   return DPMD_E_UNSUPPORTED;
 }
@@ -431,6 +438,94 @@ int32_t fit_launch(System* s, int precision) {
     g.N = D;
     g.C = s->d_dG + r0 * D; g.ldc = D;
     DPMD_TRY(gemm_dispatch<EPI_STORE>(s->stream, g, precision));
+  }
+  return DPMD_OK;
+}
+
+// ---------------------------------------------------------------------------------
+// Atomic vector (Wannier-centroid) model: fitting_net(use_final=False) (deepmd_jax/utils.py:
+// 153-166) gives the last hidden layer f, the output is out_norm sum_w f_w T3[:, w]
+// (deepmd_jax/dpmodel.py:83-98).  The ten activation buffers share the fitting scratch.
+
+int32_t fit_atomic_buffers(System* s, const Model* M, AtomicFitBufs& b) {
+  const int FK = M->fit_k;
+  if (!s->d_wc_fit) { set_error("centroid model buffers missing"); return DPMD_E_STATE; }
+  const size_t act = (size_t)s->mrows * FK;
+  float* p = s->d_wc_fit;
+  float** dst[10] = {&b.Y1, &b.T2, &b.Y2, &b.T3, &b.Y3, &b.DY3, &b.DZ3, &b.DY2, &b.DZ2, &b.DZ1};
+  for (int k = 0; k < 10; ++k) *dst[k] = p + k * act;
+  b.FK = FK;
+  return DPMD_OK;
+}
+
+int32_t fit_atomic_fwd(System* s, const AtomicFitBufs& b) {
+  Model* M = s->model;
+  const int FN = M->fit_n, FK = b.FK;
+  const int D = M->axis * M->wout;
+  const float* Wf = M->d_fit;
+  for (int t = 0; t < s->ntypes; ++t) {
+    const int rows = s->rowbase[t + 1] - s->rowbase[t];
+    if (rows == 0 || !(M->nsel_mask >> t & 1)) continue;
+    const size_t r0 = s->rowbase[t];
+    const size_t* o = M->fit_off[t];
+    GemmArgs g;
+    memset(&g, 0, sizeof(g));
+    g.M = rows;
+    g.A = (const float*)s->d_G + r0 * D; g.lda = D; g.K = D;
+    g.Bt = Wf + o[FK0T]; g.ldb = D; g.N = FN;
+    g.C = b.Y1 + r0 * FK; g.ldc = FK; g.bias = Wf + o[FB0];
+    DPMD_TRY(gemm_dispatch<EPI_TANH>(s->stream, g, DPMD_PREC_FP32));
+    g.A = b.Y1 + r0 * FK; g.lda = FK; g.K = FK;
+    g.Bt = Wf + o[FK1T]; g.ldb = FK;
+    g.C = b.Y2 + r0 * FK; g.C2 = b.T2 + r0 * FK; g.aux1 = b.Y1 + r0 * FK;
+    g.bias = Wf + o[FB1]; g.dt = Wf + o[FDT1];
+    DPMD_TRY(gemm_dispatch<EPI_RES>(s->stream, g, DPMD_PREC_FP32));
+    g.A = b.Y2 + r0 * FK;
+    g.Bt = Wf + o[FK2T];
+    g.C = b.Y3 + r0 * FK; g.C2 = b.T3 + r0 * FK; g.aux1 = b.Y2 + r0 * FK;
+    g.bias = Wf + o[FB2]; g.dt = Wf + o[FDT2];
+    DPMD_TRY(gemm_dispatch<EPI_RES>(s->stream, g, DPMD_PREC_FP32));
+  }
+  return DPMD_OK;
+}
+
+// DZ3 (= DY3 dt2 (1 - t3^2), formed by the caller) -> dG of the selected types; the rows of
+// other types get dG = 0
+int32_t fit_atomic_bwd(System* s, const AtomicFitBufs& b) {
+  Model* M = s->model;
+  const int FK = b.FK;
+  const int D = M->axis * M->wout;
+  const float* Wf = M->d_fit;
+  for (int t = 0; t < s->ntypes; ++t) {
+    const int rows = s->rowbase[t + 1] - s->rowbase[t];
+    if (rows == 0) continue;
+    const size_t r0 = s->rowbase[t];
+    if (!(M->nsel_mask >> t & 1)) {
+      DPMD_HIP(hipMemsetAsync(s->d_dG + r0 * D, 0, sizeof(float) * rows * D, s->stream));
+      continue;
+    }
+    const size_t* o = M->fit_off[t];
+    GemmArgs g;
+    memset(&g, 0, sizeof(g));
+    g.M = rows;
+    g.lda = FK; g.K = FK; g.ldb = FK; g.N = M->fit_n; g.ldc = FK;
+    // dy2 = dy3 + dz3 K2^T, dz2 = dy2 dt1 (1 - t2^2)
+    g.A = b.DZ3 + r0 * FK;
+    g.Bt = Wf + o[FK2];
+    g.C = b.DZ2 + r0 * FK; g.C2 = b.DY2 + r0 * FK; g.aux1 = b.T2 + r0 * FK; g.aux2 = b.DY3 + r0 * FK;
+    g.dt = Wf + o[FDT1];
+    DPMD_TRY(gemm_dispatch<EPI_BWD2A>(s->stream, g, DPMD_PREC_FP32));
+    // dy1 = dy2 + dz2 K1^T, dz1 = dy1 (1 - y1^2)
+    g.A = b.DZ2 + r0 * FK;
+    g.Bt = Wf + o[FK1];
+    g.C = b.DZ1 + r0 * FK; g.C2 = nullptr; g.aux1 = b.DY2 + r0 * FK; g.aux2 = b.Y1 + r0 * FK;
+    DPMD_TRY(gemm_dispatch<EPI_BWD1>(s->stream, g, DPMD_PREC_FP32));
+    // dG = dz1 K0^T
+    g.A = b.DZ1 + r0 * FK;
+    g.Bt = Wf + o[FK0]; g.ldb = FK; g.K = FK;
+    g.N = D;
+    g.C = s->d_dG + r0 * D; g.ldc = D;
+    DPMD_TRY(gemm_dispatch<EPI_STORE>(s->stream, g, DPMD_PREC_FP32));
   }
   return DPMD_OK;
 }

@@ -113,6 +113,10 @@ struct Model {
   int tab_stride = 0;              // floats per net
   float tab_srmin[kMaxT * kMaxT], tab_delta[kMaxT * kMaxT];
   float* d_tab = nullptr;
+  // atomic vector model (Wannier centroids): hidden fitting layers only, output for the
+  // centre types in nsel_mask
+  bool atomic = false;
+  int nsel_mask = 0;
 };
 void embed_pack_table(const EmbedLayout& L, int ngrids, const float* coeff, float* out);
 
@@ -240,6 +244,32 @@ struct System {
   double* d_box_traj = nullptr; // per-step boxes of the last recorded chunk (steps, nseg, 3)
   int box_traj_steps = 0;
   uint64_t rng_counter = 0;
+  // DPLR (dpmd_system_set_dplr): centroid model, charges and PPPM of one segment.  Charge
+  // layout: the NA atoms in sorted order, then the n_wc centroids in sorted atom order.
+  Model* wc_model = nullptr;
+  Pppm* pppm = nullptr;
+  int n_wc = 0;
+  int* d_wc_atom = nullptr;     // (n_wc) sorted atom of each centroid
+  int* d_wc_row = nullptr;      // (n_wc) its fitting row
+  int* d_atom_wc = nullptr;     // (NA) centroid of each atom or -1
+  int dplr_grid[3] = {0, 0, 0};
+  double dplr_beta = 0.0;
+  double* d_qxyz = nullptr;     // (NA + n_wc, 3) charge positions
+  double* d_q = nullptr;        // (NA + n_wc) charges
+  double* d_qf = nullptr;       // (NA + n_wc, 3) PPPM forces
+  double* d_lr = nullptr;       // long-range energy of the last evaluation
+  float* d_wc_g = nullptr;      // (NA, 3) dE_lr / d(centroid) of each atom's centroid (0 if none)
+  double* d_partial_wc = nullptr;  // virial partials of the centroid backward (not used)
+  float* d_wc_fit = nullptr;    // 10 activation buffers (mrows x fit_k) of the centroid model,
+                                // zeroed once: columns past the fitting width stay zero
+  std::vector<int> wc_perm;     // centroid k -> rank among the selected atoms in caller order
+  // embedding-backward hooks, set only while the centroid model's backward runs: the extra
+  // dE/dT3[c][w] = wc_norm wc_g[atom][c] wc_y3[row][w], and the virial partial output
+  const float* wc_g = nullptr;
+  const float* wc_y3 = nullptr;
+  int wc_ldy = 0;
+  float wc_norm = 0.f;
+  double* partial_out = nullptr;
   Timer timer;
   std::vector<std::pair<void**, size_t>> caps;  // allocation capacities (grow-only buffers)
   ~System();
@@ -256,6 +286,17 @@ int32_t nlist_alloc_cells(System* s);
 int32_t nlist_clear_flags(System* s);
 int32_t embed_launch(System* s, bool backward, int precision);
 int32_t fit_launch(System* s, int precision);
+// atomic (centroid) model hidden layers: forward into the activation buffers, and the backward
+// from dE/d(last hidden layer) (buffer DY3) to dG; buffers laid out by fit_atomic_buffers
+struct AtomicFitBufs {
+  float *Y1, *T2, *Y2, *T3, *Y3, *DY3, *DZ3, *DY2, *DZ2, *DZ1;
+  int FK;
+};
+int32_t fit_atomic_buffers(System* s, const Model* M, AtomicFitBufs& b);
+int32_t fit_atomic_fwd(System* s, const AtomicFitBufs& b);
+int32_t fit_atomic_bwd(System* s, const AtomicFitBufs& b);
+int32_t dplr_forces(System* s);
+int32_t dplr_add_energy(System* s);
 int32_t md_zero_force(System* s);
 int32_t md_kick(System* s, double dt_half);
 int32_t md_drift(System* s, double dt);

@@ -361,6 +361,8 @@ int32_t md_reduce_launch(System* s) {
   hipLaunchKernelGGL(reduce_kernel, dim3(s->nseg), dim3(256), 0, s->stream, s->d_eatom, s->ntypes, rb, s->d_seg,
                      s->d_partial, s->d_red);
   DPMD_HIP(hipGetLastError());
+  // DPLR: the long-range energy of the last evaluation joins the segment energy
+  if (s->wc_model) DPMD_TRY(dplr_add_energy(s));
   return DPMD_OK;
 }
 

@@ -25,7 +25,8 @@ class ModelDesc(ctypes.Structure):
                 ('emb_kernel', (c_vp * 3) * (MAXT * MAXT)), ('emb_bias', (c_vp * 3) * (MAXT * MAXT)),
                 ('fit_kernel', (c_vp * 4) * MAXT), ('fit_bias', (c_vp * 4) * MAXT), ('fit_dt', (c_vp * 3) * MAXT),
                 ('tab_ngrids', c_i32), ('tab_srmin', ctypes.c_float * (MAXT * MAXT)),
-                ('tab_srmax', ctypes.c_float * (MAXT * MAXT)), ('tab_coeff', c_vp * (MAXT * MAXT))]
+                ('tab_srmax', ctypes.c_float * (MAXT * MAXT)), ('tab_coeff', c_vp * (MAXT * MAXT)),
+                ('atomic', c_i32), ('nsel_mask', c_i32)]
 
 
 class MDParams(ctypes.Structure):
@@ -91,6 +92,8 @@ def lib():
         'dpmd_nlist_rebuilds': (c_i32, [c_vp, c_vp]),
         'dpmd_system_set_fixed': (c_i32, [c_vp, c_vp]),
         'dpmd_pppm_compute': (c_i32, [c_i32, c_vp, c_f64, c_vp, c_i32, c_vp, c_vp, c_vp, c_vp]),
+        'dpmd_system_set_dplr': (c_i32, [c_vp, c_vp, c_vp, c_vp, c_f64, c_vp]),
+        'dpmd_dplr_get': (c_i32, [c_vp, c_vp, c_vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -127,7 +130,7 @@ def exported_symbols():
             'dpmd_thermostat_state_get', 'dpmd_thermostat_state_set', 'dpmd_timing_enable',
             'dpmd_timing_get', 'dpmd_pair_count', 'dpmd_synchronize', 'dpmd_system_get_boxes',
             'dpmd_md_box_trajectory', 'dpmd_npt_state_get', 'dpmd_nlist_rebuilds', 'dpmd_system_set_fixed',
-            'dpmd_pppm_compute']
+            'dpmd_pppm_compute', 'dpmd_system_set_dplr', 'dpmd_dplr_get']
 
 
 def pppm_compute(box3, beta, grid, xyz, q, device=0):
@@ -152,8 +155,13 @@ class DeviceModel:
         P = variables['params']
         d = ModelDesc()
         nt = int(params['ntypes'])
-        if params.get('use_mp', False) or params.get('atomic', False):
-            raise NotImplementedError('only the se_a energy model (no message passing, no atomic output) is supported')
+        atomic = bool(params.get('atomic', False))
+        if params.get('use_mp', False):
+            raise NotImplementedError('only the se_a model (no message passing) is supported')
+        if atomic and params.get('type', 'atomic') != 'atomic':
+            raise NotImplementedError("atomic models of type %r are not supported (only 'atomic')" % params.get('type'))
+        if atomic and params.get('is_compressed', False):
+            raise NotImplementedError('compressed atomic models are not supported')
         if not params.get('use_2nd', True):
             raise NotImplementedError('use_2nd=False is not supported')
         if list(params.get('valid_types', range(nt))) != list(range(nt)):
@@ -184,6 +192,21 @@ class DeviceModel:
 
         d.tbias = arr(P['Tbias'])
         compressed = bool(params.get('is_compressed', False))
+        if atomic:
+            # nets of centre type i live at (position of i in nsel) * ntypes + j; centre types
+            # outside nsel carry no output and get the first selected type's nets
+            nsel = [int(t) for t in params['nsel']]
+            pos = {t: k for k, t in enumerate(nsel)}
+            d.atomic = 1
+            d.nsel_mask = sum(1 << t for t in nsel)
+            P = dict(P)
+            src = {}
+            for i in range(nt):
+                ci = pos.get(i, 0)
+                for j in range(nt):
+                    src['embedding_net_%d' % (i * nt + j)] = P['embedding_net_%d' % (ci * nt + j)]
+                src['fitting_net_%d' % i] = P['fitting_net_%d' % ci]
+            P.update(src)
         for k in range(nt * nt):
             net = P.get('embedding_net_%d' % k)
             if net is None:
@@ -215,7 +238,7 @@ class DeviceModel:
             d.tab_ngrids = int(ngrids)
         for t in range(nt):
             net = P['fitting_net_%d' % t]
-            for l in range(4):
+            for l in range(3 if atomic else 4):
                 d.fit_kernel[t][l] = arr(net['Dense_%d' % l]['kernel'])
                 b = net['Dense_%d' % l].get('bias')
                 d.fit_bias[t][l] = arr(b) if b is not None else None
@@ -226,6 +249,7 @@ class DeviceModel:
         self.handle = h
         self.params = params
         self.ntypes = nt
+        self.nsel_mask = int(d.nsel_mask)
         self.rcut = float(params['rcut'])
 
     def __del__(self):
@@ -396,6 +420,28 @@ class DeviceSystem:
         b = np.zeros((nsteps, self.nrep, 3))
         check(lib().dpmd_md_box_trajectory(self.handle, ptr(b)))
         return b
+
+    def set_dplr(self, wc_model, q_atoms, q_wc, beta, grid):
+        """DPLR long-range term: wc_model a DeviceModel of an atomic (Wannier-centroid) model,
+        q_atoms / q_wc charges per type (ntypes), PPPM beta and grid (3,); None removes it."""
+        if wc_model is None:
+            check(lib().dpmd_system_set_dplr(self.handle, None, None, None, 0.0, None))
+            self._wc_model = None
+            return
+        qa = np.ascontiguousarray(np.asarray(q_atoms, np.float64).reshape(-1))
+        qw = np.ascontiguousarray(np.asarray(q_wc, np.float64).reshape(-1))
+        g = np.ascontiguousarray(np.asarray(grid, np.int32).reshape(3))
+        check(lib().dpmd_system_set_dplr(self.handle, wc_model.handle, ptr(qa), ptr(qw), float(beta), ptr(g)))
+        self._wc_model = wc_model  # keeps the centroid model alive while the system uses it
+        self._n_wc = int(sum(np.isin(self.type_idx, [t for t in range(wc_model.ntypes)
+                                                     if int(wc_model.nsel_mask) >> t & 1])))
+
+    def dplr_get(self):
+        """(long-range energy, centroids (n_wc, 3)) of the last evaluation."""
+        e = np.zeros(1)
+        c = np.zeros((max(self._n_wc, 1), 3))
+        check(lib().dpmd_dplr_get(self.handle, ptr(e), ptr(c)))
+        return float(e[0]), c[:self._n_wc]
 
     def set_fixed(self, fixed):
         """Fixed-atom mask in caller order (flat over all atoms of all replicas), or None."""

@@ -80,6 +80,13 @@ typedef struct {
   float tab_srmin[DPMD_MAX_TYPES * DPMD_MAX_TYPES];
   float tab_srmax[DPMD_MAX_TYPES * DPMD_MAX_TYPES];
   const float* tab_coeff[DPMD_MAX_TYPES * DPMD_MAX_TYPES];
+  /* Atomic vector model (params['atomic'], type 'atomic', deepmd_jax/dpmodel.py:83-98), the
+   * Wannier-centroid model of DPLR: nonzero selects it.  The fitting nets are hidden layers only
+   * (fit_kernel[t][3] unused, fit_widths[2] == embed_widths[2]); an atom of a type in nsel_mask
+   * (bit t) gets out_norm * sum_w f_w T3[:, w].  Nets of centre types outside the mask are not
+   * evaluated for output (any weights of the right shapes may be passed for them). */
+  int32_t atomic;
+  int32_t nsel_mask;
 } dpmd_model_desc;
 
 /* Integrator parameters for dpmd_md_run. Units: fs, eV, Angstrom. */
@@ -235,6 +242,18 @@ int32_t dpmd_system_get_boxes(dpmd_system_t sys, double* boxes);
 /* Per-step boxes (steps, nseg, 3) of the last dpmd_md_run that recorded a trajectory: the
  * box trajectory of Simulation.run (deepmd_jax/md.py:846,924). */
 int32_t dpmd_md_box_trajectory(dpmd_system_t sys, double* boxes);
+/* DPLR long-range term (deepmd_jax/md.py:404-415, deepmd_jax/utils.py:174-222,355-366): wc_model (an
+ * atomic model on the same device, with the system model's embedding widths and axis) predicts a
+ * Wannier centroid x_i + dx_i for every atom of its selected types; the PPPM energy of charge
+ * q_atoms[type] at every atom and q_wc[type] at every centroid (beta, grid[3]) joins the energy, and
+ * its gradient through the centroid model joins the forces of every later evaluation and MD step.
+ * One segment without periodic images; the virial is not formed (NaN) while DPLR is set.
+ * wc_model NULL removes the term. */
+int32_t dpmd_system_set_dplr(dpmd_system_t sys, dpmd_model_t wc_model, const double* q_atoms, const double* q_wc,
+                             double beta, const int32_t* grid);
+/* Of the last evaluation with DPLR set: the long-range energy and the centroid positions (n_wc x 3,
+ * raw order among the atoms of the selected types, as wc_predict returns them), either may be NULL. */
+int32_t dpmd_dplr_get(dpmd_system_t sys, double* e_long_range, double* centroids);
 /* PPPM long-range energy and forces of n point charges in an orthorhombic periodic box, the
  * get_p3mlr_fn of deepmd_jax/utils.py:177-222 (TSC assignment onto grid[0] x grid[1] x grid[2],
  * E = sum_k K(k)|S(k)|^2 in eV for charges in e and lengths in Angstrom), evaluated on GPU

@@ -1,0 +1,81 @@
+"""DPLR on the device (deepmd_jax/md.py:404-415) against the FP64 oracle.
+
+216 H2O (L = 18.6 A), the seeded energy model plus a seeded Wannier-centroid model, charges
+O +6, H +1, centroid -8, beta 0.4, the resolution-5 grid.  Tolerances (FP32 model arithmetic,
+FP64 PPPM): centroids within 1e-5 A, long-range energy within 1e-5 relative, total energy
+within 1e-5 eV/atom, forces within 1e-4 * max |F|; 10 velocity-Verlet steps with DPLR forces
+within 1e-5 (A, A/fs) of the oracle's.  The virial is not formed (NaN) while DPLR is set.
+"""
+import numpy as np
+import pytest
+
+from oracle import dp_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+Q_ATOMS, Q_WC, BETA = [6.0, 1.0], [-8.0, 0.0], 0.4
+
+
+def _setup(device_model, model0, nmol=216, seed=40):
+    from deepmd_jax import _lib, synthetic as syn
+    from deepmd_jax.constants import MASS_UNIT_CONVERSION
+    params, var = model0
+    wp, wv = syn.init_wannier_model(4)
+    wdm = _lib.DeviceModel(wp, wv)
+    coord, box, types = syn.water_box(nmol, seed=seed)
+    m = np.array(syn.MASSES)[types] * MASS_UNIT_CONVERSION
+    s = _lib.DeviceSystem(device_model, types, 1)
+    s.set_box(box)
+    s.set_masses(m)
+    s.set_positions(coord)
+    s.nlist_allocate(params['rcut'] + 0.5, 1.2)
+    M = O.p3m_grid(box, BETA, 5)
+    s.set_dplr(wdm, Q_ATOMS, Q_WC, BETA, M)
+    return s, (wp, wv), coord, box, types, m, M
+
+
+def _oracle(model0, wc, coord, box, types, M):
+    params, var = model0
+    nb, _, _ = O.typed_nlist(coord, box, types, params['rcut'] + 1.0)
+    return O.dplr_energy_force(params, var, wc[0], wc[1], coord, box, types, Q_ATOMS, Q_WC[:1], BETA, M,
+                               nbrs_nm=nb, wc_nbrs_nm=nb)
+
+
+def test_dplr_energy_forces_centroids(device_model, model0):
+    print('case dplr eval', flush=True)
+    s, wc, coord, box, types, m, M = _setup(device_model, model0)
+    e, f, v, _ = s.compute()
+    elr, cen = s.dplr_get()
+    E, Elr, F, W = _oracle(model0, wc, coord, box, types, M)
+    N = len(types)
+    assert np.abs(cen - W).max() < 1e-5
+    assert abs(elr - Elr) <= 1e-5 * abs(Elr), (elr, Elr)
+    assert abs(e[0] - E) / N <= 1e-5, (e[0], E)
+    assert np.abs(f[0] - F).max() <= 1e-4 * np.abs(F).max()
+    assert np.isnan(v).all()
+    # removing the term restores the short-range model
+    s.set_dplr(None, None, None, 0, None)
+    e2, f2, v2, _ = s.compute()
+    assert np.isfinite(v2).all() and abs(e2[0] - (E - Elr)) / N <= 1e-5
+
+
+def test_dplr_nve_against_oracle(device_model, model0):
+    from deepmd_jax import _lib, synthetic as syn
+    print('case dplr nve', flush=True)
+    s, wc, coord, box, types, m, M = _setup(device_model, model0, seed=41)
+    vel = syn.maxwell_boltzmann(types, 300.0, seed=41)
+    s.set_velocities(vel)
+    p = _lib.MDParams()
+    p.routine, p.dt = 0, 0.5
+    nsteps = 10
+    tp, tv, err = s.md_run(p, nsteps)
+    assert err == 0
+    x, pm = coord.copy(), vel * m[:, None]
+    F = _oracle(model0, wc, x, box, types, M)[2]
+    for _ in range(nsteps):
+        pm = pm + 0.25 * F
+        x = x + 0.5 * pm / m[:, None]
+        F = _oracle(model0, wc, x, box, types, M)[2]
+        pm = pm + 0.25 * F
+    assert np.abs(tp[-1, 0] - x).max() < 1e-5
+    assert np.abs(tv[-1, 0] - pm / m[:, None]).max() < 1e-5
