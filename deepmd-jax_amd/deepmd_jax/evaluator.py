@@ -37,14 +37,40 @@ def replicate(coord, box3, type_idx, k):
     return c, box3 * k, np.tile(type_idx, len(shifts))
 
 
+def dplr_setup(params, device=0):
+    """(centroid DeviceModel, charge per type of the atoms, of the centroids, beta, resolution) of a
+    DPLR model's parameters (deepmd_jax/train.py:305-313), or None for other models."""
+    if params.get('type') != 'dplr':
+        return None
+    wc_model, wc_var = params['dplr_wannier_model_and_variables']
+    wp = wc_model.params
+    nt = int(params['ntypes'])
+    nsel = [int(t) for t in wp['nsel']]
+    q_wc = np.asarray(params['dplr_q_wc'], np.float64).reshape(-1)
+    if len(q_wc) != len(nsel):
+        raise ValueError('dplr_q_wc must correspond to atomic_sel of the Wannier model.')
+    qw = np.zeros(nt)
+    qw[nsel] = q_wc
+    qa = np.asarray(params['dplr_q_atoms'], np.float64).reshape(-1)[:nt]
+    return (_lib.DeviceModel(wp, wc_var, device), qa, qw, float(params.get('dplr_beta', 0.4)),
+            float(params.get('dplr_resolution', 5)))
+
+
+def dplr_grid(box3, beta, resolution):
+    """PPPM grid of get_p3mlr_grid_size (deepmd_jax/utils.py:174-175)."""
+    return (np.asarray(box3, np.float64) * beta * resolution).astype(int)
+
+
 class Evaluator:
-    """Energies, forces and virials of orthorhombic frames for one model."""
+    """Energies, forces and virials of orthorhombic frames for one model.  A DPLR model
+    evaluates frame by frame with its long-range term (virial NaN)."""
 
     def __init__(self, params, variables, device=0, model=None):
         self.params = params
         self.model = model if model is not None else _lib.DeviceModel(params, variables, device)
         self.rcut = float(params['rcut'])
         self._systems = {}
+        self.dplr = dplr_setup(params, device)
 
     def _system(self, type_idx, nrep):
         key = (tuple(np.asarray(type_idx).tolist()), nrep)
@@ -62,6 +88,21 @@ class Evaluator:
         k = supercell_factors(box3, self.rcut)
         nk = int(np.prod(k))
         F, N = coords.shape[:2]
+        if self.dplr is not None:
+            if nk > 1:
+                raise NotImplementedError('DPLR evaluation needs box lengths >= 2*rcut (%.3f)' % (2 * self.rcut))
+            wdm, qa, qw, beta, res = self.dplr
+            out = [], [], []
+            sys = self._system(type_idx, 1)
+            for c in coords:
+                sys.set_box(box3)
+                sys.set_positions(c)
+                sys.nlist_allocate(self.rcut, 1.2)
+                sys.set_dplr(wdm, qa, qw, beta, dplr_grid(box3, beta, res))
+                e, f, v, _ = sys.compute(precision)
+                for o, x in zip(out, (e[0], f[0], v[0])):
+                    o.append(x)
+            return np.array(out[0]), np.stack(out[1]), np.stack(out[2])
         if nk > 1:
             cs = [replicate(c, box3, type_idx, k) for c in coords]
             sc = np.stack([c[0] for c in cs])

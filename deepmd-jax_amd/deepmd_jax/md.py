@@ -17,7 +17,8 @@ Fixed atoms (fixed_indices) and per-step centre-of-mass removal (remove_com_moti
 on the device (deepmd_jax/md.py:221-234,786-790, deepmd_jax/routine.py:110-136).
 XYZ dumps (run(dump_prefix=...)) stream from a writer thread (deepmd_jax/dumps.py).
 Model deviation (model_deviation_paths) evaluates each listed model on its own device
-system at report time.  Not provided: DPLR.
+system at report time.  DPLR models (type 'dplr') add the Wannier-centroid PPPM term on the
+device (NVE / NVT / NVT_langevin, boxes >= 2 rcut; no virial, so no pressure).
 """
 import time
 
@@ -26,7 +27,7 @@ import numpy as np
 from . import _lib
 from .constants import MASS_UNIT_CONVERSION, TEMP_UNIT_CONVERSION, PRESS_UNIT_CONVERSION
 from .dumps import CHEMICAL_SYMBOLS, DumpWriter, prepare_dumps, target_label
-from .evaluator import canonical_box
+from .evaluator import canonical_box, dplr_grid, dplr_setup
 from .constants import HBAR
 from .pimd import normal_mode_transform, pile_l_factors, centroid, spring_energy
 from .utils import load_model
@@ -268,6 +269,13 @@ class Simulation:
             self._neighbor_skin = max(0.0, 0.5 * self._box.min() * (1 - 1e-9) - self._rcut)
         self._sys = _ImageSystem(self._device_model, self._type_idx, self._n_bead, self._box, k)
         self._sys.set_masses(self._mass)
+        # DPLR: the centroid model and PPPM term join every force evaluation (deepmd_jax/md.py:404-415)
+        self._dplr = dplr_setup(params, device)
+        if self._dplr is not None:
+            if routine == 'NPT' or self._is_pimd:
+                raise NotImplementedError('DPLR runs NVE, NVT and NVT_langevin without beads on this engine.')
+            if self._sys.nimg > 1:
+                raise NotImplementedError('DPLR needs box lengths >= 2*rcut (%.3f) on this engine.' % (2 * self._rcut))
         # mobile-atom mask (deepmd_jax/md.py:220-226); the device zeroes the forces on fixed
         # atoms, keeps their positions and masks their momenta after each NVE / NVT step
         self._mobile = None
@@ -280,6 +288,9 @@ class Simulation:
             self._sys.set_fixed(~mobile)
         self._sys.set_positions(pos)
         self._allocate_nlist()
+        if self._dplr is not None:
+            wdm, qa, qw, beta, res = self._dplr
+            self._sys.dev.set_dplr(wdm, qa, qw, beta, dplr_grid(self._box, beta, res))
         # models of the deviation column (deepmd_jax/md.py:299-302,559-567): each on its own
         # device system over the same atoms, evaluated at report time
         self._deviation = []
@@ -623,11 +634,15 @@ class Simulation:
         return f if self._is_pimd else f[0]
 
     def getVirial(self):
+        if self._dplr is not None:
+            raise NotImplementedError('the DPLR virial is not formed on this engine')
         _, _, v, _ = self._sys.compute(self._precision)
         return v if self._is_pimd else v[0]
 
     def getPressure(self):
         """Isotropic pressure (bar): (N kT - dU/deps / 3) / V, deepmd_jax/md_utils.py:139-159."""
+        if self._dplr is not None:
+            raise NotImplementedError('the DPLR virial is not formed on this engine')
         if self._temperature is None:
             raise ValueError('getPressure needs a temperature')
         _, _, v, _ = self._sys.compute(self._precision)

@@ -43,8 +43,8 @@ def evaluate(model_path, coord, box, type_idx, batch_size=1, precision='fp32'):
                          'type_idx (n_atoms).')
     model, variables = load_model(model_path, replicate=False)
     params = model.params
-    if params.get('type', 'energy') not in ('energy',):
-        raise NotImplementedError('evaluate() supports energy models')
+    if params.get('type', 'energy') not in ('energy', 'dplr'):
+        raise NotImplementedError('evaluate() supports energy and DPLR models')
     ct = params.get('chemical_types')
     if ct is not None:
         unknown = set(type_idx.tolist()) - set(ct)
