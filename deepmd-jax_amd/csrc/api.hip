@@ -575,7 +575,7 @@ static int32_t compute_impl(System* s, int precision, double* energy, double* fo
   for (int r = 0; r < s->nseg; ++r) {
     if (energy) energy[r] = red[r * 16] + seg_ebias(s, r);
     if (virial)
-      for (int k = 0; k < 9; ++k) virial[r * 9 + k] = s->wc_model ? NAN : red[r * 16 + 1 + k];
+      for (int k = 0; k < 9; ++k) virial[r * 9 + k] = red[r * 16 + 1 + k];
   }
   if (force) {
     std::vector<double> fd(NA * 3);
@@ -917,10 +917,7 @@ int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps, 
   }
   const bool npt = p->routine == DPMD_NPT_NH;
   const bool pnpt = p->routine == DPMD_PIMD_NPT;
-  if (s->wc_model && (npt || pnpt)) {
-    set_error("DPLR forms no virial: NPT is not supported with it");
-    return DPMD_E_UNSUPPORTED;
-  }
+
   NptArgs na;
   memset(&na, 0, sizeof(na));
   if (npt || pnpt) DPMD_TRY(npt_args(s, p, na));

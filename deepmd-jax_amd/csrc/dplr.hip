@@ -8,8 +8,15 @@
 // -(d dx / dx)^T g with g = dE/d(centroid) is the backward of the centroid model for the
 // upstream gradient g: through the hidden fitting layers into dE/dG, plus the direct
 // dE/dT3 = out_norm g f, into the embedding backward (which accumulates the forces).
+//
+// Virial (-dE/dstrain with atoms and box scaled, as the reference's perturbation does): the
+// centroids x + dx do not scale with the box, dx follows the centroid model, so
+//   W_lr = W_recip (PPPM with every charge scaling) + W' + sum_k g_k (x) dx_k,
+// W' = -d(g . dx)/dstrain, the virial the embedding backward accumulates for the centroid
+// model's pseudo-energy g . dx.
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <vector>
 
@@ -82,7 +89,34 @@ __global__ void dplr_dy3_kernel(const float* wc_g, const float* T, const float* 
 }
 
 __global__ void dplr_add_energy_kernel(double* red, const double* lr) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) red[0] += lr[0];
+  if (threadIdx.x < 10 && blockIdx.x == 0) red[threadIdx.x] += lr[threadIdx.x];
+}
+
+// lr[1..10) += W' (the centroid backward's block partials) + sum_k g_k (x) dx_k, one block
+__global__ void dplr_virial_kernel(const double* partial, int nblocks, const float* wc_g, const double* qxyz,
+                                   const double* pos, const int* wc_atom, int NA, int nwc, double* lr) {
+  __shared__ double red[256];
+  double v[9];
+  for (int c = 0; c < 9; ++c) v[c] = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+    for (int c = 0; c < 9; ++c) v[c] += partial[(size_t)b * 9 + c];
+  for (int k = threadIdx.x; k < nwc; k += blockDim.x) {
+    const int g = wc_atom[k];
+    double dx[3];
+    for (int d = 0; d < 3; ++d) dx[d] = qxyz[(size_t)(NA + k) * 3 + d] - pos[(size_t)g * 3 + d];
+    for (int a = 0; a < 3; ++a)
+      for (int bb = 0; bb < 3; ++bb) v[a * 3 + bb] += (double)wc_g[(size_t)g * 3 + a] * dx[bb];
+  }
+  for (int c = 0; c < 9; ++c) {
+    red[threadIdx.x] = v[c];
+    __syncthreads();
+    for (int k = blockDim.x / 2; k > 0; k >>= 1) {
+      if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) lr[1 + c] += red[0];
+    __syncthreads();
+  }
 }
 
 }  // namespace
@@ -105,9 +139,11 @@ static int32_t dplr_run(System* s, Model* M) {
   hipLaunchKernelGGL(dplr_charges_kernel, dim3((nq * 3 + 255) / 256), dim3(256), 0, st, s->d_pos, s->d_T, b.Y3,
                      b.FK, W, (float)M->out_norm, s->d_wc_atom, s->d_wc_row, NA, nwc, s->d_qxyz);
   // long-range energy and the forces on every charge
-  DPMD_TRY(s->pppm->setup(st, s->dplr_grid, s->seg[0].boxd, s->dplr_beta));
+  // the segment's box as the device holds it (it moves on the device under NPT)
+  const double* d_box = reinterpret_cast<const double*>(reinterpret_cast<const char*>(s->d_seg) +
+                                                        offsetof(SegInfo, boxd));
   DPMD_HIP(hipMemsetAsync(s->d_qf, 0, sizeof(double) * 3 * nq, st));
-  DPMD_TRY(s->pppm->compute(st, s->d_qxyz, s->d_q, nq, s->d_lr, s->d_qf));
+  DPMD_TRY(s->pppm->compute(st, d_box, s->d_qxyz, s->d_q, nq, s->d_lr, s->d_qf));
   hipLaunchKernelGGL(dplr_scatter_kernel, dim3((NA * 3 + 255) / 256), dim3(256), 0, st, s->d_qf, s->d_atom_wc, NA,
                      s->d_force, s->d_wc_g);
   // backward of the centroid model for the upstream gradient g
@@ -130,7 +166,11 @@ static int32_t dplr_run(System* s, Model* M) {
   s->wc_g = nullptr;
   s->wc_y3 = nullptr;
   s->partial_out = nullptr;
-  return rc;
+  DPMD_TRY(rc);
+  hipLaunchKernelGGL(dplr_virial_kernel, dim3(1), dim3(256), 0, st, s->d_partial_wc, s->nblocks, s->d_wc_g,
+                     s->d_qxyz, s->d_pos, s->d_wc_atom, NA, nwc, s->d_lr);
+  DPMD_HIP(hipGetLastError());
+  return DPMD_OK;
 }
 
 int32_t dplr_forces(System* s) {
@@ -216,7 +256,7 @@ extern "C" int32_t dpmd_system_set_dplr(dpmd_system_t sys, dpmd_model_t wc_model
   DPMD_TRY(sys_ensure(s, (void**)&s->d_qxyz, sizeof(double) * 3 * nq));
   DPMD_TRY(sys_ensure(s, (void**)&s->d_q, sizeof(double) * nq));
   DPMD_TRY(sys_ensure(s, (void**)&s->d_qf, sizeof(double) * 3 * nq));
-  DPMD_TRY(sys_ensure(s, (void**)&s->d_lr, sizeof(double)));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_lr, sizeof(double) * 10));
   DPMD_TRY(sys_ensure(s, (void**)&s->d_wc_g, sizeof(float) * 3 * NA));
   DPMD_TRY(sys_ensure(s, (void**)&s->d_partial_wc, sizeof(double) * 9 * (size_t)std::max(s->nblocks, 1)));
   const size_t fitbytes = sizeof(float) * 10 * (size_t)std::max(s->mrows, 1) * W->fit_k;
@@ -229,11 +269,11 @@ extern "C" int32_t dpmd_system_set_dplr(dpmd_system_t sys, dpmd_model_t wc_model
   }
   DPMD_HIP(hipMemcpyAsync(s->d_atom_wc, atom_wc.data(), sizeof(int) * NA, hipMemcpyHostToDevice, st));
   DPMD_HIP(hipMemcpyAsync(s->d_q, q.data(), sizeof(double) * nq, hipMemcpyHostToDevice, st));
-  DPMD_HIP(hipMemsetAsync(s->d_lr, 0, sizeof(double), st));
+  DPMD_HIP(hipMemsetAsync(s->d_lr, 0, sizeof(double) * 10, st));
   if (!s->pppm) s->pppm = new Pppm();
   for (int d = 0; d < 3; ++d) s->dplr_grid[d] = grid[d];
   s->dplr_beta = beta;
-  DPMD_TRY(s->pppm->setup(st, s->dplr_grid, S.boxd, beta));
+  DPMD_TRY(s->pppm->setup(s->dplr_grid, beta));
   DPMD_HIP(hipStreamSynchronize(st));
   s->n_wc = nwc;
   s->wc_model = W;

@@ -12,11 +12,9 @@ namespace dpmd {
 void set_error(const char* fmt, ...);
 
 // PPPM long-range electrostatics on one device (pppm.hip): grid, half spectrum, influence
-// function and hipFFT plans for a grid size; set-up recomputes the influence function when
-// the box or beta change and reallocates when the grid size changes.
+// function and hipFFT plans for a grid size (set-up reallocates when the size changes).
 struct Pppm {
   int M[3] = {0, 0, 0};
-  double box[3] = {0.0, 0.0, 0.0};
   double beta = 0.0;
   hipfftHandle_t* fwd = nullptr;
   hipfftHandle_t* inv = nullptr;
@@ -29,9 +27,11 @@ struct Pppm {
   Pppm& operator=(const Pppm&) = delete;
   ~Pppm();
   void release();
-  int32_t setup(hipStream_t st, const int* grid_size, const double* box3, double beta);
-  // energy (device scalar) of n charges xyz (n x 3) q (n); -dE/dx added to force (n x 3)
-  int32_t compute(hipStream_t st, const double* xyz, const double* q, int n, double* d_energy, double* force);
+  int32_t setup(const int* grid_size, double beta);
+  // n charges xyz (n x 3) q (n) in the box d_box (device, 3 lengths): d_out[0] energy,
+  // d_out[1..10) the 3 x 3 reciprocal virial; -dE/dx added to force (n x 3)
+  int32_t compute(hipStream_t st, const double* d_box, const double* xyz, const double* q, int n, double* d_out,
+                  double* force);
 };
 
 #define DPMD_HIP(x)                                                                    \

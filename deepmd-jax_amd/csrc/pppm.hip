@@ -10,6 +10,11 @@
 // complex-to-real transform of the half spectrum), and the force on a charge is
 // -q sum over its 27 points of phi times the gradient of the TSC weight.  Everything is FP64
 // (the reference evaluates in FP32); the energy sum is a two-pass deterministic reduction.
+// The box is read from device memory (it may move on the device under NPT), so the influence
+// function is formed for every evaluation.  With charge positions that scale with the box, the
+// strain derivative is that of K alone: dK/de_ab = K (-delta_ab + k_a k_b / 2 beta^2 +
+// 2 k_a k_b / k^2) (the TSC factors depend on k h, fixed under the scaling), and the kernel also
+// sums the reciprocal virial W_ab = -sum_k |S|^2 dK/de_ab.
 #include <hipfft/hipfft.h>
 
 #include <algorithm>
@@ -36,7 +41,7 @@ constexpr double kCoulomb = 14.399645;  // e^2 / (4 pi eps0) in eV Angstrom
 
 struct PppmGeom {
   int M[3];
-  double box[3];
+  const double* box;  // device: the three box lengths
   double beta;
 };
 
@@ -50,6 +55,7 @@ __device__ __forceinline__ double fold_k(int n, int M) {
 }
 
 __global__ void pppm_kfac_kernel(PppmGeom g, double* kfac) {
+  const double box[3] = {g.box[0], g.box[1], g.box[2]};
   const int M2h = g.M[2] / 2 + 1;
   const size_t total = (size_t)g.M[0] * g.M[1] * M2h;
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -60,9 +66,9 @@ __global__ void pppm_kfac_kernel(PppmGeom g, double* kfac) {
   const int idx[3] = {i, j, l};
   double k2 = 0.0, w3 = 1.0, sk = 1.0;
   for (int d = 0; d < 3; ++d) {
-    const double kd = 2.0 * M_PI / g.box[d] * fold_k(idx[d], g.M[d]);
+    const double kd = 2.0 * M_PI / box[d] * fold_k(idx[d], g.M[d]);
     k2 += kd * kd;
-    const double z = kd * (g.box[d] / g.M[d]) * 0.5;
+    const double z = kd * (box[d] / g.M[d]) * 0.5;
     const double sz = sin(z);
     if (z != 0.0) {
       const double sc = sin(z) / z;
@@ -71,7 +77,7 @@ __global__ void pppm_kfac_kernel(PppmGeom g, double* kfac) {
     sk *= 1.0 - sz * sz + 2.0 / 15.0 * sz * sz * sz * sz;
   }
   sk *= sk;
-  const double V = g.box[0] * g.box[1] * g.box[2];
+  const double V = box[0] * box[1] * box[2];
   kfac[t] = k2 != 0.0 ? (kCoulomb * 2.0 * M_PI / V) * exp(-k2 / (4.0 * g.beta * g.beta)) * w3 * w3 / (sk * k2) : 0.0;
 }
 
@@ -86,18 +92,18 @@ struct Tsc {
   double w[3][3], dw[3][3];  // [axis][offset]
 };
 
-__device__ __forceinline__ void tsc(const PppmGeom& g, const double* x, Tsc& t) {
+__device__ __forceinline__ void tsc(const PppmGeom& g, const double* box, const double* x, Tsc& t) {
   for (int d = 0; d < 3; ++d) {
-    double xm = fmod(x[d], g.box[d]);
-    if (xm < 0) xm += g.box[d];
-    const double u = xm / g.box[d] * g.M[d];
+    double xm = fmod(x[d], box[d]);
+    if (xm < 0) xm += box[d];
+    const double u = xm / box[d] * g.M[d];
     const double c = rint(u);
     const double r = u - c;
     t.c[d] = (int)c;
     t.w[d][0] = 0.5 * (r - 0.5) * (r - 0.5);
     t.w[d][1] = 0.75 - r * r;
     t.w[d][2] = 0.5 * (r + 0.5) * (r + 0.5);
-    const double s = g.M[d] / g.box[d];  // du / dx
+    const double s = g.M[d] / box[d];  // du / dx
     t.dw[d][0] = (r - 0.5) * s;
     t.dw[d][1] = -2.0 * r * s;
     t.dw[d][2] = (r + 0.5) * s;
@@ -113,7 +119,8 @@ __global__ void pppm_spread_kernel(PppmGeom g, const double* xyz, const double* 
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   Tsc t;
-  tsc(g, xyz + (size_t)i * 3, t);
+  const double box[3] = {g.box[0], g.box[1], g.box[2]};
+  tsc(g, box, xyz + (size_t)i * 3, t);
   const double qi = q[i];
   for (int a = 0; a < 3; ++a) {
     const int ix = wrap_idx(t.c[0] + a - 1, g.M[0]);
@@ -128,35 +135,60 @@ __global__ void pppm_spread_kernel(PppmGeom g, const double* xyz, const double* 
   }
 }
 
-// per block: sum of w K |S|^2 over the half spectrum, and S <- 2 K S for the potential
+// per block: sums over the half spectrum of w K |S|^2 (energy) and of the reciprocal virial
+// W_ab (00, 11, 22, 01, 02, 12); S <- 2 K S for the potential.  part[7 block + c]
 __global__ void pppm_energy_kernel(PppmGeom g, hipfftDoubleComplex* spec, const double* kfac, double* part) {
   __shared__ double red[256];
+  const double box[3] = {g.box[0], g.box[1], g.box[2]};
+  const double ib2 = 1.0 / (2.0 * g.beta * g.beta);
   const int M2h = g.M[2] / 2 + 1;
   const size_t total = (size_t)g.M[0] * g.M[1] * M2h;
-  double e = 0.0;
+  double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
     const int l = (int)(t % M2h);
+    const int j = (int)((t / M2h) % g.M[1]);
+    const int i = (int)(t / ((size_t)M2h * g.M[1]));
     const double w = (l == 0 || (g.M[2] % 2 == 0 && l == g.M[2] / 2)) ? 1.0 : 2.0;
     const double K = kfac[t];
     const hipfftDoubleComplex s = spec[t];
-    e += w * K * (s.x * s.x + s.y * s.y);
+    const double e = w * K * (s.x * s.x + s.y * s.y);
+    acc[0] += e;
+    if (K != 0.0) {
+      const double k0 = 2.0 * M_PI / box[0] * fold_k(i, g.M[0]);
+      const double k1 = 2.0 * M_PI / box[1] * fold_k(j, g.M[1]);
+      const double k2v = 2.0 * M_PI / box[2] * fold_k(l, g.M[2]);
+      const double f = ib2 + 2.0 / (k0 * k0 + k1 * k1 + k2v * k2v);
+      acc[1] -= e * (-1.0 + k0 * k0 * f);
+      acc[2] -= e * (-1.0 + k1 * k1 * f);
+      acc[3] -= e * (-1.0 + k2v * k2v * f);
+      acc[4] -= e * k0 * k1 * f;
+      acc[5] -= e * k0 * k2v * f;
+      acc[6] -= e * k1 * k2v * f;
+    }
     spec[t].x = 2.0 * K * s.x;
     spec[t].y = 2.0 * K * s.y;
   }
-  red[threadIdx.x] = e;
-  __syncthreads();
-  for (int k = blockDim.x / 2; k > 0; k >>= 1) {
-    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+  for (int c = 0; c < 7; ++c) {
+    red[threadIdx.x] = acc[c];
+    __syncthreads();
+    for (int k = blockDim.x / 2; k > 0; k >>= 1) {
+      if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x * 7 + c] = red[0];
     __syncthreads();
   }
-  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
+// energy and the 3 x 3 reciprocal virial (row-major) from the block partials
 __global__ void pppm_sum_kernel(const double* part, int nb, double* out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int c = threadIdx.x;
+  if (c >= 7 || blockIdx.x != 0) return;
   double s = 0.0;
-  for (int b = 0; b < nb; ++b) s += part[b];
-  out[0] = s;
+  for (int b = 0; b < nb; ++b) s += part[b * 7 + c];
+  const int dst[7][2] = {{0, -1}, {1, -1}, {5, -1}, {9, -1}, {2, 4}, {3, 7}, {6, 8}};
+  out[dst[c][0]] = s;
+  if (dst[c][1] >= 0) out[dst[c][1]] = s;
 }
 
 // F_i = -q_i sum_{27 points} phi(m) grad W(x_i, m); added to force (n x 3, FP64)
@@ -165,7 +197,8 @@ __global__ void pppm_gather_kernel(PppmGeom g, const double* xyz, const double* 
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   Tsc t;
-  tsc(g, xyz + (size_t)i * 3, t);
+  const double box[3] = {g.box[0], g.box[1], g.box[2]};
+  tsc(g, box, xyz + (size_t)i * 3, t);
   double f0 = 0.0, f1 = 0.0, f2 = 0.0;
   for (int a = 0; a < 3; ++a) {
     const int ix = wrap_idx(t.c[0] + a - 1, g.M[0]);
@@ -203,49 +236,37 @@ void Pppm::release() {
   M[0] = M[1] = M[2] = 0;
 }
 
-int32_t Pppm::setup(hipStream_t st, const int* grid_size, const double* box3, double beta_) {
+int32_t Pppm::setup(const int* grid_size, double beta_) {
   for (int d = 0; d < 3; ++d)
     if (grid_size[d] < 2 || grid_size[d] > 4096) {
       set_error("PPPM grid size %d outside [2, 4096]", grid_size[d]);
       return DPMD_E_ARG;
     }
   if (!(beta_ > 0)) { set_error("PPPM needs beta > 0"); return DPMD_E_ARG; }
-  const bool same_grid = grid_size[0] == M[0] && grid_size[1] == M[1] && grid_size[2] == M[2];
-  if (same_grid && box3[0] == box[0] && box3[1] == box[1] && box3[2] == box[2] && beta_ == beta) return DPMD_OK;
-  if (!same_grid) {
-    release();
-    for (int d = 0; d < 3; ++d) M[d] = grid_size[d];
-    const size_t nr = (size_t)M[0] * M[1] * M[2], nc = (size_t)M[0] * M[1] * (M[2] / 2 + 1);
-    DPMD_HIP(hipMalloc(&grid, sizeof(double) * nr));
-    DPMD_HIP(hipMalloc(&spec, sizeof(hipfftDoubleComplex) * nc));  // void* member
-    DPMD_HIP(hipMalloc(&kfac, sizeof(double) * nc));
-    DPMD_HIP(hipMalloc(&part, sizeof(double) * (kPppmBlocks + 1)));
-    DPMD_FFT(hipfftPlan3d(&fwd, M[0], M[1], M[2], HIPFFT_D2Z));
-    DPMD_FFT(hipfftPlan3d(&inv, M[0], M[1], M[2], HIPFFT_Z2D));
-  }
-  for (int d = 0; d < 3; ++d) box[d] = box3[d];
   beta = beta_;
-  PppmGeom g;
-  for (int d = 0; d < 3; ++d) {
-    g.M[d] = M[d];
-    g.box[d] = box[d];
-  }
-  g.beta = beta;
-  const size_t nc = (size_t)M[0] * M[1] * (M[2] / 2 + 1);
-  hipLaunchKernelGGL(pppm_kfac_kernel, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st, g, kfac);
-  DPMD_HIP(hipGetLastError());
+  if (grid_size[0] == M[0] && grid_size[1] == M[1] && grid_size[2] == M[2]) return DPMD_OK;
+  release();
+  for (int d = 0; d < 3; ++d) M[d] = grid_size[d];
+  const size_t nr = (size_t)M[0] * M[1] * M[2], nc = (size_t)M[0] * M[1] * (M[2] / 2 + 1);
+  DPMD_HIP(hipMalloc(&grid, sizeof(double) * nr));
+  DPMD_HIP(hipMalloc(&spec, sizeof(hipfftDoubleComplex) * nc));  // void* member
+  DPMD_HIP(hipMalloc(&kfac, sizeof(double) * nc));
+  DPMD_HIP(hipMalloc(&part, sizeof(double) * 7 * kPppmBlocks));
+  DPMD_FFT(hipfftPlan3d(&fwd, M[0], M[1], M[2], HIPFFT_D2Z));
+  DPMD_FFT(hipfftPlan3d(&inv, M[0], M[1], M[2], HIPFFT_Z2D));
   return DPMD_OK;
 }
 
-int32_t Pppm::compute(hipStream_t st, const double* xyz, const double* q, int n, double* d_energy, double* force) {
+int32_t Pppm::compute(hipStream_t st, const double* d_box, const double* xyz, const double* q, int n,
+                      double* d_out, double* force) {
   if (!fwd) { set_error("PPPM used before setup"); return DPMD_E_STATE; }
   PppmGeom g;
-  for (int d = 0; d < 3; ++d) {
-    g.M[d] = M[d];
-    g.box[d] = box[d];
-  }
+  for (int d = 0; d < 3; ++d) g.M[d] = M[d];
+  g.box = d_box;
   g.beta = beta;
   const size_t nr = (size_t)M[0] * M[1] * M[2];
+  const size_t nc = (size_t)M[0] * M[1] * (M[2] / 2 + 1);
+  hipLaunchKernelGGL(pppm_kfac_kernel, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st, g, kfac);
   hipLaunchKernelGGL(pppm_zero_kernel, dim3((unsigned)std::min<size_t>(4096, (nr + 255) / 256)), dim3(256), 0, st,
                      grid, nr);
   if (n > 0)
@@ -255,7 +276,7 @@ int32_t Pppm::compute(hipStream_t st, const double* xyz, const double* q, int n,
   hipfftDoubleComplex* sp = static_cast<hipfftDoubleComplex*>(spec);
   DPMD_FFT(hipfftExecD2Z(fwd, grid, sp));
   hipLaunchKernelGGL(pppm_energy_kernel, dim3(kPppmBlocks), dim3(256), 0, st, g, sp, kfac, part);
-  hipLaunchKernelGGL(pppm_sum_kernel, dim3(1), dim3(64), 0, st, part, kPppmBlocks, d_energy);
+  hipLaunchKernelGGL(pppm_sum_kernel, dim3(1), dim3(64), 0, st, part, kPppmBlocks, d_out);
   DPMD_FFT(hipfftExecZ2D(inv, sp, grid));
   if (n > 0 && force)
     hipLaunchKernelGGL(pppm_gather_kernel, dim3((n + 127) / 128), dim3(128), 0, st, g, xyz, q, n, grid, force);
@@ -269,7 +290,7 @@ using namespace dpmd;
 
 extern "C" int32_t dpmd_pppm_compute(int32_t device, const double* box3, double beta, const int32_t* grid_size,
                                      int32_t n, const double* xyz, const double* q, double* energy,
-                                     double* force) {
+                                     double* force, double* virial) {
   if (!box3 || !grid_size || !energy || (n > 0 && (!xyz || !q)) || n < 0) {
     set_error("null argument");
     return DPMD_E_ARG;
@@ -280,13 +301,14 @@ extern "C" int32_t dpmd_pppm_compute(int32_t device, const double* box3, double 
   hipStream_t st;
   DPMD_HIP(hipStreamCreate(&st));
   Pppm p;
-  double *d_x = nullptr, *d_q = nullptr, *d_f = nullptr, *d_e = nullptr;
+  double *d_x = nullptr, *d_q = nullptr, *d_f = nullptr, *d_e = nullptr, *d_b = nullptr;
   int32_t rc = DPMD_OK;
   const size_t nn = std::max(n, 1);
   const int gs[3] = {grid_size[0], grid_size[1], grid_size[2]};
   do {
     if (hipMalloc(&d_x, sizeof(double) * 3 * nn) != hipSuccess || hipMalloc(&d_q, sizeof(double) * nn) != hipSuccess ||
-        hipMalloc(&d_f, sizeof(double) * 3 * nn) != hipSuccess || hipMalloc(&d_e, sizeof(double)) != hipSuccess) {
+        hipMalloc(&d_f, sizeof(double) * 3 * nn) != hipSuccess || hipMalloc(&d_e, sizeof(double) * 10) != hipSuccess ||
+        hipMalloc(&d_b, sizeof(double) * 3) != hipSuccess) {
       set_error("PPPM: device allocation failed");
       rc = DPMD_E_HIP;
       break;
@@ -296,16 +318,18 @@ extern "C" int32_t dpmd_pppm_compute(int32_t device, const double* box3, double 
       hipMemcpyAsync(d_q, q, sizeof(double) * n, hipMemcpyHostToDevice, st);
     }
     hipMemsetAsync(d_f, 0, sizeof(double) * 3 * nn, st);
-    if ((rc = p.setup(st, gs, box3, beta)) != DPMD_OK) break;
-    if ((rc = p.compute(st, d_x, d_q, n, d_e, d_f)) != DPMD_OK) break;
+    hipMemcpyAsync(d_b, box3, sizeof(double) * 3, hipMemcpyHostToDevice, st);
+    if ((rc = p.setup(gs, beta)) != DPMD_OK) break;
+    if ((rc = p.compute(st, d_b, d_x, d_q, n, d_e, d_f)) != DPMD_OK) break;
     hipMemcpyAsync(energy, d_e, sizeof(double), hipMemcpyDeviceToHost, st);
+    if (virial) hipMemcpyAsync(virial, d_e + 1, sizeof(double) * 9, hipMemcpyDeviceToHost, st);
     if (force && n > 0) hipMemcpyAsync(force, d_f, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st);
     if (hipStreamSynchronize(st) != hipSuccess) {
       set_error("PPPM: stream synchronisation failed");
       rc = DPMD_E_HIP;
     }
   } while (0);
-  for (void* ptr_ : {(void*)d_x, (void*)d_q, (void*)d_f, (void*)d_e})
+  for (void* ptr_ : {(void*)d_x, (void*)d_q, (void*)d_f, (void*)d_e, (void*)d_b})
     if (ptr_) hipFree(ptr_);
   p.release();
   hipStreamDestroy(st);

@@ -91,7 +91,7 @@ def lib():
         'dpmd_npt_state_get': (c_i32, [c_vp, c_vp]),
         'dpmd_nlist_rebuilds': (c_i32, [c_vp, c_vp]),
         'dpmd_system_set_fixed': (c_i32, [c_vp, c_vp]),
-        'dpmd_pppm_compute': (c_i32, [c_i32, c_vp, c_f64, c_vp, c_i32, c_vp, c_vp, c_vp, c_vp]),
+        'dpmd_pppm_compute': (c_i32, [c_i32, c_vp, c_f64, c_vp, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp]),
         'dpmd_system_set_dplr': (c_i32, [c_vp, c_vp, c_vp, c_vp, c_f64, c_vp]),
         'dpmd_dplr_get': (c_i32, [c_vp, c_vp, c_vp]),
     }
@@ -134,8 +134,8 @@ def exported_symbols():
 
 
 def pppm_compute(box3, beta, grid, xyz, q, device=0):
-    """PPPM energy (eV) and forces (n, 3) of point charges q (e) at xyz (Angstrom) in an
-    orthorhombic box on the GPU (get_p3mlr_fn, deepmd_jax/utils.py:177-222)."""
+    """PPPM energy (eV), forces (n, 3) and virial (3, 3) of point charges q (e) at xyz (Angstrom)
+    in an orthorhombic box on the GPU (get_p3mlr_fn, deepmd_jax/utils.py:177-222)."""
     b = np.ascontiguousarray(np.asarray(box3, np.float64).reshape(3))
     g = np.ascontiguousarray(np.asarray(grid, np.int32).reshape(3))
     x = np.ascontiguousarray(np.asarray(xyz, np.float64).reshape(-1, 3))
@@ -144,8 +144,10 @@ def pppm_compute(box3, beta, grid, xyz, q, device=0):
         raise ValueError('xyz and q disagree in length')
     e = np.zeros(1)
     f = np.zeros_like(x)
-    check(lib().dpmd_pppm_compute(int(device), ptr(b), float(beta), ptr(g), len(x), ptr(x), ptr(qq), ptr(e), ptr(f)))
-    return float(e[0]), f
+    w = np.zeros((3, 3))
+    check(lib().dpmd_pppm_compute(int(device), ptr(b), float(beta), ptr(g), len(x), ptr(x), ptr(qq), ptr(e), ptr(f),
+                                  ptr(w)))
+    return float(e[0]), f, w
 
 
 class DeviceModel:

@@ -63,7 +63,7 @@ def dplr_grid(box3, beta, resolution):
 
 class Evaluator:
     """Energies, forces and virials of orthorhombic frames for one model.  A DPLR model
-    evaluates frame by frame with its long-range term (virial NaN)."""
+    evaluates frame by frame with its long-range term."""
 
     def __init__(self, params, variables, device=0, model=None):
         self.params = params

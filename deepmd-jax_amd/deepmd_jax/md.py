@@ -18,7 +18,7 @@ on the device (deepmd_jax/md.py:221-234,786-790, deepmd_jax/routine.py:110-136).
 XYZ dumps (run(dump_prefix=...)) stream from a writer thread (deepmd_jax/dumps.py).
 Model deviation (model_deviation_paths) evaluates each listed model on its own device
 system at report time.  DPLR models (type 'dplr') add the Wannier-centroid PPPM term on the
-device (NVE / NVT / NVT_langevin, boxes >= 2 rcut; no virial, so no pressure).
+device, with its virial (one system without beads, boxes >= 2 rcut).
 """
 import time
 
@@ -272,8 +272,8 @@ class Simulation:
         # DPLR: the centroid model and PPPM term join every force evaluation (deepmd_jax/md.py:404-415)
         self._dplr = dplr_setup(params, device)
         if self._dplr is not None:
-            if routine == 'NPT' or self._is_pimd:
-                raise NotImplementedError('DPLR runs NVE, NVT and NVT_langevin without beads on this engine.')
+            if self._is_pimd:
+                raise NotImplementedError('DPLR runs without beads on this engine.')
             if self._sys.nimg > 1:
                 raise NotImplementedError('DPLR needs box lengths >= 2*rcut (%.3f) on this engine.' % (2 * self._rcut))
         # mobile-atom mask (deepmd_jax/md.py:220-226); the device zeroes the forces on fixed
@@ -400,7 +400,13 @@ class Simulation:
             for a in sorted(a for g in self._couple_axes for a in g):
                 rep['box_' + 'xyz'[a]] = float(box[a])
             if not self._is_pimd:
-                rep['Invariant'] = ke_tot + pe + float(self._sys.thermostat_energy()[0])
+                # before the first step the barostat is at rest: its part is P V alone
+                te = float(self._sys.thermostat_energy()[0])
+                try:
+                    self._sys.dev.npt_state()
+                except _lib.DPMDError:
+                    te += self._params.pressure * float(np.prod(box))
+                rep['Invariant'] = ke_tot + pe + te
             elif self._pimd_print_invariant:
                 # deepmd_jax/md.py:540-557: the barostat part KE_box + P V - box work is kept
                 # on the device
@@ -634,15 +640,11 @@ class Simulation:
         return f if self._is_pimd else f[0]
 
     def getVirial(self):
-        if self._dplr is not None:
-            raise NotImplementedError('the DPLR virial is not formed on this engine')
         _, _, v, _ = self._sys.compute(self._precision)
         return v if self._is_pimd else v[0]
 
     def getPressure(self):
         """Isotropic pressure (bar): (N kT - dU/deps / 3) / V, deepmd_jax/md_utils.py:139-159."""
-        if self._dplr is not None:
-            raise NotImplementedError('the DPLR virial is not formed on this engine')
         if self._temperature is None:
             raise ValueError('getPressure needs a temperature')
         _, _, v, _ = self._sys.compute(self._precision)

@@ -247,8 +247,9 @@ int32_t dpmd_md_box_trajectory(dpmd_system_t sys, double* boxes);
  * Wannier centroid x_i + dx_i for every atom of its selected types; the PPPM energy of charge
  * q_atoms[type] at every atom and q_wc[type] at every centroid (beta, grid[3]) joins the energy, and
  * its gradient through the centroid model joins the forces of every later evaluation and MD step.
- * One segment without periodic images; the virial is not formed (NaN) while DPLR is set.
- * wc_model NULL removes the term. */
+ * One segment without periodic images.  The virial includes the long-range term (PPPM
+ * reciprocal virial, the centroid model's strain response and the centroid offsets), so NPT
+ * runs with it.  wc_model NULL removes the term. */
 int32_t dpmd_system_set_dplr(dpmd_system_t sys, dpmd_model_t wc_model, const double* q_atoms, const double* q_wc,
                              double beta, const int32_t* grid);
 /* Of the last evaluation with DPLR set: the long-range energy and the centroid positions (n_wc x 3,
@@ -257,9 +258,10 @@ int32_t dpmd_dplr_get(dpmd_system_t sys, double* e_long_range, double* centroids
 /* PPPM long-range energy and forces of n point charges in an orthorhombic periodic box, the
  * get_p3mlr_fn of deepmd_jax/utils.py:177-222 (TSC assignment onto grid[0] x grid[1] x grid[2],
  * E = sum_k K(k)|S(k)|^2 in eV for charges in e and lengths in Angstrom), evaluated on GPU
- * `device` in FP64.  xyz (n x 3), q (n); force (n x 3, -dE/dx) may be NULL. */
+ * `device` in FP64.  xyz (n x 3), q (n); force (n x 3, -dE/dx) and virial (3 x 3, -dE/dstrain
+ * with the charges scaling with the box) may be NULL. */
 int32_t dpmd_pppm_compute(int32_t device, const double* box3, double beta, const int32_t* grid, int32_t n,
-                          const double* xyz, const double* q, double* energy, double* force);
+                          const double* xyz, const double* q, double* energy, double* force, double* virial);
 /* NPT barostat state (64 doubles): [0..3) log box scale per group, [3..6) box momenta,
  * [6..9) box masses, [9..12) dU/deps per group, [12..15) reference box, [16..24) chain
  * positions, [24..32) chain momenta, [32..40) chain masses, [43] barostat invariant terms

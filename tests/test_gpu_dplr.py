@@ -3,8 +3,10 @@
 216 H2O (L = 18.6 A), the seeded energy model plus a seeded Wannier-centroid model, charges
 O +6, H +1, centroid -8, beta 0.4, the resolution-5 grid.  Tolerances (FP32 model arithmetic,
 FP64 PPPM): centroids within 1e-5 A, long-range energy within 1e-5 relative, total energy
-within 1e-5 eV/atom, forces within 1e-4 * max |F|; 10 velocity-Verlet steps with DPLR forces
-within 1e-5 (A, A/fs) of the oracle's.  The virial is not formed (NaN) while DPLR is set.
+within 1e-5 eV/atom, forces within 1e-4 * max |F|, the virial diagonal within 1e-4 * max |W|
+of central differences of the oracle's energy under axis strain (atoms, box and the centroid
+model's inputs scaled); 10 velocity-Verlet steps with DPLR forces within 1e-5 (A, A/fs) of the
+oracle's; NPT with DPLR conserves its invariant.
 """
 import numpy as np
 import pytest
@@ -52,7 +54,15 @@ def test_dplr_energy_forces_centroids(device_model, model0):
     assert abs(elr - Elr) <= 1e-5 * abs(Elr), (elr, Elr)
     assert abs(e[0] - E) / N <= 1e-5, (e[0], E)
     assert np.abs(f[0] - F).max() <= 1e-4 * np.abs(F).max()
-    assert np.isnan(v).all()
+    # virial diagonal: -dE/de_a of the oracle's DPLR energy with atoms and box scaled along a
+    h = 1e-5
+    for a in range(3):
+        sc = np.ones(3)
+        sc[a] = 1 + h
+        ep = _oracle(model0, wc, coord * sc, box * sc, types, M)[0]
+        sc[a] = 1 - h
+        em = _oracle(model0, wc, coord * sc, box * sc, types, M)[0]
+        assert abs(v[0][a, a] + (ep - em) / (2 * h)) <= 1e-4 * np.abs(v[0]).max(), (a, v[0][a, a], (ep - em) / (2 * h))
     # removing the term restores the short-range model
     s.set_dplr(None, None, None, 0, None)
     e2, f2, v2, _ = s.compute()
@@ -113,11 +123,39 @@ def test_dplr_model_pkl_simulation_and_evaluate(model0, tmp_path):
     inv = np.array(sim.log)[:, 4]
     assert abs(elr1 - elr0) > 1.0, (elr0, elr1)
     assert np.abs(inv - inv[0]).max() / len(types) < 2e-5, inv
-    with pytest.raises(NotImplementedError):
-        sim.getPressure()
     with contextlib.redirect_stdout(io.StringIO()):
         out = evaluate(path, coord[None], box[None], types)
     M = O.p3m_grid(box, BETA, 5)
     E, _, F, _ = _oracle(model0, (wp, wv), coord, box, types, M)
     assert abs(out['energy'][0] - E) / len(types) <= 1e-5
     assert np.abs(out['force'][0] - F).max() <= 1e-4 * np.abs(F).max()
+
+
+def test_dplr_npt_invariant(model0, tmp_path):
+    """NPT (MTK, isotropic, 1 kbar) with a DPLR model: the barostat reads the long-range virial;
+    the NPT invariant holds to 5e-5 eV/atom over 60 steps of 0.05 fs while the box moves."""
+    import contextlib
+    import io
+    from deepmd_jax import synthetic as syn
+    from deepmd_jax.dpmodel import DPModel
+    from deepmd_jax.md import Simulation
+    from deepmd_jax.utils import save_model
+    print('case dplr npt', flush=True)
+    params, var = model0
+    wp, wv = syn.init_wannier_model(4)
+    dparams = dict(params)
+    dparams.update({'type': 'dplr', 'dplr_wannier_model_and_variables': (DPModel(wp), {'params': wv['params']}),
+                    'dplr_q_atoms': Q_ATOMS, 'dplr_q_wc': Q_WC[:1], 'dplr_beta': BETA, 'dplr_resolution': 5})
+    path = str(tmp_path / 'dplr.pkl')
+    with contextlib.redirect_stdout(io.StringIO()):
+        save_model(path, DPModel(dparams), var)
+    coord, box, types = syn.water_box(216, seed=43)
+    with contextlib.redirect_stdout(io.StringIO()):
+        sim = Simulation(path, box, types, syn.MASSES, 'NPT', 0.05, coord, temperature=300.0, pressure=1000.0,
+                         tau_p=50.0, tau_t=20.0, report_interval=10, initial_velocity=syn.maxwell_boltzmann(types, 300.0, seed=43))
+        traj = sim.run(60)
+    log = np.array(sim.log)
+    inv = log[:, -2]
+    assert np.isfinite(log).all()
+    assert np.abs(traj['box'][-1] - box).max() > 1e-6
+    assert np.abs(inv - inv[0]).max() / len(types) < 5e-5, inv

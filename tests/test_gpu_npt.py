@@ -133,3 +133,6 @@ def test_npt_simulation_surface(model0, device_model):
     assert np.isfinite(sim.getPressure())
     # step, Temperature KE PE Pressure box_x box_y box_z Invariant, time
     assert len(sim.log) >= 3 and len(sim.log[0]) == 1 + 8 + 1
+    # the step-0 invariant carries P V of the box at rest (no jump at the first step)
+    inv = np.array(sim.log)[:, 8]
+    assert np.abs(inv - inv[0]).max() / len(types) < 1e-3, inv

@@ -1,5 +1,6 @@
 """Device PPPM (deepmd_jax/utils.py:177-222) against the FP64 oracle: energy within 1e-10
-relative and forces (the oracle's autograd gradient) within 1e-9 * max |F|; both FP64.
+relative, forces (the oracle's autograd gradient) within 1e-9 * max |F|, the virial diagonal
+(-dE/de_a with charges and box scaled along a) within 1e-7 of central differences; all FP64.
 Cases: a neutral random set in a non-cubic box with odd and even grid sizes, a charge set
 with atoms outside the primary cell, and a water-like box at the resolution DPLR uses."""
 import numpy as np
@@ -25,6 +26,15 @@ def test_pppm_against_oracle(L, res, n, shift):
     xt = torch.as_tensor(x).clone().requires_grad_(True)
     E = O.p3m_energy(xt, q, L, 0.4, M)
     F = -torch.autograd.grad(E, [xt])[0].numpy()
-    e, f = _lib.pppm_compute(L, 0.4, M, x, q)
+    e, f, w = _lib.pppm_compute(L, 0.4, M, x, q)
     assert abs(e - E.item()) <= 1e-10 * abs(E.item()), (e, E.item())
     assert np.abs(f - F).max() <= 1e-9 * np.abs(F).max()
+    h = 1e-5
+    for a in range(3):
+        s = np.ones(3)
+        s[a] = 1 + h
+        ep = O.p3m_energy(torch.as_tensor(x * s), q, L * s, 0.4, M).item()
+        s[a] = 1 - h
+        em = O.p3m_energy(torch.as_tensor(x * s), q, L * s, 0.4, M).item()
+        assert abs(w[a, a] + (ep - em) / (2 * h)) <= 1e-7 * max(1.0, abs(E.item())), (a, w[a, a], (ep - em) / (2 * h))
+    np.testing.assert_allclose(w, w.T, atol=1e-12 * abs(E.item()))
