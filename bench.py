@@ -385,6 +385,49 @@ def npt(model, params, coord, box, types, vel, steps, warmup):
     return out
 
 
+def dplr(model, params, coord, box, types, vel, steps, warmup):
+    """DPLR NVE on the same 98,304-atom box: the energy model plus a seeded Wannier-centroid
+    model (same embedding widths, fitting 100x3, centroids on O), charges O +6, H +1, centroid
+    -8, PPPM beta 0.4 at resolution 5 (grid 198^3); device-resident steps/s."""
+    from deepmd_jax import _lib, synthetic as syn
+    from deepmd_jax.constants import MASS_UNIT_CONVERSION
+    wp, wv = syn.init_wannier_model(4)
+    wdm = _lib.DeviceModel(wp, wv)
+    s = _lib.DeviceSystem(model, types, 1)
+    s.set_box(box)
+    s.set_masses(np.array(syn.MASSES)[types] * MASS_UNIT_CONVERSION)
+    s.set_positions(coord)
+    s.set_velocities(vel)
+    s.nlist_allocate(params['rcut'] + 1.0, 1.2)
+    grid = (np.asarray(box) * 0.4 * 5).astype(int)
+    s.set_dplr(wdm, [6.0, 1.0], [-8.0, 0.0], 0.4, grid)
+    p = _lib.MDParams()
+    p.routine, p.dt = _lib.ROUTINE['NVE'], 0.5
+    out = {'workload': 'DPLR NVE, 98,304-atom water, centroid model on O, q (6, 1, -8), beta 0.4, grid %s'
+           % 'x'.join(str(int(g)) for g in grid)}
+
+    def run(n):
+        done = 0
+        while done < n:
+            k = min(100, n - done)
+            _, _, err = s.md_run(p, k, record=False)
+            if err & 16:
+                raise RuntimeError('NaN/Inf in the DPLR run')
+            if err & 2:
+                s.nlist_allocate(params['rcut'] + 1.0, 1.3)
+                continue
+            done += k
+        s.synchronize()
+    run(warmup)
+    t0 = time.perf_counter()
+    run(steps)
+    el = time.perf_counter() - t0
+    out['fp32'] = {'steps_per_s': steps / el, 'ms_per_step': 1e3 * el / steps,
+                   'long_range_energy_eV': s.dplr_get()[0]}
+    print('# dplr fp32: %.1f steps/s' % (steps / el), file=sys.stderr, flush=True)
+    return out
+
+
 def serving(model, rates, precisions, n_requests=256, md_chunk=20, budget_gib=32):
     """The serving stream: 256 water-box requests (n log-uniform in [32, 8192] H2O, first n
     sites of a seeded grid permutation), 70 % single E/F/virial evaluations, 30 % 200-step
@@ -434,6 +477,7 @@ def main():
     ap.add_argument('--no-pimd', action='store_true')
     ap.add_argument('--no-compressed', action='store_true')
     ap.add_argument('--no-npt', action='store_true')
+    ap.add_argument('--no-dplr', action='store_true')
     ap.add_argument('--nve-run-steps', type=int, default=2000,
                     help='length of the full NVE run reported beside the timed steps (0: skip)')
     ap.add_argument('--frames', type=int, default=1, help='seeded frames for the precision errors')
@@ -605,6 +649,8 @@ def main():
                                        acc=not args.no_acc)
     if not args.no_npt:
         out['npt'] = npt(model, params, coord, box, types, vel, args.steps, args.warmup)
+    if not args.no_dplr:
+        out['dplr'] = dplr(model, params, coord, box, types, vel, args.steps, args.warmup)
     if not args.no_pimd:
         out['pimd'] = pimd(model, params, var, steps=args.pimd_steps, acc_beads=0 if args.no_acc else args.pimd_acc_beads)
     if not args.no_serving:

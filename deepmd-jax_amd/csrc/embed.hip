@@ -188,6 +188,7 @@ struct EmbedArgs {
   const float* wc_y3;          // (mrows, wc_ldy)
   int wc_ldy;
   float wc_norm;
+  int centre_mask;             // bit t: centres of type t are evaluated (an atomic model's nsel)
 };
 
 // Minimum image of d = x_j - x_i for positions wrapped into [0, L) (so |d| < L): the
@@ -493,6 +494,7 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
   const int gc = S.base + c;
   int ti = 0;
   while (ti + 1 < a.ntypes && c >= S.off[ti + 1]) ++ti;
+  if (!(a.centre_mask >> ti & 1)) return;
   const float4 pc = a.pos[gc];
 
   floatx4 Tacc[T3];
@@ -693,7 +695,12 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
 
   const SegInfo& S = a.seg[a.blk_seg[blockIdx.x]];
   const int c = (blockIdx.x - S.blk0) * 4 + wv;
-  const bool active = c < S.natoms;
+  bool active = c < S.natoms;
+  if (active) {
+    int tc = 0;
+    while (tc + 1 < a.ntypes && c >= S.off[tc + 1]) ++tc;
+    active = (a.centre_mask >> tc & 1) != 0;
+  }
   float vir[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) vir[k] = 0.f;
@@ -1101,6 +1108,7 @@ static void fill_args(System* s, EmbedArgs& a) {
   a.wc_y3 = s->wc_y3;
   a.wc_ldy = s->wc_ldy;
   a.wc_norm = s->wc_norm;
+  a.centre_mask = M->atomic ? M->nsel_mask : ~0;
   a.paircnt = reinterpret_cast<int*>(s->d_pairs);
   int cmax = 1;
   for (int j = 0; j < s->ntypes; ++j) cmax = std::max(cmax, s->cap[j]);
