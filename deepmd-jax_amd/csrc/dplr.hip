@@ -92,15 +92,17 @@ __global__ void dplr_add_energy_kernel(double* red, const double* lr) {
   if (threadIdx.x < 10 && blockIdx.x == 0) red[threadIdx.x] += lr[threadIdx.x];
 }
 
-// lr[1..10) += W' (the centroid backward's block partials) + sum_k g_k (x) dx_k, one block
+// pass 1, per block (strided share): W' (the centroid backward's block partials) +
+// sum_k g_k (x) dx_k -> part[9 block + c]
 __global__ void dplr_virial_kernel(const double* partial, int nblocks, const float* wc_g, const double* qxyz,
-                                   const double* pos, const int* wc_atom, int NA, int nwc, double* lr) {
+                                   const double* pos, const int* wc_atom, int NA, int nwc, double* part) {
   __shared__ double red[256];
   double v[9];
   for (int c = 0; c < 9; ++c) v[c] = 0.0;
-  for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+  const int stride = gridDim.x * blockDim.x, t0 = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int b = t0; b < nblocks; b += stride)
     for (int c = 0; c < 9; ++c) v[c] += partial[(size_t)b * 9 + c];
-  for (int k = threadIdx.x; k < nwc; k += blockDim.x) {
+  for (int k = t0; k < nwc; k += stride) {
     const int g = wc_atom[k];
     double dx[3];
     for (int d = 0; d < 3; ++d) dx[d] = qxyz[(size_t)(NA + k) * 3 + d] - pos[(size_t)g * 3 + d];
@@ -114,9 +116,18 @@ __global__ void dplr_virial_kernel(const double* partial, int nblocks, const flo
       if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
       __syncthreads();
     }
-    if (threadIdx.x == 0) lr[1 + c] += red[0];
+    if (threadIdx.x == 0) part[blockIdx.x * 9 + c] = red[0];
     __syncthreads();
   }
+}
+
+// pass 2: lr[1..10) += the block partials, added in a fixed order
+__global__ void dplr_virial_sum_kernel(const double* part, int nb, double* lr) {
+  const int c = threadIdx.x;
+  if (c >= 9 || blockIdx.x != 0) return;
+  double s = 0.0;
+  for (int b = 0; b < nb; ++b) s += part[b * 9 + c];
+  lr[1 + c] += s;
 }
 
 }  // namespace
@@ -167,8 +178,11 @@ static int32_t dplr_run(System* s, Model* M) {
   s->wc_y3 = nullptr;
   s->partial_out = nullptr;
   DPMD_TRY(rc);
-  hipLaunchKernelGGL(dplr_virial_kernel, dim3(1), dim3(256), 0, st, s->d_partial_wc, s->nblocks, s->d_wc_g,
-                     s->d_qxyz, s->d_pos, s->d_wc_atom, NA, nwc, s->d_lr);
+  constexpr int kVirBlocks = 64;
+  double* vpart = s->d_lr + 10;
+  hipLaunchKernelGGL(dplr_virial_kernel, dim3(kVirBlocks), dim3(256), 0, st, s->d_partial_wc, s->nblocks,
+                     s->d_wc_g, s->d_qxyz, s->d_pos, s->d_wc_atom, NA, nwc, vpart);
+  hipLaunchKernelGGL(dplr_virial_sum_kernel, dim3(1), dim3(64), 0, st, vpart, kVirBlocks, s->d_lr);
   DPMD_HIP(hipGetLastError());
   return DPMD_OK;
 }
@@ -256,7 +270,7 @@ extern "C" int32_t dpmd_system_set_dplr(dpmd_system_t sys, dpmd_model_t wc_model
   DPMD_TRY(sys_ensure(s, (void**)&s->d_qxyz, sizeof(double) * 3 * nq));
   DPMD_TRY(sys_ensure(s, (void**)&s->d_q, sizeof(double) * nq));
   DPMD_TRY(sys_ensure(s, (void**)&s->d_qf, sizeof(double) * 3 * nq));
-  DPMD_TRY(sys_ensure(s, (void**)&s->d_lr, sizeof(double) * 10));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_lr, sizeof(double) * (10 + 9 * 64)));
   DPMD_TRY(sys_ensure(s, (void**)&s->d_wc_g, sizeof(float) * 3 * NA));
   DPMD_TRY(sys_ensure(s, (void**)&s->d_partial_wc, sizeof(double) * 9 * (size_t)std::max(s->nblocks, 1)));
   const size_t fitbytes = sizeof(float) * 10 * (size_t)std::max(s->mrows, 1) * W->fit_k;
