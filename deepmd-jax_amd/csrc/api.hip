@@ -206,7 +206,8 @@ static int32_t model_create(const dpmd_model_desc* d, int device, Model** out) {
     }
   M->fit_total = pos;
   std::vector<float> fit(pos, 0.f);
-  M->atomic = d->atomic != 0;
+  if (d->atomic < 0 || d->atomic > 2) { set_error("atomic must be 0, 1 or 2"); delete M; return DPMD_E_ARG; }
+  M->atomic = d->atomic;
   M->nsel_mask = d->nsel_mask;
   if (M->atomic && (d->fit_widths[2] != M->wout || d->fit_widths[0] != d->fit_widths[2])) {
     set_error("atomic models need fit_widths[2] == embed_widths[2] (and equal hidden widths)");
@@ -423,7 +424,8 @@ static int32_t sys_layout(System* s, int nseg, const int32_t* seg_natoms, const 
   s->npt_ready = false;
   s->has_fixed = false;
   s->n_mobile = NA;
-  s->wc_model = nullptr;  // DPLR maps belong to the old layout
+  s->wc_model = nullptr;  // DPLR / atomic-output maps belong to the old layout
+  s->atomic_maps_of = nullptr;
   return DPMD_OK;
 }
 

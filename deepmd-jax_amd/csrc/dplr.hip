@@ -88,6 +88,40 @@ __global__ void dplr_dy3_kernel(const float* wc_g, const float* T, const float* 
   DZ3[o] = d * rt.dt2[ty][w] * (1.f - th * th);
 }
 
+// atomic model outputs per selected atom: mode 1 the vector out_norm sum_w f_w T3[c][w];
+// mode 2 the tensor head Y = T6 + 3to6(T3) + TW (1,1,1,0,0,0), 6 components mapped to the 9 of a
+// symmetric 3x3 (deepmd_jax/dpmodel.py:90-99)
+__global__ void atomic_out_kernel(const float* T, const float* tbias, const float* Y3, int ldy, int W,
+                                  float out_norm, const int* wc_atom, const int* wc_row, int nwc, int mode,
+                                  double* out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nwc) return;
+  const int g = wc_atom[k];
+  const float* y = Y3 + (size_t)wc_row[k] * ldy;
+  const float* Tg = T + (size_t)g * 10 * W;
+  if (mode == 1) {
+    float v[3] = {0.f, 0.f, 0.f};
+    for (int w = 0; w < W; ++w)
+      for (int c = 0; c < 3; ++c) v[c] += y[w] * Tg[(1 + c) * W + w];
+    for (int c = 0; c < 3; ++c) out[(size_t)k * 3 + c] = (double)(out_norm * v[c]);
+    return;
+  }
+  const float r2 = 1.4142135623730951f;
+  float v[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  for (int w = 0; w < W; ++w) {
+    const float tw = Tg[w] + tbias[w];
+    const float t3[3] = {Tg[W + w], Tg[2 * W + w], Tg[3 * W + w]};
+    for (int c = 0; c < 3; ++c) {
+      v[c] += y[w] * (Tg[(4 + c) * W + w] + t3[c] * t3[c] + tw);
+      v[3 + c] += y[w] * (Tg[(7 + c) * W + w] + r2 * t3[c] * t3[(c + 2) % 3]);
+    }
+  }
+  const float s = 0.7071067811865476f;
+  const int src[9] = {0, 4, 3, 4, 1, 5, 3, 5, 2};
+  const float sc[9] = {1.f, s, s, s, 1.f, s, s, s, 1.f};
+  for (int e = 0; e < 9; ++e) out[(size_t)k * 9 + e] = (double)(out_norm * v[src[e]] * sc[e]);
+}
+
 __global__ void dplr_add_energy_kernel(double* red, const double* lr) {
   if (threadIdx.x < 10 && blockIdx.x == 0) red[threadIdx.x] += lr[threadIdx.x];
 }
@@ -221,7 +255,7 @@ extern "C" int32_t dpmd_system_set_dplr(dpmd_system_t sys, dpmd_model_t wc_model
   Model* W = reinterpret_cast<Model*>(wc_model);
   Model* M = s->model;
   if (!q_atoms || !q_wc || !grid) { set_error("null argument"); return DPMD_E_ARG; }
-  if (!W->atomic) { set_error("the DPLR centroid model must be an atomic model"); return DPMD_E_ARG; }
+  if (W->atomic != 1) { set_error("the DPLR centroid model must be an atomic vector model"); return DPMD_E_ARG; }
   if (W->device != M->device) { set_error("the centroid model lives on another device"); return DPMD_E_ARG; }
   if (W->ntypes != M->ntypes || W->wout != M->wout || W->axis != M->axis || W->lay.tiles[0] != M->lay.tiles[0] ||
       W->lay.tiles[1] != M->lay.tiles[1] || W->lay.tiles[2] != M->lay.tiles[2] || W->lay.z[1] != M->lay.z[1] ||
@@ -230,6 +264,10 @@ extern "C" int32_t dpmd_system_set_dplr(dpmd_system_t sys, dpmd_model_t wc_model
     return DPMD_E_UNSUPPORTED;
   }
   if (W->tab_n > 0) { set_error("tabulated centroid models are not supported"); return DPMD_E_UNSUPPORTED; }
+  if (s->nseg != 1 || s->seg[0].nimg != 1) {
+    set_error("DPLR runs one system held without periodic images");
+    return DPMD_E_UNSUPPORTED;
+  }
   if (W->rcut > M->rcut) {
     set_error("the centroid model's cutoff %.3f exceeds the energy model's %.3f (shared neighbour list)", W->rcut,
               M->rcut);
@@ -242,37 +280,56 @@ extern "C" int32_t dpmd_system_set_dplr(dpmd_system_t sys, dpmd_model_t wc_model
   if (!(beta > 0)) { set_error("DPLR needs beta > 0"); return DPMD_E_ARG; }
   for (int d = 0; d < 3; ++d)
     if (grid[d] < 2 || grid[d] > 4096) { set_error("PPPM grid size %d outside [2, 4096]", grid[d]); return DPMD_E_ARG; }
-  const int NA = s->NA;
-  const SegInfo& S = s->seg[0];
-  std::vector<int> wc_atom, wc_row, atom_wc(NA, -1);
+  DPMD_TRY(atomic_setup(s, W));
+  const int NA = s->NA, nwc = s->n_wc;
   std::vector<double> q;
   for (int g = 0; g < NA; ++g) q.push_back(q_atoms[s->type_sorted[g]]);
-  for (int g = 0; g < NA; ++g) {
-    const int t = s->type_sorted[g];
-    if (!(W->nsel_mask >> t & 1)) continue;
-    atom_wc[g] = (int)wc_atom.size();
-    wc_atom.push_back(g);
-    wc_row.push_back(s->rowbase[t] + S.row[t] + (g - S.base - S.off[t]));
-    q.push_back(q_wc[t]);
-  }
-  const int nwc = (int)wc_atom.size();
-  // centroid k -> rank among the selected atoms in caller order (wc_predict's order)
-  std::vector<int> order(nwc);
-  for (int k = 0; k < nwc; ++k) order[k] = k;
-  std::sort(order.begin(), order.end(), [&](int a, int b) { return s->perm[wc_atom[a]] < s->perm[wc_atom[b]]; });
-  s->wc_perm.assign(nwc, 0);
-  for (int r = 0; r < nwc; ++r) s->wc_perm[order[r]] = r;
+  for (int g = 0; g < NA; ++g)
+    if (W->nsel_mask >> s->type_sorted[g] & 1) q.push_back(q_wc[s->type_sorted[g]]);
   const int nq = NA + nwc;
-  const size_t nn = std::max(nwc, 1);
-  DPMD_TRY(sys_ensure(s, (void**)&s->d_wc_atom, sizeof(int) * nn));
-  DPMD_TRY(sys_ensure(s, (void**)&s->d_wc_row, sizeof(int) * nn));
-  DPMD_TRY(sys_ensure(s, (void**)&s->d_atom_wc, sizeof(int) * NA));
   DPMD_TRY(sys_ensure(s, (void**)&s->d_qxyz, sizeof(double) * 3 * nq));
   DPMD_TRY(sys_ensure(s, (void**)&s->d_q, sizeof(double) * nq));
   DPMD_TRY(sys_ensure(s, (void**)&s->d_qf, sizeof(double) * 3 * nq));
   DPMD_TRY(sys_ensure(s, (void**)&s->d_lr, sizeof(double) * (10 + 9 * 64)));
   DPMD_TRY(sys_ensure(s, (void**)&s->d_wc_g, sizeof(float) * 3 * NA));
   DPMD_TRY(sys_ensure(s, (void**)&s->d_partial_wc, sizeof(double) * 9 * (size_t)std::max(s->nblocks, 1)));
+  const hipStream_t st = s->stream;
+  DPMD_HIP(hipMemcpyAsync(s->d_q, q.data(), sizeof(double) * nq, hipMemcpyHostToDevice, st));
+  DPMD_HIP(hipMemsetAsync(s->d_lr, 0, sizeof(double) * 10, st));
+  if (!s->pppm) s->pppm = new Pppm();
+  for (int d = 0; d < 3; ++d) s->dplr_grid[d] = grid[d];
+  s->dplr_beta = beta;
+  DPMD_TRY(s->pppm->setup(s->dplr_grid, beta));
+  DPMD_HIP(hipStreamSynchronize(st));
+  s->wc_model = W;
+  return DPMD_OK;
+}
+
+int32_t dpmd::atomic_setup(System* s, Model* W) {
+  const int NA = s->NA;
+  std::vector<int> wc_atom, wc_row, atom_wc(NA, -1);
+  for (int r = 0; r < s->nseg; ++r) {
+    const SegInfo& S = s->seg[r];
+    for (int c = 0; c < S.natoms; ++c) {
+      const int g = S.base + c;
+      const int t = s->type_sorted[g];
+      if (!(W->nsel_mask >> t & 1)) continue;
+      atom_wc[g] = (int)wc_atom.size();
+      wc_atom.push_back(g);
+      wc_row.push_back(s->rowbase[t] + S.row[t] + (c - S.off[t]));
+    }
+  }
+  const int nwc = (int)wc_atom.size();
+  // output k -> rank among the selected atoms in caller order (segment-major)
+  std::vector<int> order(nwc);
+  for (int k = 0; k < nwc; ++k) order[k] = k;
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return s->perm[wc_atom[a]] < s->perm[wc_atom[b]]; });
+  s->wc_perm.assign(nwc, 0);
+  for (int r = 0; r < nwc; ++r) s->wc_perm[order[r]] = r;
+  const size_t nn = std::max(nwc, 1);
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_wc_atom, sizeof(int) * nn));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_wc_row, sizeof(int) * nn));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_atom_wc, sizeof(int) * NA));
   const size_t fitbytes = sizeof(float) * 10 * (size_t)std::max(s->mrows, 1) * W->fit_k;
   DPMD_TRY(sys_ensure(s, (void**)&s->d_wc_fit, fitbytes));
   const hipStream_t st = s->stream;
@@ -282,15 +339,38 @@ extern "C" int32_t dpmd_system_set_dplr(dpmd_system_t sys, dpmd_model_t wc_model
     DPMD_HIP(hipMemcpyAsync(s->d_wc_row, wc_row.data(), sizeof(int) * nwc, hipMemcpyHostToDevice, st));
   }
   DPMD_HIP(hipMemcpyAsync(s->d_atom_wc, atom_wc.data(), sizeof(int) * NA, hipMemcpyHostToDevice, st));
-  DPMD_HIP(hipMemcpyAsync(s->d_q, q.data(), sizeof(double) * nq, hipMemcpyHostToDevice, st));
-  DPMD_HIP(hipMemsetAsync(s->d_lr, 0, sizeof(double) * 10, st));
-  if (!s->pppm) s->pppm = new Pppm();
-  for (int d = 0; d < 3; ++d) s->dplr_grid[d] = grid[d];
-  s->dplr_beta = beta;
-  DPMD_TRY(s->pppm->setup(s->dplr_grid, beta));
   DPMD_HIP(hipStreamSynchronize(st));
   s->n_wc = nwc;
-  s->wc_model = W;
+  s->atomic_maps_of = W;
+  return DPMD_OK;
+}
+
+extern "C" int32_t dpmd_atomic_predict(dpmd_system_t sys, double* out) {
+  SYS(sys);
+  Model* M = s->model;
+  if (!out) { set_error("null argument"); return DPMD_E_ARG; }
+  if (!M->atomic) { set_error("the system's model is not an atomic model"); return DPMD_E_STATE; }
+  if (!s->nl_ready) { set_error("no neighbour list: call dpmd_nlist_allocate first"); return DPMD_E_STATE; }
+  if (s->wc_model) { set_error("an atomic model's system carries no DPLR term"); return DPMD_E_STATE; }
+  if (s->atomic_maps_of != M) DPMD_TRY(atomic_setup(s, M));
+  DPMD_TRY(sys_upload_segments(s));
+  DPMD_TRY(md_write_pos32(s));
+  DPMD_TRY(embed_launch(s, false, DPMD_PREC_FP32));
+  AtomicFitBufs b;
+  DPMD_TRY(fit_atomic_buffers(s, M, b));
+  DPMD_TRY(fit_atomic_fwd(s, b));
+  const int nwc = s->n_wc, ncomp = M->atomic == 1 ? 3 : 9;
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_qf, sizeof(double) * ncomp * (size_t)std::max(nwc, 1)));
+  if (nwc > 0)
+    hipLaunchKernelGGL(atomic_out_kernel, dim3((nwc + 127) / 128), dim3(128), 0, s->stream, s->d_T, M->d_tbias, b.Y3,
+                       b.FK, M->wout, (float)M->out_norm, s->d_wc_atom, s->d_wc_row, nwc, M->atomic, s->d_qf);
+  DPMD_HIP(hipGetLastError());
+  std::vector<double> h((size_t)ncomp * std::max(nwc, 1));
+  DPMD_HIP(hipMemcpyAsync(h.data(), s->d_qf, sizeof(double) * ncomp * nwc, hipMemcpyDeviceToHost, s->stream));
+  DPMD_HIP(hipStreamSynchronize(s->stream));
+  for (int k = 0; k < nwc; ++k)
+    for (int e = 0; e < ncomp; ++e) out[(size_t)s->wc_perm[k] * ncomp + e] = h[(size_t)k * ncomp + e];
+  s->force_valid = false;
   return DPMD_OK;
 }
 

@@ -113,9 +113,9 @@ struct Model {
   int tab_stride = 0;              // floats per net
   float tab_srmin[kMaxT * kMaxT], tab_delta[kMaxT * kMaxT];
   float* d_tab = nullptr;
-  // atomic vector model (Wannier centroids): hidden fitting layers only, output for the
-  // centre types in nsel_mask
-  bool atomic = false;
+  // atomic model: hidden fitting layers only, output for the centre types in nsel_mask;
+  // 1 vector (Wannier centroids), 2 symmetric tensor (atomic_t2), 0 energy model
+  int atomic = 0;
   int nsel_mask = 0;
 };
 void embed_pack_table(const EmbedLayout& L, int ngrids, const float* coeff, float* out);
@@ -263,6 +263,7 @@ struct System {
   float* d_wc_fit = nullptr;    // 10 activation buffers (mrows x fit_k) of the centroid model,
                                 // zeroed once: columns past the fitting width stay zero
   std::vector<int> wc_perm;     // centroid k -> rank among the selected atoms in caller order
+  const Model* atomic_maps_of = nullptr;  // model whose selection the maps above follow
   // embedding-backward hooks, set only while the centroid model's backward runs: the extra
   // dE/dT3[c][w] = wc_norm wc_g[atom][c] wc_y3[row][w], and the virial partial output
   const float* wc_g = nullptr;
@@ -296,6 +297,9 @@ int32_t fit_atomic_buffers(System* s, const Model* M, AtomicFitBufs& b);
 int32_t fit_atomic_fwd(System* s, const AtomicFitBufs& b);
 int32_t fit_atomic_bwd(System* s, const AtomicFitBufs& b);
 int32_t dplr_forces(System* s);
+// maps and buffers of an atomic model's outputs on the system's layout (selected atoms, their
+// fitting rows, caller-order ranks)
+int32_t atomic_setup(System* s, Model* W);
 int32_t dplr_add_energy(System* s);
 int32_t md_zero_force(System* s);
 int32_t md_kick(System* s, double dt_half);

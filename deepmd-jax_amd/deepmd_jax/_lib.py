@@ -94,6 +94,7 @@ def lib():
         'dpmd_pppm_compute': (c_i32, [c_i32, c_vp, c_f64, c_vp, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp]),
         'dpmd_system_set_dplr': (c_i32, [c_vp, c_vp, c_vp, c_vp, c_f64, c_vp]),
         'dpmd_dplr_get': (c_i32, [c_vp, c_vp, c_vp]),
+        'dpmd_atomic_predict': (c_i32, [c_vp, c_vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -130,7 +131,7 @@ def exported_symbols():
             'dpmd_thermostat_state_get', 'dpmd_thermostat_state_set', 'dpmd_timing_enable',
             'dpmd_timing_get', 'dpmd_pair_count', 'dpmd_synchronize', 'dpmd_system_get_boxes',
             'dpmd_md_box_trajectory', 'dpmd_npt_state_get', 'dpmd_nlist_rebuilds', 'dpmd_system_set_fixed',
-            'dpmd_pppm_compute', 'dpmd_system_set_dplr', 'dpmd_dplr_get']
+            'dpmd_pppm_compute', 'dpmd_system_set_dplr', 'dpmd_dplr_get', 'dpmd_atomic_predict']
 
 
 def pppm_compute(box3, beta, grid, xyz, q, device=0):
@@ -160,8 +161,9 @@ class DeviceModel:
         atomic = bool(params.get('atomic', False))
         if params.get('use_mp', False):
             raise NotImplementedError('only the se_a model (no message passing) is supported')
-        if atomic and params.get('type', 'atomic') != 'atomic':
-            raise NotImplementedError("atomic models of type %r are not supported (only 'atomic')" % params.get('type'))
+        kind = params.get('type', 'atomic')
+        if atomic and kind not in ('atomic', 'atomic_t2'):
+            raise NotImplementedError("atomic models of type %r are not supported" % kind)
         if atomic and params.get('is_compressed', False):
             raise NotImplementedError('compressed atomic models are not supported')
         if not params.get('use_2nd', True):
@@ -199,7 +201,7 @@ class DeviceModel:
             # outside nsel carry no output and get the first selected type's nets
             nsel = [int(t) for t in params['nsel']]
             pos = {t: k for k, t in enumerate(nsel)}
-            d.atomic = 1
+            d.atomic = 1 if kind == 'atomic' else 2
             d.nsel_mask = sum(1 << t for t in nsel)
             P = dict(P)
             src = {}
@@ -252,6 +254,7 @@ class DeviceModel:
         self.params = params
         self.ntypes = nt
         self.nsel_mask = int(d.nsel_mask)
+        self.atomic = int(d.atomic)
         self.rcut = float(params['rcut'])
 
     def __del__(self):
@@ -437,6 +440,17 @@ class DeviceSystem:
         self._wc_model = wc_model  # keeps the centroid model alive while the system uses it
         self._n_wc = int(sum(np.isin(self.type_idx, [t for t in range(wc_model.ntypes)
                                                      if int(wc_model.nsel_mask) >> t & 1])))
+
+    def atomic_predict(self):
+        """Outputs of the system's atomic model: (nrep, M, 3) vectors or (nrep, M, 9) symmetric
+        tensors for the M atoms of the selected types per replica, in caller order."""
+        ncomp = 3 if self.model.atomic == 1 else 9
+        sel = [t for t in range(self.model.ntypes) if self.model.nsel_mask >> t & 1]
+        m = int(np.isin(self.type_idx, sel).sum()) * (1 if self.ragged else self.nrep)
+        out = np.zeros(max(m, 1) * ncomp)
+        check(lib().dpmd_atomic_predict(self.handle, ptr(out)))
+        out = out[:m * ncomp]
+        return out.reshape(-1, ncomp) if self.ragged else out.reshape(self.nrep, -1, ncomp)
 
     def dplr_get(self):
         """(long-range energy, centroids (n_wc, 3)) of the last evaluation."""

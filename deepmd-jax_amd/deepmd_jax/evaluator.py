@@ -116,6 +116,28 @@ class Evaluator:
         e, f, v, _ = sys.compute(precision)
         return e / nk, f[:, :N], v / nk
 
+    def evaluate_atomic_batch(self, coords, box3, type_idx):
+        """Atomic model outputs of frames sharing one box: (F, M, 3) vectors or (F, M, 9) tensors for
+        the M atoms of the selected types, in caller order."""
+        coords = np.asarray(coords, np.float64)
+        box3 = canonical_box(box3)
+        type_idx = np.asarray(type_idx)
+        k = supercell_factors(box3, self.rcut)
+        nk = int(np.prod(k))
+        F = coords.shape[0]
+        if nk > 1:
+            cs = [replicate(c, box3, type_idx, k) for c in coords]
+            sc, sbox, stype = np.stack([c[0] for c in cs]), cs[0][1], cs[0][2]
+        else:
+            sc, sbox, stype = coords, box3, type_idx
+        sys = self._system(stype, F)
+        sys.set_box(sbox)
+        sys.set_positions(sc)
+        sys.nlist_allocate(self.rcut, 1.2)
+        sel = [int(t) for t in self.params['nsel']]
+        m = int(np.isin(type_idx, sel).sum())
+        return sys.atomic_predict()[:, :m]  # image 0 holds the original atoms
+
     def evaluate_frame(self, coord, box3, type_idx, precision='fp32'):
         e, f, v = self.evaluate_batch(np.asarray(coord)[None], box3, type_idx, precision)
         return e[0], f[0], v[0]

@@ -156,12 +156,14 @@ def init_model(seed=0, coord=None, box=None, type_idx=None, rcut=6.0,
 
 
 def init_wannier_model(seed=0, coord=None, box=None, type_idx=None, rcut=6.0, embed_widths=(25, 50, 100),
-                       fit_widths=(100, 100, 100), axis=16, ntypes=2, nsel=(0,), fit_dt=0.1, out_norm=0.05):
+                       fit_widths=(100, 100, 100), axis=16, ntypes=2, nsel=(0,), fit_dt=0.1, out_norm=0.05,
+                       kind='atomic'):
     """Seeded random-init atomic vector model (type 'atomic', a Wannier-centroid model as
     deepmd_jax/train.py:121-150 builds it): embedding nets for the centre types in nsel
     (net k = position in nsel * ntypes + neighbour type), one fitting net of hidden layers
     per nsel type with fit_widths[-1] == embed_widths[-1], outputs scaled by out_norm (the
-    displacements stay a fraction of an Angstrom)."""
+    displacements stay a fraction of an Angstrom).  kind='atomic_t2' gives the symmetric-tensor
+    (polarizability) head instead of the vector."""
     if fit_widths[-1] != embed_widths[-1]:
         raise ValueError('For atomic models, embed_widths[-1] must equal fit_widths[-1].')
     params, var = init_model(seed, coord, box, type_idx, rcut, embed_widths, fit_widths, axis, ntypes, fit_dt)
@@ -173,6 +175,6 @@ def init_wannier_model(seed=0, coord=None, box=None, type_idx=None, rcut=6.0, em
         net = dict(P['fitting_net_%d' % i])
         net.pop('Dense_%d' % len(fit_widths))
         out['fitting_net_%d' % ci] = net
-    params.update({'type': 'atomic', 'atomic': True, 'nsel': list(nsel), 'out_norm': float(out_norm),
-                   'atomic_data_prefix': 'atomic_dipole'})
+    params.update({'type': kind, 'atomic': True, 'nsel': list(nsel), 'out_norm': float(out_norm),
+                   'atomic_data_prefix': 'atomic_dipole' if kind == 'atomic' else 'atomic_polarizability'})
     return params, {'params': out}

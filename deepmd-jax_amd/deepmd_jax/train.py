@@ -43,8 +43,9 @@ def evaluate(model_path, coord, box, type_idx, batch_size=1, precision='fp32'):
                          'type_idx (n_atoms).')
     model, variables = load_model(model_path, replicate=False)
     params = model.params
-    if params.get('type', 'energy') not in ('energy', 'dplr'):
-        raise NotImplementedError('evaluate() supports energy and DPLR models')
+    kind = params.get('type', 'energy')
+    if kind not in ('energy', 'dplr', 'atomic', 'atomic_t2'):
+        raise NotImplementedError('evaluate() supports energy, DPLR, atomic and atomic_t2 models')
     ct = params.get('chemical_types')
     if ct is not None:
         unknown = set(type_idx.tolist()) - set(ct)
@@ -57,6 +58,18 @@ def evaluate(model_path, coord, box, type_idx, batch_size=1, precision='fp32'):
         raise ValueError('type index out of trained scope of model (ntypes=%d)' % params['ntypes'])
     ev = Evaluator(params, variables)
     F = coord.shape[0]
+    if kind in ('atomic', 'atomic_t2'):
+        # atomic outputs per frame (deepmd_jax/train.py:765-788): (F, M, 3) or (F, M, 9)
+        key = params.get('atomic_data_prefix') or ('atomic_dipole' if kind == 'atomic' else 'atomic_polarizability')
+        outs = []
+        i = 0
+        while i < F:
+            j = i + 1
+            while j < F and j - i < batch_size and np.array_equal(boxes[j], boxes[i]):
+                j += 1
+            outs.append(ev.evaluate_atomic_batch(coord[i:j], boxes[i], type_idx))
+            i = j
+        return {key: np.concatenate(outs, 0)}
     energy = np.zeros(F)
     force = np.zeros(coord.shape)
     i = 0

@@ -80,11 +80,13 @@ typedef struct {
   float tab_srmin[DPMD_MAX_TYPES * DPMD_MAX_TYPES];
   float tab_srmax[DPMD_MAX_TYPES * DPMD_MAX_TYPES];
   const float* tab_coeff[DPMD_MAX_TYPES * DPMD_MAX_TYPES];
-  /* Atomic vector model (params['atomic'], type 'atomic', deepmd_jax/dpmodel.py:83-98), the
-   * Wannier-centroid model of DPLR: nonzero selects it.  The fitting nets are hidden layers only
-   * (fit_kernel[t][3] unused, fit_widths[2] == embed_widths[2]); an atom of a type in nsel_mask
-   * (bit t) gets out_norm * sum_w f_w T3[:, w].  Nets of centre types outside the mask are not
-   * evaluated for output (any weights of the right shapes may be passed for them). */
+  /* Atomic models (params['atomic'], deepmd_jax/dpmodel.py:83-99): 1 = type 'atomic' (a vector
+   * per atom, the Wannier-centroid model of DPLR), 2 = type 'atomic_t2' (a symmetric 3x3 tensor
+   * per atom), 0 = energy model.  The fitting nets are hidden layers only (fit_kernel[t][3] unused,
+   * fit_widths[2] == embed_widths[2]); an atom of a type in nsel_mask (bit t) gets
+   * out_norm * sum_w f_w Y[:, w] with Y = T3 (vector) or T6 + 3to6(T3) + TW (1,1,1,0,0,0)
+   * mapped 6 -> 9 (tensor).  Nets of centre types outside the mask are not evaluated (any weights
+   * of the right shapes may be passed for them). */
   int32_t atomic;
   int32_t nsel_mask;
 } dpmd_model_desc;
@@ -252,6 +254,11 @@ int32_t dpmd_md_box_trajectory(dpmd_system_t sys, double* boxes);
  * runs with it.  wc_model NULL removes the term. */
 int32_t dpmd_system_set_dplr(dpmd_system_t sys, dpmd_model_t wc_model, const double* q_atoms, const double* q_wc,
                              double beta, const int32_t* grid);
+/* Outputs of an atomic model (the system's model has desc.atomic != 0) at the current positions:
+ * for every atom of the selected types, segment by segment in caller order, 3 components
+ * (vector) or 9 (symmetric tensor, row-major) in FP64 (FP32 arithmetic).  Replaces the
+ * atomic branch of DPModel.apply / evaluate (deepmd_jax/dpmodel.py:83-99, train.py:765-788). */
+int32_t dpmd_atomic_predict(dpmd_system_t sys, double* out);
 /* Of the last evaluation with DPLR set: the long-range energy and the centroid positions (n_wc x 3,
  * raw order among the atoms of the selected types, as wc_predict returns them), either may be NULL. */
 int32_t dpmd_dplr_get(dpmd_system_t sys, double* e_long_range, double* centroids);

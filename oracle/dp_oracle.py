@@ -280,9 +280,10 @@ def fitting_hidden(net, x):
 
 
 def dp_atomic(params, variables, coord, box3, type_idx, nbrs_nm=None, lattice=None):
-    """Atomic vector model (params['type'] == 'atomic', deepmd_jax/dpmodel.py:83-98): for each
-    atom of the selected types params['nsel'], out_norm * sum_w f_w T3[:, w], f the last hidden
-    layer of the fitting net of its type; rows in raw atom order among the selected atoms
+    """Atomic models (deepmd_jax/dpmodel.py:83-99): for each atom of the selected types
+    params['nsel'], out_norm * sum_w f_w T3[:, w] (type 'atomic', a vector) or the symmetric 3x3
+    tensor of the T6 + 3to6(T3) + TW head (type 'atomic_t2', 9 components row-major), f the last
+    hidden layer of the fitting net of its type; rows in raw atom order among the selected atoms
     (atomic_inverse_perm, deepmd_jax/utils.py:380-390).  Embedding net k = (position of the
     centre type in nsel) * ntypes + neighbour type; fitting net t = position in nsel."""
     P = variables['params']
@@ -293,11 +294,21 @@ def dp_atomic(params, variables, coord, box3, type_idx, nbrs_nm=None, lattice=No
     cnt = [int(type_count[t]) for t in nsel]
     off = np.concatenate([[0], np.cumsum(cnt)])
     T3 = T[:, 1:4]
+    if params.get('type') == 'atomic_t2':
+        # symmetric tensor head: T6 + 3to6(T3) + TW (1,1,1,0,0,0), then 6 -> 9 (dpmodel.py:90-99)
+        TW = T[:, 0] + torch.as_tensor(P['Tbias'], dtype=T.dtype)
+        TY = T[:, 4:] + tensor_3to6(T3, axis=1) + TW[:, None] * torch.as_tensor([1., 1, 1, 0, 0, 0], dtype=T.dtype)[:, None]
+    else:
+        TY = T3
     pred = []
     for ci in range(len(nsel)):
         f = fitting_hidden(P['fitting_net_%d' % ci], G[off[ci]:off[ci + 1]])
-        pred.append((f[:, None, :] * T3[off[ci]:off[ci + 1]]).sum(-1))
-    pred = torch.cat(pred) * params['out_norm']
+        pred.append((f[:, None, :] * TY[off[ci]:off[ci + 1]]).sum(-1))
+    pred = torch.cat(pred)
+    if params.get('type') == 'atomic_t2':
+        s = 2 ** -0.5
+        pred = pred[:, [0, 4, 3, 4, 1, 5, 3, 5, 2]] * torch.as_tensor([1, s, s, s, 1, s, s, s, 1], dtype=pred.dtype)
+    pred = pred * params['out_norm']
     type_idx = np.asarray(type_idx)
     order = np.concatenate([np.where(type_idx == t)[0] for t in nsel])
     return pred[torch.as_tensor(np.argsort(order, kind='stable'))]

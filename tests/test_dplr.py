@@ -92,3 +92,20 @@ def test_dplr_forces_are_the_energy_gradient(model0):
             ep = O.dplr_energy_force(*args, xp, box, types, q_atoms, q_wc, 0.4, M, lattice=lat)[0]
             em = O.dplr_energy_force(*args, xm, box, types, q_atoms, q_wc, 0.4, M, lattice=lat)[0]
             assert abs(-(ep - em) / (2 * h) - F[i, d]) < 1e-6 * max(1.0, abs(F[i, d])), (i, d)
+
+
+def test_tensor_model_is_a_symmetric_tensor():
+    """atomic_t2 head (deepmd_jax/dpmodel.py:90-99): symmetric 3x3 per selected atom that rotates as
+    R A R^T with an isolated cluster."""
+    from deepmd_jax import synthetic as syn
+    params, var = syn.init_wannier_model(5, kind='atomic_t2')
+    coord, box, types = _cluster(1)
+    lat = O.lattice_candidate(np.diag(box), params['rcut'])
+    f = lambda x: O.dp_atomic(params, var, torch.as_tensor(x + 20.0), box, types, lattice=lat).detach().numpy()
+    a0 = f(coord).reshape(-1, 3, 3)
+    np.testing.assert_allclose(a0, a0.transpose(0, 2, 1), atol=1e-12)
+    assert np.abs(a0).max() > 1e-4
+    rng = np.random.default_rng(4)
+    Q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+    a1 = f(coord @ Q.T).reshape(-1, 3, 3)
+    np.testing.assert_allclose(a1, Q @ a0 @ Q.T, atol=1e-10)

@@ -10,6 +10,7 @@ oracle's; NPT with DPLR conserves its invariant.
 """
 import numpy as np
 import pytest
+import torch
 
 from oracle import dp_oracle as O
 
@@ -159,3 +160,36 @@ def test_dplr_npt_invariant(model0, tmp_path):
     assert np.isfinite(log).all()
     assert np.abs(traj['box'][-1] - box).max() > 1e-6
     assert np.abs(inv - inv[0]).max() / len(types) < 5e-5, inv
+
+
+@pytest.mark.parametrize('kind', ['atomic', 'atomic_t2'])
+def test_evaluate_atomic_models(tmp_path, kind):
+    """evaluate() of 'atomic' and 'atomic_t2' model.pkl files (deepmd_jax/train.py:765-788) against
+    the oracle: 2 frames of 64 H2O (small box: image supercell) and 216 H2O, outputs within 1e-5
+    of the largest component."""
+    import contextlib
+    import io
+    from deepmd_jax import synthetic as syn
+    from deepmd_jax.dpmodel import DPModel
+    from deepmd_jax.train import evaluate
+    from deepmd_jax.utils import save_model
+    print('case evaluate', kind, flush=True)
+    wp, wv = syn.init_wannier_model(6, kind=kind)
+    path = str(tmp_path / ('%s.pkl' % kind))
+    with contextlib.redirect_stdout(io.StringIO()):
+        save_model(path, DPModel(wp), wv)
+    for nmol in (64, 216):
+        frames = [syn.water_box(nmol, seed=s) for s in (50, 51)]
+        coord = np.stack([f[0] for f in frames])
+        box = np.stack([f[1] for f in frames])
+        types = frames[0][2]
+        with contextlib.redirect_stdout(io.StringIO()):
+            out = evaluate(path, coord, box, types, batch_size=2)
+        key = wp['atomic_data_prefix']
+        got = out[key]
+        ncomp = 3 if kind == 'atomic' else 9
+        assert got.shape == (2, (types == 0).sum(), ncomp)
+        for f in range(2):
+            lat = O.lattice_candidate(np.diag(box[f]), wp['rcut'])
+            want = O.dp_atomic(wp, wv, torch.as_tensor(coord[f]), box[f], types, lattice=lat).numpy()
+            assert np.abs(got[f] - want).max() <= 1e-5 * np.abs(want).max(), (nmol, f)
