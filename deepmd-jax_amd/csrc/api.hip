@@ -387,7 +387,7 @@ static int32_t sys_layout(System* s, int nseg, const int32_t* seg_natoms, const 
   DPMD_TRY(sys_ensure(s, (void**)&s->d_pos32, sizeof(float4) * NA));
   DPMD_TRY(sys_ensure(s, (void**)&s->d_force, sizeof(float) * NA * 3));
   DPMD_TRY(sys_ensure(s, (void**)&s->d_scratch, sizeof(double) * NA));
-  DPMD_TRY(sys_ensure(s, (void**)&s->d_flags, sizeof(int) * 16));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_flags, sizeof(int) * 32));
   DPMD_TRY(sys_ensure(s, (void**)&s->d_thermo, sizeof(double) * 64 * nseg));
   DPMD_TRY(sys_ensure(s, (void**)&s->d_T, sizeof(float) * (size_t)NA * 10 * W));
   DPMD_TRY(sys_ensure(s, (void**)&s->d_G, sizeof(float) * mr * D));
@@ -405,7 +405,7 @@ static int32_t sys_layout(System* s, int nseg, const int32_t* seg_natoms, const 
   DPMD_HIP(hipMemsetAsync(s->d_pos, 0, sizeof(double) * NA * 3, st));
   DPMD_HIP(hipMemsetAsync(s->d_mom, 0, sizeof(double) * NA * 3, st));
   DPMD_HIP(hipMemsetAsync(s->d_force, 0, sizeof(float) * NA * 3, st));
-  DPMD_HIP(hipMemsetAsync(s->d_flags, 0, sizeof(int) * 16, st));
+  DPMD_HIP(hipMemsetAsync(s->d_flags, 0, sizeof(int) * 32, st));
   DPMD_HIP(hipMemsetAsync(s->d_thermo, 0, sizeof(double) * 64 * nseg, st));
   DPMD_HIP(hipMemsetAsync(s->d_pairs, 0, sizeof(int) * NA, st));
   // padding rows of G and the K-padding columns of the fitting activations must read as zero
@@ -1254,6 +1254,16 @@ int32_t dpmd_system_set_fixed(dpmd_system_t sys, const int32_t* fixed) {
   if (s->has_fixed) DPMD_TRY(md_mask_momenta(s));
   DPMD_HIP(hipStreamSynchronize(s->stream));
   s->force_valid = false;
+  return DPMD_OK;
+}
+
+int32_t dpmd_nlist_update_rate(dpmd_system_t sys, double* rate) {
+  SYS(sys);
+  if (!rate) { set_error("null argument"); return DPMD_E_ARG; }
+  float f = 0.f;
+  DPMD_HIP(hipMemcpyAsync(&f, s->d_flags + 16, sizeof(float), hipMemcpyDeviceToHost, s->stream));
+  DPMD_HIP(hipStreamSynchronize(s->stream));
+  *rate = f;
   return DPMD_OK;
 }
 

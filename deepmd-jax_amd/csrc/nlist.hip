@@ -253,10 +253,14 @@ __global__ void nl_drift_kernel(const double* pos, const float4* canon, const Se
 }
 
 // flags[9] counts the rebuilds the drift test triggered since the flags were cleared
+// flags[16] (not cleared per chunk) holds, as float bits, the moving average of the rebuild
+// decisions (profile * (1 - 1/100) + update / 100, deepmd_jax/md.py:733)
 __global__ void nl_decide_kernel(int* flags) {
   flags[0] = flags[1] != 0 ? 1 : 0;
   flags[9] += flags[0];
   flags[1] = 0;
+  float* prof = reinterpret_cast<float*>(flags + 16);
+  prof[0] = prof[0] * (1.f - 1.f / 100.f) + 1.f / 100.f * (float)flags[0];
 }
 
 __global__ void nl_set_flag_kernel(int* flags, int v) { flags[0] = v; }

@@ -95,6 +95,7 @@ def lib():
         'dpmd_system_set_dplr': (c_i32, [c_vp, c_vp, c_vp, c_vp, c_f64, c_vp]),
         'dpmd_dplr_get': (c_i32, [c_vp, c_vp, c_vp]),
         'dpmd_atomic_predict': (c_i32, [c_vp, c_vp]),
+        'dpmd_nlist_update_rate': (c_i32, [c_vp, c_vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -131,7 +132,8 @@ def exported_symbols():
             'dpmd_thermostat_state_get', 'dpmd_thermostat_state_set', 'dpmd_timing_enable',
             'dpmd_timing_get', 'dpmd_pair_count', 'dpmd_synchronize', 'dpmd_system_get_boxes',
             'dpmd_md_box_trajectory', 'dpmd_npt_state_get', 'dpmd_nlist_rebuilds', 'dpmd_system_set_fixed',
-            'dpmd_pppm_compute', 'dpmd_system_set_dplr', 'dpmd_dplr_get', 'dpmd_atomic_predict']
+            'dpmd_pppm_compute', 'dpmd_system_set_dplr', 'dpmd_dplr_get', 'dpmd_atomic_predict',
+            'dpmd_nlist_update_rate']
 
 
 def pppm_compute(box3, beta, grid, xyz, q, device=0):
@@ -463,6 +465,11 @@ class DeviceSystem:
         """Fixed-atom mask in caller order (flat over all atoms of all replicas), or None."""
         f = None if fixed is None else np.ascontiguousarray(np.asarray(fixed).reshape(-1), np.int32)
         check(lib().dpmd_system_set_fixed(self.handle, ptr(f)))
+
+    def nlist_update_rate(self):
+        r = np.zeros(1)
+        check(lib().dpmd_nlist_update_rate(self.handle, ptr(r)))
+        return float(r[0])
 
     def nlist_rebuilds(self):
         """Drift-triggered neighbour-list rebuilds inside md_run since the last allocation."""

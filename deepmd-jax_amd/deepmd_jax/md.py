@@ -255,6 +255,10 @@ class Simulation:
         # 2*(rcut+skin) <= L, so the skin is shrunk to fit; a box below 2*rcut is held as a
         # k1 x k2 x k3 periodic supercell whose images move together (the lattice-image sum).
         k = np.ones(3, int)
+        # the reference's own neighbour-list condition (deepmd_jax/md.py:450-456): its debug
+        # report shows the neighbour update rate only then
+        self._reference_nlist = bool(use_neighbor_list_when_possible) and self._box.min() > 2 * (
+            self._rcut + self._neighbor_skin) * (1.02 if routine == 'NPT' else 1.0)
         if routine == 'NPT' and (self._box.min() <= 2 * (self._rcut + self._neighbor_skin) * 1.02
                                  or not use_neighbor_list_when_possible):
             # the reference's NPT condition for its neighbour list (deepmd_jax/md.py:450-456)
@@ -455,12 +459,17 @@ class Simulation:
     def _print_report(self):
         rep = self._report_values()
         keys = list(rep.keys())
+        rate = self._debug and self._reference_nlist  # NbrUpdateRate column (deepmd_jax/md.py:583-592)
         if self._is_initial_state:
-            print(' '.join(['%-8s' % 'Step'] + ['%-12s' % k for k in keys] + ['%-6s' % 'Time']))
+            print(' '.join(['%-8s' % 'Step'] + ['%-12s' % k for k in keys] + ['%-6s' % 'Time']
+                           + (['%-6s' % 'NbrUpdateRate'] if rate else [])))
         now = time.time()
         row = [self.step] + [rep[k] for k in keys] + [now - self._tic_between_report]
+        if rate:
+            row.append(self._sys.dev.nlist_update_rate())
         self.log.append(row)
-        print('%-8d ' % row[0] + ' '.join('%-12.3f' % x for x in row[1:-1]) + ' %-6.3f' % row[-1])
+        print('%-8d ' % row[0] + ' '.join('%-12.3f' % x for x in row[1:len(keys) + 1])
+              + ' %-6.3f' % row[len(keys) + 1] + (' %-6.3f' % row[-1] if rate else ''))
         self._tic_between_report = now
 
     def _initialize_run(self, steps, writer=None, label=None):

@@ -233,6 +233,9 @@ int32_t dpmd_timing_enable(dpmd_system_t sys, int32_t on);
 int32_t dpmd_timing_get(dpmd_system_t sys, int32_t which, double* ms, int64_t* count);
 /* Rebuilds triggered by the drift test inside dpmd_md_run since the last dpmd_nlist_allocate
  * (the update count behind the reference's neighbour-list profile, deepmd_jax/md.py:731). */
+/* Moving average (weight 1/100 per step) of the per-step rebuild decisions, the NbrUpdateRate
+ * column of Simulation(debug=True) (deepmd_jax/md.py:586-592,733). */
+int32_t dpmd_nlist_update_rate(dpmd_system_t sys, double* rate);
 int32_t dpmd_nlist_rebuilds(dpmd_system_t sys, int64_t* count);
 /* Mean neighbours within rcut per centre of the last evaluation (algorithmic FLOP count). */
 int32_t dpmd_pair_count(dpmd_system_t sys, double* pairs_within_rcut);

@@ -155,3 +155,29 @@ def test_simulation_fixed_indices_and_remove_com(model_pkl):
     sim.remove_com_motion()
     vv = sim.getVelocity()
     assert np.abs((vv * m[:, None]).sum(0)).max() < 1e-9
+
+
+def test_debug_neighbour_update_rate(model_pkl):
+    """Simulation(debug=True): the NbrUpdateRate column is the per-step moving average
+    profile * (1 - 1/100) + update / 100 of deepmd_jax/md.py:733.  With a zero skin every step
+    rebuilds, so after k steps it is 1 - 0.99^k; the column is absent below the reference's
+    neighbour-list condition."""
+    import contextlib
+    import io
+    from deepmd_jax import synthetic as syn
+    from deepmd_jax.md import Simulation
+    print('case debug rate', flush=True)
+    coord, box, types = syn.water_box(216, seed=60)
+    with contextlib.redirect_stdout(io.StringIO()) as out:
+        sim = Simulation(model_pkl, box, types, syn.MASSES, 'NVE', 0.5, coord, debug=True, report_interval=50,
+                         neighbor_skin=0.0)
+        sim.run(100)
+    assert 'NbrUpdateRate' in out.getvalue()
+    rates = [row[-1] for row in sim.log]
+    assert rates[0] == 0.0
+    np.testing.assert_allclose(rates[1:], [1 - 0.99 ** 50, 1 - 0.99 ** 100], atol=1e-5)
+    c2, b2, t2 = syn.water_box(64, seed=61)
+    with contextlib.redirect_stdout(io.StringIO()) as out2:
+        sim2 = Simulation(model_pkl, b2, t2, syn.MASSES, 'NVE', 0.5, c2, debug=True, report_interval=10)
+        sim2.run(10)
+    assert 'NbrUpdateRate' not in out2.getvalue()
