@@ -887,6 +887,8 @@ static int32_t npt_args(System* s, const dpmd_md_params* p, NptArgs& a) {
   a.rc = s->model->rcut;
   a.rc_skin = s->rc_skin;
   a.natoms = s->seg[0].natoms;
+  a.cs = std::max(1, p->chain_steps_p);
+  a.sy = std::max(1, p->sy_steps_p);
   return DPMD_OK;
 }
 
@@ -919,6 +921,17 @@ int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps, 
   }
   const bool npt = p->routine == DPMD_NPT_NH;
   const bool pnpt = p->routine == DPMD_PIMD_NPT;
+  // Nose-Hoover chain substeps (0 reads as 1); Suzuki-Yoshida orders 1, 3, 5, 7
+  const int cs_t = std::max(1, p->chain_steps), sy_t = std::max(1, p->sy_steps);
+  for (int v : {p->sy_steps, p->sy_steps_p})
+    if (v != 0 && v != 1 && v != 3 && v != 5 && v != 7) {
+      set_error("sy_steps must be 1, 3, 5 or 7 (got %d)", v);
+      return DPMD_E_ARG;
+    }
+  if (p->chain_steps < 0 || p->chain_steps_p < 0 || cs_t > 64 || p->chain_steps_p > 64) {
+    set_error("chain_steps must lie in [1, 64]");
+    return DPMD_E_ARG;
+  }
 
   NptArgs na;
   memset(&na, 0, sizeof(na));
@@ -1006,23 +1019,23 @@ int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps, 
       DPMD_TRY(model_forces_impl(s, prec));
       DPMD_TRY(md_kick(s, 0.5 * dt));
     } else if (p->routine == DPMD_NVT_NH) {
-      DPMD_TRY(md_nh_half(s, p->kT, dt, p->tau, p->chain_length));
+      DPMD_TRY(md_nh_half(s, p->kT, dt, p->tau, p->chain_length, cs_t, sy_t));
       DPMD_TRY(md_kick(s, 0.5 * dt));
       DPMD_TRY(md_drift(s, dt));
       DPMD_TRY(model_forces_impl(s, prec));
       DPMD_TRY(md_kick(s, 0.5 * dt));
-      DPMD_TRY(md_nh_half(s, p->kT, dt, p->tau, p->chain_length));
+      DPMD_TRY(md_nh_half(s, p->kT, dt, p->tau, p->chain_length, cs_t, sy_t));
       if (s->has_fixed) DPMD_TRY(md_mask_momenta(s));
     } else if (npt) {
       // apply_fn of deepmd_jax/routine.py:396-423 around inner_step :365-394
       DPMD_TRY(md_npt_baro_half(s, na));
-      DPMD_TRY(md_nh_half(s, p->kT, dt, p->tau, p->chain_length));
+      DPMD_TRY(md_nh_half(s, p->kT, dt, p->tau, p->chain_length, cs_t, sy_t));
       DPMD_TRY(md_npt_box(s, na, 0));
       DPMD_TRY(md_npt_particles(s, dt, true));
       DPMD_TRY(model_forces_impl(s, prec));
       DPMD_TRY(md_npt_particles(s, dt, false));
       DPMD_TRY(md_npt_box(s, na, 1));
-      DPMD_TRY(md_nh_half(s, p->kT, dt, p->tau, p->chain_length));
+      DPMD_TRY(md_nh_half(s, p->kT, dt, p->tau, p->chain_length, cs_t, sy_t));
       DPMD_TRY(md_npt_baro_half(s, na));
     } else if (pnpt) {
       // step_fn of deepmd_jax/pimd.py:430-516

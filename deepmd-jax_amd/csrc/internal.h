@@ -159,6 +159,7 @@ struct RowBase {
 struct NptArgs {
   double pressure, kT, dt, tau_p, rc, rc_skin;
   int C, G, natoms;
+  int cs, sy;  // barostat chain substeps (chain_steps_p, sy_steps_p)
   int gid[3], nper[3];
 };
 
@@ -324,7 +325,7 @@ int32_t md_remove_com(System* s);
 int32_t md_reduce_launch(System* s);
 int32_t md_ke_launch(System* s, double* d_out);
 int32_t md_langevin_o(System* s, double dt, double gamma, double kT, uint64_t seed, uint64_t step);
-int32_t md_nh_half(System* s, double kT, double dt, double tau, int chain);
+int32_t md_nh_half(System* s, double kT, double dt, double tau, int chain, int cs, int sy);
 int32_t md_pimd_free_ring(System* s, const double* cmat, const double* freqs, double dt_half);
 int32_t md_pimd_o(System* s, const double* cmat, const double* c1, const double* c2, uint64_t seed,
                   uint64_t counter);

@@ -461,12 +461,36 @@ __device__ double nhc_half_step(double* xi, double* px, const double* Q, int C, 
   return scale;
 }
 
+// chain_steps x sy_steps substeps of the chain half step with Suzuki-Yoshida weights, as
+// jax_md's half_step_chain_fn composes them (delta = dt / chain_steps * w[i % sy_steps]); the
+// coupled kinetic energy follows each substep's scale.  jax_md is not in the reference tree:
+// the weights are the published Suzuki-Yoshida table (orders 3, 5, 7).
+__device__ double nhc_half_steps(double* xi, double* px, const double* Q, int C, double KE, double dof, double kT,
+                                 double dt, int cs, int sy) {
+  if (cs <= 1 && sy <= 1) return nhc_half_step(xi, px, Q, C, KE, dof, kT, dt);
+  const double w1[1] = {1.0};
+  const double w3[3] = {0.828981543588751, -0.657963087177502, 0.828981543588751};
+  const double w5[5] = {0.2967324292201065, 0.2967324292201065, -0.186929716880426, 0.2967324292201065,
+                        0.2967324292201065};
+  const double w7[7] = {0.784513610477560, 0.235573213359357, -1.17767998417887, 1.31518632068391,
+                        -1.17767998417887, 0.235573213359357, 0.784513610477560};
+  const double* w = sy == 3 ? w3 : (sy == 5 ? w5 : (sy == 7 ? w7 : w1));
+  const int n = cs * sy;
+  double scale = 1.0;
+  for (int i = 0; i < n; ++i) {
+    const double s = nhc_half_step(xi, px, Q, C, KE, dof, kT, dt / cs * w[i % sy]);
+    scale *= s;
+    KE *= s * s;
+  }
+  return scale;
+}
+
 // Particle thermostat half step, one thread per segment (deepmd_jax/routine.py:147-158).
 // thermo layout per segment (64 doubles): [0] invariant term, [1] initialised,
 // [8..16) xi, [16..24) p_xi, [24..32) Q, [40] momentum scale.  Kinetic energy and degrees
 // of freedom are per periodic image.
 __global__ void nh_half_kernel(double* thermo, const double* ke, const SegInfo* seg, int nseg, int C, double kT,
-                               double dt, double tau, double dof_mobile) {
+                               double dt, double tau, double dof_mobile, int cs, int sy) {
   for (int rep = threadIdx.x; rep < nseg; rep += blockDim.x) {
     // degrees of freedom of the mobile atoms (with_fixed_atoms, deepmd_jax/routine.py:110-136)
     const double dof = dof_mobile > 0 ? dof_mobile : 3.0 * (seg[rep].natoms / seg[rep].nimg);
@@ -480,7 +504,7 @@ __global__ void nh_half_kernel(double* thermo, const double* ke, const SegInfo* 
     }
     for (int k = 0; k < C; ++k) Q[k] = kT * tau * tau;
     Q[0] *= dof;
-    th[40] = nhc_half_step(xi, px, Q, C, ke[rep], dof, kT, dt);
+    th[40] = nhc_half_steps(xi, px, Q, C, ke[rep], dof, kT, dt, cs, sy);
     double e = px[0] * px[0] / (2.0 * Q[0]) + dof * kT * xi[0];
     for (int k = 1; k < C; ++k) e += px[k] * px[k] / (2.0 * Q[k]) + kT * xi[k];
     th[0] = e;
@@ -492,7 +516,7 @@ __global__ void nh_scale_kernel(double* mom, const double* thermo, const int* at
   if (i < total3) mom[i] *= thermo[atom_seg[i / 3] * 64 + 40];
 }
 
-int32_t md_nh_half(System* s, double kT, double dt, double tau, int chain) {
+int32_t md_nh_half(System* s, double kT, double dt, double tau, int chain, int cs, int sy) {
   if (chain < 1 || chain > 8) {
     set_error("Nose-Hoover chain length %d outside [1, 8]", chain);
     return DPMD_E_ARG;
@@ -500,7 +524,7 @@ int32_t md_nh_half(System* s, double kT, double dt, double tau, int chain) {
   double* d_ke = s->d_red + 16 * s->nseg;
   DPMD_TRY(md_ke_launch(s, d_ke));
   hipLaunchKernelGGL(nh_half_kernel, dim3(1), dim3(256), 0, s->stream, s->d_thermo, d_ke, s->d_seg, s->nseg, chain,
-                     kT, dt, tau, s->has_fixed ? 3.0 * s->n_mobile / s->seg[0].nimg : 0.0);
+                     kT, dt, tau, s->has_fixed ? 3.0 * s->n_mobile / s->seg[0].nimg : 0.0, cs, sy);
   const int n3 = s->NA * 3;
   hipLaunchKernelGGL(nh_scale_kernel, dim3((n3 + 255) / 256), dim3(256), 0, s->stream, s->d_mom, s->d_thermo,
                      s->d_atom_seg, n3);
@@ -575,7 +599,7 @@ __global__ void npt_baro_half_kernel(NptArgs a, double* npt, const SegInfo* seg)
     npt[6 + g] = a.nper[g] * (a.natoms + 1.0) * tp2;
     ke += npt[3 + g] * npt[3 + g] / (2.0 * npt[6 + g]);
   }
-  const double sc = nhc_half_step(xi, px, Q, a.C, ke, (double)a.G, a.kT, a.dt);
+  const double sc = nhc_half_steps(xi, px, Q, a.C, ke, (double)a.G, a.kT, a.dt, a.cs, a.sy);
   double e = 0.0;
   for (int g = 0; g < a.G; ++g) {
     npt[3 + g] *= sc;

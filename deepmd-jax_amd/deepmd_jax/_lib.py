@@ -33,7 +33,8 @@ class MDParams(ctypes.Structure):
     _fields_ = [('routine', c_i32), ('precision', c_i32), ('dt', c_f64), ('kT', c_f64), ('gamma', c_f64),
                 ('tau', c_f64), ('chain_length', c_i32), ('record_every', c_i32), ('seed', ctypes.c_uint64),
                 ('pimd_lambda', c_f64), ('pressure', c_f64), ('tau_p', c_f64), ('chain_length_p', c_i32),
-                ('couple', c_i32 * 3), ('remove_com', c_i32)]
+                ('couple', c_i32 * 3), ('remove_com', c_i32), ('chain_steps', c_i32), ('sy_steps', c_i32),
+                ('chain_steps_p', c_i32), ('sy_steps_p', c_i32)]
 
 
 class DPMDError(RuntimeError):

@@ -180,8 +180,11 @@ class Simulation:
         self._remove_com_motion = bool(remove_com_motion)
         if not (isinstance(couple_axes, tuple) and couple_axes == ((0, 1, 2),)) and routine != 'NPT':
             raise ValueError("couple_axes is only meaningful for routine='NPT'")
-        if chain_steps_t != 1 or sy_steps_t != 1 or chain_steps_p != 1 or sy_steps_p != 1:
-            raise NotImplementedError('Nose-Hoover chain_steps / sy_steps other than 1 are not provided.')
+        for v in (sy_steps_t, sy_steps_p):
+            if v not in (1, 3, 5, 7):
+                raise ValueError('sy_steps must be 1, 3, 5 or 7')
+        if min(chain_steps_t, chain_steps_p) < 1:
+            raise ValueError('chain_steps must be >= 1')
         self._couple_axes = ((0, 1, 2),)
         if routine == 'NPT':
             if pressure is None:
@@ -318,6 +321,9 @@ class Simulation:
         self._params.gamma = 1.0 / float(tau_t)
         self._params.pimd_lambda = self._pimd_lambda
         self._params.remove_com = int(self._remove_com_motion)
+        # Nose-Hoover chain substeps (jax_md chain_steps / sy_steps, deepmd_jax/md.py:80-87)
+        self._params.chain_steps, self._params.sy_steps = int(chain_steps_t), int(sy_steps_t)
+        self._params.chain_steps_p, self._params.sy_steps_p = int(chain_steps_p), int(sy_steps_p)
         if routine == 'NPT':
             # deepmd_jax/md.py:357-366: pressure in eV/A^3, barostat chain of tau_p
             self._params.pressure = self._pressure * PRESS_UNIT_CONVERSION

@@ -113,6 +113,10 @@ typedef struct {
                            first-appearance order) or -1 for a fixed axis (couple_axes) */
   int32_t remove_com;   /* nonzero: remove the centre-of-mass velocity of the mobile atoms after
                            every step (Simulation(remove_com_motion=True), deepmd_jax/md.py:786-790) */
+  int32_t chain_steps;  /* Nose-Hoover chain substeps per half step of the thermostat and their */
+  int32_t sy_steps;     /* Suzuki-Yoshida order (1, 3, 5, 7), jax_md's chain_steps / sy_steps;  */
+  int32_t chain_steps_p;/* _p: the NPT barostat chain.  0 reads as 1 (deepmd_jax/md.py:80-87)     */
+  int32_t sy_steps_p;
 } dpmd_md_params;
 
 const char* dpmd_last_error(void);

@@ -517,7 +517,7 @@ def vv_step(x, p, m, force_fn, dt, F, mobile=None):
     return x, p, F
 
 
-def nvt_nh_step(x, p, m, force_fn, dt, F, xi, px, kT, tau, mobile=None):
+def nvt_nh_step(x, p, m, force_fn, dt, F, xi, px, kT, tau, mobile=None, chain_steps=1, sy_steps=1):
     """One Nose-Hoover chain NVT step (chain half step, velocity Verlet, chain half step) with
     chain masses Q0 = dof kT tau^2, Q_k = kT tau^2 over dof = 3 x (mobile atoms).  The chain
     (xi, px) is advanced in place; returns (x, p, F)."""
@@ -525,13 +525,13 @@ def nvt_nh_step(x, p, m, force_fn, dt, F, xi, px, kT, tau, mobile=None):
     Q = np.full(len(xi), kT * tau ** 2)
     Q[0] *= dof
     ke = lambda q: 0.5 * (q ** 2 / m[:, None]).sum()
-    p = p * nhc_half_step(xi, px, Q, ke(p), dof, kT, dt)
+    p = p * nhc_half_steps(xi, px, Q, ke(p), dof, kT, dt, chain_steps, sy_steps)
     p = p + 0.5 * dt * F
     dx = dt * p / m[:, None]
     x = x + (dx if mobile is None else dx * mobile[:, None])
     F = force_fn(x)
     p = p + 0.5 * dt * F
-    p = p * nhc_half_step(xi, px, Q, ke(p), dof, kT, dt)
+    p = p * nhc_half_steps(xi, px, Q, ke(p), dof, kT, dt, chain_steps, sy_steps)
     if mobile is not None:
         p = p * mobile[:, None]
     return x, p, F
@@ -672,6 +672,26 @@ def nhc_half_step(xi, px, Q, KE, dof, kT, dt):
         px[m] = pu
     if M > 0:
         px[M] += d4 * G
+    return scale
+
+
+SUZUKI_YOSHIDA = {1: [1.0], 3: [0.828981543588751, -0.657963087177502, 0.828981543588751],
+                  5: [0.2967324292201065, 0.2967324292201065, -0.186929716880426, 0.2967324292201065,
+                      0.2967324292201065],
+                  7: [0.784513610477560, 0.235573213359357, -1.17767998417887, 1.31518632068391,
+                      -1.17767998417887, 0.235573213359357, 0.784513610477560]}
+
+
+def nhc_half_steps(xi, px, Q, KE, dof, kT, dt, chain_steps=1, sy_steps=1):
+    """chain_steps x sy_steps substeps of nhc_half_step with delta = dt / chain_steps * w_i
+    (jax_md's half_step_chain_fn; the Suzuki-Yoshida weights are the published table, parity
+    unpinned: jax_md is absent), the kinetic energy following each substep's scale."""
+    w = SUZUKI_YOSHIDA[sy_steps]
+    scale = 1.0
+    for i in range(chain_steps * sy_steps):
+        s = nhc_half_step(xi, px, Q, KE, dof, kT, dt / chain_steps * w[i % sy_steps])
+        scale *= s
+        KE *= s * s
     return scale
 
 

@@ -88,10 +88,10 @@ def test_fixed_nve_against_oracle(device_model, model0, remove_com):
         assert np.all(tv[:, 0][:, ~mob] == 0)
 
 
-@pytest.mark.parametrize('fixed', [False, True])
-def test_nvt_nose_hoover_against_oracle(device_model, model0, fixed):
-    """20 Nose-Hoover chain steps (chain length 3, tau 20 fs, 400 K over a 300 K start)
-    against the oracle step, and the chain energy of the invariant."""
+@pytest.mark.parametrize('fixed,cs,sy', [(False, 1, 1), (True, 1, 1), (False, 2, 3)])
+def test_nvt_nose_hoover_against_oracle(device_model, model0, fixed, cs, sy):
+    """20 Nose-Hoover chain steps (chain length 3, tau 20 fs, 400 K over a 300 K start; chain_steps
+    x sy_steps substeps) against the oracle step, and the chain energy of the invariant."""
     from deepmd_jax import _lib
     from deepmd_jax.constants import TEMP_UNIT_CONVERSION
     print('case nvt nh fixed=%s' % fixed, flush=True)
@@ -101,6 +101,7 @@ def test_nvt_nose_hoover_against_oracle(device_model, model0, fixed):
     kT, tau = 400 * TEMP_UNIT_CONVERSION, 20.0
     p = _lib.MDParams()
     p.routine, p.dt, p.kT, p.tau, p.chain_length = 1, 0.5, kT, tau, 3
+    p.chain_steps, p.sy_steps = cs, sy
     nsteps = 20
     tp, tv, err = s.md_run(p, nsteps)
     assert err == 0
@@ -110,7 +111,7 @@ def test_nvt_nose_hoover_against_oracle(device_model, model0, fixed):
     F = ff(x)
     xi, px = np.zeros(3), np.zeros(3)
     for _ in range(nsteps):
-        x, pm, F = O.nvt_nh_step(x, pm, m, ff, 0.5, F, xi, px, kT, tau, mobile=mob)
+        x, pm, F = O.nvt_nh_step(x, pm, m, ff, 0.5, F, xi, px, kT, tau, mobile=mob, chain_steps=cs, sy_steps=sy)
     assert np.abs(tp[-1, 0] - x).max() < 1e-5
     assert np.abs(tv[-1, 0] - pm / m[:, None]).max() < 1e-5
     dof = 3 * (len(types) if mob is None else int(mob.sum()))
