@@ -220,6 +220,7 @@ def pimd(model, params, var, steps=100, warmup=10, precisions=('fp32', 'bf16'), 
                        'skin 1 A' % box[0], 'atoms_total': P * N}
 
     def reset():
+        s.set_box(box)
         s.set_positions(x)
         s.set_velocities(v)
         s.nlist_allocate(params['rcut'] + 1.0, 1.2)
@@ -246,6 +247,23 @@ def pimd(model, params, var, steps=100, warmup=10, precisions=('fp32', 'bf16'), 
         el = time.perf_counter() - t0
         out[prec] = {'steps_per_s': steps / el, 'bead_steps_per_s': P * steps / el,
                      'atoms_steps_per_s': P * N * steps / el, 'ms_per_step': 1e3 * el / steps, 'steps': steps}
+    # path-integral NPT on the same beads (1 bar, isotropic, tau_p 1000 fs, Langevin barostat)
+    from deepmd_jax.constants import PRESS_UNIT_CONVERSION
+    p.routine, p.precision, p.gamma = _lib.ROUTINE['PIMD_NPT'], _lib.PREC['fp32'], 1.0 / 1000.0
+    p.pressure, p.tau_p = 1.0 * PRESS_UNIT_CONVERSION, 1000.0
+    for a in range(3):
+        p.couple[a] = 0
+    reset()
+    run(warmup)
+    s.synchronize()
+    t0 = time.perf_counter()
+    run(steps)
+    s.synchronize()
+    el = time.perf_counter() - t0
+    out['npt_fp32'] = {'steps_per_s': steps / el, 'bead_steps_per_s': P * steps / el,
+                       'ms_per_step': 1e3 * el / steps, 'steps': steps,
+                       'box_rel_change': float(s.get_boxes()[0][0] / box[0] - 1.0)}
+    p.routine, p.gamma = _lib.ROUTINE['PIMD'], 0.01
     if acc_beads:
         import torch
         sys.path.insert(0, ROOT)

@@ -599,8 +599,19 @@ class Simulation:
         return centroid(p, self.getBox()) if self._is_pimd else p[0]
 
     def setPosition(self, position, box=None):
+        """Set positions; under NPT an isotropically scaled box may come with them
+        (deepmd_jax/md.py:985-1015: it holds for the forces until the barostat moves the box
+        again, which starts from the barostat's own state, as there)."""
         if box is not None:
-            raise ValueError('Box can only be changed in NPT simulations.')
+            if self._routine != 'NPT':
+                raise ValueError('Box can only be changed in NPT simulations.')
+            b = np.asarray(box, np.float64)
+            b = np.full(3, float(b.reshape(()))) if b.size == 1 else b
+            cur = self.getBox()
+            if b.shape != cur.shape:
+                raise ValueError('Box must have the same shape as the initial box.')
+            if not np.allclose(b, cur * (b[0] / cur[0]), rtol=1e-4, atol=1e-6):
+                raise ValueError('Only isotropic box fluctuations are allowed in the current implementation.')
         position = np.asarray(position, np.float64)
         if self._is_pimd and position.ndim == 2:
             position = np.broadcast_to(position, (self._n_bead,) + position.shape)
@@ -608,6 +619,8 @@ class Simulation:
         if position.shape != expect:
             raise ValueError('Position must have the same shape as the initial position, or you have to create a '
                              'new Simulation instance.')
+        if box is not None:
+            self._sys.dev.set_box(b)
         self._sys.set_positions(position.reshape(self._n_bead, self._natoms, 3))
         if self._sys.nlist_update():
             self._allocate_nlist()

@@ -136,3 +136,10 @@ def test_npt_simulation_surface(model0, device_model):
     # the step-0 invariant carries P V of the box at rest (no jump at the first step)
     inv = np.array(sim.log)[:, 8]
     assert np.abs(inv - inv[0]).max() / len(types) < 1e-3, inv
+    # setPosition with an isotropically scaled box (deepmd_jax/md.py:985-1015)
+    b0, x0 = sim.getBox(), sim.getPosition()
+    with pytest.raises(ValueError):
+        sim.setPosition(x0, box=b0 * np.array([1.0, 1.0, 1.01]))
+    sim.setPosition(x0 * 1.005, box=b0 * 1.005)
+    np.testing.assert_allclose(sim.getBox(), b0 * 1.005, rtol=1e-12)
+    assert np.isfinite(sim.getEnergy())
