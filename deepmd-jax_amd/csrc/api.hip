@@ -2,6 +2,7 @@
 // neighbour-list protocol, energy/force/virial evaluation and the device MD loop.
 #include "internal.h"
 #include <math.h>
+#include <cmath>
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
@@ -602,11 +603,64 @@ static int32_t set_seg_box(System* s, int r, const double* b) {
   for (int d = 0; d < 3; ++d)
     if (!(b[d] > 0)) { set_error("box lengths must be positive"); return DPMD_E_ARG; }
   SegInfo& S = s->seg[r];
+  if (S.tri) {
+    // the list was binned in the general cell's fractional coordinates
+    S.tri = 0;
+    s->nl_ready = false;
+  }
   for (int d = 0; d < 3; ++d) {
     S.boxd[d] = b[d];
     S.box[d] = (float)b[d];
     if (!s->nl_ready) S.L[d] = (float)b[d];
     S.scale[d] = S.L[d] / (float)b[d];
+  }
+  s->seg_dirty = true;
+  s->force_valid = false;
+  return DPMD_OK;
+}
+
+// General cell H (rows: lattice vectors, deepmd_jax/utils.py:24-27).  A diagonal H is the
+// orthorhombic box.  Otherwise the segment keeps H and its inverse, and box/boxd hold the
+// perpendicular widths 1/|column d of H^-1|, which the minimum-image checks read.  A change
+// of a general cell drops the neighbour list (it bins fractional coordinates of the cell).
+static int32_t set_seg_cell(System* s, int r, const double* H) {
+  for (int k = 0; k < 9; ++k)
+    if (!std::isfinite(H[k])) { set_error("cell entries must be finite"); return DPMD_E_ARG; }
+  if (H[1] == 0 && H[2] == 0 && H[3] == 0 && H[5] == 0 && H[6] == 0 && H[7] == 0) {
+    const double b[3] = {H[0], H[4], H[8]};
+    return set_seg_box(s, r, b);
+  }
+  const double det = H[0] * (H[4] * H[8] - H[5] * H[7]) - H[1] * (H[3] * H[8] - H[5] * H[6]) +
+                     H[2] * (H[3] * H[7] - H[4] * H[6]);
+  const double scale = std::max(std::fabs(H[0]), std::max(std::fabs(H[4]), std::fabs(H[8])));
+  if (!(std::fabs(det) > 1e-12 * scale * scale * scale)) { set_error("cell is singular"); return DPMD_E_ARG; }
+  double Hi[9];
+  Hi[0] = (H[4] * H[8] - H[5] * H[7]) / det;
+  Hi[1] = (H[2] * H[7] - H[1] * H[8]) / det;
+  Hi[2] = (H[1] * H[5] - H[2] * H[4]) / det;
+  Hi[3] = (H[5] * H[6] - H[3] * H[8]) / det;
+  Hi[4] = (H[0] * H[8] - H[2] * H[6]) / det;
+  Hi[5] = (H[2] * H[3] - H[0] * H[5]) / det;
+  Hi[6] = (H[3] * H[7] - H[4] * H[6]) / det;
+  Hi[7] = (H[1] * H[6] - H[0] * H[7]) / det;
+  Hi[8] = (H[0] * H[4] - H[1] * H[3]) / det;
+  SegInfo& S = s->seg[r];
+  bool same = S.tri != 0;
+  for (int k = 0; k < 9 && same; ++k) same = S.Hd[k] == H[k];
+  if (!same) s->nl_ready = false;
+  S.tri = 1;
+  for (int k = 0; k < 9; ++k) {
+    S.Hd[k] = H[k];
+    S.Hid[k] = Hi[k];
+    S.H[k] = (float)H[k];
+    S.Hi[k] = (float)Hi[k];
+  }
+  for (int d = 0; d < 3; ++d) {
+    const double w = 1.0 / std::sqrt(Hi[d] * Hi[d] + Hi[3 + d] * Hi[3 + d] + Hi[6 + d] * Hi[6 + d]);
+    S.boxd[d] = w;
+    S.box[d] = (float)w;
+    if (!s->nl_ready) S.L[d] = (float)w;
+    S.scale[d] = 1.f;
   }
   s->seg_dirty = true;
   s->force_valid = false;
@@ -695,6 +749,13 @@ int32_t dpmd_system_set_boxes(dpmd_system_t sys, const double* boxes) {
   SYS(sys);
   if (!boxes) { set_error("null argument"); return DPMD_E_ARG; }
   for (int r = 0; r < s->nseg; ++r) DPMD_TRY(set_seg_box(s, r, boxes + 3 * r));
+  return DPMD_OK;
+}
+
+int32_t dpmd_system_set_cells(dpmd_system_t sys, const double* cells) {
+  SYS(sys);
+  if (!cells) { set_error("null argument"); return DPMD_E_ARG; }
+  for (int r = 0; r < s->nseg; ++r) DPMD_TRY(set_seg_cell(s, r, cells + 9 * r));
   return DPMD_OK;
 }
 
@@ -845,6 +906,8 @@ int32_t dpmd_compute(dpmd_system_t sys, int32_t precision, double* energy, doubl
 // NPT parameters from dpmd_md_params; couple_axes rules of deepmd_jax/routine.py:196-223
 static int32_t npt_args(System* s, const dpmd_md_params* p, NptArgs& a) {
   const bool pimd = p->routine == DPMD_PIMD_NPT;
+  for (const SegInfo& S : s->seg)
+    if (S.tri) { set_error("NPT runs orthorhombic boxes (deepmd_jax/routine.py:161-170)"); return DPMD_E_UNSUPPORTED; }
   if (!pimd && (s->nseg != 1 || s->seg[0].nimg != 1)) {
     set_error("NPT runs one system held without periodic images");
     return DPMD_E_UNSUPPORTED;

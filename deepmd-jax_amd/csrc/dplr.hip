@@ -222,6 +222,10 @@ static int32_t dplr_run(System* s, Model* M) {
 }
 
 int32_t dplr_forces(System* s) {
+  if (s->seg[0].tri) {
+    set_error("DPLR runs orthorhombic boxes (deepmd_jax/md.py:391-392)");
+    return DPMD_E_UNSUPPORTED;
+  }
   Model* sr = s->model;
   s->model = s->wc_model;
   const int32_t rc = dplr_run(s, s->wc_model);
@@ -273,8 +277,8 @@ extern "C" int32_t dpmd_system_set_dplr(dpmd_system_t sys, dpmd_model_t wc_model
               M->rcut);
     return DPMD_E_UNSUPPORTED;
   }
-  if (s->nseg != 1 || s->seg[0].nimg != 1) {
-    set_error("DPLR runs one system held without periodic images");
+  if (s->seg[0].tri) {
+    set_error("DPLR runs orthorhombic boxes (deepmd_jax/md.py:391-392)");
     return DPMD_E_UNSUPPORTED;
   }
   if (!(beta > 0)) { set_error("DPLR needs beta > 0"); return DPMD_E_ARG; }

@@ -447,9 +447,20 @@ __device__ __forceinline__ int compact_row(const EmbedArgs& a, const SegInfo& S,
       const int idx = row[e];
       const int g = S.base + S.off[j] + idx;
       const float4 pj = a.pos[g];
-      const float dx = minimg(pj.x - pc.x, S.box[0]);
-      const float dy = minimg(pj.y - pc.y, S.box[1]);
-      const float dz = minimg(pj.z - pc.z, S.box[2]);
+      float dx, dy, dz;
+      if (S.tri) {
+        // general cell: fractional minimum image (deepmd_jax/utils.py:24-27)
+        float s[3], d[3] = {pj.x - pc.x, pj.y - pc.y, pj.z - pc.z};
+        cell_mul(S.Hi, d, s);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) s[k] -= rintf(s[k]);
+        cell_mul(S.H, s, d);
+        dx = d[0], dy = d[1], dz = d[2];
+      } else {
+        dx = minimg(pj.x - pc.x, S.box[0]);
+        dy = minimg(pj.y - pc.y, S.box[1]);
+        dz = minimg(pj.z - pc.z, S.box[2]);
+      }
       const float r = pair_r(dx, dy, dz);
       keep = (r > 1e-14f) && (r / a.rcut < 1.f);
       out = make_float4(dx, dy, dz, __int_as_float(g));

@@ -143,11 +143,24 @@ struct SegInfo {
   float thr;           // neighbour-list drift threshold
   float L[3];          // neighbour-list reference box
   float scale[3];      // reference box / box
-  float box[3];        // current box, model minimum image
-  double boxd[3];      // current box, FP64 wrap
+  float box[3];        // current box, model minimum image (general cell: perpendicular widths)
+  double boxd[3];      // current box, FP64 wrap (general cell: perpendicular widths)
   unsigned long long seed;  // stochastic stream of the segment
   unsigned long long step;  // integrator steps taken (counter of the stream)
+  // General (triclinic) cell, tri != 0: rows of H are the lattice vectors, x = s H and
+  // s = x Hi for fractional s.  Positions wrap to s in [0, 1), pair vectors take the
+  // fractional minimum image, and the neighbour list bins fractional coordinates.
+  int tri;
+  float H[9], Hi[9];
+  double Hd[9], Hid[9];
 };
+
+// s = x Hi and x = s H for row-vector coordinates and a row-major 3 x 3 matrix.
+template <class T>
+__host__ __device__ __forceinline__ void cell_mul(const T* M, const T x[3], T y[3]) {
+#pragma unroll
+  for (int e = 0; e < 3; ++e) y[e] = x[0] * M[e] + x[1] * M[3 + e] + x[2] * M[6 + e];
+}
 
 struct RowBase {
   int r[kMaxT];

@@ -27,13 +27,27 @@ __device__ __forceinline__ float wrap32(double x, double L) {
   return v;
 }
 
+// FP32 mirror of a position wrapped into the segment's cell: per axis in an orthorhombic
+// box, through fractional coordinates in [0, 1) in a general cell
+__device__ __forceinline__ float4 wrap_pos32(const SegInfo& S, const double x[3]) {
+  if (S.tri) {
+    double f[3], w[3];
+    cell_mul(S.Hid, x, f);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) f[d] -= floor(f[d]);
+    cell_mul(S.Hd, f, w);
+    return make_float4((float)w[0], (float)w[1], (float)w[2], 0.f);
+  }
+  return make_float4(wrap32(x[0], S.boxd[0]), wrap32(x[1], S.boxd[1]), wrap32(x[2], S.boxd[2]), 0.f);
+}
+
 __global__ void write_pos32_kernel(const double* pos, float4* p32, const SegInfo* seg, const int* atom_seg,
                                    int total) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= total) return;
   const SegInfo& S = seg[atom_seg[g]];
-  p32[g] = make_float4(wrap32(pos[(size_t)g * 3], S.boxd[0]), wrap32(pos[(size_t)g * 3 + 1], S.boxd[1]),
-                       wrap32(pos[(size_t)g * 3 + 2], S.boxd[2]), 0.f);
+  const double x[3] = {pos[(size_t)g * 3], pos[(size_t)g * 3 + 1], pos[(size_t)g * 3 + 2]};
+  p32[g] = wrap_pos32(S, x);
 }
 
 __global__ void zero_kernel(float* x, size_t n) {
@@ -54,7 +68,7 @@ __global__ void drift_kernel(double* pos, const double* mom, const double* mass,
   const SegInfo& S = seg[atom_seg[g]];
   const double m = mass[g];
   if (mobile && mobile[g] == 0.f) dt = 0.0;  // fixed atoms keep their positions
-  float v[3];
+  double v[3];
   bool bad = false;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -62,9 +76,9 @@ __global__ void drift_kernel(double* pos, const double* mom, const double* mass,
     const double x = pos[(size_t)g * 3 + d] + dt * p / m;
     pos[(size_t)g * 3 + d] = x;
     bad |= !isfinite(x) || !isfinite(p);
-    v[d] = wrap32(x, S.boxd[d]);
+    v[d] = x;
   }
-  p32[g] = make_float4(v[0], v[1], v[2], 0.f);
+  p32[g] = wrap_pos32(S, v);
   if (bad) atomicOr(&flags[2], 16);
 }
 
@@ -216,6 +230,24 @@ __global__ void langevin_o_kernel(double* mom, const double* mass, const SegInfo
   dke_out[g] = dke;
 }
 
+// Beads of a general cell onto bead 0's image through the fractional minimum image
+// (deepmd_jax/pimd.py:210-227, general branch); one (bead >= 1, atom) per thread.
+__global__ void pimd_unwrap_cell_kernel(double* pos, int natoms, int P, const SegInfo* seg) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= natoms * (P - 1)) return;
+  const int b = 1 + t / natoms, i = t % natoms;
+  const SegInfo& S = seg[0];
+  double* x = pos + ((size_t)b * natoms + i) * 3;
+  const double* x0 = pos + (size_t)i * 3;
+  double d[3] = {x[0] - x0[0], x[1] - x0[1], x[2] - x0[2]}, f[3];
+  cell_mul(S.Hid, d, f);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) f[k] -= rint(f[k]);
+  cell_mul(S.Hd, f, d);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) x[k] = x0[k] + d[k];
+}
+
 // PIMD exact free ring-polymer step for one (atom, dim) per thread: unwrap beads onto
 // bead 0's image, rotate to normal modes, propagate, rotate back.
 __global__ void pimd_free_ring_kernel(double* pos, double* mom, const double* mass, int natoms, int P,
@@ -231,7 +263,7 @@ __global__ void pimd_free_ring_kernel(double* pos, double* mom, const double* ma
   for (int b = 0; b < P; ++b) {
     const double x = pos[((size_t)b * natoms + i) * 3 + d];
     double dl = x - x0;
-    dl -= L * rint(dl / L);
+    if (L > 0) dl -= L * rint(dl / L);  // L = 0: beads already unwrapped (general cell)
     q[b] = x0 + dl;
     p[b] = mom[((size_t)b * natoms + i) * 3 + d];
   }
@@ -402,8 +434,14 @@ int32_t md_langevin_o(System* s, double dt, double gamma, double kT, uint64_t se
 int32_t md_pimd_free_ring(System* s, const double* d_C, const double* d_freqs, double h) {
   const SegInfo& S = s->seg[0];
   const int n = S.natoms * 3;
+  if (S.tri) {
+    const int nu = S.natoms * (s->nseg - 1);
+    hipLaunchKernelGGL(pimd_unwrap_cell_kernel, dim3((nu + 255) / 256), dim3(256), 0, s->stream, s->d_pos, S.natoms,
+                       s->nseg, s->d_seg);
+  }
+  const double Lx = S.tri ? 0.0 : S.boxd[0], Ly = S.tri ? 0.0 : S.boxd[1], Lz = S.tri ? 0.0 : S.boxd[2];
   hipLaunchKernelGGL(pimd_free_ring_kernel, dim3((n + 127) / 128), dim3(128), 0, s->stream, s->d_pos, s->d_mom,
-                     s->d_mass, S.natoms, s->nseg, d_C, d_freqs, h, S.boxd[0], S.boxd[1], S.boxd[2]);
+                     s->d_mass, S.natoms, s->nseg, d_C, d_freqs, h, Lx, Ly, Lz);
   DPMD_HIP(hipGetLastError());
   return DPMD_OK;
 }

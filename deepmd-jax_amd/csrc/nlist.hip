@@ -14,6 +14,10 @@
 //                  > ((rc+skin) - rc*max(scale)*(1.02 under NPT))/2 - 1e-6 L_min
 // All float32 expressions are evaluated without contraction so the neighbour sets are
 // bit-exact with a float32 evaluation of the same formulas.
+// General (triclinic) cells, which the reference evaluates on its dense lattice-image path
+// (deepmd_jax/utils.py:20-27,66-78): canonical coordinates are fractional in [0, 1), cells
+// bin them per lattice direction (edge: perpendicular width / count >= the cutoff), and the
+// pair and drift tests use the fractional minimum image mapped back through the cell.
 #include "internal.h"
 #include <algorithm>
 #include <string.h>
@@ -42,6 +46,28 @@ __device__ __forceinline__ float pair_d2(float4 a, float4 b, float Lx, float Ly,
   dz = fmod_floor(dz + hz, Lz) - hz;
   const float sx = dx * dx, sy = dy * dy, sz = dz * dz;
   return (sx + sy) + sz;
+}
+
+// general cell: canonical coordinates are fractional; the fractional minimum image is the
+// nearest image for every pair within the cutoff when all perpendicular widths exceed
+// twice the cutoff (checked at allocation)
+__device__ __forceinline__ float pair_d2_cell(float4 a, float4 b, const float* H) {
+  float s[3] = {a.x - b.x, a.y - b.y, a.z - b.z}, d[3];
+#pragma unroll
+  for (int e = 0; e < 3; ++e) s[e] -= rintf(s[e]);
+  cell_mul(H, s, d);
+  return (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
+}
+
+// fractional coordinates in [0, 1) of a position in a general cell
+__device__ __forceinline__ void cell_frac01(const SegInfo& S, const double* x, float f[3]) {
+  double s[3];
+  cell_mul(S.Hid, x, s);
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    float v = (float)(s[e] - floor(s[e]));
+    f[e] = v >= 1.f ? 0.f : v;
+  }
 }
 
 struct NlArgs {
@@ -80,19 +106,23 @@ __global__ void nl_canon_kernel(NlArgs a) {
   const SegInfo& S = a.seg[a.atom_seg[g]];
   const float shrink = (float)(1.0 - 5e-7);
   float c[3];
+  if (S.tri) {
+    cell_frac01(S, a.pos + (size_t)g * 3, c);
+  } else {
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    float x = (float)a.pos[(size_t)g * 3 + d];
-    x = fmod_floor(x, S.box[d]);  // wrap in the current box, then scale to the reference box
-    x = x * S.scale[d];
-    c[d] = x * shrink;
+    for (int d = 0; d < 3; ++d) {
+      float x = (float)a.pos[(size_t)g * 3 + d];
+      x = fmod_floor(x, S.box[d]);  // wrap in the current box, then scale to the reference box
+      x = x * S.scale[d];
+      c[d] = x * shrink;
+    }
   }
   a.canon[g] = make_float4(c[0], c[1], c[2], 0.f);
   if (S.use_cells) {
     int ci[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      int v = (int)(c[d] * ((float)S.nc[d] / S.L[d]));
+      int v = S.tri ? (int)(c[d] * (float)S.nc[d]) : (int)(c[d] * ((float)S.nc[d] / S.L[d]));
       ci[d] = v < 0 ? 0 : (v >= S.nc[d] ? S.nc[d] - 1 : v);
     }
     const int cell = (ci[2] * S.nc[1] + ci[1]) * S.nc[0] + ci[0];
@@ -157,10 +187,12 @@ __global__ void __launch_bounds__(256) nl_build_kernel(NlArgs a) {
   const float4* cn = a.canon + S.base;
   const float4 ci = cn[i];
   const float Lx = S.L[0], Ly = S.L[1], Lz = S.L[2];
+  const bool tri = S.tri != 0;
   int cnt = 0;
   auto consider = [&](int atom, bool in) {
     bool keep = false;
-    if (in && atom != i) keep = pair_d2(ci, cn[atom], Lx, Ly, Lz) < a.cut_sq;
+    if (in && atom != i)
+      keep = (tri ? pair_d2_cell(ci, cn[atom], S.H) : pair_d2(ci, cn[atom], Lx, Ly, Lz)) < a.cut_sq;
     const unsigned long long bal = __ballot(keep);
     const int pre = __popcll(bal & ((1ull << lane) - 1ull));
     if (keep && cnt + pre < a.lcap) list[cnt + pre] = atom;
@@ -237,14 +269,25 @@ __global__ void nl_drift_kernel(const double* pos, const float4* canon, const Se
     const float4 r = canon[g];
     const float rv[3] = {r.x, r.y, r.z};
     float d[3];
+    if (S.tri) {
+      float f[3];
+      cell_frac01(S, pos + (size_t)g * 3, f);
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      float x = (float)pos[(size_t)g * 3 + k];
-      x = fmod_floor(x, S.box[k]) * S.scale[k];
-      float dd = x - rv[k];
-      const float h = S.L[k] / 2.f;
-      dd = fmod_floor(dd - h, S.L[k]) - h;
-      d[k] = dd;
+      for (int k = 0; k < 3; ++k) {
+        f[k] -= rv[k];
+        f[k] -= rintf(f[k]);
+      }
+      cell_mul(S.H, f, d);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        float x = (float)pos[(size_t)g * 3 + k];
+        x = fmod_floor(x, S.box[k]) * S.scale[k];
+        float dd = x - rv[k];
+        const float h = S.L[k] / 2.f;
+        dd = fmod_floor(dd - h, S.L[k]) - h;
+        d[k] = dd;
+      }
     }
     const float n = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
     over |= n > S.thr;

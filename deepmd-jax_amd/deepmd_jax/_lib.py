@@ -64,6 +64,7 @@ def lib():
         'dpmd_system_destroy': (c_i32, [c_vp]),
         'dpmd_system_set_box': (c_i32, [c_vp, c_vp]),
         'dpmd_system_set_boxes': (c_i32, [c_vp, c_vp]),
+        'dpmd_system_set_cells': (c_i32, [c_vp, c_vp]),
         'dpmd_system_set_images': (c_i32, [c_vp, c_vp]),
         'dpmd_system_set_rng': (c_i32, [c_vp, c_vp, c_vp]),
         'dpmd_system_get_rng': (c_i32, [c_vp, c_vp, c_vp]),
@@ -125,7 +126,7 @@ def exported_symbols():
     lib()
     return ['dpmd_last_error', 'dpmd_version', 'dpmd_model_create', 'dpmd_model_destroy', 'dpmd_system_create',
             'dpmd_system_create_ragged', 'dpmd_system_reshape', 'dpmd_system_destroy', 'dpmd_system_set_box',
-            'dpmd_system_set_boxes', 'dpmd_system_set_images', 'dpmd_system_set_rng', 'dpmd_system_get_rng',
+            'dpmd_system_set_boxes', 'dpmd_system_set_cells', 'dpmd_system_set_images', 'dpmd_system_set_rng', 'dpmd_system_get_rng',
             'dpmd_system_set_positions', 'dpmd_system_get_positions',
             'dpmd_system_set_velocities', 'dpmd_system_get_velocities', 'dpmd_system_set_masses',
             'dpmd_nlist_allocate', 'dpmd_nlist_update', 'dpmd_nlist_export', 'dpmd_nlist_capacity', 'dpmd_compute',
@@ -325,8 +326,19 @@ class DeviceSystem:
             self.handle = None
 
     def set_box(self, box3):
-        b = np.ascontiguousarray(np.asarray(box3, np.float64).reshape(3))
+        """Box lengths (3,) or a cell (3, 3) (rows: lattice vectors) for every replica."""
+        b = np.asarray(box3, np.float64)
+        if b.size == 9:
+            self.set_cells(np.broadcast_to(b.reshape(1, 3, 3), (self.nrep, 3, 3)))
+            return
+        b = np.ascontiguousarray(b.reshape(3))
         check(lib().dpmd_system_set_box(self.handle, ptr(b)))
+
+    def set_cells(self, cells):
+        """Per-replica cells (nrep, 3, 3), rows the lattice vectors; a general cell drops the
+        neighbour list."""
+        c = np.ascontiguousarray(np.asarray(cells, np.float64).reshape(self.nrep, 3, 3))
+        check(lib().dpmd_system_set_cells(self.handle, ptr(c)))
 
     def set_boxes(self, boxes):
         b = np.ascontiguousarray(np.asarray(boxes, np.float64).reshape(self.nrep, 3))

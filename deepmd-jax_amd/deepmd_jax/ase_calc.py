@@ -2,8 +2,9 @@
 
 Like the reference module this one imports ASE at the top, so it needs ASE installed.
 Energies, forces and the stress (Voigt, ASE sign convention, eV/A^3) come from
-deepmd_jax.evaluator.energy_forces_stress; cells must be orthorhombic, and cells shorter
-than 2 rcut are evaluated as periodic supercells (the reference's lattice-image path).
+deepmd_jax.evaluator.energy_forces_stress; orthorhombic and general (triclinic) cells, and
+cells narrower than 2 rcut are evaluated as periodic supercells (the reference's
+lattice-image path).
 """
 import numpy as np
 from ase.calculators.calculator import Calculator, all_changes

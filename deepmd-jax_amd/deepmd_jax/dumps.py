@@ -133,7 +133,7 @@ def target_label(dumps, dump_prefix):
 
 def format_frame(symbols, data, box, step=None):
     """Extended-XYZ text of one frame (deepmd_jax/md_utils.py:115-136): atom count, the comment
-    line (Lattice of the orthorhombic cell row by row, Properties, step, pbc), then one
+    line (Lattice of the cell row by row, Properties, step, pbc), then one
     `%-2s x y z` line per atom.  data (N,3) float32 or float64."""
     data = np.asarray(data)
     fmt = '.8f' if data.dtype.itemsize <= 4 else '.15f'

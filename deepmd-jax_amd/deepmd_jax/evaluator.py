@@ -1,40 +1,68 @@
 """Frame evaluation on the GPU engine.
 
-Boxes shorter than 2*rcut, which deepmd-jax evaluates on its dense lattice-image
-path (deepmd_jax/utils.py:66-78, deepmd_jax/data.py:347-364), are evaluated here as
-a k1 x k2 x k3 periodic supercell with every k_d*L_d >= 2*rcut: each atom's images
-have identical environments, so E = E_super / prod(k), the force on an atom is the
-force on its first image and the virial is W_super / prod(k).
+Boxes (orthorhombic lengths, or general cells whose rows are the lattice vectors) with a
+perpendicular width shorter than 2*rcut, which deepmd-jax evaluates on its dense
+lattice-image path (deepmd_jax/utils.py:66-78, deepmd_jax/data.py:347-364), are evaluated
+here as a k1 x k2 x k3 periodic supercell with every k_d*w_d >= 2*rcut: each atom's images
+have identical environments, so E = E_super / prod(k), the force on an atom is the force on
+its first image and the virial is W_super / prod(k).
 """
 import numpy as np
 
 from . import _lib
 
 
-def canonical_box(box):
-    """(3,) orthorhombic box lengths from a scalar, (3,), (9,) or (3,3) box."""
+def canonical_cell(box):
+    """(3,3) cell, rows the lattice vectors, from a scalar, (3,), (9,) or (3,3) box."""
     b = np.asarray(box, np.float64)
     if b.size == 1:
-        return np.full(3, float(b.reshape(())))
+        return np.eye(3) * float(b.reshape(()))
     if b.size == 3:
-        return b.reshape(3).copy()
+        return np.diag(b.reshape(3))
     if b.size == 9:
-        m = b.reshape(3, 3)
-        if np.any(m - np.diag(np.diag(m))):
-            raise NotImplementedError('only orthorhombic boxes are supported')
-        return np.diag(m).copy()
+        return b.reshape(3, 3).copy()
     raise ValueError('box must be a scalar, (3,), (9,) or (3,3)')
 
 
-def supercell_factors(box3, rcut):
-    return np.maximum(1, np.ceil(2.0 * rcut / np.asarray(box3) * (1 + 1e-12))).astype(int)
+def canonical_box(box):
+    """(3,) box lengths of an orthorhombic box (deepmd_jax/md.py:208-215 keeps those as (3,)),
+    the (3,3) cell of a general one."""
+    c = canonical_cell(box)
+    if np.any(c - np.diag(np.diag(c))):
+        return c
+    return np.diag(c).copy()
 
 
-def replicate(coord, box3, type_idx, k):
-    """Coordinates of the k[0] x k[1] x k[2] supercell; image 0 holds the original atoms."""
+def cell_matrix(box):
+    b = np.asarray(box, np.float64)
+    return np.diag(b) if b.shape == (3,) else b.reshape(3, 3)
+
+
+def cell_volume(box):
+    b = np.asarray(box, np.float64)
+    return float(np.prod(b)) if b.shape == (3,) else abs(float(np.linalg.det(b.reshape(3, 3))))
+
+
+def cell_widths(box):
+    """Distances between opposite faces: the box lengths, or 1/|column d of H^-1| of a cell."""
+    b = np.asarray(box, np.float64)
+    if b.shape == (3,):
+        return b.copy()
+    return 1.0 / np.linalg.norm(np.linalg.inv(b.reshape(3, 3)), axis=0)
+
+
+def supercell_factors(box, rcut):
+    return np.maximum(1, np.ceil(2.0 * rcut / cell_widths(box) * (1 + 1e-12))).astype(int)
+
+
+def replicate(coord, box, type_idx, k):
+    """Coordinates of the k[0] x k[1] x k[2] supercell (images shifted by whole lattice
+    vectors); image 0 holds the original atoms.  Returns (coords, supercell box, types)."""
     shifts = np.stack(np.meshgrid(np.arange(k[0]), np.arange(k[1]), np.arange(k[2]), indexing='ij'), -1).reshape(-1, 3)
-    c = (coord[None, :, :] + (shifts * box3)[:, None, :]).reshape(-1, 3)
-    return c, box3 * k, np.tile(type_idx, len(shifts))
+    b = np.asarray(box, np.float64)
+    c = (coord[None, :, :] + (shifts @ cell_matrix(b))[:, None, :]).reshape(-1, 3)
+    sbox = b * k if b.shape == (3,) else np.asarray(k, np.float64)[:, None] * b.reshape(3, 3)
+    return c, sbox, np.tile(type_idx, len(shifts))
 
 
 def dplr_setup(params, device=0):
@@ -62,8 +90,9 @@ def dplr_grid(box3, beta, resolution):
 
 
 class Evaluator:
-    """Energies, forces and virials of orthorhombic frames for one model.  A DPLR model
-    evaluates frame by frame with its long-range term."""
+    """Energies, forces and virials of frames for one model (box: (3,) lengths or a (3,3)
+    cell).  A DPLR model evaluates frame by frame with its long-range term (orthorhombic
+    boxes, deepmd_jax/train.py:796-797)."""
 
     def __init__(self, params, variables, device=0, model=None):
         self.params = params
@@ -89,6 +118,8 @@ class Evaluator:
         nk = int(np.prod(k))
         F, N = coords.shape[:2]
         if self.dplr is not None:
+            if box3.shape != (3,):
+                raise ValueError('For "dplr" currently only orthorhombic boxes are supported.')
             if nk > 1:
                 raise NotImplementedError('DPLR evaluation needs box lengths >= 2*rcut (%.3f)' % (2 * self.rcut))
             wdm, qa, qw, beta, res = self.dplr
@@ -150,5 +181,5 @@ def energy_forces_stress(evaluator, positions, cell, type_idx, precision='fp32')
     engine's virial W = -dU/d(strain) that is -W / V."""
     box3 = canonical_box(cell)
     e, f, w = evaluator.evaluate_frame(positions, box3, type_idx, precision)
-    s = -np.asarray(w) / float(np.prod(box3))
+    s = -np.asarray(w) / cell_volume(box3)
     return float(e), f, np.array([s[0, 0], s[1, 1], s[2, 2], s[1, 2], s[0, 2], s[0, 1]])

@@ -27,7 +27,7 @@ import numpy as np
 from . import _lib
 from .constants import MASS_UNIT_CONVERSION, TEMP_UNIT_CONVERSION, PRESS_UNIT_CONVERSION
 from .dumps import CHEMICAL_SYMBOLS, DumpWriter, prepare_dumps, target_label
-from .evaluator import canonical_box, dplr_grid, dplr_setup
+from .evaluator import canonical_box, cell_volume, cell_widths, dplr_grid, dplr_setup
 from .constants import HBAR
 from .pimd import normal_mode_transform, pile_l_factors, centroid, spring_energy
 from .utils import load_model
@@ -79,10 +79,10 @@ class _ImageSystem:
         self.nimg = int(np.prod(self.k))
         self.N = len(type_idx)
         self.box = np.asarray(box, np.float64)
-        self._shift = replicate(np.zeros((self.N, 3)), self.box, type_idx, self.k)[0]
+        self._shift, sbox, _ = replicate(np.zeros((self.N, 3)), self.box, type_idx, self.k)
         types = np.tile(np.asarray(type_idx), self.nimg)
         self.dev = _lib.DeviceSystem(model, types, nrep)
-        self.dev.set_box(self.box * self.k)
+        self.dev.set_box(sbox)
         if self.nimg > 1:
             self.dev.set_images(np.full(nrep, self.nimg))
 
@@ -251,7 +251,13 @@ class Simulation:
         self._step_chunk_size = max(10, min(100, 100000 // (self._natoms * self._n_bead)))
         self._neighbor_skin = (0.5 if routine == 'NPT' else 0.3) if neighbor_skin is None else float(neighbor_skin)
         self._neighbor_buffer_ratio = float(neighbor_buffer_ratio)
+        # (3,) lengths of an orthorhombic box, else the (3,3) cell (deepmd_jax/md.py:208-215)
         self._box = canonical_box(box)
+        self._general_cell = self._box.shape == (3, 3)
+        if self._general_cell and routine == 'NPT':
+            raise NotImplementedError('NPT supports orthorhombic boxes (deepmd_jax/routine.py:161-170).')
+        # distances between opposite faces: the box lengths of an orthorhombic box
+        widths = cell_widths(self._box)
         self._rcut = float(params['rcut'])
         # The reference switches to its dense lattice-image path below 2*(rcut+skin)
         # (deepmd_jax/utils.py:66-78).  Here the neighbour list is exact for any skin with
@@ -260,25 +266,27 @@ class Simulation:
         k = np.ones(3, int)
         # the reference's own neighbour-list condition (deepmd_jax/md.py:450-456): its debug
         # report shows the neighbour update rate only then
-        self._reference_nlist = bool(use_neighbor_list_when_possible) and self._box.min() > 2 * (
+        self._reference_nlist = bool(use_neighbor_list_when_possible) and not self._general_cell and widths.min() > 2 * (
             self._rcut + self._neighbor_skin) * (1.02 if routine == 'NPT' else 1.0)
-        if routine == 'NPT' and (self._box.min() <= 2 * (self._rcut + self._neighbor_skin) * 1.02
+        if routine == 'NPT' and (widths.min() <= 2 * (self._rcut + self._neighbor_skin) * 1.02
                                  or not use_neighbor_list_when_possible):
             # the reference's NPT condition for its neighbour list (deepmd_jax/md.py:450-456)
             raise NotImplementedError('NPT needs box lengths above 2*(rcut+skin)*1.02 = %.3f A (neighbour-list '
                                       'regime); smaller NPT boxes are not provided by this engine.'
                                       % (2 * (self._rcut + self._neighbor_skin) * 1.02))
-        if self._box.min() < 2 * self._rcut:
+        if widths.min() < 2 * self._rcut:
             if self._is_pimd:
-                raise NotImplementedError('path-integral MD needs box lengths >= 2*rcut (%.3f)' % (2 * self._rcut))
-            k = np.maximum(1, np.ceil(2 * (self._rcut + self._neighbor_skin) / self._box * (1 + 1e-9))).astype(int)
-        elif self._box.min() <= 2 * (self._rcut + self._neighbor_skin) or not use_neighbor_list_when_possible:
-            self._neighbor_skin = max(0.0, 0.5 * self._box.min() * (1 - 1e-9) - self._rcut)
+                raise NotImplementedError('path-integral MD needs box widths >= 2*rcut (%.3f)' % (2 * self._rcut))
+            k = np.maximum(1, np.ceil(2 * (self._rcut + self._neighbor_skin) / widths * (1 + 1e-9))).astype(int)
+        elif widths.min() <= 2 * (self._rcut + self._neighbor_skin) or not use_neighbor_list_when_possible:
+            self._neighbor_skin = max(0.0, 0.5 * widths.min() * (1 - 1e-9) - self._rcut)
         self._sys = _ImageSystem(self._device_model, self._type_idx, self._n_bead, self._box, k)
         self._sys.set_masses(self._mass)
         # DPLR: the centroid model and PPPM term join every force evaluation (deepmd_jax/md.py:404-415)
         self._dplr = dplr_setup(params, device)
         if self._dplr is not None:
+            if self._general_cell:
+                raise NotImplementedError('dplr model currently only supports orthorhombic box')
             if self._is_pimd:
                 raise NotImplementedError('DPLR runs without beads on this engine.')
             if self._sys.nimg > 1:
@@ -406,7 +414,7 @@ class Simulation:
             # deepmd_jax/md.py:523-538: pressure, moving box lengths, invariant
             box = self.getBox()
             # iso_pressure of the (bead-averaged) energy (deepmd_jax/md_utils.py:139-159)
-            rep['Pressure'] = (N * self._kT + float(np.trace(v.mean(0))) / 3.0) / float(np.prod(box)) / PRESS_UNIT_CONVERSION
+            rep['Pressure'] = (N * self._kT + float(np.trace(v.mean(0))) / 3.0) / cell_volume(box) / PRESS_UNIT_CONVERSION
             for a in sorted(a for g in self._couple_axes for a in g):
                 rep['box_' + 'xyz'[a]] = float(box[a])
             if not self._is_pimd:
@@ -486,7 +494,7 @@ class Simulation:
         if writer is None:
             self._position_trajectory = np.zeros((traj_length,) + shape)
             self._velocity_trajectory = np.zeros((traj_length,) + shape)
-            self._box_trajectory = np.zeros((traj_length, 3))
+            self._box_trajectory = np.zeros((traj_length,) + self._box.shape)
             if self._is_pimd:
                 self._centroid_trajectory = np.zeros((traj_length, self._natoms, 3))
             if self._is_initial_state:
@@ -677,5 +685,5 @@ class Simulation:
             raise ValueError('getPressure needs a temperature')
         _, _, v, _ = self._sys.compute(self._precision)
         dUdeps = -float(np.trace(v.mean(0)))
-        V = float(np.prod(self.getBox()))
+        V = cell_volume(self.getBox())
         return (self._natoms * self._kT - dUdeps / 3.0) / V / PRESS_UNIT_CONVERSION

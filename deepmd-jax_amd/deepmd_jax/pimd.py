@@ -32,12 +32,19 @@ def pile_l_factors(freqs, dt, kT, n_bead, tau_t, pimd_lambda=1.0):
 
 
 def unwrap_across_beads(coord, box):
-    """deepmd_jax/pimd.py:210-227 (orthorhombic): beads onto bead 0's periodic image."""
+    """deepmd_jax/pimd.py:210-227: beads onto bead 0's periodic image, per axis for (3,)
+    box lengths, through the fractional minimum image for a (3,3) cell."""
     coord = np.asarray(coord)
     ref = coord[0:1]
-    box = np.asarray(box).reshape(3)
+    box = np.asarray(box, np.float64)
     delta = coord - ref
-    delta = delta - box * np.round(delta / box)
+    if box.size == 3:
+        box = box.reshape(3)
+        delta = delta - box * np.round(delta / box)
+    else:
+        cell = box.reshape(3, 3)
+        frac = delta @ np.linalg.inv(cell)
+        delta = (frac - np.round(frac)) @ cell
     return ref + delta
 
 

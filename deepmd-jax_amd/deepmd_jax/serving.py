@@ -30,7 +30,7 @@ import numpy as np
 
 from . import _lib
 from .constants import MASS_UNIT_CONVERSION, TEMP_UNIT_CONVERSION
-from .evaluator import canonical_box, replicate
+from .evaluator import canonical_box, cell_matrix, cell_widths, replicate
 from .synthetic import MASSES
 
 
@@ -39,7 +39,7 @@ class Request:
     rid: int
     kind: str                       # 'eval' or 'md'
     coord: np.ndarray               # (N, 3) Angstrom
-    box: np.ndarray                 # (3,) Angstrom
+    box: np.ndarray                 # (3,) Angstrom, or a (3,3) cell (rows: lattice vectors)
     type_idx: np.ndarray            # (N,)
     nsteps: int = 0                 # MD steps ('md')
     velocity: np.ndarray = None     # (N, 3) Angstrom/fs ('md')
@@ -74,8 +74,8 @@ class Result:
 
 
 def image_factors(box3, cutoff):
-    """Images per axis so that every supercell edge is >= 2 * cutoff."""
-    return np.maximum(1, np.ceil(2.0 * cutoff / np.asarray(box3) * (1 + 1e-9))).astype(int)
+    """Images per lattice direction so that every supercell width is >= 2 * cutoff."""
+    return np.maximum(1, np.ceil(2.0 * cutoff / cell_widths(box3) * (1 + 1e-9))).astype(int)
 
 
 def request_bytes(natoms_with_images, model_params):
@@ -266,7 +266,7 @@ class Server:
         else:
             self._eval_sys.reshape(segs)
         s = self._eval_sys
-        s.set_boxes(np.stack(boxes))
+        s.set_cells(np.stack([cell_matrix(b) for b in boxes]))
         s.set_images(nimg)
         s.set_positions(np.concatenate(coords))
         s.nlist_allocate(self.rcut, 1.2)
@@ -336,7 +336,7 @@ class Server:
         else:
             self._md_sys.reshape(segs)
         s = self._md_sys
-        s.set_boxes(np.stack([sg.box for sg in active]))
+        s.set_cells(np.stack([cell_matrix(sg.box) for sg in active]))
         s.set_images([sg.nimg for sg in active])
         s.set_masses(np.concatenate([sg.mass for sg in active]))
         s.set_positions(np.concatenate([sg.coord for sg in active]))

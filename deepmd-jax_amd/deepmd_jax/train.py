@@ -4,20 +4,15 @@ Training (deepmd_jax/train.py:15-520) is not part of this engine.
 """
 import numpy as np
 
-from .evaluator import Evaluator, canonical_box
+from .evaluator import Evaluator, canonical_cell
 from .utils import load_model
 
 
 def _boxes(box, nframes):
+    """(F, 3, 3) cells of the frames' boxes: scalars, (3,) lengths, (9,) or (3,3) cells."""
     box = np.asarray(box, np.float64)
-    if box.ndim == 1:
-        return np.stack([canonical_box(b) for b in box])
-    if box.ndim == 2:
-        if box.shape[1] in (1, 3, 9):
-            return np.stack([canonical_box(b) for b in box])
-        raise ValueError
-    if box.ndim == 3 and box.shape[1:] == (3, 3):
-        return np.stack([canonical_box(b) for b in box])
+    if box.ndim == 1 or (box.ndim == 2 and box.shape[1] in (1, 3, 9)) or (box.ndim == 3 and box.shape[1:] == (3, 3)):
+        return np.stack([canonical_cell(b) for b in box])
     raise ValueError
 
 

@@ -150,6 +150,15 @@ int32_t dpmd_system_destroy(dpmd_system_t sys);
 int32_t dpmd_system_set_box(dpmd_system_t sys, const double* box3);
 /* Per-segment box lengths (nseg, 3). */
 int32_t dpmd_system_set_boxes(dpmd_system_t sys, const double* boxes);
+/* Per-segment general (triclinic) cells (nseg, 3, 3), rows the lattice vectors, as the
+ * reference's (3,3) boxes (deepmd_jax/utils.py:20-27 shift, deepmd_jax/md.py:208-215): a
+ * diagonal cell is the orthorhombic box; otherwise pair vectors take the fractional minimum
+ * image, every perpendicular width must exceed 2 (rcut + skin) (dpmd_system_set_images holds
+ * the supercell of a smaller cell), and dpmd_system_get_boxes reports the widths.  Setting a
+ * different general cell drops the neighbour list (call dpmd_nlist_allocate again).  NPT,
+ * path-integral NPT and DPLR reject general cells, as the reference does
+ * (deepmd_jax/md.py:391-392, deepmd_jax/routine.py:161-170). */
+int32_t dpmd_system_set_cells(dpmd_system_t sys, const double* cells);
 /* Per-segment count of periodic images held (nseg): a segment that holds the k^3 images
  * of a box smaller than 2 rc (the supercell equivalent of the reference's lattice-image
  * branch, deepmd_jax/utils.py:66-78) lists its atoms image-major; energies, virials,

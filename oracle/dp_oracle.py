@@ -10,6 +10,7 @@ virial come from torch autograd, the analogue of the reference's `jax.grad`
 
 Function -> reference file:line it follows:
   shift_ortho           deepmd_jax/utils.py:20-23
+  shift_cell            deepmd_jax/utils.py:24-27 (general cell, fractional minimum image)
   sr                    deepmd_jax/utils.py:29-32
   tensor_3to6           deepmd_jax/utils.py:51-52
   relative_coord_nlist  deepmd_jax/utils.py:79-82
@@ -38,6 +39,10 @@ Function -> reference file:line it follows:
   p3m_energy            deepmd_jax/utils.py:174-222 (PPPM, TSC assignment); pinned against
                         the direct Ewald reciprocal sum (ewald_reciprocal)
   dplr_energy_force     deepmd_jax/md.py:404-415, deepmd_jax/utils.py:355-366
+  neighbor_sets_cell    general-cell neighbour sets (FP64 brute force, fractional minimum
+                        image); the reference evaluates general cells on its lattice path
+                        only, against which the nlist form is pinned for widths > 2 rcut
+  unwrap_beads (cell)   deepmd_jax/pimd.py:210-227 (general branch)
 Pinning (tests/test_oracle.py): closed forms of the switch and of tensor_3to6,
 rotation / translation / permutation invariance, the T3=T6=0 outer-product
 reduction of G, all-zero-weight nets, autograd against central finite
@@ -52,6 +57,22 @@ import torch
 
 def shift_ortho(d, box3):
     return torch.remainder(d - box3 / 2, box3) - box3 / 2
+
+
+def shift_cell(d, box33):
+    """Fractional minimum image in a general cell (rows: lattice vectors)."""
+    frac = d @ torch.linalg.inv(box33)
+    frac = torch.remainder(frac - 0.5, 1.0) - 0.5
+    return frac @ box33
+
+
+def _shift(d, box):
+    """box (3,) lengths or a (3,3) cell; a diagonal cell takes the orthorhombic form."""
+    if box.ndim == 2:
+        if torch.count_nonzero(box - torch.diag(torch.diagonal(box))) == 0:
+            return shift_ortho(d, torch.diagonal(box))
+        return shift_cell(d, box)
+    return shift_ortho(d, box)
 
 
 def sr(r, rcut):
@@ -76,7 +97,7 @@ def relative_coord_nlist(coord_i, coord_j, box3, nbrs):
     if Nj == 0:
         x = torch.zeros(coord_i.shape[0], 3, nbrs.shape[1], dtype=coord_i.dtype)
         return x, torch.zeros(coord_i.shape[0], nbrs.shape[1], dtype=coord_i.dtype)
-    x = shift_ortho(coord_j[idx] - coord_i[:, None], box3).permute(0, 2, 1)
+    x = _shift(coord_j[idx] - coord_i[:, None], box3).permute(0, 2, 1)
     r = _safe_norm(x, 1) * valid
     x = x * valid[:, None]
     return x, r
@@ -99,10 +120,9 @@ def lattice_candidate(box33, rcut):
 
 
 def relative_coord_lattice(coord_i, coord_j, box33, cand, lattice_max):
-    """Dense all-pairs block with the lattice_max nearest images (orthorhombic box)."""
+    """Dense all-pairs block with the lattice_max nearest images."""
     Ni, Nj = coord_i.shape[0], coord_j.shape[0]
-    box3 = torch.diagonal(box33)
-    x = shift_ortho(coord_j[None] - coord_i[:, None], box3).permute(0, 2, 1)  # (Ni,3,Nj)
+    x = _shift(coord_j[None] - coord_i[:, None], box33).permute(0, 2, 1)  # (Ni,3,Nj)
     X = len(cand)
     if X > 1:
         shifts = (torch.as_tensor(cand, dtype=box33.dtype) @ box33).T  # (3,X)
@@ -205,9 +225,12 @@ def _descriptor(params, variables, coord, box3, type_idx, nbrs_nm, lattice, stra
     off = np.concatenate([[0], np.cumsum(type_count)])
     cs = [c[off[t]:off[t + 1]] for t in range(ntypes)]
     box3 = torch.as_tensor(np.asarray(box3, np.float64), dtype=dtype)
-    if box3.ndim == 2:
-        box3 = torch.diagonal(box3)
-    box33 = torch.diag(box3)
+    if box3.numel() == 9:
+        box3 = box3.reshape(3, 3)
+        if torch.count_nonzero(box3 - torch.diag(torch.diagonal(box3))) == 0:
+            box3 = torch.diagonal(box3)
+    # box3: (3,) lengths of an orthorhombic box or the (3,3) cell of a general one
+    box33 = torch.diag(box3) if box3.ndim == 1 else box3
     rcut, A = params['rcut'], params['axis']
     mean = np.asarray(params['sr_mean'], np.float64)
     std = np.asarray(params['sr_std'], np.float64)
@@ -412,11 +435,12 @@ def energy_force_virial(params, variables, coord, box3, type_idx, nbrs_nm=None, 
     coord = leaf
     if mobile is not None:
         coord = torch.where(torch.as_tensor(np.asarray(mobile, bool))[:, None], leaf, leaf.detach())
-    box3 = np.asarray(box3, np.float64).reshape(-1)[:3]
+    box3 = np.asarray(box3, np.float64)
+    box3 = box3.reshape(3, 3) if box3.size == 9 else box3.reshape(-1)[:3]
     eps = torch.zeros(3, 3, dtype=dtype, requires_grad=True)
     strain = torch.eye(3, dtype=dtype) + eps
     if isinstance(lattice, str) and lattice == 'auto':
-        lattice = lattice_candidate(np.diag(box3), params['rcut'])
+        lattice = lattice_candidate(box3 if box3.ndim == 2 else np.diag(box3), params['rcut'])
     E = dp_energy(params, variables, coord, box3, type_idx, nbrs_nm=nbrs_nm, lattice=lattice, strain=strain)
     gx, ge = torch.autograd.grad(E, [leaf, eps])
     return E.detach(), -gx.detach(), -ge.detach().T
@@ -466,6 +490,25 @@ def neighbor_sets(coord, box, type_idx, rc_skin, reference_box=None):
     return c, rows, border
 
 
+def neighbor_sets_cell(coord, cell, type_idx, rc):
+    """General cell: per centre (type-sorted order) the sorted type-sorted indices of the
+    atoms within rc under the fractional minimum image, FP64 brute force (exact for
+    perpendicular widths > 2 rc)."""
+    type_idx = np.asarray(type_idx)
+    perm = np.argsort(type_idx, kind='stable')
+    c = np.asarray(coord, np.float64)[perm]
+    H = np.asarray(cell, np.float64).reshape(3, 3)
+    Hi = np.linalg.inv(H)
+    rows = []
+    for i in range(len(c)):
+        f = (c - c[i]) @ Hi
+        d = (f - np.round(f)) @ H
+        m = (d ** 2).sum(1) < rc * rc
+        m[i] = False
+        rows.append(np.nonzero(m)[0])
+    return c, rows
+
+
 def knbr_from_counts(kmax, buffer_ratio=1.2):
     """deepmd_jax/md_utils.py:399-403."""
     k = np.asarray(kmax)
@@ -474,12 +517,17 @@ def knbr_from_counts(kmax, buffer_ratio=1.2):
 
 
 def typed_nlist(coord, box, type_idx, rc_skin, ntypes=2, knbr=None, buffer_ratio=1.2):
-    """nbrs_nm[i][j] (N_i, knbr_j-1) int32, padding N; overflow flag; knbr."""
+    """nbrs_nm[i][j] (N_i, knbr_j-1) int32, padding N; overflow flag; knbr.  box: (3,)
+    lengths, or a (3,3) cell (general cells take neighbor_sets_cell)."""
     type_idx = np.asarray(type_idx)
     N = len(type_idx)
     tc = np.bincount(type_idx, minlength=ntypes)
     off = np.concatenate([[0], np.cumsum(tc)])
-    _, rows, _ = neighbor_sets(coord, box, type_idx, rc_skin)
+    b = np.asarray(box, np.float64)
+    if b.size == 9 and np.any(b.reshape(3, 3) - np.diag(np.diag(b.reshape(3, 3)))):
+        _, rows = neighbor_sets_cell(coord, b, type_idx, rc_skin)
+    else:
+        _, rows, _ = neighbor_sets(coord, np.diag(b.reshape(3, 3)) if b.size == 9 else b, type_idx, rc_skin)
     counts = np.zeros((N, ntypes), int)
     for a, r in enumerate(rows):
         for j in range(ntypes):
@@ -563,7 +611,13 @@ def ring_modes(n_bead, kT, hbar=6.582119569e-1):
 
 def unwrap_beads(x, box):
     d = x - x[0:1]
-    d = d - box * np.round(d / box)
+    box = np.asarray(box, np.float64)
+    if box.size == 9:
+        H = box.reshape(3, 3)
+        f = d @ np.linalg.inv(H)
+        d = (f - np.round(f)) @ H
+    else:
+        d = d - box * np.round(d / box)
     return x[0:1] + d
 
 
