@@ -441,7 +441,7 @@ def dplr(model, params, coord, box, types, vel, steps, warmup):
     run(steps)
     el = time.perf_counter() - t0
     out['fp32'] = {'steps_per_s': steps / el, 'ms_per_step': 1e3 * el / steps,
-                   'long_range_energy_eV': s.dplr_get()[0]}
+                   'long_range_energy_eV': float(s.dplr_get()[0][0])}
     print('# dplr fp32: %.1f steps/s' % (steps / el), file=sys.stderr, flush=True)
     return out
 

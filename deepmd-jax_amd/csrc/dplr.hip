@@ -32,13 +32,15 @@ struct RowTypes {
   const float* dt2[kMaxT];
 };
 
-// charge positions: the atoms, then the centroids x + out_norm sum_w f_w T3[c][w]
+// charge positions: the atoms and the centroids x + out_norm sum_w f_w T3[c][w], each at
+// its charge index (segment blocks)
 __global__ void dplr_charges_kernel(const double* pos, const float* T, const float* Y3, int ldy, int W,
-                                    float out_norm, const int* wc_atom, const int* wc_row, int NA, int nwc,
-                                    double* qxyz) {
+                                    float out_norm, const int* wc_atom, const int* wc_row, const int* atom_q,
+                                    const int* wc_q, int NA, int nwc, double* qxyz) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < NA * 3) {
-    qxyz[t] = pos[t];
+    const int g = t / 3;
+    qxyz[(size_t)atom_q[g] * 3 + (t - 3 * g)] = pos[t];
     return;
   }
   const int u = t - NA * 3;
@@ -49,19 +51,20 @@ __global__ void dplr_charges_kernel(const double* pos, const float* T, const flo
   const float* t3 = T + ((size_t)g * 10 + 1 + c) * W;
   float s = 0.f;
   for (int w = 0; w < W; ++w) s += y[w] * t3[w];
-  qxyz[(size_t)(NA + k) * 3 + c] = pos[(size_t)g * 3 + c] + (double)(out_norm * s);
+  qxyz[(size_t)wc_q[k] * 3 + c] = pos[(size_t)g * 3 + c] + (double)(out_norm * s);
 }
 
 // forces of the atom and centroid charges onto the atoms; g = dE/d(centroid) per atom
-__global__ void dplr_scatter_kernel(const double* qf, const int* atom_wc, int NA, float* force, float* wc_g) {
+__global__ void dplr_scatter_kernel(const double* qf, const int* atom_wc, const int* atom_q, const int* wc_q, int NA,
+                                    float* force, float* wc_g) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= NA * 3) return;
   const int g = t / 3, c = t - 3 * g;
   const int k = atom_wc[g];
-  double f = qf[t];
+  double f = qf[(size_t)atom_q[g] * 3 + c];
   float gw = 0.f;
   if (k >= 0) {
-    const double fw = qf[(size_t)(NA + k) * 3 + c];
+    const double fw = qf[(size_t)wc_q[k] * 3 + c];
     f += fw;
     gw = (float)(-fw);
   }
@@ -122,24 +125,29 @@ __global__ void atomic_out_kernel(const float* T, const float* tbias, const floa
   for (int e = 0; e < 9; ++e) out[(size_t)k * 9 + e] = (double)(out_norm * v[src[e]] * sc[e]);
 }
 
+// segment y of the grid: red[16 y + i] += lr[kLrStride y + i], i < 10
 __global__ void dplr_add_energy_kernel(double* red, const double* lr) {
-  if (threadIdx.x < 10 && blockIdx.x == 0) red[threadIdx.x] += lr[threadIdx.x];
+  if (threadIdx.x < 10) red[(size_t)blockIdx.x * 16 + threadIdx.x] += lr[(size_t)blockIdx.x * kLrStride + threadIdx.x];
 }
 
-// pass 1, per block (strided share): W' (the centroid backward's block partials) +
-// sum_k g_k (x) dx_k -> part[9 block + c]
-__global__ void dplr_virial_kernel(const double* partial, int nblocks, const float* wc_g, const double* qxyz,
-                                   const double* pos, const int* wc_atom, int NA, int nwc, double* part) {
+// pass 1, per (block, segment) (strided share of the segment): W' (the centroid backward's
+// block partials) + sum_k g_k (x) dx_k -> lr[kLrStride seg + 10 + 9 block + c]
+__global__ void dplr_virial_kernel(const double* partial, const SegInfo* seg, const int* wc_seg, const float* wc_g,
+                                   const double* qxyz, const double* pos, const int* wc_atom, const int* wc_q,
+                                   double* lr) {
   __shared__ double red[256];
+  const SegInfo& S = seg[blockIdx.y];
+  double* part = lr + (size_t)blockIdx.y * kLrStride + 10;
   double v[9];
   for (int c = 0; c < 9; ++c) v[c] = 0.0;
   const int stride = gridDim.x * blockDim.x, t0 = blockIdx.x * blockDim.x + threadIdx.x;
-  for (int b = t0; b < nblocks; b += stride)
-    for (int c = 0; c < 9; ++c) v[c] += partial[(size_t)b * 9 + c];
-  for (int k = t0; k < nwc; k += stride) {
+  for (int b = t0; b < S.nblk; b += stride)
+    for (int c = 0; c < 9; ++c) v[c] += partial[(size_t)(S.blk0 + b) * 9 + c];
+  const int k0 = wc_seg[blockIdx.y], k1 = wc_seg[blockIdx.y + 1];
+  for (int k = k0 + t0; k < k1; k += stride) {
     const int g = wc_atom[k];
     double dx[3];
-    for (int d = 0; d < 3; ++d) dx[d] = qxyz[(size_t)(NA + k) * 3 + d] - pos[(size_t)g * 3 + d];
+    for (int d = 0; d < 3; ++d) dx[d] = qxyz[(size_t)wc_q[k] * 3 + d] - pos[(size_t)g * 3 + d];
     for (int a = 0; a < 3; ++a)
       for (int bb = 0; bb < 3; ++bb) v[a * 3 + bb] += (double)wc_g[(size_t)g * 3 + a] * dx[bb];
   }
@@ -155,19 +163,20 @@ __global__ void dplr_virial_kernel(const double* partial, int nblocks, const flo
   }
 }
 
-// pass 2: lr[1..10) += the block partials, added in a fixed order
-__global__ void dplr_virial_sum_kernel(const double* part, int nb, double* lr) {
+// pass 2, one block per segment: lr[1..10) += the block partials, added in a fixed order
+__global__ void dplr_virial_sum_kernel(double* lr) {
   const int c = threadIdx.x;
-  if (c >= 9 || blockIdx.x != 0) return;
+  if (c >= 9) return;
+  double* l = lr + (size_t)blockIdx.x * kLrStride;
   double s = 0.0;
-  for (int b = 0; b < nb; ++b) s += part[b * 9 + c];
-  lr[1 + c] += s;
+  for (int b = 0; b < kLrBlocks; ++b) s += l[10 + b * 9 + c];
+  l[1 + c] += s;
 }
 
 }  // namespace
 
 int32_t dplr_add_energy(System* s) {
-  hipLaunchKernelGGL(dplr_add_energy_kernel, dim3(1), dim3(64), 0, s->stream, s->d_red, s->d_lr);
+  hipLaunchKernelGGL(dplr_add_energy_kernel, dim3(s->nseg), dim3(64), 0, s->stream, s->d_red, s->d_lr);
   DPMD_HIP(hipGetLastError());
   return DPMD_OK;
 }
@@ -182,15 +191,20 @@ static int32_t dplr_run(System* s, Model* M) {
   DPMD_TRY(fit_atomic_fwd(s, b));
   const int nq = NA + nwc;
   hipLaunchKernelGGL(dplr_charges_kernel, dim3((nq * 3 + 255) / 256), dim3(256), 0, st, s->d_pos, s->d_T, b.Y3,
-                     b.FK, W, (float)M->out_norm, s->d_wc_atom, s->d_wc_row, NA, nwc, s->d_qxyz);
-  // long-range energy and the forces on every charge
-  // the segment's box as the device holds it (it moves on the device under NPT)
-  const double* d_box = reinterpret_cast<const double*>(reinterpret_cast<const char*>(s->d_seg) +
-                                                        offsetof(SegInfo, boxd));
+                     b.FK, W, (float)M->out_norm, s->d_wc_atom, s->d_wc_row, s->d_atom_q, s->d_wc_q, NA, nwc,
+                     s->d_qxyz);
+  // long-range energy and the forces on every charge, one PPPM per segment in the segment's
+  // box as the device holds it (it moves on the device under NPT)
   DPMD_HIP(hipMemsetAsync(s->d_qf, 0, sizeof(double) * 3 * nq, st));
-  DPMD_TRY(s->pppm->compute(st, d_box, s->d_qxyz, s->d_q, nq, s->d_lr, s->d_qf));
-  hipLaunchKernelGGL(dplr_scatter_kernel, dim3((NA * 3 + 255) / 256), dim3(256), 0, st, s->d_qf, s->d_atom_wc, NA,
-                     s->d_force, s->d_wc_g);
+  for (int r = 0; r < s->nseg; ++r) {
+    const double* d_box = reinterpret_cast<const double*>(reinterpret_cast<const char*>(s->d_seg + r) +
+                                                          offsetof(SegInfo, boxd));
+    const size_t q0 = (size_t)s->q_base[r];
+    DPMD_TRY(s->pppm->compute(st, d_box, s->d_qxyz + 3 * q0, s->d_q + q0, s->q_n[r], s->d_lr + (size_t)r * kLrStride,
+                              s->d_qf + 3 * q0));
+  }
+  hipLaunchKernelGGL(dplr_scatter_kernel, dim3((NA * 3 + 255) / 256), dim3(256), 0, st, s->d_qf, s->d_atom_wc,
+                     s->d_atom_q, s->d_wc_q, NA, s->d_force, s->d_wc_g);
   // backward of the centroid model for the upstream gradient g
   RowTypes rt;
   memset(&rt, 0, sizeof(rt));
@@ -212,20 +226,19 @@ static int32_t dplr_run(System* s, Model* M) {
   s->wc_y3 = nullptr;
   s->partial_out = nullptr;
   DPMD_TRY(rc);
-  constexpr int kVirBlocks = 64;
-  double* vpart = s->d_lr + 10;
-  hipLaunchKernelGGL(dplr_virial_kernel, dim3(kVirBlocks), dim3(256), 0, st, s->d_partial_wc, s->nblocks,
-                     s->d_wc_g, s->d_qxyz, s->d_pos, s->d_wc_atom, NA, nwc, vpart);
-  hipLaunchKernelGGL(dplr_virial_sum_kernel, dim3(1), dim3(64), 0, st, vpart, kVirBlocks, s->d_lr);
+  hipLaunchKernelGGL(dplr_virial_kernel, dim3(kLrBlocks, s->nseg), dim3(256), 0, st, s->d_partial_wc, s->d_seg,
+                     s->d_wc_seg, s->d_wc_g, s->d_qxyz, s->d_pos, s->d_wc_atom, s->d_wc_q, s->d_lr);
+  hipLaunchKernelGGL(dplr_virial_sum_kernel, dim3(s->nseg), dim3(64), 0, st, s->d_lr);
   DPMD_HIP(hipGetLastError());
   return DPMD_OK;
 }
 
 int32_t dplr_forces(System* s) {
-  if (s->seg[0].tri) {
-    set_error("DPLR runs orthorhombic boxes (deepmd_jax/md.py:391-392)");
-    return DPMD_E_UNSUPPORTED;
-  }
+  for (const SegInfo& S : s->seg)
+    if (S.tri) {
+      set_error("DPLR runs orthorhombic boxes (deepmd_jax/md.py:391-392)");
+      return DPMD_E_UNSUPPORTED;
+    }
   Model* sr = s->model;
   s->model = s->wc_model;
   const int32_t rc = dplr_run(s, s->wc_model);
@@ -268,38 +281,69 @@ extern "C" int32_t dpmd_system_set_dplr(dpmd_system_t sys, dpmd_model_t wc_model
     return DPMD_E_UNSUPPORTED;
   }
   if (W->tab_n > 0) { set_error("tabulated centroid models are not supported"); return DPMD_E_UNSUPPORTED; }
-  if (s->nseg != 1 || s->seg[0].nimg != 1) {
-    set_error("DPLR runs one system held without periodic images");
-    return DPMD_E_UNSUPPORTED;
-  }
+  for (const SegInfo& S : s->seg)
+    if (S.nimg != 1) {
+      set_error("DPLR runs systems held without periodic images");
+      return DPMD_E_UNSUPPORTED;
+    }
   if (W->rcut > M->rcut) {
     set_error("the centroid model's cutoff %.3f exceeds the energy model's %.3f (shared neighbour list)", W->rcut,
               M->rcut);
     return DPMD_E_UNSUPPORTED;
   }
-  if (s->seg[0].tri) {
-    set_error("DPLR runs orthorhombic boxes (deepmd_jax/md.py:391-392)");
-    return DPMD_E_UNSUPPORTED;
-  }
+  for (const SegInfo& S : s->seg)
+    if (S.tri) {
+      set_error("DPLR runs orthorhombic boxes (deepmd_jax/md.py:391-392)");
+      return DPMD_E_UNSUPPORTED;
+    }
   if (!(beta > 0)) { set_error("DPLR needs beta > 0"); return DPMD_E_ARG; }
   for (int d = 0; d < 3; ++d)
     if (grid[d] < 2 || grid[d] > 4096) { set_error("PPPM grid size %d outside [2, 4096]", grid[d]); return DPMD_E_ARG; }
   DPMD_TRY(atomic_setup(s, W));
-  const int NA = s->NA, nwc = s->n_wc;
-  std::vector<double> q;
-  for (int g = 0; g < NA; ++g) q.push_back(q_atoms[s->type_sorted[g]]);
-  for (int g = 0; g < NA; ++g)
-    if (W->nsel_mask >> s->type_sorted[g] & 1) q.push_back(q_wc[s->type_sorted[g]]);
-  const int nq = NA + nwc;
+  const int NA = s->NA, nwc = s->n_wc, nq = NA + nwc;
+  // charge blocks per segment: the atoms, then the centroids in the order atomic_setup gives
+  // them (segment by segment, sorted atom order)
+  std::vector<int> atom_q(NA), wc_q(std::max(nwc, 1)), wc_seg(s->nseg + 1);
+  std::vector<double> q(nq);
+  s->q_base.assign(s->nseg, 0);
+  s->q_n.assign(s->nseg, 0);
+  int k = 0, qi = 0;
+  for (int r = 0; r < s->nseg; ++r) {
+    const SegInfo& S = s->seg[r];
+    wc_seg[r] = k;
+    s->q_base[r] = qi;
+    for (int c = 0; c < S.natoms; ++c) {
+      const int g = S.base + c, t = s->type_sorted[g];
+      atom_q[g] = qi + c;
+      q[qi + c] = q_atoms[t];
+    }
+    for (int c = 0; c < S.natoms; ++c) {
+      const int t = s->type_sorted[S.base + c];
+      if (!(W->nsel_mask >> t & 1)) continue;
+      wc_q[k] = qi + S.natoms + (k - wc_seg[r]);
+      q[wc_q[k]] = q_wc[t];
+      ++k;
+    }
+    s->q_n[r] = S.natoms + (k - wc_seg[r]);
+    qi += s->q_n[r];
+  }
+  wc_seg[s->nseg] = k;
+  if (k != nwc || qi != nq) { set_error("DPLR charge layout mismatch"); return DPMD_E_STATE; }
   DPMD_TRY(sys_ensure(s, (void**)&s->d_qxyz, sizeof(double) * 3 * nq));
   DPMD_TRY(sys_ensure(s, (void**)&s->d_q, sizeof(double) * nq));
   DPMD_TRY(sys_ensure(s, (void**)&s->d_qf, sizeof(double) * 3 * nq));
-  DPMD_TRY(sys_ensure(s, (void**)&s->d_lr, sizeof(double) * (10 + 9 * 64)));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_lr, sizeof(double) * kLrStride * (size_t)s->nseg));
   DPMD_TRY(sys_ensure(s, (void**)&s->d_wc_g, sizeof(float) * 3 * NA));
   DPMD_TRY(sys_ensure(s, (void**)&s->d_partial_wc, sizeof(double) * 9 * (size_t)std::max(s->nblocks, 1)));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_atom_q, sizeof(int) * NA));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_wc_q, sizeof(int) * wc_q.size()));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_wc_seg, sizeof(int) * wc_seg.size()));
   const hipStream_t st = s->stream;
   DPMD_HIP(hipMemcpyAsync(s->d_q, q.data(), sizeof(double) * nq, hipMemcpyHostToDevice, st));
-  DPMD_HIP(hipMemsetAsync(s->d_lr, 0, sizeof(double) * 10, st));
+  DPMD_HIP(hipMemcpyAsync(s->d_atom_q, atom_q.data(), sizeof(int) * NA, hipMemcpyHostToDevice, st));
+  DPMD_HIP(hipMemcpyAsync(s->d_wc_q, wc_q.data(), sizeof(int) * wc_q.size(), hipMemcpyHostToDevice, st));
+  DPMD_HIP(hipMemcpyAsync(s->d_wc_seg, wc_seg.data(), sizeof(int) * wc_seg.size(), hipMemcpyHostToDevice, st));
+  DPMD_HIP(hipMemsetAsync(s->d_lr, 0, sizeof(double) * kLrStride * (size_t)s->nseg, st));
   if (!s->pppm) s->pppm = new Pppm();
   for (int d = 0; d < 3; ++d) s->dplr_grid[d] = grid[d];
   s->dplr_beta = beta;
@@ -381,15 +425,21 @@ extern "C" int32_t dpmd_atomic_predict(dpmd_system_t sys, double* out) {
 extern "C" int32_t dpmd_dplr_get(dpmd_system_t sys, double* e_lr, double* centroids) {
   SYS(sys);
   if (!s->wc_model) { set_error("DPLR is not set on this system"); return DPMD_E_STATE; }
-  const int NA = s->NA, nwc = s->n_wc;
-  if (e_lr) DPMD_HIP(hipMemcpyAsync(e_lr, s->d_lr, sizeof(double), hipMemcpyDeviceToHost, s->stream));
-  std::vector<double> c((size_t)3 * std::max(nwc, 1));
-  if (centroids && nwc > 0)
-    DPMD_HIP(hipMemcpyAsync(c.data(), s->d_qxyz + (size_t)NA * 3, sizeof(double) * 3 * nwc, hipMemcpyDeviceToHost,
-                            s->stream));
+  const int NA = s->NA, nwc = s->n_wc, nq = NA + nwc;
+  std::vector<double> lr((size_t)kLrStride * s->nseg);
+  if (e_lr)
+    DPMD_HIP(hipMemcpyAsync(lr.data(), s->d_lr, sizeof(double) * lr.size(), hipMemcpyDeviceToHost, s->stream));
+  std::vector<double> c((size_t)3 * nq);
+  std::vector<int> wq(std::max(nwc, 1));
+  if (centroids && nwc > 0) {
+    DPMD_HIP(hipMemcpyAsync(c.data(), s->d_qxyz, sizeof(double) * 3 * nq, hipMemcpyDeviceToHost, s->stream));
+    DPMD_HIP(hipMemcpyAsync(wq.data(), s->d_wc_q, sizeof(int) * nwc, hipMemcpyDeviceToHost, s->stream));
+  }
   DPMD_HIP(hipStreamSynchronize(s->stream));
+  if (e_lr)
+    for (int r = 0; r < s->nseg; ++r) e_lr[r] = lr[(size_t)r * kLrStride];
   if (centroids)
     for (int k = 0; k < nwc; ++k)
-      for (int d = 0; d < 3; ++d) centroids[(size_t)s->wc_perm[k] * 3 + d] = c[(size_t)k * 3 + d];
+      for (int d = 0; d < 3; ++d) centroids[(size_t)s->wc_perm[k] * 3 + d] = c[(size_t)wq[k] * 3 + d];
   return DPMD_OK;
 }

@@ -51,6 +51,10 @@ struct Pppm {
   } while (0)
 
 constexpr int kMaxT = DPMD_MAX_TYPES;
+// DPLR per-segment record: [0] long-range energy, [1..10) its virial, then 64 x 9 block
+// partials of the virial reduction
+constexpr int kLrBlocks = 64;
+constexpr int kLrStride = 10 + 9 * kLrBlocks;
 constexpr int kKeBlocks = 64;   // most blocks per segment of the kinetic-energy reductions
 constexpr int kWave = 64;
 
@@ -258,8 +262,9 @@ struct System {
   double* d_box_traj = nullptr; // per-step boxes of the last recorded chunk (steps, nseg, 3)
   int box_traj_steps = 0;
   uint64_t rng_counter = 0;
-  // DPLR (dpmd_system_set_dplr): centroid model, charges and PPPM of one segment.  Charge
-  // layout: the NA atoms in sorted order, then the n_wc centroids in sorted atom order.
+  // DPLR (dpmd_system_set_dplr): centroid model, charges and one PPPM per segment.  Charge
+  // layout: segment by segment, the segment's atoms in sorted order, then its centroids in
+  // sorted atom order (block r holds q_n[r] charges from q_base[r]).
   Model* wc_model = nullptr;
   Pppm* pppm = nullptr;
   int n_wc = 0;
@@ -268,10 +273,14 @@ struct System {
   int* d_atom_wc = nullptr;     // (NA) centroid of each atom or -1
   int dplr_grid[3] = {0, 0, 0};
   double dplr_beta = 0.0;
+  int* d_atom_q = nullptr;      // (NA) charge index of each atom
+  int* d_wc_q = nullptr;        // (n_wc) charge index of each centroid
+  int* d_wc_seg = nullptr;      // (nseg + 1) first centroid of each segment
+  std::vector<int> q_base, q_n; // charge block of each segment
   double* d_qxyz = nullptr;     // (NA + n_wc, 3) charge positions
   double* d_q = nullptr;        // (NA + n_wc) charges
   double* d_qf = nullptr;       // (NA + n_wc, 3) PPPM forces
-  double* d_lr = nullptr;       // long-range energy of the last evaluation
+  double* d_lr = nullptr;       // per segment kLrStride doubles: long-range energy, virial, partials
   float* d_wc_g = nullptr;      // (NA, 3) dE_lr / d(centroid) of each atom's centroid (0 if none)
   double* d_partial_wc = nullptr;  // virial partials of the centroid backward (not used)
   float* d_wc_fit = nullptr;    // 10 activation buffers (mrows x fit_k) of the centroid model,

@@ -468,11 +468,15 @@ class DeviceSystem:
         return out.reshape(-1, ncomp) if self.ragged else out.reshape(self.nrep, -1, ncomp)
 
     def dplr_get(self):
-        """(long-range energy, centroids (n_wc, 3)) of the last evaluation."""
-        e = np.zeros(1)
-        c = np.zeros((max(self._n_wc, 1), 3))
+        """(long-range energy per replica (nrep,), centroids) of the last evaluation: centroids
+        (nrep, M, 3) for the M atoms of the selected types per replica in caller order (ragged
+        systems: (sum M, 3) segment by segment)."""
+        m = self._n_wc * (1 if self.ragged else self.nrep)
+        e = np.zeros(self.nrep)
+        c = np.zeros((max(m, 1), 3))
         check(lib().dpmd_dplr_get(self.handle, ptr(e), ptr(c)))
-        return float(e[0]), c[:self._n_wc]
+        c = c[:m]
+        return e, (c if self.ragged else c.reshape(self.nrep, -1, 3))
 
     def set_fixed(self, fixed):
         """Fixed-atom mask in caller order (flat over all atoms of all replicas), or None."""

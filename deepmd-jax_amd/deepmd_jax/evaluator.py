@@ -91,8 +91,8 @@ def dplr_grid(box3, beta, resolution):
 
 class Evaluator:
     """Energies, forces and virials of frames for one model (box: (3,) lengths or a (3,3)
-    cell).  A DPLR model evaluates frame by frame with its long-range term (orthorhombic
-    boxes, deepmd_jax/train.py:796-797)."""
+    cell).  A DPLR model adds its long-range term to every frame (orthorhombic boxes,
+    deepmd_jax/train.py:796-797)."""
 
     def __init__(self, params, variables, device=0, model=None):
         self.params = params
@@ -122,18 +122,15 @@ class Evaluator:
                 raise ValueError('For "dplr" currently only orthorhombic boxes are supported.')
             if nk > 1:
                 raise NotImplementedError('DPLR evaluation needs box lengths >= 2*rcut (%.3f)' % (2 * self.rcut))
+            # the frames of a batch as segments of one system, each with its own PPPM
             wdm, qa, qw, beta, res = self.dplr
-            out = [], [], []
-            sys = self._system(type_idx, 1)
-            for c in coords:
-                sys.set_box(box3)
-                sys.set_positions(c)
-                sys.nlist_allocate(self.rcut, 1.2)
-                sys.set_dplr(wdm, qa, qw, beta, dplr_grid(box3, beta, res))
-                e, f, v, _ = sys.compute(precision)
-                for o, x in zip(out, (e[0], f[0], v[0])):
-                    o.append(x)
-            return np.array(out[0]), np.stack(out[1]), np.stack(out[2])
+            sys = self._system(type_idx, F)
+            sys.set_box(box3)
+            sys.set_positions(coords)
+            sys.nlist_allocate(self.rcut, 1.2)
+            sys.set_dplr(wdm, qa, qw, beta, dplr_grid(box3, beta, res))
+            e, f, v, _ = sys.compute(precision)
+            return e, f, v
         if nk > 1:
             cs = [replicate(c, box3, type_idx, k) for c in coords]
             sc = np.stack([c[0] for c in cs])

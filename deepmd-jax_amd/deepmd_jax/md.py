@@ -18,7 +18,7 @@ on the device (deepmd_jax/md.py:221-234,786-790, deepmd_jax/routine.py:110-136).
 XYZ dumps (run(dump_prefix=...)) stream from a writer thread (deepmd_jax/dumps.py).
 Model deviation (model_deviation_paths) evaluates each listed model on its own device
 system at report time.  DPLR models (type 'dplr') add the Wannier-centroid PPPM term on the
-device, with its virial (one system without beads, boxes >= 2 rcut).
+device, with its virial, per bead under PIMD (orthorhombic boxes >= 2 rcut).
 """
 import time
 
@@ -287,8 +287,6 @@ class Simulation:
         if self._dplr is not None:
             if self._general_cell:
                 raise NotImplementedError('dplr model currently only supports orthorhombic box')
-            if self._is_pimd:
-                raise NotImplementedError('DPLR runs without beads on this engine.')
             if self._sys.nimg > 1:
                 raise NotImplementedError('DPLR needs box lengths >= 2*rcut (%.3f) on this engine.' % (2 * self._rcut))
         # mobile-atom mask (deepmd_jax/md.py:220-226); the device zeroes the forces on fixed

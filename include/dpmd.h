@@ -265,9 +265,11 @@ int32_t dpmd_md_box_trajectory(dpmd_system_t sys, double* boxes);
  * Wannier centroid x_i + dx_i for every atom of its selected types; the PPPM energy of charge
  * q_atoms[type] at every atom and q_wc[type] at every centroid (beta, grid[3]) joins the energy, and
  * its gradient through the centroid model joins the forces of every later evaluation and MD step.
- * One segment without periodic images.  The virial includes the long-range term (PPPM
- * reciprocal virial, the centroid model's strain response and the centroid offsets), so NPT
- * runs with it.  wc_model NULL removes the term. */
+ * Every segment (a frame, a bead) carries its own PPPM in its own box, as the reference vmaps
+ * the per-configuration energy over beads (deepmd_jax/md.py:436-440); segments hold no periodic
+ * images, boxes are orthorhombic and all share the grid.  The virial includes the long-range
+ * term (PPPM reciprocal virial, the centroid model's strain response and the centroid offsets),
+ * so NPT and path-integral NPT run with it.  wc_model NULL removes the term. */
 int32_t dpmd_system_set_dplr(dpmd_system_t sys, dpmd_model_t wc_model, const double* q_atoms, const double* q_wc,
                              double beta, const int32_t* grid);
 /* Outputs of an atomic model (the system's model has desc.atomic != 0) at the current positions:
@@ -275,8 +277,9 @@ int32_t dpmd_system_set_dplr(dpmd_system_t sys, dpmd_model_t wc_model, const dou
  * (vector) or 9 (symmetric tensor, row-major) in FP64 (FP32 arithmetic).  Replaces the
  * atomic branch of DPModel.apply / evaluate (deepmd_jax/dpmodel.py:83-99, train.py:765-788). */
 int32_t dpmd_atomic_predict(dpmd_system_t sys, double* out);
-/* Of the last evaluation with DPLR set: the long-range energy and the centroid positions (n_wc x 3,
- * raw order among the atoms of the selected types, as wc_predict returns them), either may be NULL. */
+/* Of the last evaluation with DPLR set: the long-range energy of every segment (nseg) and the
+ * centroid positions (n_wc x 3, segment by segment, caller order among the atoms of the selected
+ * types, as wc_predict returns them), either may be NULL. */
 int32_t dpmd_dplr_get(dpmd_system_t sys, double* e_long_range, double* centroids);
 /* PPPM long-range energy and forces of n point charges in an orthorhombic periodic box, the
  * get_p3mlr_fn of deepmd_jax/utils.py:177-222 (TSC assignment onto grid[0] x grid[1] x grid[2],

@@ -49,6 +49,7 @@ def test_dplr_energy_forces_centroids(device_model, model0):
     s, wc, coord, box, types, m, M = _setup(device_model, model0)
     e, f, v, _ = s.compute()
     elr, cen = s.dplr_get()
+    elr, cen = elr[0], cen[0]
     E, Elr, F, W = _oracle(model0, wc, coord, box, types, M)
     N = len(types)
     assert np.abs(cen - W).max() < 1e-5
@@ -118,9 +119,9 @@ def test_dplr_model_pkl_simulation_and_evaluate(model0, tmp_path):
     with contextlib.redirect_stdout(io.StringIO()):
         sim = Simulation(path, box, types, syn.MASSES, 'NVE', 0.05, coord, initial_velocity=
                          syn.maxwell_boltzmann(types, 300.0, seed=42), report_interval=10)
-        elr0 = sim._sys.dev.dplr_get()[0] if sim.getEnergy() else None
+        elr0 = sim._sys.dev.dplr_get()[0][0] if sim.getEnergy() else None
         sim.run(40)
-    elr1 = sim._sys.dev.dplr_get()[0]
+    elr1 = sim._sys.dev.dplr_get()[0][0]
     inv = np.array(sim.log)[:, 4]
     assert abs(elr1 - elr0) > 1.0, (elr0, elr1)
     assert np.abs(inv - inv[0]).max() / len(types) < 2e-5, inv
@@ -193,3 +194,114 @@ def test_evaluate_atomic_models(tmp_path, kind):
             lat = O.lattice_candidate(np.diag(box[f]), wp['rcut'])
             want = O.dp_atomic(wp, wv, torch.as_tensor(coord[f]), box[f], types, lattice=lat).numpy()
             assert np.abs(got[f] - want).max() <= 1e-5 * np.abs(want).max(), (nmol, f)
+
+
+def test_dplr_segments_with_own_boxes(device_model, model0):
+    """Two frames as two segments of one system, the second in a box 2 % larger (same grid):
+    each segment's energy, forces, long-range energy and centroids against the oracle."""
+    from deepmd_jax import _lib, synthetic as syn
+    print('case dplr segments', flush=True)
+    params, var = model0
+    wp, wv = syn.init_wannier_model(4)
+    wdm = _lib.DeviceModel(wp, wv)
+    c0, box, types = syn.water_box(128, seed=60)
+    c1 = syn.water_box(128, seed=61)[0] * 1.02
+    boxes = np.stack([box, box * 1.02])
+    M = O.p3m_grid(box, BETA, 5)
+    s = _lib.DeviceSystem(device_model, types, 2)
+    s.set_boxes(boxes)
+    s.set_masses(np.ones(len(types)))
+    s.set_positions(np.stack([c0, c1]))
+    s.nlist_allocate(params['rcut'] + 0.5, 1.2)
+    s.set_dplr(wdm, Q_ATOMS, Q_WC, BETA, M)
+    e, f, v, _ = s.compute()
+    elr, cen = s.dplr_get()
+    N = len(types)
+    for k, (c, b) in enumerate(((c0, box), (c1, box * 1.02))):
+        E, Elr, F, Wc = _oracle(model0, (wp, wv), c, b, types, M)
+        assert abs(elr[k] - Elr) <= 1e-5 * abs(Elr), (k, elr[k], Elr)
+        assert np.abs(cen[k] - Wc).max() < 1e-5
+        assert abs(e[k] - E) / N <= 1e-5, (k, e[k], E)
+        assert np.abs(f[k] - F).max() <= 1e-4 * np.abs(F).max()
+
+
+def test_dplr_pimd_step_against_oracle(device_model, model0):
+    """Ring-polymer BAOAB without noise (4 beads x 64 H2O) with the DPLR term on every bead,
+    3 steps, against the oracle's ring-polymer step with per-bead DPLR forces."""
+    from deepmd_jax import _lib, synthetic as syn
+    from deepmd_jax.constants import MASS_UNIT_CONVERSION, TEMP_UNIT_CONVERSION
+    from deepmd_jax.pimd import normal_mode_transform
+    print('case dplr pimd', flush=True)
+    params, var = model0
+    P, dt = 4, 0.1
+    wp, wv = syn.init_wannier_model(4)
+    wdm = _lib.DeviceModel(wp, wv)
+    coord, box, types = syn.water_box(64, seed=62)
+    rng = np.random.default_rng(62)
+    x = coord[None] + rng.uniform(-0.03, 0.03, (P,) + coord.shape)
+    m = np.array(syn.MASSES)[types] * MASS_UNIT_CONVERSION
+    kT = 300 * TEMP_UNIT_CONVERSION
+    pm = rng.standard_normal(x.shape) * np.sqrt(m * kT * P)[None, :, None]
+    freqs, C = normal_mode_transform(P, kT)
+    M = O.p3m_grid(box, BETA, 5)
+    s = _lib.DeviceSystem(device_model, types, P)
+    s.set_box(box)
+    s.set_masses(m)
+    s.set_positions(x)
+    s.set_velocities(pm / m[None, :, None])
+    s.nlist_allocate(params['rcut'] + 0.2, 1.2)
+    s.set_dplr(wdm, Q_ATOMS, Q_WC, BETA, M)
+    s.pimd_setup(C, freqs, np.ones(P), np.zeros(P))
+    p = _lib.MDParams()
+    p.routine, p.dt = 3, dt
+    tp, tv, err = s.md_run(p, 3)
+    assert err == 0
+
+    def forces(xx):
+        return np.stack([_oracle(model0, (wp, wv), xx[b], box, types, M)[2] for b in range(P)])
+    F = forces(x)
+    xo, po = x.copy(), pm.copy()
+    for _ in range(3):
+        xo, po, F = O.pimd_step(xo, po, m, forces, dt, C, freqs, np.ones(P), np.zeros(P), F, box)
+    assert np.abs(tp[-1] - xo).max() < 1e-5
+    assert np.abs(tv[-1] - po / m[None, :, None]).max() < 1e-5
+
+
+def test_dplr_evaluate_batches_and_pimd_simulation(model0, tmp_path):
+    """evaluate() of a 'dplr' model.pkl with batch_size 3 (frames as segments) equals the oracle
+    frame by frame; Simulation with 4 beads and DPLR runs and keeps its path-integral invariant."""
+    import contextlib
+    import io
+    from deepmd_jax import synthetic as syn
+    from deepmd_jax.dpmodel import DPModel
+    from deepmd_jax.md import Simulation
+    from deepmd_jax.train import evaluate
+    from deepmd_jax.utils import save_model
+    print('case dplr batches', flush=True)
+    params, var = model0
+    wp, wv = syn.init_wannier_model(4)
+    dparams = dict(params)
+    dparams.update({'type': 'dplr', 'dplr_wannier_model_and_variables': (DPModel(wp), {'params': wv['params']}),
+                    'dplr_q_atoms': Q_ATOMS, 'dplr_q_wc': Q_WC[:1], 'dplr_beta': BETA, 'dplr_resolution': 5})
+    path = str(tmp_path / 'dplr.pkl')
+    with contextlib.redirect_stdout(io.StringIO()):
+        save_model(path, DPModel(dparams), var)
+    frames = [syn.water_box(64, seed=s) for s in (63, 64, 65)]
+    coord = np.stack([f[0] for f in frames])
+    box = frames[0][1]
+    types = frames[0][2]
+    with contextlib.redirect_stdout(io.StringIO()):
+        out = evaluate(path, coord, np.stack([box] * 3), types, batch_size=3)
+    M = O.p3m_grid(box, BETA, 5)
+    for k in range(3):
+        E, _, F, _ = _oracle(model0, (wp, wv), coord[k], box, types, M)
+        assert abs(out['energy'][k] - E) / len(types) <= 1e-5, k
+        assert np.abs(out['force'][k] - F).max() <= 1e-4 * np.abs(F).max()
+    with contextlib.redirect_stdout(io.StringIO()):
+        sim = Simulation(path, box, types, syn.MASSES, 'NVT', 0.05, coord[0], temperature=300.0, n_bead=4,
+                         pimd_print_invariant=True, report_interval=10, tau_t=100.0)
+        sim.run(40)
+    log = np.array(sim.log)
+    inv = log[:, 4]
+    assert np.isfinite(log).all()
+    assert np.abs(inv - inv[0]).max() / len(types) < 1e-4, inv
