@@ -360,6 +360,7 @@ static int32_t sys_layout(System* s, int nseg, const int32_t* seg_natoms, const 
     for (int b = 0; b < S.nblk; ++b) s->blk_seg.push_back(r);
     blk += S.nblk;
     S.nimg = 1;
+    S.kimg[0] = S.kimg[1] = S.kimg[2] = 1;
     S.seed = (unsigned long long)r;
     S.step = 0;
     base += n;
@@ -613,6 +614,7 @@ static int32_t set_seg_box(System* s, int r, const double* b) {
     S.box[d] = (float)b[d];
     if (!s->nl_ready) S.L[d] = (float)b[d];
     S.scale[d] = S.L[d] / (float)b[d];
+    S.cellbox[d] = S.kimg[d] > 0 ? b[d] / S.kimg[d] : 0.0;
   }
   s->seg_dirty = true;
   s->force_valid = false;
@@ -771,8 +773,35 @@ int32_t dpmd_system_set_images(dpmd_system_t sys, const int32_t* nimg) {
         return DPMD_E_ARG;
       }
     S.nimg = nimg[r];
+    for (int d = 0; d < 3; ++d) {
+      S.kimg[d] = S.nimg == 1 ? 1 : 0;  // per-axis factors unknown
+      S.cellbox[d] = S.nimg == 1 ? S.boxd[d] : 0.0;
+    }
   }
   s->seg_dirty = true;
+  s->force_valid = false;
+  return DPMD_OK;
+}
+
+int32_t dpmd_system_set_image_grid(dpmd_system_t sys, const int32_t* k3) {
+  SYS(sys);
+  if (!k3) { set_error("null argument"); return DPMD_E_ARG; }
+  std::vector<int32_t> n(s->nseg);
+  for (int r = 0; r < s->nseg; ++r) {
+    if (k3[3 * r] < 1 || k3[3 * r + 1] < 1 || k3[3 * r + 2] < 1) {
+      set_error("segment %d: image factors must be >= 1", r);
+      return DPMD_E_ARG;
+    }
+    n[r] = k3[3 * r] * k3[3 * r + 1] * k3[3 * r + 2];
+  }
+  DPMD_TRY(dpmd_system_set_images(sys, n.data()));
+  for (int r = 0; r < s->nseg; ++r) {
+    SegInfo& S = s->seg[r];
+    for (int d = 0; d < 3; ++d) {
+      S.kimg[d] = k3[3 * r + d];
+      S.cellbox[d] = S.boxd[d] / S.kimg[d];
+    }
+  }
   return DPMD_OK;
 }
 

@@ -32,20 +32,25 @@ struct RowTypes {
   const float* dt2[kMaxT];
 };
 
+// Charge index maps: v >= 0 for an atom or centroid that owns its charge (the primary cell of
+// a supercell segment), -(q + 1) for an image, which reads its primary's charge q.
+__host__ __device__ __forceinline__ int qdec(int v) { return v >= 0 ? v : -v - 1; }
+
 // charge positions: the atoms and the centroids x + out_norm sum_w f_w T3[c][w], each at
-// its charge index (segment blocks)
+// its charge index (segment blocks; images write nothing)
 __global__ void dplr_charges_kernel(const double* pos, const float* T, const float* Y3, int ldy, int W,
                                     float out_norm, const int* wc_atom, const int* wc_row, const int* atom_q,
                                     const int* wc_q, int NA, int nwc, double* qxyz) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < NA * 3) {
     const int g = t / 3;
-    qxyz[(size_t)atom_q[g] * 3 + (t - 3 * g)] = pos[t];
+    if (atom_q[g] >= 0) qxyz[(size_t)atom_q[g] * 3 + (t - 3 * g)] = pos[t];
     return;
   }
   const int u = t - NA * 3;
   if (u >= nwc * 3) return;
   const int k = u / 3, c = u - 3 * k;
+  if (wc_q[k] < 0) return;
   const int g = wc_atom[k];
   const float* y = Y3 + (size_t)wc_row[k] * ldy;
   const float* t3 = T + ((size_t)g * 10 + 1 + c) * W;
@@ -61,10 +66,10 @@ __global__ void dplr_scatter_kernel(const double* qf, const int* atom_wc, const 
   if (t >= NA * 3) return;
   const int g = t / 3, c = t - 3 * g;
   const int k = atom_wc[g];
-  double f = qf[(size_t)atom_q[g] * 3 + c];
+  double f = qf[(size_t)qdec(atom_q[g]) * 3 + c];
   float gw = 0.f;
   if (k >= 0) {
-    const double fw = qf[(size_t)wc_q[k] * 3 + c];
+    const double fw = qf[(size_t)qdec(wc_q[k]) * 3 + c];
     f += fw;
     gw = (float)(-fw);
   }
@@ -131,7 +136,8 @@ __global__ void dplr_add_energy_kernel(double* red, const double* lr) {
 }
 
 // pass 1, per (block, segment) (strided share of the segment): W' (the centroid backward's
-// block partials) + sum_k g_k (x) dx_k -> lr[kLrStride seg + 10 + 9 block + c]
+// block partials, per image) + sum_k g_k (x) dx_k over the primary centroids ->
+// lr[kLrStride seg + 10 + 9 block + c]
 __global__ void dplr_virial_kernel(const double* partial, const SegInfo* seg, const int* wc_seg, const float* wc_g,
                                    const double* qxyz, const double* pos, const int* wc_atom, const int* wc_q,
                                    double* lr) {
@@ -141,10 +147,12 @@ __global__ void dplr_virial_kernel(const double* partial, const SegInfo* seg, co
   double v[9];
   for (int c = 0; c < 9; ++c) v[c] = 0.0;
   const int stride = gridDim.x * blockDim.x, t0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const double inv = 1.0 / S.nimg;
   for (int b = t0; b < S.nblk; b += stride)
-    for (int c = 0; c < 9; ++c) v[c] += partial[(size_t)(S.blk0 + b) * 9 + c];
+    for (int c = 0; c < 9; ++c) v[c] += partial[(size_t)(S.blk0 + b) * 9 + c] * inv;
   const int k0 = wc_seg[blockIdx.y], k1 = wc_seg[blockIdx.y + 1];
   for (int k = k0 + t0; k < k1; k += stride) {
+    if (wc_q[k] < 0) continue;
     const int g = wc_atom[k];
     double dx[3];
     for (int d = 0; d < 3; ++d) dx[d] = qxyz[(size_t)wc_q[k] * 3 + d] - pos[(size_t)g * 3 + d];
@@ -197,8 +205,10 @@ static int32_t dplr_run(System* s, Model* M) {
   // box as the device holds it (it moves on the device under NPT)
   DPMD_HIP(hipMemsetAsync(s->d_qf, 0, sizeof(double) * 3 * nq, st));
   for (int r = 0; r < s->nseg; ++r) {
-    const double* d_box = reinterpret_cast<const double*>(reinterpret_cast<const char*>(s->d_seg + r) +
-                                                          offsetof(SegInfo, boxd));
+    // a supercell segment's long-range term is that of its primary cell
+    const size_t box_at = s->seg[r].nimg > 1 ? offsetof(SegInfo, cellbox) : offsetof(SegInfo, boxd);
+    const double* d_box =
+        reinterpret_cast<const double*>(reinterpret_cast<const char*>(s->d_seg + r) + box_at);
     const size_t q0 = (size_t)s->q_base[r];
     DPMD_TRY(s->pppm->compute(st, d_box, s->d_qxyz + 3 * q0, s->d_q + q0, s->q_n[r], s->d_lr + (size_t)r * kLrStride,
                               s->d_qf + 3 * q0));
@@ -282,9 +292,9 @@ extern "C" int32_t dpmd_system_set_dplr(dpmd_system_t sys, dpmd_model_t wc_model
   }
   if (W->tab_n > 0) { set_error("tabulated centroid models are not supported"); return DPMD_E_UNSUPPORTED; }
   for (const SegInfo& S : s->seg)
-    if (S.nimg != 1) {
-      set_error("DPLR runs systems held without periodic images");
-      return DPMD_E_UNSUPPORTED;
+    if (S.nimg > 1 && (S.kimg[0] < 1 || S.kimg[1] < 1 || S.kimg[2] < 1)) {
+      set_error("DPLR on periodic images needs their per-axis counts (dpmd_system_set_image_grid)");
+      return DPMD_E_STATE;
     }
   if (W->rcut > M->rcut) {
     set_error("the centroid model's cutoff %.3f exceeds the energy model's %.3f (shared neighbour list)", W->rcut,
@@ -301,34 +311,54 @@ extern "C" int32_t dpmd_system_set_dplr(dpmd_system_t sys, dpmd_model_t wc_model
     if (grid[d] < 2 || grid[d] > 4096) { set_error("PPPM grid size %d outside [2, 4096]", grid[d]); return DPMD_E_ARG; }
   DPMD_TRY(atomic_setup(s, W));
   const int NA = s->NA, nwc = s->n_wc, nq = NA + nwc;
-  // charge blocks per segment: the atoms, then the centroids in the order atomic_setup gives
-  // them (segment by segment, sorted atom order)
+  // charge blocks per segment: the primary cell's atoms, then its centroids, in the order
+  // atomic_setup gives them (segment by segment, sorted atom order).  A segment holding nimg
+  // images lists each type's atoms image-major, so the first n_t / nimg of type t are the
+  // primary cell; every image reads its primary's charge (encoded -(q + 1)).
   std::vector<int> atom_q(NA), wc_q(std::max(nwc, 1)), wc_seg(s->nseg + 1);
-  std::vector<double> q(nq);
+  std::vector<double> q;
   s->q_base.assign(s->nseg, 0);
   s->q_n.assign(s->nseg, 0);
   int k = 0, qi = 0;
   for (int r = 0; r < s->nseg; ++r) {
     const SegInfo& S = s->seg[r];
+    const int ncell = S.natoms / S.nimg;
     wc_seg[r] = k;
     s->q_base[r] = qi;
+    // primary atom and centroid charges of the cell
+    std::vector<int> prim_q(S.natoms), prim_wc(S.natoms, -1);
+    int nw = 0;
     for (int c = 0; c < S.natoms; ++c) {
-      const int g = S.base + c, t = s->type_sorted[g];
-      atom_q[g] = qi + c;
-      q[qi + c] = q_atoms[t];
+      const int t = s->type_sorted[S.base + c];
+      const int nt_ = (S.off[t + 1] - S.off[t]) / S.nimg;
+      const int local = c - S.off[t];
+      if (local < nt_) prim_q[c] = qi + S.off[t] / S.nimg + local;
     }
     for (int c = 0; c < S.natoms; ++c) {
       const int t = s->type_sorted[S.base + c];
+      const int nt_ = (S.off[t + 1] - S.off[t]) / S.nimg;
+      if (c - S.off[t] < nt_ && (W->nsel_mask >> t & 1)) prim_wc[c] = qi + ncell + nw++;
+    }
+    q.resize((size_t)qi + ncell + nw);
+    for (int c = 0; c < S.natoms; ++c) {
+      const int g = S.base + c, t = s->type_sorted[g];
+      const int nt_ = (S.off[t + 1] - S.off[t]) / S.nimg;
+      const int local = c - S.off[t];
+      const int pc = S.off[t] + local % nt_;  // primary copy
+      const bool own = local < nt_;
+      atom_q[g] = own ? prim_q[c] : -prim_q[pc] - 1;
+      if (own) q[prim_q[c]] = q_atoms[t];
       if (!(W->nsel_mask >> t & 1)) continue;
-      wc_q[k] = qi + S.natoms + (k - wc_seg[r]);
-      q[wc_q[k]] = q_wc[t];
+      wc_q[k] = own ? prim_wc[c] : -prim_wc[pc] - 1;
+      if (own) q[prim_wc[c]] = q_wc[t];
       ++k;
     }
-    s->q_n[r] = S.natoms + (k - wc_seg[r]);
+    s->q_n[r] = ncell + nw;
     qi += s->q_n[r];
   }
   wc_seg[s->nseg] = k;
-  if (k != nwc || qi != nq) { set_error("DPLR charge layout mismatch"); return DPMD_E_STATE; }
+  const int nqc = qi;  // charges (primary cells only)
+  if (k != nwc || nqc > nq) { set_error("DPLR charge layout mismatch"); return DPMD_E_STATE; }
   DPMD_TRY(sys_ensure(s, (void**)&s->d_qxyz, sizeof(double) * 3 * nq));
   DPMD_TRY(sys_ensure(s, (void**)&s->d_q, sizeof(double) * nq));
   DPMD_TRY(sys_ensure(s, (void**)&s->d_qf, sizeof(double) * 3 * nq));
@@ -339,7 +369,7 @@ extern "C" int32_t dpmd_system_set_dplr(dpmd_system_t sys, dpmd_model_t wc_model
   DPMD_TRY(sys_ensure(s, (void**)&s->d_wc_q, sizeof(int) * wc_q.size()));
   DPMD_TRY(sys_ensure(s, (void**)&s->d_wc_seg, sizeof(int) * wc_seg.size()));
   const hipStream_t st = s->stream;
-  DPMD_HIP(hipMemcpyAsync(s->d_q, q.data(), sizeof(double) * nq, hipMemcpyHostToDevice, st));
+  DPMD_HIP(hipMemcpyAsync(s->d_q, q.data(), sizeof(double) * nqc, hipMemcpyHostToDevice, st));
   DPMD_HIP(hipMemcpyAsync(s->d_atom_q, atom_q.data(), sizeof(int) * NA, hipMemcpyHostToDevice, st));
   DPMD_HIP(hipMemcpyAsync(s->d_wc_q, wc_q.data(), sizeof(int) * wc_q.size(), hipMemcpyHostToDevice, st));
   DPMD_HIP(hipMemcpyAsync(s->d_wc_seg, wc_seg.data(), sizeof(int) * wc_seg.size(), hipMemcpyHostToDevice, st));
@@ -440,6 +470,6 @@ extern "C" int32_t dpmd_dplr_get(dpmd_system_t sys, double* e_lr, double* centro
     for (int r = 0; r < s->nseg; ++r) e_lr[r] = lr[(size_t)r * kLrStride];
   if (centroids)
     for (int k = 0; k < nwc; ++k)
-      for (int d = 0; d < 3; ++d) centroids[(size_t)s->wc_perm[k] * 3 + d] = c[(size_t)wq[k] * 3 + d];
+      for (int d = 0; d < 3; ++d) centroids[(size_t)s->wc_perm[k] * 3 + d] = c[(size_t)qdec(wq[k]) * 3 + d];
   return DPMD_OK;
 }

@@ -157,6 +157,10 @@ struct SegInfo {
   int tri;
   float H[9], Hi[9];
   double Hd[9], Hid[9];
+  // images per axis of a supercell segment (0 when only their count is known) and the
+  // primary cell's lengths (box / images), which the DPLR long-range term reads
+  int kimg[3];
+  double cellbox[3];
 };
 
 // s = x Hi and x = s H for row-vector coordinates and a row-major 3 x 3 matrix.

@@ -66,6 +66,7 @@ def lib():
         'dpmd_system_set_boxes': (c_i32, [c_vp, c_vp]),
         'dpmd_system_set_cells': (c_i32, [c_vp, c_vp]),
         'dpmd_system_set_images': (c_i32, [c_vp, c_vp]),
+        'dpmd_system_set_image_grid': (c_i32, [c_vp, c_vp]),
         'dpmd_system_set_rng': (c_i32, [c_vp, c_vp, c_vp]),
         'dpmd_system_get_rng': (c_i32, [c_vp, c_vp, c_vp]),
         'dpmd_system_set_positions': (c_i32, [c_vp, c_vp]),
@@ -126,7 +127,8 @@ def exported_symbols():
     lib()
     return ['dpmd_last_error', 'dpmd_version', 'dpmd_model_create', 'dpmd_model_destroy', 'dpmd_system_create',
             'dpmd_system_create_ragged', 'dpmd_system_reshape', 'dpmd_system_destroy', 'dpmd_system_set_box',
-            'dpmd_system_set_boxes', 'dpmd_system_set_cells', 'dpmd_system_set_images', 'dpmd_system_set_rng', 'dpmd_system_get_rng',
+            'dpmd_system_set_boxes', 'dpmd_system_set_cells', 'dpmd_system_set_images',
+            'dpmd_system_set_image_grid', 'dpmd_system_set_rng', 'dpmd_system_get_rng',
             'dpmd_system_set_positions', 'dpmd_system_get_positions',
             'dpmd_system_set_velocities', 'dpmd_system_get_velocities', 'dpmd_system_set_masses',
             'dpmd_nlist_allocate', 'dpmd_nlist_update', 'dpmd_nlist_export', 'dpmd_nlist_capacity', 'dpmd_compute',
@@ -347,6 +349,11 @@ class DeviceSystem:
     def set_images(self, nimg):
         n = np.ascontiguousarray(np.asarray(nimg, np.int32).reshape(self.nrep))
         check(lib().dpmd_system_set_images(self.handle, ptr(n)))
+
+    def set_image_grid(self, k):
+        """Images per axis (3,) or (nrep, 3) of supercell replicas."""
+        kk = np.ascontiguousarray(np.broadcast_to(np.asarray(k, np.int32).reshape(-1, 3), (self.nrep, 3)), np.int32)
+        check(lib().dpmd_system_set_image_grid(self.handle, ptr(kk)))
 
     def set_rng(self, seeds=None, steps=None):
         sd = None if seeds is None else np.ascontiguousarray(np.asarray(seeds, np.uint64).reshape(self.nrep))

@@ -120,17 +120,24 @@ class Evaluator:
         if self.dplr is not None:
             if box3.shape != (3,):
                 raise ValueError('For "dplr" currently only orthorhombic boxes are supported.')
-            if nk > 1:
-                raise NotImplementedError('DPLR evaluation needs box lengths >= 2*rcut (%.3f)' % (2 * self.rcut))
-            # the frames of a batch as segments of one system, each with its own PPPM
+            # the frames of a batch as segments of one system, each with its own PPPM on the
+            # cell (a narrow box as its supercell of images: the short-range model sees the
+            # images, the long-range term the cell)
             wdm, qa, qw, beta, res = self.dplr
-            sys = self._system(type_idx, F)
-            sys.set_box(box3)
-            sys.set_positions(coords)
+            if nk > 1:
+                cs = [replicate(c, box3, type_idx, k) for c in coords]
+                sc, sbox, stype = np.stack([c[0] for c in cs]), cs[0][1], cs[0][2]
+            else:
+                sc, sbox, stype = coords, box3, type_idx
+            sys = self._system(stype, F)
+            sys.set_box(sbox)
+            if nk > 1:
+                sys.set_image_grid(k)
+            sys.set_positions(sc)
             sys.nlist_allocate(self.rcut, 1.2)
             sys.set_dplr(wdm, qa, qw, beta, dplr_grid(box3, beta, res))
             e, f, v, _ = sys.compute(precision)
-            return e, f, v
+            return e, f[:, :N], v
         if nk > 1:
             cs = [replicate(c, box3, type_idx, k) for c in coords]
             sc = np.stack([c[0] for c in cs])

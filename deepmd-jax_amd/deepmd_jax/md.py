@@ -18,7 +18,8 @@ on the device (deepmd_jax/md.py:221-234,786-790, deepmd_jax/routine.py:110-136).
 XYZ dumps (run(dump_prefix=...)) stream from a writer thread (deepmd_jax/dumps.py).
 Model deviation (model_deviation_paths) evaluates each listed model on its own device
 system at report time.  DPLR models (type 'dplr') add the Wannier-centroid PPPM term on the
-device, with its virial, per bead under PIMD (orthorhombic boxes >= 2 rcut).
+device, with its virial, per bead under PIMD (orthorhombic boxes; below 2 rcut on the
+supercell of images with the long-range term of the cell).
 """
 import time
 
@@ -84,7 +85,7 @@ class _ImageSystem:
         self.dev = _lib.DeviceSystem(model, types, nrep)
         self.dev.set_box(sbox)
         if self.nimg > 1:
-            self.dev.set_images(np.full(nrep, self.nimg))
+            self.dev.set_image_grid(self.k)
 
     def _up(self, x):  # (P, N, 3) -> (P, nimg N, 3)
         x = np.asarray(x, np.float64)
@@ -287,8 +288,6 @@ class Simulation:
         if self._dplr is not None:
             if self._general_cell:
                 raise NotImplementedError('dplr model currently only supports orthorhombic box')
-            if self._sys.nimg > 1:
-                raise NotImplementedError('DPLR needs box lengths >= 2*rcut (%.3f) on this engine.' % (2 * self._rcut))
         # mobile-atom mask (deepmd_jax/md.py:220-226); the device zeroes the forces on fixed
         # atoms, keeps their positions and masks their momenta after each NVE / NVT step
         self._mobile = None

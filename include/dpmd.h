@@ -165,6 +165,10 @@ int32_t dpmd_system_set_cells(dpmd_system_t sys, const double* cells);
  * kinetic energies and thermostat work are then reported per image and all images draw
  * the same thermostat noise. */
 int32_t dpmd_system_set_images(dpmd_system_t sys, const int32_t* nimg);
+/* Per-segment images per axis (nseg, 3): dpmd_system_set_images with their product, and the
+ * primary cell (box / images per axis) on which a DPLR term of the segment evaluates its
+ * long-range energy (the reference's lattice path keeps PPPM on the cell, deepmd_jax/md.py:404-415). */
+int32_t dpmd_system_set_image_grid(dpmd_system_t sys, const int32_t* k3);
 /* Per-segment stochastic stream (nseg each, either may be NULL): the Langevin normals of
  * a segment depend only on (params.seed, seed, step, atom), so a segment draws the same
  * stream alone or in any batch.  steps counts integrator steps already taken. */

@@ -305,3 +305,60 @@ def test_dplr_evaluate_batches_and_pimd_simulation(model0, tmp_path):
     inv = log[:, 4]
     assert np.isfinite(log).all()
     assert np.abs(inv - inv[0]).max() / len(types) < 1e-4, inv
+
+
+def test_dplr_small_box_on_images(model0):
+    """32 H2O (L = 9.86 A < 2 rcut): the short-range and centroid models see the 2x2x2 supercell
+    of images, the PPPM term the cell (the reference's lattice path, deepmd_jax/utils.py:66-78,
+    with PPPM on the cell).  Energy, forces and the virial diagonal against the oracle's lattice
+    form; NVE on the images conserves KE + PE."""
+    import contextlib
+    import io
+    from deepmd_jax import _lib, synthetic as syn
+    from deepmd_jax.evaluator import Evaluator
+    print('case dplr images', flush=True)
+    params, var = model0
+    wp, wv = syn.init_wannier_model(4)
+    coord, box, types = syn.water_box(32, seed=66)
+    assert box.min() < 2 * params['rcut']
+    M = O.p3m_grid(box, BETA, 5)
+    lat = O.lattice_candidate(np.diag(box), params['rcut'])
+
+    def oracle(c, b):
+        la = O.lattice_candidate(np.diag(b), params['rcut'])
+        return O.dplr_energy_force(params, var, wp, wv, c, b, types, Q_ATOMS, Q_WC[:1], BETA, M, lattice=la)
+    ev = Evaluator(params, var)
+    ev.dplr = (_lib.DeviceModel(wp, wv), np.array(Q_ATOMS), np.array(Q_WC), BETA, 5.0)
+    e, f, v = ev.evaluate_batch(coord[None], box, types)
+    E, Elr, F, _ = O.dplr_energy_force(params, var, wp, wv, coord, box, types, Q_ATOMS, Q_WC[:1], BETA, M,
+                                       lattice=lat)
+    N = len(types)
+    assert abs(e[0] - E) / N <= 1e-5, (e[0], E)
+    assert np.abs(f[0] - F).max() <= 1e-4 * np.abs(F).max()
+    h = 1e-5
+    for a in range(3):
+        sc = np.ones(3)
+        sc[a] = 1 + h
+        ep = oracle(coord * sc, box * sc)[0]
+        sc[a] = 1 - h
+        em = oracle(coord * sc, box * sc)[0]
+        assert abs(v[0][a, a] + (ep - em) / (2 * h)) <= 1e-4 * np.abs(v[0]).max(), (a, v[0][a, a], (ep - em) / (2 * h))
+    # NVE with the images moving together
+    from deepmd_jax.dpmodel import DPModel
+    from deepmd_jax.md import Simulation
+    from deepmd_jax.utils import save_model
+    import tempfile
+    import os
+    dparams = dict(params)
+    dparams.update({'type': 'dplr', 'dplr_wannier_model_and_variables': (DPModel(wp), {'params': wv['params']}),
+                    'dplr_q_atoms': Q_ATOMS, 'dplr_q_wc': Q_WC[:1], 'dplr_beta': BETA, 'dplr_resolution': 5})
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, 'dplr.pkl')
+        with contextlib.redirect_stdout(io.StringIO()):
+            save_model(path, DPModel(dparams), var)
+            sim = Simulation(path, box, types, syn.MASSES, 'NVE', 0.05, coord, report_interval=10,
+                             initial_velocity=syn.maxwell_boltzmann(types, 300.0, seed=66))
+            sim.run(40)
+    inv = np.array(sim.log)[:, 4]
+    assert np.isfinite(inv).all()
+    assert np.abs(inv - inv[0]).max() / N < 2e-5, inv
