@@ -109,44 +109,55 @@ class Evaluator:
             self._systems[key] = s
         return s
 
+    def batch_key(self, box):
+        """Frames with equal keys can share one batch (one launch, a box per frame): the same
+        images per axis and, for DPLR, the same PPPM grid."""
+        b = canonical_box(box)
+        k = tuple(int(x) for x in supercell_factors(b, self.rcut))
+        if self.dplr is not None and b.shape == (3,):
+            _, _, _, beta, res = self.dplr
+            return k, tuple(int(x) for x in dplr_grid(b, beta, res))
+        return k
+
+    def _load(self, coords, boxes, type_idx):
+        """A system holding the frames (supercells of images for narrow boxes, all with the
+        images per axis of the first frame), a cell per frame, positions set."""
+        coords = np.asarray(coords, np.float64)
+        boxes = [canonical_box(b) for b in boxes]
+        type_idx = np.asarray(type_idx)
+        k = supercell_factors(boxes[0], self.rcut)
+        nk = int(np.prod(k))
+        if nk > 1:
+            cs = [replicate(c, b, type_idx, k) for c, b in zip(coords, boxes)]
+            sc, sboxes, stype = np.stack([c[0] for c in cs]), [c[1] for c in cs], cs[0][2]
+        else:
+            sc, sboxes, stype = coords, boxes, type_idx
+        sys = self._system(stype, len(coords))
+        sys.set_cells(np.stack([cell_matrix(b) for b in sboxes]))
+        sys.set_positions(sc)
+        return sys, boxes, k, nk
+
     def evaluate_batch(self, coords, box3, type_idx, precision='fp32'):
         """coords (F,N,3) sharing one box -> (E (F,), F (F,N,3), virial (F,3,3))."""
-        coords = np.asarray(coords, np.float64)
-        box3 = canonical_box(box3)
-        type_idx = np.asarray(type_idx)
-        k = supercell_factors(box3, self.rcut)
-        nk = int(np.prod(k))
-        F, N = coords.shape[:2]
+        return self.evaluate_frames(coords, [box3] * len(coords), type_idx, precision)
+
+    def evaluate_frames(self, coords, boxes, type_idx, precision='fp32'):
+        """coords (F,N,3), one box per frame (frames of one batch_key) -> (E (F,), F (F,N,3),
+        virial (F,3,3)); the frames run as the segments of one system."""
+        N = np.asarray(coords).shape[1]
+        sys, boxes, k, nk = self._load(coords, boxes, type_idx)
         if self.dplr is not None:
-            if box3.shape != (3,):
+            if boxes[0].shape != (3,):
                 raise ValueError('For "dplr" currently only orthorhombic boxes are supported.')
-            # the frames of a batch as segments of one system, each with its own PPPM on the
-            # cell (a narrow box as its supercell of images: the short-range model sees the
-            # images, the long-range term the cell)
+            # each frame's PPPM on its cell (a narrow box as its supercell of images: the
+            # short-range model sees the images, the long-range term the cell)
             wdm, qa, qw, beta, res = self.dplr
             if nk > 1:
-                cs = [replicate(c, box3, type_idx, k) for c in coords]
-                sc, sbox, stype = np.stack([c[0] for c in cs]), cs[0][1], cs[0][2]
-            else:
-                sc, sbox, stype = coords, box3, type_idx
-            sys = self._system(stype, F)
-            sys.set_box(sbox)
-            if nk > 1:
                 sys.set_image_grid(k)
-            sys.set_positions(sc)
             sys.nlist_allocate(self.rcut, 1.2)
-            sys.set_dplr(wdm, qa, qw, beta, dplr_grid(box3, beta, res))
+            sys.set_dplr(wdm, qa, qw, beta, dplr_grid(boxes[0], beta, res))
             e, f, v, _ = sys.compute(precision)
             return e, f[:, :N], v
-        if nk > 1:
-            cs = [replicate(c, box3, type_idx, k) for c in coords]
-            sc = np.stack([c[0] for c in cs])
-            sbox, stype = cs[0][1], cs[0][2]
-        else:
-            sc, sbox, stype = coords, box3, type_idx
-        sys = self._system(stype, F)
-        sys.set_box(sbox)
-        sys.set_positions(sc)
         sys.nlist_allocate(self.rcut, 1.2)
         e, f, v, _ = sys.compute(precision)
         return e / nk, f[:, :N], v / nk
@@ -154,23 +165,14 @@ class Evaluator:
     def evaluate_atomic_batch(self, coords, box3, type_idx):
         """Atomic model outputs of frames sharing one box: (F, M, 3) vectors or (F, M, 9) tensors for
         the M atoms of the selected types, in caller order."""
-        coords = np.asarray(coords, np.float64)
-        box3 = canonical_box(box3)
-        type_idx = np.asarray(type_idx)
-        k = supercell_factors(box3, self.rcut)
-        nk = int(np.prod(k))
-        F = coords.shape[0]
-        if nk > 1:
-            cs = [replicate(c, box3, type_idx, k) for c in coords]
-            sc, sbox, stype = np.stack([c[0] for c in cs]), cs[0][1], cs[0][2]
-        else:
-            sc, sbox, stype = coords, box3, type_idx
-        sys = self._system(stype, F)
-        sys.set_box(sbox)
-        sys.set_positions(sc)
+        return self.evaluate_atomic_frames(coords, [box3] * len(coords), type_idx)
+
+    def evaluate_atomic_frames(self, coords, boxes, type_idx):
+        """evaluate_atomic_batch with one box per frame (frames of one batch_key)."""
+        sys, _, _, _ = self._load(coords, boxes, type_idx)
         sys.nlist_allocate(self.rcut, 1.2)
         sel = [int(t) for t in self.params['nsel']]
-        m = int(np.isin(type_idx, sel).sum())
+        m = int(np.isin(np.asarray(type_idx), sel).sum())
         return sys.atomic_predict()[:, :m]  # image 0 holds the original atoms
 
     def evaluate_frame(self, coord, box3, type_idx, precision='fp32'):

@@ -57,23 +57,26 @@ def evaluate(model_path, coord, box, type_idx, batch_size=1, precision='fp32'):
         # atomic outputs per frame (deepmd_jax/train.py:765-788): (F, M, 3) or (F, M, 9)
         key = params.get('atomic_data_prefix') or ('atomic_dipole' if kind == 'atomic' else 'atomic_polarizability')
         outs = []
-        i = 0
-        while i < F:
-            j = i + 1
-            while j < F and j - i < batch_size and np.array_equal(boxes[j], boxes[i]):
-                j += 1
-            outs.append(ev.evaluate_atomic_batch(coord[i:j], boxes[i], type_idx))
-            i = j
+        for i, j in _batches(ev, boxes, batch_size):
+            outs.append(ev.evaluate_atomic_frames(coord[i:j], boxes[i:j], type_idx))
         return {key: np.concatenate(outs, 0)}
     energy = np.zeros(F)
     force = np.zeros(coord.shape)
-    i = 0
-    while i < F:
-        j = i + 1
-        while j < F and j - i < batch_size and np.array_equal(boxes[j], boxes[i]):
-            j += 1
-        e, f, _ = ev.evaluate_batch(coord[i:j], boxes[i], type_idx, precision)
+    for i, j in _batches(ev, boxes, batch_size):
+        e, f, _ = ev.evaluate_frames(coord[i:j], boxes[i:j], type_idx, precision)
         energy[i:j] = e
         force[i:j] = f
-        i = j
     return {'energy': energy, 'force': force}
+
+
+def _batches(ev, boxes, batch_size):
+    """Runs of at most batch_size consecutive frames with one batch key (the same images per
+    axis and DPLR grid); each frame keeps its own box."""
+    keys = [ev.batch_key(b) for b in boxes]
+    i, F = 0, len(boxes)
+    while i < F:
+        j = i + 1
+        while j < F and j - i < batch_size and keys[j] == keys[i]:
+            j += 1
+        yield i, j
+        i = j

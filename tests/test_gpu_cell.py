@@ -210,3 +210,29 @@ def test_ase_stress_general_cell(device_model, model0, nmol):
     assert abs(e - E.item()) / len(types) <= 1e-5
     assert np.abs(f - F.numpy()).max() <= 1e-4 * max(1.0, np.abs(F.numpy()).max())
     assert np.abs(s - ref).max() <= 1e-4 * np.abs(ref).max(), (s, ref)
+
+
+def test_evaluate_frames_with_their_own_boxes(model_pkl, model0):
+    """evaluate() batches consecutive frames with different boxes (same images per axis) as
+    the segments of one system, each with its own cell: three scaled orthorhombic frames and
+    two general cells, against the oracle frame by frame."""
+    from deepmd_jax import synthetic as syn
+    from deepmd_jax.train import evaluate
+    print('case cell frames', flush=True)
+    params, var = model0
+    c0, box, types = syn.water_box(64, seed=29)
+    coords, boxes = [], []
+    for s in (1.0, 1.01, 0.995):
+        coords.append(c0 * s)
+        boxes.append(np.diag(box * s))
+    for tilt in ((0.1, 0.0, -0.05), (0.0, 0.12, 0.07)):
+        c, H = skewed(c0 * 1.3, box * 1.3, tilt)
+        coords.append(c)
+        boxes.append(H)
+    coords, boxes = np.stack(coords), np.stack(boxes)
+    with contextlib.redirect_stdout(io.StringIO()):
+        out = evaluate(model_pkl, coords, boxes, types, batch_size=4)
+    for k in range(len(coords)):
+        E, F, _ = O.energy_force_virial(params, var, coords[k], boxes[k], types, lattice='auto')
+        assert abs(out['energy'][k] - E.item()) / len(types) <= 1e-5, k
+        assert np.abs(out['force'][k] - F.numpy()).max() <= 1e-4 * max(1.0, np.abs(F.numpy()).max()), k
