@@ -192,3 +192,67 @@ def tabulated_jet(cv, x):
     for i in range(4):
         d2 = d2 * x0 + (5 - i) * (4 - i) * cc[:, i]
     return v, d1, d2
+
+
+def get_p3mlr_grid_size(box3, beta, resolution=5):
+    """PPPM grid of a box, (box * beta * resolution) truncated (deepmd_jax/utils.py:174-176)."""
+    return tuple(int(m) for m in (np.asarray(box3, np.float64) * beta * resolution).astype(int))
+
+
+def get_p3mlr_fn(box3_ref, beta, M=None, resolution=5, device=0):
+    """PPPM long-range energy on the GPU (deepmd_jax/utils.py:178-222): returns
+    p3mlr_fn(coord_N3, q_N, box3) -> energy (eV) for charges (e) at coord (A) in the orthorhombic
+    box3, TSC assignment on the grid frozen from box3_ref (or M).  The reference differentiates
+    its function with jax; here p3mlr_fn.energy_force_virial(coord_N3, q_N, box3) returns the
+    energy, -dE/dx (N, 3) and the virial -dE/dstrain (3, 3) directly."""
+    from ._lib import pppm_compute
+    grid = get_p3mlr_grid_size(box3_ref, beta, resolution) if M is None else tuple(int(m) for m in M)
+
+    def energy_force_virial(coord_N3, q_N, box3):
+        return pppm_compute(box3, beta, grid, coord_N3, q_N, device)
+
+    def p3mlr_fn(coord_N3, q_N, box3):
+        return energy_force_virial(coord_N3, q_N, box3)[0]
+
+    p3mlr_fn.energy_force_virial = energy_force_virial
+    p3mlr_fn.grid = grid
+    return p3mlr_fn
+
+
+def periodic_replicate(n_copy, coord, box, type_idx=None, force=None, velocity=None):
+    """Replicate a periodic system n_copy times per direction (deepmd_jax/utils.py:299-327):
+    returns [coord, box] plus type_idx, force, velocity when given.  As there, copy i along
+    direction d is shifted by i times column d of the (3,3) box (the box lengths of an
+    orthorhombic one), directions nest with the last outermost, and the returned box is
+    diag(n_copy) @ box (or n_copy * box for lengths)."""
+    n_copy = (np.asarray(n_copy) * np.ones(3)).astype(int)
+    box = np.array(box, np.float64)
+    coord = np.asarray(coord, np.float64)
+    if box.size <= 3:
+        return_box = n_copy * box
+        cell = np.diag(box * np.ones(3))
+    else:
+        cell = box.reshape(3, 3)
+        return_box = np.diag(n_copy) @ cell
+    for d in range(3):
+        coord = np.concatenate([(coord + i * cell[:, d])[None] for i in range(n_copy[d])]).reshape(-1, 3)
+    ret = [coord, return_box]
+    n = int(n_copy.prod())
+    if type_idx is not None:
+        ret.append(np.tile(type_idx, n))
+    if force is not None:
+        ret.append(np.concatenate([force] * n))
+    if velocity is not None:
+        ret.append(np.concatenate([velocity] * n))
+    return ret
+
+
+def dplr_charges(type_idx, q_atoms, q_wc, nsel, ntypes):
+    """Per-atom charges (N,) and per-centroid charges (M,) in caller order over the atoms of the
+    nsel types (deepmd_jax/utils.py:355-366)."""
+    type_idx = np.asarray(type_idx)
+    nsel = np.asarray(list(nsel))
+    pos = np.full(ntypes, -1, dtype=int)
+    pos[nsel] = np.arange(len(nsel))
+    sel = np.isin(type_idx, nsel)
+    return np.asarray(q_atoms)[type_idx], np.asarray(q_wc)[pos[type_idx[sel]]]

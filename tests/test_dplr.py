@@ -109,3 +109,33 @@ def test_tensor_model_is_a_symmetric_tensor():
     Q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
     a1 = f(coord @ Q.T).reshape(-1, 3, 3)
     np.testing.assert_allclose(a1, Q @ a0 @ Q.T, atol=1e-10)
+
+
+def test_p3mlr_grid_size_and_host_helpers():
+    """get_p3mlr_grid_size (truncation of box * beta * resolution), dplr_charges and
+    periodic_replicate (deepmd_jax/utils.py:174-176,299-327,355-366) on closed cases."""
+    from deepmd_jax.evaluator import replicate
+    from deepmd_jax.utils import dplr_charges, get_p3mlr_grid_size, periodic_replicate
+    assert get_p3mlr_grid_size(np.array([12.42, 9.99, 20.0]), 0.4, 5) == (24, 19, 40)
+    qa, qw = dplr_charges([1, 0, 1, 0, 0], [6.0, 1.0], [-8.0], [0], 2)
+    np.testing.assert_array_equal(qa, [1, 6, 1, 6, 6])
+    np.testing.assert_array_equal(qw, [-8, -8, -8])
+    rng = np.random.default_rng(0)
+    c = rng.uniform(0, 5, (4, 3))
+    t = np.array([0, 1, 1, 0])
+    H = np.array([[5.0, 0, 0], [1.0, 6.0, 0], [0.5, -1.0, 7.0]])
+    for box in (np.array([5.0, 6.0, 7.0]), H):
+        out_c, out_b, out_t, out_f = periodic_replicate([2, 1, 3], c, box, t, force=np.ones((4, 3)))
+        assert out_c.shape == (24, 3) and out_f.shape == (24, 3)
+        np.testing.assert_array_equal(out_t, np.tile(t, 6))
+        cell = np.diag(box) if box.shape == (3,) else box
+        # every block of 4 atoms is the cell shifted by i0 col0 + i1 col1 + i2 col2, last direction outermost
+        for i2 in range(3):
+            for i0 in range(2):
+                blk = out_c[(i2 * 2 + i0) * 4:(i2 * 2 + i0 + 1) * 4]
+                np.testing.assert_allclose(blk, c + i0 * cell[:, 0] + i2 * cell[:, 2])
+        np.testing.assert_allclose(out_b, np.array([2, 1, 3]) * box if box.shape == (3,) else np.diag([2, 1, 3]) @ H)
+    # orthorhombic: the same set of images as the evaluator's supercell
+    a = periodic_replicate([2, 2, 1], c, np.array([5.0, 6.0, 7.0]))[0]
+    b = replicate(c, np.array([5.0, 6.0, 7.0]), t, np.array([2, 2, 1]))[0]
+    np.testing.assert_allclose(np.sort(a, axis=0), np.sort(b, axis=0))

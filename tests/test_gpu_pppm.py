@@ -38,3 +38,24 @@ def test_pppm_against_oracle(L, res, n, shift):
         em = O.p3m_energy(torch.as_tensor(x * s), q, L * s, 0.4, M).item()
         assert abs(w[a, a] + (ep - em) / (2 * h)) <= 1e-7 * max(1.0, abs(E.item())), (a, w[a, a], (ep - em) / (2 * h))
     np.testing.assert_allclose(w, w.T, atol=1e-12 * abs(E.item()))
+
+
+def test_get_p3mlr_fn_matches_oracle():
+    """deepmd_jax.utils.get_p3mlr_fn (deepmd_jax/utils.py:178-222) on the GPU: energy of random
+    neutral charges against the oracle's PPPM, forces and virial from energy_force_virial."""
+    from deepmd_jax.utils import get_p3mlr_fn
+    print('case get_p3mlr_fn', flush=True)
+    rng = np.random.default_rng(7)
+    box = np.array([11.0, 12.5, 10.2])
+    x = rng.uniform(0, 1, (60, 3)) * box
+    q = rng.uniform(-1, 1, 60)
+    q -= q.mean()
+    fn = get_p3mlr_fn(box, 0.4)
+    M = O.p3m_grid(box, 0.4, 5)
+    assert tuple(fn.grid) == tuple(int(m) for m in M)
+    e_ref = O.p3m_energy(torch.as_tensor(x), q, box, 0.4, M).item()
+    assert abs(fn(x, q, box) - e_ref) <= 1e-10 * max(1.0, abs(e_ref))
+    e, f, w = fn.energy_force_virial(x, q, box)
+    xt = torch.as_tensor(x).clone().requires_grad_(True)
+    g, = torch.autograd.grad(O.p3m_energy(xt, q, box, 0.4, M), [xt])
+    np.testing.assert_allclose(f, -g.numpy(), rtol=0, atol=1e-9 * np.abs(g.numpy()).max())
