@@ -168,3 +168,35 @@ def test_serving_md_images_stay_together(device_model):
     if full is not None:
         d = full - np.tile(full[:N], (8, 1)) - sh
         assert np.abs(d).max() <= 1e-4
+
+
+def test_serving_general_cells(device_model, model0):
+    """Evaluation requests with (3,3) cells, one wide and one narrow (held as a supercell of
+    images), served together with an orthorhombic one: energy, forces and virial against the
+    oracle's lattice form at the plain FP32 tolerances."""
+    from deepmd_jax import synthetic as syn
+    from deepmd_jax.serving import Request, Server
+    print('case serving cells', flush=True)
+    params, var = model0
+    reqs = []
+    for i, (n, tilt) in enumerate(((128, (0.2, -0.1, 0.15)), (24, (0.1, 0.05, -0.2)), (64, None))):
+        c, b, t = syn.water_box(n, seed=70 + i)
+        if tilt is not None:
+            H = np.array([[b[0], 0, 0], [tilt[0] * b[1], b[1], 0], [tilt[1] * b[2], tilt[2] * b[2], b[2]]])
+            c, b = (c / b) @ H, H
+        reqs.append(Request(rid=i, kind='eval', coord=c, box=b, type_idx=t, arrival=0.0))
+    srv = Server(device_model)
+    try:
+        for r in reqs:
+            srv.submit(r)
+        assert srv.wait(len(reqs), timeout=120)
+    finally:
+        srv.close()
+    for r in reqs:
+        res = srv.results[r.rid]
+        assert res.error == 0
+        E, F, W = O.energy_force_virial(params, var, r.coord, r.box, r.type_idx, lattice='auto')
+        N = len(r.type_idx)
+        assert abs(res.energy - E.item()) / N <= 1e-5, r.rid
+        assert np.abs(res.force - F.numpy()).max() <= 1e-4 * max(1.0, np.abs(F.numpy()).max()), r.rid
+        assert np.abs(res.virial - W.numpy()).max() <= 1e-4 * np.abs(W.numpy()).max(), r.rid
