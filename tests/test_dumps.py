@@ -27,6 +27,11 @@ def test_frame_text():
     assert 'step=' not in t
     assert t.splitlines()[2] == 'O  1.500000000000000 2.250000000000000 3.000000000000000'
     assert t.splitlines()[3].startswith('H  0.100000001490116 -0.200000002980232')
+    # a general (3,3) cell is written row by row (deepmd_jax/md_utils.py:236-239)
+    H = np.array([[10, 0, 0], [2.5, 11, 0], [-1, 3, 12]], np.float32)
+    t = D.format_frame(['O', 'H'], x, H, 5)
+    assert t.splitlines()[1].startswith('Lattice="10.00000000 0.00000000 0.00000000 2.50000000 11.00000000 '
+                                        '0.00000000 -1.00000000 3.00000000 12.00000000" ')
 
 
 def test_content_rules():
