@@ -563,10 +563,14 @@ def main():
         done += k
     sys_.synchronize()
 
+    # chunks of the reference's run protocol (deepmd_jax/md.py:874-916): max(10, min(100, 100000 // N))
+    # steps per device call; an overflowing chunk is re-run after the reallocation
+    chunk = max(10, min(100, 100000 // N))
+
     def run_steps(n):
         done, realloc = 0, 0
         while done < n:
-            k = min(100, n - done)
+            k = min(chunk, n - done)
             _, _, err = sys_.md_run(p, k, record=False)
             if err & 16:
                 raise RuntimeError('NaN/Inf during the benchmark')
