@@ -1038,6 +1038,14 @@ int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps, 
     return DPMD_E_UNSUPPORTED;
   }
   if (p->routine == DPMD_PIMD_PILE_L && !s->uniform) { set_error("PIMD needs identical beads"); return DPMD_E_STATE; }
+  if (p->routine == DPMD_PIMD_PILE_L && s->seg[0].tri) {
+    // the free-ring step unwraps every bead through bead 0's cell
+    for (const SegInfo& S : s->seg) {
+      bool same = S.tri != 0;
+      for (int k = 0; k < 9 && same; ++k) same = S.Hd[k] == s->seg[0].Hd[k];
+      if (!same) { set_error("PIMD beads must share one cell"); return DPMD_E_ARG; }
+    }
+  }
   const size_t NA = s->NA;
   const bool rec = traj_pos || traj_vel;
   if (rec) {
