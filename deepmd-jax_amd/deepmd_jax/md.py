@@ -314,6 +314,14 @@ class Simulation:
                 ds.set_fixed(~self._mobile)
             ds.set_positions(pos)
             ds.nlist_allocate(float(dmodel.params['rcut']) + self._neighbor_skin, self._neighbor_buffer_ratio)
+            # a DPLR deviation model carries its long-range term, as its energy function does in the
+            # reference (deepmd_jax/md.py:299-302,385-415)
+            dd = dplr_setup(dmodel.params, device)
+            if dd is not None:
+                if self._general_cell:
+                    raise NotImplementedError('dplr model currently only supports orthorhombic box')
+                wdm, qa, qw, beta, res = dd
+                ds.dev.set_dplr(wdm, qa, qw, beta, dplr_grid(self._box, beta, res))
             self._deviation.append((ds, float(dmodel.params['rcut'])))
         self._kT = (temperature if temperature is not None else 0.0) * TEMP_UNIT_CONVERSION
         self._params = _lib.MDParams()
