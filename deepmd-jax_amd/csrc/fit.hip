@@ -130,14 +130,20 @@ constexpr int LS32 = KC32 + 4;       // fp32 LDS row stride (floats)
 constexpr int KC16 = 32;             // 16-bit K slice
 constexpr int LS16 = KC16 + 8;       // 16-bit LDS row stride (elements)
 
-template <int NT, int EPI>
+// FP32 GEMM: block tile 64 rows x 16 NT columns, four waves of 16 rows each over all NT
+// column tiles; K staged in KC-wide slices, KC = 32 when K allows (half the block barriers of
+// KC = 16: fitting time -4 %; 64 for the narrow tiles measured slower), else 16.
+template <int NT, int EPI, int KC>
 __global__ void __launch_bounds__(256, 2) gemm_f32_kernel(const GemmArgs g) {
-  __shared__ float As[64 * LS32];
-  __shared__ float Bs[16 * NT * LS32];
+  constexpr int LS = KC + 4;                     // LDS row stride (floats)
+  __shared__ float As[64 * LS];
+  __shared__ float Bs[16 * NT * LS];
   constexpr int BN = 16 * NT;
-  constexpr int BV = BN * KC32 / 4;  // float4 per B slice
+  constexpr int C4 = KC / 4;                     // float4 per row of a slice
+  constexpr int NAP = 64 * C4 / 256;             // A float4 per thread
+  constexpr int BV = BN * C4;                    // float4 per B slice
   constexpr int NB = (BV + 255) / 256;
-  static_assert(NB <= 4, "B staging unrolled for at most 4 pieces per thread");
+  static_assert(NAP >= 1 && NAP <= 2 && NB <= 8, "staging unrolled for at most 2 A / 8 B pieces");
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
   const int q = lane >> 4, n = lane & 15;
@@ -147,43 +153,46 @@ __global__ void __launch_bounds__(256, 2) gemm_f32_kernel(const GemmArgs g) {
   floatx4 acc[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
-  const int arow = tid / 4, ac4 = tid % 4;
-  // staging registers are named vectors (an array of them was placed in scratch)
-  floatx4 ra, rb0, rb1, rb2, rb3;
-  auto bsrc = [&](int piece, int k) {
-    const int idx = tid + 256 * piece;
-    return reinterpret_cast<const floatx4*>(Bt + (size_t)(n0 + idx / 4) * g.ldb + k + 4 * (idx % 4));
-  };
-  auto bdst = [&](int piece) {
-    const int idx = tid + 256 * piece;
-    return reinterpret_cast<floatx4*>(&Bs[(idx / 4) * LS32 + 4 * (idx % 4)]);
-  };
+  floatx4 ra[NAP], rb[NB];
   auto live = [&](int piece) { return piece + 1 < NB || tid + 256 * piece < BV; };
   auto fetch = [&](int k) {
-    ra = *reinterpret_cast<const floatx4*>(g.A + (size_t)(m0 + arow) * g.lda + k + 4 * ac4);
-    if constexpr (NB > 0) { if (live(0)) rb0 = *bsrc(0, k); }
-    if constexpr (NB > 1) { if (live(1)) rb1 = *bsrc(1, k); }
-    if constexpr (NB > 2) { if (live(2)) rb2 = *bsrc(2, k); }
-    if constexpr (NB > 3) { if (live(3)) rb3 = *bsrc(3, k); }
+#pragma unroll
+    for (int p = 0; p < NAP; ++p) {
+      const int idx = tid + 256 * p;
+      ra[p] = *reinterpret_cast<const floatx4*>(g.A + (size_t)(m0 + idx / C4) * g.lda + k + 4 * (idx % C4));
+    }
+#pragma unroll
+    for (int p = 0; p < NB; ++p) {
+      const int idx = tid + 256 * p;
+      if (live(p)) rb[p] = *reinterpret_cast<const floatx4*>(Bt + (size_t)(n0 + idx / C4) * g.ldb + k + 4 * (idx % C4));
+    }
   };
   fetch(0);
-  for (int k0 = 0; k0 < g.K; k0 += KC32) {
+  for (int k0 = 0; k0 < g.K; k0 += KC) {
     __syncthreads();
-    *reinterpret_cast<floatx4*>(&As[arow * LS32 + 4 * ac4]) = ra;
-    if constexpr (NB > 0) { if (live(0)) *bdst(0) = rb0; }
-    if constexpr (NB > 1) { if (live(1)) *bdst(1) = rb1; }
-    if constexpr (NB > 2) { if (live(2)) *bdst(2) = rb2; }
-    if constexpr (NB > 3) { if (live(3)) *bdst(3) = rb3; }
-    __syncthreads();
-    if (k0 + KC32 < g.K) fetch(k0 + KC32);
-    const floatx4 av = *reinterpret_cast<const floatx4*>(&As[(16 * wv + n) * LS32 + 4 * q]);
 #pragma unroll
-    for (int nt = 0; nt < NT; ++nt) {
-      const floatx4 bv = *reinterpret_cast<const floatx4*>(&Bs[(16 * nt + n) * LS32 + 4 * q]);
-      acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, bv.x, acc[nt], 0, 0, 0);
-      acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, bv.y, acc[nt], 0, 0, 0);
-      acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.z, bv.z, acc[nt], 0, 0, 0);
-      acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.w, bv.w, acc[nt], 0, 0, 0);
+    for (int p = 0; p < NAP; ++p) {
+      const int idx = tid + 256 * p;
+      *reinterpret_cast<floatx4*>(&As[(idx / C4) * LS + 4 * (idx % C4)]) = ra[p];
+    }
+#pragma unroll
+    for (int p = 0; p < NB; ++p) {
+      const int idx = tid + 256 * p;
+      if (live(p)) *reinterpret_cast<floatx4*>(&Bs[(idx / C4) * LS + 4 * (idx % C4)]) = rb[p];
+    }
+    __syncthreads();
+    if (k0 + KC < g.K) fetch(k0 + KC);
+#pragma unroll
+    for (int sub = 0; sub < KC / 16; ++sub) {
+      const floatx4 av = *reinterpret_cast<const floatx4*>(&As[(16 * wv + n) * LS + 16 * sub + 4 * q]);
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) {
+        const floatx4 bv = *reinterpret_cast<const floatx4*>(&Bs[(16 * nt + n) * LS + 16 * sub + 4 * q]);
+        acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, bv.x, acc[nt], 0, 0, 0);
+        acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, bv.y, acc[nt], 0, 0, 0);
+        acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.z, bv.z, acc[nt], 0, 0, 0);
+        acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.w, bv.w, acc[nt], 0, 0, 0);
+      }
     }
   }
   epilogue<NT, EPI>(g, acc, m0, n0, wv, q, n);
@@ -338,7 +347,8 @@ static int32_t go(hipStream_t st, const GemmArgs& g, int prec) {
   dim3 grid(g.M / 64, g.N / (16 * NT));
   if (prec == DPMD_PREC_BF16) hipLaunchKernelGGL((gemm_h_kernel<NT, EPI, 1>), grid, dim3(256), 0, st, g);
   else if (prec == DPMD_PREC_FP16) hipLaunchKernelGGL((gemm_h_kernel<NT, EPI, 2>), grid, dim3(256), 0, st, g);
-  else hipLaunchKernelGGL((gemm_f32_kernel<NT, EPI>), grid, dim3(256), 0, st, g);
+  else if (g.K % 32 == 0) hipLaunchKernelGGL((gemm_f32_kernel<NT, EPI, 32>), grid, dim3(256), 0, st, g);
+  else hipLaunchKernelGGL((gemm_f32_kernel<NT, EPI, 16>), grid, dim3(256), 0, st, g);
   DPMD_HIP(hipGetLastError());
   return DPMD_OK;
 }
