@@ -125,10 +125,8 @@ struct Unroll {
 // three column blocks (5 tiles each) put three times as many blocks on the CUs as one
 // 240-wide block (measured: fitting time -2.4 %); the energy-head GEMM keeps whole rows.
 constexpr int kNtHidden = 5;
-constexpr int KC32 = 16;             // fp32 K slice
-constexpr int LS32 = KC32 + 4;       // fp32 LDS row stride (floats)
-constexpr int KC16 = 32;             // 16-bit K slice
-constexpr int LS16 = KC16 + 8;       // 16-bit LDS row stride (elements)
+constexpr int KC32 = 16;             // fp32 K granularity (slices of 16 or 32)
+constexpr int KC16 = 32;             // 16-bit K granularity (slices of 32 or 64)
 
 // FP32 GEMM: block tile 64 rows x 16 NT columns, four waves of 16 rows each over all NT
 // column tiles; K staged in KC-wide slices, KC = 32 when K allows (half the block barriers of
@@ -219,19 +217,24 @@ struct Half8<2> {
 
 // 16-bit GEMM: block tile 64 rows x 16 NT columns, four waves as 2 x 2, each owning 32
 // rows (two 16-row tiles) and half of the column tiles, so every B fragment read from LDS
-// feeds two MFMAs (the LDS read rate, not the MFMA rate, bounds a 16-row wave tile).
-template <int NT, int EPI, int PREC>
+// feeds two MFMAs (the LDS read rate, not the MFMA rate, bounds a 16-row wave tile).  K is
+// staged in 64-wide slices when K allows (two MFMA k-steps per block barrier: BF16 fitting
+// time -10 % against 32-wide slices), else 32.
+template <int NT, int EPI, int PREC, int KC>
 __global__ void __launch_bounds__(256, 2) gemm_h_kernel(const GemmArgs g) {
   typedef typename Half8<PREC>::T HT;
   typedef typename Half8<PREC>::E HE;
-  __shared__ __attribute__((aligned(16))) uint16_t As[64 * LS16];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs[16 * NT * LS16];
+  constexpr int LS = KC + 8;          // LDS row stride (elements)
+  __shared__ __attribute__((aligned(16))) uint16_t As[64 * LS];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[16 * NT * LS];
   __shared__ float Es[64];
   constexpr int BN = 16 * NT;
   constexpr int NH = (NT + 1) / 2;   // column tiles per wave
-  constexpr int BV = BN * KC16 / 8;  // 16-byte pieces per B slice
+  constexpr int P8 = KC / 8;         // 8-element pieces per row of a slice
+  constexpr int NAP = 64 * P8 / 256; // A pieces per thread
+  constexpr int BV = BN * P8;        // 16-byte pieces per B slice
   constexpr int NB = (BV + 255) / 256;
-  static_assert(NB <= 4, "B staging unrolled for at most 4 pieces per thread");
+  static_assert(NAP >= 1 && NAP <= 2 && NB <= 8, "staging unrolled for at most 2 A / 8 B pieces per thread");
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
   const int wm = wv & 1, wn = wv >> 1;
@@ -244,51 +247,54 @@ __global__ void __launch_bounds__(256, 2) gemm_h_kernel(const GemmArgs g) {
   for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
     for (int t = 0; t < NH; ++t) acc[mi][t] = floatx4{0.f, 0.f, 0.f, 0.f};
-  const int arow = tid / 4, ac8 = tid % 4;  // 64 rows x 4 pieces of 8 elements
-  floatx4 ra0, ra1;
-  uintx4 rb0, rb1, rb2, rb3;
-  auto bsrc = [&](int piece, int k) {
-    const int idx = tid + 256 * piece;
-    return reinterpret_cast<const uintx4*>(Bt + (size_t)(n0 + idx / 4) * g.ldb + k + 8 * (idx % 4));
-  };
-  auto bdst = [&](int piece) {
-    const int idx = tid + 256 * piece;
-    return reinterpret_cast<uintx4*>(&Bs[(idx / 4) * LS16 + 8 * (idx % 4)]);
-  };
+  floatx4 ra[2 * NAP];
+  uintx4 rb[NB];
   auto live = [&](int piece) { return piece + 1 < NB || tid + 256 * piece < BV; };
   auto fetch = [&](int k) {
-    const float* ap = g.A + (size_t)(m0 + arow) * g.lda + k + 8 * ac8;
-    ra0 = *reinterpret_cast<const floatx4*>(ap);
-    ra1 = *reinterpret_cast<const floatx4*>(ap + 4);
-    if constexpr (NB > 0) { if (live(0)) rb0 = *bsrc(0, k); }
-    if constexpr (NB > 1) { if (live(1)) rb1 = *bsrc(1, k); }
-    if constexpr (NB > 2) { if (live(2)) rb2 = *bsrc(2, k); }
-    if constexpr (NB > 3) { if (live(3)) rb3 = *bsrc(3, k); }
+#pragma unroll
+    for (int p = 0; p < NAP; ++p) {
+      const int idx = tid + 256 * p;
+      const float* ap = g.A + (size_t)(m0 + idx / P8) * g.lda + k + 8 * (idx % P8);
+      ra[2 * p] = *reinterpret_cast<const floatx4*>(ap);
+      ra[2 * p + 1] = *reinterpret_cast<const floatx4*>(ap + 4);
+    }
+#pragma unroll
+    for (int p = 0; p < NB; ++p) {
+      const int idx = tid + 256 * p;
+      if (live(p)) rb[p] = *reinterpret_cast<const uintx4*>(Bt + (size_t)(n0 + idx / P8) * g.ldb + k + 8 * (idx % P8));
+    }
   };
   fetch(0);
-  for (int k0 = 0; k0 < g.K; k0 += KC16) {
+  for (int k0 = 0; k0 < g.K; k0 += KC) {
     __syncthreads();
-    {
-      HT h;
-      h[0] = (HE)ra0.x; h[1] = (HE)ra0.y; h[2] = (HE)ra0.z; h[3] = (HE)ra0.w;
-      h[4] = (HE)ra1.x; h[5] = (HE)ra1.y; h[6] = (HE)ra1.z; h[7] = (HE)ra1.w;
-      *reinterpret_cast<HT*>(&As[arow * LS16 + 8 * ac8]) = h;
-    }
-    if constexpr (NB > 0) { if (live(0)) *bdst(0) = rb0; }
-    if constexpr (NB > 1) { if (live(1)) *bdst(1) = rb1; }
-    if constexpr (NB > 2) { if (live(2)) *bdst(2) = rb2; }
-    if constexpr (NB > 3) { if (live(3)) *bdst(3) = rb3; }
-    __syncthreads();
-    if (k0 + KC16 < g.K) fetch(k0 + KC16);
-    const HT a0 = *reinterpret_cast<const HT*>(&As[(32 * wm + n) * LS16 + 8 * q]);
-    const HT a1 = *reinterpret_cast<const HT*>(&As[(32 * wm + 16 + n) * LS16 + 8 * q]);
 #pragma unroll
-    for (int nt = 0; nt < NH; ++nt) {
-      const int tile = wn * NH + nt;
-      if (NT % 2 == 0 || tile < NT) {
-        const HT bv = *reinterpret_cast<const HT*>(&Bs[(16 * tile + n) * LS16 + 8 * q]);
-        acc[0][nt] = Half8<PREC>::mfma(a0, bv, acc[0][nt]);
-        acc[1][nt] = Half8<PREC>::mfma(a1, bv, acc[1][nt]);
+    for (int p = 0; p < NAP; ++p) {
+      const int idx = tid + 256 * p;
+      const floatx4 x0 = ra[2 * p], x1 = ra[2 * p + 1];
+      HT h;
+      h[0] = (HE)x0.x; h[1] = (HE)x0.y; h[2] = (HE)x0.z; h[3] = (HE)x0.w;
+      h[4] = (HE)x1.x; h[5] = (HE)x1.y; h[6] = (HE)x1.z; h[7] = (HE)x1.w;
+      *reinterpret_cast<HT*>(&As[(idx / P8) * LS + 8 * (idx % P8)]) = h;
+    }
+#pragma unroll
+    for (int p = 0; p < NB; ++p) {
+      const int idx = tid + 256 * p;
+      if (live(p)) *reinterpret_cast<uintx4*>(&Bs[(idx / P8) * LS + 8 * (idx % P8)]) = rb[p];
+    }
+    __syncthreads();
+    if (k0 + KC < g.K) fetch(k0 + KC);
+#pragma unroll
+    for (int sub = 0; sub < KC / 32; ++sub) {
+      const HT a0 = *reinterpret_cast<const HT*>(&As[(32 * wm + n) * LS + 32 * sub + 8 * q]);
+      const HT a1 = *reinterpret_cast<const HT*>(&As[(32 * wm + 16 + n) * LS + 32 * sub + 8 * q]);
+#pragma unroll
+      for (int nt = 0; nt < NH; ++nt) {
+        const int tile = wn * NH + nt;
+        if (NT % 2 == 0 || tile < NT) {
+          const HT bv = *reinterpret_cast<const HT*>(&Bs[(16 * tile + n) * LS + 32 * sub + 8 * q]);
+          acc[0][nt] = Half8<PREC>::mfma(a0, bv, acc[0][nt]);
+          acc[1][nt] = Half8<PREC>::mfma(a1, bv, acc[1][nt]);
+        }
       }
     }
   }
@@ -345,8 +351,11 @@ __global__ void __launch_bounds__(256, 2) gemm_h_kernel(const GemmArgs g) {
 template <int NT, int EPI>
 static int32_t go(hipStream_t st, const GemmArgs& g, int prec) {
   dim3 grid(g.M / 64, g.N / (16 * NT));
-  if (prec == DPMD_PREC_BF16) hipLaunchKernelGGL((gemm_h_kernel<NT, EPI, 1>), grid, dim3(256), 0, st, g);
-  else if (prec == DPMD_PREC_FP16) hipLaunchKernelGGL((gemm_h_kernel<NT, EPI, 2>), grid, dim3(256), 0, st, g);
+  const bool k64 = g.K % 64 == 0;
+  if (prec == DPMD_PREC_BF16 && k64) hipLaunchKernelGGL((gemm_h_kernel<NT, EPI, 1, 64>), grid, dim3(256), 0, st, g);
+  else if (prec == DPMD_PREC_BF16) hipLaunchKernelGGL((gemm_h_kernel<NT, EPI, 1, 32>), grid, dim3(256), 0, st, g);
+  else if (prec == DPMD_PREC_FP16 && k64) hipLaunchKernelGGL((gemm_h_kernel<NT, EPI, 2, 64>), grid, dim3(256), 0, st, g);
+  else if (prec == DPMD_PREC_FP16) hipLaunchKernelGGL((gemm_h_kernel<NT, EPI, 2, 32>), grid, dim3(256), 0, st, g);
   else if (g.K % 32 == 0) hipLaunchKernelGGL((gemm_f32_kernel<NT, EPI, 32>), grid, dim3(256), 0, st, g);
   else hipLaunchKernelGGL((gemm_f32_kernel<NT, EPI, 16>), grid, dim3(256), 0, st, g);
   DPMD_HIP(hipGetLastError());
