@@ -14,7 +14,6 @@
 #include "internal.h"
 #include <algorithm>
 #include <string.h>
-#include <type_traits>
 
 namespace dpmd {
 
@@ -108,18 +107,6 @@ __device__ __forceinline__ void epilogue(const GemmArgs& g, const floatx4 (&acc)
     }
   }
 }
-
-// compile-time loop: f(integral_constant<V>) for V in [V0, V1)
-template <int V0, int V1>
-struct Unroll {
-  template <class F>
-  static __device__ __forceinline__ void run(F&& f) {
-    if constexpr (V0 < V1) {
-      f(std::integral_constant<int, V0>());
-      Unroll<V0 + 1, V1>::run(f);
-    }
-  }
-};
 
 // Column tiles per block of the 240-wide hidden-layer GEMMs whose epilogue is per element:
 // three column blocks (5 tiles each) put three times as many blocks on the CUs as one
