@@ -453,6 +453,7 @@ static int32_t model_forces_impl(System* s, int precision) {
   hipEvent_t ev;
   DPMD_TRY(sys_upload_segments(s));
   s->force_valid = true;
+  s->force_prec = precision;
   DPMD_TRY(md_zero_force(s));
   timer_begin(s, 1, &ev);
   DPMD_TRY(embed_launch(s, false, precision));
@@ -1073,7 +1074,8 @@ int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps, 
   DPMD_TRY(nlist_clear_flags(s));
   const double dt = p->dt;
   const int prec = p->precision;
-  const bool fresh_forces = !s->force_valid;
+  // forces held from an evaluation at another precision are recomputed at the run's
+  const bool fresh_forces = !s->force_valid || s->force_prec != prec;
   if (fresh_forces) {
     DPMD_TRY(md_write_pos32(s));
     DPMD_TRY(model_forces_impl(s, prec));

@@ -250,6 +250,7 @@ struct System {
   int fpad = 0;                 // fitting output width
   int fstride = 0;              // fitting activation row stride
   bool force_valid = false;
+  int force_prec = 0;           // precision the valid forces were computed in
   // integrator
   double* d_thermo = nullptr;   // 64 doubles per segment
   double* d_traj = nullptr;     // device trajectory staging: positions then velocities,
