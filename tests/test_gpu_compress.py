@@ -134,3 +134,59 @@ def test_compressed_model_pkl_evaluate_and_simulation(tmp_path, model0c):
         assert abs(sim.getEnergy() - E.item()) / len(types) <= 1e-5
         traj = sim.run(10)
     assert np.isfinite(traj['position']).all()
+
+
+def test_compressed_pimd_and_general_cell(device_model_c, model0c):
+    """The tabulated kernels over several segments and in a general cell: ring-polymer BAOAB
+    without noise (4 beads x 64 H2O, 3 steps) against the oracle's tabulated restatement, and
+    the energy / forces of a skewed cell."""
+    from deepmd_jax import _lib, synthetic as syn
+    from deepmd_jax.constants import MASS_UNIT_CONVERSION, TEMP_UNIT_CONVERSION
+    from deepmd_jax.pimd import normal_mode_transform
+    print('case compressed pimd / cell', flush=True)
+    params, var = model0c
+    P, dt = 4, 0.25
+    coord, box, types = syn.water_box(64, seed=33)
+    rng = np.random.default_rng(33)
+    x = coord[None] + rng.uniform(-0.05, 0.05, (P,) + coord.shape)
+    m = np.array(syn.MASSES)[types] * MASS_UNIT_CONVERSION
+    kT = 300 * TEMP_UNIT_CONVERSION
+    pm = rng.standard_normal(x.shape) * np.sqrt(m * kT * P)[None, :, None]
+    freqs, C = normal_mode_transform(P, kT)
+    s = _lib.DeviceSystem(device_model_c, types, P)
+    s.set_box(box)
+    s.set_masses(m)
+    s.set_positions(x)
+    s.set_velocities(pm / m[None, :, None])
+    s.nlist_allocate(6.2, 1.2)
+    s.pimd_setup(C, freqs, np.ones(P), np.zeros(P))
+    p = _lib.MDParams()
+    p.routine, p.dt = 3, dt
+    tp, tv, err = s.md_run(p, 3)
+    assert err == 0
+    lat = O.lattice_candidate(np.diag(box), params['rcut'])
+
+    def forces(xx):
+        return np.stack([O.energy_force_virial(params, var, xx[b], box, types, lattice=lat)[1].numpy()
+                         for b in range(P)])
+    F = forces(x)
+    xo, po = x.copy(), pm.copy()
+    for _ in range(3):
+        xo, po, F = O.pimd_step(xo, po, m, forces, dt, C, freqs, np.ones(P), np.zeros(P), F, box)
+    assert np.abs(tp[-1] - xo).max() < 1e-5
+    assert np.abs(tv[-1] - po / m[None, :, None]).max() < 1e-5
+    # general cell
+    c0, b0, t0 = syn.water_box(216, seed=34)
+    H = np.array([[b0[0], 0, 0], [0.2 * b0[1], b0[1], 0], [-0.1 * b0[2], 0.15 * b0[2], b0[2]]])
+    c = (c0 / b0) @ H
+    s2 = _lib.DeviceSystem(device_model_c, t0, 1)
+    s2.set_box(H)
+    s2.set_masses(np.ones(len(t0)))
+    s2.set_positions(c)
+    s2.nlist_allocate(params['rcut'], 1.2)
+    e, f, v, _ = s2.compute('fp32')
+    nb, _, _ = O.typed_nlist(c, H, t0, params['rcut'])
+    E, Fo, W = O.energy_force_virial(params, var, c, H, t0, nbrs_nm=nb)
+    assert abs(e[0] - E.item()) / len(t0) <= 1e-5
+    assert np.abs(f[0] - Fo.numpy()).max() <= 1e-4 * max(1.0, np.abs(Fo.numpy()).max())
+    assert np.abs(v[0] - W.numpy()).max() <= 1e-4 * np.abs(W.numpy()).max()
