@@ -110,3 +110,28 @@ def test_compressed_three_types():
     assert abs(e[0] - E) / N <= 1e-5, (e[0], E)
     assert np.abs(f[0] - F).max() <= 1e-4 * max(1.0, np.abs(F).max())
     assert np.abs(v[0] - W).max() <= 1e-4 * np.abs(W).max()
+
+
+def test_simulation_bf16_precision(model_pkl, model0):
+    """Simulation(precision='bf16'): the run integrates with BF16-GEMM forces (their RMSE against
+    the FP64 oracle within 1e-3 eV/A at the end state), KE + PE holds to 1e-3 eV/atom over 40 steps
+    of 0.25 fs, and the reported PE is the BF16 evaluation's."""
+    import contextlib
+    import io
+    from deepmd_jax import synthetic as syn
+    from deepmd_jax.md import Simulation
+    print('case simulation bf16', flush=True)
+    params, var = model0
+    coord, box, types = syn.water_box(64, seed=95)
+    with contextlib.redirect_stdout(io.StringIO()):
+        sim = Simulation(model_pkl, box, types, syn.MASSES, 'NVE', 0.25, coord, report_interval=10, precision='bf16',
+                         initial_velocity=syn.maxwell_boltzmann(types, 300.0, seed=95))
+        sim.run(40)
+    log = np.array(sim.log)
+    inv = log[:, 4]
+    assert np.abs(inv - inv[0]).max() / len(types) < 1e-3, inv
+    x = sim.getPosition()
+    F = O.energy_force_virial(params, var, x, box, types, lattice='auto')[1].numpy()
+    f = sim.getForce()
+    assert np.sqrt(((f - F) ** 2).mean()) <= 1e-3
+    assert abs(sim.getEnergy() - log[-1, 3]) <= 1e-6 * abs(log[-1, 3]) + 1e-6
