@@ -21,6 +21,7 @@ Function -> reference file:line it follows:
   compress_net          deepmd_jax/utils.py:250-286 (derivatives by autograd, the
                         quintic Hermite by scipy's BPoly.from_derivatives, as there)
   fitting_net           deepmd_jax/utils.py:153-166
+  descriptor_from_T     deepmd_jax/dpmodel.py:74-78
   dp_energy             deepmd_jax/dpmodel.py:30-81,100
   energy_force_virial   deepmd_jax/dpmodel.py:106-108; virial as the strain
                         derivative of deepmd_jax/md_utils.py:139-159
@@ -261,12 +262,20 @@ def _descriptor(params, variables, coord, box3, type_idx, nbrs_nm, lattice, stra
                 Ti = Ti + embedding_net(P['embedding_net_%d' % k], s_cn[:, :, None], RX)
         T_list.append(Ti)
     T = torch.cat(T_list, 0) / params['Nnbrs']
-    TW = T[:, 0] + torch.as_tensor(P['Tbias'], dtype=dtype)
+    G = descriptor_from_T(T, P['Tbias'], A)
+    return T, G.reshape(G.shape[0], -1), type_count
+
+
+def descriptor_from_T(T, Tbias, A):
+    """G (rows, A, W) from T (rows, 10, W), deepmd_jax/dpmodel.py:74-78 with use_2nd:
+    TW = T[:,0] + Tbias, T3 = T[:,1:4], T6 = T[:,4:10];
+    G[a,w] = TW[w] TW[a] + sum_c T3[c,w] T3[c,a]
+             + sum_k (3to6(T3[:, A:2A]) + T6[:, A:2A])[k, a] T6[k, w]."""
+    TW = T[:, 0] + torch.as_tensor(np.asarray(Tbias), dtype=T.dtype)
     T3, T6 = T[:, 1:4], T[:, 4:]
     G = TW[:, None] * TW[:, :A, None] + (T3[:, :, None] * T3[:, :, :A, None]).sum(1)
     G2 = tensor_3to6(T3[:, :, A:2 * A], axis=1) + T6[:, :, A:2 * A]
-    G = G + (G2[..., None] * T6[:, :, None]).sum(1)
-    return T, G.reshape(G.shape[0], -1), type_count
+    return G + (G2[..., None] * T6[:, :, None]).sum(1)
 
 
 def dp_energy(params, variables, coord, box3, type_idx, nbrs_nm=None, lattice=None, return_T=False,

@@ -65,12 +65,10 @@ def test_descriptor_reduces_to_outer_product(model0):
     A, W = params['axis'], params['embed_widths'][-1]
     T = torch.zeros(1, 10, W, dtype=torch.float64)
     T[0, 0] = torch.randn(W, dtype=torch.float64)
-    TW = T[:, 0]
-    T3, T6 = T[:, 1:4], T[:, 4:]
-    G = TW[:, None] * TW[:, :A, None] + (T3[:, :, None] * T3[:, :, :A, None]).sum(1)
-    G2 = O.tensor_3to6(T3[:, :, A:2 * A], axis=1) + T6[:, :, A:2 * A]
-    G = G + (G2[..., None] * T6[:, :, None]).sum(1)
-    assert torch.allclose(G[0], torch.outer(TW[0, :A], TW[0]))
+    tb = np.random.default_rng(2).standard_normal(W)
+    G = O.descriptor_from_T(T, tb, A)
+    TW = T[0, 0] + torch.as_tensor(tb)
+    assert torch.allclose(G[0], torch.outer(TW[:A], TW))
 
 
 def test_invariances(model0):
@@ -275,3 +273,168 @@ def test_local_fd_matches_autograd(model0):
     atoms = [0, 5, 1001]
     fd = O.local_fd_forces(params, var, coord, box, types, atoms)
     np.testing.assert_allclose(fd, F.numpy()[atoms], atol=2e-6)
+
+
+# ---------------------------------------------------------------------------------
+# Hand-computed pins of the forward conventions of deepmd_jax/utils.py:114-124 (repeat
+# residual), :161-163 (fitting dt residual where widths are equal), dpmodel.py:38-39
+# (normalisation by the centre type), :50-51 (net index i*Y + j, centre type outer) and
+# :74-78 (second-order term on T[:, :, A:2A]).  Every expected value below is written
+# out from the formulas, never by calling the oracle's own helpers.
+
+def test_embedding_repeat_residual_hand_values():
+    """1 -> 2 -> 4 -> 8 net with distinct parameters: the residual repeats each element in
+    place ([a, a, b, b, ...], jnp.repeat), not a tile ([a, b, a, b, ...])."""
+    k0, b0 = np.array([[0.7, -1.3]]), np.array([0.2, -0.4])
+    k1 = np.array([[0.5, -0.2, 0.1, 0.9], [-0.6, 0.3, 0.8, -0.1]])
+    b1 = np.array([0.05, -0.3, 0.6, -0.9])
+    k2 = np.arange(32, dtype=np.float64).reshape(4, 8) / 40.0 - 0.4
+    b2 = np.linspace(-0.5, 0.5, 8)
+    net = {'Dense_0': {'kernel': k0, 'bias': b0}, 'Dense_1': {'kernel': k1, 'bias': b1},
+           'Dense_2': {'kernel': k2, 'bias': b2}}
+    s = 0.37
+    h1 = np.tanh([0.7 * s + 0.2, -1.3 * s - 0.4])
+    z1 = [0.5 * h1[0] - 0.6 * h1[1] + 0.05, -0.2 * h1[0] + 0.3 * h1[1] - 0.3,
+          0.1 * h1[0] + 0.8 * h1[1] + 0.6, 0.9 * h1[0] - 0.1 * h1[1] - 0.9]
+    h2 = np.tanh(z1) + np.array([h1[0], h1[0], h1[1], h1[1]])
+    z2 = [sum(h2[r] * k2[r, c] for r in range(4)) + b2[c] for c in range(8)]
+    h3 = np.tanh(z2) + np.array([h2[0], h2[0], h2[1], h2[1], h2[2], h2[2], h2[3], h2[3]])
+    out = O.embedding_net(net, torch.tensor([[[s]]], dtype=torch.float64),
+                          torch.ones(1, 1, 1, dtype=torch.float64))[0, 0].numpy()
+    np.testing.assert_allclose(out, h3, rtol=0, atol=1e-14)
+    # zero kernels: the layers reduce to tanh(bias) plus the repeated parent
+    zn = {k: {'kernel': np.zeros_like(v['kernel']), 'bias': v['bias']} for k, v in net.items()}
+    g1 = np.tanh(b0)
+    g2 = np.tanh(b1) + np.array([g1[0], g1[0], g1[1], g1[1]])
+    g3 = np.tanh(b2) + np.array([g2[0], g2[0], g2[1], g2[1], g2[2], g2[2], g2[3], g2[3]])
+    out = O.embedding_net(zn, torch.tensor([[[s]]], dtype=torch.float64),
+                          torch.ones(1, 1, 1, dtype=torch.float64))[0, 0].numpy()
+    np.testing.assert_allclose(out, g3, rtol=0, atol=1e-14)
+
+
+def test_fitting_dt_residual_hand_values():
+    """Equal-width hidden layers add x_prev after scaling by dt; a width change has no
+    residual (utils.py:158-163).  Zero hidden kernels make every layer tanh(b)."""
+    b0, b1, b2 = np.array([0.3, -0.8, 1.1]), np.array([-0.5, 0.2, 0.9]), np.array([0.7, -1.2, 0.05])
+    dt1, dt2 = np.array([0.1, 0.25, -0.4]), np.array([0.6, -0.05, 0.3])
+    k3, b3 = np.array([[1.5], [-0.7], [0.25]]), np.array([0.125])
+    net = {'Dense_0': {'kernel': np.zeros((5, 3)), 'bias': b0},
+           'Dense_1': {'kernel': np.zeros((3, 3)), 'bias': b1}, 'dt1': dt1,
+           'Dense_2': {'kernel': np.zeros((3, 3)), 'bias': b2}, 'dt2': dt2,
+           'Dense_3': {'kernel': k3, 'bias': b3}}
+    x = torch.tensor([[0.4, -1.0, 2.0, 0.0, 3.0]], dtype=torch.float64)
+    y1 = np.tanh(b0)
+    y2 = np.tanh(b1) * dt1 + y1
+    y3 = np.tanh(b2) * dt2 + y2
+    e = 1.5 * y3[0] - 0.7 * y3[1] + 0.25 * y3[2] + 0.125
+    assert abs(O.fitting_net(net, x)[0, 0].item() - e) < 1e-14
+    # 5 -> 3 -> 2 -> 1: the width change has no residual and no dt
+    net2 = {'Dense_0': {'kernel': np.zeros((5, 3)), 'bias': b0},
+            'Dense_1': {'kernel': np.array([[1.0, 0.0], [0.0, 2.0], [0.5, -1.0]]), 'bias': np.array([0.1, -0.2])},
+            'Dense_2': {'kernel': np.array([[2.0], [-3.0]]), 'bias': np.array([0.0])}}
+    z = np.tanh([y1[0] + 0.5 * y1[2] + 0.1, 2.0 * y1[1] - y1[2] - 0.2])
+    assert abs(O.fitting_net(net2, x)[0, 0].item() - (2.0 * z[0] - 3.0 * z[1])) < 1e-14
+
+
+def test_descriptor_hand_entries():
+    """G of a tiny T (W = 4, A = 2) with every block nonzero, entries written out:
+    G[a,w] = TW_w TW_a + sum_c T3[c,w] T3[c,a] + sum_k (3to6(T3[:,A+a]) + T6[k,A+a]) T6[k,w],
+    3to6(v) = (v0^2, v1^2, v2^2, sqrt2 v0 v2, sqrt2 v1 v0, sqrt2 v2 v1)."""
+    T = torch.tensor([[[1, 2, 0, -1],                  # T0
+                       [0, 1, 1, 2], [1, 0, -1, 1], [2, 1, 0, 0],   # T3 rows x, y, z
+                       [1, 0, 0, 1], [0, 1, 1, 0], [0, 0, 1, 1],    # T6 rows 0..2
+                       [1, 1, 0, 0], [0, 0, 0, 1], [1, 0, 1, 0]]],  # T6 rows 3..5
+                     dtype=torch.float64)
+    Tbias = np.array([0.5, 0.0, -1.0, 0.0])
+    G = O.descriptor_from_T(T, Tbias, 2)[0].numpy()
+    r2 = 2 ** 0.5
+    # TW = (1.5, 2, -1, -1)
+    # a = 0: T3[:, 0] = (0, 1, 2); T3[:, A+0] = T3[:, 2] = (1, -1, 0);
+    #        3to6 = (1, 1, 0, 0, -r2, 0); T6[:, 2] = (0, 1, 1, 0, 0, 1) -> v = (1, 2, 1, 0, -r2, 1)
+    # a = 1: T3[:, 1] = (1, 0, 1); T3[:, 3] = (2, 1, 0); 3to6 = (4, 1, 0, 0, 2 r2, 0);
+    #        T6[:, 3] = (1, 0, 1, 0, 1, 0) -> v = (5, 1, 1, 0, 1 + 2 r2, 0)
+    # T6 columns w: (1,0,0,1,0,1), (0,1,0,1,0,0), (0,1,1,0,0,1), (1,0,1,0,1,0)
+    expect = np.array([
+        [1.5 * 1.5 + 5 + (1 + 0 + 0 + 0 + 0 + 1),            # w=0: T3 dot (0,1,2)(0,1,2)=5
+         2 * 1.5 + (1 * 0 + 0 * 1 + 1 * 2) + (0 + 2 + 0 + 0 + 0 + 0),
+         -1 * 1.5 + (1 * 0 + (-1) * 1 + 0 * 2) + (0 + 2 + 1 + 0 + 0 + 1),
+         -1 * 1.5 + (2 * 0 + 1 * 1 + 0 * 2) + (1 + 0 + 1 + 0 - r2 + 0)],
+        [1.5 * 2 + (0 * 1 + 1 * 0 + 2 * 1) + (5 + 0 + 0 + 0 + 0 + 0),
+         2 * 2 + (1 + 0 + 1) + (0 + 1 + 0 + 0 + 0 + 0),
+         -1 * 2 + (1 * 1 + (-1) * 0 + 0 * 1) + (0 + 1 + 1 + 0 + 0 + 0),
+         -1 * 2 + (2 * 1 + 1 * 0 + 0 * 1) + (5 + 0 + 1 + 0 + (1 + 2 * r2) + 0)]])
+    np.testing.assert_allclose(G, expect, rtol=0, atol=1e-13)
+
+
+def _pair_model(Y=2, W=8, A=2):
+    """A tiny two-type model whose only nonzero embedding nets are (O centre, H neighbour)
+    and (H centre, O neighbour), distinct per-type statistics, Tbias, Nnbrs and fitting nets."""
+    rng = np.random.default_rng(11)
+
+    def dense(i, o, scale=1.0):
+        return {'kernel': rng.standard_normal((i, o)) * scale / np.sqrt(i), 'bias': rng.standard_normal(o)}
+
+    def zero_net():
+        return {'Dense_0': {'kernel': np.zeros((1, 2)), 'bias': np.zeros(2)},
+                'Dense_1': {'kernel': np.zeros((2, 4)), 'bias': np.zeros(4)},
+                'Dense_2': {'kernel': np.zeros((4, 8)), 'bias': np.zeros(8)}}
+    P = {'embedding_net_%d' % k: zero_net() for k in range(Y * Y)}
+    for k in (1, 2):
+        P['embedding_net_%d' % k] = {'Dense_0': dense(1, 2), 'Dense_1': dense(2, 4), 'Dense_2': dense(4, 8)}
+    P['Tbias'] = np.array([0.3, -0.2, 0.1, 0.0, 0.25, -0.15, 0.05, 0.4])
+    for t in range(Y):
+        P['fitting_net_%d' % t] = {'Dense_0': dense(A * W, 3, 4.0), 'Dense_1': dense(3, 1)}
+    params = {'type': 'energy', 'embed_widths': [2, 4, 8], 'fit_widths': [3], 'axis': A, 'rcut': 6.0,
+              'use_2nd': True, 'atomic': False, 'ntypes': Y, 'out_norm': 1.7, 'Ebias': np.array([0.5, -0.25]),
+              'sr_mean': np.array([0.11, 0.37]), 'sr_std': np.array([0.53, 1.9]), 'Nnbrs': 3.7}
+    return params, {'params': P}
+
+
+def _net_hand(net, s):
+    """1 -> 2 -> 4 -> 8 embedding net on a scalar, residual parents written out."""
+    d0, d1, d2 = net['Dense_0'], net['Dense_1'], net['Dense_2']
+    h1 = np.tanh(s * d0['kernel'][0] + d0['bias'])
+    h2 = np.tanh(h1 @ d1['kernel'] + d1['bias']) + h1[[0, 0, 1, 1]]
+    return np.tanh(h2 @ d2['kernel'] + d2['bias']) + h2[[0, 0, 1, 1, 2, 2, 3, 3]]
+
+
+def test_single_pair_energy_closed_form():
+    """One O and one H, 2.3 A apart along a general direction in a 30 A box.  Centre i of
+    type t sees one neighbour: s = sr(r), s_hat = s / std_t, s_til = (s - mean_t) / std_t
+    (the centre's statistics), h = net_{t*Y + other}(s_til), T = R_X h^T / Nnbrs.  With
+    c = 1/Nnbrs and R0^2 + |R3|^2 = 4 s_hat^2, sum_k 3to6(R3)_k R6_k = 6 s_hat^3,
+    sum_k R6_k^2 = 6 s_hat^2 for any unit direction:
+      G[a,w] = (c s_hat h_w + tb_w)(c s_hat h_a + tb_a) + 3 c^2 s_hat^2 h_a h_w
+               + 6 c^3 s_hat^3 h_{A+a}^2 h_w + 6 c^2 s_hat^2 h_{A+a} h_w,
+    and E = out_norm * sum_t (fit_t(G_t) + Ebias_t)."""
+    params, var = _pair_model()
+    P = var['params']
+    A, Y, rc = params['axis'], params['ntypes'], params['rcut']
+    u = np.array([0.48, -0.6, 0.64])
+    u /= np.linalg.norm(u)
+    r = 2.3
+    coord = np.array([[12.0, 14.0, 15.0], [12.0, 14.0, 15.0] + r * u])
+    types = np.array([0, 1])
+    box = np.full(3, 30.0)
+    t_ = r / rc
+    s = (1 - 3 * t_ ** 2 + 2 * t_ ** 3) / (r + 1e-15)
+    c = 1.0 / params['Nnbrs']
+    tb = P['Tbias']
+    E = 0.0
+    for t in range(Y):
+        sh = s / params['sr_std'][t]
+        st = (s - params['sr_mean'][t]) / params['sr_std'][t]
+        h = _net_hand(P['embedding_net_%d' % (t * Y + (1 - t))], st)
+        G = np.empty((A, 8))
+        for a in range(A):
+            for w in range(8):
+                G[a, w] = ((c * sh * h[w] + tb[w]) * (c * sh * h[a] + tb[a]) + 3 * c * c * sh * sh * h[a] * h[w]
+                           + 6 * c ** 3 * sh ** 3 * h[A + a] ** 2 * h[w] + 6 * c * c * sh * sh * h[A + a] * h[w])
+        f = P['fitting_net_%d' % t]
+        y = np.tanh(G.reshape(-1) @ f['Dense_0']['kernel'] + f['Dense_0']['bias'])
+        E += (y @ f['Dense_1']['kernel'])[0] + f['Dense_1']['bias'][0] + params['Ebias'][t]
+    E *= params['out_norm']
+    lat = O.lattice_candidate(np.diag(box), rc)
+    for order in ([0, 1], [1, 0]):          # atom order in the frame does not matter
+        E0 = O.dp_energy(params, var, torch.as_tensor(coord[order]), box, types[order], lattice=lat).item()
+        assert abs(E0 - E) < 1e-12 * max(1.0, abs(E)), (E0, E)
