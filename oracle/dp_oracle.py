@@ -33,6 +33,11 @@ Function -> reference file:line it follows:
   nvt_nh_step           deepmd_jax/routine.py:110-136,147-158 (jax_md nvt_nose_hoover,
                         chain masses from the mobile degrees of freedom)
   remove_com            deepmd_jax/md.py:786-790
+  nvt_langevin_step     deepmd_jax/routine.py:46-107 (BAOAB, O step :86-101)
+  philox4x32_10, normal4, langevin_noise, pimd_noise
+                        the engine's counter-based normals (Philox4x32-10, Box-Muller), the
+                        stand-in for jax.random.normal at routine.py:92-96, pimd.py:103-105
+  pimd_step             deepmd_jax/pimd.py:92-111,183-207 (PILE-L O step in normal modes)
   PIMDNPTOracle         deepmd_jax/pimd.py:297-518 (npt_langevin_pimd), invariant
                         deepmd_jax/md.py:540-557 (pinned by its conservation)
   energy_force_virial(mobile=) deepmd_jax/md.py:428-430 (stop-gradient on fixed atoms)
@@ -644,6 +649,99 @@ def free_ring(x, p, m, C, freqs, h, box):
     pnn = pn.copy()
     pnn[1:] = pn[1:] * cs - mm * w[1:] * q[1:] * sn
     return np.tensordot(C, qn, axes=(1, 0)), np.tensordot(C, pnn, axes=(1, 0))
+
+
+# ---------------------------------------------------------------------------------
+# Stochastic steps.  The reference draws jax.random.normal (threefry) for the BAOAB O step
+# (deepmd_jax/routine.py:92-96) and the PILE-L O step (deepmd_jax/pimd.py:103-105); without
+# jax that stream cannot be reproduced, so the engine draws counter-based normals of its
+# own and the oracle restates that stream exactly:
+#   Philox4x32-10 (J. Salmon, M. Moraes, R. Dror, D. Shaw, "Parallel random numbers: as
+#   easy as 1, 2, 3", SC'11): ten rounds of two 32x32->64 multiplies (M0 = 0xD2511F53,
+#   M1 = 0xCD9E8D57) with the Weyl key schedule (W0 = 0x9E3779B9, W1 = 0xBB67AE85);
+#   Box-Muller on the two 32-bit pairs: u1 = (a + 1)/(2^32 + 1), u2 = b/2^32,
+#   z = sqrt(-2 ln u1) (cos 2 pi u2, sin 2 pi u2).
+# Pinned by the published known-answer vectors of Philox4x32-10 and by the moments and the
+# distribution function of the normals (tests/test_stochastic.py).
+
+_PHILOX_M = (np.uint64(0xD2511F53), np.uint64(0xCD9E8D57))
+_PHILOX_W = (np.uint64(0x9E3779B9), np.uint64(0xBB67AE85))
+_U32 = np.uint64(0xFFFFFFFF)
+LANGEVIN_STREAM = 0x4c414e47     # 'LANG'
+PIMD_STREAM = 0x50494d00         # 'PIM' + mode block
+
+
+def philox4x32_10(ctr, key):
+    """ctr (n, 4), key (n, 2) unsigned 32-bit words -> (n, 4) uint32."""
+    c = [np.asarray(ctr, np.uint64)[:, i].copy() for i in range(4)]
+    k = [np.asarray(key, np.uint64)[:, i].copy() for i in range(2)]
+    for _ in range(10):
+        p0 = _PHILOX_M[0] * c[0]
+        p1 = _PHILOX_M[1] * c[2]
+        c = [(p1 >> np.uint64(32)) ^ c[1] ^ k[0], p1 & _U32, (p0 >> np.uint64(32)) ^ c[3] ^ k[1], p0 & _U32]
+        k = [(k[0] + _PHILOX_W[0]) & _U32, (k[1] + _PHILOX_W[1]) & _U32]
+    return np.stack(c, 1).astype(np.uint32)
+
+
+def normal4(seed, counter, idx, stream):
+    """Four standard normals per idx from the counter (counter lo, counter hi, idx, stream)
+    under the key (seed lo, seed hi); idx an int array, the rest Python ints."""
+    idx = np.asarray(idx, np.uint64).reshape(-1)
+    n = len(idx)
+    ctr = np.stack([np.full(n, counter & 0xFFFFFFFF, np.uint64), np.full(n, counter >> 32, np.uint64), idx,
+                    np.full(n, stream, np.uint64)], 1)
+    key = np.tile(np.array([seed & 0xFFFFFFFF, seed >> 32], np.uint64), (n, 1))
+    c = philox4x32_10(ctr, key).astype(np.float64)
+    two32 = 4294967296.0
+    z = np.empty((n, 4))
+    for h in range(2):
+        u1 = (c[:, 2 * h] + 1.0) / (two32 + 1.0)
+        u2 = c[:, 2 * h + 1] / two32
+        r = np.sqrt(-2.0 * np.log(u1))
+        z[:, 2 * h] = r * np.cos(6.283185307179586 * u2)
+        z[:, 2 * h + 1] = r * np.sin(6.283185307179586 * u2)
+    return z
+
+
+def segment_key(seed, seg_seed):
+    """Key of one segment's Langevin stream: seed xor (segment seed * 0x9E3779B97F4A7C15) mod 2^64."""
+    return (int(seed) ^ (int(seg_seed) * 0x9E3779B97F4A7C15)) & 0xFFFFFFFFFFFFFFFF
+
+
+def langevin_noise(seed, seg_seed, step, n):
+    """(n, 3) O-step normals of one segment at its step counter, atoms in the segment's
+    type-sorted order (index = position in that order)."""
+    return normal4(segment_key(seed, seg_seed), int(step), np.arange(n), LANGEVIN_STREAM)[:, :3]
+
+
+def pimd_noise(seed, counter, natoms, P):
+    """(P, natoms, 3) PILE-L normals in normal-mode order k for atom i (type-sorted order
+    inside a bead), component d: index 3 i + d, stream PIMD_STREAM + 4 floor(k / 4), word k mod 4."""
+    t = np.arange(natoms * 3)
+    out = np.empty((P, natoms, 3))
+    for k in range(0, P, 4):
+        z = normal4(int(seed), int(counter), t, PIMD_STREAM + k)
+        for u in range(min(4, P - k)):
+            out[k + u] = z[:, u].reshape(natoms, 3)
+    return out
+
+
+def nvt_langevin_step(x, p, m, force_fn, dt, F, gamma, kT, noise, mobile=None):
+    """One BAOAB step of deepmd_jax/routine.py:81-107: B dt/2, A dt/2, O (p = c1 p + c2 sqrt(m)
+    xi, c1 = exp(-gamma dt), c2 = sqrt(max(kT (1 - c1^2), 0)), xi masked on fixed atoms),
+    A dt/2, forces, B dt/2.  Returns (x, p, F, dKE of the O step)."""
+    mm = np.asarray(m, np.float64)[:, None]
+    p = p + 0.5 * dt * F
+    x = x + 0.5 * dt * p / mm
+    c1 = np.exp(-gamma * dt)
+    c2 = np.sqrt(max(kT * (1.0 - c1 ** 2), 0.0))
+    xi = noise if mobile is None else noise * np.asarray(mobile, np.float64)[:, None]
+    p_new = c1 * p + c2 * np.sqrt(mm) * xi
+    dke = 0.5 * float(((p_new ** 2 - p ** 2) / mm).sum())
+    x = x + 0.5 * dt * p_new / mm
+    F = force_fn(x)
+    p = p_new + 0.5 * dt * F
+    return x, p, F, dke
 
 
 def pimd_step(x, p, m, force_fn, dt, C, freqs, c1, c2, F, box, noise=None):
