@@ -617,6 +617,8 @@ static int32_t set_seg_box(System* s, int r, const double* b) {
     S.scale[d] = S.L[d] / (float)b[d];
     S.cellbox[d] = S.kimg[d] > 0 ? b[d] / S.kimg[d] : 0.0;
   }
+  // the drift test follows the box through scale = L_ref / box (deepmd_jax/md.py:724-732)
+  if (s->nl_ready) seg_set_threshold(S, s->rc_skin, s->model->rcut, false);
   s->seg_dirty = true;
   s->force_valid = false;
   return DPMD_OK;
@@ -987,13 +989,7 @@ static int32_t npt_args(System* s, const dpmd_md_params* p, NptArgs& a) {
 
 // drift threshold with the NPT safety factor on the host copy of the segments
 static void npt_threshold(System* s, const NptArgs& a) {
-  for (SegInfo& S : s->seg) {
-    float sc = 0.f;
-    for (int d = 0; d < 3; ++d) sc = std::max(sc, S.scale[d]);
-    const float allowed = ((float)a.rc_skin - (float)a.rc * (float)((double)sc * 1.02)) / 2.f;
-    const double lmin = std::min((double)S.L[0], std::min((double)S.L[1], (double)S.L[2]));
-    S.thr = allowed - (float)(lmin * 1e-6);
-  }
+  for (SegInfo& S : s->seg) seg_set_threshold(S, a.rc_skin, a.rc, true);
   s->seg_dirty = true;
 }
 
@@ -1100,7 +1096,7 @@ int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps, 
   }
   const uint64_t counter0 = s->rng_counter;
   for (int step = 0; step < nsteps; ++step) {
-    DPMD_TRY(nlist_drift_check(s));
+    DPMD_TRY(nlist_drift_check(s, npt || pnpt));
     DPMD_TRY(nlist_build(s, false, true));
     hipEvent_t ev;
     if (p->routine == DPMD_NVE) {
@@ -1207,12 +1203,13 @@ int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps, 
   int h[16];
   DPMD_TRY(nl_read_flags(s, h));
   s->nl_rebuilds += h[9];
-  uint32_t code = (uint32_t)(h[2] & (2 | 16));
+  uint32_t code = (uint32_t)(h[2] & (1 | 2 | 16));
   if (h[8]) code |= 2;
   if (error_code) *error_code |= code;
-  if (code & 2) {
-    // neighbour capacity exceeded during the chunk: restore the entry state so the caller
-    // can reallocate and re-run (deepmd_jax/md.py:914-916)
+  if (code & 3) {
+    // neighbour capacity exceeded, or (NPT) the box left the list's reference box behind,
+    // during the chunk: restore the entry state so the caller can reallocate and re-run
+    // (deepmd_jax/md.py:914-916)
     DPMD_HIP(hipMemcpyAsync(s->d_pos, s->d_pos_save, sizeof(double) * NA * 3, hipMemcpyDeviceToDevice, s->stream));
     DPMD_HIP(hipMemcpyAsync(s->d_mom, s->d_mom_save, sizeof(double) * NA * 3, hipMemcpyDeviceToDevice, s->stream));
     DPMD_HIP(hipMemcpyAsync(s->d_thermo, thermo_save.data(), sizeof(double) * thermo_save.size(),

@@ -170,6 +170,21 @@ __host__ __device__ __forceinline__ void cell_mul(const T* M, const T x[3], T y[
   for (int e = 0; e < 3; ++e) y[e] = x[0] * M[e] + x[1] * M[3 + e] + x[2] * M[6 + e];
 }
 
+// Rebuild threshold of the drift test in FP32, as deepmd_jax/md.py:724-732 evaluates it:
+// scale_max = max(L_ref / box) (x 1.02 under NPT), allowed = (f32(rc + skin) - f32(rc)
+// scale_max) / 2, threshold = allowed - L_ref,min * f32(1e-6); a list is rebuilt when an
+// atom's drift exceeds it.
+__host__ __device__ __forceinline__ void seg_set_threshold(SegInfo& S, double rc_skin, double rc, bool npt) {
+#pragma clang fp contract(off)  // one rounding per FP32 operation, no fused multiply-add
+  float sc = S.scale[0] > S.scale[1] ? S.scale[0] : S.scale[1];
+  sc = sc > S.scale[2] ? sc : S.scale[2];
+  const float safe = npt ? sc * 1.02f : sc;
+  const float allowed = ((float)rc_skin - (float)rc * safe) / 2.f;
+  float lmin = S.L[0] < S.L[1] ? S.L[0] : S.L[1];
+  lmin = lmin < S.L[2] ? lmin : S.L[2];
+  S.thr = allowed - lmin * 1e-6f;
+}
+
 struct RowBase {
   int r[kMaxT];
 };
@@ -310,7 +325,7 @@ int32_t sys_upload_segments(System* s);
 
 // kernels / launchers (return DPMD_OK or error)
 int32_t nlist_build(System* s, bool measure, bool conditional);
-int32_t nlist_drift_check(System* s);
+int32_t nlist_drift_check(System* s, bool npt = false);
 int32_t nlist_alloc_cells(System* s);
 int32_t nlist_clear_flags(System* s);
 int32_t embed_launch(System* s, bool backward, int precision);

@@ -249,10 +249,11 @@ __global__ void pimd_unwrap_cell_kernel(double* pos, int natoms, int P, const Se
 }
 
 // PIMD exact free ring-polymer step for one (atom, dim) per thread: unwrap beads onto
-// bead 0's image, rotate to normal modes, propagate, rotate back.
+// bead 0's image, rotate to normal modes, propagate, rotate back.  A position or momentum
+// that is not finite sets error bit 16 (deepmd_jax/md.py:637-640).
 __global__ void pimd_free_ring_kernel(double* pos, double* mom, const double* mass, int natoms, int P,
                                       const double* C, const double* freqs, double h, double Lx, double Ly,
-                                      double Lz) {
+                                      double Lz, int* flags) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= natoms * 3) return;
   const int i = t / 3, d = t % 3;
@@ -285,6 +286,7 @@ __global__ void pimd_free_ring_kernel(double* pos, double* mom, const double* ma
     qn[k] = qq;
     pn[k] = pp;
   }
+  bool bad = false;
   for (int b = 0; b < P; ++b) {
     double sq = 0, sp = 0;
     for (int k = 0; k < P; ++k) {
@@ -293,7 +295,9 @@ __global__ void pimd_free_ring_kernel(double* pos, double* mom, const double* ma
     }
     pos[((size_t)b * natoms + i) * 3 + d] = sq;
     mom[((size_t)b * natoms + i) * 3 + d] = sp;
+    bad |= !isfinite(sq) || !isfinite(sp);
   }
+  if (bad) atomicOr(&flags[2], 16);
 }
 
 // PILE-L O step in normal modes; work[0] accumulates the thermostat work
@@ -441,7 +445,7 @@ int32_t md_pimd_free_ring(System* s, const double* d_C, const double* d_freqs, d
   }
   const double Lx = S.tri ? 0.0 : S.boxd[0], Ly = S.tri ? 0.0 : S.boxd[1], Lz = S.tri ? 0.0 : S.boxd[2];
   hipLaunchKernelGGL(pimd_free_ring_kernel, dim3((n + 127) / 128), dim3(128), 0, s->stream, s->d_pos, s->d_mom,
-                     s->d_mass, S.natoms, s->nseg, d_C, d_freqs, h, Lx, Ly, Lz);
+                     s->d_mass, S.natoms, s->nseg, d_C, d_freqs, h, Lx, Ly, Lz, s->d_flags);
   DPMD_HIP(hipGetLastError());
   return DPMD_OK;
 }
@@ -586,19 +590,15 @@ __device__ __forceinline__ double sinhx_x(double x) {
 }
 
 __device__ __forceinline__ void npt_set_box(const NptArgs& a, double* npt, SegInfo& S) {
-  double sc = 0.0;
   for (int d = 0; d < 3; ++d) {
     const int g = a.gid[d];
     const double L = g >= 0 ? npt[12 + d] * exp(npt[g]) : npt[12 + d];
     S.boxd[d] = L;
     S.box[d] = (float)L;
     S.scale[d] = S.L[d] / (float)L;
-    sc = fmax(sc, (double)S.scale[d]);
   }
   // drift threshold with the NPT safety factor (deepmd_jax/md.py:724-728)
-  const float allowed = ((float)a.rc_skin - (float)a.rc * (float)(sc * 1.02)) / 2.f;
-  const double lmin = fmin((double)S.L[0], fmin((double)S.L[1], (double)S.L[2]));
-  S.thr = allowed - (float)(lmin * 1e-6);
+  seg_set_threshold(S, a.rc_skin, a.rc, true);
 }
 
 // dU/deps of each group from the virial diagonal (virial = -dU/dstrain): red[1 + 4 a] = W_aa

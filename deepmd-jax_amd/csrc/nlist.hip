@@ -38,6 +38,18 @@ __device__ __forceinline__ float fmod_floor(float x, float y) {
   return r;
 }
 
+// One coordinate wrapped into [0, box) in FP64, rounded to FP32 and wrapped again against
+// the FP32 box (a value that rounds up to the box becomes 0).  The reference keeps its FP32
+// positions wrapped every step, so its canonicalisation (f32(x) mod box,
+// deepmd_jax/md_utils.py:364-373) sees the correctly rounded wrapped coordinate; wrapping
+// the engine's unwrapped FP64 positions first gives that same value, where rounding an
+// unwrapped coordinate to FP32 first would lose the low bits of atoms that left the box.
+__device__ __forceinline__ float wrap_axis_f32(double x, double Ld, float Lf) {
+  double r = fmod(x, Ld);
+  if (r < 0) r += Ld;
+  return fmod_floor((float)r, Lf);
+}
+
 __device__ __forceinline__ float pair_d2(float4 a, float4 b, float Lx, float Ly, float Lz) {
   float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
   const float hx = Lx * 0.5f, hy = Ly * 0.5f, hz = Lz * 0.5f;
@@ -111,8 +123,8 @@ __global__ void nl_canon_kernel(NlArgs a) {
   } else {
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      float x = (float)a.pos[(size_t)g * 3 + d];
-      x = fmod_floor(x, S.box[d]);  // wrap in the current box, then scale to the reference box
+      // wrap in the current box, then scale to the reference box
+      float x = wrap_axis_f32(a.pos[(size_t)g * 3 + d], S.boxd[d], S.box[d]);
       x = x * S.scale[d];
       c[d] = x * shrink;
     }
@@ -122,8 +134,10 @@ __global__ void nl_canon_kernel(NlArgs a) {
     int ci[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      int v = S.tri ? (int)(c[d] * (float)S.nc[d]) : (int)(c[d] * ((float)S.nc[d] / S.L[d]));
-      ci[d] = v < 0 ? 0 : (v >= S.nc[d] ? S.nc[d] - 1 : v);
+      const float u = S.tri ? c[d] * (float)S.nc[d] : c[d] * ((float)S.nc[d] / S.L[d]);
+      // clamped before the conversion, so a non-finite position (error bit 16 ends the
+      // run) lands in cell 0 instead of an undefined integer
+      ci[d] = u >= 0.f && u < (float)S.nc[d] ? (int)u : (u >= (float)S.nc[d] ? S.nc[d] - 1 : 0);
     }
     const int cell = (ci[2] * S.nc[1] + ci[1]) * S.nc[0] + ci[0];
     a.atom_cell[g] = cell;
@@ -260,12 +274,24 @@ __global__ void __launch_bounds__(256) nl_build_kernel(NlArgs a) {
 }
 
 // drift of every atom against the canonical coordinates of the last build; sets
-// flags[1] when any atom of any segment moved beyond its segment's threshold
+// flags[1] when any atom of any segment moved beyond its segment's threshold.  With
+// shrink_max > 0 (NPT) the thread of a segment's first atom also runs the box test of
+// deepmd_jax/md.py:630-636 in FP32: the list's reference box no longer bounds the cutoff
+// when max(1 - box / L_ref) > shrink_max (0.8 skin / (rc + skin)) or min(box) < box_min
+// (2 (rc + skin) 1.02); that sets error bit 1, which the caller resolves by widening the
+// skin and reallocating.
 __global__ void nl_drift_kernel(const double* pos, const float4* canon, const SegInfo* seg, const int* atom_seg,
-                                int total, int* flags) {
+                                int total, int* flags, float shrink_max, float box_min) {
   bool over = false;
   for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += gridDim.x * blockDim.x) {
     const SegInfo& S = seg[atom_seg[g]];
+    if (shrink_max > 0.f && g == S.base) {
+      float shrink = 1.f - S.box[0] / S.L[0];
+      shrink = fmaxf(shrink, 1.f - S.box[1] / S.L[1]);
+      shrink = fmaxf(shrink, 1.f - S.box[2] / S.L[2]);
+      const float bmin = fminf(S.box[0], fminf(S.box[1], S.box[2]));
+      if (shrink > shrink_max || bmin < box_min) atomicOr(&flags[2], 1);
+    }
     const float4 r = canon[g];
     const float rv[3] = {r.x, r.y, r.z};
     float d[3];
@@ -281,8 +307,7 @@ __global__ void nl_drift_kernel(const double* pos, const float4* canon, const Se
     } else {
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        float x = (float)pos[(size_t)g * 3 + k];
-        x = fmod_floor(x, S.box[k]) * S.scale[k];
+        const float x = wrap_axis_f32(pos[(size_t)g * 3 + k], S.boxd[k], S.box[k]) * S.scale[k];
         float dd = x - rv[k];
         const float h = S.L[k] / 2.f;
         dd = fmod_floor(dd - h, S.L[k]) - h;
@@ -355,12 +380,7 @@ int32_t nlist_alloc_cells(System* s) {
     if (!cells) S.nc[0] = S.nc[1] = S.nc[2] = 1;
     S.cell0 = cell0;
     cell0 += cells ? S.nc[0] * S.nc[1] * S.nc[2] : 0;
-    // deepmd_jax/md.py:693-741: ((rc+skin) - rc*scale)/2 - 1e-6 L_min
-    float sc = 0.f;
-    for (int d = 0; d < 3; ++d) sc = std::max(sc, S.scale[d]);
-    const float allowed = ((float)s->rc_skin - (float)rc * sc) / 2.f;
-    const double lmin = std::min((double)S.L[0], std::min((double)S.L[1], (double)S.L[2]));
-    S.thr = allowed - (float)(lmin * 1e-6);
+    seg_set_threshold(S, s->rc_skin, rc, false);
   }
   s->ncell_total = std::max(1, cell0);
   s->seg_dirty = true;
@@ -399,11 +419,14 @@ int32_t nlist_build(System* s, bool measure, bool conditional) {
   return DPMD_OK;
 }
 
-int32_t nlist_drift_check(System* s) {
+int32_t nlist_drift_check(System* s, bool npt) {
   const int NA = s->NA;
   const int blocks = std::min(1024, (NA + 255) / 256);
+  const double skin = s->rc_skin - s->model->rcut;
+  const float shrink_max = npt ? (float)(0.8 * skin / s->rc_skin) : 0.f;
+  const float box_min = npt ? (float)(2.0 * s->rc_skin * 1.02) : 0.f;
   hipLaunchKernelGGL(nl_drift_kernel, dim3(blocks), dim3(256), 0, s->stream, s->d_pos, s->d_canon, s->d_seg,
-                     s->d_atom_seg, NA, s->d_flags);
+                     s->d_atom_seg, NA, s->d_flags, shrink_max, box_min);
   hipLaunchKernelGGL(nl_decide_kernel, dim3(1), dim3(1), 0, s->stream, s->d_flags);
   DPMD_HIP(hipGetLastError());
   return DPMD_OK;

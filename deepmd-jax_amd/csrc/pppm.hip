@@ -99,7 +99,7 @@ __device__ __forceinline__ void tsc(const PppmGeom& g, const double* box, const 
     const double u = xm / box[d] * g.M[d];
     const double c = rint(u);
     const double r = u - c;
-    t.c[d] = (int)c;
+    t.c[d] = isfinite(c) ? (int)c : 0;  // a non-finite position (bit 16 ends the run) bins to 0
     t.w[d][0] = 0.5 * (r - 0.5) * (r - 0.5);
     t.w[d][1] = 0.75 - r * r;
     t.w[d][2] = 0.5 * (r + 0.5) * (r + 0.5);

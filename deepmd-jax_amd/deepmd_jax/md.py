@@ -523,12 +523,27 @@ class Simulation:
         self._is_initial_state = False
 
     def _resolve_overflow(self):
-        """deepmd_jax/md.py:644-676: reallocate; grow the buffer ratio on a repeat."""
-        if self._error_code & 4:
-            self._neighbor_buffer_ratio += 0.05
-        self._allocate_nlist()
+        """deepmd_jax/md.py:644-676.  Bit 1 (NPT: the box shrank past what the list's skin
+        covers): widen the skin by 0.4 A and rebuild the list on the current box.  Bit 2:
+        reallocate, growing the buffer ratio when bit 4 says an overflow was already resolved
+        in this run.  Bit 4 then stays set until the next run() (md.py:870)."""
+        flag4 = False
+        if self._error_code & 1:
+            self._neighbor_skin += 0.4
+            box = np.asarray(self.getBox(), np.float64).reshape(-1)
+            if box.size == 3 and box.min() > 2 * (self._rcut + self._neighbor_skin) * 1.02:
+                self._allocate_nlist()
+            else:
+                raise NotImplementedError('NPT box shrank to %s: below 2*(rcut+skin)*1.02 = %.3f A with the widened '
+                                          'skin, outside the neighbour-list regime this engine runs NPT in'
+                                          % (box, 2 * (self._rcut + self._neighbor_skin) * 1.02))
+        elif self._error_code & 2:
+            if self._error_code & 4:
+                self._neighbor_buffer_ratio += 0.05
+            self._allocate_nlist()
+            flag4 = True
         print('# Successfully resolved overflow code', self._error_code)
-        self._error_code = 4
+        self._error_code = 4 if flag4 else 0
 
     def _put_frame(self, writer, pos, vel, box, step):
         """Queue one frame ((P,N,3) positions and velocities) for the dump files, as float32
@@ -572,12 +587,10 @@ class Simulation:
                 print('# Warning: Nan or Inf encountered in simulation. Terminating.')
                 self._error_code |= 16
                 remaining = 0
-            if err & 2:
-                self._error_code |= 2
+            if err & 3:
+                self._error_code |= err & 3
                 self._resolve_overflow()
                 continue
-            if not err & 16:
-                self._error_code &= ~4
             boxes = self._sys.dev.box_trajectory(nxt)[:, 0] if self._routine == 'NPT' else None
             if writer is not None:
                 for k in range(nxt):

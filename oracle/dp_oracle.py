@@ -466,10 +466,15 @@ def energy_force_virial(params, variables, coord, box3, type_idx, nbrs_nm=None, 
 # Brute force over all pairs in float32, op order (Ra - Rb), ((dx^2+dy^2)+dz^2).
 
 def canonicalize_f32(coord, box, reference_box=None):
-    box = np.asarray(box, np.float32).reshape(-1)[:3]
+    """(f32(x) mod box) * scale * f32(1 - 5e-7) (deepmd_jax/md_utils.py:364-373) on the
+    FP32 wrapped coordinate: x is wrapped in FP64 first, as the reference's positions are
+    kept wrapped by its integrator."""
+    box64 = np.asarray(box, np.float64).reshape(-1)[:3]
+    box = box64.astype(np.float32)
     ref = box if reference_box is None else np.asarray(reference_box, np.float32)
     scale = (ref / box).astype(np.float32)
-    c = np.remainder(np.asarray(coord, np.float32), box).astype(np.float32) * scale
+    c = np.remainder(np.remainder(np.asarray(coord, np.float64), box64).astype(np.float32), box)
+    c = c.astype(np.float32) * scale
     return (c * np.float32(1 - 5e-7)).astype(np.float32)
 
 
@@ -483,9 +488,15 @@ def pair_d2_f32(ca, cb, side):
     return ((sq[..., 0] + sq[..., 1]).astype(np.float32) + sq[..., 2]).astype(np.float32)
 
 
-def neighbor_sets(coord, box, type_idx, rc_skin, reference_box=None):
+def neighbor_sets(coord, box, type_idx, rc_skin, reference_box=None, fast=None):
     """Per centre (type-sorted order): sorted arrays of type-sorted neighbour indices.
-    Returns (sorted_coord_f32, list of index arrays, d2 matrix rows' borderline count)."""
+    Returns (sorted_coord_f32, list of index arrays, d2 matrix rows' borderline count).
+
+    The decision is always the FP32 test d2 < f32(rc_skin)^2 of pair_d2_f32.  The brute
+    force applies it to every pair; fast=True (default above 4,000 atoms) applies it to
+    the candidates scipy's periodic KD-tree finds within rc_skin (1 + 1e-4) + 1e-3 of the
+    FP64 values of the same FP32 canonical coordinates, a superset of every pair the FP32
+    test can accept (its rounding error is ~1e-6 relative)."""
     type_idx = np.asarray(type_idx)
     perm = np.argsort(type_idx, kind='stable')
     c = canonicalize_f32(np.asarray(coord)[perm], box, reference_box)
@@ -493,8 +504,27 @@ def neighbor_sets(coord, box, type_idx, rc_skin, reference_box=None):
     cut = np.float32(rc_skin)
     cut_sq = np.float32(cut * cut)
     N = len(c)
+    if fast is None:
+        fast = N > 4000
     rows = []
     border = 0
+    if fast:
+        from scipy.spatial import cKDTree
+        c64 = c.astype(np.float64)
+        bs = side.astype(np.float64)
+        tree = cKDTree(np.mod(c64, bs), boxsize=bs)
+        pairs = tree.query_pairs(float(rc_skin) * (1 + 1e-4) + 1e-3, output_type='ndarray')
+        a = np.concatenate([pairs[:, 0], pairs[:, 1]])
+        b = np.concatenate([pairs[:, 1], pairs[:, 0]])
+        d2 = pair_d2_f32(c[a], c[b], side)
+        keep = d2 < cut_sq
+        border = int((np.abs(d2.astype(np.float64) - float(cut_sq)) < 4e-6 * float(cut_sq)).sum())
+        a, b = a[keep], b[keep]
+        order = np.lexsort((b, a))
+        a, b = a[order], b[order]
+        bounds = np.searchsorted(a, np.arange(N + 1))
+        rows = [b[bounds[i]:bounds[i + 1]] for i in range(N)]
+        return c, rows, border
     for i in range(N):
         d2 = pair_d2_f32(c[i][None], c, side)
         m = d2 < cut_sq
@@ -530,9 +560,12 @@ def knbr_from_counts(kmax, buffer_ratio=1.2):
     return [int(v) for v in kb.astype(int)]
 
 
-def typed_nlist(coord, box, type_idx, rc_skin, ntypes=2, knbr=None, buffer_ratio=1.2):
-    """nbrs_nm[i][j] (N_i, knbr_j-1) int32, padding N; overflow flag; knbr.  box: (3,)
-    lengths, or a (3,3) cell (general cells take neighbor_sets_cell)."""
+def typed_nlist(coord, box, type_idx, rc_skin, ntypes=2, knbr=None, buffer_ratio=1.2, reference_box=None):
+    """nbrs_nm[i][j] (N_i, knbr_j-1) int32, padding N; overflow flag; knbr
+    (deepmd_jax/md_utils.py:393-459: per neighbour type the ascending first knbr_j - 1
+    type-local indices, pad N).  box: (3,) lengths, or a (3,3) cell (general cells take
+    neighbor_sets_cell); reference_box: the list's box when it differs from the current one
+    (canonical coordinates scaled by reference_box / box)."""
     type_idx = np.asarray(type_idx)
     N = len(type_idx)
     tc = np.bincount(type_idx, minlength=ntypes)
@@ -541,21 +574,27 @@ def typed_nlist(coord, box, type_idx, rc_skin, ntypes=2, knbr=None, buffer_ratio
     if b.size == 9 and np.any(b.reshape(3, 3) - np.diag(np.diag(b.reshape(3, 3)))):
         _, rows = neighbor_sets_cell(coord, b, type_idx, rc_skin)
     else:
-        _, rows, _ = neighbor_sets(coord, np.diag(b.reshape(3, 3)) if b.size == 9 else b, type_idx, rc_skin)
+        _, rows, _ = neighbor_sets(coord, np.diag(b.reshape(3, 3)) if b.size == 9 else b, type_idx, rc_skin,
+                                   reference_box=reference_box)
+    lens = np.array([len(r) for r in rows])
+    flat = np.concatenate(rows) if N else np.zeros(0, int)
+    centre = np.repeat(np.arange(N), lens)
+    ntype = np.searchsorted(off, flat, side='right') - 1          # neighbour type of each entry
     counts = np.zeros((N, ntypes), int)
-    for a, r in enumerate(rows):
-        for j in range(ntypes):
-            counts[a, j] = ((r >= off[j]) & (r < off[j + 1])).sum()
+    np.add.at(counts, (centre, ntype), 1)
     if knbr is None:
         knbr = knbr_from_counts(counts.max(0), buffer_ratio)
     overflow = bool((counts >= np.asarray(knbr)[None]).any())
+    # rank of each entry among its centre's neighbours of the same type (rows are ascending)
+    first = np.zeros((N, ntypes), int)
+    starts = np.concatenate([[0], np.cumsum(lens)[:-1]]) if N else np.zeros(0, int)
+    cum = np.cumsum(counts, 1) - counts                              # entries of lower types
+    rank = np.arange(len(flat)) - starts[centre] - cum[centre, ntype] if len(flat) else flat
     full = []
     for j in range(ntypes):
         arr = np.full((N, knbr[j] - 1), N, np.int32)
-        for a, r in enumerate(rows):
-            rj = r[(r >= off[j]) & (r < off[j + 1])] - off[j]
-            rj = rj[:knbr[j] - 1]
-            arr[a, :len(rj)] = rj
+        sel = (ntype == j) & (rank < knbr[j] - 1)
+        arr[centre[sel], rank[sel]] = flat[sel] - off[j]
         full.append(arr)
     nbrs_nm = [[full[j][off[i]:off[i + 1]] for j in range(ntypes)] for i in range(ntypes)]
     return nbrs_nm, overflow, knbr

@@ -438,3 +438,36 @@ def test_single_pair_energy_closed_form():
     for order in ([0, 1], [1, 0]):          # atom order in the frame does not matter
         E0 = O.dp_energy(params, var, torch.as_tensor(coord[order]), box, types[order], lattice=lat).item()
         assert abs(E0 - E) < 1e-12 * max(1.0, abs(E)), (E0, E)
+
+
+@pytest.mark.parametrize('rc_skin,factor', [(7.0, 1.0), (6.3, 1.0), (7.0, 1.004)])
+def test_nlist_fast_path_equals_brute_force(rc_skin, factor):
+    """The KD-tree candidate path of neighbor_sets gives the brute force's FP32 sets, and
+    typed_nlist's rows equal a per-row restatement of get_nm (md_utils.py:431-459)."""
+    coord, box, types = _small(512, seed=5)
+    nbox = box * factor
+    x = coord * factor
+    ref_box = None if factor == 1.0 else box
+    _, r1, _ = O.neighbor_sets(x, nbox, types, rc_skin, reference_box=ref_box, fast=False)
+    _, r2, _ = O.neighbor_sets(x, nbox, types, rc_skin, reference_box=ref_box, fast=True)
+    assert all(np.array_equal(a, b) for a, b in zip(r1, r2))
+    nb, _, knbr = O.typed_nlist(x, nbox, types, rc_skin, reference_box=ref_box)
+    N = len(types)
+    off = np.concatenate([[0], np.cumsum(np.bincount(types))])
+    for i in range(2):
+        for j in range(2):
+            for a in range(off[i], off[i + 1]):
+                r = r1[a][(r1[a] >= off[j]) & (r1[a] < off[j + 1])] - off[j]
+                row = np.full(knbr[j] - 1, N)
+                row[:min(len(r), knbr[j] - 1)] = r[:knbr[j] - 1]
+                np.testing.assert_array_equal(nb[i][j][a - off[i]], row)
+
+
+def test_canonicalize_wraps_in_fp64_first():
+    """An unwrapped coordinate canonicalises to the FP32 value of its wrapped FP64 position,
+    as the reference's FP32 positions (kept wrapped every step) do."""
+    box = np.full(3, 24.83)
+    x = np.array([[3.123456789, 12.5, 0.001]])
+    for k in (-3, -1, 1, 4):
+        u = x + k * box
+        np.testing.assert_array_equal(O.canonicalize_f32(u, box), O.canonicalize_f32(x, box))
