@@ -88,15 +88,21 @@ def test_npt_invariant_conserved_on_device(device_model, couple):
     def H():
         e, _, _, _ = s.compute('fp32')
         return e[0] + s.kinetic_energy()[0] + s.thermostat_energy()[0] + (G - 1) * KT300 * s.npt_state()[16]
+    rc_skin = [6.5]
+
     def run(n):
-        # the compressing box can overflow the neighbour rows: reallocate and re-run the
-        # chunk from its entry state, as Simulation.run does (deepmd_jax/md.py:644-676)
-        for _ in range(4):
+        # the compressing box can overflow the neighbour rows (bit 2) or shrink past what the
+        # list's skin covers (bit 1): widen the skin by 0.4 A on bit 1, reallocate on the
+        # current box and re-run the chunk from its entry state, as Simulation.run does
+        # (deepmd_jax/md.py:644-676)
+        for _ in range(6):
             _, _, err = s.md_run(p, n, record=False)
-            if not err & 2:
+            if not err & 3:
                 assert err == 0
                 return
-            s.nlist_allocate(6.5, 1.3)
+            if err & 1:
+                rc_skin[0] += 0.4
+            s.nlist_allocate(rc_skin[0], 1.3)
         raise AssertionError('neighbour list kept overflowing')
     run(1)
     h0, b0 = H(), s.get_boxes()[0]
