@@ -7,6 +7,7 @@
 #include <stdio.h>
 #include <string.h>
 #include <algorithm>
+#include <mutex>
 #include <numeric>
 #include <string>
 
@@ -1406,3 +1407,160 @@ int32_t dpmd_synchronize(dpmd_system_t sys) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------------
+// Allocation-free evaluation (dpmd_compute_noalloc).  The per-evaluation work buffers --
+// T, G, dE/dG, the seven fitting activations, per-row energies, per-block virial partials,
+// per-segment sums and the caller-order outputs -- live in one file-scope device array
+// instead of the system's own buffers, so the call makes no device allocation.  Every GPU
+// the library runs on holds its own copy of the array until the process ends; its address
+// on a device comes from hipGetSymbolAddress (the array's name is a host-side stand-in,
+// never a device address).  The array keeps the last call's contents: the regions that
+// must read as zero (padding rows of G, K-padding columns of the activations) are cleared
+// by a fill kernel on every call.  Calls that overlapped on one GPU would race on the
+// array; a mutex serialises the calls of one process.
+
+namespace dpmd {
+
+constexpr size_t kScratchBytes = (size_t)256 << 20;
+constexpr int kScratchDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch_ptr[kScratchDevices];  // per device, guarded by g_scratch_mu
+
+// device address of `scratch` on the current device (set up on the first call there)
+static int32_t scratch_device_ptr(int device, unsigned char** out) {
+  if (device < 0 || device >= kScratchDevices) {
+    set_error("device %d outside the %d devices the scratch table covers", device, kScratchDevices);
+    return DPMD_E_ARG;
+  }
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error("hipGetSymbolAddress(scratch) failed on device %d: %s", device, hipGetErrorString(e));
+      return DPMD_E_HIP;
+    }
+    g_scratch_ptr[device] = static_cast<unsigned char*>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return DPMD_OK;
+}
+
+__global__ void scratch_zero_kernel(float4* x, size_t n4) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x)
+    x[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// forces to caller order in FP64; per-segment energy and virial out of the 16-double sums
+__global__ void scratch_out_kernel(const float* force, const int* perm, int NA, const double* red, int nseg,
+                                   double* fout, double* eout, double* vout) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < NA * 3) {
+    const int g = i / 3, d = i - 3 * g;
+    fout[(size_t)perm[g] * 3 + d] = (double)force[i];
+  }
+  if (i < nseg) {
+    eout[i] = red[(size_t)i * 16];
+    for (int k = 0; k < 9; ++k) vout[(size_t)i * 9 + k] = red[(size_t)i * 16 + 1 + k];
+  }
+}
+
+// The system's work-buffer pointers, pointed into the scratch array for one call and
+// restored on every exit path.
+struct WorkBuffers {
+  System* s;
+  float* T;
+  void* G;
+  float* dG;
+  float* fit;
+  float* eatom;
+  double* partial;
+  double* red;
+  explicit WorkBuffers(System* sys)
+      : s(sys), T(sys->d_T), G(sys->d_G), dG(sys->d_dG), fit(sys->d_fitbuf), eatom(sys->d_eatom),
+        partial(sys->d_partial), red(sys->d_red) {}
+  ~WorkBuffers() {
+    s->d_T = T;
+    s->d_G = G;
+    s->d_dG = dG;
+    s->d_fitbuf = fit;
+    s->d_eatom = eatom;
+    s->d_partial = partial;
+    s->d_red = red;
+  }
+};
+
+}  // namespace dpmd
+
+extern "C" int32_t dpmd_compute_noalloc(dpmd_system_t sys, int32_t precision, double* energy, double* force,
+                                        double* virial) {
+  SYS(sys);
+  if (precision != DPMD_PREC_FP32 && precision != DPMD_PREC_BF16 && precision != DPMD_PREC_FP16) {
+    set_error("unknown precision %d", precision);
+    return DPMD_E_ARG;
+  }
+  if (!s->nl_ready) { set_error("no neighbour list: call dpmd_nlist_allocate first"); return DPMD_E_STATE; }
+  if (s->wc_model) {
+    set_error("dpmd_compute_noalloc does not evaluate DPLR systems (their PPPM plans allocate)");
+    return DPMD_E_UNSUPPORTED;
+  }
+  const Model* M = s->model;
+  const size_t NA = s->NA, W = M->wout, D = (size_t)M->axis * M->wout;
+  const size_t mr = std::max(1, s->mrows);
+  // scratch layout, every region 256-byte aligned
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = off;
+    off += (bytes + 255) / 256 * 256;
+    return o;
+  };
+  const size_t oT = take(sizeof(float) * NA * 10 * W);
+  const size_t oG = take(sizeof(float) * mr * D);
+  const size_t odG = take(sizeof(float) * mr * D);
+  const size_t oFit = take(sizeof(float) * 7 * mr * s->fstride);
+  const size_t oE = take(sizeof(float) * mr);
+  const size_t oP = take(sizeof(double) * 9 * (size_t)s->nblocks);
+  const size_t oR = take(sizeof(double) * (32 * (size_t)s->nseg + 64));
+  const size_t oF = take(sizeof(double) * NA * 3);
+  const size_t oEo = take(sizeof(double) * s->nseg);
+  const size_t oVo = take(sizeof(double) * 9 * s->nseg);
+  if (off > kScratchBytes) {
+    set_error("system needs %zu bytes of evaluation scratch; the allocation-free entry point holds at most %zu "
+              "(about %zu atoms of this model)", off, kScratchBytes, kScratchBytes / std::max<size_t>(off / NA, 1));
+    return DPMD_E_ARG;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  unsigned char* base = nullptr;
+  DPMD_TRY(scratch_device_ptr(M->device, &base));
+  WorkBuffers keep(s);
+  s->d_T = reinterpret_cast<float*>(base + oT);
+  s->d_G = base + oG;
+  s->d_dG = reinterpret_cast<float*>(base + odG);
+  s->d_fitbuf = reinterpret_cast<float*>(base + oFit);
+  s->d_eatom = reinterpret_cast<float*>(base + oE);
+  s->d_partial = reinterpret_cast<double*>(base + oP);
+  s->d_red = reinterpret_cast<double*>(base + oR);
+  // G and the activations must start at zero (padding rows and K-padding columns)
+  const size_t n4 = (oE - oG) / 16;  // G, dE/dG and the activations: one contiguous span
+  hipLaunchKernelGGL(scratch_zero_kernel, dim3((unsigned)std::min<size_t>(4096, (n4 + 255) / 256)), dim3(256), 0,
+                     s->stream, reinterpret_cast<float4*>(base + oG), n4);
+  DPMD_HIP(hipGetLastError());
+  DPMD_TRY(md_write_pos32(s));
+  DPMD_TRY(model_forces_impl(s, precision));
+  DPMD_TRY(md_reduce_launch(s));
+  double* fout = reinterpret_cast<double*>(base + oF);
+  double* eout = reinterpret_cast<double*>(base + oEo);
+  double* vout = reinterpret_cast<double*>(base + oVo);
+  const int nthr = (int)std::max(NA * 3, (size_t)s->nseg);
+  hipLaunchKernelGGL(scratch_out_kernel, dim3((nthr + 255) / 256), dim3(256), 0, s->stream, s->d_force, s->d_perm,
+                     (int)NA, s->d_red, s->nseg, fout, eout, vout);
+  DPMD_HIP(hipGetLastError());
+  if (force) DPMD_HIP(hipMemcpyAsync(force, fout, sizeof(double) * NA * 3, hipMemcpyDeviceToHost, s->stream));
+  if (energy) DPMD_HIP(hipMemcpyAsync(energy, eout, sizeof(double) * s->nseg, hipMemcpyDeviceToHost, s->stream));
+  if (virial) DPMD_HIP(hipMemcpyAsync(virial, vout, sizeof(double) * 9 * s->nseg, hipMemcpyDeviceToHost, s->stream));
+  DPMD_HIP(hipStreamSynchronize(s->stream));
+  if (energy)
+    for (int r = 0; r < s->nseg; ++r) energy[r] += seg_ebias(s, r);
+  return DPMD_OK;
+}

@@ -79,6 +79,7 @@ def lib():
         'dpmd_nlist_export': (c_i32, [c_vp, c_i32, c_vp]),
         'dpmd_nlist_capacity': (c_i32, [c_vp, c_vp]),
         'dpmd_compute': (c_i32, [c_vp, c_i32, c_vp, c_vp, c_vp, c_vp]),
+        'dpmd_compute_noalloc': (c_i32, [c_vp, c_i32, c_vp, c_vp, c_vp]),
         'dpmd_md_run': (c_i32, [c_vp, P(MDParams), c_i32, c_vp, c_vp, c_vp]),
         'dpmd_pimd_setup': (c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp]),
         'dpmd_kinetic_energy': (c_i32, [c_vp, c_vp]),
@@ -137,7 +138,7 @@ def exported_symbols():
             'dpmd_timing_get', 'dpmd_pair_count', 'dpmd_synchronize', 'dpmd_system_get_boxes',
             'dpmd_md_box_trajectory', 'dpmd_npt_state_get', 'dpmd_nlist_rebuilds', 'dpmd_system_set_fixed',
             'dpmd_pppm_compute', 'dpmd_system_set_dplr', 'dpmd_dplr_get', 'dpmd_atomic_predict',
-            'dpmd_nlist_update_rate']
+            'dpmd_nlist_update_rate', 'dpmd_compute_noalloc']
 
 
 def pppm_compute(box3, beta, grid, xyz, q, device=0):
@@ -418,6 +419,15 @@ class DeviceSystem:
         ea = np.zeros(self._shape()) if want_atom else None
         check(lib().dpmd_compute(self.handle, PREC[precision], ptr(e), ptr(f), ptr(v), ptr(ea)))
         return e, f, v.reshape(self.nrep, 3, 3), ea
+
+    def compute_noalloc(self, precision='fp32'):
+        """dpmd_compute_noalloc: energy, forces and virial with the work buffers in the
+        library's per-GPU scratch array (no device allocation in the call)."""
+        e = np.zeros(self.nrep)
+        f = np.zeros(self._shape(3))
+        v = np.zeros((self.nrep, 9))
+        check(lib().dpmd_compute_noalloc(self.handle, PREC[precision], ptr(e), ptr(f), ptr(v)))
+        return e, f, v.reshape(self.nrep, 3, 3)
 
     def md_run(self, params, nsteps, record=True, out=None):
         """Run nsteps device steps; with record, the per-step positions and velocities in

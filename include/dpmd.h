@@ -217,6 +217,20 @@ int32_t dpmd_nlist_capacity(dpmd_system_t sys, int32_t* knbr);
 int32_t dpmd_compute(dpmd_system_t sys, int32_t precision, double* energy, double* force,
                      double* virial, double* atom_energy);
 
+/* Allocation-free variant of dpmd_compute (same energy, force and virial outputs, no
+ * per-atom energies).  It has no counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The call makes no device allocation: its work
+ * buffers (T, G, dE/dG, fitting activations, partial sums, staged outputs) live in one
+ * 256 MiB library-owned device array per GPU, held from the library's load until the
+ * process ends, instead of the system's own buffers.  A system whose work needs more
+ * (about 10,000 atoms of the 25/50/100, 240x3, axis-16 model) gets DPMD_E_ARG; DPLR
+ * systems get DPMD_E_UNSUPPORTED.  Every call on a GPU shares that GPU's array and
+ * leaves its contents behind; calls that overlap on one GPU would race on it, so the
+ * calls of one process are serialised. */
+int32_t dpmd_compute_noalloc(dpmd_system_t sys, int32_t precision, double* energy, double* force,
+                             double* virial);
+
 /* nsteps MD steps fully on the device: per step the drift test and conditional
  * neighbour rebuild (deepmd_jax/md.py:693-741), the integrator step with one force
  * evaluation, minimum-image unwrap (deepmd_jax/md_utils.py:162-169).  Replaces the
