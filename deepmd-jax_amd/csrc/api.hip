@@ -455,7 +455,10 @@ static int32_t model_forces_impl(System* s, int precision) {
   DPMD_TRY(sys_upload_segments(s));
   s->force_valid = true;
   s->force_prec = precision;
-  DPMD_TRY(md_zero_force(s));
+  // deterministic forces: the backward stores pair gradients, a gather sums them per atom
+  // (no DPLR: its long-range and centroid terms still add forces by atomics)
+  const bool det = s->deterministic && !s->wc_model && s->d_pgrad;
+  if (!det) DPMD_TRY(md_zero_force(s));
   timer_begin(s, 1, &ev);
   DPMD_TRY(embed_launch(s, false, precision));
   timer_end(s, 1, ev);
@@ -464,6 +467,7 @@ static int32_t model_forces_impl(System* s, int precision) {
   timer_end(s, 2, ev);
   timer_begin(s, 3, &ev);
   DPMD_TRY(embed_launch(s, true, precision));
+  if (det) DPMD_TRY(force_gather(s));
   timer_end(s, 3, ev);
   // DPLR long-range term (centroid model FP32 whatever the energy model's precision)
   if (s->wc_model) DPMD_TRY(dplr_forces(s));
@@ -475,6 +479,18 @@ static int32_t model_forces_impl(System* s, int precision) {
 static int32_t nl_read_flags(System* s, int* h) {
   DPMD_HIP(hipMemcpyAsync(h, s->d_flags, sizeof(int) * 16, hipMemcpyDeviceToHost, s->stream));
   DPMD_HIP(hipStreamSynchronize(s->stream));
+  return DPMD_OK;
+}
+
+// buffers of the deterministic force gather for the current capacities
+static int32_t det_buffers(System* s) {
+  size_t ksum = 0;
+  for (int j = 0; j < s->ntypes; ++j) {
+    ksum += s->cap[j];
+    DPMD_TRY(sys_ensure(s, (void**)&s->d_rev[j], sizeof(int) * (size_t)s->NA * s->cap[j]));
+  }
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_pgrad, sizeof(float) * (size_t)s->NA * ksum * 3));
+  DPMD_TRY(sys_ensure(s, (void**)&s->d_fcen, sizeof(float) * (size_t)s->NA * 3));
   return DPMD_OK;
 }
 
@@ -529,6 +545,7 @@ static int32_t nl_allocate(System* s, double rc_skin, double ratio, const int32_
     DPMD_TRY(sys_ensure(s, (void**)&s->d_nbr[j], sizeof(int) * (size_t)s->NA * s->cap[j]));
     DPMD_TRY(sys_ensure(s, (void**)&s->d_nbr_cnt[j], sizeof(int) * (size_t)s->NA));
   }
+  if (s->deterministic) DPMD_TRY(det_buffers(s));
   while (s->nl_lds_list < ksum + 64) s->nl_lds_list *= 2;
   DPMD_TRY(nlist_clear_flags(s));
   DPMD_TRY(nlist_build(s, false, false));
@@ -923,6 +940,20 @@ int32_t dpmd_nlist_capacity(dpmd_system_t sys, int32_t* knbr) {
   SYS(sys);
   if (!knbr) { set_error("null argument"); return DPMD_E_ARG; }
   for (int j = 0; j < s->ntypes; ++j) knbr[j] = s->knbr[j];
+  return DPMD_OK;
+}
+
+int32_t dpmd_system_set_deterministic(dpmd_system_t sys, int32_t on) {
+  SYS(sys);
+  const bool want = on != 0;
+  if (want == s->deterministic) return DPMD_OK;
+  s->deterministic = want;
+  s->force_valid = false;
+  if (want && s->nl_ready) {
+    DPMD_TRY(det_buffers(s));
+    DPMD_TRY(nlist_reverse(s, false));
+    DPMD_HIP(hipStreamSynchronize(s->stream));
+  }
   return DPMD_OK;
 }
 

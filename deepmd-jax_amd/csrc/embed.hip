@@ -189,6 +189,11 @@ struct EmbedArgs {
   int wc_ldy;
   float wc_norm;
   int centre_mask;             // bit t: centres of type t are evaluated (an atomic model's nsel)
+  // deterministic forces (backward): pair gradients by skin slot and centre parts instead of
+  // force atomics; pgrad row of centre g and neighbour type j starts at (g ksum + koff[j]) 3
+  float* pgrad;                // (NA, ksum, 3) or null (atomics)
+  float* fcen;                 // (NA, 3)
+  int ksum, koff[kMaxT];
 };
 
 // Minimum image of d = x_j - x_i for positions wrapped into [0, L) (so |d| < L): the
@@ -432,9 +437,11 @@ __device__ __forceinline__ float* load_small_tables(const EmbedArgs& a, float* s
 }
 
 // Compact the skin row of neighbour type j to the pairs within rcut, in ascending
-// index order, into the wave's LDS list (float4: x, y, z, global index bits).
+// index order, into the wave's LDS list (float4: x, y, z, global index bits).  With zrow
+// (deterministic backward) the fourth word is the pair's skin slot instead, and the slots
+// of dropped pairs get a zero gradient in zrow.
 __device__ __forceinline__ int compact_row(const EmbedArgs& a, const SegInfo& S, int c, int j, float4 pc,
-                                           float4* list, int lane) {
+                                           float4* list, int lane, float* zrow = nullptr) {
   const int gc = S.base + c;
   const int cnt = a.nbr_cnt[j][gc];
   const int* row = a.nbr[j] + (size_t)gc * a.cap[j];
@@ -463,7 +470,12 @@ __device__ __forceinline__ int compact_row(const EmbedArgs& a, const SegInfo& S,
       }
       const float r = pair_r(dx, dy, dz);
       keep = (r > 1e-14f) && (r / a.rcut < 1.f);
-      out = make_float4(dx, dy, dz, __int_as_float(g));
+      out = make_float4(dx, dy, dz, __int_as_float(zrow ? e : g));
+      if (zrow && !keep) {
+        zrow[3 * e] = 0.f;
+        zrow[3 * e + 1] = 0.f;
+        zrow[3 * e + 2] = 0.f;
+      }
     }
     const unsigned long long bal = __ballot(keep);
     const int pre = __popcll(bal & ((1ull << lane) - 1ull));
@@ -826,7 +838,8 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
     const float* dTc2 = dTp + (8 + q < 10 ? 8 + q : 10) * HS + n;
     float fc[3] = {0.f, 0.f, 0.f};
     for (int j = 0; j < a.ntypes; ++j) {
-      const int m = compact_row(a, S, c, j, pc, list, lane);
+      float* prow = a.pgrad ? a.pgrad + ((size_t)gc * a.ksum + a.koff[j]) * 3 : nullptr;
+      const int m = compact_row(a, S, c, j, pc, list, lane, prow);
       wave_sync();
       const int netb = __builtin_amdgcn_readfirstlane((ti * a.ntypes + j) * a.net_stride * 4);
       constexpr int O_K0 = 0, O_B0 = 16 * T1, O_B1 = 32 * T1, O_B2 = 32 * T1 + 16 * T2;
@@ -1029,10 +1042,11 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
         float gx[3];
         env_backward(g, dR, dst, a.inv_std[ti], gx);
         if (valid && q == 0) {
-          const int gj = __float_as_int(e.w);
+          const int gj = __float_as_int(e.w);  // skin slot when prow is set
 #pragma unroll
           for (int k = 0; k < 3; ++k) {
-            atomicAdd(a.force + (size_t)gj * 3 + k, -gx[k]);
+            if (prow) prow[3 * gj + k] = gx[k];
+            else atomicAdd(a.force + (size_t)gj * 3 + k, -gx[k]);
             fc[k] += gx[k];
           }
 #pragma unroll
@@ -1055,7 +1069,10 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
     }
     if (lane == 0) {
 #pragma unroll
-      for (int k = 0; k < 3; ++k) atomicAdd(a.force + (size_t)gc * 3 + k, fc[k]);
+      for (int k = 0; k < 3; ++k) {
+        if (a.pgrad) a.fcen[(size_t)gc * 3 + k] = fc[k];
+        else atomicAdd(a.force + (size_t)gc * 3 + k, fc[k]);
+      }
     }
   }
   // virial: wave reduce, block reduce in fp64, one partial per block
@@ -1120,6 +1137,16 @@ static void fill_args(System* s, EmbedArgs& a) {
   a.wc_ldy = s->wc_ldy;
   a.wc_norm = s->wc_norm;
   a.centre_mask = M->atomic ? M->nsel_mask : ~0;
+  if (s->deterministic && !s->wc_model && !M->atomic && s->d_pgrad) {
+    a.pgrad = s->d_pgrad;
+    a.fcen = s->d_fcen;
+    int k = 0;
+    for (int j = 0; j < s->ntypes; ++j) {
+      a.koff[j] = k;
+      k += s->cap[j];
+    }
+    a.ksum = k;
+  }
   a.paircnt = reinterpret_cast<int*>(s->d_pairs);
   int cmax = 1;
   for (int j = 0; j < s->ntypes; ++j) cmax = std::max(cmax, s->cap[j]);
@@ -1195,6 +1222,67 @@ int32_t embed_launch(System* s, bool backward, int precision) {
     return launch_prec<2, 2, 4, 1, 2, 8>(s, backward, precision);
   set_error("embedding widths (%d,%d,%d) have no kernel instantiation", L.w[0], L.w[1], L.w[2]);
   return DPMD_E_UNSUPPORTED;
+}
+
+// Deterministic forces: F_g = (centre part of g) - sum over g's row entries (neighbour i,
+// slot rev in i's row of g's type) of the pair gradient dE/dx_ig that i's backward stored,
+// summed in row order and combined across lanes by a fixed butterfly, so two evaluations of
+// the same positions give bit-identical forces.  One wave per atom.
+__global__ void __launch_bounds__(256) force_gather_kernel(const SegInfo* seg, const int* atom_seg, int NA,
+                                                           int ntypes, NbrRef nr, const float* pgrad,
+                                                           const float* fcen, int ksum, float* force) {
+  const int lane = threadIdx.x & 63;
+  const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (g >= NA) return;
+  const SegInfo& S = seg[atom_seg[g]];
+  const int i0 = g - S.base;
+  int tg = 0;
+  while (tg + 1 < ntypes && i0 >= S.off[tg + 1]) ++tg;
+  int koff = 0;
+  for (int t = 0; t < tg; ++t) koff += nr.cap[t];
+  float acc[3] = {0.f, 0.f, 0.f};
+  for (int t = 0; t < ntypes; ++t) {
+    const int capt = nr.cap[t];
+    const int cnt = nr.cnt[t][g];
+    const int* row = nr.nbr[t] + (size_t)g * capt;
+    const int* rev = nr.rev[t] + (size_t)g * capt;
+    for (int p = lane; p < cnt; p += 64) {
+      const int e = rev[p];
+      if (e < 0) continue;
+      const size_t gi = (size_t)S.base + S.off[t] + row[p];
+      const float* pg = pgrad + (gi * ksum + koff + e) * 3;
+      acc[0] += pg[0];
+      acc[1] += pg[1];
+      acc[2] += pg[2];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    float v = acc[k];
+#pragma unroll
+    for (int sft = 1; sft < 64; sft <<= 1) v += __shfl_xor(v, sft);
+    acc[k] = v;
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) force[(size_t)g * 3 + k] = fcen[(size_t)g * 3 + k] - acc[k];
+  }
+}
+
+int32_t force_gather(System* s) {
+  NbrRef nr;
+  for (int t = 0; t < s->ntypes; ++t) {
+    nr.nbr[t] = s->d_nbr[t];
+    nr.cnt[t] = s->d_nbr_cnt[t];
+    nr.rev[t] = s->d_rev[t];
+    nr.cap[t] = s->cap[t];
+  }
+  int ksum = 0;
+  for (int t = 0; t < s->ntypes; ++t) ksum += s->cap[t];
+  hipLaunchKernelGGL(force_gather_kernel, dim3((s->NA + 3) / 4), dim3(256), 0, s->stream, s->d_seg, s->d_atom_seg,
+                     s->NA, s->ntypes, nr, s->d_pgrad, s->d_fcen, ksum, s->d_force);
+  DPMD_HIP(hipGetLastError());
+  return DPMD_OK;
 }
 
 // 16-bit operand tables derived from the packed fp32 tables: [ot][s2][lane][8] with

@@ -189,6 +189,14 @@ struct RowBase {
   int r[kMaxT];
 };
 
+// neighbour rows, their counts and reverse index, by type
+struct NbrRef {
+  const int* nbr[kMaxT];
+  const int* cnt[kMaxT];
+  const int* rev[kMaxT];
+  int cap[kMaxT];
+};
+
 // NPT step parameters (one segment): target pressure, thermostat temperature, step,
 // barostat time, cutoffs for the drift threshold, barostat chain length, group count,
 // atoms, per-axis group (-1 fixed) and axes per group.
@@ -238,6 +246,13 @@ struct System {
   int cap[kMaxT];               // knbr - 1 columns
   int* d_nbr[kMaxT] = {nullptr};
   int* d_nbr_cnt[kMaxT] = {nullptr};
+  // deterministic forces: per skin slot the pair gradient dE/dx_ij of the last backward
+  // (NA, sum cap, 3), per atom its centre part (NA, 3), and per row entry the entry's slot in
+  // the neighbour's own row (d_rev[j]: (NA, cap_j), -1 when absent)
+  bool deterministic = false;
+  float* d_pgrad = nullptr;
+  float* d_fcen = nullptr;
+  int* d_rev[kMaxT] = {nullptr};
   float4* d_canon = nullptr;    // canonical coordinates of the last build (reference_position)
   int ncell_total = 0;
   int* d_cell_count = nullptr;
@@ -326,6 +341,8 @@ int32_t sys_upload_segments(System* s);
 // kernels / launchers (return DPMD_OK or error)
 int32_t nlist_build(System* s, bool measure, bool conditional);
 int32_t nlist_drift_check(System* s, bool npt = false);
+int32_t nlist_reverse(System* s, bool conditional);
+int32_t force_gather(System* s);
 int32_t nlist_alloc_cells(System* s);
 int32_t nlist_clear_flags(System* s);
 int32_t embed_launch(System* s, bool backward, int precision);

@@ -95,6 +95,7 @@ struct NlArgs {
   int* nbr[kMaxT];
   int* nbr_cnt[kMaxT];
   int cap[kMaxT];
+  int* rev[kMaxT];     // deterministic forces: slot of this atom in each neighbour's row
   int ntypes, NA, ncell_total;
   float cut_sq;
   int* flags;
@@ -273,6 +274,41 @@ __global__ void __launch_bounds__(256) nl_build_kernel(NlArgs a) {
   }
 }
 
+// Reverse index of the rows for the deterministic force gather: for entry p of atom g's
+// row of type t (neighbour i), the position e of g in i's row of g's type, found by binary
+// search (rows hold ascending type-local indices), or -1 when i's row does not list g (a
+// pair at the skin edge whose FP32 test came out differently in the two directions; such a
+// pair lies beyond rcut and carries no gradient).  One wave per atom.
+__global__ void __launch_bounds__(256) nl_reverse_kernel(NlArgs a) {
+  if (a.conditional && a.flags[0] == 0) return;
+  const int lane = threadIdx.x & 63;
+  const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (g >= a.NA) return;
+  const SegInfo& S = a.seg[a.atom_seg[g]];
+  const int i0 = g - S.base;
+  int tg = 0;
+  while (tg + 1 < a.ntypes && i0 >= S.off[tg + 1]) ++tg;
+  const int me = i0 - S.off[tg];
+  const int capg = a.cap[tg];
+  for (int t = 0; t < a.ntypes; ++t) {
+    const int capt = a.cap[t];
+    const int cnt = a.nbr_cnt[t][g];
+    const int* row = a.nbr[t] + (size_t)g * capt;
+    int* rev = a.rev[t] + (size_t)g * capt;
+    for (int p = lane; p < cnt; p += 64) {
+      const int gi = S.base + S.off[t] + row[p];
+      const int* ri = a.nbr[tg] + (size_t)gi * capg;
+      int lo = 0, hi = a.nbr_cnt[tg][gi];
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (ri[mid] < me) lo = mid + 1;
+        else hi = mid;
+      }
+      rev[p] = (lo < a.nbr_cnt[tg][gi] && ri[lo] == me) ? lo : -1;
+    }
+  }
+}
+
 // drift of every atom against the canonical coordinates of the last build; sets
 // flags[1] when any atom of any segment moved beyond its segment's threshold.  With
 // shrink_max > 0 (NPT) the thread of a segment's first atom also runs the box test of
@@ -354,6 +390,7 @@ static void fill_nl(System* s, NlArgs& a) {
     a.nbr[j] = s->d_nbr[j];
     a.nbr_cnt[j] = s->d_nbr_cnt[j];
     a.cap[j] = s->cap[j];
+    a.rev[j] = s->d_rev[j];
   }
   a.ntypes = s->ntypes;
   a.NA = s->NA;
@@ -414,7 +451,20 @@ int32_t nlist_build(System* s, bool measure, bool conditional) {
   hipLaunchKernelGGL(nl_scatter_kernel, dim3((NA + 255) / 256), dim3(256), 0, s->stream, a);
   const size_t lds = sizeof(int) * 2 * a.lcap * 4;
   hipLaunchKernelGGL(nl_build_kernel, dim3((NA + 3) / 4), dim3(256), lds, s->stream, a);
+  if (s->deterministic && !measure && s->d_rev[0])
+    hipLaunchKernelGGL(nl_reverse_kernel, dim3((NA + 3) / 4), dim3(256), 0, s->stream, a);
   timer_end(s, 0, ev);
+  DPMD_HIP(hipGetLastError());
+  return DPMD_OK;
+}
+
+// reverse index of the current rows (after a change of the deterministic setting)
+int32_t nlist_reverse(System* s, bool conditional) {
+  DPMD_TRY(sys_upload_segments(s));
+  NlArgs a;
+  fill_nl(s, a);
+  a.conditional = conditional ? 1 : 0;
+  hipLaunchKernelGGL(nl_reverse_kernel, dim3((s->NA + 3) / 4), dim3(256), 0, s->stream, a);
   DPMD_HIP(hipGetLastError());
   return DPMD_OK;
 }

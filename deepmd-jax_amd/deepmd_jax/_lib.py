@@ -80,6 +80,7 @@ def lib():
         'dpmd_nlist_capacity': (c_i32, [c_vp, c_vp]),
         'dpmd_compute': (c_i32, [c_vp, c_i32, c_vp, c_vp, c_vp, c_vp]),
         'dpmd_compute_noalloc': (c_i32, [c_vp, c_i32, c_vp, c_vp, c_vp]),
+        'dpmd_system_set_deterministic': (c_i32, [c_vp, c_i32]),
         'dpmd_md_run': (c_i32, [c_vp, P(MDParams), c_i32, c_vp, c_vp, c_vp]),
         'dpmd_pimd_setup': (c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp]),
         'dpmd_kinetic_energy': (c_i32, [c_vp, c_vp]),
@@ -138,7 +139,7 @@ def exported_symbols():
             'dpmd_timing_get', 'dpmd_pair_count', 'dpmd_synchronize', 'dpmd_system_get_boxes',
             'dpmd_md_box_trajectory', 'dpmd_npt_state_get', 'dpmd_nlist_rebuilds', 'dpmd_system_set_fixed',
             'dpmd_pppm_compute', 'dpmd_system_set_dplr', 'dpmd_dplr_get', 'dpmd_atomic_predict',
-            'dpmd_nlist_update_rate', 'dpmd_compute_noalloc']
+            'dpmd_nlist_update_rate', 'dpmd_compute_noalloc', 'dpmd_system_set_deterministic']
 
 
 def pppm_compute(box3, beta, grid, xyz, q, device=0):
@@ -419,6 +420,11 @@ class DeviceSystem:
         ea = np.zeros(self._shape()) if want_atom else None
         check(lib().dpmd_compute(self.handle, PREC[precision], ptr(e), ptr(f), ptr(v), ptr(ea)))
         return e, f, v.reshape(self.nrep, 3, 3), ea
+
+    def set_deterministic(self, on=True):
+        """Bit-reproducible forces: pair gradients gathered per atom in a fixed order instead
+        of FP32 atomics (dpmd_system_set_deterministic)."""
+        check(lib().dpmd_system_set_deterministic(self.handle, 1 if on else 0))
 
     def compute_noalloc(self, precision='fp32'):
         """dpmd_compute_noalloc: energy, forces and virial with the work buffers in the

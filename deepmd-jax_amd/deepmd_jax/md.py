@@ -157,7 +157,7 @@ class Simulation:
                  neighbor_buffer_ratio=1.2, remove_com_motion=False, tau_t=100., tau_p=1000., chain_length_t=3,
                  chain_length_p=3, chain_steps_t=1, chain_steps_p=1, sy_steps_t=1, sy_steps_p=1,
                  fixed_indices=None, couple_axes=((0, 1, 2),), n_bead=1, type_symbols=None, pimd_lambda=1.0,
-                 pimd_print_invariant=False, precision='fp32', device=0, model=None):
+                 pimd_print_invariant=False, precision='fp32', device=0, model=None, deterministic=False):
         self.report_interval = int(report_interval)
         self.log = []
         self.step = 0
@@ -282,6 +282,8 @@ class Simulation:
         elif widths.min() <= 2 * (self._rcut + self._neighbor_skin) or not use_neighbor_list_when_possible:
             self._neighbor_skin = max(0.0, 0.5 * widths.min() * (1 - 1e-9) - self._rcut)
         self._sys = _ImageSystem(self._device_model, self._type_idx, self._n_bead, self._box, k)
+        # engine option: bit-reproducible forces (pair gradients gathered in a fixed order)
+        self._sys.dev.set_deterministic(bool(deterministic))
         self._sys.set_masses(self._mass)
         # DPLR: the centroid model and PPPM term join every force evaluation (deepmd_jax/md.py:404-415)
         self._dplr = dplr_setup(params, device)

@@ -217,6 +217,16 @@ int32_t dpmd_nlist_capacity(dpmd_system_t sys, int32_t* knbr);
 int32_t dpmd_compute(dpmd_system_t sys, int32_t precision, double* energy, double* force,
                      double* virial, double* atom_energy);
 
+/* Deterministic forces on (1) or off (0, the default).  On, the backward writes each pair's
+ * gradient dE/dx_ij once (by its slot in the centre's neighbour row) and a gather sums, per
+ * atom and in row order, its centre part and the gradients its neighbours stored, through a
+ * reverse index built with every neighbour-list rebuild; two evaluations of the same
+ * positions then give bit-identical forces.  Off, neighbour contributions are FP32 device
+ * atomics, whose order varies from run to run.  Replaces the scatter-add that jax.grad makes
+ * of the neighbour gather (deepmd_jax/utils.py:80); the reference runs it in whatever order
+ * XLA chooses.  DPLR systems keep atomics for their long-range and centroid terms. */
+int32_t dpmd_system_set_deterministic(dpmd_system_t sys, int32_t on);
+
 /* Allocation-free variant of dpmd_compute (same energy, force and virial outputs, no
  * per-atom energies).  It has no counterpart in the reference: it is for callers that
  * cannot allocate device memory (code under HIP graph capture, tight serving loops,
