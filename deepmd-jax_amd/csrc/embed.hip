@@ -194,6 +194,7 @@ struct EmbedArgs {
   float* pgrad;                // (NA, ksum, 3) or null (atomics)
   float* fcen;                 // (NA, 3)
   int ksum, koff[kMaxT];
+  int g16;                     // forward: G stored as FP32 (0), BF16 (1) or FP16 (2) for the fitting GEMMs
 };
 
 // Minimum image of d = x_j - x_i for positions wrapped into [0, L) (so |d| < L): the
@@ -662,6 +663,9 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
   float* Gout = a.G + (size_t)grow * A * W;
   const float* tb = a.tbias;
   const float r2 = 1.4142135623730951f;
+  // the 16-bit fitting GEMMs read G in their operand format: rounded here once (the same
+  // round-to-nearest-even conversion the GEMM would apply on staging), half the bytes
+  uint16_t* Gh = reinterpret_cast<uint16_t*>(a.G) + (size_t)grow * A * W;
   for (int e = lane; e < A * W; e += 64) {
     const int aa = e / W, w = e % W;
     const float tww = Tl[w] + tb[w], twa = Tl[aa] + tb[aa];
@@ -680,7 +684,9 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
                        Tl[(7 + k) * W + A + aa];
       v += g2 * Tl[(7 + k) * W + w];
     }
-    Gout[e] = v;
+    if (a.g16 == 1) Gh[e] = __builtin_bit_cast(uint16_t, (__bf16)v);
+    else if (a.g16 == 2) Gh[e] = __builtin_bit_cast(uint16_t, (_Float16)v);
+    else Gout[e] = v;
   }
   if (lane == 0) a.paircnt[gc] = pairs;
 }
@@ -1164,9 +1170,10 @@ static void fill_args(System* s, EmbedArgs& a) {
 }
 
 template <int T1, int T2, int T3, int Z2, int Z3, int PREC, int V1, bool TAB>
-static int32_t launch_embed(System* s, bool backward) {
+static int32_t launch_embed(System* s, bool backward, int precision) {
   EmbedArgs a;
   fill_args(s, a);
+  a.g16 = backward ? 0 : precision;
   if (PREC) {
     Model* M = s->model;
     a.hnet = M->d_emb_h[PREC - 1];
@@ -1208,10 +1215,10 @@ static int32_t launch_embed(System* s, bool backward) {
 // precision of the fitting GEMMs.
 template <int T1, int T2, int T3, int Z2, int Z3, int V1>
 static int32_t launch_prec(System* s, bool backward, int precision) {
-  if (s->model->tab_n > 0) return launch_embed<T1, T2, T3, Z2, Z3, 0, V1, true>(s, backward);
-  if (precision == DPMD_PREC_BF16) return launch_embed<T1, T2, T3, Z2, Z3, 1, V1, false>(s, backward);
-  if (precision == DPMD_PREC_FP16) return launch_embed<T1, T2, T3, Z2, Z3, 2, V1, false>(s, backward);
-  return launch_embed<T1, T2, T3, Z2, Z3, 0, V1, false>(s, backward);
+  if (s->model->tab_n > 0) return launch_embed<T1, T2, T3, Z2, Z3, 0, V1, true>(s, backward, precision);
+  if (precision == DPMD_PREC_BF16) return launch_embed<T1, T2, T3, Z2, Z3, 1, V1, false>(s, backward, precision);
+  if (precision == DPMD_PREC_FP16) return launch_embed<T1, T2, T3, Z2, Z3, 2, V1, false>(s, backward, precision);
+  return launch_embed<T1, T2, T3, Z2, Z3, 0, V1, false>(s, backward, precision);
 }
 
 int32_t embed_launch(System* s, bool backward, int precision) {

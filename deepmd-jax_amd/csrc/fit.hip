@@ -26,6 +26,7 @@ enum FitEpi { EPI_TANH = 0, EPI_RES = 1, EPI_E3 = 2, EPI_BWD2 = 3, EPI_BWD1 = 4,
 
 struct GemmArgs {
   const float* A;   // M x K fp32, row stride lda
+  const uint16_t* Ah;  // or M x K already in the 16-bit operand format (16-bit GEMMs), stride lda
   int lda;
   const void* Bt;   // N x K (fp32 or 16-bit), row stride ldb elements
   int ldb;
@@ -207,7 +208,7 @@ struct Half8<2> {
 // feeds two MFMAs (the LDS read rate, not the MFMA rate, bounds a 16-row wave tile).  K is
 // staged in 64-wide slices when K allows (two MFMA k-steps per block barrier: BF16 fitting
 // time -10 % against 32-wide slices), else 32.
-template <int NT, int EPI, int PREC, int KC>
+template <int NT, int EPI, int PREC, int KC, bool A16 = false>
 __global__ void __launch_bounds__(256, 2) gemm_h_kernel(const GemmArgs g) {
   typedef typename Half8<PREC>::T HT;
   typedef typename Half8<PREC>::E HE;
@@ -235,15 +236,20 @@ __global__ void __launch_bounds__(256, 2) gemm_h_kernel(const GemmArgs g) {
 #pragma unroll
     for (int t = 0; t < NH; ++t) acc[mi][t] = floatx4{0.f, 0.f, 0.f, 0.f};
   floatx4 ra[2 * NAP];
+  uintx4 rh[NAP];
   uintx4 rb[NB];
   auto live = [&](int piece) { return piece + 1 < NB || tid + 256 * piece < BV; };
   auto fetch = [&](int k) {
 #pragma unroll
     for (int p = 0; p < NAP; ++p) {
       const int idx = tid + 256 * p;
-      const float* ap = g.A + (size_t)(m0 + idx / P8) * g.lda + k + 8 * (idx % P8);
-      ra[2 * p] = *reinterpret_cast<const floatx4*>(ap);
-      ra[2 * p + 1] = *reinterpret_cast<const floatx4*>(ap + 4);
+      if constexpr (A16) {
+        rh[p] = *reinterpret_cast<const uintx4*>(g.Ah + (size_t)(m0 + idx / P8) * g.lda + k + 8 * (idx % P8));
+      } else {
+        const float* ap = g.A + (size_t)(m0 + idx / P8) * g.lda + k + 8 * (idx % P8);
+        ra[2 * p] = *reinterpret_cast<const floatx4*>(ap);
+        ra[2 * p + 1] = *reinterpret_cast<const floatx4*>(ap + 4);
+      }
     }
 #pragma unroll
     for (int p = 0; p < NB; ++p) {
@@ -257,11 +263,15 @@ __global__ void __launch_bounds__(256, 2) gemm_h_kernel(const GemmArgs g) {
 #pragma unroll
     for (int p = 0; p < NAP; ++p) {
       const int idx = tid + 256 * p;
-      const floatx4 x0 = ra[2 * p], x1 = ra[2 * p + 1];
-      HT h;
-      h[0] = (HE)x0.x; h[1] = (HE)x0.y; h[2] = (HE)x0.z; h[3] = (HE)x0.w;
-      h[4] = (HE)x1.x; h[5] = (HE)x1.y; h[6] = (HE)x1.z; h[7] = (HE)x1.w;
-      *reinterpret_cast<HT*>(&As[(idx / P8) * LS + 8 * (idx % P8)]) = h;
+      if constexpr (A16) {
+        *reinterpret_cast<uintx4*>(&As[(idx / P8) * LS + 8 * (idx % P8)]) = rh[p];
+      } else {
+        const floatx4 x0 = ra[2 * p], x1 = ra[2 * p + 1];
+        HT h;
+        h[0] = (HE)x0.x; h[1] = (HE)x0.y; h[2] = (HE)x0.z; h[3] = (HE)x0.w;
+        h[4] = (HE)x1.x; h[5] = (HE)x1.y; h[6] = (HE)x1.z; h[7] = (HE)x1.w;
+        *reinterpret_cast<HT*>(&As[(idx / P8) * LS + 8 * (idx % P8)]) = h;
+      }
     }
 #pragma unroll
     for (int p = 0; p < NB; ++p) {
@@ -339,6 +349,21 @@ template <int NT, int EPI>
 static int32_t go(hipStream_t st, const GemmArgs& g, int prec) {
   dim3 grid(g.M / 64, g.N / (16 * NT));
   const bool k64 = g.K % 64 == 0;
+  if constexpr (EPI == EPI_TANH) {
+    // layer 0 of a 16-bit form reads G as the embedding kernel stored it (16-bit)
+    if (g.Ah) {
+      if (prec == DPMD_PREC_BF16 && k64) hipLaunchKernelGGL((gemm_h_kernel<NT, EPI, 1, 64, true>), grid, dim3(256), 0, st, g);
+      else if (prec == DPMD_PREC_BF16) hipLaunchKernelGGL((gemm_h_kernel<NT, EPI, 1, 32, true>), grid, dim3(256), 0, st, g);
+      else if (prec == DPMD_PREC_FP16 && k64) hipLaunchKernelGGL((gemm_h_kernel<NT, EPI, 2, 64, true>), grid, dim3(256), 0, st, g);
+      else if (prec == DPMD_PREC_FP16) hipLaunchKernelGGL((gemm_h_kernel<NT, EPI, 2, 32, true>), grid, dim3(256), 0, st, g);
+      else {
+        set_error("a 16-bit G needs a 16-bit GEMM");
+        return DPMD_E_STATE;
+      }
+      DPMD_HIP(hipGetLastError());
+      return DPMD_OK;
+    }
+  }
   if (prec == DPMD_PREC_BF16 && k64) hipLaunchKernelGGL((gemm_h_kernel<NT, EPI, 1, 64>), grid, dim3(256), 0, st, g);
   else if (prec == DPMD_PREC_BF16) hipLaunchKernelGGL((gemm_h_kernel<NT, EPI, 1, 32>), grid, dim3(256), 0, st, g);
   else if (prec == DPMD_PREC_FP16 && k64) hipLaunchKernelGGL((gemm_h_kernel<NT, EPI, 2, 64>), grid, dim3(256), 0, st, g);
@@ -408,13 +433,15 @@ int32_t fit_launch(System* s, int precision) {
     GemmArgs g;
     memset(&g, 0, sizeof(g));
     g.M = rows;
-    // 1: y1 = tanh(G K0 + b0)
-    g.A = (const float*)s->d_G + r0 * D; g.lda = D; g.K = D;
+    // 1: y1 = tanh(G K0 + b0); a 16-bit form's G was stored 16-bit by the embedding kernel
+    if (precision == DPMD_PREC_FP32) g.A = (const float*)s->d_G + r0 * D;
+    else g.Ah = (const uint16_t*)s->d_G + r0 * D;
+    g.lda = D; g.K = D;
     g.Bt = B(FK0T); g.ldb = D; g.N = FN;
     g.C = Y1 + r0 * FK; g.ldc = FK; g.bias = Wf + o[FB0];
     DPMD_TRY(gemm_dispatch<EPI_TANH>(s->stream, g, precision));
     // 2: t2 = tanh(y1 K1 + b1), y2 = t2 dt1 + y1
-    g.A = Y1 + r0 * FK; g.lda = FK; g.K = FK;
+    g.A = Y1 + r0 * FK; g.Ah = nullptr; g.lda = FK; g.K = FK;
     g.Bt = B(FK1T); g.ldb = FK;
     g.C = Y2 + r0 * FK; g.C2 = T2 + r0 * FK; g.aux1 = Y1 + r0 * FK;
     g.bias = Wf + o[FB1]; g.dt = Wf + o[FDT1];
