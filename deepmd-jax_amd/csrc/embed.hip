@@ -666,27 +666,79 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
   // the 16-bit fitting GEMMs read G in their operand format: rounded here once (the same
   // round-to-nearest-even conversion the GEMM would apply on staging), half the bytes
   uint16_t* Gh = reinterpret_cast<uint16_t*>(a.G) + (size_t)grow * A * W;
-  for (int e = lane; e < A * W; e += 64) {
-    const int aa = e / W, w = e % W;
-    const float tww = Tl[w] + tb[w], twa = Tl[aa] + tb[aa];
-    float v = tww * twa;
-#pragma unroll
-    for (int cc = 0; cc < 3; ++cc) v += Tl[(1 + cc) * W + w] * Tl[(1 + cc) * W + aa];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const float t3 = Tl[(1 + k) * W + A + aa];
-      const float g2 = t3 * t3 + Tl[(4 + k) * W + A + aa];
-      v += g2 * Tl[(4 + k) * W + w];
-    }
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const float g2 = r2 * Tl[(1 + k) * W + A + aa] * Tl[(1 + (k + 2) % 3) * W + A + aa] +
-                       Tl[(7 + k) * W + A + aa];
-      v += g2 * Tl[(7 + k) * W + w];
-    }
+  auto gstore = [&](int e, float v) {
     if (a.g16 == 1) Gh[e] = __builtin_bit_cast(uint16_t, (__bf16)v);
     else if (a.g16 == 2) Gh[e] = __builtin_bit_cast(uint16_t, (_Float16)v);
     else Gout[e] = v;
+  };
+  if (A <= 16) {
+    // G[a][w] = sum_x X[x][a] Y[x][w] (dpmodel.py:74-78) with Y = (TW, T3, T6) rows of T and
+    // X[x][a] = TW[a], T3[c][a], then the six second-order factors
+    // G2[k][a] = 3to6(T3[:, A + a])_k + T6[k][A + a]; on MFMA: M = a (16), K = x (10 of 12),
+    // N = w in 16-wide tiles
+    float* Xl = Tl + 10 * W;  // [12][16], rows 10, 11 and columns a >= A zero
+    for (int e = lane; e < 12 * 16; e += 64) {
+      const int x = e >> 4, aa = e & 15;
+      float v = 0.f;
+      if (x < 10 && aa < A) {
+        if (x == 0) {
+          v = Tl[aa] + tb[aa];
+        } else if (x < 4) {
+          v = Tl[x * W + aa];
+        } else if (x < 7) {
+          const float t3 = Tl[(x - 3) * W + A + aa];
+          v = t3 * t3 + Tl[x * W + A + aa];
+        } else {
+          const int k = x - 7;
+          v = r2 * Tl[(1 + k) * W + A + aa] * Tl[(1 + (k + 2) % 3) * W + A + aa] + Tl[x * W + A + aa];
+        }
+      }
+      Xl[e] = v;
+    }
+    wave_sync();
+    float xa[3];
+#pragma unroll
+    for (int kk = 0; kk < 3; ++kk) xa[kk] = Xl[(4 * kk + q) * 16 + n];
+    const int ntile = (W + 15) / 16;
+    for (int t = 0; t < ntile; ++t) {
+      const int w = 16 * t + n;
+      floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kk = 0; kk < 3; ++kk) {
+        const int x = 4 * kk + q;
+        float y = 0.f;
+        if (x < 10 && w < W) y = Tl[x * W + w] + (x == 0 ? tb[w] : 0.f);
+        acc = mfma4(xa[kk], y, acc);
+      }
+      if (w < W) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int aa = 4 * q + i;
+          if (aa < A) gstore(aa * W + w, acc[i]);
+        }
+      }
+    }
+  } else {
+    for (int e = lane; e < A * W; e += 64) {
+      const int aa = e / W, w = e % W;
+      const float tww = Tl[w] + tb[w], twa = Tl[aa] + tb[aa];
+      float v = tww * twa;
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) v += Tl[(1 + cc) * W + w] * Tl[(1 + cc) * W + aa];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const float t3 = Tl[(1 + k) * W + A + aa];
+        const float g2 = t3 * t3 + Tl[(4 + k) * W + A + aa];
+        v += g2 * Tl[(4 + k) * W + w];
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const float g2 = r2 * Tl[(1 + k) * W + A + aa] * Tl[(1 + (k + 2) % 3) * W + A + aa] +
+                         Tl[(7 + k) * W + A + aa];
+        v += g2 * Tl[(7 + k) * W + w];
+      }
+      gstore(e, v);
+    }
   }
   if (lane == 0) a.paircnt[gc] = pairs;
 }
@@ -1190,7 +1242,7 @@ static int32_t launch_embed(System* s, bool backward, int precision) {
   size_t wave_floats;
   if (!backward) {
     wave_floats = 4 * a.list_cap + 16 * kRS + 16 * (16 * kGroup + 1);
-    wave_floats = std::max(wave_floats, (size_t)(10 * W));
+    wave_floats = std::max(wave_floats, (size_t)(10 * W + 12 * 16));
   } else {
     wave_floats = (11 * HS + 3) / 4 * 4 + std::max((size_t)(2 * 16 * kRS + 4 * a.list_cap), (size_t)(10 * W + 20 * A));
   }
