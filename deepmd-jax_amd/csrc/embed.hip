@@ -807,84 +807,177 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
       for (int e = lane; e < 11 * HS; e += 64) dTp[e] = 0.f;
       wave_sync();
       const float r2 = 1.4142135623730951f;
-      for (int e = lane; e < 10 * A; e += 64) {
-        const int x = e / A, aa = e % A;
-        float v;
-        if (x < 4) {
-          v = Tl[x * W + aa];
-        } else if (x < 7) {
-          const int k = x - 4;
-          const float t3 = Tl[(1 + k) * W + A + aa];
-          v = t3 * t3 + Tl[x * W + A + aa];
-        } else {
-          const int kp = x - 7;
-          v = r2 * Tl[(1 + kp) * W + A + aa] * Tl[(1 + (kp + 2) % 3) * W + A + aa] + Tl[x * W + A + aa];
+      if (A <= 16) {
+        // On MFMA: U = Tx dG^T (M = x, K = w, N = a) and V = X dG (M = x, K = a, N = w); the U
+        // terms are added to V in its output registers, lane (q, n) holding V[4q + i][16t + n].
+        float* Xm = Tl + 10 * W;   // [12][16]: X[x][a], rows 10, 11 and columns a >= A zero
+        float* Um = Xm + 12 * 16;  // [10][16]: U[x][a]
+        for (int e = lane; e < 12 * 16; e += 64) {
+          const int x = e >> 4, aa = e & 15;
+          float v = 0.f;
+          if (x < 10 && aa < A) {
+            if (x < 4) {
+              v = Tl[x * W + aa];
+            } else if (x < 7) {
+              const float t3 = Tl[(x - 3) * W + A + aa];
+              v = t3 * t3 + Tl[x * W + A + aa];
+            } else {
+              const int kp = x - 7;
+              v = r2 * Tl[(1 + kp) * W + A + aa] * Tl[(1 + (kp + 2) % 3) * W + A + aa] + Tl[x * W + A + aa];
+            }
+          }
+          Xm[e] = v;
         }
-        Xl[e] = v;
-      }
-      {
-        // U: lane (a = lane%16, wq = lane/16) sums w = wq mod 4, then the 4 lanes of a combine
-        const int aa = lane & 15, wq = lane >> 4;
-        float u[10];
+        // W <= 16 T3 (the layer-3 tiles hold every output feature): compile-time trip counts,
+        // every dG load issued before the MFMAs that consume it
+        {
+          float dgu[4 * T3];  // dG[a = n][w = 4 kk + q]
 #pragma unroll
-        for (int x = 0; x < 10; ++x) u[x] = 0.f;
-        if (aa < A) {
-          for (int w = wq; w < W; w += 4) {
+          for (int kk = 0; kk < 4 * T3; ++kk) {
+            const int w = 4 * kk + q;
+            dgu[kk] = (n < A && w < W) ? dGg[n * W + w] : 0.f;
+          }
+          floatx4 u0 = floatx4{0.f, 0.f, 0.f, 0.f}, u1 = u0;
+#pragma unroll
+          for (int kk = 0; kk < 4 * T3; ++kk) {
+            const int w = 4 * kk + q;
+            const float av = (n < 10 && w < W) ? Tl[n * W + w] : 0.f;
+            if (kk & 1) u1 = mfma4(av, dgu[kk], u1);
+            else u0 = mfma4(av, dgu[kk], u0);
+          }
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            if (4 * q + i < 10) Um[(4 * q + i) * 16 + n] = u0[i] + u1[i];
+        }
+        float dgv[T3][4];  // dG[a = 4 kk + q][w = 16 t + n]
+#pragma unroll
+        for (int t = 0; t < T3; ++t)
+#pragma unroll
+          for (int kk = 0; kk < 4; ++kk) {
+            const int aa = 4 * kk + q, w = 16 * t + n;
+            dgv[t][kk] = (aa < A && w < W) ? dGg[aa * W + w] : 0.f;
+          }
+        wave_sync();
+        float xa[4];
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) xa[kk] = n < 12 ? Xm[n * 16 + 4 * kk + q] : 0.f;
+        const float* gg = a.wc_g ? a.wc_g + (size_t)gc * 3 : nullptr;
+#pragma unroll
+        for (int t = 0; t < T3; ++t) {
+          const int w = 16 * t + n;
+          floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int kk = 0; kk < 4; ++kk) acc = mfma4(xa[kk], dgv[t][kk], acc);
+          if (w >= W) continue;
+          int slot = -1;
+          for (int qq = 0; qq < 4; ++qq) {
+            const int rr = w - a.base3[qq];
+            if (rr >= 0 && rr < a.size3[qq]) slot = 16 * (rr / 4) + 4 * qq + (rr % 4);
+          }
+          const float y = gg ? a.wc_norm * a.wc_y3[(size_t)grow * a.wc_ldy + w] : 0.f;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int x = 4 * q + i;
+            if (x >= 10) continue;
+            float v = acc[i];
+            if (w < A) {
+              if (x < 4) v += Um[x * 16 + w];
+            } else if (w < 2 * A) {
+              const int aa = w - A;
+              if (x >= 4) {
+                v += Um[x * 16 + aa];
+              } else if (x >= 1) {
+                const int cc = x - 1;
+                v += 2.f * Tl[x * W + w] * Um[(4 + cc) * 16 + aa] +
+                     r2 * Tl[(1 + (cc + 2) % 3) * W + w] * Um[(7 + cc) * 16 + aa] +
+                     r2 * Tl[(1 + (cc + 1) % 3) * W + w] * Um[(7 + (cc + 1) % 3) * 16 + aa];
+              }
+            }
+            // the centroid output out_norm sum_w f_w T3[c][w] depends on T3 directly
+            if (gg && x >= 1 && x <= 3) v += gg[x - 1] * y;
+            if (slot >= 0) dTp[x * HS + slot] = v * a.inv_nnbrs;
+          }
+        }
+      } else {
+        for (int e = lane; e < 10 * A; e += 64) {
+          const int x = e / A, aa = e % A;
+          float v;
+          if (x < 4) {
+            v = Tl[x * W + aa];
+          } else if (x < 7) {
+            const int k = x - 4;
+            const float t3 = Tl[(1 + k) * W + A + aa];
+            v = t3 * t3 + Tl[x * W + A + aa];
+          } else {
+            const int kp = x - 7;
+            v = r2 * Tl[(1 + kp) * W + A + aa] * Tl[(1 + (kp + 2) % 3) * W + A + aa] + Tl[x * W + A + aa];
+          }
+          Xl[e] = v;
+        }
+        {
+          // U: lane (a = lane%16, wq = lane/16) sums w = wq mod 4, then the 4 lanes of a combine
+          const int aa = lane & 15, wq = lane >> 4;
+          float u[10];
+#pragma unroll
+          for (int x = 0; x < 10; ++x) u[x] = 0.f;
+          if (aa < A) {
+            for (int w = wq; w < W; w += 4) {
+              const float d = dGg[aa * W + w];
+#pragma unroll
+              for (int x = 0; x < 10; ++x) u[x] += d * Tl[x * W + w];
+            }
+          }
+#pragma unroll
+          for (int x = 0; x < 10; ++x) {
+            u[x] += __shfl_xor(u[x], 16);
+            u[x] += __shfl_xor(u[x], 32);
+          }
+          if (wq == 0 && aa < A) {
+#pragma unroll
+            for (int x = 0; x < 10; ++x) Ul[x * A + aa] = u[x];
+          }
+        }
+        wave_sync();
+        for (int w = lane; w < W; w += 64) {
+          float v[10];
+#pragma unroll
+          for (int x = 0; x < 10; ++x) v[x] = 0.f;
+          for (int aa = 0; aa < A; ++aa) {
             const float d = dGg[aa * W + w];
 #pragma unroll
-            for (int x = 0; x < 10; ++x) u[x] += d * Tl[x * W + w];
+            for (int x = 0; x < 10; ++x) v[x] += d * Xl[x * A + aa];
           }
-        }
+          if (w < A) {
 #pragma unroll
-        for (int x = 0; x < 10; ++x) {
-          u[x] += __shfl_xor(u[x], 16);
-          u[x] += __shfl_xor(u[x], 32);
-        }
-        if (wq == 0 && aa < A) {
+            for (int x = 0; x < 4; ++x) v[x] += Ul[x * A + w];
+          } else if (w < 2 * A) {
+            const int aa = w - A;
 #pragma unroll
-          for (int x = 0; x < 10; ++x) Ul[x * A + aa] = u[x];
-        }
-      }
-      wave_sync();
-      for (int w = lane; w < W; w += 64) {
-        float v[10];
+            for (int x = 4; x < 10; ++x) v[x] += Ul[x * A + aa];
 #pragma unroll
-        for (int x = 0; x < 10; ++x) v[x] = 0.f;
-        for (int aa = 0; aa < A; ++aa) {
-          const float d = dGg[aa * W + w];
-#pragma unroll
-          for (int x = 0; x < 10; ++x) v[x] += d * Xl[x * A + aa];
-        }
-        if (w < A) {
-#pragma unroll
-          for (int x = 0; x < 4; ++x) v[x] += Ul[x * A + w];
-        } else if (w < 2 * A) {
-          const int aa = w - A;
-#pragma unroll
-          for (int x = 4; x < 10; ++x) v[x] += Ul[x * A + aa];
-#pragma unroll
-          for (int cc = 0; cc < 3; ++cc) {
-            v[1 + cc] += 2.f * Tl[(1 + cc) * W + w] * Ul[(4 + cc) * A + aa] +
-                         r2 * Tl[(1 + (cc + 2) % 3) * W + w] * Ul[(7 + cc) * A + aa] +
-                         r2 * Tl[(1 + (cc + 1) % 3) * W + w] * Ul[(7 + (cc + 1) % 3) * A + aa];
+            for (int cc = 0; cc < 3; ++cc) {
+              v[1 + cc] += 2.f * Tl[(1 + cc) * W + w] * Ul[(4 + cc) * A + aa] +
+                           r2 * Tl[(1 + (cc + 2) % 3) * W + w] * Ul[(7 + cc) * A + aa] +
+                           r2 * Tl[(1 + (cc + 1) % 3) * W + w] * Ul[(7 + (cc + 1) % 3) * A + aa];
+            }
           }
-        }
-        int slot = -1;
-        for (int qq = 0; qq < 4; ++qq) {
-          const int rr = w - a.base3[qq];
-          if (rr >= 0 && rr < a.size3[qq]) slot = 16 * (rr / 4) + 4 * qq + (rr % 4);
-        }
-        if (a.wc_g) {
-          // the centroid output out_norm sum_w f_w T3[c][w] depends on T3 directly
-          const float y = a.wc_norm * a.wc_y3[(size_t)grow * a.wc_ldy + w];
-          const float* gg = a.wc_g + (size_t)gc * 3;
-          v[1] += gg[0] * y;
-          v[2] += gg[1] * y;
-          v[3] += gg[2] * y;
-        }
-        if (slot >= 0) {
+          int slot = -1;
+          for (int qq = 0; qq < 4; ++qq) {
+            const int rr = w - a.base3[qq];
+            if (rr >= 0 && rr < a.size3[qq]) slot = 16 * (rr / 4) + 4 * qq + (rr % 4);
+          }
+          if (a.wc_g) {
+            // the centroid output out_norm sum_w f_w T3[c][w] depends on T3 directly
+            const float y = a.wc_norm * a.wc_y3[(size_t)grow * a.wc_ldy + w];
+            const float* gg = a.wc_g + (size_t)gc * 3;
+            v[1] += gg[0] * y;
+            v[2] += gg[1] * y;
+            v[3] += gg[2] * y;
+          }
+          if (slot >= 0) {
 #pragma unroll
-          for (int x = 0; x < 10; ++x) dTp[x * HS + slot] = v[x] * a.inv_nnbrs;
+            for (int x = 0; x < 10; ++x) dTp[x * HS + slot] = v[x] * a.inv_nnbrs;
+          }
         }
       }
       wave_sync();
@@ -1244,7 +1337,8 @@ static int32_t launch_embed(System* s, bool backward, int precision) {
     wave_floats = 4 * a.list_cap + 16 * kRS + 16 * (16 * kGroup + 1);
     wave_floats = std::max(wave_floats, (size_t)(10 * W + 12 * 16));
   } else {
-    wave_floats = (11 * HS + 3) / 4 * 4 + std::max((size_t)(2 * 16 * kRS + 4 * a.list_cap), (size_t)(10 * W + 20 * A));
+    wave_floats = (11 * HS + 3) / 4 * 4 +
+                  std::max((size_t)(2 * 16 * kRS + 4 * a.list_cap), (size_t)(10 * W + std::max(20 * A, 22 * 16)));
   }
   wave_floats = (wave_floats + 3) / 4 * 4;
   a.lds_wave_floats = (int)wave_floats;
