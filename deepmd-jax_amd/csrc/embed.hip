@@ -803,7 +803,19 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
       float* Tl = dTp + ODT;       // 10*W   (aliases Rl, dRl, list)
       float* Xl = Tl + 10 * W;     // 10*A
       float* Ul = Xl + 10 * A;     // 10*A
-      for (int e = lane; e < 10 * W; e += 64) Tl[e] = Tg[e] + (e < W ? a.tbias[e] : 0.f);
+      // compile-time trip count (10 W <= 160 T3): all loads of T issue before the stores
+      constexpr int TN = (160 * T3 + 63) / 64;
+      float tv[TN];
+#pragma unroll
+      for (int k = 0; k < TN; ++k) {
+        const int e = lane + 64 * k;
+        tv[k] = e < 10 * W ? Tg[e] + (e < W ? a.tbias[e] : 0.f) : 0.f;
+      }
+#pragma unroll
+      for (int k = 0; k < TN; ++k) {
+        const int e = lane + 64 * k;
+        if (e < 10 * W) Tl[e] = tv[k];
+      }
       for (int e = lane; e < 11 * HS; e += 64) dTp[e] = 0.f;
       wave_sync();
       const float r2 = 1.4142135623730951f;
