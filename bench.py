@@ -673,7 +673,8 @@ def main():
     ap.add_argument('--no-conservation', action='store_true', help='skip the NVE energy-conservation leg')
     ap.add_argument('--pimd-steps', type=int, default=100)
     ap.add_argument('--pimd-acc-beads', type=int, default=2)
-    ap.add_argument('--serving-rates', default='20', help='comma-separated Poisson arrival rates (requests/s)')
+    ap.add_argument('--serving-rates', default='20,200,1000',
+                    help='comma-separated Poisson arrival rates (requests/s)')
     ap.add_argument('--serving-requests', type=int, default=256)
     ap.add_argument('--calibrate-copy', action='store_true',
                     help='run a plain 256 MiB elementwise copy (read 256 MiB, write 256 MiB) five times, for '

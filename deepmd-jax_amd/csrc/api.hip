@@ -577,6 +577,11 @@ static double seg_ebias(System* s, int r) {
 
 static int32_t compute_impl(System* s, int precision, double* energy, double* force, double* virial, double* eatom) {
   if (!s->nl_ready) { set_error("no neighbour list: call dpmd_nlist_allocate first"); return DPMD_E_STATE; }
+  if (s->nl_overflow) {
+    // rows were truncated at the capacity: the evaluation would miss neighbours
+    set_error("neighbour list overflowed its capacity: call dpmd_nlist_allocate again");
+    return DPMD_E_STATE;
+  }
   Model* M = s->model;
   DPMD_TRY(md_write_pos32(s));
   DPMD_TRY(model_forces_impl(s, precision));
@@ -933,6 +938,13 @@ int32_t dpmd_nlist_export(dpmd_system_t sys, int32_t j, int32_t* out) {
   DPMD_HIP(hipMemcpyAsync(out, s->d_nbr[j], sizeof(int) * (size_t)s->NA * s->cap[j], hipMemcpyDeviceToHost,
                           s->stream));
   DPMD_HIP(hipStreamSynchronize(s->stream));
+  return DPMD_OK;
+}
+
+int32_t dpmd_nlist_overflow(dpmd_system_t sys, int32_t* overflow) {
+  SYS(sys);
+  if (!overflow) { set_error("null argument"); return DPMD_E_ARG; }
+  *overflow = s->nl_overflow ? 1 : 0;
   return DPMD_OK;
 }
 
@@ -1532,6 +1544,10 @@ extern "C" int32_t dpmd_compute_noalloc(dpmd_system_t sys, int32_t precision, do
     return DPMD_E_ARG;
   }
   if (!s->nl_ready) { set_error("no neighbour list: call dpmd_nlist_allocate first"); return DPMD_E_STATE; }
+  if (s->nl_overflow) {
+    set_error("neighbour list overflowed its capacity: call dpmd_nlist_allocate again");
+    return DPMD_E_STATE;
+  }
   if (s->wc_model) {
     set_error("dpmd_compute_noalloc does not evaluate DPLR systems (their PPPM plans allocate)");
     return DPMD_E_UNSUPPORTED;

@@ -81,6 +81,7 @@ def lib():
         'dpmd_compute': (c_i32, [c_vp, c_i32, c_vp, c_vp, c_vp, c_vp]),
         'dpmd_compute_noalloc': (c_i32, [c_vp, c_i32, c_vp, c_vp, c_vp]),
         'dpmd_system_set_deterministic': (c_i32, [c_vp, c_i32]),
+        'dpmd_nlist_overflow': (c_i32, [c_vp, c_vp]),
         'dpmd_md_run': (c_i32, [c_vp, P(MDParams), c_i32, c_vp, c_vp, c_vp]),
         'dpmd_pimd_setup': (c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp]),
         'dpmd_kinetic_energy': (c_i32, [c_vp, c_vp]),
@@ -139,7 +140,8 @@ def exported_symbols():
             'dpmd_timing_get', 'dpmd_pair_count', 'dpmd_synchronize', 'dpmd_system_get_boxes',
             'dpmd_md_box_trajectory', 'dpmd_npt_state_get', 'dpmd_nlist_rebuilds', 'dpmd_system_set_fixed',
             'dpmd_pppm_compute', 'dpmd_system_set_dplr', 'dpmd_dplr_get', 'dpmd_atomic_predict',
-            'dpmd_nlist_update_rate', 'dpmd_compute_noalloc', 'dpmd_system_set_deterministic']
+            'dpmd_nlist_update_rate', 'dpmd_compute_noalloc', 'dpmd_system_set_deterministic',
+            'dpmd_nlist_overflow']
 
 
 def pppm_compute(box3, beta, grid, xyz, q, device=0):
@@ -401,6 +403,12 @@ class DeviceSystem:
         self.knbr = [int(k) for k in out[:self.model.ntypes]]
         self.rc_skin_last = float(rc_skin)
         return self.knbr
+
+    def nlist_overflow(self):
+        """True when the last build truncated a row at its capacity."""
+        ov = np.zeros(1, np.int32)
+        check(lib().dpmd_nlist_overflow(self.handle, ptr(ov)))
+        return bool(ov[0])
 
     def nlist_update(self):
         ov = np.zeros(1, np.int32)

@@ -45,6 +45,7 @@ class Request:
     velocity: np.ndarray = None     # (N, 3) Angstrom/fs ('md')
     seed: int = 0                   # Langevin stream of this request
     arrival: float = 0.0            # seconds after the stream start
+    masses: np.ndarray = None       # per-type masses (Dalton) of an 'md' request; None: the server's
 
     @property
     def natoms(self):
@@ -67,6 +68,7 @@ class Result:
     t_arrival: float = 0.0
     t_admit: float = 0.0
     t_done: float = 0.0
+    image_position: np.ndarray = None   # every image of a supercell segment ('md', keep_images=True)
 
     @property
     def latency(self):
@@ -102,7 +104,8 @@ class _Seg:
         c, sbox, stype = replicate(np.asarray(req.coord, np.float64), req.box, req.type_idx, k)
         self.coord, self.box, self.types = c, sbox, np.asarray(stype, np.int32)
         self.vel = np.tile(np.asarray(req.velocity, np.float64), (self.nimg, 1))
-        self.mass = np.asarray(masses, np.float64)[self.types] * MASS_UNIT_CONVERSION
+        m = masses if req.masses is None else req.masses
+        self.mass = np.asarray(m, np.float64)[self.types] * MASS_UNIT_CONVERSION
         self.thermo = np.zeros(64)
         self.step = 0
         self.remaining = int(req.nsteps)
@@ -117,7 +120,8 @@ class Server:
     """
 
     def __init__(self, model, *, precision='fp32', hbm_budget=16 << 30, max_batch_atoms=600_000, md_chunk=20,
-                 dt=0.5, temperature=300.0, tau_t=100.0, skin=1.0, masses=MASSES, seed=0, clock=None):
+                 dt=0.5, temperature=300.0, tau_t=100.0, skin=1.0, masses=MASSES, seed=0, clock=None,
+                 keep_images=False):
         self.model = model
         self.params = model.params
         self.rcut = float(model.rcut)
@@ -127,6 +131,8 @@ class Server:
         self.md_chunk = int(md_chunk)
         self.skin = float(skin)
         self.masses = masses
+        self.keep_images = bool(keep_images)
+        self._eval_knbr = None   # capacities of the last evaluation batch, reused while they hold
         self.md_params = _lib.MDParams()
         self.md_params.routine = _lib.ROUTINE['NVT_langevin']
         self.md_params.precision = _lib.PREC[precision]
@@ -146,7 +152,8 @@ class Server:
         self.results = {}
         self.errors = []
         self.stats = {'eval_batches': 0, 'md_chunks': 0, 'md_relayouts': 0, 'max_in_flight_bytes': 0,
-                      'max_md_segments': 0, 'max_eval_batch': 0}
+                      'max_md_segments': 0, 'max_eval_batch': 0, 'eval_requests': 0,
+                      'eval_capacity_measurements': 0}
         self._eval_sys = None
         self._md_sys = None
         self._threads = [threading.Thread(target=self._guard, args=(self._eval_worker,), daemon=True),
@@ -249,6 +256,7 @@ class Server:
                     self.results[req.rid] = r
                     self._release(b)
                 self.stats['eval_batches'] += 1
+                self.stats['eval_requests'] += len(batch)
                 self.stats['max_eval_batch'] = max(self.stats['max_eval_batch'], len(batch))
 
     def _run_eval(self, batch, t_admit):
@@ -269,7 +277,13 @@ class Server:
         s.set_cells(np.stack([cell_matrix(b) for b in boxes]))
         s.set_images(nimg)
         s.set_positions(np.concatenate(coords))
-        s.nlist_allocate(self.rcut, 1.2)
+        # reuse the last batch's row capacities (no measuring pass, no host round trip for it);
+        # measure again only when a row overflowed them
+        if self._eval_knbr is not None:
+            s.nlist_allocate(self.rcut, 1.2, knbr_override=self._eval_knbr)
+        if self._eval_knbr is None or s.nlist_overflow():
+            self._eval_knbr = list(s.nlist_allocate(self.rcut, 1.2))
+            self.stats['eval_capacity_measurements'] += 1
         e, f, v, _ = s.compute(self.precision)
         out = []
         for i, ((req, t_arr, _), fs) in enumerate(zip(batch, s.split(f))):
@@ -319,7 +333,8 @@ class Server:
             finished = [(sg, Result(rid=sg.req.rid, kind='md', natoms=sg.req.natoms, steps=sg.req.nsteps,
                                     position=sg.coord[:sg.req.natoms].copy(), velocity=sg.vel[:sg.req.natoms].copy(),
                                     thermostat_work=float(sg.thermo[0]), error=int(err), t_arrival=sg.t_arrival,
-                                    t_admit=sg.t_admit)) for sg in done]
+                                    t_admit=sg.t_admit,
+                                    image_position=sg.coord.copy() if self.keep_images else None)) for sg in done]
             active = [sg for sg in active if sg.remaining > 0 and not failed]
             if active:
                 self._md_layout(active)

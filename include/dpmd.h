@@ -206,6 +206,10 @@ int32_t dpmd_nlist_update(dpmd_system_t sys, int32_t* overflow);
 int32_t dpmd_nlist_export(dpmd_system_t sys, int32_t j, int32_t* out);
 /* Current capacities knbr (ntypes). */
 int32_t dpmd_nlist_capacity(dpmd_system_t sys, int32_t* knbr);
+/* 1 when the last build truncated a row at its capacity (the reference's did_buffer_overflow,
+ * deepmd_jax/md_utils.py:455-458), else 0.  dpmd_compute refuses such a list (DPMD_E_STATE)
+ * until dpmd_nlist_allocate measures new capacities. */
+int32_t dpmd_nlist_overflow(dpmd_system_t sys, int32_t* overflow);
 
 /* Energy, forces and virial of the current positions.  Replaces
  * DPModel.energy_and_force (deepmd_jax/dpmodel.py:106-108) and the strain derivative

@@ -150,24 +150,25 @@ def test_serving_md_images_stay_together(device_model):
     r = _requests()[1]
     k = image_factors(r.box, 7.0)
     assert int(np.prod(k)) == 8
-    srv = Server(device_model, md_chunk=10)
+    srv = Server(device_model, md_chunk=10, keep_images=True)
     try:
         srv.submit(r)
         assert srv.wait(1, timeout=120)
-        s = srv._md_sys
     finally:
         srv.close()
     res = srv.results[r.rid]
     assert res.error == 0 and np.isfinite(res.position).all()
-    # the finished segment's last device state: images are the primary shifted by box multiples
+    # the finished segment's last device state (all 8 images, pulled at completion): every
+    # image is the primary shifted by its box multiple
     from deepmd_jax.evaluator import replicate
     N = r.natoms
     sh = replicate(np.zeros((N, 3)), r.box, r.type_idx, k)[0]
-    # positions pulled at completion hold all images of the segment
-    full = s.split(s.get_positions())[0] if s is not None and s.nrep == 1 and s.natoms == 8 * N else None
-    if full is not None:
-        d = full - np.tile(full[:N], (8, 1)) - sh
-        assert np.abs(d).max() <= 1e-4
+    full = res.image_position
+    assert full.shape == (8 * N, 3)
+    np.testing.assert_allclose(full[:N], res.position, rtol=0, atol=0)
+    d = full - np.tile(full[:N], (8, 1)) - sh
+    assert np.abs(d).max() <= 1e-4
+    assert np.abs(full[:N] - r.coord).max() > 1e-3   # the segment moved
 
 
 def test_serving_general_cells(device_model, model0):
@@ -200,3 +201,49 @@ def test_serving_general_cells(device_model, model0):
         assert abs(res.energy - E.item()) / N <= 1e-5, r.rid
         assert np.abs(res.force - F.numpy()).max() <= 1e-4 * max(1.0, np.abs(F.numpy()).max()), r.rid
         assert np.abs(res.virial - W.numpy()).max() <= 1e-4 * np.abs(W.numpy()).max(), r.rid
+
+
+def test_serving_request_masses_and_capacity_reuse(device_model):
+    """An MD request carrying its own masses (heavy water, D = 2.0141 Da) gives the same end
+    state as a server whose default masses are those, and a different one from plain water;
+    a stream of evaluation batches reuses the first batch's row capacities (measuring again
+    only when a row overflows) and still matches the requests evaluated alone."""
+    from deepmd_jax import _lib
+    from deepmd_jax.serving import Request, Server
+    from deepmd_jax import synthetic as syn
+    print('case serving masses / capacity reuse', flush=True)
+    c, b, t = syn.water_box(216, seed=31, permute_sites=True)
+    v = syn.maxwell_boltzmann(t, 300.0, seed=31)
+    heavy = (15.9994, 2.0141)
+    ends = []
+    for req_m, srv_m in ((heavy, syn.MASSES), (None, heavy), (None, syn.MASSES)):
+        srv = Server(device_model, md_chunk=10, masses=srv_m)
+        try:
+            srv.submit(Request(rid=0, kind='md', coord=c, box=b, type_idx=t, nsteps=30, velocity=v, seed=5,
+                               masses=req_m))
+            assert srv.wait(1, timeout=120)
+        finally:
+            srv.close()
+        ends.append(srv.results[0].position)
+    np.testing.assert_allclose(ends[0], ends[1], rtol=0, atol=1e-5)   # force-atomic order only
+    assert np.abs(ends[0] - ends[2]).max() > 1e-4
+    # evaluation batches: capacities measured once, reused while they hold
+    boxes = _boxes([64, 125, 216, 100, 343, 150], seed0=40)
+    srv = Server(device_model)
+    try:
+        for i, (cc, bb, tt) in enumerate(boxes):
+            srv.submit(Request(rid=i, kind='eval', coord=cc, box=bb, type_idx=tt))
+            assert srv.wait(i + 1, timeout=120)
+    finally:
+        srv.close()
+    st = srv.stats
+    assert st['eval_batches'] == len(boxes) and st['eval_capacity_measurements'] < st['eval_batches']
+    for i, (cc, bb, tt) in enumerate(boxes[2:4]):
+        s = _lib.DeviceSystem(device_model, tt, 1)
+        s.set_box(bb)
+        s.set_positions(cc[None])
+        s.nlist_allocate(6.0, 1.2)
+        e1, f1, _, _ = s.compute('fp32')
+        r = srv.results[i + 2]
+        assert abs(r.energy - e1[0]) <= 1e-6 * abs(e1[0]) + 1e-6
+        assert np.abs(r.force - f1[0]).max() <= 1e-5 * np.abs(f1).max()
