@@ -312,11 +312,46 @@ def nve_conservation(model, params, var, nmol=32768, seeds=(0, 1, 2), clean_step
     return out
 
 
-def pimd(model, params, var, steps=100, warmup=10, precisions=('fp32', 'bf16'), acc_beads=4):
+def config0(model, params, var, steps=1000):
+    """The CPU-runnable configuration on the device: 64 H2O (192 atoms, L = 12.42 A, seed 0),
+    one E/F/virial evaluation against the FP64 C++ restatement, then 1,000 velocity-Verlet
+    steps at dt 0.5 fs from 300 K velocities through Simulation (the box is below
+    2 (rcut + skin), so the skin shrinks to fit the neighbour-list regime, as
+    deepmd_jax/md.py:360-370 decides between its list and lattice paths)."""
+    import contextlib
+    import io
+    sys.path.insert(0, ROOT)
+    from oracle import dp_oracle as O, dp_cpu
+    from deepmd_jax import synthetic as syn
+    from deepmd_jax.dpmodel import DPModel
+    from deepmd_jax.md import Simulation
+    coord, box, types = syn.water_box(64, seed=0)
+    vel = syn.maxwell_boltzmann(types, 300.0, seed=0)
+    with contextlib.redirect_stdout(io.StringIO()):
+        sim = Simulation(None, box, types, syn.MASSES, 'NVE', 0.5, coord, initial_velocity=vel,
+                         report_interval=10 ** 9, model=(DPModel(params), var, model))
+        E, F = sim.getEnergy(), sim.getForce()
+        t0 = time.perf_counter()
+        traj = sim.run(steps)
+        el = time.perf_counter() - t0
+    if not os.path.exists(dp_cpu.LIB):
+        dp_cpu.build()
+    nb, _, _ = O.typed_nlist(coord, box, types, params['rcut'] + 0.2)
+    rows, ks = dp_cpu.neighbour_rows(nb, types)
+    Er, Fr, _, _ = dp_cpu.energy_force(dp_cpu.Packed(params, var), coord, box, types, rows, ks, 'f64')
+    N = len(types)
+    return {'workload': '64 H2O (192 atoms, L=%.2f A), Simulation NVE dt 0.5 fs, %d steps' % (box[0], steps),
+            'steps_per_s': steps / el, 'wall_s': el, 'finite': bool(np.isfinite(traj['position']).all()),
+            'energy_per_atom_abs_err_vs_fp64': abs(E - Er) / N,
+            'force_max_abs_err_vs_fp64': float(np.abs(F - Fr).max())}
+
+
+def pimd(model, params, var, steps=1000, warmup=10, precisions=('fp32', 'bf16'), acc_beads=4, npt_steps=200):
     """Ring-polymer NVT (PILE-L, 300 K, dt 0.25 fs): 32 beads x 4,096 H2O (12,288 atoms per
     bead, L = 49.7 A), per-bead +-0.05 A uniform jitter from seed 3, all beads in one
-    system (one launch per kernel for all 393,216 atoms).  Device-resident bead*steps/s per
-    precision, and the centroid-force error against the FP64 oracle over the first
+    system (one launch per kernel for all 393,216 atoms), the configured 1,000 steps per
+    precision.  Device-resident bead*steps/s per precision, the path-integral NPT rate on
+    `npt_steps`, and the centroid-force error against the FP64 C++ restatement over the first
     `acc_beads` beads of the initial configuration."""
     from deepmd_jax import _lib, synthetic as syn
     from deepmd_jax.constants import MASS_UNIT_CONVERSION, TEMP_UNIT_CONVERSION
@@ -378,26 +413,30 @@ def pimd(model, params, var, steps=100, warmup=10, precisions=('fp32', 'bf16'), 
     run(warmup)
     s.synchronize()
     t0 = time.perf_counter()
-    run(steps)
+    run(npt_steps)
     s.synchronize()
     el = time.perf_counter() - t0
-    out['npt_fp32'] = {'steps_per_s': steps / el, 'bead_steps_per_s': P * steps / el,
-                       'ms_per_step': 1e3 * el / steps, 'steps': steps,
+    out['npt_fp32'] = {'steps_per_s': npt_steps / el, 'bead_steps_per_s': P * npt_steps / el,
+                       'ms_per_step': 1e3 * el / npt_steps, 'steps': npt_steps,
                        'box_rel_change': float(s.get_boxes()[0][0] / box[0] - 1.0)}
     p.routine, p.gamma = _lib.ROUTINE['PIMD'], 0.01
     if acc_beads:
-        import torch
         sys.path.insert(0, ROOT)
-        from oracle import dp_oracle as O
-        torch.set_num_threads(len(os.sched_getaffinity(0)))
+        from oracle import dp_cpu
+        if not os.path.exists(dp_cpu.LIB):
+            dp_cpu.build()
+        pk = dp_cpu.Packed(params, var)
         reset()
+        # the device's rows (bit-exact against the oracle's list) at rcut + 1 A; FP64 distances
+        off = np.concatenate([[0], np.cumsum(np.bincount(types))])
+        ex = [s.nlist_export(j) for j in range(2)]
         Fo = []
         for b in range(acc_beads):
-            nb, _, _ = O.typed_nlist(x[b], box, types, params['rcut'])
-            Fo.append(O.energy_force_virial(params, var, x[b], box, types, nbrs_nm=nb,
-                                            dtype=torch.float64)[1].numpy())
+            nb = [[ex[j][b][off[i]:off[i + 1]] for j in range(2)] for i in range(2)]
+            rows, ks = dp_cpu.neighbour_rows(nb, types)
+            Fo.append(dp_cpu.energy_force(pk, x[b], box, types, rows, ks, 'f64')[1])
         Fc = np.mean(Fo, 0)
-        out['centroid_force'] = {'beads': acc_beads, 'oracle': 'FP64, per bead',
+        out['centroid_force'] = {'beads': acc_beads, 'oracle': 'FP64 C++ restatement, per bead',
                                  'centroid_force_rms_eV_per_A': float(np.sqrt((Fc ** 2).mean()))}
         for prec in precisions:
             _, f, _, _ = s.compute(prec)
@@ -671,7 +710,9 @@ def main():
     ap.add_argument('--frames', type=int, default=16, help='seeded 98,304-atom frames for the precision errors')
     ap.add_argument('--acc-budget', type=float, default=240.0, help='CPU seconds for the FP64 reference frames')
     ap.add_argument('--no-conservation', action='store_true', help='skip the NVE energy-conservation leg')
-    ap.add_argument('--pimd-steps', type=int, default=100)
+    ap.add_argument('--deterministic', action='store_true',
+                    help='the timed system gathers pair gradients in a fixed order instead of force atomics')
+    ap.add_argument('--pimd-steps', type=int, default=1000)
     ap.add_argument('--pimd-acc-beads', type=int, default=2)
     ap.add_argument('--serving-rates', default='20,200,1000',
                     help='comma-separated Poisson arrival rates (requests/s)')
@@ -703,6 +744,7 @@ def main():
     t_setup = time.perf_counter()
     model = _lib.DeviceModel(params, var)
     sys_ = _lib.DeviceSystem(model, types, 1)
+    sys_.set_deterministic(args.deterministic)
     sys_.set_box(box)
     sys_.set_masses(np.array(syn.MASSES)[types] * MASS_UNIT_CONVERSION)
 
@@ -784,7 +826,7 @@ def main():
         'data': 'synthetic (seeded water box + seeded random-init model)',
         'config': {'workload': 'NVE velocity Verlet, %d-atom water (%d H2O, L=%.3f A), se_a 25/50/100 fit 240x3 '
                                'axis 16 rcut 6, dt 0.5 fs, skin 1 A' % (N, args.nmol, box[0]),
-                   'atoms': N, 'precision': args.precision},
+                   'atoms': N, 'precision': args.precision, 'deterministic_forces': bool(args.deterministic)},
         'atoms_steps_per_s': steps_per_s * N,
         'ns_per_day': steps_per_s * 0.5 * 86400 * 1e-6,
         # the reference's run-profile unit (deepmd_jax/md_utils.py:172-179): 1e-6 steps natoms / s
@@ -851,6 +893,7 @@ def main():
         out['dplr'] = dplr(model, params, coord, box, types, vel, leg_steps, args.warmup)
     if not args.no_published and args.nmol == 32768:
         out['published_model'] = published_model(coord, box, types, vel, leg_steps, args.warmup)
+    out['config0'] = config0(model, params, var)
     if not args.no_pimd:
         out['pimd'] = pimd(model, params, var, steps=args.pimd_steps, acc_beads=0 if args.no_acc else args.pimd_acc_beads)
     if not args.no_serving:
