@@ -30,10 +30,10 @@ PEAK_HBM = 8.0e12
 KERNELS = {0: 'nlist', 1: 'embed_fwd', 2: 'fit', 3: 'embed_bwd', 4: 'integrator'}
 # HBM bytes per launch of the FP32 embed_bwd kernel on the 98,304-atom box, from two
 # rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE in KiB per dispatch) scaled by what the
-# same counters read for a 256 MiB elementwise copy (x2.0 and x1.0): 1,075.7 MiB read +
-# 471.5 MiB written (profiles/embed_fit_nlist_water98k_trace.md).  Counters cannot be read
+# same counters read for a 256 MiB elementwise copy (x2.0 and x1.0): 1,074.8 MiB read +
+# 468.4 MiB written (profiles/embed_fit_nlist_water98k_trace.md).  Counters cannot be read
 # without the profiler, so the figure is carried here and re-measured with the kernel.
-EMBED_BWD_TRAFFIC_FP32 = (1075.7 + 471.5) * 2 ** 20
+EMBED_BWD_TRAFFIC_FP32 = (1074.8 + 468.4) * 2 ** 20
 
 
 def flops_per_step(N, pairs, A=16, W=100, emb=(25, 50, 100), fit=240):
@@ -710,6 +710,7 @@ def main():
     ap.add_argument('--frames', type=int, default=16, help='seeded 98,304-atom frames for the precision errors')
     ap.add_argument('--acc-budget', type=float, default=240.0, help='CPU seconds for the FP64 reference frames')
     ap.add_argument('--no-conservation', action='store_true', help='skip the NVE energy-conservation leg')
+    ap.add_argument('--no-config0', action='store_true', help='skip the 192-atom configuration leg')
     ap.add_argument('--deterministic', action='store_true',
                     help='the timed system gathers pair gradients in a fixed order instead of force atomics')
     ap.add_argument('--pimd-steps', type=int, default=1000)
@@ -893,7 +894,8 @@ def main():
         out['dplr'] = dplr(model, params, coord, box, types, vel, leg_steps, args.warmup)
     if not args.no_published and args.nmol == 32768:
         out['published_model'] = published_model(coord, box, types, vel, leg_steps, args.warmup)
-    out['config0'] = config0(model, params, var)
+    if not args.no_config0:
+        out['config0'] = config0(model, params, var)
     if not args.no_pimd:
         out['pimd'] = pimd(model, params, var, steps=args.pimd_steps, acc_beads=0 if args.no_acc else args.pimd_acc_beads)
     if not args.no_serving:

@@ -116,6 +116,23 @@ constexpr int kNtHidden = 5;
 constexpr int KC32 = 16;             // fp32 K granularity (slices of 16 or 32)
 constexpr int KC16 = 32;             // 16-bit K granularity (slices of 32 or 64)
 
+// Block tile (row block mb, column block nb) under an XCD-aware order.  The dispatcher deals
+// consecutive block ids (x fastest) round-robin over the 8 XCDs, each with its own L2; in
+// grid order the column blocks of one row panel reach their XCD a whole column pass apart,
+// after the panel has left that L2, so every column block fetched the panel from HBM again.
+// Renumbered, each XCD takes a contiguous run of tiles with the column blocks of a row
+// panel adjacent, and the panel is fetched once per XCD.  A bijection for any grid size.
+__device__ __forceinline__ void xcd_tile(int& mb, int& nb) {
+  const int nn = gridDim.y;
+  const int total = gridDim.x * nn;
+  const int bid = blockIdx.x + gridDim.x * blockIdx.y;
+  const int xcd = bid & 7, local = bid >> 3;
+  const int per = total >> 3, rem = total & 7;
+  const int t = xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+  mb = t / nn;
+  nb = t - mb * nn;
+}
+
 // FP32 GEMM: block tile 64 rows x 16 NT columns, four waves of 16 rows each over all NT
 // column tiles; K staged in KC-wide slices, KC = 32 when K allows (half the block barriers of
 // KC = 16: fitting time -4 %; 64 for the narrow tiles measured slower), else 16.
@@ -133,8 +150,10 @@ __global__ void __launch_bounds__(256, 2) gemm_f32_kernel(const GemmArgs g) {
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
   const int q = lane >> 4, n = lane & 15;
-  const int m0 = blockIdx.x * 64;
-  const int n0 = blockIdx.y * BN;
+  int mb, nb;
+  xcd_tile(mb, nb);
+  const int m0 = mb * 64;
+  const int n0 = nb * BN;
   const float* Bt = (const float*)g.Bt;
   floatx4 acc[NT];
 #pragma unroll
@@ -227,8 +246,10 @@ __global__ void __launch_bounds__(256, 2) gemm_h_kernel(const GemmArgs g) {
   const int lane = tid & 63, wv = tid >> 6;
   const int wm = wv & 1, wn = wv >> 1;
   const int q = lane >> 4, n = lane & 15;
-  const int m0 = blockIdx.x * 64;
-  const int n0 = blockIdx.y * BN;
+  int mb, nb;
+  xcd_tile(mb, nb);
+  const int m0 = mb * 64;
+  const int n0 = nb * BN;
   const uint16_t* Bt = (const uint16_t*)g.Bt;
   floatx4 acc[2][NH];
 #pragma unroll
