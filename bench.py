@@ -690,6 +690,43 @@ def serving(model, rates, precisions, n_requests=256, md_chunk=20, budget_gib=32
     return out
 
 
+def probe_issue(blocks=512, iters=20000, reps=5):
+    """Issue-overlap probe: does matrix-core work overlap FP32 VALU work on this GPU?
+
+    Each case runs 2 blocks per CU; every wave issues the same chains in every case (4
+    independent MFMA accumulators, 8 independent FMA chains x 4 per iteration).  If MFMA and
+    VALU work overlap, "separate waves" takes about the longer of the two alone; if they share
+    the issue path it takes about their sum."""
+    import ctypes
+    path = os.path.join(ROOT, 'deepmd-jax_amd', 'deepmd_jax', 'libdpmd_probe.so')
+    if not os.path.exists(path):
+        raise SystemExit('libdpmd_probe.so is not built (python deepmd-jax_amd/build.py)')
+    lib = ctypes.CDLL(path)
+    lib.dpmd_probe_issue.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_float)]
+    names = ['f32_mfma', 'valu', 'f32_mfma+valu_same_wave', 'f32_mfma|valu_separate_waves_halves',
+             'bf16_mfma', 'bf16_mfma+valu_same_wave', 'bf16_mfma|valu_separate_waves_halves',
+             'f32_mfma|valu_separate_waves_parity', 'bf16_mfma|valu_separate_waves_parity']
+    out = {}
+    for which, name in enumerate(names):
+        ms = ctypes.c_float()
+        if lib.dpmd_probe_issue(which, blocks, iters, reps, ctypes.byref(ms)) != 0:
+            raise SystemExit('probe case %s failed' % name)
+        out[name] = round(ms.value, 4)
+        print('probe %s: %.4f ms' % (name, ms.value), file=sys.stderr, flush=True)
+    m, v = out['f32_mfma'], out['valu']
+    mb = out['bf16_mfma']
+    for kind, mk in (('f32', m), ('bf16', mb)):
+        same = out['%s_mfma+valu_same_wave' % kind]
+        sep = min(out['%s_mfma|valu_separate_waves_halves' % kind], out['%s_mfma|valu_separate_waves_parity' % kind])
+        out['%s_same_wave_over_sum' % kind] = round(same / (mk + v), 3)
+        out['%s_separate_over_sum' % kind] = round(sep / (mk + v), 3)
+        out['%s_separate_over_max' % kind] = round(sep / max(mk, v), 3)
+    out['config'] = {'blocks': blocks, 'iters': iters, 'launches': reps, 'mfma_per_wave_iter': 4,
+                     'valu_fma_per_lane_iter': 32}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -721,7 +758,13 @@ def main():
     ap.add_argument('--calibrate-copy', action='store_true',
                     help='run a plain 256 MiB elementwise copy (read 256 MiB, write 256 MiB) five times, for '
                          'scaling FETCH_SIZE / WRITE_SIZE counters')
+    ap.add_argument('--probe-issue', action='store_true',
+                    help='time FP32 / BF16 MFMA chains and FP32 VALU chains alone, in one wave and in separate '
+                         'waves of one block (libdpmd_probe.so), and exit')
     args = ap.parse_args()
+    if args.probe_issue:
+        probe_issue()
+        return
     if args.calibrate_copy:
         import torch
         x = torch.ones(64 * 1024 * 1024, device='cuda')
@@ -732,8 +775,7 @@ def main():
         print(json.dumps({'copy_bytes_read': x.numel() * 4, 'copy_bytes_written': y.numel() * 4, 'launches': 5}))
         return
     if args.gpus != 1:
-        print('bench.py runs on one GPU; multi-GPU runs are not part of this build (--gpus %d requested)' % args.gpus,
-              file=sys.stderr)
+        print('bench.py runs on one GPU; --gpus %d is not supported' % args.gpus, file=sys.stderr)
         sys.exit(2)
     from deepmd_jax import _lib, synthetic as syn
     from deepmd_jax.constants import MASS_UNIT_CONVERSION

@@ -15,6 +15,9 @@ OUT_DIR = os.path.join(HERE, 'deepmd_jax')
 BUILD_DIR = os.path.join(HERE, '_build')
 SOURCES = ['api.hip', 'embed.hip', 'fit.hip', 'nlist.hip', 'md.hip', 'pppm.hip', 'dplr.hip']
 LIB_NAME = 'libdpmd_hip.so'
+# the issue-overlap probe (bench.py --probe-issue): a library of its own, outside the product
+PROBE_SOURCE = 'probe.hip'
+PROBE_LIB_NAME = 'libdpmd_probe.so'
 
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 FLAGS = [
@@ -37,18 +40,23 @@ def _compile(src):
 def build(jobs=4, verbose=True):
     os.makedirs(BUILD_DIR, exist_ok=True)
     with ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+        objs = list(ex.map(_compile, SOURCES + [PROBE_SOURCE]))
+    probe_obj = objs.pop()
+    _link([probe_obj], os.path.join(OUT_DIR, PROBE_LIB_NAME), [])
     lib = os.path.join(OUT_DIR, LIB_NAME)
+    _link(objs, lib, ['-L/opt/rocm/lib', '-lhipfft', '-Wl,-rpath,/opt/rocm/lib'])
+    if verbose:
+        print('built', lib)
+    return lib
+
+
+def _link(objs, lib, extra):
     tmp = lib + '.tmp'
-    cmd = [HIPCC, '-shared', '--offload-arch=gfx950', '-o', tmp] + objs + ['-L/opt/rocm/lib', '-lhipfft',
-                                                                            '-Wl,-rpath,/opt/rocm/lib']
+    cmd = [HIPCC, '-shared', '--offload-arch=gfx950', '-o', tmp] + objs + extra
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError('link failed:\n%s\n%s' % (' '.join(cmd), r.stderr[-8000:]))
     os.replace(tmp, lib)
-    if verbose:
-        print('built', lib)
-    return lib
 
 
 if __name__ == '__main__':
