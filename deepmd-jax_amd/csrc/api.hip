@@ -1,6 +1,7 @@
 // C ABI of libdpmd_hip.so (declared in include/dpmd.h): model upload, system state,
 // neighbour-list protocol, energy/force/virial evaluation and the device MD loop.
 #include "internal.h"
+#include <cstdlib>
 #include <math.h>
 #include <cmath>
 #include <stdarg.h>
@@ -438,6 +439,10 @@ static int32_t sys_create(Model* M, int nseg, const int32_t* seg_natoms, const i
   System* s = new System();
   s->model = M;
   for (int t = 0; t < kMaxT; ++t) { s->knbr[t] = 0; s->cap[t] = 0; }
+  // DPMD_FIT_PER_GEMM=1 keeps the fitting net's hidden layers on one GEMM launch each (the
+  // fused kernel is the default); for comparing the two paths
+  const char* pg = std::getenv("DPMD_FIT_PER_GEMM");
+  s->fit_fused = !(pg && pg[0] == '1');
   hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     set_error("HIP error '%s' creating a stream", hipGetErrorString(e));

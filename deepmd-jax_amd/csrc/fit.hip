@@ -424,6 +424,234 @@ static int32_t gemm_dispatch(hipStream_t st, const GemmArgs& g, int prec) {
 // Fitting-weight block offsets in Model::fit_off[t][*] (elements)
 enum { FK0T = 0, FK1T, FK2T, FK0, FK1, FK2, FB0, FB1, FB2, FDT1, FDT2, FK3, FB3 };
 
+// ---------------------------------------------------------------------------------
+// The hidden layers in one kernel: GEMMs 2-5 of fit_launch (the two hidden forward layers
+// with the energy head, and their reverse pass down to dE/dz1) for 64 atoms per block.
+// Orientation is transposed: MFMA rows are features, MFMA columns 16 atoms of one wave, so
+// a layer's output, register (t, i) of lane group q = feature 16 t + 4 q + i, is the next
+// layer's B operand as it stands (the K order of a sum is free; the weight operand is read
+// in the same order).  Activations, t2 and dy2 stay in registers; only the weights (A,
+// features x features) move, staged through LDS in K slices shared by the block's four
+// waves.  HBM traffic per atom: y1 read twice, dz1 written, one energy.
+struct HiddenArgs {
+  const float* Y1;   // rows x ld: y1 = tanh(G K0 + b0)
+  int ld;            // row stride of the activations and of the weight blocks (elements)
+  const void* W[4];  // A operands, row-major [m][k], stride ld: K1^T, K2^T (out x in), K2, K1 (in x out)
+  const float* b1;
+  const float* b2;
+  const float* dt1;
+  const float* dt2;
+  const float* k3;
+  float b3;
+  float* eout;       // per-row energy
+  float* DZ1;        // rows x ld: dz1 = dE/d(G K0 + b0)
+};
+
+// Live state per lane: NT x 4 accumulators, the layer input, and t2 (later dy2): 12 NT
+// registers before weight operands.  Up to 128-wide layers two blocks share a CU (256
+// registers per lane; the FP16 form's conversions need more); wider ones run one block per
+// CU with the full 512.
+// Block barrier ordering LDS only: __syncthreads() also orders global memory, which makes
+// every wave wait for all of its global loads in flight, the weight prefetch included.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+template <int NT, int PREC>
+__global__ void __launch_bounds__(256, (NT <= 8 && PREC != 2) ? 2 : 1) fit_hidden_kernel(const HiddenArgs h) {
+  constexpr bool H16 = PREC != 0;
+  constexpr int NS = H16 ? (NT + 1) / 2 : NT;   // K slices per layer: 32 features (16-bit) or 16
+  constexpr int LSW = H16 ? 40 : 20;            // LDS row stride of a staged slice (elements)
+  constexpr int ESZ = H16 ? 2 : 4;
+  constexpr int SLICE = 16 * NT * LSW * ESZ;    // bytes
+  constexpr int CH = 16 * NT * 4;               // 16-byte pieces per slice (4 per weight row)
+  constexpr int NCH = (CH + 255) / 256;
+  __shared__ __attribute__((aligned(16))) unsigned char Ws[2][SLICE];
+  __shared__ __attribute__((aligned(16))) unsigned char spill_slot[32];  // stores of pieces past CH
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int q = lane >> 4, n = lane & 15;
+  const int ld = h.ld;
+  const size_t row = (size_t)blockIdx.x * 64 + 16 * wv + n;  // this lane's atom
+
+  // slice s of a weight block into registers / from registers into LDS buffer b.  FP32: 16
+  // columns, LDS row = the 16 values in order.  16-bit: 32 columns, LDS row permuted so lane
+  // group q finds the 8 operands of its k-slots (columns 4q..4q+3 and 16+4q..16+4q+3) together.
+  // Branch free: only the last piece can lie past CH (256 (NCH - 1) < CH); there the load
+  // re-reads piece CH - 1 and the store goes to spill_slot.
+  auto piece = [&](int p) {
+    const int c = tid + 256 * p;
+    return (p + 1 < NCH || CH % 256 == 0) ? c : min(c, CH - 1);
+  };
+  auto stage_load = [&](const void* Wg, int s, uintx4 (&st)[NCH]) {
+#pragma unroll
+    for (int p = 0; p < NCH; ++p) {
+      const int c = piece(p);
+      const int r = c >> 2, u = c & 3;
+      const unsigned char* src = (const unsigned char*)Wg + ((size_t)r * ld + (size_t)s * (H16 ? 32 : 16)) * ESZ + 16 * u;
+      st[p] = *reinterpret_cast<const uintx4*>(src);
+    }
+  };
+  auto stage_store = [&](int b, const uintx4 (&st)[NCH]) {
+#pragma unroll
+    for (int p = 0; p < NCH; ++p) {
+      const int c = piece(p);
+      const int r = c >> 2, u = c & 3;
+      const bool past = (p + 1 == NCH && CH % 256 != 0) && tid + 256 * p >= CH;
+      if constexpr (H16) {
+        uint16_t* d = reinterpret_cast<uint16_t*>(Ws[b]) + r * LSW + 16 * (u & 1) + 4 * (u >> 1);
+        if (p + 1 == NCH && CH % 256 != 0) d = past ? reinterpret_cast<uint16_t*>(spill_slot) : d;
+        *reinterpret_cast<uint2*>(d) = make_uint2(st[p][0], st[p][1]);
+        *reinterpret_cast<uint2*>(d + 8) = make_uint2(st[p][2], st[p][3]);
+      } else {
+        float* d = reinterpret_cast<float*>(Ws[b]) + r * LSW + 4 * u;
+        if (p + 1 == NCH && CH % 256 != 0) d = past ? reinterpret_cast<float*>(spill_slot) : d;
+        *reinterpret_cast<uintx4*>(d) = st[p];
+      }
+    }
+  };
+  // The four layers' weight slices form one sequence g = l NS + s, loaded PF slices ahead
+  // into registers (a 16-bit slice is only NT MFMAs of work, far shorter than an L2 round
+  // trip) and stored into the LDS buffer g & 1 at the end of slice g - 1; the prefetch runs
+  // across layer boundaries, so the epilogues overlap the next layer's first loads.
+  constexpr int PF = H16 ? 4 : 1;
+  constexpr int NG = 4 * NS;
+  uintx4 st[PF][NCH];
+  // acc[ot] = sum_k W_l[16 ot + .][k] act[k][.] over layer l's K slices
+  auto layer = [&](auto lc, floatx4 (&acc)[NT], const float (&bop)[NT][4]) {
+    constexpr int l = decltype(lc)::value;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const int g = l * NS + s;
+      if (g + PF < NG) stage_load(h.W[(g + PF) / NS], (g + PF) % NS, st[g % PF]);
+      // keep every slice's loads, MFMAs and stores in their slice (the scheduler would
+      // otherwise hoist the prefetch of many slices at once and wait for all of them)
+      __builtin_amdgcn_sched_barrier(0);
+      if constexpr (H16) {
+        typedef typename Half8<PREC>::T HT;
+        typedef typename Half8<PREC>::E HE;
+        HT b;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int t = 2 * s + j / 4;
+          b[j] = t < NT ? (HE)bop[t < NT ? t : 0][j % 4] : (HE)0.f;
+        }
+        const uint16_t* Wl = reinterpret_cast<const uint16_t*>(Ws[g & 1]);
+#pragma unroll
+        for (int ot = 0; ot < NT; ++ot) {
+          const HT av = *reinterpret_cast<const HT*>(Wl + (16 * ot + n) * LSW + 8 * q);
+          acc[ot] = Half8<PREC>::mfma(av, b, acc[ot]);
+        }
+      } else {
+        const float* Wl = reinterpret_cast<const float*>(Ws[g & 1]);
+#pragma unroll
+        for (int ot = 0; ot < NT; ++ot) {
+          const floatx4 av = *reinterpret_cast<const floatx4*>(Wl + (16 * ot + n) * LSW + 4 * q);
+#pragma unroll
+          for (int i = 0; i < 4; ++i) acc[ot] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[i], bop[s][i], acc[ot], 0, 0, 0);
+        }
+      }
+      if (g + 1 < NG) {
+        stage_store((g + 1) & 1, st[(g + 1) % PF]);
+        lds_barrier();
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  };
+  auto vec4 = [&](const float* p, int t) { return *reinterpret_cast<const floatx4*>(p + 16 * t + 4 * q); };
+
+  float bop[NT][4], keep[NT][4];
+  floatx4 acc[NT];
+  const float* y1row = h.Y1 + row * ld;
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const floatx4 v = vec4(y1row, t);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) bop[t][i] = v[i];
+  }
+#pragma unroll
+  for (int p = 0; p < PF; ++p) stage_load(h.W[p / NS], p % NS, st[p]);
+  stage_store(0, st[0]);
+  lds_barrier();
+  // t2 = tanh(y1 K1 + b1), y2 = t2 dt1 + y1
+  layer(std::integral_constant<int, 0>(), acc, bop);
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const floatx4 bb = vec4(h.b1, t), dd = vec4(h.dt1, t);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float t2 = tanh_fit(acc[t][i] + bb[i]);
+      keep[t][i] = t2;
+      bop[t][i] = t2 * dd[i] + bop[t][i];
+    }
+  }
+  // t3 = tanh(y2 K2 + b2), y3 = t3 dt2 + y2, e = y3 k3 + b3, dz3 = k3 dt2 (1 - t3^2)
+  layer(std::integral_constant<int, 1>(), acc, bop);
+  float esum = 0.f;
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const floatx4 bb = vec4(h.b2, t), dd = vec4(h.dt2, t), kk = vec4(h.k3, t);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float t3 = tanh_fit(acc[t][i] + bb[i]);
+      const float y3 = t3 * dd[i] + bop[t][i];
+      esum += y3 * kk[i];
+      bop[t][i] = kk[i] * dd[i] * (1.f - t3 * t3);
+    }
+  }
+  esum += __shfl_xor(esum, 16);
+  esum += __shfl_xor(esum, 32);
+  if (q == 0) h.eout[row] = esum + h.b3;
+  // dy2 = k3 + dz3 K2^T, dz2 = dy2 dt1 (1 - t2^2)
+  layer(std::integral_constant<int, 2>(), acc, bop);
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const floatx4 kk = vec4(h.k3, t), dd = vec4(h.dt1, t);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float dy2 = kk[i] + acc[t][i];
+      bop[t][i] = dy2 * dd[i] * (1.f - keep[t][i] * keep[t][i]);
+      keep[t][i] = dy2;
+    }
+  }
+  // dy1 = dy2 + dz2 K1^T, dz1 = dy1 (1 - y1^2)
+  layer(std::integral_constant<int, 3>(), acc, bop);
+  float* dzrow = h.DZ1 + row * ld;
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const floatx4 y1 = vec4(y1row, t);
+    floatx4 o;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) o[i] = (keep[t][i] + acc[t][i]) * (1.f - y1[i] * y1[i]);
+    *reinterpret_cast<floatx4*>(dzrow + 16 * t + 4 * q) = o;
+  }
+}
+
+template <int NT>
+static int32_t hidden_go(hipStream_t st, const HiddenArgs& h, int rows, int prec) {
+  const dim3 grid(rows / 64);
+  if (prec == DPMD_PREC_BF16) hipLaunchKernelGGL((fit_hidden_kernel<NT, 1>), grid, dim3(256), 0, st, h);
+  else if (prec == DPMD_PREC_FP16) hipLaunchKernelGGL((fit_hidden_kernel<NT, 2>), grid, dim3(256), 0, st, h);
+  else hipLaunchKernelGGL((fit_hidden_kernel<NT, 0>), grid, dim3(256), 0, st, h);
+  DPMD_HIP(hipGetLastError());
+  return DPMD_OK;
+}
+
+// Launches the fused hidden layers when the width has an instantiation; returns false (and
+// launches nothing) otherwise, for the per-GEMM path.
+static bool hidden_fused(hipStream_t st, const HiddenArgs& h, int FN, int rows, int prec, int32_t* rc) {
+  if (rows % 64) return false;
+  switch (FN / 16) {
+    case 8: *rc = hidden_go<8>(st, h, rows, prec); return true;
+    case 15: *rc = hidden_go<15>(st, h, rows, prec); return true;
+    case 16: *rc = hidden_go<16>(st, h, rows, prec); return true;
+    default: return false;
+  }
+}
+
 int32_t fit_launch(System* s, int precision) {
   Model* M = s->model;
   const int FN = s->fpad;          // output width (multiple of 16)
@@ -461,33 +689,55 @@ int32_t fit_launch(System* s, int precision) {
     g.Bt = B(FK0T); g.ldb = D; g.N = FN;
     g.C = Y1 + r0 * FK; g.ldc = FK; g.bias = Wf + o[FB0];
     DPMD_TRY(gemm_dispatch<EPI_TANH>(s->stream, g, precision));
-    // 2: t2 = tanh(y1 K1 + b1), y2 = t2 dt1 + y1
-    g.A = Y1 + r0 * FK; g.Ah = nullptr; g.lda = FK; g.K = FK;
-    g.Bt = B(FK1T); g.ldb = FK;
-    g.C = Y2 + r0 * FK; g.C2 = T2 + r0 * FK; g.aux1 = Y1 + r0 * FK;
-    g.bias = Wf + o[FB1]; g.dt = Wf + o[FDT1];
-    DPMD_TRY(gemm_dispatch<EPI_RES>(s->stream, g, precision));
-    // 3: t3, y3, e, dz3
-    g.A = Y2 + r0 * FK;
-    g.Bt = B(FK2T);
-    g.C = DZ3 + r0 * FK; g.C2 = nullptr; g.aux1 = Y2 + r0 * FK;
-    g.bias = Wf + o[FB2]; g.dt = Wf + o[FDT2]; g.k3 = Wf + o[FK3];
-    g.b3 = M->fit_b3[t];
-    g.eout = s->d_eatom + r0;
-    DPMD_TRY(gemm_dispatch<EPI_E3>(s->stream, g, precision));
-    // 4: dy2 = k3 + dz3 K2^T, dz2 = dy2 dt1 (1 - t2^2)
-    g.A = DZ3 + r0 * FK;
-    g.Bt = B(FK2);
-    g.C = DZ2 + r0 * FK; g.C2 = DY2 + r0 * FK; g.aux1 = T2 + r0 * FK;
-    g.dt = Wf + o[FDT1];
-    DPMD_TRY(gemm_dispatch<EPI_BWD2>(s->stream, g, precision));
-    // 5: dy1 = dy2 + dz2 K1^T, dz1 = dy1 (1 - y1^2)
-    g.A = DZ2 + r0 * FK;
-    g.Bt = B(FK1);
-    g.C = DZ1 + r0 * FK; g.C2 = nullptr; g.aux1 = DY2 + r0 * FK; g.aux2 = Y1 + r0 * FK;
-    DPMD_TRY(gemm_dispatch<EPI_BWD1>(s->stream, g, precision));
+    // 2-5 in one kernel where the width has an instantiation
+    HiddenArgs hd;
+    memset(&hd, 0, sizeof(hd));
+    hd.Y1 = Y1 + r0 * FK;
+    hd.ld = FK;
+    hd.W[0] = B(FK1T);
+    hd.W[1] = B(FK2T);
+    hd.W[2] = B(FK2);
+    hd.W[3] = B(FK1);
+    hd.b1 = Wf + o[FB1];
+    hd.b2 = Wf + o[FB2];
+    hd.dt1 = Wf + o[FDT1];
+    hd.dt2 = Wf + o[FDT2];
+    hd.k3 = Wf + o[FK3];
+    hd.b3 = M->fit_b3[t];
+    hd.eout = s->d_eatom + r0;
+    hd.DZ1 = DZ1 + r0 * FK;
+    int32_t hrc = DPMD_OK;
+    if (s->fit_fused && hidden_fused(s->stream, hd, FN, rows, precision, &hrc)) {
+      DPMD_TRY(hrc);
+    } else {
+      // 2: t2 = tanh(y1 K1 + b1), y2 = t2 dt1 + y1
+      g.A = Y1 + r0 * FK; g.Ah = nullptr; g.lda = FK; g.K = FK;
+      g.Bt = B(FK1T); g.ldb = FK;
+      g.C = Y2 + r0 * FK; g.C2 = T2 + r0 * FK; g.aux1 = Y1 + r0 * FK;
+      g.bias = Wf + o[FB1]; g.dt = Wf + o[FDT1];
+      DPMD_TRY(gemm_dispatch<EPI_RES>(s->stream, g, precision));
+      // 3: t3, y3, e, dz3
+      g.A = Y2 + r0 * FK;
+      g.Bt = B(FK2T);
+      g.C = DZ3 + r0 * FK; g.C2 = nullptr; g.aux1 = Y2 + r0 * FK;
+      g.bias = Wf + o[FB2]; g.dt = Wf + o[FDT2]; g.k3 = Wf + o[FK3];
+      g.b3 = M->fit_b3[t];
+      g.eout = s->d_eatom + r0;
+      DPMD_TRY(gemm_dispatch<EPI_E3>(s->stream, g, precision));
+      // 4: dy2 = k3 + dz3 K2^T, dz2 = dy2 dt1 (1 - t2^2)
+      g.A = DZ3 + r0 * FK;
+      g.Bt = B(FK2);
+      g.C = DZ2 + r0 * FK; g.C2 = DY2 + r0 * FK; g.aux1 = T2 + r0 * FK;
+      g.dt = Wf + o[FDT1];
+      DPMD_TRY(gemm_dispatch<EPI_BWD2>(s->stream, g, precision));
+      // 5: dy1 = dy2 + dz2 K1^T, dz1 = dy1 (1 - y1^2)
+      g.A = DZ2 + r0 * FK;
+      g.Bt = B(FK1);
+      g.C = DZ1 + r0 * FK; g.C2 = nullptr; g.aux1 = DY2 + r0 * FK; g.aux2 = Y1 + r0 * FK;
+      DPMD_TRY(gemm_dispatch<EPI_BWD1>(s->stream, g, precision));
+    }
     // 6: dG = dz1 K0^T
-    g.A = DZ1 + r0 * FK;
+    g.A = DZ1 + r0 * FK; g.Ah = nullptr; g.lda = FK;
     g.Bt = B(FK0); g.ldb = FK; g.K = FK;
     g.N = D;
     g.C = s->d_dG + r0 * D; g.ldc = D;

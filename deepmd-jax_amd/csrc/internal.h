@@ -250,6 +250,7 @@ struct System {
   // (NA, sum cap, 3), per atom its centre part (NA, 3), and per row entry the entry's slot in
   // the neighbour's own row (d_rev[j]: (NA, cap_j), -1 when absent)
   bool deterministic = false;
+  bool fit_fused = true;        // fitting hidden layers in one kernel (fit_hidden_kernel)
   float* d_pgrad = nullptr;
   float* d_fcen = nullptr;
   int* d_rev[kMaxT] = {nullptr};
