@@ -322,6 +322,16 @@ struct H8<2> {
   }
 };
 
+// 16-bit operand bits of v (round to nearest even)
+template <int PREC>
+__device__ __forceinline__ uint16_t h16_bits(float v) {
+  return __builtin_bit_cast(uint16_t, (typename H8<PREC>::E)v);
+}
+
+// Per-wave LDS of the 16-bit forms' FP16 dE/dT operand copies (floats): dTa [tile pair][lane
+// group][x 0..10][8] for the dR products, dTb [slot][x 0..15] for the dh3 products.
+constexpr int embed_dt16_floats(int T3) { return ((11 * ((T3 + 1) / 2) * 32 + 16 * T3 * 16) / 2 + 3) / 4 * 4; }
+
 // B operand of a 16x16x32 step from 8 consecutive fp32 registers r0..r0+7 (zero past n)
 template <int PREC, int NREG, typename GET>
 __device__ __forceinline__ typename H8<PREC>::T pack8(GET get, int s2) {
@@ -511,6 +521,15 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
   float* Rl = wbase + 4 * a.list_cap;  // [16 neighbours][kRS]
   float* Hl = Rl + 16 * kRS;            // one tile group of h3: [16 neighbours][HG]
   constexpr int HG = 16 * kGroup + 1;
+  // 16-bit MLP forms: the T reducer on FP16 MFMAs (R, h3: bounded values, FP16's 11-bit
+  // significand; BF16 operands here would double the form's force error), h3 of a tile
+  // group held FP16 as [slot][nbr] in Hl's space, row stride kSH
+  constexpr bool H16 = PREC != 0 && !TAB;
+  typedef typename H8<2>::T HT16;
+  typedef typename H8<2>::E HE16;
+  constexpr int kSH = 24;
+  static_assert(16 * kGroup * kSH * 2 <= 16 * HG * 4, "16-bit h3 tile group exceeds its LDS");
+  uint16_t* HlT = reinterpret_cast<uint16_t*>(Hl);
 
   const SegInfo& S = a.seg[a.blk_seg[blockIdx.x]];
   const int c = (blockIdx.x - S.blk0) * 4 + wv;
@@ -558,16 +577,34 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
         for (int y = 0; y < 10; ++y) v = x == y ? R[y] : v;
         Rl[n * kRS + x] = v;
       }
+      // 16-bit forms: the reducer's A operand R[nbr 8q + j][x = n] (lane groups 0, 1; the
+      // 16 neighbours fill half of the K = 32 of a 16x16x32 step)
+      HT16 aR{};
+      if constexpr (H16) {
+        wave_sync();
+#pragma unroll
+        for (int j = 0; j < 8; ++j) aR[j] = (HE16)(q < 2 ? Rl[(8 * (q & 1) + j) * kRS + n] : 0.f);
+      }
       // reducer for one group of layer-3 tiles held in Hl:
       // T[x][slot] += sum_nbr R[nbr][x] h3[nbr][slot]
       auto reduce_group = [&](auto g0c, auto szc) {
         constexpr int g0 = decltype(g0c)::value, SZ = decltype(szc)::value;
         wave_sync();
+        if constexpr (H16) {
+          // h3 held 16-bit as [slot][nbr] (HlT): one 16-byte read per tile; lane groups 2, 3
+          // re-read the finite values of groups 0, 1 against their zero A operand
 #pragma unroll
-        for (int kk = 0; kk < 4; ++kk) {
-          const float av = Rl[(4 * kk + q) * kRS + n];
+          for (int k = 0; k < SZ; ++k) {
+            const HT16 bH = *reinterpret_cast<const HT16*>(HlT + (16 * k + n) * kSH + 8 * (q & 1));
+            Tacc[g0 + k] = H8<2>::mfma(aR, bH, Tacc[g0 + k]);
+          }
+        } else {
 #pragma unroll
-          for (int k = 0; k < SZ; ++k) Tacc[g0 + k] = mfma4(av, Hl[(4 * kk + q) * HG + 16 * k + n], Tacc[g0 + k]);
+          for (int kk = 0; kk < 4; ++kk) {
+            const float av = Rl[(4 * kk + q) * kRS + n];
+#pragma unroll
+            for (int k = 0; k < SZ; ++k) Tacc[g0 + k] = mfma4(av, Hl[(4 * kk + q) * HG + 16 * k + n], Tacc[g0 + k]);
+          }
         }
         wave_sync();
       };
@@ -636,7 +673,8 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
               const int r = 4 * (g0 + k) + i;
               float v = tanh_s(acc3[k][i] + b2(r));
               if (r / Z3 < 4 * T2) v += h2[r / Z3];
-              Hl[n * HG + 16 * k + 4 * q + i] = v;
+              if constexpr (H16) HlT[(16 * k + 4 * q + i) * kSH + n] = h16_bits<2>(v);
+              else Hl[n * HG + 16 * k + 4 * q + i] = v;
             }
           reduce_group(g0c, szc);
         });
@@ -766,11 +804,23 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
   float* small = smem;
   float* wbase = load_small_tables<T1, T2, T3>(a, smem) + wv * a.lds_wave_floats;
   // chunk-phase layout
+  // 16-bit MLP forms: the dR and dh3 products on FP16 MFMAs (dE/dT, h3, R: bounded values;
+  // FP16's 11-bit significand), from FP16 copies of dE/dT laid out as their operands (dTa, dTb)
+  constexpr bool H16 = PREC != 0 && !TAB;
+  typedef typename H8<2>::T HT16;
+  typedef typename H8<2>::E HE16;
+  constexpr int NG2 = (T3 + 1) / 2;
+  constexpr int DTA = 11 * NG2 * 32;
+  constexpr int DTB = 16 * T3 * 16;
+  // the copies occupy the start of the wave's LDS in the chunk phase (dTp, which they are
+  // made from, lives there only during the prologue)
+  uint16_t* dTa = reinterpret_cast<uint16_t*>(wbase);
+  uint16_t* dTb = dTa + DTA;
   // dE/dT rows 0..9 and a zero row 10: MFMA operand rows beyond the ten environment rows
   // read the zero row instead of being selected away per product
   constexpr int ODT = (11 * HS + 3) / 4 * 4;
   float* dTp = wbase;                    // 10 * HS (rows x >= 10 of the MFMA operands are zero)
-  float* Rl = dTp + ODT;                 // [16 neighbours][kRS]
+  float* Rl = wbase + (H16 ? embed_dt16_floats(T3) : ODT);  // [16 neighbours][kRS]
   float* dRl = Rl + 16 * kRS;            // [16 neighbours][kRS]
   float4* list = reinterpret_cast<float4*>(dRl + 16 * kRS);  // 16-byte aligned
 
@@ -993,6 +1043,32 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
         }
       }
       wave_sync();
+      if constexpr (H16) {
+        // dTa[pair g][lane group][x][j] = dT[x][16 (2g + j/4) + 4 lane group + j%4] (x = 10
+        // zero): the 16 lanes of a group read 16 consecutive byte-16 pieces.  dTb[slot][x] =
+        // dT[x][slot] (x >= 10 zero).  Both overwrite dTp: read into registers first.
+        static_assert(DTA % 64 == 0 && DTB % 64 == 0, "conversion loops assume whole passes");
+        float va[DTA / 64], vb[DTB / 64];
+#pragma unroll
+        for (int it = 0; it < DTA / 64; ++it) {
+          const int e = lane + 64 * it;
+          const int j = e & 7, rest = e >> 3, x = rest % 11, gq = rest / 11;
+          const int qq = gq & 3, t = 2 * (gq >> 2) + j / 4;
+          va[it] = x < 10 && t < T3 ? dTp[x * HS + 16 * t + 4 * qq + (j & 3)] : 0.f;
+        }
+#pragma unroll
+        for (int it = 0; it < DTB / 64; ++it) {
+          const int e = lane + 64 * it;
+          const int x = e & 15, slot = e >> 4;
+          vb[it] = x < 10 ? dTp[x * HS + slot] : 0.f;
+        }
+        wave_sync();
+#pragma unroll
+        for (int it = 0; it < DTA / 64; ++it) dTa[lane + 64 * it] = h16_bits<2>(va[it]);
+#pragma unroll
+        for (int it = 0; it < DTB / 64; ++it) dTb[lane + 64 * it] = h16_bits<2>(vb[it]);
+        wave_sync();
+      }
     }
     // ---- chunks
     // per-lane dE/dT operand rows: dR products use row n, dh3 products rows 4 kk + q
@@ -1033,6 +1109,12 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
           Rl[n * kRS + x] = v;
         }
         float dst;
+        // 16-bit MLP forms: the dh3 products' B operand R[nbr n][x = 8q + j] from this lane's R
+        HT16 bR{};
+        if constexpr (H16) {
+#pragma unroll
+          for (int j = 0; j < 8; ++j) bR[j] = (HE16)(q == 0 ? R[j] : (q == 1 && j < 2) ? R[8 + j] : 0.f);
+        }
         if constexpr (TAB) {
           // tabulated net: h3 = p(x0) and dh3/ds~ = p'(x0) from one read of the interval's
           // coefficients; dE/ds~ = sum_slot dh3[slot] p'[slot] (no gradient through idx)
@@ -1130,24 +1212,47 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
             for (int k = 0; k < SZ; ++k)
 #pragma unroll
               for (int i = 0; i < 4; ++i) t3[4 * k + i] = tanh_s(acc3[k][i] + b2(4 * (g0 + k) + i));
-            // dR^T[x][nbr] += sum_slot dT[x][slot] h3[slot][nbr] over this group's slots
-#pragma unroll
-            for (int k = 0; k < SZ; ++k)
-#pragma unroll
-              for (int i = 0; i < 4; ++i) {
-                const int r = 4 * (g0 + k) + i;
-                const float h3v = t3[4 * k + i] + ((r / Z3 < 4 * T2) ? H2(r / Z3, t2) : 0.f);
-                dr = mfma4(dTr[16 * (g0 + k) + i], h3v, dr);
-              }
-            // dh3[slot][nbr] = sum_x dT[x][slot] R[nbr][x]
             floatx4 dh3[SZ];
+            if constexpr (H16) {
+              // dR^T[x][nbr] += sum over the group's 32 slots in one 16x16x32 step: A = dT rows
+              // (dTa, row n, zero row past 9), B = this lane's h3 of the group
+              static_assert(g0 % 2 == 0, "tile groups start on a tile pair");
+              HT16 bh;
 #pragma unroll
-            for (int k = 0; k < SZ; ++k) {
-              floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
+              for (int j = 0; j < 8; ++j) {
+                const int k = j / 4, r = 4 * (g0 + k) + (j & 3);
+                const float h3v = k < SZ ? t3[4 * (k < SZ ? k : 0) + (j & 3)] + ((r / Z3 < 4 * T2) ? H2(r / Z3, t2) : 0.f)
+                                         : 0.f;
+                bh[j] = (HE16)h3v;
+              }
+              const HT16 adt = *reinterpret_cast<const HT16*>(dTa + (((g0 / 2) * 4 + q) * 11 + (n < 10 ? n : 10)) * 8);
+              dr = H8<2>::mfma(adt, bh, dr);
+              // dh3[slot][nbr] = sum_x dT[x][slot] R[nbr][x]: K = x (10 of 32); lane groups 2,
+              // 3 re-read the finite values of groups 0, 1 against their zero B operand
 #pragma unroll
-              for (int kk = 0; kk < 3; ++kk)
-                acc = mfma4(kk < 2 ? dTp[(4 * kk + q) * HS + 16 * (g0 + k) + n] : dTc2[16 * (g0 + k)], Rl[n * kRS + 4 * kk + q], acc);
-              dh3[k] = acc;
+              for (int k = 0; k < SZ; ++k) {
+                const HT16 at = *reinterpret_cast<const HT16*>(dTb + (16 * (g0 + k) + n) * 16 + 8 * (q & 1));
+                dh3[k] = H8<2>::mfma(at, bR, floatx4{0.f, 0.f, 0.f, 0.f});
+              }
+            } else {
+              // dR^T[x][nbr] += sum_slot dT[x][slot] h3[slot][nbr] over this group's slots
+#pragma unroll
+              for (int k = 0; k < SZ; ++k)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                  const int r = 4 * (g0 + k) + i;
+                  const float h3v = t3[4 * k + i] + ((r / Z3 < 4 * T2) ? H2(r / Z3, t2) : 0.f);
+                  dr = mfma4(dTr[16 * (g0 + k) + i], h3v, dr);
+                }
+              // dh3[slot][nbr] = sum_x dT[x][slot] R[nbr][x]
+#pragma unroll
+              for (int k = 0; k < SZ; ++k) {
+                floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int kk = 0; kk < 3; ++kk)
+                  acc = mfma4(kk < 2 ? dTp[(4 * kk + q) * HS + 16 * (g0 + k) + n] : dTc2[16 * (g0 + k)], Rl[n * kRS + 4 * kk + q], acc);
+                dh3[k] = acc;
+              }
             }
             // repeat-residual: dh2[r/Z3] += dh3[r]; then dz3 = dh3 (1 - t3^2)
 #pragma unroll
@@ -1351,6 +1456,9 @@ static int32_t launch_embed(System* s, bool backward, int precision) {
   } else {
     wave_floats = (11 * HS + 3) / 4 * 4 +
                   std::max((size_t)(2 * 16 * kRS + 4 * a.list_cap), (size_t)(10 * W + std::max(20 * A, 22 * 16)));
+    // 16-bit MLP forms: the FP16 dE/dT copies replace dTp in the chunk phase
+    if (PREC != 0 && !TAB)
+      wave_floats = std::max(wave_floats, (size_t)embed_dt16_floats(T3) + 2 * 16 * kRS + 4 * a.list_cap);
   }
   wave_floats = (wave_floats + 3) / 4 * 4;
   a.lds_wave_floats = (int)wave_floats;
