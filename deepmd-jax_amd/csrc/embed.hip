@@ -432,6 +432,10 @@ __device__ __forceinline__ void for_groups(F&& f) {
 }
 
 constexpr int kGroup = 2;  // layer-3 tiles processed together
+// Row stride (floats) of the tabulated backward's [neighbour][slot] tile of a group: a multiple
+// of 4 (16-byte rows), 36 mod 64 so that 16 rows' 16-byte accesses hit distinct banks.
+constexpr int kTS = 36;
+static_assert(16 * kGroup <= kTS && kTS % 4 == 0, "a tile group's slots fit one 16-byte aligned row");
 // Row stride (floats) of the per-chunk [16 neighbours][env row] LDS tiles: odd, so the
 // 16 neighbours of a column read or write land in distinct banks.
 constexpr int kRS = 17;
@@ -554,7 +558,7 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
     constexpr int O_K0 = 0, O_B0 = 16 * T1, O_B1 = 32 * T1, O_B2 = 32 * T1 + 16 * T2;
     constexpr int P0 = ((16 * T1 * 2 + 16 * T2 + 16 * T3) + 63) / 64 * 64;
     constexpr int O_W1A = P0, O_W2A = O_W1A + T2 * 4 * T1 * 64;
-    const int vl4 = lane * 4, vq16 = q * 16, vl16 = lane * 16;
+    const int vl4 = lane * 4, vl16 = lane * 16;
     const int hnetb = PREC ? __builtin_amdgcn_readfirstlane((ti * a.ntypes + j) * a.hnet_stride) : 0;
     for (int ch = 0; ch < m; ch += 16) {
       const int nb = netb + opaque_zero();
@@ -609,11 +613,18 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
         wave_sync();
       };
       if constexpr (TAB) {
-        // tabulated net: h3 = sum_c coeff[row][c] x0^(5-c) (Horner), four slots per load
+        // tabulated net: h3 = sum_c coeff[row][c] x0^(5-c) (Horner), four slots per load.
+        // The gathers run with lane 4m + p on neighbour m, slots 4p..4p+3 of a tile: the four
+        // consecutive lanes of a neighbour read its 64 contiguous bytes, one cache access where
+        // lane (q, n) = 16q + n made four (the L1 handles about one access per clock, which
+        // bounded these kernels); h3 lands in LDS in the same [neighbour][slot] layout.
         const int net = ti * a.ntypes + j;
         float x0;
         const int row = tab_interval(g.st, a.tab_srmin[net], a.tab_delta[net], a.tab_n, x0);
-        const int voff = row * (6 * 64 * T3) + vq16;
+        const int gm = lane >> 2, gp = lane & 3;  // gather neighbour and quarter of this lane
+        const int rowm = __shfl(row, gm);
+        const float x0m = __shfl(x0, gm);
+        const int voff = rowm * (6 * 64 * T3) + 16 * gp;
         const int tb = __builtin_amdgcn_readfirstlane(net * a.tab_stride * 4) + opaque_zero();
         const rsrc_t trs = __builtin_amdgcn_make_buffer_rsrc((void*)a.tab, (short)0, a.tab_bytes, 0x00020000);
         for_groups<0, T3, kGroup>([&](auto g0c, auto szc) {
@@ -627,8 +638,8 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
             for (int i = 0; i < 4; ++i) {
               float v = cf[0][i];
 #pragma unroll
-              for (int cc = 1; cc < 6; ++cc) v = __builtin_fmaf(v, x0, cf[cc][i]);
-              Hl[n * HG + 16 * k + 4 * q + i] = v;
+              for (int cc = 1; cc < 6; ++cc) v = __builtin_fmaf(v, x0m, cf[cc][i]);
+              Hl[gm * HG + 16 * k + 4 * gp + i] = v;
             }
           }
           reduce_group(g0c, szc);
@@ -824,6 +835,8 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
   float* Rl = wbase + (H16 ? embed_dt16_floats(T3) : ODT);  // [16 neighbours][kRS]
   float* dRl = Rl + 16 * kRS;            // [16 neighbours][kRS]
   float4* list = reinterpret_cast<float4*>(dRl + 16 * kRS);  // 16-byte aligned
+  // tabulated nets: h3 and p' of one tile group, [neighbour][slot] with row stride kTS
+  float* Tb = reinterpret_cast<float*>(list + a.list_cap);
 
   const SegInfo& S = a.seg[a.blk_seg[blockIdx.x]];
   const int c = (blockIdx.x - S.blk0) * 4 + wv;
@@ -1086,7 +1099,7 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
       constexpr int P0 = ((16 * T1 * 2 + 16 * T2 + 16 * T3) + 63) / 64 * 64;
       constexpr int O_W1A = P0, O_W2A = O_W1A + T2 * 4 * T1 * 64;
       constexpr int O_W2B = O_W2A + T3 * 4 * T2 * 64, O_W1B = O_W2B + T2 * 4 * T3 * 64;
-      const int vl4 = lane * 4, vq16 = q * 16, vl16 = lane * 16;
+      const int vl4 = lane * 4, vl16 = lane * 16;
       const int hnetb = PREC ? __builtin_amdgcn_readfirstlane((ti * a.ntypes + j) * a.hnet_stride) : 0;
       for (int ch = 0; ch < m; ch += 16) {
         const int nb = netb + opaque_zero();
@@ -1119,10 +1132,16 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
         if constexpr (TAB) {
           // tabulated net: h3 = p(x0) and dh3/ds~ = p'(x0) from one read of the interval's
           // coefficients; dE/ds~ = sum_slot dh3[slot] p'[slot] (no gradient through idx)
+          // The gathers run with lane 4m + p on neighbour m (coalesced, as in the forward);
+          // h3 and p' pass through LDS ([neighbour][slot], Tb) into the (q, n) layout the MFMA
+          // operands need.
           const int net = ti * a.ntypes + j;
           float x0;
           const int row = tab_interval(g.st, a.tab_srmin[net], a.tab_delta[net], a.tab_n, x0);
-          const int voff = row * (6 * 64 * T3) + vq16;
+          const int gm = lane >> 2, gp = lane & 3;
+          const int rowm = __shfl(row, gm);
+          const float x0m = __shfl(x0, gm);
+          const int voff = rowm * (6 * 64 * T3) + 16 * gp;
           const int tb = __builtin_amdgcn_readfirstlane(net * a.tab_stride * 4) + opaque_zero();
           const rsrc_t trs = __builtin_amdgcn_make_buffer_rsrc((void*)a.tab, (short)0, a.tab_bytes, 0x00020000);
           wave_sync();  // Rl visible to the dh3 products
@@ -1136,16 +1155,30 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
               floatx4 cf[6];
 #pragma unroll
               for (int cc = 0; cc < 6; ++cc) cf[cc] = tab_load(trs, voff + 64 * (g0 + k), tb + cc * 64 * T3);
+              floatx4 hv, dv;
 #pragma unroll
               for (int i = 0; i < 4; ++i) {
                 float v = cf[0][i], d = 5.f * cf[0][i];
 #pragma unroll
                 for (int cc = 1; cc < 5; ++cc) {
-                  v = __builtin_fmaf(v, x0, cf[cc][i]);
-                  d = __builtin_fmaf(d, x0, (float)(5 - cc) * cf[cc][i]);
+                  v = __builtin_fmaf(v, x0m, cf[cc][i]);
+                  d = __builtin_fmaf(d, x0m, (float)(5 - cc) * cf[cc][i]);
                 }
-                h3[4 * k + i] = __builtin_fmaf(v, x0, cf[5][i]);
-                dp[4 * k + i] = d;
+                hv[i] = __builtin_fmaf(v, x0m, cf[5][i]);
+                dv[i] = d;
+              }
+              *reinterpret_cast<floatx4*>(Tb + gm * kTS + 16 * k + 4 * gp) = hv;
+              *reinterpret_cast<floatx4*>(Tb + (16 + gm) * kTS + 16 * k + 4 * gp) = dv;
+            }
+            wave_sync();
+#pragma unroll
+            for (int k = 0; k < SZ; ++k) {
+              const floatx4 hv = *reinterpret_cast<const floatx4*>(Tb + n * kTS + 16 * k + 4 * q);
+              const floatx4 dv = *reinterpret_cast<const floatx4*>(Tb + (16 + n) * kTS + 16 * k + 4 * q);
+#pragma unroll
+              for (int i = 0; i < 4; ++i) {
+                h3[4 * k + i] = hv[i];
+                dp[4 * k + i] = dv[i];
               }
             }
             // dR^T[x][nbr] += sum_slot dT[x][slot] h3[slot][nbr]
@@ -1456,7 +1489,8 @@ static int32_t launch_embed(System* s, bool backward, int precision) {
     wave_floats = std::max(wave_floats, (size_t)(10 * W + 12 * 16));
   } else {
     wave_floats = (11 * HS + 3) / 4 * 4 +
-                  std::max((size_t)(2 * 16 * kRS + 4 * a.list_cap), (size_t)(10 * W + std::max(20 * A, 22 * 16)));
+                  std::max((size_t)(2 * 16 * kRS + 4 * a.list_cap + (TAB ? 2 * 16 * kTS : 0)),
+                           (size_t)(10 * W + std::max(20 * A, 22 * 16)));
     // 16-bit MLP forms: the FP16 dE/dT copies replace dTp in the chunk phase
     if (PREC != 0 && !TAB)
       wave_floats = std::max(wave_floats, (size_t)embed_dt16_floats(T3) + 2 * 16 * kRS + 4 * a.list_cap);
