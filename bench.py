@@ -706,7 +706,11 @@ def probe_issue(blocks=512, iters=20000, reps=5):
                                      ctypes.POINTER(ctypes.c_float)]
     names = ['f32_mfma', 'valu', 'f32_mfma+valu_same_wave', 'f32_mfma|valu_separate_waves_halves',
              'bf16_mfma', 'bf16_mfma+valu_same_wave', 'bf16_mfma|valu_separate_waves_halves',
-             'f32_mfma|valu_separate_waves_parity', 'bf16_mfma|valu_separate_waves_parity']
+             'f32_mfma|valu_separate_waves_parity', 'bf16_mfma|valu_separate_waves_parity',
+             # the same with the MFMA accumulators in AccVGPRs (asm constraint) instead of ArchVGPRs
+             'f32agpr_mfma', 'f32agpr_mfma+valu_same_wave', 'f32agpr_mfma|valu_separate_waves_parity',
+             'bf16agpr_mfma', 'bf16agpr_mfma+valu_same_wave', 'bf16agpr_mfma|valu_separate_waves_parity',
+             'f32agpr_mfma|valu_separate_waves_halves', 'bf16agpr_mfma|valu_separate_waves_halves']
     out = {}
     for which, name in enumerate(names):
         ms = ctypes.c_float()
@@ -716,7 +720,7 @@ def probe_issue(blocks=512, iters=20000, reps=5):
         print('probe %s: %.4f ms' % (name, ms.value), file=sys.stderr, flush=True)
     m, v = out['f32_mfma'], out['valu']
     mb = out['bf16_mfma']
-    for kind, mk in (('f32', m), ('bf16', mb)):
+    for kind, mk in (('f32', m), ('bf16', mb), ('f32agpr', out['f32agpr_mfma']), ('bf16agpr', out['bf16agpr_mfma'])):
         same = out['%s_mfma+valu_same_wave' % kind]
         sep = min(out['%s_mfma|valu_separate_waves_halves' % kind], out['%s_mfma|valu_separate_waves_parity' % kind])
         out['%s_same_wave_over_sum' % kind] = round(same / (mk + v), 3)

@@ -330,6 +330,7 @@ __device__ __forceinline__ uint16_t h16_bits(float v) {
 
 // Per-wave LDS of the 16-bit forms' FP16 dE/dT operand copies (floats): dTa [tile pair][lane
 // group][x 0..10][8] for the dR products, dTb [slot][x 0..15] for the dh3 products.
+constexpr int embed_dt_floats(int T3) { return (10 * (16 * T3 + 1) + 3) / 4 * 4; }
 constexpr int embed_dt16_floats(int T3) { return ((11 * ((T3 + 1) / 2) * 32 + 16 * T3 * 16) / 2 + 3) / 4 * 4; }
 
 // B operand of a 16x16x32 step from 8 consecutive fp32 registers r0..r0+7 (zero past n)
@@ -828,13 +829,16 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
   // made from, lives there only during the prologue)
   uint16_t* dTa = reinterpret_cast<uint16_t*>(wbase);
   uint16_t* dTb = dTa + DTA;
-  // dE/dT rows 0..9 and a zero row 10: MFMA operand rows beyond the ten environment rows
-  // read the zero row instead of being selected away per product
-  constexpr int ODT = (11 * HS + 3) / 4 * 4;
-  float* dTp = wbase;                    // 10 * HS (rows x >= 10 of the MFMA operands are zero)
+  // dE/dT rows 0..9.  MFMA operand rows beyond the ten environment rows read row 9 instead:
+  // in the dR products those rows give output rows x >= 10, which nothing reads; in the dh3
+  // products they meet the zero columns x >= 10 of the R tile (finite x 0 = 0).
+  constexpr int ODT = embed_dt_floats(T3);
+  float* dTp = wbase;                    // 10 * HS
   float* Rl = wbase + (H16 ? embed_dt16_floats(T3) : ODT);  // [16 neighbours][kRS]
-  float* dRl = Rl + 16 * kRS;            // [16 neighbours][kRS]
-  float4* list = reinterpret_cast<float4*>(dRl + 16 * kRS);  // 16-byte aligned
+  // dE/dR of the chunk reuses the R tile: the wave's last read of R (the dh3 products) is
+  // issued before its first write of dR, and one wave's LDS operations complete in order
+  float* dRl = Rl;                       // [16 neighbours][kRS]
+  float4* list = reinterpret_cast<float4*>(Rl + 16 * kRS);  // 16-byte aligned
   // tabulated nets: h3 and p' of one tile group, [neighbour][slot] with row stride kTS
   float* Tb = reinterpret_cast<float*>(list + a.list_cap);
 
@@ -864,7 +868,7 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
     {
       const float* Tg = a.T + (size_t)gc * 10 * W;
       const float* dGg = a.dG + (size_t)grow * A * W;
-      float* Tl = dTp + ODT;       // 10*W   (aliases Rl, dRl, list)
+      float* Tl = dTp + ODT;       // 10*W   (aliases Rl and list)
       float* Xl = Tl + 10 * W;     // 10*A
       float* Ul = Xl + 10 * A;     // 10*A
       // compile-time trip count (10 W <= 160 T3): all loads of T issue before the stores
@@ -880,14 +884,17 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
         const int e = lane + 64 * k;
         if (e < 10 * W) Tl[e] = tv[k];
       }
-      for (int e = lane; e < 11 * HS; e += 64) dTp[e] = 0.f;
+      // with A <= 16 the X and U tables use dTp's space, which is cleared after them
+      if (A > 16)
+        for (int e = lane; e < 10 * HS; e += 64) dTp[e] = 0.f;
       wave_sync();
       const float r2 = 1.4142135623730951f;
       if (A <= 16) {
         // On MFMA: U = Tx dG^T (M = x, K = w, N = a) and V = X dG (M = x, K = a, N = w); the U
         // terms are added to V in its output registers, lane (q, n) holding V[4q + i][16t + n].
-        float* Xm = Tl + 10 * W;   // [12][16]: X[x][a], rows 10, 11 and columns a >= A zero
+        float* Xm = dTp;           // [12][16]: X[x][a], rows 10, 11 and columns a >= A zero
         float* Um = Xm + 12 * 16;  // [10][16]: U[x][a]
+        static_assert(12 * 16 + 10 * 16 <= 10 * HS, "X and U fit dTp's space");
         for (int e = lane; e < 12 * 16; e += 64) {
           const int x = e >> 4, aa = e & 15;
           float v = 0.f;
@@ -938,12 +945,16 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) xa[kk] = n < 12 ? Xm[n * 16 + 4 * kk + q] : 0.f;
         const float* gg = a.wc_g ? a.wc_g + (size_t)gc * 3 : nullptr;
+        // results held until X and U have been read, then written over the cleared dTp
+        float res[T3][4];
+        int rslot[T3];
 #pragma unroll
         for (int t = 0; t < T3; ++t) {
           const int w = 16 * t + n;
           floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
           for (int kk = 0; kk < 4; ++kk) acc = mfma4(xa[kk], dgv[t][kk], acc);
+          rslot[t] = -1;
           if (w >= W) continue;
           int slot = -1;
           for (int qq = 0; qq < 4; ++qq) {
@@ -971,9 +982,18 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
             }
             // the centroid output out_norm sum_w f_w T3[c][w] depends on T3 directly
             if (gg && x >= 1 && x <= 3) v += gg[x - 1] * y;
-            if (slot >= 0) dTp[x * HS + slot] = v * a.inv_nnbrs;
+            res[t][i] = v * a.inv_nnbrs;
           }
+          rslot[t] = slot;
         }
+        wave_sync();
+        for (int e = lane; e < 10 * HS; e += 64) dTp[e] = 0.f;
+        wave_sync();
+#pragma unroll
+        for (int t = 0; t < T3; ++t)
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            if (rslot[t] >= 0 && 4 * q + i < 10) dTp[(4 * q + i) * HS + rslot[t]] = res[t][i];
       } else {
         for (int e = lane; e < 10 * A; e += 64) {
           const int x = e / A, aa = e % A;
@@ -1087,8 +1107,8 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
     // ---- chunks
     // per-lane dE/dT operand rows: dR products use row n, dh3 products rows 4 kk + q
     // (only kk = 2 can pass row 9); rows past 9 read the zero row
-    const float* dTr = dTp + (n < 10 ? n : 10) * HS + 4 * q;
-    const float* dTc2 = dTp + (8 + q < 10 ? 8 + q : 10) * HS + n;
+    const float* dTr = dTp + (n < 10 ? n : 9) * HS + 4 * q;
+    const float* dTc2 = dTp + (8 + q < 10 ? 8 + q : 9) * HS + n;
     float fc[3] = {0.f, 0.f, 0.f};
     for (int j = 0; j < a.ntypes; ++j) {
       float* prow = a.pgrad ? a.pgrad + ((size_t)gc * a.ksum + a.koff[j]) * 3 : nullptr;
@@ -1114,13 +1134,16 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
         Geo g;
         float R[10];
         env_forward(e, valid, a.inv_rcut, a.mean[ti], a.inv_std[ti], g, R);
+        // the R tile feeds the FP32 dh3 products; the 16-bit ones take R from registers
+        if constexpr (!H16) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int x = 4 * q + i;
-          float v = 0.f;
+          for (int i = 0; i < 4; ++i) {
+            const int x = 4 * q + i;
+            float v = 0.f;
 #pragma unroll
-          for (int y = 0; y < 10; ++y) v = x == y ? R[y] : v;
-          Rl[n * kRS + x] = v;
+            for (int y = 0; y < 10; ++y) v = x == y ? R[y] : v;
+            Rl[n * kRS + x] = v;
+          }
         }
         float dst;
         // 16-bit MLP forms: the dh3 products' B operand R[nbr n][x = 8q + j] from this lane's R
@@ -1207,10 +1230,13 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
           float h1[4 * T1];
 #pragma unroll
           for (int r = 0; r < 4 * T1; ++r) h1[r] = r < V1 ? tanh_s(g.st * k0(r) + b0(r)) : 0.f;
-          // h2 = t2 + h1[r/Z2] is formed once (t2 is kept for tanh'); h3 = t3 + h2[r/Z3] is
-          // re-formed where used
+          // h2 = t2 + h1[r/Z2] is formed once; h3 = t3 + h2[r/Z3] is re-formed where used.  The
+          // FP32 form re-forms t2 = h2 - h1[r/Z2] for tanh' (within an ulp of h2), which frees the
+          // registers of a fourth wave per SIMD; the 16-bit forms are VALU bound and keep t2.
+          constexpr bool KEEP_T2 = PREC != 0;
+          float t2k[KEEP_T2 ? 4 * T2 : 1];
           float h2[4 * T2];
-          auto H2 = [&](int r, const float*) -> float { return h2[r]; };
+          auto H2 = [&](int r) -> float { return h2[r]; };
           floatx4 acc2[T2];
 #pragma unroll
           for (int t = 0; t < T2; ++t) acc2[t] = floatx4{b1(4 * t), b1(4 * t + 1), b1(4 * t + 2), b1(4 * t + 3)};
@@ -1219,13 +1245,13 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
           else
             mfma_chain_h<PREC, T2, S21, 0, S21>(hrs, hb + H_W1A, vl16, [&](int s2) {
               return pack8<PREC, 4 * T1>([&](int r) { return h1[r]; }, s2); }, acc2);
-          float t2[4 * T2];
 #pragma unroll
           for (int r = 0; r < 4 * T2; ++r) {
-            t2[r] = r < V2 ? tanh_s(acc2[r / 4][r % 4]) : 0.f;
-            h2[r] = t2[r] + ((r / Z2 < 4 * T1) ? h1[r / Z2] : 0.f);
+            const float t = r < V2 ? tanh_s(acc2[r / 4][r % 4]) : 0.f;
+            if constexpr (KEEP_T2) t2k[r] = t;
+            h2[r] = t + ((r / Z2 < 4 * T1) ? h1[r / Z2] : 0.f);
           }
-          wave_sync();  // Rl visible to the dh3 products
+          if constexpr (!H16) wave_sync();  // Rl visible to the dh3 products
           floatx4 dr = floatx4{0.f, 0.f, 0.f, 0.f};
           floatx4 dh2[T2];
 #pragma unroll
@@ -1237,10 +1263,10 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
             for (int t = 0; t < SZ; ++t) acc3[t] = floatx4{b2(4 * (g0 + t)), b2(4 * (g0 + t) + 1), b2(4 * (g0 + t) + 2), b2(4 * (g0 + t) + 3)};
             if constexpr (PREC == 0)
               mfma_chain<SZ, 4 * T2, 0, V2>(wrs, nb + 4 * (O_W2A + g0 * 4 * T2 * 64), vl4,
-                                            [&](int s_) { return H2(s_, t2); }, acc3);
+                                            [&](int s_) { return H2(s_); }, acc3);
             else
               mfma_chain_h<PREC, SZ, S22, 0, S22>(hrs, hb + H_W2A + g0 * S22 * 1024, vl16, [&](int s2) {
-                return pack8<PREC, 4 * T2>([&](int r) { return H2(r, t2); }, s2); }, acc3);
+                return pack8<PREC, 4 * T2>([&](int r) { return H2(r); }, s2); }, acc3);
             float t3[4 * SZ];
 #pragma unroll
             for (int k = 0; k < SZ; ++k)
@@ -1255,7 +1281,7 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
 #pragma unroll
               for (int j = 0; j < 8; ++j) {
                 const int k = j / 4, r = 4 * (g0 + k) + (j & 3);
-                const float h3v = k < SZ ? t3[4 * (k < SZ ? k : 0) + (j & 3)] + ((r / Z3 < 4 * T2) ? H2(r / Z3, t2) : 0.f)
+                const float h3v = k < SZ ? t3[4 * (k < SZ ? k : 0) + (j & 3)] + ((r / Z3 < 4 * T2) ? H2(r / Z3) : 0.f)
                                          : 0.f;
                 bh[j] = (HE16)h3v;
               }
@@ -1275,7 +1301,7 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                   const int r = 4 * (g0 + k) + i;
-                  const float h3v = t3[4 * k + i] + ((r / Z3 < 4 * T2) ? H2(r / Z3, t2) : 0.f);
+                  const float h3v = t3[4 * k + i] + ((r / Z3 < 4 * T2) ? H2(r / Z3) : 0.f);
                   dr = mfma4(dTr[16 * (g0 + k) + i], h3v, dr);
                 }
               // dh3[slot][nbr] = sum_x dT[x][slot] R[nbr][x]
@@ -1324,7 +1350,12 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
             }
           }
 #pragma unroll
-          for (int r = 0; r < 4 * T2; ++r) dh2[r / 4][r % 4] *= (1.f - t2[r] * t2[r]);
+          for (int r = 0; r < 4 * T2; ++r) {
+            float t;
+            if constexpr (KEEP_T2) t = t2k[r];
+            else t = h2[r] - ((r / Z2 < 4 * T1) ? h1[r / Z2] : 0.f);
+            dh2[r / 4][r % 4] *= (1.f - t * t);
+          }
           if constexpr (PREC == 0)
             mfma_chain<T1, 4 * T2, 0, V2>(wrs, nb + 4 * O_W1B, vl4, [&](int s_) { return dh2[s_ / 4][s_ % 4]; }, dh1);
           else
@@ -1488,12 +1519,14 @@ static int32_t launch_embed(System* s, bool backward, int precision) {
     wave_floats = 4 * a.list_cap + 16 * kRS + 16 * (16 * kGroup + 1);
     wave_floats = std::max(wave_floats, (size_t)(10 * W + 12 * 16));
   } else {
-    wave_floats = (11 * HS + 3) / 4 * 4 +
-                  std::max((size_t)(2 * 16 * kRS + 4 * a.list_cap + (TAB ? 2 * 16 * kTS : 0)),
-                           (size_t)(10 * W + std::max(20 * A, 22 * 16)));
+    // dE/dT, then the chunk tiles (R / dR, pair list, tabulated h3 and p') or, in the prologue,
+    // T (and with A > 16 the X and U tables; with A <= 16 they sit in dE/dT's space)
+    wave_floats = (size_t)embed_dt_floats(T3) +
+                  std::max((size_t)(16 * kRS + 4 * a.list_cap + (TAB ? 2 * 16 * kTS : 0)),
+                           (size_t)(10 * W + (A <= 16 ? 0 : 20 * A)));
     // 16-bit MLP forms: the FP16 dE/dT copies replace dTp in the chunk phase
     if (PREC != 0 && !TAB)
-      wave_floats = std::max(wave_floats, (size_t)embed_dt16_floats(T3) + 2 * 16 * kRS + 4 * a.list_cap);
+      wave_floats = std::max(wave_floats, (size_t)embed_dt16_floats(T3) + 16 * kRS + 4 * a.list_cap);
   }
   wave_floats = (wave_floats + 3) / 4 * 4;
   a.lds_wave_floats = (int)wave_floats;

@@ -23,12 +23,22 @@ constexpr int kValuRepeat = 4;   // FMAs per chain per iteration
 
 enum ProbeKind { MFMA_F32 = 0, MFMA_BF16 = 1 };
 
+// AGPR forms (KIND + 2): the same instructions with the accumulators pinned to AccVGPRs by the
+// asm constraint (the builtins let the compiler choose ArchVGPRs, which it does here).  The
+// chains read their own accumulator back only as the next instruction's SrcC of the same
+// opcode, which the hardware forwards; the VALU reads them after the loop, past the wait below.
+enum ProbeKindAcc { MFMA_F32_AGPR = 2, MFMA_BF16_AGPR = 3 };
+
 template <int KIND>
 __device__ __forceinline__ void mfma_step(floatx4 (&acc)[kMfmaChains], float a, float b, bf16x8 ah, bf16x8 bh) {
 #pragma unroll
   for (int k = 0; k < kMfmaChains; ++k) {
     if constexpr (KIND == MFMA_F32) acc[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[k], 0, 0, 0);
-    else acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[k], 0, 0, 0);
+    else if constexpr (KIND == MFMA_BF16) acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[k], 0, 0, 0);
+    else if constexpr (KIND == MFMA_F32_AGPR)
+      asm volatile("v_mfma_f32_16x16x4_f32 %0, %1, %2, %0" : "+a"(acc[k]) : "v"(a), "v"(b));
+    else
+      asm volatile("v_mfma_f32_16x16x32_bf16 %0, %1, %2, %0" : "+a"(acc[k]) : "v"(ah), "v"(bh));
   }
 }
 
@@ -70,6 +80,11 @@ __global__ void __launch_bounds__(512) issue_probe_kernel(float* out, int iters,
     if (run_m) mfma_step<KIND>(acc, a, b, ah, bh);
     if (run_v) valu_step(v, x, y);
   }
+  // the last matrix results are read by VALU moves: wait out the longest MFMA-write ->
+  // VALU-read distance (the compiler does not see into the asm forms)
+  if constexpr (KIND >= MFMA_F32_AGPR)
+    asm volatile("s_nop 7\n\ts_nop 7\n\ts_nop 7\n\ts_nop 7\n\ts_nop 7"
+                 : "+a"(acc[0]), "+a"(acc[1]), "+a"(acc[2]), "+a"(acc[3]));
   float s = 0.f;
 #pragma unroll
   for (int k = 0; k < kMfmaChains; ++k) s += acc[k][0] + acc[k][1] + acc[k][2] + acc[k][3];
@@ -96,6 +111,14 @@ hipError_t launch_case(int which, float* out, int blocks, int iters) {
     case 6: return launch<MFMA_BF16, false, false, 1>(out, blocks, iters);
     case 7: return launch<MFMA_F32, false, false, 2>(out, blocks, iters);
     case 8: return launch<MFMA_BF16, false, false, 2>(out, blocks, iters);
+    case 9: return launch<MFMA_F32_AGPR, true, false, 0>(out, blocks, iters);
+    case 10: return launch<MFMA_F32_AGPR, true, true, 0>(out, blocks, iters);
+    case 11: return launch<MFMA_F32_AGPR, false, false, 2>(out, blocks, iters);
+    case 12: return launch<MFMA_BF16_AGPR, true, false, 0>(out, blocks, iters);
+    case 13: return launch<MFMA_BF16_AGPR, true, true, 0>(out, blocks, iters);
+    case 14: return launch<MFMA_BF16_AGPR, false, false, 2>(out, blocks, iters);
+    case 15: return launch<MFMA_F32_AGPR, false, false, 1>(out, blocks, iters);
+    case 16: return launch<MFMA_BF16_AGPR, false, false, 1>(out, blocks, iters);
     default: return hipErrorInvalidValue;
   }
 }
@@ -104,11 +127,12 @@ hipError_t launch_case(int which, float* out, int blocks, int iters) {
 
 // which: 0 FP32 MFMA alone, 1 VALU alone, 2 FP32 MFMA + VALU in one wave, 3 FP32 MFMA and VALU
 // in separate waves of one block (halves), 4 BF16 MFMA alone, 5 BF16 MFMA + VALU in one wave,
-// 6 BF16 MFMA and VALU in separate waves (halves), 7 and 8: cases 3 and 6 split by wave parity.
+// 6 BF16 MFMA and VALU in separate waves (halves), 7 and 8: cases 3 and 6 split by wave parity;
+// 9-14: cases 0, 2, 7, 4, 5, 8 with the accumulators in AccVGPRs; 15, 16: cases 3, 6 likewise.
 // blocks: grid size (2 x 256 CUs puts two blocks on every CU).  Writes the average kernel time
 // over `reps` launches (after one warm-up) to *ms.  Returns 0, or -1 with a message on stderr.
 extern "C" int dpmd_probe_issue(int which, int blocks, int iters, int reps, float* ms) {
-  if (which < 0 || which > 8 || blocks <= 0 || blocks > (1 << 16) || iters <= 0 || reps <= 0 || !ms) {
+  if (which < 0 || which > 16 || blocks <= 0 || blocks > (1 << 16) || iters <= 0 || reps <= 0 || !ms) {
     std::fprintf(stderr, "dpmd_probe_issue: bad arguments\n");
     return -1;
   }
