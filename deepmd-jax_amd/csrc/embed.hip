@@ -871,19 +871,21 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
       float* Tl = dTp + ODT;       // 10*W   (aliases Rl and list)
       float* Xl = Tl + 10 * W;     // 10*A
       float* Ul = Xl + 10 * A;     // 10*A
-      // compile-time trip count (10 W <= 160 T3): all loads of T issue before the stores
+      // compile-time trip count (10 W <= 160 T3): all loads of T issue before the stores.  Buffer
+      // loads over T and the bias vector: entries past their ends read as zero, so no lane
+      // branches around its load (a branch per load had each one wait for the previous)
       constexpr int TN = (160 * T3 + 63) / 64;
+      const rsrc_t Trs = __builtin_amdgcn_make_buffer_rsrc((void*)Tg, (short)0, 10 * W * 4, 0x00020000);
+      const rsrc_t Brs = __builtin_amdgcn_make_buffer_rsrc((void*)a.tbias, (short)0, W * 4, 0x00020000);
       float tv[TN];
 #pragma unroll
       for (int k = 0; k < TN; ++k) {
         const int e = lane + 64 * k;
-        tv[k] = e < 10 * W ? Tg[e] + (e < W ? a.tbias[e] : 0.f) : 0.f;
+        tv[k] = bld(Trs, 4 * e, 0) + bld(Brs, 4 * e, 0);
       }
+      // the region holds TN * 64 floats: the zeros past 10 W are stored as well (no branch)
 #pragma unroll
-      for (int k = 0; k < TN; ++k) {
-        const int e = lane + 64 * k;
-        if (e < 10 * W) Tl[e] = tv[k];
-      }
+      for (int k = 0; k < TN; ++k) Tl[lane + 64 * k] = tv[k];
       // with A <= 16 the X and U tables use dTp's space, which is cleared after them
       if (A > 16)
         for (int e = lane; e < 10 * HS; e += 64) dTp[e] = 0.f;
@@ -912,19 +914,22 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
           Xm[e] = v;
         }
         // W <= 16 T3 (the layer-3 tiles hold every output feature): compile-time trip counts,
-        // every dG load issued before the MFMAs that consume it
+        // every dG load issued before the MFMAs that consume it; masked entries are buffer loads
+        // past the end of the centre's dG rows, which read as zero
+        const rsrc_t Grs = __builtin_amdgcn_make_buffer_rsrc((void*)dGg, (short)0, A * W * 4, 0x00020000);
         {
           float dgu[4 * T3];  // dG[a = n][w = 4 kk + q]
 #pragma unroll
           for (int kk = 0; kk < 4 * T3; ++kk) {
             const int w = 4 * kk + q;
-            dgu[kk] = (n < A && w < W) ? dGg[n * W + w] : 0.f;
+            dgu[kk] = bld(Grs, (n < A && w < W) ? 4 * (n * W + w) : 4 * A * W, 0);
           }
           floatx4 u0 = floatx4{0.f, 0.f, 0.f, 0.f}, u1 = u0;
+          // rows n >= 10 read row 9 (their outputs are not kept); columns w >= W read past
+          // the row, into the next one or the zeros after T, against a zero dG entry
 #pragma unroll
           for (int kk = 0; kk < 4 * T3; ++kk) {
-            const int w = 4 * kk + q;
-            const float av = (n < 10 && w < W) ? Tl[n * W + w] : 0.f;
+            const float av = Tl[(n < 10 ? n : 9) * W + 4 * kk + q];
             if (kk & 1) u1 = mfma4(av, dgu[kk], u1);
             else u0 = mfma4(av, dgu[kk], u0);
           }
@@ -938,7 +943,7 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
 #pragma unroll
           for (int kk = 0; kk < 4; ++kk) {
             const int aa = 4 * kk + q, w = 16 * t + n;
-            dgv[t][kk] = (aa < A && w < W) ? dGg[aa * W + w] : 0.f;
+            dgv[t][kk] = bld(Grs, (aa < A && w < W) ? 4 * (aa * W + w) : 4 * A * W, 0);
           }
         wave_sync();
         float xa[4];
@@ -1520,10 +1525,16 @@ static int32_t launch_embed(System* s, bool backward, int precision) {
     wave_floats = std::max(wave_floats, (size_t)(10 * W + 12 * 16));
   } else {
     // dE/dT, then the chunk tiles (R / dR, pair list, tabulated h3 and p') or, in the prologue,
-    // T (and with A > 16 the X and U tables; with A <= 16 they sit in dE/dT's space)
+    // T in whole 64-float passes (and with A > 16 the X and U tables; with A <= 16 they sit in
+    // dE/dT's space)
+    constexpr int TPASS = 64 * ((160 * T3 + 63) / 64);
+    if (10 * W > TPASS) {
+      set_error("embedding output width %d exceeds the kernel's tiles", W);
+      return DPMD_E_UNSUPPORTED;
+    }
     wave_floats = (size_t)embed_dt_floats(T3) +
                   std::max((size_t)(16 * kRS + 4 * a.list_cap + (TAB ? 2 * 16 * kTS : 0)),
-                           (size_t)(10 * W + (A <= 16 ? 0 : 20 * A)));
+                           (size_t)(A <= 16 ? TPASS : std::max(TPASS, 10 * W + 20 * A)));
     // 16-bit MLP forms: the FP16 dE/dT copies replace dTp in the chunk phase
     if (PREC != 0 && !TAB)
       wave_floats = std::max(wave_floats, (size_t)embed_dt16_floats(T3) + 16 * kRS + 4 * a.list_cap);
