@@ -881,7 +881,28 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
 #pragma unroll
       for (int k = 0; k < TN; ++k) {
         const int e = lane + 64 * k;
-        tv[k] = bld(Trs, 4 * e, 0) + bld(Brs, 4 * e, 0);
+        // the bias has W <= 16 T3 entries: only the first passes can hold one
+        tv[k] = bld(Trs, 4 * e, 0) + (64 * k < 16 * T3 ? bld(Brs, 4 * e, 0) : 0.f);
+      }
+      // the dG operands of U and V (A <= 16), loaded with T so that the two reads overlap.  W <=
+      // 16 T3 (the layer-3 tiles hold every output feature): compile-time trip counts; masked
+      // entries are buffer loads past the end of the centre's dG rows, which read as zero
+      const rsrc_t Grs = __builtin_amdgcn_make_buffer_rsrc((void*)dGg, (short)0, A * W * 4, 0x00020000);
+      float dgu[4 * T3];  // dG[a = n][w = 4 kk + q]
+      float dgv[T3][4];   // dG[a = 4 kk + q][w = 16 t + n]
+      if (A <= 16) {
+#pragma unroll
+        for (int kk = 0; kk < 4 * T3; ++kk) {
+          const int w = 4 * kk + q;
+          dgu[kk] = bld(Grs, (n < A && w < W) ? 4 * (n * W + w) : 4 * A * W, 0);
+        }
+#pragma unroll
+        for (int t = 0; t < T3; ++t)
+#pragma unroll
+          for (int kk = 0; kk < 4; ++kk) {
+            const int aa = 4 * kk + q, w = 16 * t + n;
+            dgv[t][kk] = bld(Grs, (aa < A && w < W) ? 4 * (aa * W + w) : 4 * A * W, 0);
+          }
       }
       // the region holds TN * 64 floats: the zeros past 10 W are stored as well (no branch)
 #pragma unroll
@@ -913,17 +934,7 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
           }
           Xm[e] = v;
         }
-        // W <= 16 T3 (the layer-3 tiles hold every output feature): compile-time trip counts,
-        // every dG load issued before the MFMAs that consume it; masked entries are buffer loads
-        // past the end of the centre's dG rows, which read as zero
-        const rsrc_t Grs = __builtin_amdgcn_make_buffer_rsrc((void*)dGg, (short)0, A * W * 4, 0x00020000);
         {
-          float dgu[4 * T3];  // dG[a = n][w = 4 kk + q]
-#pragma unroll
-          for (int kk = 0; kk < 4 * T3; ++kk) {
-            const int w = 4 * kk + q;
-            dgu[kk] = bld(Grs, (n < A && w < W) ? 4 * (n * W + w) : 4 * A * W, 0);
-          }
           floatx4 u0 = floatx4{0.f, 0.f, 0.f, 0.f}, u1 = u0;
           // rows n >= 10 read row 9 (their outputs are not kept); columns w >= W read past
           // the row, into the next one or the zeros after T, against a zero dG entry
@@ -937,14 +948,6 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
           for (int i = 0; i < 4; ++i)
             if (4 * q + i < 10) Um[(4 * q + i) * 16 + n] = u0[i] + u1[i];
         }
-        float dgv[T3][4];  // dG[a = 4 kk + q][w = 16 t + n]
-#pragma unroll
-        for (int t = 0; t < T3; ++t)
-#pragma unroll
-          for (int kk = 0; kk < 4; ++kk) {
-            const int aa = 4 * kk + q, w = 16 * t + n;
-            dgv[t][kk] = bld(Grs, (aa < A && w < W) ? 4 * (aa * W + w) : 4 * A * W, 0);
-          }
         wave_sync();
         float xa[4];
 #pragma unroll
@@ -992,13 +995,34 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
           rslot[t] = slot;
         }
         wave_sync();
-        for (int e = lane; e < 10 * HS; e += 64) dTp[e] = 0.f;
-        wave_sync();
+        if constexpr (H16) {
+          // the FP16 operand copies of the 16-bit forms, written from the registers (layouts
+          // below); entries no result reaches (padding slots, row x = 10) stay zero
+          for (int e = lane; e < embed_dt16_floats(T3); e += 64) wbase[e] = 0.f;
+          wave_sync();
 #pragma unroll
-        for (int t = 0; t < T3; ++t)
+          for (int t = 0; t < T3; ++t) {
+            const int sl = rslot[t];
+            const int tt = sl >> 4, qq = (sl >> 2) & 3, jj = 4 * (tt & 1) + (sl & 3);
 #pragma unroll
-          for (int i = 0; i < 4; ++i)
-            if (rslot[t] >= 0 && 4 * q + i < 10) dTp[(4 * q + i) * HS + rslot[t]] = res[t][i];
+            for (int i = 0; i < 4; ++i) {
+              const int x = 4 * q + i;
+              if (sl >= 0 && x < 10) {
+                const uint16_t h = h16_bits<2>(res[t][i]);
+                dTa[((((tt >> 1) * 4 + qq) * 11 + x) << 3) + jj] = h;
+                dTb[sl * 16 + x] = h;
+              }
+            }
+          }
+        } else {
+          for (int e = lane; e < 10 * HS; e += 64) dTp[e] = 0.f;
+          wave_sync();
+#pragma unroll
+          for (int t = 0; t < T3; ++t)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+              if (rslot[t] >= 0 && 4 * q + i < 10) dTp[(4 * q + i) * HS + rslot[t]] = res[t][i];
+        }
       } else {
         for (int e = lane; e < 10 * A; e += 64) {
           const int x = e / A, aa = e % A;
@@ -1082,10 +1106,11 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
         }
       }
       wave_sync();
-      if constexpr (H16) {
+      if (H16 && A > 16) {
         // dTa[pair g][lane group][x][j] = dT[x][16 (2g + j/4) + 4 lane group + j%4] (x = 10
         // zero): the 16 lanes of a group read 16 consecutive byte-16 pieces.  dTb[slot][x] =
-        // dT[x][slot] (x >= 10 zero).  Both overwrite dTp: read into registers first.
+        // dT[x][slot] (x >= 10 zero).  Both overwrite dTp: read into registers first.  (With
+        // A <= 16 the copies were written from the registers above.)
         static_assert(DTA % 64 == 0 && DTB % 64 == 0, "conversion loops assume whole passes");
         float va[DTA / 64], vb[DTB / 64];
 #pragma unroll
