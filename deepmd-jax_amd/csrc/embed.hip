@@ -461,37 +461,41 @@ __device__ __forceinline__ int compact_row(const EmbedArgs& a, const SegInfo& S,
   const int gc = S.base + c;
   const int cnt = a.nbr_cnt[j][gc];
   const int* row = a.nbr[j] + (size_t)gc * a.cap[j];
+  // row entries through a buffer of cnt entries: lanes past the end read index 0 (a valid atom
+  // of type j, masked below), so no lane branches around its loads, and the next piece's
+  // indices are read while this piece's positions are gathered
+  const rsrc_t rrs = __builtin_amdgcn_make_buffer_rsrc((void*)row, (short)0, cnt * 4, 0x00020000);
+  const int gbase = S.base + S.off[j];
+  int idx_next = __builtin_amdgcn_raw_buffer_load_b32(rrs, 4 * lane, 0, 0);
   int m = 0;
   for (int e0 = 0; e0 < cnt; e0 += 64) {
     const int e = e0 + lane;
-    bool keep = false;
-    float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (e < cnt) {
-      const int idx = row[e];
-      const int g = S.base + S.off[j] + idx;
-      const float4 pj = a.pos[g];
-      float dx, dy, dz;
-      if (S.tri) {
-        // general cell: fractional minimum image (deepmd_jax/utils.py:24-27)
-        float s[3], d[3] = {pj.x - pc.x, pj.y - pc.y, pj.z - pc.z};
-        cell_mul(S.Hi, d, s);
+    const int idx = idx_next;
+    idx_next = __builtin_amdgcn_raw_buffer_load_b32(rrs, 4 * (e + 64), 0, 0);
+    const int g = gbase + idx;
+    const float4 pj = a.pos[g];
+    float dx, dy, dz;
+    if (S.tri) {
+      // general cell: fractional minimum image (deepmd_jax/utils.py:24-27)
+      float sf[3], d[3] = {pj.x - pc.x, pj.y - pc.y, pj.z - pc.z};
+      cell_mul(S.Hi, d, sf);
 #pragma unroll
-        for (int k = 0; k < 3; ++k) s[k] -= rintf(s[k]);
-        cell_mul(S.H, s, d);
-        dx = d[0], dy = d[1], dz = d[2];
-      } else {
-        dx = minimg(pj.x - pc.x, S.box[0]);
-        dy = minimg(pj.y - pc.y, S.box[1]);
-        dz = minimg(pj.z - pc.z, S.box[2]);
-      }
-      const float r = pair_r(dx, dy, dz);
-      keep = (r > 1e-14f) && (r / a.rcut < 1.f);
-      out = make_float4(dx, dy, dz, __int_as_float(zrow ? e : g));
-      if (zrow && !keep) {
-        zrow[3 * e] = 0.f;
-        zrow[3 * e + 1] = 0.f;
-        zrow[3 * e + 2] = 0.f;
-      }
+      for (int k = 0; k < 3; ++k) sf[k] -= rintf(sf[k]);
+      cell_mul(S.H, sf, d);
+      dx = d[0], dy = d[1], dz = d[2];
+    } else {
+      dx = minimg(pj.x - pc.x, S.box[0]);
+      dy = minimg(pj.y - pc.y, S.box[1]);
+      dz = minimg(pj.z - pc.z, S.box[2]);
+    }
+    const float r = pair_r(dx, dy, dz);
+    const bool in_row = e < cnt;
+    const bool keep = in_row && (r > 1e-14f) && (r / a.rcut < 1.f);
+    const float4 out = make_float4(dx, dy, dz, __int_as_float(zrow ? e : g));
+    if (zrow && in_row && !keep) {
+      zrow[3 * e] = 0.f;
+      zrow[3 * e + 1] = 0.f;
+      zrow[3 * e + 2] = 0.f;
     }
     const unsigned long long bal = __ballot(keep);
     const int pre = __popcll(bal & ((1ull << lane) - 1ull));
