@@ -922,21 +922,22 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
         float* Xm = dTp;           // [12][16]: X[x][a], rows 10, 11 and columns a >= A zero
         float* Um = Xm + 12 * 16;  // [10][16]: U[x][a]
         static_assert(12 * 16 + 10 * 16 <= 10 * HS, "X and U fit dTp's space");
-        for (int e = lane; e < 12 * 16; e += 64) {
-          const int x = e >> 4, aa = e & 15;
-          float v = 0.f;
-          if (x < 10 && aa < A) {
-            if (x < 4) {
-              v = Tl[x * W + aa];
-            } else if (x < 7) {
-              const float t3 = Tl[(x - 3) * W + A + aa];
-              v = t3 * t3 + Tl[x * W + A + aa];
-            } else {
-              const int kp = x - 7;
-              v = r2 * Tl[(1 + kp) * W + A + aa] * Tl[(1 + (kp + 2) % 3) * W + A + aa] + Tl[x * W + A + aa];
-            }
-          }
-          Xm[e] = v;
+        // X[x][a], x = 4k + q (lane group q, pass k), a = n: every candidate term read from an
+        // in-range row and selected per lane group, no branches.  x < 4: T[x][a]; 4 <= x < 7:
+        // T3[x-3][A+a]^2 + T[x][A+a]; 7 <= x < 10: sqrt2 T3[kp][A+a] T3[(kp+2)%3][A+a] + T[x][A+a]
+        // (kp = x - 7; rows of T3 are rows 1..3 of T)
+        {
+          const int aa = n;
+          const float* TA = Tl + A + aa;
+          const float x0 = Tl[q * W + aa];
+          const float t31 = TA[(1 + (q < 3 ? q : 0)) * W];
+          const float x1 = q < 3 ? t31 * t31 + TA[(4 + q) * W] : r2 * TA[W] * TA[3 * W] + TA[7 * W];
+          const int q2 = q < 2 ? q : 1;
+          const float x2 = q < 2 ? r2 * TA[(2 + q2) * W] * TA[(1 + q2) * W] + TA[(8 + q2) * W] : 0.f;
+          const bool in = aa < A;
+          Xm[lane] = in ? x0 : 0.f;
+          Xm[lane + 64] = in ? x1 : 0.f;
+          Xm[lane + 128] = in ? x2 : 0.f;
         }
         {
           floatx4 u0 = floatx4{0.f, 0.f, 0.f, 0.f}, u1 = u0;
@@ -956,8 +957,17 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
         float xa[4];
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) xa[kk] = n < 12 ? Xm[n * 16 + 4 * kk + q] : 0.f;
-        const float* gg = a.wc_g ? a.wc_g + (size_t)gc * 3 : nullptr;
-        // results held until X and U have been read, then written over the cleared dTp
+        // centroid-model hook: the output out_norm sum_w f_w T3[c][w] depends on T3 directly
+        float gv[3] = {0.f, 0.f, 0.f};
+        if (a.wc_g) {
+          const float* gg = a.wc_g + (size_t)gc * 3;
+          gv[0] = gg[0], gv[1] = gg[1], gv[2] = gg[2];
+        }
+        // results held until X and U have been read, then written over the cleared dTp.  Lane
+        // (q, n) holds V[x = 4q + i][w = 16t + n]; the U terms reach w < 2A <= 32 only (tiles 0,
+        // 1).  Lane group 0 (x = i < 4) adds U[x][w] for w < A and, for x >= 1 and A <= w < 2A,
+        // the terms of the second-order factors; groups 1-3 (x >= 4) add U[x][w - A] for A <= w
+        // < 2A.  Every read is in range (clamped index) and selected, no branches.
         float res[T3][4];
         int rslot[T3];
 #pragma unroll
@@ -966,44 +976,49 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
           floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
           for (int kk = 0; kk < 4; ++kk) acc = mfma4(xa[kk], dgv[t][kk], acc);
-          rslot[t] = -1;
-          if (w >= W) continue;
           int slot = -1;
+#pragma unroll
           for (int qq = 0; qq < 4; ++qq) {
             const int rr = w - a.base3[qq];
-            if (rr >= 0 && rr < a.size3[qq]) slot = 16 * (rr / 4) + 4 * qq + (rr % 4);
+            slot = (rr >= 0 && rr < a.size3[qq]) ? 16 * (rr >> 2) + 4 * qq + (rr & 3) : slot;
           }
-          const float y = gg ? a.wc_norm * a.wc_y3[(size_t)grow * a.wc_ldy + w] : 0.f;
+          rslot[t] = w < W ? slot : -1;
+          float y = 0.f;
+          if (a.wc_g && w < W) y = a.wc_norm * a.wc_y3[(size_t)grow * a.wc_ldy + w];
+          const bool lo = w < A, mid = w >= A && w < 2 * A;
+          const int wl = lo ? w : 0, wm = mid ? w : 0, am = mid ? w - A : 0;
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
-            const int x = 4 * q + i;
-            if (x >= 10) continue;
             float v = acc[i];
-            if (w < A) {
-              if (x < 4) v += Um[x * 16 + w];
-            } else if (w < 2 * A) {
-              const int aa = w - A;
-              if (x >= 4) {
-                v += Um[x * 16 + aa];
-              } else if (x >= 1) {
-                const int cc = x - 1;
-                v += 2.f * Tl[x * W + w] * Um[(4 + cc) * 16 + aa] +
-                     r2 * Tl[(1 + (cc + 2) % 3) * W + w] * Um[(7 + cc) * 16 + aa] +
-                     r2 * Tl[(1 + (cc + 1) % 3) * W + w] * Um[(7 + (cc + 1) % 3) * 16 + aa];
+            if (t < 2) {
+              float c0 = lo ? Um[i * 16 + wl] : 0.f;
+              if (i >= 1) {
+                const int cc = i - 1, cm = (cc + 2) % 3, cp = (cc + 1) % 3;
+                const float m2 = 2.f * Tl[i * W + wm] * Um[(4 + cc) * 16 + am] +
+                                 r2 * Tl[(1 + cm) * W + wm] * Um[(7 + cc) * 16 + am] +
+                                 r2 * Tl[(1 + cp) * W + wm] * Um[(7 + cp) * 16 + am];
+                c0 += mid ? m2 : 0.f;
               }
+              const int xr = 4 * q + i < 10 ? 4 * q + i : 9;
+              const float c1 = mid ? Um[xr * 16 + am] : 0.f;
+              v += q == 0 ? c0 : c1;
             }
-            // the centroid output out_norm sum_w f_w T3[c][w] depends on T3 directly
-            if (gg && x >= 1 && x <= 3) v += gg[x - 1] * y;
+            if (i >= 1) v += q == 0 ? gv[i - 1] * y : 0.f;
             res[t][i] = v * a.inv_nnbrs;
           }
-          rslot[t] = slot;
         }
         wave_sync();
         if constexpr (H16) {
           // the FP16 operand copies of the 16-bit forms, written from the registers (layouts
-          // below); entries no result reaches (padding slots, row x = 10) stay zero
-          for (int e = lane; e < embed_dt16_floats(T3); e += 64) wbase[e] = 0.f;
+          // below); entries no result reaches (padding slots, row x = 10) stay zero.  Stores
+          // without a target go to a scratch word past the copies (the chunk tiles, written
+          // before they are read)
+          constexpr int D16 = embed_dt16_floats(T3);
+          // whole clearing passes: the last one may run into the chunk tiles (written before read)
+#pragma unroll
+          for (int k = 0; k < (D16 + 63) / 64; ++k) wbase[lane + 64 * k] = 0.f;
           wave_sync();
+          uint16_t* h16 = reinterpret_cast<uint16_t*>(wbase);
 #pragma unroll
           for (int t = 0; t < T3; ++t) {
             const int sl = rslot[t];
@@ -1011,21 +1026,27 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
               const int x = 4 * q + i;
-              if (sl >= 0 && x < 10) {
-                const uint16_t h = h16_bits<2>(res[t][i]);
-                dTa[((((tt >> 1) * 4 + qq) * 11 + x) << 3) + jj] = h;
-                dTb[sl * 16 + x] = h;
-              }
+              const bool ok = sl >= 0 && x < 10;
+              const uint16_t h = h16_bits<2>(res[t][i]);
+              h16[ok ? ((((tt >> 1) * 4 + qq) * 11 + x) << 3) + jj : 2 * D16 + lane] = h;
+              h16[ok ? DTA + sl * 16 + x : 2 * D16 + 64 + lane] = h;
             }
           }
         } else {
-          for (int e = lane; e < 10 * HS; e += 64) dTp[e] = 0.f;
+          // whole clearing passes (the last one runs into T's space, no longer read); stores
+          // without a target go to a scratch word in T's space
+          constexpr int ZP = (10 * HS + 63) / 64;
+          static_assert(64 * ZP <= ODT + 160 * T3, "clearing passes stay in the wave's space");
+#pragma unroll
+          for (int k = 0; k < ZP; ++k) dTp[lane + 64 * k] = 0.f;
           wave_sync();
 #pragma unroll
           for (int t = 0; t < T3; ++t)
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
-              if (rslot[t] >= 0 && 4 * q + i < 10) dTp[(4 * q + i) * HS + rslot[t]] = res[t][i];
+            for (int i = 0; i < 4; ++i) {
+              const bool ok = rslot[t] >= 0 && 4 * q + i < 10;
+              dTp[ok ? (4 * q + i) * HS + rslot[t] : 64 * ZP + lane] = res[t][i];
+            }
         }
       } else {
         for (int e = lane; e < 10 * A; e += 64) {
