@@ -255,9 +255,20 @@ static int32_t model_create(const dpmd_model_desc* d, int device, Model** out) {
   M->fit_k = FKS;
   DPMD_HIP(hipMalloc(&M->d_fit, sizeof(float) * fit.size()));
   DPMD_HIP(hipMemcpy(M->d_fit, fit.data(), sizeof(float) * fit.size(), hipMemcpyHostToDevice));
-  for (int h = 0; h < 2; ++h) {
+  // BF16 and FP16 copies, and the BF16 low parts bf16(w - bf16(w)) of the split form
+  for (int h = 0; h < 3; ++h) {
     std::vector<uint16_t> fh(fit.size());
-    for (size_t i = 0; i < fit.size(); ++i) fh[i] = h ? half_bits(fit[i]) : bf16_bits(fit[i]);
+    for (size_t i = 0; i < fit.size(); ++i) {
+      if (h == 2) {
+        const uint16_t hi = bf16_bits(fit[i]);
+        const uint32_t u = (uint32_t)hi << 16;
+        float fhi;
+        memcpy(&fhi, &u, 4);
+        fh[i] = bf16_bits(fit[i] - fhi);
+      } else {
+        fh[i] = h ? half_bits(fit[i]) : bf16_bits(fit[i]);
+      }
+    }
     DPMD_HIP(hipMalloc(&M->d_fit_h[h], sizeof(uint16_t) * fh.size()));
     DPMD_HIP(hipMemcpy(M->d_fit_h[h], fh.data(), sizeof(uint16_t) * fh.size(), hipMemcpyHostToDevice));
   }
@@ -465,13 +476,16 @@ static int32_t model_forces_impl(System* s, int precision) {
   const bool det = s->deterministic && !s->wc_model && s->d_pgrad;
   if (!det) DPMD_TRY(md_zero_force(s));
   timer_begin(s, 1, &ev);
-  DPMD_TRY(embed_launch(s, false, precision));
+  // embedding and fitting precisions of the form; the descriptor is stored in the format the
+  // fitting reads (16-bit only for the plain 16-bit GEMMs; the split and FP32 ones read FP32)
+  const int pe = prec_embed(precision), pf = prec_fit(precision);
+  DPMD_TRY(embed_launch(s, false, pe, pf == DPMD_PREC_BF16 || pf == DPMD_PREC_FP16 ? pf : 0));
   timer_end(s, 1, ev);
   timer_begin(s, 2, &ev);
-  DPMD_TRY(fit_launch(s, precision));
+  DPMD_TRY(fit_launch(s, pf));
   timer_end(s, 2, ev);
   timer_begin(s, 3, &ev);
-  DPMD_TRY(embed_launch(s, true, precision));
+  DPMD_TRY(embed_launch(s, true, pe));
   if (det) DPMD_TRY(force_gather(s));
   timer_end(s, 3, ev);
   // DPLR long-range term (centroid model FP32 whatever the energy model's precision)
@@ -728,10 +742,10 @@ int32_t dpmd_model_destroy(dpmd_model_t model) {
   if (M->d_tbias) hipFree(M->d_tbias);
   if (M->d_fit) hipFree(M->d_fit);
   if (M->d_tab) hipFree(M->d_tab);
-  for (int k = 0; k < 2; ++k) {
+  for (int k = 0; k < 2; ++k)
     if (M->d_emb_h[k]) hipFree(M->d_emb_h[k]);
+  for (int k = 0; k < 3; ++k)
     if (M->d_fit_h[k]) hipFree(M->d_fit_h[k]);
-  }
   delete M;
   return DPMD_OK;
 }
@@ -977,7 +991,7 @@ int32_t dpmd_system_set_deterministic(dpmd_system_t sys, int32_t on) {
 int32_t dpmd_compute(dpmd_system_t sys, int32_t precision, double* energy, double* force, double* virial,
                      double* atom_energy) {
   SYS(sys);
-  if (precision != DPMD_PREC_FP32 && precision != DPMD_PREC_BF16 && precision != DPMD_PREC_FP16) {
+  if (!prec_valid(precision)) {
     set_error("unknown precision %d", precision);
     return DPMD_E_ARG;
   }
@@ -1074,7 +1088,7 @@ int32_t dpmd_md_run(dpmd_system_t sys, const dpmd_md_params* p, int32_t nsteps, 
   NptArgs na;
   memset(&na, 0, sizeof(na));
   if (npt || pnpt) DPMD_TRY(npt_args(s, p, na));
-  if (p->precision != DPMD_PREC_FP32 && p->precision != DPMD_PREC_BF16 && p->precision != DPMD_PREC_FP16) {
+  if (!prec_valid(p->precision)) {
     set_error("unknown precision %d", p->precision);
     return DPMD_E_ARG;
   }
@@ -1544,7 +1558,7 @@ struct WorkBuffers {
 extern "C" int32_t dpmd_compute_noalloc(dpmd_system_t sys, int32_t precision, double* energy, double* force,
                                         double* virial) {
   SYS(sys);
-  if (precision != DPMD_PREC_FP32 && precision != DPMD_PREC_BF16 && precision != DPMD_PREC_FP16) {
+  if (!prec_valid(precision)) {
     set_error("unknown precision %d", precision);
     return DPMD_E_ARG;
   }

@@ -1552,10 +1552,10 @@ static void fill_args(System* s, EmbedArgs& a) {
 }
 
 template <int T1, int T2, int T3, int Z2, int Z3, int PREC, int V1, bool TAB>
-static int32_t launch_embed(System* s, bool backward, int precision) {
+static int32_t launch_embed(System* s, bool backward, int precision, int g16) {
   EmbedArgs a;
   fill_args(s, a);
-  a.g16 = backward ? 0 : precision;
+  a.g16 = backward ? 0 : (g16 >= 0 ? g16 : precision);
   if (PREC) {
     Model* M = s->model;
     a.hnet = M->d_emb_h[PREC - 1];
@@ -1609,19 +1609,19 @@ static int32_t launch_embed(System* s, bool backward, int precision) {
 // A tabulated (compressed) model has no embedding GEMMs: its kernel is FP32 whatever the
 // precision of the fitting GEMMs.
 template <int T1, int T2, int T3, int Z2, int Z3, int V1>
-static int32_t launch_prec(System* s, bool backward, int precision) {
-  if (s->model->tab_n > 0) return launch_embed<T1, T2, T3, Z2, Z3, 0, V1, true>(s, backward, precision);
-  if (precision == DPMD_PREC_BF16) return launch_embed<T1, T2, T3, Z2, Z3, 1, V1, false>(s, backward, precision);
-  if (precision == DPMD_PREC_FP16) return launch_embed<T1, T2, T3, Z2, Z3, 2, V1, false>(s, backward, precision);
-  return launch_embed<T1, T2, T3, Z2, Z3, 0, V1, false>(s, backward, precision);
+static int32_t launch_prec(System* s, bool backward, int precision, int g16) {
+  if (s->model->tab_n > 0) return launch_embed<T1, T2, T3, Z2, Z3, 0, V1, true>(s, backward, precision, g16);
+  if (precision == DPMD_PREC_BF16) return launch_embed<T1, T2, T3, Z2, Z3, 1, V1, false>(s, backward, precision, g16);
+  if (precision == DPMD_PREC_FP16) return launch_embed<T1, T2, T3, Z2, Z3, 2, V1, false>(s, backward, precision, g16);
+  return launch_embed<T1, T2, T3, Z2, Z3, 0, V1, false>(s, backward, precision, g16);
 }
 
-int32_t embed_launch(System* s, bool backward, int precision) {
+int32_t embed_launch(System* s, bool backward, int precision, int g16) {
   const EmbedLayout& L = s->model->lay;
   if (L.tiles[0] == 2 && L.tiles[1] == 4 && L.tiles[2] == 7 && L.z[1] == 2 && L.z[2] == 2)
-    return launch_prec<2, 4, 7, 2, 2, 7>(s, backward, precision);  // 7 layer-3 tiles imply v1 <= 7
+    return launch_prec<2, 4, 7, 2, 2, 7>(s, backward, precision, g16);  // 7 layer-3 tiles imply v1 <= 7
   if (L.tiles[0] == 2 && L.tiles[1] == 2 && L.tiles[2] == 4 && L.z[1] == 1 && L.z[2] == 2)
-    return launch_prec<2, 2, 4, 1, 2, 8>(s, backward, precision);
+    return launch_prec<2, 2, 4, 1, 2, 8>(s, backward, precision, g16);
   set_error("embedding widths (%d,%d,%d) have no kernel instantiation", L.w[0], L.w[1], L.w[2]);
   return DPMD_E_UNSUPPORTED;
 }

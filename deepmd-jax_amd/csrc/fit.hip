@@ -29,6 +29,7 @@ struct GemmArgs {
   const uint16_t* Ah;  // or M x K already in the 16-bit operand format (16-bit GEMMs), stride lda
   int lda;
   const void* Bt;   // N x K (fp32 or 16-bit), row stride ldb elements
+  const void* Btl;  // split BF16 form: the low parts of Bt (same layout)
   int ldb;
   int M, N, K;
   float* C;         // primary output, row stride ldc
@@ -221,6 +222,13 @@ struct Half8<2> {
     return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
   }
 };
+// PREC 3: the split BF16 form.  Operands are BF16 high parts and BF16 low parts of the
+// residual, x = hi + lo + O(2^-16 |x|); a product is hi*hi + hi*lo + lo*hi on BF16 MFMAs with
+// FP32 accumulation (the lo*lo term is below the residual).
+template <>
+struct Half8<3> : Half8<1> {};
+// the low part of the split: bf16(x - hi), hi = bf16(x) (round to nearest even)
+__device__ __forceinline__ __bf16 bf16_low(float x, __bf16 hi) { return (__bf16)(x - (float)hi); }
 
 // 16-bit GEMM: block tile 64 rows x 16 NT columns, four waves as 2 x 2, each owning 32
 // rows (two 16-row tiles) and half of the column tiles, so every B fragment read from LDS
@@ -231,9 +239,13 @@ template <int NT, int EPI, int PREC, int KC, bool A16 = false>
 __global__ void __launch_bounds__(256, 2) gemm_h_kernel(const GemmArgs g) {
   typedef typename Half8<PREC>::T HT;
   typedef typename Half8<PREC>::E HE;
+  constexpr bool SPLIT = PREC == 3;   // low-part tiles beside the high parts
+  static_assert(!(SPLIT && A16), "the split form reads FP32 A");
   constexpr int LS = KC + 8;          // LDS row stride (elements)
   __shared__ __attribute__((aligned(16))) uint16_t As[64 * LS];
   __shared__ __attribute__((aligned(16))) uint16_t Bs[16 * NT * LS];
+  __shared__ __attribute__((aligned(16))) uint16_t Asl[SPLIT ? 64 * LS : 8];
+  __shared__ __attribute__((aligned(16))) uint16_t Bsl[SPLIT ? 16 * NT * LS : 8];
   __shared__ float Es[64];
   constexpr int BN = 16 * NT;
   constexpr int NH = (NT + 1) / 2;   // column tiles per wave
@@ -259,6 +271,8 @@ __global__ void __launch_bounds__(256, 2) gemm_h_kernel(const GemmArgs g) {
   floatx4 ra[2 * NAP];
   uintx4 rh[NAP];
   uintx4 rb[NB];
+  uintx4 rbl[SPLIT ? NB : 1];
+  const uint16_t* Btl = (const uint16_t*)g.Btl;
   auto live = [&](int piece) { return piece + 1 < NB || tid + 256 * piece < BV; };
   auto fetch = [&](int k) {
 #pragma unroll
@@ -275,7 +289,11 @@ __global__ void __launch_bounds__(256, 2) gemm_h_kernel(const GemmArgs g) {
 #pragma unroll
     for (int p = 0; p < NB; ++p) {
       const int idx = tid + 256 * p;
-      if (live(p)) rb[p] = *reinterpret_cast<const uintx4*>(Bt + (size_t)(n0 + idx / P8) * g.ldb + k + 8 * (idx % P8));
+      const size_t off = (size_t)(n0 + idx / P8) * g.ldb + k + 8 * (idx % P8);
+      if (live(p)) {
+        rb[p] = *reinterpret_cast<const uintx4*>(Bt + off);
+        if constexpr (SPLIT) rbl[p] = *reinterpret_cast<const uintx4*>(Btl + off);
+      }
     }
   };
   fetch(0);
@@ -286,6 +304,19 @@ __global__ void __launch_bounds__(256, 2) gemm_h_kernel(const GemmArgs g) {
       const int idx = tid + 256 * p;
       if constexpr (A16) {
         *reinterpret_cast<uintx4*>(&As[(idx / P8) * LS + 8 * (idx % P8)]) = rh[p];
+      } else if constexpr (SPLIT) {
+        const floatx4 x0 = ra[2 * p], x1 = ra[2 * p + 1];
+        HT h, l;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const __bf16 h0 = (__bf16)x0[j], h1 = (__bf16)x1[j];
+          h[j] = h0;
+          h[4 + j] = h1;
+          l[j] = bf16_low(x0[j], h0);
+          l[4 + j] = bf16_low(x1[j], h1);
+        }
+        *reinterpret_cast<HT*>(&As[(idx / P8) * LS + 8 * (idx % P8)]) = h;
+        *reinterpret_cast<HT*>(&Asl[(idx / P8) * LS + 8 * (idx % P8)]) = l;
       } else {
         const floatx4 x0 = ra[2 * p], x1 = ra[2 * p + 1];
         HT h;
@@ -297,7 +328,10 @@ __global__ void __launch_bounds__(256, 2) gemm_h_kernel(const GemmArgs g) {
 #pragma unroll
     for (int p = 0; p < NB; ++p) {
       const int idx = tid + 256 * p;
-      if (live(p)) *reinterpret_cast<uintx4*>(&Bs[(idx / P8) * LS + 8 * (idx % P8)]) = rb[p];
+      if (live(p)) {
+        *reinterpret_cast<uintx4*>(&Bs[(idx / P8) * LS + 8 * (idx % P8)]) = rb[p];
+        if constexpr (SPLIT) *reinterpret_cast<uintx4*>(&Bsl[(idx / P8) * LS + 8 * (idx % P8)]) = rbl[p];
+      }
     }
     __syncthreads();
     if (k0 + KC < g.K) fetch(k0 + KC);
@@ -305,6 +339,11 @@ __global__ void __launch_bounds__(256, 2) gemm_h_kernel(const GemmArgs g) {
     for (int sub = 0; sub < KC / 32; ++sub) {
       const HT a0 = *reinterpret_cast<const HT*>(&As[(32 * wm + n) * LS + 32 * sub + 8 * q]);
       const HT a1 = *reinterpret_cast<const HT*>(&As[(32 * wm + 16 + n) * LS + 32 * sub + 8 * q]);
+      HT a0l, a1l;
+      if constexpr (SPLIT) {
+        a0l = *reinterpret_cast<const HT*>(&Asl[(32 * wm + n) * LS + 32 * sub + 8 * q]);
+        a1l = *reinterpret_cast<const HT*>(&Asl[(32 * wm + 16 + n) * LS + 32 * sub + 8 * q]);
+      }
 #pragma unroll
       for (int nt = 0; nt < NH; ++nt) {
         const int tile = wn * NH + nt;
@@ -312,6 +351,13 @@ __global__ void __launch_bounds__(256, 2) gemm_h_kernel(const GemmArgs g) {
           const HT bv = *reinterpret_cast<const HT*>(&Bs[(16 * tile + n) * LS + 32 * sub + 8 * q]);
           acc[0][nt] = Half8<PREC>::mfma(a0, bv, acc[0][nt]);
           acc[1][nt] = Half8<PREC>::mfma(a1, bv, acc[1][nt]);
+          if constexpr (SPLIT) {
+            const HT bl = *reinterpret_cast<const HT*>(&Bsl[(16 * tile + n) * LS + 32 * sub + 8 * q]);
+            acc[0][nt] = Half8<PREC>::mfma(a0, bl, acc[0][nt]);
+            acc[1][nt] = Half8<PREC>::mfma(a1, bl, acc[1][nt]);
+            acc[0][nt] = Half8<PREC>::mfma(a0l, bv, acc[0][nt]);
+            acc[1][nt] = Half8<PREC>::mfma(a1l, bv, acc[1][nt]);
+          }
         }
       }
     }
@@ -387,6 +433,7 @@ static int32_t go(hipStream_t st, const GemmArgs& g, int prec) {
   }
   if (prec == DPMD_PREC_BF16 && k64) hipLaunchKernelGGL((gemm_h_kernel<NT, EPI, 1, 64>), grid, dim3(256), 0, st, g);
   else if (prec == DPMD_PREC_BF16) hipLaunchKernelGGL((gemm_h_kernel<NT, EPI, 1, 32>), grid, dim3(256), 0, st, g);
+  else if (prec == DPMD_PREC_BF16X3) hipLaunchKernelGGL((gemm_h_kernel<NT, EPI, 3, 32>), grid, dim3(256), 0, st, g);
   else if (prec == DPMD_PREC_FP16 && k64) hipLaunchKernelGGL((gemm_h_kernel<NT, EPI, 2, 64>), grid, dim3(256), 0, st, g);
   else if (prec == DPMD_PREC_FP16) hipLaunchKernelGGL((gemm_h_kernel<NT, EPI, 2, 32>), grid, dim3(256), 0, st, g);
   else if (g.K % 32 == 0) hipLaunchKernelGGL((gemm_f32_kernel<NT, EPI, 32>), grid, dim3(256), 0, st, g);
@@ -437,6 +484,7 @@ struct HiddenArgs {
   const float* Y1;   // rows x ld: y1 = tanh(G K0 + b0)
   int ld;            // row stride of the activations and of the weight blocks (elements)
   const void* W[4];  // A operands, row-major [m][k], stride ld: K1^T, K2^T (out x in), K2, K1 (in x out)
+  const void* Wl[4]; // split BF16 form: the low parts of W (same layout)
   const float* b1;
   const float* b2;
   const float* dt1;
@@ -460,8 +508,9 @@ __device__ __forceinline__ void lds_barrier() {
 }
 
 template <int NT, int PREC>
-__global__ void __launch_bounds__(256, (NT <= 8 && PREC != 2) ? 2 : 1) fit_hidden_kernel(const HiddenArgs h) {
+__global__ void __launch_bounds__(256, (NT <= 8 && PREC <= 1) ? 2 : 1) fit_hidden_kernel(const HiddenArgs h) {
   constexpr bool H16 = PREC != 0;
+  constexpr bool SPLIT = PREC == 3;  // low-part weight slices beside the high parts
   constexpr int NS = H16 ? (NT + 1) / 2 : NT;   // K slices per layer: 32 features (16-bit) or 16
   constexpr int LSW = H16 ? 40 : 20;            // LDS row stride of a staged slice (elements)
   constexpr int ESZ = H16 ? 2 : 4;
@@ -469,6 +518,7 @@ __global__ void __launch_bounds__(256, (NT <= 8 && PREC != 2) ? 2 : 1) fit_hidde
   constexpr int CH = 16 * NT * 4;               // 16-byte pieces per slice (4 per weight row)
   constexpr int NCH = (CH + 255) / 256;
   __shared__ __attribute__((aligned(16))) unsigned char Ws[2][SLICE];
+  __shared__ __attribute__((aligned(16))) unsigned char Wsl[SPLIT ? 2 : 1][SPLIT ? SLICE : 16];
   __shared__ __attribute__((aligned(16))) unsigned char spill_slot[32];  // stores of pieces past CH
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int q = lane >> 4, n = lane & 15;
@@ -493,14 +543,16 @@ __global__ void __launch_bounds__(256, (NT <= 8 && PREC != 2) ? 2 : 1) fit_hidde
       st[p] = *reinterpret_cast<const uintx4*>(src);
     }
   };
-  auto stage_store = [&](int b, const uintx4 (&st)[NCH]) {
+  auto stage_store = [&](int b, const uintx4 (&st)[NCH], bool low = false) {
 #pragma unroll
     for (int p = 0; p < NCH; ++p) {
       const int c = piece(p);
       const int r = c >> 2, u = c & 3;
       const bool past = (p + 1 == NCH && CH % 256 != 0) && tid + 256 * p >= CH;
       if constexpr (H16) {
-        uint16_t* d = reinterpret_cast<uint16_t*>(Ws[b]) + r * LSW + 16 * (u & 1) + 4 * (u >> 1);
+        unsigned char* base = Ws[b];
+        if constexpr (SPLIT) base = low ? Wsl[b] : Ws[b];
+        uint16_t* d = reinterpret_cast<uint16_t*>(base) + r * LSW + 16 * (u & 1) + 4 * (u >> 1);
         if (p + 1 == NCH && CH % 256 != 0) d = past ? reinterpret_cast<uint16_t*>(spill_slot) : d;
         *reinterpret_cast<uint2*>(d) = make_uint2(st[p][0], st[p][1]);
         *reinterpret_cast<uint2*>(d + 8) = make_uint2(st[p][2], st[p][3]);
@@ -518,6 +570,7 @@ __global__ void __launch_bounds__(256, (NT <= 8 && PREC != 2) ? 2 : 1) fit_hidde
   constexpr int PF = H16 ? 4 : 1;
   constexpr int NG = 4 * NS;
   uintx4 st[PF][NCH];
+  uintx4 stl[SPLIT ? PF : 1][NCH];
   // acc[ot] = sum_k W_l[16 ot + .][k] act[k][.] over layer l's K slices
   auto layer = [&](auto lc, floatx4 (&acc)[NT], const float (&bop)[NT][4]) {
     constexpr int l = decltype(lc)::value;
@@ -526,24 +579,35 @@ __global__ void __launch_bounds__(256, (NT <= 8 && PREC != 2) ? 2 : 1) fit_hidde
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
       const int g = l * NS + s;
-      if (g + PF < NG) stage_load(h.W[(g + PF) / NS], (g + PF) % NS, st[g % PF]);
+      if (g + PF < NG) {
+        stage_load(h.W[(g + PF) / NS], (g + PF) % NS, st[g % PF]);
+        if constexpr (SPLIT) stage_load(h.Wl[(g + PF) / NS], (g + PF) % NS, stl[g % PF]);
+      }
       // keep every slice's loads, MFMAs and stores in their slice (the scheduler would
       // otherwise hoist the prefetch of many slices at once and wait for all of them)
       __builtin_amdgcn_sched_barrier(0);
       if constexpr (H16) {
         typedef typename Half8<PREC>::T HT;
         typedef typename Half8<PREC>::E HE;
-        HT b;
+        HT b, bl;
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
           const int t = 2 * s + j / 4;
-          b[j] = t < NT ? (HE)bop[t < NT ? t : 0][j % 4] : (HE)0.f;
+          const float x = t < NT ? bop[t < NT ? t : 0][j % 4] : 0.f;
+          b[j] = (HE)x;
+          if constexpr (SPLIT) bl[j] = bf16_low(x, b[j]);
         }
         const uint16_t* Wl = reinterpret_cast<const uint16_t*>(Ws[g & 1]);
 #pragma unroll
         for (int ot = 0; ot < NT; ++ot) {
           const HT av = *reinterpret_cast<const HT*>(Wl + (16 * ot + n) * LSW + 8 * q);
           acc[ot] = Half8<PREC>::mfma(av, b, acc[ot]);
+          if constexpr (SPLIT) {
+            const HT avl = *reinterpret_cast<const HT*>(reinterpret_cast<const uint16_t*>(Wsl[g & 1]) +
+                                                        (16 * ot + n) * LSW + 8 * q);
+            acc[ot] = Half8<PREC>::mfma(av, bl, acc[ot]);
+            acc[ot] = Half8<PREC>::mfma(avl, b, acc[ot]);
+          }
         }
       } else {
         const float* Wl = reinterpret_cast<const float*>(Ws[g & 1]);
@@ -556,6 +620,7 @@ __global__ void __launch_bounds__(256, (NT <= 8 && PREC != 2) ? 2 : 1) fit_hidde
       }
       if (g + 1 < NG) {
         stage_store((g + 1) & 1, st[(g + 1) % PF]);
+        if constexpr (SPLIT) stage_store((g + 1) & 1, stl[(g + 1) % PF], true);
         lds_barrier();
       }
       __builtin_amdgcn_sched_barrier(0);
@@ -573,8 +638,12 @@ __global__ void __launch_bounds__(256, (NT <= 8 && PREC != 2) ? 2 : 1) fit_hidde
     for (int i = 0; i < 4; ++i) bop[t][i] = v[i];
   }
 #pragma unroll
-  for (int p = 0; p < PF; ++p) stage_load(h.W[p / NS], p % NS, st[p]);
+  for (int p = 0; p < PF; ++p) {
+    stage_load(h.W[p / NS], p % NS, st[p]);
+    if constexpr (SPLIT) stage_load(h.Wl[p / NS], p % NS, stl[p]);
+  }
   stage_store(0, st[0]);
+  if constexpr (SPLIT) stage_store(0, stl[0], true);
   lds_barrier();
   // t2 = tanh(y1 K1 + b1), y2 = t2 dt1 + y1
   layer(std::integral_constant<int, 0>(), acc, bop);
@@ -635,6 +704,7 @@ static int32_t hidden_go(hipStream_t st, const HiddenArgs& h, int rows, int prec
   const dim3 grid(rows / 64);
   if (prec == DPMD_PREC_BF16) hipLaunchKernelGGL((fit_hidden_kernel<NT, 1>), grid, dim3(256), 0, st, h);
   else if (prec == DPMD_PREC_FP16) hipLaunchKernelGGL((fit_hidden_kernel<NT, 2>), grid, dim3(256), 0, st, h);
+  else if (prec == DPMD_PREC_BF16X3) hipLaunchKernelGGL((fit_hidden_kernel<NT, 3>), grid, dim3(256), 0, st, h);
   else hipLaunchKernelGGL((fit_hidden_kernel<NT, 0>), grid, dim3(256), 0, st, h);
   DPMD_HIP(hipGetLastError());
   return DPMD_OK;
@@ -666,8 +736,12 @@ int32_t fit_launch(System* s, int precision) {
   float* DZ2 = DY2 + act;
   float* DZ1 = DZ2 + act;
   const float* Wf = M->d_fit;
-  const uint16_t* Wh = precision == DPMD_PREC_FP32 ? nullptr : M->d_fit_h[precision - 1];
-  if (precision != DPMD_PREC_FP32 && !Wh) {
+  // 16-bit weight copies: BF16 (also the high parts of the split form), FP16; the split form
+  // adds the BF16 low parts
+  const bool split = precision == DPMD_PREC_BF16X3;
+  const uint16_t* Wh = precision == DPMD_PREC_FP32 ? nullptr : M->d_fit_h[split ? 0 : precision - 1];
+  const uint16_t* Wlo = split ? M->d_fit_h[2] : nullptr;
+  if ((precision != DPMD_PREC_FP32 && !Wh) || (split && !Wlo)) {
     set_error("16-bit fitting weights missing");
     return DPMD_E_STATE;
   }
@@ -679,14 +753,16 @@ int32_t fit_launch(System* s, int precision) {
     auto B = [&](int blk) -> const void* {
       return Wh ? (const void*)(Wh + o[blk]) : (const void*)(Wf + o[blk]);
     };
+    auto BL = [&](int blk) -> const void* { return Wlo ? (const void*)(Wlo + o[blk]) : nullptr; };
     GemmArgs g;
     memset(&g, 0, sizeof(g));
     g.M = rows;
-    // 1: y1 = tanh(G K0 + b0); a 16-bit form's G was stored 16-bit by the embedding kernel
-    if (precision == DPMD_PREC_FP32) g.A = (const float*)s->d_G + r0 * D;
+    // 1: y1 = tanh(G K0 + b0); a plain 16-bit form's G was stored 16-bit by the embedding
+    // kernel, the FP32 and split forms read it FP32
+    if (precision == DPMD_PREC_FP32 || split) g.A = (const float*)s->d_G + r0 * D;
     else g.Ah = (const uint16_t*)s->d_G + r0 * D;
     g.lda = D; g.K = D;
-    g.Bt = B(FK0T); g.ldb = D; g.N = FN;
+    g.Bt = B(FK0T); g.Btl = BL(FK0T); g.ldb = D; g.N = FN;
     g.C = Y1 + r0 * FK; g.ldc = FK; g.bias = Wf + o[FB0];
     DPMD_TRY(gemm_dispatch<EPI_TANH>(s->stream, g, precision));
     // 2-5 in one kernel where the width has an instantiation
@@ -698,6 +774,10 @@ int32_t fit_launch(System* s, int precision) {
     hd.W[1] = B(FK2T);
     hd.W[2] = B(FK2);
     hd.W[3] = B(FK1);
+    hd.Wl[0] = BL(FK1T);
+    hd.Wl[1] = BL(FK2T);
+    hd.Wl[2] = BL(FK2);
+    hd.Wl[3] = BL(FK1);
     hd.b1 = Wf + o[FB1];
     hd.b2 = Wf + o[FB2];
     hd.dt1 = Wf + o[FDT1];
@@ -712,13 +792,13 @@ int32_t fit_launch(System* s, int precision) {
     } else {
       // 2: t2 = tanh(y1 K1 + b1), y2 = t2 dt1 + y1
       g.A = Y1 + r0 * FK; g.Ah = nullptr; g.lda = FK; g.K = FK;
-      g.Bt = B(FK1T); g.ldb = FK;
+      g.Bt = B(FK1T); g.Btl = BL(FK1T); g.ldb = FK;
       g.C = Y2 + r0 * FK; g.C2 = T2 + r0 * FK; g.aux1 = Y1 + r0 * FK;
       g.bias = Wf + o[FB1]; g.dt = Wf + o[FDT1];
       DPMD_TRY(gemm_dispatch<EPI_RES>(s->stream, g, precision));
       // 3: t3, y3, e, dz3
       g.A = Y2 + r0 * FK;
-      g.Bt = B(FK2T);
+      g.Bt = B(FK2T); g.Btl = BL(FK2T);
       g.C = DZ3 + r0 * FK; g.C2 = nullptr; g.aux1 = Y2 + r0 * FK;
       g.bias = Wf + o[FB2]; g.dt = Wf + o[FDT2]; g.k3 = Wf + o[FK3];
       g.b3 = M->fit_b3[t];
@@ -726,19 +806,19 @@ int32_t fit_launch(System* s, int precision) {
       DPMD_TRY(gemm_dispatch<EPI_E3>(s->stream, g, precision));
       // 4: dy2 = k3 + dz3 K2^T, dz2 = dy2 dt1 (1 - t2^2)
       g.A = DZ3 + r0 * FK;
-      g.Bt = B(FK2);
+      g.Bt = B(FK2); g.Btl = BL(FK2);
       g.C = DZ2 + r0 * FK; g.C2 = DY2 + r0 * FK; g.aux1 = T2 + r0 * FK;
       g.dt = Wf + o[FDT1];
       DPMD_TRY(gemm_dispatch<EPI_BWD2>(s->stream, g, precision));
       // 5: dy1 = dy2 + dz2 K1^T, dz1 = dy1 (1 - y1^2)
       g.A = DZ2 + r0 * FK;
-      g.Bt = B(FK1);
+      g.Bt = B(FK1); g.Btl = BL(FK1);
       g.C = DZ1 + r0 * FK; g.C2 = nullptr; g.aux1 = DY2 + r0 * FK; g.aux2 = Y1 + r0 * FK;
       DPMD_TRY(gemm_dispatch<EPI_BWD1>(s->stream, g, precision));
     }
     // 6: dG = dz1 K0^T
     g.A = DZ1 + r0 * FK; g.Ah = nullptr; g.lda = FK;
-    g.Bt = B(FK0); g.ldb = FK; g.K = FK;
+    g.Bt = B(FK0); g.Btl = BL(FK0); g.ldb = FK; g.K = FK;
     g.N = D;
     g.C = s->d_dG + r0 * D; g.ldc = D;
     DPMD_TRY(gemm_dispatch<EPI_STORE>(s->stream, g, precision));

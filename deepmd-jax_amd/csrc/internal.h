@@ -110,7 +110,7 @@ struct Model {
   size_t fit_total = 0;
   float fit_b3[kMaxT] = {0, 0, 0, 0};
   int fit_n = 0, fit_k = 0;  // output width (x16) and K stride (x32) of the hidden GEMMs
-  uint16_t* d_fit_h[2] = {nullptr, nullptr};  // half-precision copies of kernels
+  uint16_t* d_fit_h[3] = {nullptr, nullptr, nullptr};  // BF16, FP16 copies of the kernels; BF16 low parts
   // tabulated embedding nets (compressed model): per net [interval][6 powers][16*tiles3]
   // in the layer-3 physical slot order, so one float4 load fills four accumulator rows
   int tab_n = 0;                   // intervals; 0 = evaluate the MLPs
@@ -346,8 +346,15 @@ int32_t nlist_reverse(System* s, bool conditional);
 int32_t force_gather(System* s);
 int32_t nlist_alloc_cells(System* s);
 int32_t nlist_clear_flags(System* s);
-int32_t embed_launch(System* s, bool backward, int precision);
+// precision: DPMD_PREC_FP32 / BF16 / FP16 of the embedding GEMMs; g16 (forward): the format
+// the descriptor is stored in for the fitting (0 FP32, 1 BF16, 2 FP16; -1: as precision)
+int32_t embed_launch(System* s, bool backward, int precision, int g16 = -1);
+// precision: FP32 / BF16 / FP16, or DPMD_PREC_BF16X3 (the three-product BF16 split)
 int32_t fit_launch(System* s, int precision);
+// the embedding and fitting precisions of a DPMD_PREC_* code, and whether it is one
+inline int prec_embed(int p) { return p == DPMD_PREC_BF16X3 || p == DPMD_PREC_BF16_FIT32 ? DPMD_PREC_BF16 : p; }
+inline int prec_fit(int p) { return p == DPMD_PREC_BF16_FIT32 ? DPMD_PREC_FP32 : p; }
+inline bool prec_valid(int p) { return p >= DPMD_PREC_FP32 && p <= DPMD_PREC_BF16_FIT32; }
 // atomic (centroid) model hidden layers: forward into the activation buffers, and the backward
 // from dE/d(last hidden layer) (buffer DY3) to dG; buffers laid out by fit_atomic_buffers
 struct AtomicFitBufs {

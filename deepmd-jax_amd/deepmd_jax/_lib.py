@@ -9,7 +9,9 @@ import os
 import numpy as np
 
 MAXT = 4
-PREC = {'fp32': 0, 'bf16': 1, 'fp16': 2}
+# arithmetic of the embedding / fitting GEMMs (DPMD_PREC_*): FP32, BF16, FP16, and the mixed
+# forms with BF16 embedding GEMMs and split-BF16 ('bf16x3') or FP32 ('bf16-fit32') fitting GEMMs
+PREC = {'fp32': 0, 'bf16': 1, 'fp16': 2, 'bf16x3': 3, 'bf16-fit32': 4}
 ROUTINE = {'NVE': 0, 'NVT': 1, 'NVT_langevin': 2, 'PIMD': 3, 'NPT': 4, 'PIMD_NPT': 5}
 
 _HERE = os.path.dirname(os.path.abspath(__file__))

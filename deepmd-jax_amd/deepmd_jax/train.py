@@ -21,7 +21,9 @@ def evaluate(model_path, coord, box, type_idx, batch_size=1, precision='fp32'):
 
     coord (F, N, 3); box (F,), (F,1), (F,3), (F,9) or (F,3,3); type_idx (N,).
     Returns {'energy': (F,), 'force': (F, N, 3)} in the caller's atom order.
-    precision: 'fp32' (default), 'bf16' or 'fp16' embedding/fitting GEMMs.
+    precision: 'fp32' (default), 'bf16' or 'fp16' embedding/fitting GEMMs, or the mixed forms
+        'bf16x3' (BF16 embedding GEMMs, fitting GEMMs as three BF16 products of split operands)
+        and 'bf16-fit32' (BF16 embedding, FP32 fitting GEMMs).
     """
     try:
         coord = np.asarray(coord)

@@ -37,6 +37,12 @@ extern "C" {
 #define DPMD_PREC_FP32 0
 #define DPMD_PREC_BF16 1
 #define DPMD_PREC_FP16 2
+/* Mixed forms: BF16 embedding GEMMs with fitting GEMMs as three BF16 products (operands
+ * split into BF16 high and low parts: hi*hi + hi*lo + lo*hi, FP32 accumulation), or with
+ * FP32 fitting GEMMs.  The fitting net's operand rounding sets the energy bias whose change
+ * along a trajectory shows as NVE drift; the split keeps its GEMMs on BF16 MFMAs. */
+#define DPMD_PREC_BF16X3 3
+#define DPMD_PREC_BF16_FIT32 4
 
 /* Integrators for dpmd_md_run (deepmd_jax/md.py:313-383). */
 #define DPMD_NVE 0          /* velocity Verlet, deepmd_jax/routine.py:139-144 */
