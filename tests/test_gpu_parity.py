@@ -375,3 +375,62 @@ def test_ase_energy_forces_stress(device_model, model0, nmol):
     assert abs(e - E.item()) / N <= 1e-5
     assert np.abs(f - F.numpy()).max() <= 1e-4 * max(1.0, np.abs(F.numpy()).max())
     assert np.abs(s - ref).max() <= 1e-4 * np.abs(ref).max(), (s, ref)
+
+
+@pytest.mark.parametrize('prec', ['bf16x3', 'bf16-fit32'])
+def test_mixed_precision_forms(device_model, model0, prec):
+    """BF16 embedding GEMMs with split-BF16 ('bf16x3': hi*hi + hi*lo + lo*hi on BF16 MFMAs) or
+    FP32 ('bf16-fit32') fitting GEMMs: within the 16-bit forms' tolerances against the FP64
+    oracle, and the fitting's share of the error nearly gone.  With the embedding precision
+    fixed (BF16), force(plain BF16 fitting) - force(FP32 fitting) is the BF16 fitting's error;
+    the split fitting must leave under a twentieth of it (its products keep ~16 significand
+    bits against BF16's 8), and the two mixed forms agree to the same degree."""
+    from deepmd_jax import synthetic as syn
+    print('case prec', prec, flush=True)
+    params, var = model0
+    coord, box, types = syn.water_box(216, seed=11)
+    E, F, W = O.energy_force_virial(params, var, coord, box, types, lattice='auto')
+    E, F, W = E.item(), F.numpy(), W.numpy()
+    s = _system(device_model, coord, box, types, params['rcut'])
+    N = len(types)
+    e, f, v, _ = s.compute(prec)
+    rmse = np.sqrt(((f[0] - F) ** 2).mean())
+    print(prec, 'dE/atom', abs(e[0] - E) / N, 'F rmse', rmse, 'W maxrel', np.abs(v[0] - W).max() / np.abs(W).max())
+    assert abs(e[0] - E) / N <= 1e-3
+    assert rmse <= 1e-3
+    assert np.abs(v[0] - W).max() <= 1e-2 * np.abs(W).max()
+    eb, fb, _, _ = s.compute('bf16')
+    ef, ff, _, _ = s.compute('bf16-fit32')
+    fit_err = np.sqrt(((fb[0] - ff[0]) ** 2).mean())
+    split_err = np.sqrt(((s.compute('bf16x3')[1][0] - ff[0]) ** 2).mean())
+    print('BF16 fitting error', fit_err, 'split fitting error', split_err, 'energy', abs(eb[0] - ef[0]) / N)
+    assert fit_err > 0
+    assert split_err <= 0.05 * fit_err
+
+
+def test_simulation_bf16x3_runs(device_model, model0):
+    """Simulation with the split-BF16 form: NVE steps stay finite and conserve KE + PE to the
+    16-bit forms' level over 100 steps of a 216-molecule box."""
+    from deepmd_jax import synthetic as syn
+    from deepmd_jax.md import Simulation
+    from deepmd_jax.dpmodel import DPModel
+    from deepmd_jax.constants import MASS_UNIT_CONVERSION
+    import contextlib
+    import io
+    params, var = model0
+    coord, box, types = syn.water_box(216, seed=3)
+    vel = syn.maxwell_boltzmann(types, 300.0, seed=4)
+    m = np.array(syn.MASSES)[types] * MASS_UNIT_CONVERSION
+
+    def h(sim):
+        return float(sim.getEnergy()) + 0.5 * float((m[:, None] * np.asarray(sim.getVelocity()) ** 2).sum())
+    with contextlib.redirect_stdout(io.StringIO()):
+        sim = Simulation(None, box, types, syn.MASSES, 'NVE', 0.5, coord, initial_velocity=vel,
+                         report_interval=10 ** 9, neighbor_skin=1.0, precision='bf16x3',
+                         model=(DPModel(params), var, device_model))
+        h0 = h(sim)
+        sim.run(100)
+        h1 = h(sim)
+    assert np.isfinite(sim.getPosition()).all()
+    print('H change per atom', (h1 - h0) / len(types))
+    assert abs(h1 - h0) / len(types) < 1e-3
