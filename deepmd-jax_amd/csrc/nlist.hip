@@ -50,12 +50,20 @@ __device__ __forceinline__ float wrap_axis_f32(double x, double Ld, float Lf) {
   return fmod_floor((float)r, Lf);
 }
 
+// fmod_floor(x, L) for x in (-L, 2L), the range of d + L/2 for two canonical coordinates in
+// [0, L): x - L is exact there (Sterbenz), and x + L is the same rounded sum fmod_floor forms,
+// so the result is bit-identical without the library fmod.
+__device__ __forceinline__ float wrap_half_open(float x, float L) {
+  const float r = x >= L ? x - L : x;
+  return r < 0.f ? r + L : r;
+}
+
 __device__ __forceinline__ float pair_d2(float4 a, float4 b, float Lx, float Ly, float Lz) {
   float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
   const float hx = Lx * 0.5f, hy = Ly * 0.5f, hz = Lz * 0.5f;
-  dx = fmod_floor(dx + hx, Lx) - hx;
-  dy = fmod_floor(dy + hy, Ly) - hy;
-  dz = fmod_floor(dz + hz, Lz) - hz;
+  dx = wrap_half_open(dx + hx, Lx) - hx;
+  dy = wrap_half_open(dy + hy, Ly) - hy;
+  dz = wrap_half_open(dz + hz, Lz) - hz;
   const float sx = dx * dx, sy = dy * dy, sz = dz * dz;
   return (sx + sy) + sz;
 }
