@@ -21,6 +21,7 @@
 // backward's dT tables go through LDS.
 #include "internal.h"
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 #include <algorithm>
 #include <type_traits>
@@ -38,6 +39,33 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+
+// Lane swaps of gfx950 (VALU, no LDS): permlane32_swap(x, y) exchanges the upper 32 lanes of x
+// with the lower 32 lanes of y, permlane16_swap(x, y) the odd 16-lane rows of x with the even
+// rows of y; each returns the new (x, y).
+__device__ __forceinline__ float swap32_lo(float x, float y) {
+  return __uint_as_float(__builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(y), false, false)[0]);
+}
+__device__ __forceinline__ float swap16_lo(float x, float y) {
+  return __uint_as_float(__builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(y), false, false)[0]);
+}
+
+// Sum over the four 16-lane rows (lane groups): every lane receives its column's sum
+// ((x0 + x2) + (x1 + x3), the same order in every row).
+__device__ __forceinline__ float row_sum4(float x) {
+  const auto a = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+  const float y = __uint_as_float(a[0]) + __uint_as_float(a[1]);
+  const auto b = __builtin_amdgcn_permlane16_swap(__float_as_uint(y), __float_as_uint(y), false, false);
+  return __uint_as_float(b[0]) + __uint_as_float(b[1]);
+}
+
+// Lane (q, n) receives register q of lane (0, n): four values that only lane group 0 holds,
+// spread over the groups as one K step of a 16x16x4 product.  fold2: groups 0, 1 receive
+// registers 0, 1 of group 0 (groups 2, 3 those of group 2).
+__device__ __forceinline__ float fold4(float r0, float r1, float r2, float r3) {
+  return swap16_lo(swap32_lo(r0, r2), swap32_lo(r1, r3));
+}
+__device__ __forceinline__ float fold2(float r0, float r1) { return swap16_lo(r0, r1); }
 
 // ---------------------------------------------------------------------------------
 // Host-side layout and packing
@@ -70,6 +98,14 @@ bool EmbedLayout::init(const int* widths) {
     for (int q = 0; q < 4; ++q) mx = std::max(mx, size[l][q]);
     tiles[l] = (mx + 3) / 4;
   }
+  // fold form: sizes (s + 1, s, s, s) in layer 1, and in layers 2 and 3 (S + d, S, S, S) with
+  // S a multiple of 4 and d <= 4, so the last tile of those layers is group 0's d extra rows
+  const int s1 = size[0][1];
+  fold = size[0][0] == s1 + 1 && size[0][2] == s1 && size[0][3] == s1;
+  for (int l = 1; l < 3 && fold; ++l) {
+    const int S = size[l][1], d = size[l][0] - S;
+    fold = S % 4 == 0 && d >= 1 && d <= 4 && size[l][2] == S && size[l][3] == S && tiles[l] == S / 4 + 1;
+  }
   return true;
 }
 
@@ -93,6 +129,10 @@ EmbedNetOffsets embed_net_offsets(const EmbedLayout& L) {
   o.w2a = p; p += t3 * 4 * t2 * 64;
   o.w2b = p; p += t2 * 4 * t3 * 64;
   o.w1b = p; p += t1 * 4 * t2 * 64;
+  o.f2a = p; p += t3 * (4 * (t2 - 1) + 1) * 64;
+  o.f2b = p; p += t2 * (4 * (t3 - 1) + 1) * 64;
+  o.f1b = p; p += t1 * (4 * (t2 - 1) + 1) * 64;
+  o.v1a = p; p += 4 * 2 * 4 * t1;
   o.total = p;
   return o;
 }
@@ -147,6 +187,43 @@ void embed_pack_net(const EmbedLayout& L, const float* K0, const float* B0, cons
         out[o.w1b + (at * 4 * t2 + st) * 64 + lane] = (a >= 0 && c >= 0) ? K1[a * w1 + c] : 0.f;
       }
   }
+  if (!L.fold) return;
+  // Fold-form copies of w2a (output tiles 0..t3), w2b (output tiles 0..t2-1) and w1b: the K steps
+  // of the full tiles, then one folded step for the last tile of the K layer.  That tile holds
+  // lane group 0's extra rows only; the folded step gives lane group q group 0's register q of
+  // it, so its A operand pairs row m with feature (group 0, register 4 (tiles - 1) + q).
+  const int e2 = 16 * (t2 - 1), e3 = 16 * (t3 - 1);
+  const int s2 = 4 * (t2 - 1) + 1, s3 = 4 * (t3 - 1) + 1;  // K steps of the copies
+  for (int lane = 0; lane < 64; ++lane) {
+    const int m = lane % 16, q = lane / 16;
+    const int c_last = L.logical(1, e2 + q), w_last = L.logical(2, e3 + q);
+    for (int ot = 0; ot < t3; ++ot) {  // layer-3 forward
+      const int w = L.logical(2, 16 * ot + m);
+      for (int st = 0; st < s2 - 1; ++st) out[o.f2a + (ot * s2 + st) * 64 + lane] = out[o.w2a + (ot * 4 * t2 + st) * 64 + lane];
+      out[o.f2a + (ot * s2 + s2 - 1) * 64 + lane] = (c_last >= 0 && w >= 0) ? kTanhScale * K2[c_last * w2 + w] : 0.f;
+    }
+    for (int ct = 0; ct < t2; ++ct) {  // layer-3 backward
+      const int c = L.logical(1, 16 * ct + m);
+      for (int st = 0; st < s3 - 1; ++st) out[o.f2b + (ct * s3 + st) * 64 + lane] = out[o.w2b + (ct * 4 * t3 + st) * 64 + lane];
+      out[o.f2b + (ct * s3 + s3 - 1) * 64 + lane] = (c >= 0 && w_last >= 0) ? K2[c * w2 + w_last] : 0.f;
+    }
+    for (int at = 0; at < t1; ++at) {  // layer-2 backward
+      const int a = L.logical(0, 16 * at + m);
+      for (int st = 0; st < s2 - 1; ++st) out[o.f1b + (at * s2 + st) * 64 + lane] = out[o.w1b + (at * 4 * t2 + st) * 64 + lane];
+      out[o.f1b + (at * s2 + s2 - 1) * 64 + lane] = (a >= 0 && c_last >= 0) ? K1[a * w1 + c_last] : 0.f;
+    }
+  }
+  // The backward's forward recompute forms the last layer-2 tile's rows (group 0, register
+  // 4 (t2 - 1) + i, i < 2) as dot products on the VALU: per lane group q, the weights of q's
+  // own layer-1 registers ([q][i][r])
+  for (int q = 0; q < 4; ++q)
+    for (int i = 0; i < 2; ++i) {
+      const int c = L.logical(1, e2 + i);
+      for (int r = 0; r < 4 * t1; ++r) {
+        const int a = L.logical(0, 16 * (r / 4) + 4 * q + r % 4);
+        out[o.v1a + (q * 2 + i) * 4 * t1 + r] = (a >= 0 && c >= 0) ? kTanhScale * K1[a * w1 + c] : 0.f;
+      }
+    }
 }
 
 // ---------------------------------------------------------------------------------
@@ -366,8 +443,10 @@ __device__ __forceinline__ floatx4 tab_load(rsrc_t rs, int voff, int soff) {
   return __builtin_bit_cast(floatx4, (u32x4)__builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, 0));
 }
 
-template <int OT, int S, int S0, int SN, typename BF>
-__device__ __forceinline__ void mfma_chain(rsrc_t rs, int sbase, int vl4, BF bfun, floatx4 (&acc)[OT]) {
+// acc[0..OT) += A tiles (OT of them, S steps apart) x B steps S0 .. S0 + SN (bfun(step))
+template <int OT, int S, int S0, int SN, typename BF, int NA>
+__device__ __forceinline__ void mfma_chain(rsrc_t rs, int sbase, int vl4, BF bfun, floatx4 (&acc)[NA]) {
+  static_assert(OT <= NA, "more tiles than accumulators");
   constexpr int PF = DPMD_WEIGHT_PREFETCH;
   float wb[PF + 1][OT];
 #pragma unroll
@@ -443,6 +522,28 @@ constexpr int kRS = 17;
 
 // Per-net first-layer weights and the three bias vectors (physical slot order) of all
 // ntypes^2 nets, copied once per block into LDS: [net][k0 | b0 | b1 | b2].
+// Fold form: pre-activations of the last layer-2 tile's D2 rows (lane group 0's extra features)
+// on the VALU: each lane group's layer-1 registers against its weights ([q][i][r] at vbase
+// bytes), summed over the groups, plus the rows' bias.
+template <int T1, int D2>
+__device__ __forceinline__ void fold_l2_rows(rsrc_t rs, int vbase, int q, const float (&h1)[4 * T1],
+                                             const float* bias, float (&xv)[2]) {
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    float p = 0.f;
+    if (i < D2) {
+#pragma unroll
+      for (int r = 0; r < 4 * T1; r += 4) {
+        const floatx4 w = tab_load(rs, 4 * ((q * 2 + i) * 4 * T1 + r), vbase);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) p = __builtin_fmaf(w[k], h1[r + k], p);
+      }
+      p = row_sum4(p) + bias[i];
+    }
+    xv[i] = p;
+  }
+}
+
 template <int T1, int T2, int T3>
 __device__ __forceinline__ float* load_small_tables(const EmbedArgs& a, float* smem) {
   constexpr int SM = 16 * (2 * T1 + T2 + T3);
@@ -511,10 +612,14 @@ __device__ __forceinline__ int compact_row(const EmbedArgs& a, const SegInfo& S,
 // V1: registers of layer 1 that hold a feature in some lane group (the rest are padding
 // in every group); layer 2 then has V2 = Z2 V1 such registers.  MFMA k-steps and tanh
 // evaluations over registers that are padding everywhere are skipped.
-template <int T1, int T2, int T3, int Z2, int Z3, int PREC, int V1, bool TAB>
+template <int T1, int T2, int T3, int Z2, int Z3, int PREC, int V1, bool TAB, bool FOLD>
 __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
   constexpr int V2 = Z2 * V1;
   static_assert(V1 <= 4 * T1 && V2 <= 4 * T2, "valid registers exceed the tiles");
+  // fold form (FP32 MLP only): the registers of the last layer-2 tile (D2 rows of lane group 0)
+  // enter the layer-3 products as one folded K step
+  static_assert(!FOLD || (PREC == 0 && !TAB && Z2 * Z3 <= 4 && T2 >= 2), "fold form");
+  constexpr int D2 = Z2;
   extern __shared__ float smem[];
   const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const rsrc_t wrs = __builtin_amdgcn_make_buffer_rsrc((void*)a.net, (short)0, a.net_bytes, 0x00020000);
@@ -563,6 +668,11 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
     constexpr int O_K0 = 0, O_B0 = 16 * T1, O_B1 = 32 * T1, O_B2 = 32 * T1 + 16 * T2;
     constexpr int P0 = ((16 * T1 * 2 + 16 * T2 + 16 * T3) + 63) / 64 * 64;
     constexpr int O_W1A = P0, O_W2A = O_W1A + T2 * 4 * T1 * 64;
+    constexpr int O_W2B = O_W2A + T3 * 4 * T2 * 64, O_W1B = O_W2B + T2 * 4 * T3 * 64;
+    constexpr int SF2 = 4 * (T2 - 1) + 1, SF3 = 4 * (T3 - 1) + 1;  // K steps of the fold-form copies
+    constexpr int O_F2A = O_W1B + T1 * 4 * T2 * 64, O_F2B = O_F2A + T3 * SF2 * 64, O_F1B = O_F2B + T2 * SF3 * 64;
+    (void)O_F2B;
+    (void)O_F1B;
     const int vl4 = lane * 4, vl16 = lane * 16;
     const int hnetb = PREC ? __builtin_amdgcn_readfirstlane((ti * a.ntypes + j) * a.hnet_stride) : 0;
     for (int ch = 0; ch < m; ch += 16) {
@@ -672,15 +782,21 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
           }
           h2[r] = v;
         }
+        // folded K step: group 0's last-tile layer-2 registers spread over the lane groups
+        const float x2 = !FOLD ? 0.f : (D2 == 2 ? fold2(h2[4 * (T2 - 1)], h2[4 * (T2 - 1) + 1]) : h2[4 * (T2 - 1)]);
         for_groups<0, T3, kGroup>([&](auto g0c, auto szc) {
           constexpr int g0 = decltype(g0c)::value, SZ = decltype(szc)::value;
           floatx4 acc3[SZ];
 #pragma unroll
           for (int t = 0; t < SZ; ++t) acc3[t] = floatx4{b2(4 * (g0 + t)), b2(4 * (g0 + t) + 1), b2(4 * (g0 + t) + 2), b2(4 * (g0 + t) + 3)};
-          if constexpr (PREC == 0)
-            mfma_chain<SZ, 4 * T2, 0, V2>(wrs, nb + 4 * (O_W2A + g0 * 4 * T2 * 64), vl4,
-                                          [&](int s_) { return h2[s_]; }, acc3);
-          else
+          if constexpr (PREC == 0) {
+            if constexpr (FOLD)
+              mfma_chain<SZ, SF2, 0, SF2>(wrs, nb + 4 * (O_F2A + g0 * SF2 * 64), vl4,
+                                          [&](int s_) { return s_ < SF2 - 1 ? h2[s_] : x2; }, acc3);
+            else
+              mfma_chain<SZ, 4 * T2, 0, V2>(wrs, nb + 4 * (O_W2A + g0 * 4 * T2 * 64), vl4,
+                                            [&](int s_) { return h2[s_]; }, acc3);
+          } else
             mfma_chain_h<PREC, SZ, S22, 0, S22>(hrs, hb + H_W2A + g0 * S22 * 1024, vl16, [&](int s2) {
               return pack8<PREC, 4 * T2>([&](int r) { return h2[r]; }, s2); }, acc3);
 #pragma unroll
@@ -801,10 +917,15 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
 // ---------------------------------------------------------------------------------
 // Backward: forces (scatter) and virial partials.  One wave per centre.
 
-template <int T1, int T2, int T3, int Z2, int Z3, int PREC, int V1, bool TAB>
-__global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
+template <int T1, int T2, int T3, int Z2, int Z3, int PREC, int V1, bool TAB, bool FOLD>
+__global__ void __launch_bounds__(256, FOLD ? 4 : 2) embed_bwd_kernel(const EmbedArgs a) {
   constexpr int V2 = Z2 * V1;
   static_assert(V1 <= 4 * T1 && V2 <= 4 * T2, "valid registers exceed the tiles");
+  // fold form (FP32 MLP only): the last layer-2 tile (D2 rows of lane group 0) of the forward
+  // recompute on the VALU; the registers of the last layer-2 and layer-3 tiles enter the
+  // products whose K runs over them as one folded K step each
+  static_assert(!FOLD || (PREC == 0 && !TAB && Z2 * Z3 <= 4 && T2 >= 2 && T3 >= 2), "fold form");
+  constexpr int D2 = Z2, D3 = Z2 * Z3, OT2 = FOLD ? T2 - 1 : T2;
   extern __shared__ float smem[];
   __shared__ double vir_s[4][9];
   const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1164,6 +1285,9 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
     // (only kk = 2 can pass row 9); rows past 9 read the zero row
     const float* dTr = dTp + (n < 10 ? n : 9) * HS + 4 * q;
     const float* dTc2 = dTp + (8 + q < 10 ? 8 + q : 9) * HS + n;
+    // fold form: the folded step of dR reads slot (group 0, register 4 (T3 - 1) + q) of row n
+    const float* dTf = dTp + (n < 10 ? n : 9) * HS + q;
+    (void)dTf;
     float fc[3] = {0.f, 0.f, 0.f};
     for (int j = 0; j < a.ntypes; ++j) {
       float* prow = a.pgrad ? a.pgrad + ((size_t)gc * a.ksum + a.koff[j]) * 3 : nullptr;
@@ -1174,6 +1298,9 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
       constexpr int P0 = ((16 * T1 * 2 + 16 * T2 + 16 * T3) + 63) / 64 * 64;
       constexpr int O_W1A = P0, O_W2A = O_W1A + T2 * 4 * T1 * 64;
       constexpr int O_W2B = O_W2A + T3 * 4 * T2 * 64, O_W1B = O_W2B + T2 * 4 * T3 * 64;
+      constexpr int SF2 = 4 * (T2 - 1) + 1, SF3 = 4 * (T3 - 1) + 1;  // K steps of the fold-form copies
+      constexpr int O_F2A = O_W1B + T1 * 4 * T2 * 64, O_F2B = O_F2A + T3 * SF2 * 64, O_F1B = O_F2B + T2 * SF3 * 64;
+      constexpr int O_V1A = O_F1B + T1 * SF2 * 64;
       const int vl4 = lane * 4, vl16 = lane * 16;
       const int hnetb = PREC ? __builtin_amdgcn_readfirstlane((ti * a.ntypes + j) * a.hnet_stride) : 0;
       for (int ch = 0; ch < m; ch += 16) {
@@ -1292,20 +1419,26 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
           float t2k[KEEP_T2 ? 4 * T2 : 1];
           float h2[4 * T2];
           auto H2 = [&](int r) -> float { return h2[r]; };
-          floatx4 acc2[T2];
+          floatx4 acc2[OT2];
 #pragma unroll
-          for (int t = 0; t < T2; ++t) acc2[t] = floatx4{b1(4 * t), b1(4 * t + 1), b1(4 * t + 2), b1(4 * t + 3)};
+          for (int t = 0; t < OT2; ++t) acc2[t] = floatx4{b1(4 * t), b1(4 * t + 1), b1(4 * t + 2), b1(4 * t + 3)};
           if constexpr (PREC == 0)
-            mfma_chain<T2, 4 * T1, 0, V1>(wrs, nb + 4 * O_W1A, vl4, [&](int s_) { return h1[s_]; }, acc2);
+            mfma_chain<OT2, 4 * T1, 0, V1>(wrs, nb + 4 * O_W1A, vl4, [&](int s_) { return h1[s_]; }, acc2);
           else
             mfma_chain_h<PREC, T2, S21, 0, S21>(hrs, hb + H_W1A, vl16, [&](int s2) {
               return pack8<PREC, 4 * T1>([&](int r) { return h1[r]; }, s2); }, acc2);
+          float xv[2] = {0.f, 0.f};
+          if constexpr (FOLD) fold_l2_rows<T1, D2>(wrs, nb + 4 * O_V1A, q, h1, sm - 4 * q + O_B1 + 16 * OT2, xv);
 #pragma unroll
           for (int r = 0; r < 4 * T2; ++r) {
-            const float t = r < V2 ? tanh_s(acc2[r / 4][r % 4]) : 0.f;
+            float t = 0.f;
+            if (r < 4 * OT2) t = r < V2 ? tanh_s(acc2[r / 4][r % 4]) : 0.f;
+            else if (r - 4 * OT2 < D2) t = q == 0 ? tanh_s(xv[r - 4 * OT2]) : 0.f;
             if constexpr (KEEP_T2) t2k[r] = t;
             h2[r] = t + ((r / Z2 < 4 * T1) ? h1[r / Z2] : 0.f);
           }
+          // folded K step: group 0's last-tile layer-2 registers spread over the lane groups
+          const float x2 = !FOLD ? 0.f : (D2 == 2 ? fold2(h2[4 * (T2 - 1)], h2[4 * (T2 - 1) + 1]) : h2[4 * (T2 - 1)]);
           if constexpr (!H16) wave_sync();  // Rl visible to the dh3 products
           floatx4 dr = floatx4{0.f, 0.f, 0.f, 0.f};
           floatx4 dh2[T2];
@@ -1313,13 +1446,19 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
           for (int t = 0; t < T2; ++t) dh2[t] = floatx4{0.f, 0.f, 0.f, 0.f};
           for_groups<0, T3, kGroup>([&](auto g0c, auto szc) {
             constexpr int g0 = decltype(g0c)::value, SZ = decltype(szc)::value;
+            // fold form: this group holds the last layer-3 tile, whose four K steps fold to one
+            constexpr bool LAST = FOLD && g0 + SZ == T3;
             floatx4 acc3[SZ];
 #pragma unroll
             for (int t = 0; t < SZ; ++t) acc3[t] = floatx4{b2(4 * (g0 + t)), b2(4 * (g0 + t) + 1), b2(4 * (g0 + t) + 2), b2(4 * (g0 + t) + 3)};
-            if constexpr (PREC == 0)
-              mfma_chain<SZ, 4 * T2, 0, V2>(wrs, nb + 4 * (O_W2A + g0 * 4 * T2 * 64), vl4,
-                                            [&](int s_) { return H2(s_); }, acc3);
-            else
+            if constexpr (PREC == 0) {
+              if constexpr (FOLD)
+                mfma_chain<SZ, SF2, 0, SF2>(wrs, nb + 4 * (O_F2A + g0 * SF2 * 64), vl4,
+                                            [&](int s_) { return s_ < SF2 - 1 ? H2(s_) : x2; }, acc3);
+              else
+                mfma_chain<SZ, 4 * T2, 0, V2>(wrs, nb + 4 * (O_W2A + g0 * 4 * T2 * 64), vl4,
+                                              [&](int s_) { return H2(s_); }, acc3);
+            } else
               mfma_chain_h<PREC, SZ, S22, 0, S22>(hrs, hb + H_W2A + g0 * S22 * 1024, vl16, [&](int s2) {
                 return pack8<PREC, 4 * T2>([&](int r) { return H2(r); }, s2); }, acc3);
             float t3[4 * SZ];
@@ -1352,13 +1491,21 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
             } else {
               // dR^T[x][nbr] += sum_slot dT[x][slot] h3[slot][nbr] over this group's slots
 #pragma unroll
-              for (int k = 0; k < SZ; ++k)
+              for (int k = 0; k < SZ; ++k) {
+                float h3v[4];
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                   const int r = 4 * (g0 + k) + i;
-                  const float h3v = t3[4 * k + i] + ((r / Z3 < 4 * T2) ? H2(r / Z3) : 0.f);
-                  dr = mfma4(dTr[16 * (g0 + k) + i], h3v, dr);
+                  h3v[i] = t3[4 * k + i] + ((r / Z3 < 4 * T2) ? H2(r / Z3) : 0.f);
                 }
+                if (LAST && k == SZ - 1) {
+                  const float x3 = D3 >= 3 ? fold4(h3v[0], h3v[1], h3v[2], h3v[3]) : D3 == 2 ? fold2(h3v[0], h3v[1]) : h3v[0];
+                  dr = mfma4(dTf[16 * (T3 - 1)], x3, dr);
+                } else {
+#pragma unroll
+                  for (int i = 0; i < 4; ++i) dr = mfma4(dTr[16 * (g0 + k) + i], h3v[i], dr);
+                }
+              }
               // dh3[slot][nbr] = sum_x dT[x][slot] R[nbr][x]
 #pragma unroll
               for (int k = 0; k < SZ; ++k) {
@@ -1378,10 +1525,22 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
                 if (r / Z3 < 4 * T2) dh2[(r / Z3) / 4][(r / Z3) % 4] += dh3[k][i];
                 dh3[k][i] *= (1.f - t3[4 * k + i] * t3[4 * k + i]);
               }
-            if constexpr (PREC == 0)
-              mfma_chain<T2, 4 * T3, 4 * g0, 4 * SZ>(wrs, nb + 4 * O_W2B, vl4,
-                                                    [&](int s_) { return dh3[(s_ - 4 * g0) / 4][(s_ - 4 * g0) % 4]; }, dh2);
-            else
+            if constexpr (PREC == 0) {
+              if constexpr (FOLD) {
+                // K steps of the group's full tiles, then (last group) the folded step
+                constexpr int SZM = LAST ? SZ - 1 : SZ;
+                float x3 = 0.f;
+                if constexpr (LAST) {
+                  const floatx4& d = dh3[SZ - 1];
+                  x3 = D3 >= 3 ? fold4(d[0], d[1], d[2], d[3]) : D3 == 2 ? fold2(d[0], d[1]) : d[0];
+                }
+                mfma_chain<T2, SF3, 4 * g0, 4 * SZM + (LAST ? 1 : 0)>(wrs, nb + 4 * O_F2B, vl4, [&](int s_) {
+                  return s_ == SF3 - 1 ? x3 : dh3[(s_ - 4 * g0) / 4][(s_ - 4 * g0) % 4]; }, dh2);
+              } else {
+                mfma_chain<T2, 4 * T3, 4 * g0, 4 * SZ>(wrs, nb + 4 * O_W2B, vl4,
+                                                      [&](int s_) { return dh3[(s_ - 4 * g0) / 4][(s_ - 4 * g0) % 4]; }, dh2);
+              }
+            } else
               mfma_chain_h<PREC, T2, S23, g0 / 2, 1>(hrs, hb + H_W2B, vl16, [&](int s2) {
                 return pack8<PREC, 4 * SZ>([&](int r) { return dh3[r / 4][r % 4]; }, 0); }, dh2);
           });
@@ -1411,9 +1570,15 @@ __global__ void __launch_bounds__(256, 2) embed_bwd_kernel(const EmbedArgs a) {
             else t = h2[r] - ((r / Z2 < 4 * T1) ? h1[r / Z2] : 0.f);
             dh2[r / 4][r % 4] *= (1.f - t * t);
           }
-          if constexpr (PREC == 0)
-            mfma_chain<T1, 4 * T2, 0, V2>(wrs, nb + 4 * O_W1B, vl4, [&](int s_) { return dh2[s_ / 4][s_ % 4]; }, dh1);
-          else
+          if constexpr (PREC == 0) {
+            if constexpr (FOLD) {
+              const float x = D2 == 2 ? fold2(dh2[T2 - 1][0], dh2[T2 - 1][1]) : dh2[T2 - 1][0];
+              mfma_chain<T1, SF2, 0, SF2>(wrs, nb + 4 * O_F1B, vl4,
+                                          [&](int s_) { return s_ < SF2 - 1 ? dh2[s_ / 4][s_ % 4] : x; }, dh1);
+            } else {
+              mfma_chain<T1, 4 * T2, 0, V2>(wrs, nb + 4 * O_W1B, vl4, [&](int s_) { return dh2[s_ / 4][s_ % 4]; }, dh1);
+            }
+          } else
             mfma_chain_h<PREC, T1, S22, 0, S22>(hrs, hb + H_W1B, vl16, [&](int s2) {
               return pack8<PREC, 4 * T2>([&](int r) { return dh2[r / 4][r % 4]; }, s2); }, dh1);
           dst = 0.f;
@@ -1551,7 +1716,7 @@ static void fill_args(System* s, EmbedArgs& a) {
   }
 }
 
-template <int T1, int T2, int T3, int Z2, int Z3, int PREC, int V1, bool TAB>
+template <int T1, int T2, int T3, int Z2, int Z3, int PREC, int V1, bool TAB, bool FOLD = false>
 static int32_t launch_embed(System* s, bool backward, int precision, int g16) {
   EmbedArgs a;
   fill_args(s, a);
@@ -1598,9 +1763,9 @@ static int32_t launch_embed(System* s, bool backward, int precision, int g16) {
     return DPMD_E_UNSUPPORTED;
   }
   if (!backward) {
-    hipLaunchKernelGGL((embed_fwd_kernel<T1, T2, T3, Z2, Z3, PREC, V1, TAB>), dim3(nblocks), dim3(256), lds, s->stream, a);
+    hipLaunchKernelGGL((embed_fwd_kernel<T1, T2, T3, Z2, Z3, PREC, V1, TAB, FOLD>), dim3(nblocks), dim3(256), lds, s->stream, a);
   } else {
-    hipLaunchKernelGGL((embed_bwd_kernel<T1, T2, T3, Z2, Z3, PREC, V1, TAB>), dim3(nblocks), dim3(256), lds, s->stream, a);
+    hipLaunchKernelGGL((embed_bwd_kernel<T1, T2, T3, Z2, Z3, PREC, V1, TAB, FOLD>), dim3(nblocks), dim3(256), lds, s->stream, a);
   }
   DPMD_HIP(hipGetLastError());
   return DPMD_OK;
@@ -1608,18 +1773,31 @@ static int32_t launch_embed(System* s, bool backward, int precision, int g16) {
 
 // A tabulated (compressed) model has no embedding GEMMs: its kernel is FP32 whatever the
 // precision of the fitting GEMMs.
-template <int T1, int T2, int T3, int Z2, int Z3, int V1>
+// DPMD_EMBED_FOLD=0 runs the unfolded FP32 kernels on a folding layout (comparison runs)
+static bool fold_disabled() {
+  static const bool off = [] {
+    const char* e = getenv("DPMD_EMBED_FOLD");
+    return e && e[0] == '0';
+  }();
+  return off;
+}
+
+// FOLDS: the instantiation has a fold-form variant (its layouts that fold take it)
+template <int T1, int T2, int T3, int Z2, int Z3, int V1, bool FOLDS = false>
 static int32_t launch_prec(System* s, bool backward, int precision, int g16) {
   if (s->model->tab_n > 0) return launch_embed<T1, T2, T3, Z2, Z3, 0, V1, true>(s, backward, precision, g16);
   if (precision == DPMD_PREC_BF16) return launch_embed<T1, T2, T3, Z2, Z3, 1, V1, false>(s, backward, precision, g16);
   if (precision == DPMD_PREC_FP16) return launch_embed<T1, T2, T3, Z2, Z3, 2, V1, false>(s, backward, precision, g16);
+  if constexpr (FOLDS) {
+    if (s->model->lay.fold && !fold_disabled()) return launch_embed<T1, T2, T3, Z2, Z3, 0, V1, false, true>(s, backward, precision, g16);
+  }
   return launch_embed<T1, T2, T3, Z2, Z3, 0, V1, false>(s, backward, precision, g16);
 }
 
 int32_t embed_launch(System* s, bool backward, int precision, int g16) {
   const EmbedLayout& L = s->model->lay;
   if (L.tiles[0] == 2 && L.tiles[1] == 4 && L.tiles[2] == 7 && L.z[1] == 2 && L.z[2] == 2)
-    return launch_prec<2, 4, 7, 2, 2, 7>(s, backward, precision, g16);  // 7 layer-3 tiles imply v1 <= 7
+    return launch_prec<2, 4, 7, 2, 2, 7, true>(s, backward, precision, g16);  // 7 layer-3 tiles imply v1 <= 7
   if (L.tiles[0] == 2 && L.tiles[1] == 2 && L.tiles[2] == 4 && L.z[1] == 1 && L.z[2] == 2)
     return launch_prec<2, 2, 4, 1, 2, 8>(s, backward, precision, g16);
   set_error("embedding widths (%d,%d,%d) have no kernel instantiation", L.w[0], L.w[1], L.w[2]);

@@ -70,6 +70,9 @@ struct EmbedLayout {
   int tiles[3];     // 16-row MFMA tiles per layer
   int base[3][4];
   int size[3][4];
+  // first width 4s + 1 (lane group 0 holds one feature more per layer) with the last tile of
+  // layers 2 and 3 holding only group 0's extra features: the FP32 kernels fold those
+  bool fold;
   bool init(const int* widths);
   // physical slot (16*tile + 4*q + i) -> logical feature or -1
   int logical(int layer, int slot) const;
@@ -77,7 +80,9 @@ struct EmbedLayout {
 
 // Per-net device tables (fp32), see embed.hip for the meaning of each block.
 struct EmbedNetOffsets {
-  int k0, b0, b1, b2, w1a, w2a, w2b, w1b, total;
+  int k0, b0, b1, b2, w1a, w2a, w2b, w1b;
+  int f2a, f2b, f1b, v1a;  // fold-form tables (zero unless the layout folds)
+  int total;
 };
 EmbedNetOffsets embed_net_offsets(const EmbedLayout& L);
 void embed_pack_net(const EmbedLayout& L, const float* K0, const float* B0, const float* K1,
