@@ -34,6 +34,12 @@ __device__ __forceinline__ floatx4 mfma4(float a, float b, floatx4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
+// 16 blocks of 4x4x1: lane l is row (A) and column (B) l % 4 of block l / 4; register j of the
+// result is row j of the lane's block and column (a quarter of mfma4's cost)
+__device__ __forceinline__ floatx4 mfma4x4(float a, float b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 0, 0, 0);
+}
+
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -132,7 +138,8 @@ EmbedNetOffsets embed_net_offsets(const EmbedLayout& L) {
   o.f2a = p; p += t3 * (4 * (t2 - 1) + 1) * 64;
   o.f2b = p; p += t2 * (4 * (t3 - 1) + 1) * 64;
   o.f1b = p; p += t1 * (4 * (t2 - 1) + 1) * 64;
-  o.v1a = p; p += 4 * 2 * 4 * t1;
+  o.f1a = p; p += t2 * 4 * t1 * 64;
+  o.s3a = p; p += 4 * t2 * 64;
   o.total = p;
   return o;
 }
@@ -188,42 +195,68 @@ void embed_pack_net(const EmbedLayout& L, const float* K0, const float* B0, cons
       }
   }
   if (!L.fold) return;
-  // Fold-form copies of w2a (output tiles 0..t3), w2b (output tiles 0..t2-1) and w1b: the K steps
-  // of the full tiles, then one folded step for the last tile of the K layer.  That tile holds
-  // lane group 0's extra rows only; the folded step gives lane group q group 0's register q of
-  // it, so its A operand pairs row m with feature (group 0, register 4 (tiles - 1) + q).
+  // Fold form.  The last tile of layers 2 and 3 holds lane group 0's extra rows only (e2, e3: its
+  // first slot).  A folded K step gives lane group q group 0's register q of such a tile, so its
+  // A operand pairs row m with feature (group 0, register 4 (tiles - 1) + q).  A thin tile (the
+  // last layer-2 tile as an output) runs on v_mfma_f32_4x4x1_16b_f32, which treats lane l as row
+  // (A) and column (B) l % 4 of block l / 4: step r multiplies each lane's register r (feature
+  // (q, r) of neighbour n, or the folded feature) by A = weight[thin row n % 4][that feature], and
+  // block (q, n / 4) accumulates lane group q's part of the thin rows; a sum over the lane groups
+  // ends it.  Tables (tiles of [step][lane]):
+  //   f1a: layer-2 forward, K = layer-1 registers: w1a's full tiles, then the thin tile;
+  //   f2a: layer-3 forward, K = layer-2 registers with the last tile folded;
+  //   f2b: layer-3 backward, K = layer-3 registers with the last tile folded: full tiles, thin tile;
+  //   f1b: layer-2 backward, K = layer-2 registers with the last tile folded;
+  //   s3a: the last layer-3 tile (rows of group 0) in 4x4 blocks, K = layer-2 registers.
   const int e2 = 16 * (t2 - 1), e3 = 16 * (t3 - 1);
-  const int s2 = 4 * (t2 - 1) + 1, s3 = 4 * (t3 - 1) + 1;  // K steps of the copies
+  const int s2 = 4 * (t2 - 1) + 1, s3 = 4 * (t3 - 1) + 1;  // K steps of the folded tables
   for (int lane = 0; lane < 64; ++lane) {
-    const int m = lane % 16, q = lane / 16;
-    const int c_last = L.logical(1, e2 + q), w_last = L.logical(2, e3 + q);
-    for (int ot = 0; ot < t3; ++ot) {  // layer-3 forward
+    const int m = lane % 16, q = lane / 16, m4 = lane % 4;
+    const int c_last = L.logical(1, e2 + q), w_last = L.logical(2, e3 + q);  // folded features
+    const int c_thin = L.logical(1, e2 + m4), w_thin = L.logical(2, e3 + m4);  // thin rows
+    for (int at = 0; at < t2; ++at)
+      for (int st = 0; st < 4 * t1; ++st) {
+        float v;
+        if (at < t2 - 1) {
+          v = out[o.w1a + (at * 4 * t1 + st) * 64 + lane];
+        } else {
+          const int a = L.logical(0, 16 * (st / 4) + 4 * q + st % 4);
+          v = (a >= 0 && c_thin >= 0) ? kTanhScale * K1[a * w1 + c_thin] : 0.f;
+        }
+        out[o.f1a + (at * 4 * t1 + st) * 64 + lane] = v;
+      }
+    for (int ot = 0; ot < t3; ++ot) {
       const int w = L.logical(2, 16 * ot + m);
       for (int st = 0; st < s2 - 1; ++st) out[o.f2a + (ot * s2 + st) * 64 + lane] = out[o.w2a + (ot * 4 * t2 + st) * 64 + lane];
       out[o.f2a + (ot * s2 + s2 - 1) * 64 + lane] = (c_last >= 0 && w >= 0) ? kTanhScale * K2[c_last * w2 + w] : 0.f;
     }
-    for (int ct = 0; ct < t2; ++ct) {  // layer-3 backward
-      const int c = L.logical(1, 16 * ct + m);
-      for (int st = 0; st < s3 - 1; ++st) out[o.f2b + (ct * s3 + st) * 64 + lane] = out[o.w2b + (ct * 4 * t3 + st) * 64 + lane];
-      out[o.f2b + (ct * s3 + s3 - 1) * 64 + lane] = (c >= 0 && w_last >= 0) ? K2[c * w2 + w_last] : 0.f;
+    for (int ct = 0; ct < t2; ++ct) {
+      for (int st = 0; st < s3; ++st) {
+        float v;
+        if (ct < t2 - 1) {
+          if (st < s3 - 1) {
+            v = out[o.w2b + (ct * 4 * t3 + st) * 64 + lane];
+          } else {
+            const int c = L.logical(1, 16 * ct + m);
+            v = (c >= 0 && w_last >= 0) ? K2[c * w2 + w_last] : 0.f;
+          }
+        } else {
+          const int w = st < s3 - 1 ? L.logical(2, 16 * (st / 4) + 4 * q + st % 4) : w_last;
+          v = (c_thin >= 0 && w >= 0) ? K2[c_thin * w2 + w] : 0.f;
+        }
+        out[o.f2b + (ct * s3 + st) * 64 + lane] = v;
+      }
     }
-    for (int at = 0; at < t1; ++at) {  // layer-2 backward
+    for (int at = 0; at < t1; ++at) {
       const int a = L.logical(0, 16 * at + m);
       for (int st = 0; st < s2 - 1; ++st) out[o.f1b + (at * s2 + st) * 64 + lane] = out[o.w1b + (at * 4 * t2 + st) * 64 + lane];
       out[o.f1b + (at * s2 + s2 - 1) * 64 + lane] = (a >= 0 && c_last >= 0) ? K1[a * w1 + c_last] : 0.f;
     }
-  }
-  // The backward's forward recompute forms the last layer-2 tile's rows (group 0, register
-  // 4 (t2 - 1) + i, i < 2) as dot products on the VALU: per lane group q, the weights of q's
-  // own layer-1 registers ([q][i][r])
-  for (int q = 0; q < 4; ++q)
-    for (int i = 0; i < 2; ++i) {
-      const int c = L.logical(1, e2 + i);
-      for (int r = 0; r < 4 * t1; ++r) {
-        const int a = L.logical(0, 16 * (r / 4) + 4 * q + r % 4);
-        out[o.v1a + (q * 2 + i) * 4 * t1 + r] = (a >= 0 && c >= 0) ? kTanhScale * K1[a * w1 + c] : 0.f;
-      }
+    for (int r = 0; r < 4 * t2; ++r) {
+      const int c = L.logical(1, 16 * (r / 4) + 4 * q + r % 4);
+      out[o.s3a + r * 64 + lane] = (c >= 0 && w_thin >= 0) ? kTanhScale * K2[c * w2 + w_thin] : 0.f;
     }
+  }
 }
 
 // ---------------------------------------------------------------------------------
@@ -443,8 +476,9 @@ __device__ __forceinline__ floatx4 tab_load(rsrc_t rs, int voff, int soff) {
   return __builtin_bit_cast(floatx4, (u32x4)__builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, 0));
 }
 
-// acc[0..OT) += A tiles (OT of them, S steps apart) x B steps S0 .. S0 + SN (bfun(step))
-template <int OT, int S, int S0, int SN, typename BF, int NA>
+// acc[0..OT) += A tiles (OT of them, S steps apart) x B steps S0 .. S0 + SN (bfun(step)), on
+// 16x16x4 MFMAs; the last OS tiles on 4x4x1 blocks
+template <int OT, int S, int S0, int SN, int OS = 0, typename BF, int NA>
 __device__ __forceinline__ void mfma_chain(rsrc_t rs, int sbase, int vl4, BF bfun, floatx4 (&acc)[NA]) {
   static_assert(OT <= NA, "more tiles than accumulators");
   constexpr int PF = DPMD_WEIGHT_PREFETCH;
@@ -461,9 +495,26 @@ __device__ __forceinline__ void mfma_chain(rsrc_t rs, int sbase, int vl4, BF bfu
     }
     const float b = bfun(S0 + s);
 #pragma unroll
-    for (int ot = 0; ot < OT; ++ot) acc[ot] = mfma4(wb[s % (PF + 1)][ot], b, acc[ot]);
+    for (int ot = 0; ot < OT; ++ot)
+      acc[ot] = ot >= OT - OS ? mfma4x4(wb[s % (PF + 1)][ot], b, acc[ot]) : mfma4(wb[s % (PF + 1)][ot], b, acc[ot]);
     __builtin_amdgcn_sched_barrier(0);
   }
+}
+
+// acc += one tile of SN steps on 4x4x1 blocks alone: every weight load issued first and the
+// steps spread over two accumulators (a lone 4x4 step is short next to a load or its own result)
+template <int SN, typename BF>
+__device__ __forceinline__ void thin_chain(rsrc_t rs, int sbase, int vl4, BF bfun, floatx4& acc) {
+  float w[SN];
+#pragma unroll
+  for (int s = 0; s < SN; ++s) w[s] = bld(rs, vl4, sbase + s * 256);
+  floatx4 acc1 = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int s = 0; s < SN; ++s) {
+    if (s % 2 == 0) acc = mfma4x4(w[s], bfun(s), acc);
+    else acc1 = mfma4x4(w[s], bfun(s), acc1);
+  }
+  acc += acc1;
 }
 
 // 16-bit A tables are [ot][s2][lane][8]: one 16-byte load per lane per 16x16x32 MFMA.
@@ -522,25 +573,15 @@ constexpr int kRS = 17;
 
 // Per-net first-layer weights and the three bias vectors (physical slot order) of all
 // ntypes^2 nets, copied once per block into LDS: [net][k0 | b0 | b1 | b2].
-// Fold form: pre-activations of the last layer-2 tile's D2 rows (lane group 0's extra features)
-// on the VALU: each lane group's layer-1 registers against its weights ([q][i][r] at vbase
-// bytes), summed over the groups, plus the rows' bias.
-template <int T1, int D2>
-__device__ __forceinline__ void fold_l2_rows(rsrc_t rs, int vbase, int q, const float (&h1)[4 * T1],
-                                             const float* bias, float (&xv)[2]) {
+// Thin tile (4x4-block form): lane group 0's rows of a tile from the per-group partial sums
+// of the 4x4 blocks (sum over the four lane groups); the other groups' (padding) slots are zero
+template <int D>
+__device__ __forceinline__ void thin_rows(floatx4& acc, int q) {
 #pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    float p = 0.f;
-    if (i < D2) {
-#pragma unroll
-      for (int r = 0; r < 4 * T1; r += 4) {
-        const floatx4 w = tab_load(rs, 4 * ((q * 2 + i) * 4 * T1 + r), vbase);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) p = __builtin_fmaf(w[k], h1[r + k], p);
-      }
-      p = row_sum4(p) + bias[i];
-    }
-    xv[i] = p;
+  for (int j = 0; j < 4; ++j) {
+    float v = 0.f;
+    if (j < D) v = row_sum4(acc[j]);
+    acc[j] = q == 0 ? v : 0.f;
   }
 }
 
@@ -616,10 +657,11 @@ template <int T1, int T2, int T3, int Z2, int Z3, int PREC, int V1, bool TAB, bo
 __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
   constexpr int V2 = Z2 * V1;
   static_assert(V1 <= 4 * T1 && V2 <= 4 * T2, "valid registers exceed the tiles");
-  // fold form (FP32 MLP only): the registers of the last layer-2 tile (D2 rows of lane group 0)
-  // enter the layer-3 products as one folded K step
-  static_assert(!FOLD || (PREC == 0 && !TAB && Z2 * Z3 <= 4 && T2 >= 2), "fold form");
-  constexpr int D2 = Z2;
+  // fold form (FP32 MLP only): the last layer-2 tile (D2 rows of lane group 0) and the last
+  // layer-3 tile (D3 rows) on 4x4x1 blocks; the last layer-2 tile's registers enter the
+  // layer-3 products as one folded K step
+  static_assert(!FOLD || (PREC == 0 && !TAB && Z2 * Z3 <= 4 && T2 >= 2 && T3 >= 2), "fold form");
+  constexpr int D2 = Z2, D3 = Z2 * Z3;
   extern __shared__ float smem[];
   const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const rsrc_t wrs = __builtin_amdgcn_make_buffer_rsrc((void*)a.net, (short)0, a.net_bytes, 0x00020000);
@@ -670,9 +712,9 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
     constexpr int O_W1A = P0, O_W2A = O_W1A + T2 * 4 * T1 * 64;
     constexpr int O_W2B = O_W2A + T3 * 4 * T2 * 64, O_W1B = O_W2B + T2 * 4 * T3 * 64;
     constexpr int SF2 = 4 * (T2 - 1) + 1, SF3 = 4 * (T3 - 1) + 1;  // K steps of the fold-form copies
-    constexpr int O_F2A = O_W1B + T1 * 4 * T2 * 64, O_F2B = O_F2A + T3 * SF2 * 64, O_F1B = O_F2B + T2 * SF3 * 64;
-    (void)O_F2B;
-    (void)O_F1B;
+    constexpr int O_F2A = O_W1B + T1 * 4 * T2 * 64, O_F2B = O_F2A + T3 * SF2 * 64;
+    constexpr int O_F1B = O_F2B + T2 * SF3 * 64, O_F1A = O_F1B + T1 * SF2 * 64;
+    constexpr int O_S3A = O_F1A + T2 * 4 * T1 * 64;
     const int vl4 = lane * 4, vl16 = lane * 16;
     const int hnetb = PREC ? __builtin_amdgcn_readfirstlane((ti * a.ntypes + j) * a.hnet_stride) : 0;
     for (int ch = 0; ch < m; ch += 16) {
@@ -767,11 +809,18 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
         floatx4 acc2[T2];
 #pragma unroll
         for (int t = 0; t < T2; ++t) acc2[t] = floatx4{b1(4 * t), b1(4 * t + 1), b1(4 * t + 2), b1(4 * t + 3)};
-        if constexpr (PREC == 0)
-          mfma_chain<T2, 4 * T1, 0, V1>(wrs, nb + 4 * O_W1A, vl4, [&](int s_) { return h1[s_]; }, acc2);
-        else
+        if constexpr (PREC == 0) {
+          if constexpr (FOLD) {
+            // full tiles and the thin last tile (4x4 blocks) in one chain
+            mfma_chain<T2, 4 * T1, 0, V1, 1>(wrs, nb + 4 * O_F1A, vl4, [&](int s_) { return h1[s_]; }, acc2);
+            thin_rows<D2>(acc2[T2 - 1], q);
+          } else {
+            mfma_chain<T2, 4 * T1, 0, V1>(wrs, nb + 4 * O_W1A, vl4, [&](int s_) { return h1[s_]; }, acc2);
+          }
+        } else {
           mfma_chain_h<PREC, T2, S21, 0, S21>(hrs, hb + H_W1A, vl16, [&](int s2) {
             return pack8<PREC, 4 * T1>([&](int r) { return h1[r]; }, s2); }, acc2);
+        }
         float h2[4 * T2];
 #pragma unroll
         for (int r = 0; r < 4 * T2; ++r) {
@@ -790,12 +839,20 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
 #pragma unroll
           for (int t = 0; t < SZ; ++t) acc3[t] = floatx4{b2(4 * (g0 + t)), b2(4 * (g0 + t) + 1), b2(4 * (g0 + t) + 2), b2(4 * (g0 + t) + 3)};
           if constexpr (PREC == 0) {
-            if constexpr (FOLD)
-              mfma_chain<SZ, SF2, 0, SF2>(wrs, nb + 4 * (O_F2A + g0 * SF2 * 64), vl4,
-                                          [&](int s_) { return s_ < SF2 - 1 ? h2[s_] : x2; }, acc3);
-            else
+            if constexpr (FOLD) {
+              // the group's full tiles on the folded K steps; the last tile (last group) in 4x4 blocks
+              constexpr bool LAST = g0 + SZ == T3;
+              if constexpr (!LAST || SZ > 1)
+                mfma_chain<LAST ? SZ - 1 : SZ, SF2, 0, SF2>(wrs, nb + 4 * (O_F2A + g0 * SF2 * 64), vl4,
+                                                           [&](int s_) { return s_ < SF2 - 1 ? h2[s_] : x2; }, acc3);
+              if constexpr (LAST) {
+                thin_chain<V2>(wrs, nb + 4 * O_S3A, vl4, [&](int s_) { return h2[s_]; }, acc3[SZ - 1]);
+                thin_rows<D3>(acc3[SZ - 1], q);
+              }
+            } else {
               mfma_chain<SZ, 4 * T2, 0, V2>(wrs, nb + 4 * (O_W2A + g0 * 4 * T2 * 64), vl4,
                                             [&](int s_) { return h2[s_]; }, acc3);
+            }
           } else
             mfma_chain_h<PREC, SZ, S22, 0, S22>(hrs, hb + H_W2A + g0 * S22 * 1024, vl16, [&](int s2) {
               return pack8<PREC, 4 * T2>([&](int r) { return h2[r]; }, s2); }, acc3);
@@ -921,11 +978,12 @@ template <int T1, int T2, int T3, int Z2, int Z3, int PREC, int V1, bool TAB, bo
 __global__ void __launch_bounds__(256, FOLD ? 4 : 2) embed_bwd_kernel(const EmbedArgs a) {
   constexpr int V2 = Z2 * V1;
   static_assert(V1 <= 4 * T1 && V2 <= 4 * T2, "valid registers exceed the tiles");
-  // fold form (FP32 MLP only): the last layer-2 tile (D2 rows of lane group 0) of the forward
-  // recompute on the VALU; the registers of the last layer-2 and layer-3 tiles enter the
-  // products whose K runs over them as one folded K step each
+  // fold form (FP32 MLP only): the thin tiles -- the last layer-2 tile (D2 rows of lane group 0),
+  // forward and backward, and the last layer-3 tile (D3 rows) of the forward recompute -- on
+  // 4x4x1 blocks; the registers of the last layer-2 and layer-3 tiles enter the products whose
+  // K runs over them as one folded K step each
   static_assert(!FOLD || (PREC == 0 && !TAB && Z2 * Z3 <= 4 && T2 >= 2 && T3 >= 2), "fold form");
-  constexpr int D2 = Z2, D3 = Z2 * Z3, OT2 = FOLD ? T2 - 1 : T2;
+  constexpr int D2 = Z2, D3 = Z2 * Z3;
   extern __shared__ float smem[];
   __shared__ double vir_s[4][9];
   const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1299,8 +1357,9 @@ __global__ void __launch_bounds__(256, FOLD ? 4 : 2) embed_bwd_kernel(const Embe
       constexpr int O_W1A = P0, O_W2A = O_W1A + T2 * 4 * T1 * 64;
       constexpr int O_W2B = O_W2A + T3 * 4 * T2 * 64, O_W1B = O_W2B + T2 * 4 * T3 * 64;
       constexpr int SF2 = 4 * (T2 - 1) + 1, SF3 = 4 * (T3 - 1) + 1;  // K steps of the fold-form copies
-      constexpr int O_F2A = O_W1B + T1 * 4 * T2 * 64, O_F2B = O_F2A + T3 * SF2 * 64, O_F1B = O_F2B + T2 * SF3 * 64;
-      constexpr int O_V1A = O_F1B + T1 * SF2 * 64;
+      constexpr int O_F2A = O_W1B + T1 * 4 * T2 * 64, O_F2B = O_F2A + T3 * SF2 * 64;
+      constexpr int O_F1B = O_F2B + T2 * SF3 * 64, O_F1A = O_F1B + T1 * SF2 * 64;
+      constexpr int O_S3A = O_F1A + T2 * 4 * T1 * 64;
       const int vl4 = lane * 4, vl16 = lane * 16;
       const int hnetb = PREC ? __builtin_amdgcn_readfirstlane((ti * a.ntypes + j) * a.hnet_stride) : 0;
       for (int ch = 0; ch < m; ch += 16) {
@@ -1419,21 +1478,24 @@ __global__ void __launch_bounds__(256, FOLD ? 4 : 2) embed_bwd_kernel(const Embe
           float t2k[KEEP_T2 ? 4 * T2 : 1];
           float h2[4 * T2];
           auto H2 = [&](int r) -> float { return h2[r]; };
-          floatx4 acc2[OT2];
+          floatx4 acc2[T2];
 #pragma unroll
-          for (int t = 0; t < OT2; ++t) acc2[t] = floatx4{b1(4 * t), b1(4 * t + 1), b1(4 * t + 2), b1(4 * t + 3)};
-          if constexpr (PREC == 0)
-            mfma_chain<OT2, 4 * T1, 0, V1>(wrs, nb + 4 * O_W1A, vl4, [&](int s_) { return h1[s_]; }, acc2);
-          else
+          for (int t = 0; t < T2; ++t) acc2[t] = floatx4{b1(4 * t), b1(4 * t + 1), b1(4 * t + 2), b1(4 * t + 3)};
+          if constexpr (PREC == 0) {
+            if constexpr (FOLD) {
+              // full tiles and the thin last tile (4x4 blocks) in one chain
+              mfma_chain<T2, 4 * T1, 0, V1, 1>(wrs, nb + 4 * O_F1A, vl4, [&](int s_) { return h1[s_]; }, acc2);
+              thin_rows<D2>(acc2[T2 - 1], q);
+            } else {
+              mfma_chain<T2, 4 * T1, 0, V1>(wrs, nb + 4 * O_W1A, vl4, [&](int s_) { return h1[s_]; }, acc2);
+            }
+          } else {
             mfma_chain_h<PREC, T2, S21, 0, S21>(hrs, hb + H_W1A, vl16, [&](int s2) {
               return pack8<PREC, 4 * T1>([&](int r) { return h1[r]; }, s2); }, acc2);
-          float xv[2] = {0.f, 0.f};
-          if constexpr (FOLD) fold_l2_rows<T1, D2>(wrs, nb + 4 * O_V1A, q, h1, sm - 4 * q + O_B1 + 16 * OT2, xv);
+          }
 #pragma unroll
           for (int r = 0; r < 4 * T2; ++r) {
-            float t = 0.f;
-            if (r < 4 * OT2) t = r < V2 ? tanh_s(acc2[r / 4][r % 4]) : 0.f;
-            else if (r - 4 * OT2 < D2) t = q == 0 ? tanh_s(xv[r - 4 * OT2]) : 0.f;
+            const float t = r < V2 ? tanh_s(acc2[r / 4][r % 4]) : 0.f;
             if constexpr (KEEP_T2) t2k[r] = t;
             h2[r] = t + ((r / Z2 < 4 * T1) ? h1[r / Z2] : 0.f);
           }
@@ -1452,10 +1514,16 @@ __global__ void __launch_bounds__(256, FOLD ? 4 : 2) embed_bwd_kernel(const Embe
 #pragma unroll
             for (int t = 0; t < SZ; ++t) acc3[t] = floatx4{b2(4 * (g0 + t)), b2(4 * (g0 + t) + 1), b2(4 * (g0 + t) + 2), b2(4 * (g0 + t) + 3)};
             if constexpr (PREC == 0) {
-              if constexpr (FOLD)
-                mfma_chain<SZ, SF2, 0, SF2>(wrs, nb + 4 * (O_F2A + g0 * SF2 * 64), vl4,
-                                            [&](int s_) { return s_ < SF2 - 1 ? H2(s_) : x2; }, acc3);
-              else
+              if constexpr (FOLD) {
+                // the group's full tiles on the folded K steps; the last tile (last group) in 4x4 blocks
+                if constexpr (!LAST || SZ > 1)
+                  mfma_chain<LAST ? SZ - 1 : SZ, SF2, 0, SF2>(wrs, nb + 4 * (O_F2A + g0 * SF2 * 64), vl4,
+                                                             [&](int s_) { return s_ < SF2 - 1 ? H2(s_) : x2; }, acc3);
+                if constexpr (LAST) {
+                  thin_chain<V2>(wrs, nb + 4 * O_S3A, vl4, [&](int s_) { return H2(s_); }, acc3[SZ - 1]);
+                  thin_rows<D3>(acc3[SZ - 1], q);
+                }
+              } else
                 mfma_chain<SZ, 4 * T2, 0, V2>(wrs, nb + 4 * (O_W2A + g0 * 4 * T2 * 64), vl4,
                                               [&](int s_) { return H2(s_); }, acc3);
             } else
@@ -1534,7 +1602,8 @@ __global__ void __launch_bounds__(256, FOLD ? 4 : 2) embed_bwd_kernel(const Embe
                   const floatx4& d = dh3[SZ - 1];
                   x3 = D3 >= 3 ? fold4(d[0], d[1], d[2], d[3]) : D3 == 2 ? fold2(d[0], d[1]) : d[0];
                 }
-                mfma_chain<T2, SF3, 4 * g0, 4 * SZM + (LAST ? 1 : 0)>(wrs, nb + 4 * O_F2B, vl4, [&](int s_) {
+                // full tiles and the thin last layer-2 tile (lane group partials in 4x4 blocks)
+                mfma_chain<T2, SF3, 4 * g0, 4 * SZM + (LAST ? 1 : 0), 1>(wrs, nb + 4 * O_F2B, vl4, [&](int s_) {
                   return s_ == SF3 - 1 ? x3 : dh3[(s_ - 4 * g0) / 4][(s_ - 4 * g0) % 4]; }, dh2);
               } else {
                 mfma_chain<T2, 4 * T3, 4 * g0, 4 * SZ>(wrs, nb + 4 * O_W2B, vl4,
@@ -1546,6 +1615,8 @@ __global__ void __launch_bounds__(256, FOLD ? 4 : 2) embed_bwd_kernel(const Embe
           });
 #pragma unroll
           for (int i = 0; i < 4; ++i) dRl[n * kRS + 4 * q + i] = dr[i];
+          // fold form: the last layer-2 tile's gradient (residual terms on lane group 0 included)
+          if constexpr (FOLD) thin_rows<D2>(dh2[T2 - 1], q);
           // layer 2 backward
           floatx4 dh1[T1];
 #pragma unroll

@@ -81,7 +81,7 @@ struct EmbedLayout {
 // Per-net device tables (fp32), see embed.hip for the meaning of each block.
 struct EmbedNetOffsets {
   int k0, b0, b1, b2, w1a, w2a, w2b, w1b;
-  int f2a, f2b, f1b, v1a;  // fold-form tables (zero unless the layout folds)
+  int f2a, f2b, f1b, f1a, s3a;  // fold-form tables (zero unless the layout folds)
   int total;
 };
 EmbedNetOffsets embed_net_offsets(const EmbedLayout& L);
