@@ -33,7 +33,7 @@ KERNELS = {0: 'nlist', 1: 'embed_fwd', 2: 'fit', 3: 'embed_bwd', 4: 'integrator'
 # same counters read for a 256 MiB elementwise copy (x2.0 and x1.0): 1,074.0 MiB read +
 # 460.2 MiB written (profiles/embed_fit_nlist_water98k_trace.md).  Counters cannot be read
 # without the profiler, so the figure is carried here and re-measured with the kernel.
-EMBED_BWD_TRAFFIC_FP32 = (1074.0 + 460.2) * 2 ** 20
+EMBED_BWD_TRAFFIC_FP32 = (1090.3 + 484.0) * 2 ** 20
 
 
 def flops_per_step(N, pairs, A=16, W=100, emb=(25, 50, 100), fit=240):
