@@ -82,7 +82,9 @@ __device__ __forceinline__ float epi_one(const GemmArgs& g, size_t row, int col,
     const float y1 = g.aux2[o];
     g.C[o] = dy1 * (1.f - y1 * y1);
   } else {
-    g.C[o] = cv;
+    // EPI_STORE writes dE/dG (hundreds of MiB, read once by the next kernel): streamed past the
+    // caches (BF16 fitting 0.58 -> 0.54 ms per 98,304-atom step; the 16-bit GEMM is bound by it)
+    __builtin_nontemporal_store(cv, &g.C[o]);
   }
   return 0.f;
 }
