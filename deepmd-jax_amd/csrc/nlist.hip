@@ -212,10 +212,11 @@ __global__ void __launch_bounds__(256) nl_build_kernel(NlArgs a) {
   const float Lx = S.L[0], Ly = S.L[1], Lz = S.L[2];
   const bool tri = S.tri != 0;
   int cnt = 0;
-  auto consider = [&](int atom, bool in) {
+  // pa: the position of `atom` (read for every lane; masked lanes carry atom 0)
+  auto consider = [&](int atom, bool in, float4 pa) {
     bool keep = false;
     if (in && atom != i)
-      keep = (tri ? pair_d2_cell(ci, cn[atom], S.H) : pair_d2(ci, cn[atom], Lx, Ly, Lz)) < a.cut_sq;
+      keep = (tri ? pair_d2_cell(ci, pa, S.H) : pair_d2(ci, pa, Lx, Ly, Lz)) < a.cut_sq;
     const unsigned long long bal = __ballot(keep);
     const int pre = __popcll(bal & ((1ull << lane) - 1ull));
     if (keep && cnt + pre < a.lcap) list[cnt + pre] = atom;
@@ -226,25 +227,61 @@ __global__ void __launch_bounds__(256) nl_build_kernel(NlArgs a) {
     const int nx = S.nc[0], ny = S.nc[1], nz = S.nc[2];
     const int cx = cell % nx, cy = (cell / nx) % ny, cz = cell / (nx * ny);
     const int* catoms = a.cell_atoms + S.base;
-    for (int dz = -1; dz <= 1; ++dz)
-      for (int dy = -1; dy <= 1; ++dy)
-        for (int dx = -1; dx <= 1; ++dx) {
-          const int x = (cx + dx + nx) % nx;
-          const int y = (cy + dy + ny) % ny;
-          const int z = (cz + dz + nz) % nz;
-          const int cc = S.cell0 + (z * ny + y) * nx + x;
-          const int st = a.cell_start[cc];
-          const int num = a.cell_count[cc];
-          for (int b = 0; b < num; b += 64) {
-            const bool in = b + lane < num;
-            const int atom = in ? catoms[st + b + lane] : 0;
-            consider(atom, in);
-          }
-        }
+    // the 27 cells (dz outer, dx inner) with their start and count in lanes 0..26, read at once
+    int cst = 0, ccnt = 0;
+    if (lane < 27) {
+      const int x = (cx + lane % 3 - 1 + nx) % nx;
+      const int y = (cy + (lane / 3) % 3 - 1 + ny) % ny;
+      const int z = (cz + lane / 9 - 1 + nz) % nz;
+      const int cc = S.cell0 + (z * ny + y) * nx + x;
+      cst = a.cell_start[cc];
+      ccnt = a.cell_count[cc];
+    }
+    // the cells' atoms in pieces of 64 (cell k, offset b; k = 27: past the end), tested in
+    // the order of the cells; a piece's atom indices are read two pieces ahead and their
+    // positions one piece ahead, so neither dependent read waits in line
+    auto settle = [&](int& k, int& b) {
+      while (k < 27 && b >= __builtin_amdgcn_readlane(ccnt, k)) {
+        ++k;
+        b = 0;
+      }
+    };
+    auto advance = [&](int& k, int& b) {
+      if (k < 27) {
+        b += 64;
+        settle(k, b);
+      }
+    };
+    auto load_atom = [&](int k, int b, bool& in) -> int {
+      in = false;
+      if (k >= 27) return 0;
+      const int st = __builtin_amdgcn_readlane(cst, k), num = __builtin_amdgcn_readlane(ccnt, k);
+      in = b + lane < num;
+      return in ? catoms[st + b + lane] : 0;
+    };
+    int kA = 0, bA = 0;
+    settle(kA, bA);
+    int kB = kA, bB = bA;
+    advance(kB, bB);
+    bool inA, inB;
+    int atomA = load_atom(kA, bA, inA);
+    int atomB = load_atom(kB, bB, inB);
+    float4 pA = cn[atomA];
+    while (kA < 27) {
+      int kC = kB, bC = bB;
+      advance(kC, bC);
+      bool inC;
+      const int atomC = load_atom(kC, bC, inC);
+      const float4 pB = cn[atomB];
+      consider(atomA, inA, pA);
+      kA = kB; bA = bB; atomA = atomB; inA = inB; pA = pB;
+      kB = kC; bB = bC; atomB = atomC; inB = inC;
+    }
   } else {
     for (int b = 0; b < N; b += 64) {
       const bool in = b + lane < N;
-      consider(in ? b + lane : 0, in);
+      const int atom = in ? b + lane : 0;
+      consider(atom, in, cn[atom]);
     }
   }
   if (cnt > a.lcap) {
