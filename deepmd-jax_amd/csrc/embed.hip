@@ -567,9 +567,41 @@ constexpr int kGroup = 2;  // layer-3 tiles processed together
 // of 4 (16-byte rows), 36 mod 64 so that 16 rows' 16-byte accesses hit distinct banks.
 constexpr int kTS = 36;
 static_assert(16 * kGroup <= kTS && kTS % 4 == 0, "a tile group's slots fit one 16-byte aligned row");
-// Row stride (floats) of the per-chunk [16 neighbours][env row] LDS tiles: odd, so the
-// 16 neighbours of a column read or write land in distinct banks.
+
+// Environment table of a super-chunk (16 SCH neighbours of a compacted row, processed as SCH
+// chunks of 16): row l holds neighbour c0 + l's ten environment values in columns 0..9, zeros
+// in columns 10 and 11 (the K padding of the dh3 products) and s~ in column kESc; then four
+// zero floats for operand reads that run past the last row.  Each neighbour's environment is
+// formed once (lane l % sn) instead of once per lane group.
+constexpr int kES = 13;  // odd row stride: a column's 16 rows fall in distinct banks
+constexpr int kESc = 12;
+constexpr int es_floats(int sch) { return 16 * sch * kES + 4; }
+// Row stride (floats) of the FP32 forward's per-chunk [16 neighbours][env row] tile: odd, so
+// the 16 neighbours of a column land in distinct banks; columns 10..15 hold zeros.
 constexpr int kRS = 17;
+// LDS of the forward's environment rows: a 64-neighbour table, or the FP32 MLP form's tile
+constexpr int fwd_env_floats(int prec, bool tab) { return prec == 0 && !tab ? 16 * kRS : es_floats(4); }
+// chunks per backward super-chunk: four in the 16-bit MLP forms; two and one in the FP32 and
+// tabulated ones, whose LDS would otherwise cost an occupancy step
+constexpr int bwd_chunks(int prec, bool tab) { return tab ? 1 : prec != 0 ? 4 : 2; }
+__device__ __forceinline__ void env_table(const EmbedArgs& a, const float4* list, float* Et, int c0, int m, int sn,
+                                          int lane, int ti) {
+  const int l = lane % sn;
+  const bool vl = c0 + l < m;
+  const float4 e = vl ? list[c0 + l] : make_float4(0.f, 0.f, 0.f, 0.f);
+  Geo g;
+  float R[10];
+  env_forward(e, vl, a.inv_rcut, a.mean[ti], a.inv_std[ti], g, R);
+  if (lane < sn) {
+    float* row = Et + l * kES;
+#pragma unroll
+    for (int x = 0; x < 10; ++x) row[x] = R[x];
+    row[10] = 0.f;
+    row[11] = 0.f;
+    row[kESc] = g.st;
+  }
+  if (lane < 4) Et[sn * kES + lane] = 0.f;
+}
 
 // Per-net first-layer weights and the three bias vectors (physical slot order) of all
 // ntypes^2 nets, copied once per block into LDS: [net][k0 | b0 | b1 | b2].
@@ -674,8 +706,13 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
   float* small = smem;
   float* wbase = load_small_tables<T1, T2, T3>(a, smem) + wv * a.lds_wave_floats;
   float4* list = reinterpret_cast<float4*>(wbase);
-  float* Rl = wbase + 4 * a.list_cap;  // [16 neighbours][kRS]
-  float* Hl = Rl + 16 * kRS;            // one tile group of h3: [16 neighbours][HG]
+  // ETAB: an environment table per super-chunk of SCH chunks (16-bit and tabulated forms); the
+  // FP32 MLP form stages each chunk's R tile from every lane's own environment row instead (the
+  // table measured slower there)
+  constexpr bool ETAB = !(PREC == 0 && !TAB);
+  constexpr int SCH = ETAB ? 4 : 1, SN = 16 * SCH, RS = ETAB ? kES : kRS;
+  float* Et = wbase + 4 * a.list_cap;    // environment table [SN neighbours][RS]
+  float* Hl = Et + fwd_env_floats(PREC, TAB);  // one tile group of h3: [16 neighbours][HG]
   constexpr int HG = 16 * kGroup + 1;
   // 16-bit MLP forms: the T reducer on FP16 MFMAs (R, h3: bounded values, FP16's 11-bit
   // significand; BF16 operands here would double the form's force error), h3 of a tile
@@ -717,7 +754,13 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
     constexpr int O_S3A = O_F1A + T2 * 4 * T1 * 64;
     const int vl4 = lane * 4, vl16 = lane * 16;
     const int hnetb = PREC ? __builtin_amdgcn_readfirstlane((ti * a.ntypes + j) * a.hnet_stride) : 0;
-    for (int ch = 0; ch < m; ch += 16) {
+    for (int c0 = 0; c0 < m; c0 += SN) {
+    // the super-chunk's environment rows, one neighbour per lane
+    if constexpr (ETAB) {
+      env_table(a, list, Et, c0, m, SN, lane, ti);
+      wave_sync();
+    }
+    for (int cs = 0; cs < SCH && c0 + 16 * cs < m; ++cs) {
       const int nb = netb + opaque_zero();
       const int hb = hnetb + opaque_zero();
       const float* sm = small + (ti * a.ntypes + j) * (16 * (2 * T1 + T2 + T3)) + 4 * q + (nb - netb);
@@ -725,26 +768,34 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
       auto b0 = [&](int r) { return sm[O_B0 + 16 * (r / 4) + (r % 4)]; };
       auto b1 = [&](int r) { return sm[O_B1 + 16 * (r / 4) + (r % 4)]; };
       auto b2 = [&](int r) { return sm[O_B2 + 16 * (r / 4) + (r % 4)]; };
-      const bool valid = ch + n < m;
-      const float4 e = valid ? list[ch + n] : make_float4(0.f, 0.f, 0.f, 0.f);
-      Geo g;
-      float R[10];
-      env_forward(e, valid, a.inv_rcut, a.mean[ti], a.inv_std[ti], g, R);
+      // this chunk's R tile: rows are neighbours; columns x >= 10 that the reducer reads give
+      // output rows of T that are not stored
+      const float* Rl = Et + 16 * cs * RS;
+      float st;
+      if constexpr (ETAB) {
+        st = Rl[n * RS + kESc];
+      } else {
+        const bool valid = c0 + n < m;
+        const float4 e = valid ? list[c0 + n] : make_float4(0.f, 0.f, 0.f, 0.f);
+        Geo g;
+        float R[10];
+        env_forward(e, valid, a.inv_rcut, a.mean[ti], a.inv_std[ti], g, R);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int x = 4 * q + i;  // lane group q stores env rows 4q..4q+3 of its neighbour
-        float v = 0.f;
+        for (int i = 0; i < 4; ++i) {
+          const int x = 4 * q + i;  // lane group q stores env rows 4q..4q+3 of its neighbour
+          float v = 0.f;
 #pragma unroll
-        for (int y = 0; y < 10; ++y) v = x == y ? R[y] : v;
-        Rl[n * kRS + x] = v;
+          for (int y = 0; y < 10; ++y) v = x == y ? R[y] : v;
+          Et[n * RS + x] = v;
+        }
+        st = g.st;
       }
       // 16-bit forms: the reducer's A operand R[nbr 8q + j][x = n] (lane groups 0, 1; the
       // 16 neighbours fill half of the K = 32 of a 16x16x32 step)
       HT16 aR{};
       if constexpr (H16) {
-        wave_sync();
 #pragma unroll
-        for (int j = 0; j < 8; ++j) aR[j] = (HE16)(q < 2 ? Rl[(8 * (q & 1) + j) * kRS + n] : 0.f);
+        for (int j = 0; j < 8; ++j) aR[j] = (HE16)(q < 2 ? Rl[(8 * (q & 1) + j) * RS + n] : 0.f);
       }
       // reducer for one group of layer-3 tiles held in Hl:
       // T[x][slot] += sum_nbr R[nbr][x] h3[nbr][slot]
@@ -762,7 +813,7 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
         } else {
 #pragma unroll
           for (int kk = 0; kk < 4; ++kk) {
-            const float av = Rl[(4 * kk + q) * kRS + n];
+            const float av = Rl[(4 * kk + q) * RS + n];
 #pragma unroll
             for (int k = 0; k < SZ; ++k) Tacc[g0 + k] = mfma4(av, Hl[(4 * kk + q) * HG + 16 * k + n], Tacc[g0 + k]);
           }
@@ -777,7 +828,7 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
         // bounded these kernels); h3 lands in LDS in the same [neighbour][slot] layout.
         const int net = ti * a.ntypes + j;
         float x0;
-        const int row = tab_interval(g.st, a.tab_srmin[net], a.tab_delta[net], a.tab_n, x0);
+        const int row = tab_interval(st, a.tab_srmin[net], a.tab_delta[net], a.tab_n, x0);
         const int gm = lane >> 2, gp = lane & 3;  // gather neighbour and quarter of this lane
         const int rowm = __shfl(row, gm);
         const float x0m = __shfl(x0, gm);
@@ -804,7 +855,7 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
       } else {
         float h1[4 * T1];
 #pragma unroll
-        for (int r = 0; r < 4 * T1; ++r) h1[r] = r < V1 ? tanh_s(g.st * k0(r) + b0(r)) : 0.f;
+        for (int r = 0; r < 4 * T1; ++r) h1[r] = r < V1 ? tanh_s(st * k0(r) + b0(r)) : 0.f;
         // layer accumulators start at the (scaled) bias: one VALU add fewer per activation
         floatx4 acc2[T2];
 #pragma unroll
@@ -869,6 +920,7 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
           reduce_group(g0c, szc);
         });
       }
+    }
     }
   }
   // T in logical order into LDS, then T and G to global
@@ -1017,11 +1069,10 @@ __global__ void __launch_bounds__(256, FOLD ? 4 : 2) embed_bwd_kernel(const Embe
   // products they meet the zero columns x >= 10 of the R tile (finite x 0 = 0).
   constexpr int ODT = embed_dt_floats(T3);
   float* dTp = wbase;                    // 10 * HS
-  float* Rl = wbase + (H16 ? embed_dt16_floats(T3) : ODT);  // [16 neighbours][kRS]
-  // dE/dR of the chunk reuses the R tile: the wave's last read of R (the dh3 products) is
-  // issued before its first write of dR, and one wave's LDS operations complete in order
-  float* dRl = Rl;                       // [16 neighbours][kRS]
-  float4* list = reinterpret_cast<float4*>(Rl + 16 * kRS);  // 16-byte aligned
+  // environment table of a super-chunk of SCH chunks (16 SCH neighbours), the chunks' R tiles
+  constexpr int SCH = bwd_chunks(PREC, TAB), SN = 16 * SCH;
+  float* Et = wbase + (H16 ? embed_dt16_floats(T3) : ODT);  // [SN neighbours][kES]
+  float4* list = reinterpret_cast<float4*>(Et + es_floats(SCH));  // 16-byte aligned
   // tabulated nets: h3 and p' of one tile group, [neighbour][slot] with row stride kTS
   float* Tb = reinterpret_cast<float*>(list + a.list_cap);
 
@@ -1362,7 +1413,12 @@ __global__ void __launch_bounds__(256, FOLD ? 4 : 2) embed_bwd_kernel(const Embe
       constexpr int O_S3A = O_F1A + T2 * 4 * T1 * 64;
       const int vl4 = lane * 4, vl16 = lane * 16;
       const int hnetb = PREC ? __builtin_amdgcn_readfirstlane((ti * a.ntypes + j) * a.hnet_stride) : 0;
-      for (int ch = 0; ch < m; ch += 16) {
+      for (int c0 = 0; c0 < m; c0 += SN) {
+      // the super-chunk's environment rows, one neighbour per lane (lane l % SN)
+      env_table(a, list, Et, c0, m, SN, lane, ti);
+      wave_sync();
+      for (int cs = 0; cs < SCH && c0 + 16 * cs < m; ++cs) {
+        const int ch = c0 + 16 * cs;
         const int nb = netb + opaque_zero();
         const int hb = hnetb + opaque_zero();
         const float* sm = small + (ti * a.ntypes + j) * (16 * (2 * T1 + T2 + T3)) + 4 * q + (nb - netb);
@@ -1370,28 +1426,18 @@ __global__ void __launch_bounds__(256, FOLD ? 4 : 2) embed_bwd_kernel(const Embe
         auto b0 = [&](int r) { return sm[O_B0 + 16 * (r / 4) + (r % 4)]; };
         auto b1 = [&](int r) { return sm[O_B1 + 16 * (r / 4) + (r % 4)]; };
         auto b2 = [&](int r) { return sm[O_B2 + 16 * (r / 4) + (r % 4)]; };
-        const bool valid = ch + n < m;
-        const float4 e = valid ? list[ch + n] : make_float4(0.f, 0.f, 0.f, 0.f);
-        Geo g;
-        float R[10];
-        env_forward(e, valid, a.inv_rcut, a.mean[ti], a.inv_std[ti], g, R);
-        // the R tile feeds the FP32 dh3 products; the 16-bit ones take R from registers
-        if constexpr (!H16) {
-#pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const int x = 4 * q + i;
-            float v = 0.f;
-#pragma unroll
-            for (int y = 0; y < 10; ++y) v = x == y ? R[y] : v;
-            Rl[n * kRS + x] = v;
-          }
-        }
+        // this chunk's R tile (rows n, the FP32 dh3 products' B operand) and s~; dE/dR and
+        // dE/ds~ are written over them: the wave's last reads of R (the dh3 products) and s~
+        // are issued before its first writes, and one wave's LDS operations complete in order
+        float* Rl = Et + 16 * cs * kES;
+        float* dRl = Rl;
+        const float st = Rl[n * kES + kESc];
         float dst;
-        // 16-bit MLP forms: the dh3 products' B operand R[nbr n][x = 8q + j] from this lane's R
+        // 16-bit MLP forms: the dh3 products' B operand R[nbr n][x = 8q + j]
         HT16 bR{};
         if constexpr (H16) {
 #pragma unroll
-          for (int j = 0; j < 8; ++j) bR[j] = (HE16)(q == 0 ? R[j] : (q == 1 && j < 2) ? R[8 + j] : 0.f);
+          for (int j = 0; j < 8; ++j) bR[j] = (HE16)((q == 0 || (q == 1 && j < 2)) ? Rl[n * kES + 8 * q + j] : 0.f);
         }
         if constexpr (TAB) {
           // tabulated net: h3 = p(x0) and dh3/ds~ = p'(x0) from one read of the interval's
@@ -1401,7 +1447,7 @@ __global__ void __launch_bounds__(256, FOLD ? 4 : 2) embed_bwd_kernel(const Embe
           // operands need.
           const int net = ti * a.ntypes + j;
           float x0;
-          const int row = tab_interval(g.st, a.tab_srmin[net], a.tab_delta[net], a.tab_n, x0);
+          const int row = tab_interval(st, a.tab_srmin[net], a.tab_delta[net], a.tab_n, x0);
           const int gm = lane >> 2, gp = lane & 3;
           const int rowm = __shfl(row, gm);
           const float x0m = __shfl(x0, gm);
@@ -1457,20 +1503,21 @@ __global__ void __launch_bounds__(256, FOLD ? 4 : 2) embed_bwd_kernel(const Embe
               floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
               for (int kk = 0; kk < 3; ++kk)
-                acc = mfma4(4 * kk + q < 10 ? dTp[(4 * kk + q) * HS + 16 * (g0 + k) + n] : 0.f, Rl[n * kRS + 4 * kk + q], acc);
+                acc = mfma4(4 * kk + q < 10 ? dTp[(4 * kk + q) * HS + 16 * (g0 + k) + n] : 0.f, Rl[n * kES + 4 * kk + q], acc);
 #pragma unroll
               for (int i = 0; i < 4; ++i) ds = __builtin_fmaf(acc[i], dp[4 * k + i], ds);
             }
           });
 #pragma unroll
-          for (int i = 0; i < 4; ++i) dRl[n * kRS + 4 * q + i] = dr[i];
+          for (int i = 0; i < 4; ++i)
+            if (4 * q + i < 10) dRl[n * kES + 4 * q + i] = dr[i];
           ds += __shfl_xor(ds, 16);
           ds += __shfl_xor(ds, 32);
           dst = ds;
         } else {
           float h1[4 * T1];
 #pragma unroll
-          for (int r = 0; r < 4 * T1; ++r) h1[r] = r < V1 ? tanh_s(g.st * k0(r) + b0(r)) : 0.f;
+          for (int r = 0; r < 4 * T1; ++r) h1[r] = r < V1 ? tanh_s(st * k0(r) + b0(r)) : 0.f;
           // h2 = t2 + h1[r/Z2] is formed once; h3 = t3 + h2[r/Z3] is re-formed where used.  The
           // FP32 form re-forms t2 = h2 - h1[r/Z2] for tanh' (within an ulp of h2), which frees the
           // registers of a fourth wave per SIMD; the 16-bit forms are VALU bound and keep t2.
@@ -1580,7 +1627,7 @@ __global__ void __launch_bounds__(256, FOLD ? 4 : 2) embed_bwd_kernel(const Embe
                 floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                 for (int kk = 0; kk < 3; ++kk)
-                  acc = mfma4(kk < 2 ? dTp[(4 * kk + q) * HS + 16 * (g0 + k) + n] : dTc2[16 * (g0 + k)], Rl[n * kRS + 4 * kk + q], acc);
+                  acc = mfma4(kk < 2 ? dTp[(4 * kk + q) * HS + 16 * (g0 + k) + n] : dTc2[16 * (g0 + k)], Rl[n * kES + 4 * kk + q], acc);
                 dh3[k] = acc;
               }
             }
@@ -1614,7 +1661,8 @@ __global__ void __launch_bounds__(256, FOLD ? 4 : 2) embed_bwd_kernel(const Embe
                 return pack8<PREC, 4 * SZ>([&](int r) { return dh3[r / 4][r % 4]; }, 0); }, dh2);
           });
 #pragma unroll
-          for (int i = 0; i < 4; ++i) dRl[n * kRS + 4 * q + i] = dr[i];
+          for (int i = 0; i < 4; ++i)
+            if (4 * q + i < 10) dRl[n * kES + 4 * q + i] = dr[i];
           // fold form: the last layer-2 tile's gradient (residual terms on lane group 0 included)
           if constexpr (FOLD) thin_rows<D2>(dh2[T2 - 1], q);
           // layer 2 backward
@@ -1659,13 +1707,23 @@ __global__ void __launch_bounds__(256, FOLD ? 4 : 2) embed_bwd_kernel(const Embe
           dst += __shfl_xor(dst, 16);
           dst += __shfl_xor(dst, 32);
         }
-        wave_sync();
+        if (q == 0) dRl[n * kES + kESc] = dst;
+      }
+      wave_sync();
+      // pair gradients of the super-chunk, one neighbour per lane: the geometry re-formed
+      // from the pair vector, dE/dR and dE/ds~ from the table
+      if (lane < SN) {
+        const bool vl = c0 + lane < m;
+        const float4 e = vl ? list[c0 + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+        Geo g;
+        float R[10];
+        env_forward(e, vl, a.inv_rcut, a.mean[ti], a.inv_std[ti], g, R);
         float dR[10];
 #pragma unroll
-        for (int x = 0; x < 10; ++x) dR[x] = dRl[n * kRS + x];
+        for (int x = 0; x < 10; ++x) dR[x] = Et[lane * kES + x];
         float gx[3];
-        env_backward(g, dR, dst, a.inv_std[ti], gx);
-        if (valid && q == 0) {
+        env_backward(g, dR, Et[lane * kES + kESc], a.inv_std[ti], gx);
+        if (vl) {
           const int gj = __float_as_int(e.w);  // skin slot when prow is set
 #pragma unroll
           for (int k = 0; k < 3; ++k) {
@@ -1678,17 +1736,16 @@ __global__ void __launch_bounds__(256, FOLD ? 4 : 2) embed_bwd_kernel(const Embe
 #pragma unroll
             for (int bb = 0; bb < 3; ++bb) vir[aa * 3 + bb] -= g.x[aa] * gx[bb];
         }
-        wave_sync();
+      }
+      wave_sync();  // the table is rewritten by the next super-chunk
       }
     }
-    // centre force: reduce over the 16 lanes of group 0
+    // centre force: reduce over the wave (each lane summed its own neighbours)
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       float v = fc[k];
-      v += __shfl_xor(v, 1);
-      v += __shfl_xor(v, 2);
-      v += __shfl_xor(v, 4);
-      v += __shfl_xor(v, 8);
+#pragma unroll
+      for (int sft = 1; sft < 64; sft <<= 1) v += __shfl_xor(v, sft);
       fc[k] = v;
     }
     if (lane == 0) {
@@ -1699,14 +1756,13 @@ __global__ void __launch_bounds__(256, FOLD ? 4 : 2) embed_bwd_kernel(const Embe
       }
     }
   }
-  // virial: wave reduce, block reduce in fp64, one partial per block
+  // virial: wave reduce (each lane summed its own neighbours), block reduce in fp64, one
+  // partial per block
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
     float v = vir[k];
-    v += __shfl_xor(v, 1);
-    v += __shfl_xor(v, 2);
-    v += __shfl_xor(v, 4);
-    v += __shfl_xor(v, 8);
+#pragma unroll
+    for (int sft = 1; sft < 64; sft <<= 1) v += __shfl_xor(v, sft);
     vir[k] = v;
   }
   if (lane == 0) {
@@ -1807,7 +1863,7 @@ static int32_t launch_embed(System* s, bool backward, int precision, int g16) {
   const int nblocks = s->nblocks;
   size_t wave_floats;
   if (!backward) {
-    wave_floats = 4 * a.list_cap + 16 * kRS + 16 * (16 * kGroup + 1);
+    wave_floats = 4 * a.list_cap + fwd_env_floats(PREC, TAB) + 16 * (16 * kGroup + 1);
     wave_floats = std::max(wave_floats, (size_t)(10 * W + 12 * 16));
   } else {
     // dE/dT, then the chunk tiles (R / dR, pair list, tabulated h3 and p') or, in the prologue,
@@ -1819,11 +1875,11 @@ static int32_t launch_embed(System* s, bool backward, int precision, int g16) {
       return DPMD_E_UNSUPPORTED;
     }
     wave_floats = (size_t)embed_dt_floats(T3) +
-                  std::max((size_t)(16 * kRS + 4 * a.list_cap + (TAB ? 2 * 16 * kTS : 0)),
+                  std::max((size_t)(es_floats(bwd_chunks(PREC, TAB)) + 4 * a.list_cap + (TAB ? 2 * 16 * kTS : 0)),
                            (size_t)(A <= 16 ? TPASS : std::max(TPASS, 10 * W + 20 * A)));
     // 16-bit MLP forms: the FP16 dE/dT copies replace dTp in the chunk phase
     if (PREC != 0 && !TAB)
-      wave_floats = std::max(wave_floats, (size_t)embed_dt16_floats(T3) + 16 * kRS + 4 * a.list_cap);
+      wave_floats = std::max(wave_floats, (size_t)embed_dt16_floats(T3) + es_floats(bwd_chunks(PREC, TAB)) + 4 * a.list_cap);
   }
   wave_floats = (wave_floats + 3) / 4 * 4;
   a.lds_wave_floats = (int)wave_floats;
