@@ -305,6 +305,7 @@ struct EmbedArgs {
   float* fcen;                 // (NA, 3)
   int ksum, koff[kMaxT];
   int g16;                     // forward: G stored as FP32 (0), BF16 (1) or FP16 (2) for the fitting GEMMs
+  int dg16;                    // backward: dG stored as FP32 (0) or FP16 (1, the plain 16-bit fitting forms)
 };
 
 // Minimum image of d = x_j - x_i for positions wrapped into [0, L) (so |d| < L): the
@@ -1101,7 +1102,13 @@ __global__ void __launch_bounds__(256, FOLD ? 4 : 2) embed_bwd_kernel(const Embe
     // plus the U terms of the factors T[:, :A] and of G2's arguments T[:, A:2A].
     {
       const float* Tg = a.T + (size_t)gc * 10 * W;
-      const float* dGg = a.dG + (size_t)grow * A * W;
+      // the centre's dE/dG rows: FP32, or FP16 after a plain 16-bit fitting
+      const size_t dgo = (size_t)grow * A * W;
+      const void* dGv = a.dg16 ? (const void*)(reinterpret_cast<const uint16_t*>(a.dG) + dgo) : (const void*)(a.dG + dgo);
+      auto dG_at = [&](int e) -> float {
+        return a.dg16 ? (float)__builtin_bit_cast(_Float16, reinterpret_cast<const uint16_t*>(dGv)[e])
+                      : reinterpret_cast<const float*>(dGv)[e];
+      };
       float* Tl = dTp + ODT;       // 10*W   (aliases Rl and list)
       float* Xl = Tl + 10 * W;     // 10*A
       float* Ul = Xl + 10 * A;     // 10*A
@@ -1121,22 +1128,34 @@ __global__ void __launch_bounds__(256, FOLD ? 4 : 2) embed_bwd_kernel(const Embe
       // the dG operands of U and V (A <= 16), loaded with T so that the two reads overlap.  W <=
       // 16 T3 (the layer-3 tiles hold every output feature): compile-time trip counts; masked
       // entries are buffer loads past the end of the centre's dG rows, which read as zero
-      const rsrc_t Grs = __builtin_amdgcn_make_buffer_rsrc((void*)dGg, (short)0, A * W * 4, 0x00020000);
+      const rsrc_t Grs = __builtin_amdgcn_make_buffer_rsrc((void*)dGv, (short)0, A * W * (a.dg16 ? 2 : 4), 0x00020000);
       float dgu[4 * T3];  // dG[a = n][w = 4 kk + q]
       float dgv[T3][4];   // dG[a = 4 kk + q][w = 16 t + n]
-      if (A <= 16) {
+      // entry e of the centre's rows (e = A W: past the end, reads zero), in either format
+      auto load_dg = [&](auto g16c) {
+        constexpr bool G16 = decltype(g16c)::value;
+        auto ld = [&](int e) -> float {
+          if constexpr (G16)
+            return (float)__builtin_bit_cast(_Float16, (unsigned short)__builtin_amdgcn_raw_buffer_load_b16(Grs, 2 * e, 0, 0));
+          else
+            return bld(Grs, 4 * e, 0);
+        };
 #pragma unroll
         for (int kk = 0; kk < 4 * T3; ++kk) {
           const int w = 4 * kk + q;
-          dgu[kk] = bld(Grs, (n < A && w < W) ? 4 * (n * W + w) : 4 * A * W, 0);
+          dgu[kk] = ld((n < A && w < W) ? n * W + w : A * W);
         }
 #pragma unroll
         for (int t = 0; t < T3; ++t)
 #pragma unroll
           for (int kk = 0; kk < 4; ++kk) {
             const int aa = 4 * kk + q, w = 16 * t + n;
-            dgv[t][kk] = bld(Grs, (aa < A && w < W) ? 4 * (aa * W + w) : 4 * A * W, 0);
+            dgv[t][kk] = ld((aa < A && w < W) ? aa * W + w : A * W);
           }
+      };
+      if (A <= 16) {
+        if (a.dg16) load_dg(std::true_type{});
+        else load_dg(std::false_type{});
       }
       // the region holds TN * 64 floats: the zeros past 10 W are stored as well (no branch)
 #pragma unroll
@@ -1302,7 +1321,7 @@ __global__ void __launch_bounds__(256, FOLD ? 4 : 2) embed_bwd_kernel(const Embe
           for (int x = 0; x < 10; ++x) u[x] = 0.f;
           if (aa < A) {
             for (int w = wq; w < W; w += 4) {
-              const float d = dGg[aa * W + w];
+              const float d = dG_at(aa * W + w);
 #pragma unroll
               for (int x = 0; x < 10; ++x) u[x] += d * Tl[x * W + w];
             }
@@ -1323,7 +1342,7 @@ __global__ void __launch_bounds__(256, FOLD ? 4 : 2) embed_bwd_kernel(const Embe
 #pragma unroll
           for (int x = 0; x < 10; ++x) v[x] = 0.f;
           for (int aa = 0; aa < A; ++aa) {
-            const float d = dGg[aa * W + w];
+            const float d = dG_at(aa * W + w);
 #pragma unroll
             for (int x = 0; x < 10; ++x) v[x] += d * Xl[x * A + aa];
           }
@@ -1810,6 +1829,7 @@ static void fill_args(System* s, EmbedArgs& a) {
   a.T = s->d_T;
   a.G = (float*)s->d_G;
   a.dG = s->d_dG;
+  a.dg16 = s->dg16;
   a.force = s->d_force;
   a.partial = s->partial_out ? s->partial_out : s->d_partial;
   a.wc_g = s->wc_g;

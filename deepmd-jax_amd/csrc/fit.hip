@@ -33,6 +33,7 @@ struct GemmArgs {
   int ldb;
   int M, N, K;
   float* C;         // primary output, row stride ldc
+  uint16_t* C16;    // EPI_STORE: FP16 output in place of C (same element layout), or null
   int ldc;
   float* C2;        // secondary output (same stride)
   const float* aux1;
@@ -83,8 +84,10 @@ __device__ __forceinline__ float epi_one(const GemmArgs& g, size_t row, int col,
     g.C[o] = dy1 * (1.f - y1 * y1);
   } else {
     // EPI_STORE writes dE/dG (hundreds of MiB, read once by the next kernel): streamed past the
-    // caches (BF16 fitting 0.58 -> 0.54 ms per 98,304-atom step; the 16-bit GEMM is bound by it)
-    __builtin_nontemporal_store(cv, &g.C[o]);
+    // caches (BF16 fitting 0.58 -> 0.54 ms per 98,304-atom step; the 16-bit GEMM is bound by it);
+    // FP16 in the plain 16-bit forms
+    if (g.C16) __builtin_nontemporal_store(__builtin_bit_cast(uint16_t, (_Float16)cv), &g.C16[o]);
+    else __builtin_nontemporal_store(cv, &g.C[o]);
   }
   return 0.f;
 }
@@ -365,6 +368,35 @@ __global__ void __launch_bounds__(256, 2) gemm_h_kernel(const GemmArgs g) {
     }
   }
   // epilogue: lane holds C[m0 + 32 wm + 16 mi + 4q + i][n0 + 16 (wn NH + nt) + n]
+  if constexpr (EPI == EPI_STORE && KC >= 56) {
+    if (g.C16) {
+      // FP16 output (dE/dG of the plain 16-bit forms): the block's 64 x BN tile is written to LDS
+      // row-major (the B staging space, 16 NT rows of KC + 8 >= 64 elements), then stored in
+      // 16-byte pieces, whole rows of BN elements per block (2-byte stores from the MFMA layout
+      // wrote 32-byte row pieces and ran slower than the FP32 output)
+      uint16_t* Ct = Bs;
+      __syncthreads();
+#pragma unroll
+      for (int nt = 0; nt < NH; ++nt) {
+        const int tile = wn * NH + nt;
+        if (NT % 2 == 0 || tile < NT) {
+#pragma unroll
+          for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+              Ct[(32 * wm + 16 * mi + 4 * q + i) * BN + 16 * tile + n] = __builtin_bit_cast(uint16_t, (_Float16)acc[mi][nt][i]);
+        }
+      }
+      __syncthreads();
+      constexpr int PR = BN / 8;  // 16-byte pieces per row
+      for (int pc = tid; pc < 64 * PR; pc += 256) {
+        const int row = pc / PR, pe = pc - row * PR;
+        const uintx4 v = *reinterpret_cast<const uintx4*>(&Ct[row * BN + 8 * pe]);
+        __builtin_nontemporal_store(v, reinterpret_cast<uintx4*>(g.C16 + (size_t)(m0 + row) * g.ldc + n0 + 8 * pe));
+      }
+      return;
+    }
+  }
   float esum[2][4];
 #pragma unroll
   for (int mi = 0; mi < 2; ++mi)
@@ -741,6 +773,8 @@ int32_t fit_launch(System* s, int precision) {
   // 16-bit weight copies: BF16 (also the high parts of the split form), FP16; the split form
   // adds the BF16 low parts
   const bool split = precision == DPMD_PREC_BF16X3;
+  const bool dg16 = precision == DPMD_PREC_BF16 || precision == DPMD_PREC_FP16;
+  s->dg16 = dg16 ? 1 : 0;
   const uint16_t* Wh = precision == DPMD_PREC_FP32 ? nullptr : M->d_fit_h[split ? 0 : precision - 1];
   const uint16_t* Wlo = split ? M->d_fit_h[2] : nullptr;
   if ((precision != DPMD_PREC_FP32 && !Wh) || (split && !Wlo)) {
@@ -818,11 +852,13 @@ int32_t fit_launch(System* s, int precision) {
       g.C = DZ1 + r0 * FK; g.C2 = nullptr; g.aux1 = DY2 + r0 * FK; g.aux2 = Y1 + r0 * FK;
       DPMD_TRY(gemm_dispatch<EPI_BWD1>(s->stream, g, precision));
     }
-    // 6: dG = dz1 K0^T
+    // 6: dG = dz1 K0^T; the plain 16-bit forms store it FP16 (the embedding backward reads it
+    // once per centre; its GEMM operands were 16-bit already)
     g.A = DZ1 + r0 * FK; g.Ah = nullptr; g.lda = FK;
     g.Bt = B(FK0); g.Btl = BL(FK0); g.ldb = FK; g.K = FK;
     g.N = D;
     g.C = s->d_dG + r0 * D; g.ldc = D;
+    g.C16 = dg16 ? reinterpret_cast<uint16_t*>(s->d_dG) + r0 * D : nullptr;
     DPMD_TRY(gemm_dispatch<EPI_STORE>(s->stream, g, precision));
   }
   return DPMD_OK;
@@ -879,6 +915,7 @@ int32_t fit_atomic_fwd(System* s, const AtomicFitBufs& b) {
 // other types get dG = 0
 int32_t fit_atomic_bwd(System* s, const AtomicFitBufs& b) {
   Model* M = s->model;
+  s->dg16 = 0;
   const int FK = b.FK;
   const int D = M->axis * M->wout;
   const float* Wf = M->d_fit;

@@ -278,6 +278,7 @@ struct System {
   float* d_T = nullptr;         // (NA, 10, W)
   void* d_G = nullptr;          // (mrows, A*W)
   float* d_dG = nullptr;        // (mrows, A*W)
+  int dg16 = 0;                 // d_dG holds FP16 values (written by a plain 16-bit fitting)
   float* d_fitbuf = nullptr;    // 7 activations (mrows, fstride)
   float* d_eatom = nullptr;     // (mrows)
   double* d_partial = nullptr;  // per-block virial partials (nblocks, 9)
