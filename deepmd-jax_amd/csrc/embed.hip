@@ -582,9 +582,10 @@ constexpr int es_floats(int sch) { return 16 * sch * kES + 4; }
 constexpr int kRS = 17;
 // LDS of the forward's environment rows: a 64-neighbour table, or the FP32 MLP form's tile
 constexpr int fwd_env_floats(int prec, bool tab) { return prec == 0 && !tab ? 16 * kRS : es_floats(4); }
-// chunks per backward super-chunk: four in the 16-bit MLP forms; two and one in the FP32 and
-// tabulated ones, whose LDS would otherwise cost an occupancy step
-constexpr int bwd_chunks(int prec, bool tab) { return tab ? 1 : prec != 0 ? 4 : 2; }
+// chunks per backward super-chunk: two in the MLP forms, one in the tabulated one (a larger
+// table costs an occupancy step: in the 16-bit forms once the neighbour capacity grows past
+// ~180, in the FP32 and tabulated forms at once)
+constexpr int bwd_chunks(int prec, bool tab) { return tab ? 1 : 2; }
 __device__ __forceinline__ void env_table(const EmbedArgs& a, const float4* list, float* Et, int c0, int m, int sn,
                                           int lane, int ti) {
   const int l = lane % sn;

@@ -83,11 +83,10 @@ __device__ __forceinline__ float epi_one(const GemmArgs& g, size_t row, int col,
     const float y1 = g.aux2[o];
     g.C[o] = dy1 * (1.f - y1 * y1);
   } else {
-    // EPI_STORE writes dE/dG (hundreds of MiB, read once by the next kernel): streamed past the
-    // caches (BF16 fitting 0.58 -> 0.54 ms per 98,304-atom step; the 16-bit GEMM is bound by it);
-    // FP16 in the plain 16-bit forms
-    if (g.C16) __builtin_nontemporal_store(__builtin_bit_cast(uint16_t, (_Float16)cv), &g.C16[o]);
-    else __builtin_nontemporal_store(cv, &g.C[o]);
+    // EPI_STORE writes dE/dG: FP32, or FP16 in the plain 16-bit forms (whose GEMM normally writes
+    // it through LDS in 16-byte row pieces, see gemm_h_kernel)
+    if (g.C16) g.C16[o] = __builtin_bit_cast(uint16_t, (_Float16)cv);
+    else g.C[o] = cv;
   }
   return 0.f;
 }
