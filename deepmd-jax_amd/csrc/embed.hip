@@ -577,15 +577,16 @@ static_assert(16 * kGroup <= kTS && kTS % 4 == 0, "a tile group's slots fit one 
 constexpr int kES = 13;  // odd row stride: a column's 16 rows fall in distinct banks
 constexpr int kESc = 12;
 constexpr int es_floats(int sch) { return 16 * sch * kES + 4; }
-// Row stride (floats) of the FP32 forward's per-chunk [16 neighbours][env row] tile: odd, so
-// the 16 neighbours of a column land in distinct banks; columns 10..15 hold zeros.
+// Row stride (floats) of the per-chunk [16 neighbours][env row] tiles (the FP32 forward, the
+// backward): odd, so the 16 neighbours of a column land in distinct banks; columns 10..15 hold
+// zeros.
 constexpr int kRS = 17;
-// LDS of the forward's environment rows: a 64-neighbour table, or the FP32 MLP form's tile
+// LDS of the forward's environment rows: a 64-neighbour table, or the FP32 MLP form's tile (the
+// table measured slower there)
 constexpr int fwd_env_floats(int prec, bool tab) { return prec == 0 && !tab ? 16 * kRS : es_floats(4); }
-// chunks per backward super-chunk: two in the MLP forms, one in the tabulated one (a larger
-// table costs an occupancy step: in the 16-bit forms once the neighbour capacity grows past
-// ~180, in the FP32 and tabulated forms at once)
-constexpr int bwd_chunks(int prec, bool tab) { return tab ? 1 : 2; }
+// ... and of the backward's: a two-chunk table in the FP32 MLP form, the per-chunk tile in the
+// 16-bit and tabulated forms (there the table saved less than its LDS round trips cost)
+constexpr int bwd_env_floats(int prec, bool tab) { return prec == 0 && !tab ? es_floats(2) : 16 * kRS; }
 __device__ __forceinline__ void env_table(const EmbedArgs& a, const float4* list, float* Et, int c0, int m, int sn,
                                           int lane, int ti) {
   const int l = lane % sn;
@@ -603,6 +604,26 @@ __device__ __forceinline__ void env_table(const EmbedArgs& a, const float4* list
     row[kESc] = g.st;
   }
   if (lane < 4) Et[sn * kES + lane] = 0.f;
+}
+
+// A pair's gradient gx = dE/dx_ij into the forces (atomics on the neighbour, the centre's part
+// summed in fc) or, with deterministic forces, into the centre's pair-gradient row by skin slot
+// (e.w); the virial -x (x) gx.  `on`: this lane owns a valid pair.
+__device__ __forceinline__ void pair_out(const EmbedArgs& a, float* prow, bool on, const float4 e, const Geo& g,
+                                         const float gx[3], float fc[3], float vir[9]) {
+  if (on) {
+    const int gj = __float_as_int(e.w);  // skin slot when prow is set
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      if (prow) prow[3 * gj + k] = gx[k];
+      else atomicAdd(a.force + (size_t)gj * 3 + k, -gx[k]);
+      fc[k] += gx[k];
+    }
+#pragma unroll
+    for (int aa = 0; aa < 3; ++aa)
+#pragma unroll
+      for (int bb = 0; bb < 3; ++bb) vir[aa * 3 + bb] -= g.x[aa] * gx[bb];
+  }
 }
 
 // Per-net first-layer weights and the three bias vectors (physical slot order) of all
@@ -1071,10 +1092,13 @@ __global__ void __launch_bounds__(256, FOLD ? 4 : 2) embed_bwd_kernel(const Embe
   // products they meet the zero columns x >= 10 of the R tile (finite x 0 = 0).
   constexpr int ODT = embed_dt_floats(T3);
   float* dTp = wbase;                    // 10 * HS
-  // environment table of a super-chunk of SCH chunks (16 SCH neighbours), the chunks' R tiles
-  constexpr int SCH = bwd_chunks(PREC, TAB), SN = 16 * SCH;
-  float* Et = wbase + (H16 ? embed_dt16_floats(T3) : ODT);  // [SN neighbours][kES]
-  float4* list = reinterpret_cast<float4*>(Et + es_floats(SCH));  // 16-byte aligned
+  // ETAB (the FP32 MLP form): an environment table per super-chunk of SCH = 2 chunks, the pair
+  // gradients formed once per neighbour after it; the 16-bit and tabulated forms stage each
+  // chunk's R tile from every lane's own environment row (the table measured slower there)
+  constexpr bool ETAB = PREC == 0 && !TAB;
+  constexpr int SCH = ETAB ? 2 : 1, SN = 16 * SCH, RS = ETAB ? kES : kRS;
+  float* Et = wbase + (H16 ? embed_dt16_floats(T3) : ODT);  // [SN neighbours][RS]
+  float4* list = reinterpret_cast<float4*>(Et + bwd_env_floats(PREC, TAB));  // 16-byte aligned
   // tabulated nets: h3 and p' of one tile group, [neighbour][slot] with row stride kTS
   float* Tb = reinterpret_cast<float*>(list + a.list_cap);
 
@@ -1434,9 +1458,11 @@ __global__ void __launch_bounds__(256, FOLD ? 4 : 2) embed_bwd_kernel(const Embe
       const int vl4 = lane * 4, vl16 = lane * 16;
       const int hnetb = PREC ? __builtin_amdgcn_readfirstlane((ti * a.ntypes + j) * a.hnet_stride) : 0;
       for (int c0 = 0; c0 < m; c0 += SN) {
-      // the super-chunk's environment rows, one neighbour per lane (lane l % SN)
-      env_table(a, list, Et, c0, m, SN, lane, ti);
-      wave_sync();
+      if constexpr (ETAB) {
+        // the super-chunk's environment rows, one neighbour per lane (lane l % SN)
+        env_table(a, list, Et, c0, m, SN, lane, ti);
+        wave_sync();
+      }
       for (int cs = 0; cs < SCH && c0 + 16 * cs < m; ++cs) {
         const int ch = c0 + 16 * cs;
         const int nb = netb + opaque_zero();
@@ -1446,19 +1472,39 @@ __global__ void __launch_bounds__(256, FOLD ? 4 : 2) embed_bwd_kernel(const Embe
         auto b0 = [&](int r) { return sm[O_B0 + 16 * (r / 4) + (r % 4)]; };
         auto b1 = [&](int r) { return sm[O_B1 + 16 * (r / 4) + (r % 4)]; };
         auto b2 = [&](int r) { return sm[O_B2 + 16 * (r / 4) + (r % 4)]; };
-        // this chunk's R tile (rows n, the FP32 dh3 products' B operand) and s~; dE/dR and
-        // dE/ds~ are written over them: the wave's last reads of R (the dh3 products) and s~
-        // are issued before its first writes, and one wave's LDS operations complete in order
-        float* Rl = Et + 16 * cs * kES;
+        // this chunk's R tile (rows n, the FP32 dh3 products' B operand); dE/dR (and with ETAB
+        // dE/ds~) is written over it: the wave's last reads of R (the dh3 products) and s~ are
+        // issued before its first writes, and one wave's LDS operations complete in order
+        float* Rl = Et + 16 * cs * RS;
         float* dRl = Rl;
-        const float st = Rl[n * kES + kESc];
-        float dst;
-        // 16-bit MLP forms: the dh3 products' B operand R[nbr n][x = 8q + j]
+        const bool valid = ch + n < m;
+        const float4 e = (!ETAB && valid) ? list[ch + n] : make_float4(0.f, 0.f, 0.f, 0.f);
+        Geo g;
+        float st;
+        // 16-bit MLP forms: the dh3 products' B operand R[nbr n][x = 8q + j] from this lane's R
         HT16 bR{};
-        if constexpr (H16) {
+        if constexpr (ETAB) {
+          st = Rl[n * RS + kESc];
+        } else {
+          float R[10];
+          env_forward(e, valid, a.inv_rcut, a.mean[ti], a.inv_std[ti], g, R);
+          st = g.st;
+          // the R tile feeds the FP32 dh3 products; the 16-bit ones take R from registers
+          if constexpr (!H16) {
 #pragma unroll
-          for (int j = 0; j < 8; ++j) bR[j] = (HE16)((q == 0 || (q == 1 && j < 2)) ? Rl[n * kES + 8 * q + j] : 0.f);
+            for (int i = 0; i < 4; ++i) {
+              const int x = 4 * q + i;
+              float v = 0.f;
+#pragma unroll
+              for (int y = 0; y < 10; ++y) v = x == y ? R[y] : v;
+              Rl[n * RS + x] = v;
+            }
+          } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) bR[j] = (HE16)(q == 0 ? R[j] : (q == 1 && j < 2) ? R[8 + j] : 0.f);
+          }
         }
+        float dst;
         if constexpr (TAB) {
           // tabulated net: h3 = p(x0) and dh3/ds~ = p'(x0) from one read of the interval's
           // coefficients; dE/ds~ = sum_slot dh3[slot] p'[slot] (no gradient through idx)
@@ -1523,14 +1569,14 @@ __global__ void __launch_bounds__(256, FOLD ? 4 : 2) embed_bwd_kernel(const Embe
               floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
               for (int kk = 0; kk < 3; ++kk)
-                acc = mfma4(4 * kk + q < 10 ? dTp[(4 * kk + q) * HS + 16 * (g0 + k) + n] : 0.f, Rl[n * kES + 4 * kk + q], acc);
+                acc = mfma4(4 * kk + q < 10 ? dTp[(4 * kk + q) * HS + 16 * (g0 + k) + n] : 0.f, Rl[n * RS + 4 * kk + q], acc);
 #pragma unroll
               for (int i = 0; i < 4; ++i) ds = __builtin_fmaf(acc[i], dp[4 * k + i], ds);
             }
           });
 #pragma unroll
           for (int i = 0; i < 4; ++i)
-            if (4 * q + i < 10) dRl[n * kES + 4 * q + i] = dr[i];
+            if (!ETAB || 4 * q + i < 10) dRl[n * RS + 4 * q + i] = dr[i];
           ds += __shfl_xor(ds, 16);
           ds += __shfl_xor(ds, 32);
           dst = ds;
@@ -1647,7 +1693,7 @@ __global__ void __launch_bounds__(256, FOLD ? 4 : 2) embed_bwd_kernel(const Embe
                 floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                 for (int kk = 0; kk < 3; ++kk)
-                  acc = mfma4(kk < 2 ? dTp[(4 * kk + q) * HS + 16 * (g0 + k) + n] : dTc2[16 * (g0 + k)], Rl[n * kES + 4 * kk + q], acc);
+                  acc = mfma4(kk < 2 ? dTp[(4 * kk + q) * HS + 16 * (g0 + k) + n] : dTc2[16 * (g0 + k)], Rl[n * RS + 4 * kk + q], acc);
                 dh3[k] = acc;
               }
             }
@@ -1682,7 +1728,7 @@ __global__ void __launch_bounds__(256, FOLD ? 4 : 2) embed_bwd_kernel(const Embe
           });
 #pragma unroll
           for (int i = 0; i < 4; ++i)
-            if (4 * q + i < 10) dRl[n * kES + 4 * q + i] = dr[i];
+            if (!ETAB || 4 * q + i < 10) dRl[n * RS + 4 * q + i] = dr[i];
           // fold form: the last layer-2 tile's gradient (residual terms on lane group 0 included)
           if constexpr (FOLD) thin_rows<D2>(dh2[T2 - 1], q);
           // layer 2 backward
@@ -1727,40 +1773,42 @@ __global__ void __launch_bounds__(256, FOLD ? 4 : 2) embed_bwd_kernel(const Embe
           dst += __shfl_xor(dst, 16);
           dst += __shfl_xor(dst, 32);
         }
-        if (q == 0) dRl[n * kES + kESc] = dst;
-      }
-      wave_sync();
-      // pair gradients of the super-chunk, one neighbour per lane: the geometry re-formed
-      // from the pair vector, dE/dR and dE/ds~ from the table
-      if (lane < SN) {
-        const bool vl = c0 + lane < m;
-        const float4 e = vl ? list[c0 + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
-        Geo g;
-        float R[10];
-        env_forward(e, vl, a.inv_rcut, a.mean[ti], a.inv_std[ti], g, R);
-        float dR[10];
+        if constexpr (ETAB) {
+          if (q == 0) dRl[n * RS + kESc] = dst;
+        } else {
+          wave_sync();
+          float dR[10];
 #pragma unroll
-        for (int x = 0; x < 10; ++x) dR[x] = Et[lane * kES + x];
-        float gx[3];
-        env_backward(g, dR, Et[lane * kES + kESc], a.inv_std[ti], gx);
-        if (vl) {
-          const int gj = __float_as_int(e.w);  // skin slot when prow is set
-#pragma unroll
-          for (int k = 0; k < 3; ++k) {
-            if (prow) prow[3 * gj + k] = gx[k];
-            else atomicAdd(a.force + (size_t)gj * 3 + k, -gx[k]);
-            fc[k] += gx[k];
-          }
-#pragma unroll
-          for (int aa = 0; aa < 3; ++aa)
-#pragma unroll
-            for (int bb = 0; bb < 3; ++bb) vir[aa * 3 + bb] -= g.x[aa] * gx[bb];
+          for (int x = 0; x < 10; ++x) dR[x] = dRl[n * RS + x];
+          float gx[3];
+          env_backward(g, dR, dst, a.inv_std[ti], gx);
+          pair_out(a, prow, valid && q == 0, e, g, gx, fc, vir);
+          wave_sync();
         }
       }
-      wave_sync();  // the table is rewritten by the next super-chunk
+      if constexpr (ETAB) {
+        // pair gradients of the super-chunk, one neighbour per lane: the geometry re-formed
+        // from the pair vector, dE/dR and dE/ds~ from the table
+        wave_sync();
+        if (lane < SN) {
+          const bool vl = c0 + lane < m;
+          const float4 e = vl ? list[c0 + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+          Geo g;
+          float R[10];
+          env_forward(e, vl, a.inv_rcut, a.mean[ti], a.inv_std[ti], g, R);
+          float dR[10];
+#pragma unroll
+          for (int x = 0; x < 10; ++x) dR[x] = Et[lane * RS + x];
+          float gx[3];
+          env_backward(g, dR, Et[lane * RS + kESc], a.inv_std[ti], gx);
+          pair_out(a, prow, vl, e, g, gx, fc, vir);
+        }
+        wave_sync();  // the table is rewritten by the next super-chunk
+      }
       }
     }
-    // centre force: reduce over the wave (each lane summed its own neighbours)
+    // centre force: reduce over the wave (the lanes that summed pair gradients: group 0, or with
+    // ETAB the first SN lanes; the others hold zeros)
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       float v = fc[k];
@@ -1776,8 +1824,7 @@ __global__ void __launch_bounds__(256, FOLD ? 4 : 2) embed_bwd_kernel(const Embe
       }
     }
   }
-  // virial: wave reduce (each lane summed its own neighbours), block reduce in fp64, one
-  // partial per block
+  // virial: wave reduce (see the centre force), block reduce in fp64, one partial per block
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
     float v = vir[k];
@@ -1896,11 +1943,11 @@ static int32_t launch_embed(System* s, bool backward, int precision, int g16) {
       return DPMD_E_UNSUPPORTED;
     }
     wave_floats = (size_t)embed_dt_floats(T3) +
-                  std::max((size_t)(es_floats(bwd_chunks(PREC, TAB)) + 4 * a.list_cap + (TAB ? 2 * 16 * kTS : 0)),
+                  std::max((size_t)(bwd_env_floats(PREC, TAB) + 4 * a.list_cap + (TAB ? 2 * 16 * kTS : 0)),
                            (size_t)(A <= 16 ? TPASS : std::max(TPASS, 10 * W + 20 * A)));
     // 16-bit MLP forms: the FP16 dE/dT copies replace dTp in the chunk phase
     if (PREC != 0 && !TAB)
-      wave_floats = std::max(wave_floats, (size_t)embed_dt16_floats(T3) + es_floats(bwd_chunks(PREC, TAB)) + 4 * a.list_cap);
+      wave_floats = std::max(wave_floats, (size_t)embed_dt16_floats(T3) + bwd_env_floats(PREC, TAB) + 4 * a.list_cap);
   }
   wave_floats = (wave_floats + 3) / 4 * 4;
   a.lds_wave_floats = (int)wave_floats;
