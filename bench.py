@@ -30,10 +30,10 @@ PEAK_HBM = 8.0e12
 KERNELS = {0: 'nlist', 1: 'embed_fwd', 2: 'fit', 3: 'embed_bwd', 4: 'integrator'}
 # HBM bytes per launch of the FP32 embed_bwd kernel on the 98,304-atom box, from two
 # rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE in KiB per dispatch) scaled by what the
-# same counters read for a 256 MiB elementwise copy (x2.0 and x1.0): 1,074.0 MiB read +
-# 460.2 MiB written (profiles/embed_fit_nlist_water98k_trace.md).  Counters cannot be read
+# same counters read for a 256 MiB elementwise copy (x2.0 and x1.0): 1,073.6 MiB read +
+# 447.0 MiB written (profiles/embed_fit_nlist_water98k_trace.md).  Counters cannot be read
 # without the profiler, so the figure is carried here and re-measured with the kernel.
-EMBED_BWD_TRAFFIC_FP32 = (1090.3 + 484.0) * 2 ** 20
+EMBED_BWD_TRAFFIC_FP32 = (1073.6 + 447.0) * 2 ** 20
 
 
 def flops_per_step(N, pairs, A=16, W=100, emb=(25, 50, 100), fit=240):
