@@ -195,13 +195,20 @@ __global__ void __launch_bounds__(256, 2) gemm_f32_kernel(const GemmArgs g) {
 #pragma unroll
     for (int sub = 0; sub < KC / 16; ++sub) {
       const floatx4 av = *reinterpret_cast<const floatx4*>(&As[(16 * wv + n) * LS + 16 * sub + 4 * q]);
+      // column tiles in pairs: the two accumulators alternate (no MFMA waits on the one before
+      // it), and the next pair's B operands are read while this pair's MFMAs run
+      auto ldb = [&](int nt) { return *reinterpret_cast<const floatx4*>(&Bs[(16 * nt + n) * LS + 16 * sub + 4 * q]); };
+      floatx4 b0 = ldb(0), b1 = ldb(NT > 1 ? 1 : 0);
 #pragma unroll
-      for (int nt = 0; nt < NT; ++nt) {
-        const floatx4 bv = *reinterpret_cast<const floatx4*>(&Bs[(16 * nt + n) * LS + 16 * sub + 4 * q]);
-        acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, bv.x, acc[nt], 0, 0, 0);
-        acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, bv.y, acc[nt], 0, 0, 0);
-        acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.z, bv.z, acc[nt], 0, 0, 0);
-        acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.w, bv.w, acc[nt], 0, 0, 0);
+      for (int nt = 0; nt < NT; nt += 2) {
+        const floatx4 c0 = b0, c1 = b1;
+        if (nt + 2 < NT) b0 = ldb(nt + 2);
+        if (nt + 3 < NT) b1 = ldb(nt + 3);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[i], c0[i], acc[nt], 0, 0, 0);
+          if (nt + 1 < NT) acc[nt + 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[i], c1[i], acc[nt + 1], 0, 0, 0);
+        }
       }
     }
   }
@@ -644,11 +651,22 @@ __global__ void __launch_bounds__(256, (NT <= 8 && PREC <= 1) ? 2 : 1) fit_hidde
         }
       } else {
         const float* Wl = reinterpret_cast<const float*>(Ws[g & 1]);
+        // output tiles in pairs: the two accumulators alternate (no MFMA waits on the one before it)
+        auto lda = [&](int ot) { return *reinterpret_cast<const floatx4*>(Wl + (16 * ot + n) * LSW + 4 * q); };
+        floatx4 a0 = lda(0), a1 = lda(NT > 1 ? 1 : 0);
 #pragma unroll
-        for (int ot = 0; ot < NT; ++ot) {
-          const floatx4 av = *reinterpret_cast<const floatx4*>(Wl + (16 * ot + n) * LSW + 4 * q);
+        for (int ot = 0; ot < NT; ot += 2) {
+          const floatx4 c0 = a0, c1 = a1;
+          if (ot + 2 < NT) a0 = lda(ot + 2);
+          if (ot + 3 < NT) a1 = lda(ot + 3);
+          // the next pair's reads stay ahead of these MFMAs (the wide layers run one wave per
+          // SIMD, which nothing else hides an LDS wait behind)
+          if constexpr (NT > 8) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-          for (int i = 0; i < 4; ++i) acc[ot] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[i], bop[s][i], acc[ot], 0, 0, 0);
+          for (int i = 0; i < 4; ++i) {
+            acc[ot] = __builtin_amdgcn_mfma_f32_16x16x4f32(c0[i], bop[s][i], acc[ot], 0, 0, 0);
+            if (ot + 1 < NT) acc[ot + 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(c1[i], bop[s][i], acc[ot + 1], 0, 0, 0);
+          }
         }
       }
       if (g + 1 < NG) {
