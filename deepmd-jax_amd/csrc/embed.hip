@@ -441,6 +441,9 @@ __device__ __forceinline__ uint16_t h16_bits(float v) {
 
 // Per-wave LDS of the 16-bit forms' FP16 dE/dT operand copies (floats): dTa [tile pair][lane
 // group][x 0..10][8] for the dR products, dTb [slot][x 0..15] for the dh3 products.
+// output width of the fold form: first width 4 (V1 - 1) + 1 (V1 - 1 features in each lane group, one
+// more in group 0), widened Z2 and Z3 times by the repeat residuals
+constexpr int fold_width(int V1, int Z2, int Z3) { return Z2 * Z3 * (4 * (V1 - 1) + 1); }
 constexpr int embed_dt_floats(int T3) { return (10 * (16 * T3 + 1) + 3) / 4 * 4; }
 constexpr int embed_dt16_floats(int T3) { return ((11 * ((T3 + 1) / 2) * 32 + 16 * T3 * 16) / 2 + 3) / 4 * 4; }
 
@@ -948,7 +951,8 @@ __global__ void __launch_bounds__(256, 2) embed_fwd_kernel(const EmbedArgs a) {
   }
   // T in logical order into LDS, then T and G to global
   float* Tl = wbase;
-  const int W = a.W, A = a.A;
+  // the fold form's layout fixes the output width (first width 4 (V1 - 1) + 1, doubled Z2 Z3 times)
+  const int W = FOLD ? fold_width(V1, Z2, Z3) : a.W, A = a.A;
 #pragma unroll
   for (int nt = 0; nt < T3; ++nt) {
     const int qq = n / 4, ii = n % 4, r = 4 * nt + ii;
@@ -1071,7 +1075,8 @@ __global__ void __launch_bounds__(256, FOLD ? 4 : 2) embed_bwd_kernel(const Embe
   static_assert(kGroup * 4 == 8, "16-bit backward assumes two layer-3 tiles per group");
   const int q = lane >> 4, n = lane & 15;
   constexpr int HS = 16 * T3 + 1;
-  const int W = a.W, A = a.A;
+  // the fold form's layout fixes the output width (compile-time addresses of T and dE/dG)
+  const int W = FOLD ? fold_width(V1, Z2, Z3) : a.W, A = a.A;
   float* small = smem;
   float* wbase = load_small_tables<T1, T2, T3>(a, smem) + wv * a.lds_wave_floats;
   // chunk-phase layout
@@ -1165,17 +1170,19 @@ __global__ void __launch_bounds__(256, FOLD ? 4 : 2) embed_bwd_kernel(const Embe
           else
             return bld(Grs, 4 * e, 0);
         };
+        // rows a >= A start at or past entry A W and read zero like the entries past the end; with a
+        // compile-time W (fold form) the offsets are a per-lane base plus constants
 #pragma unroll
         for (int kk = 0; kk < 4 * T3; ++kk) {
           const int w = 4 * kk + q;
-          dgu[kk] = ld((n < A && w < W) ? n * W + w : A * W);
+          dgu[kk] = ld(w < W ? n * W + w : A * W);
         }
 #pragma unroll
         for (int t = 0; t < T3; ++t)
 #pragma unroll
           for (int kk = 0; kk < 4; ++kk) {
             const int aa = 4 * kk + q, w = 16 * t + n;
-            dgv[t][kk] = ld((aa < A && w < W) ? aa * W + w : A * W);
+            dgv[t][kk] = ld(w < W ? aa * W + w : A * W);
           }
       };
       if (A <= 16) {
@@ -1928,6 +1935,10 @@ static int32_t launch_embed(System* s, bool backward, int precision, int g16) {
   }
   constexpr int HS = 16 * T3 + 1;
   const int W = a.W, A = a.A;
+  if (FOLD && W != fold_width(V1, Z2, Z3)) {
+    set_error("fold-form embedding kernel launched for output width %d", W);
+    return DPMD_E_STATE;
+  }
   const int nblocks = s->nblocks;
   size_t wave_floats;
   if (!backward) {
