@@ -65,6 +65,36 @@ def test_energy_force_virial(device_model, model0, nmol):
     assert abs(ea[0].sum() - e[0]) <= 1e-6 * abs(E)
 
 
+def test_isolated_atoms_and_empty_rows(device_model, model0):
+    """Atoms with no neighbour within rcut (empty rows of both types) beside a dense cluster: the
+    isolated atoms' energies come from the fitting net on a zero descriptor, their forces are zero,
+    and the cluster is unaffected."""
+    from deepmd_jax import synthetic as syn
+    print('case isolated', flush=True)
+    params, var = model0
+    cl, cbox, ctypes = syn.water_box(27, seed=3)
+    L = 40.0
+    iso = np.array([[20., 20., 20.], [28., 20., 20.], [20., 28., 20.], [20., 20., 28.],
+                    [28., 28., 28.], [20., 28., 28.]])
+    coord = _f32(np.concatenate([np.mod(cl, np.asarray(cbox).reshape(-1)[:3]), iso]))
+    types = np.concatenate([ctypes, [0, 1, 0, 1, 1, 0]]).astype(np.int32)
+    box = np.array([L, L, L])
+    E, F, W = O.energy_force_virial(params, var, coord, box, types, lattice='auto')
+    E, F, W = E.item(), F.numpy(), W.numpy()
+    s = _system(device_model, coord, box, types, params['rcut'])
+    e, f, v, ea = s.compute('fp32', want_atom=True)
+    N = len(types)
+    assert abs(e[0] - E) / N <= 1e-5 and abs(e[0] - E) / abs(E) <= 1e-5, (e[0], E)
+    assert np.abs(f[0] - F).max() <= 1e-4 * max(1.0, np.abs(F).max()), np.abs(f[0] - F).max()
+    assert np.abs(v[0] - W).max() <= 1e-4 * np.abs(W).max(), (v[0], W)
+    # the isolated atoms: no force, and each one's energy is its type's zero-descriptor energy
+    assert np.abs(f[0][len(ctypes):]).max() == 0.0
+    ea_iso = ea[0][len(ctypes):]
+    for t in (0, 1):
+        sel = types[len(ctypes):] == t
+        assert np.ptp(ea_iso[sel]) == 0.0
+
+
 def test_replicas_and_skin_independent(device_model, model0):
     """Replicas of one system give identical results; the skin does not change E/F."""
     from deepmd_jax import synthetic as syn
